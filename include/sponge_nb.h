@@ -1,0 +1,233 @@
+/*
+ * sponge_nb.h -- C ABI of the gfx950 nonbonded-force library (libsponge_nb.so).
+ *
+ * The library replaces three modules of SPONGE 2.0 on one GPU:
+ *   NEIGHBOR_LIST              SPONGE/neighbor_list/neighbor_list.{h,cpp}
+ *   LENNARD_JONES_INFORMATION  SPONGE/Lennard_Jones_force/Lennard_Jones_force.{h,cpp}
+ *   Particle_Mesh (PME)        SPONGE/PM_force/PM_force.{h,cpp}
+ * Each entry point below names the reference method it stands in for; the
+ * argument lists keep the reference's order and meaning, with the module
+ * object replaced by an snb_context.
+ *
+ * Conventions
+ *  - Pointers named d_* are device pointers owned by the caller (as the
+ *    reference's dd.* buffers are, Domain_decomposition.h:24-72); h_* are
+ *    host pointers read during the call.  Every launch goes to the context's
+ *    stream; calls return without synchronising unless stated.
+ *  - Forces, energies and virials are accumulated (+=) into caller buffers,
+ *    as in the reference.  Internally every term is summed in int64 fixed
+ *    point (scale 2^32) and converted once at the boundary.  Passing NULL for
+ *    an output keeps the contribution in the context's fixed-point
+ *    accumulators until snb_flush_forces / snb_flush_atom_energy.
+ *  - Every function returns an SNB_* status and never exits the process
+ *    (the reference calls Throw_SPONGE_Error -> exit, utils/control/error.hpp:30).
+ *  - One rank only: local order = global order, ghost_numbers must be 0,
+ *    solvent_numbers must be 0 (this library owns the water atoms too, in
+ *    place of SOLVENT_LENNARD_JONES, Lennard_Jones_force/solvent_LJ.cpp).
+ */
+#ifndef SPONGE_NB_H
+#define SPONGE_NB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Value types with the reference's memory layout. */
+typedef struct { float x, y, z; } snb_vector;                          /* VECTOR, utils/vector.hpp:30-33 */
+typedef struct { float a11, a21, a22, a31, a32, a33; } snb_ltmatrix3;  /* LTMatrix3, utils/vector.hpp:256-258 */
+typedef struct { int atom_numbers; int ghost_numbers; int* atom_serial; } snb_atom_group; /* ATOM_GROUP, common.h:175-180 */
+
+typedef struct snb_context snb_context;
+
+/* return codes (reference error codes: utils/control/error.hpp:4-27) */
+#define SNB_OK 0
+#define SNB_ERR_ARGUMENT 1      /* bad argument value or NULL where required */
+#define SNB_ERR_HIP 2           /* a HIP runtime call failed */
+#define SNB_ERR_BOX_TOO_SMALL 3 /* fewer than 3 NL cells along an axis (neighbor_list.cpp:809-814) */
+#define SNB_ERR_UNSUPPORTED 4   /* ghosts, solvent split, triclinic NL, FFT size with a prime > 7 */
+#define SNB_ERR_NOT_INITIALIZED 5
+#define SNB_ERR_OUT_OF_MEMORY 6
+#define SNB_ERR_DEVICE_STATUS 7 /* a sticky device flag is set, see snb_read_status */
+
+/* sticky device flags (snb_read_status); the reference keeps the first three
+ * as device ints but never reads them (neighbor_list.cpp:887-965) */
+#define SNB_FLAG_NL_OVERFLOW 1      /* a row exceeded max_neighbor_numbers; row truncated */
+#define SNB_FLAG_CELL_OVERFLOW 2    /* a cell held more than max_atom_in_grid_numbers atoms */
+#define SNB_FLAG_NONFINITE_CRD 4    /* a coordinate was NaN/Inf (PME uses 0 for it, PM_force.cpp:710-714) */
+
+int snb_abi_version(void);
+const char* snb_status_string(int code);
+
+/* Creates a context for atom_numbers atoms on HIP device `device`, issuing
+ * work on hip_stream (hipStream_t; NULL = the null stream). */
+int snb_context_create(snb_context** out, int device, void* hip_stream, int atom_numbers);
+int snb_context_destroy(snb_context* ctx);
+int snb_set_stream(snb_context* ctx, void* hip_stream);
+int snb_synchronize(snb_context* ctx);
+/* Reads (and with clear != 0, resets) the sticky SNB_FLAG_* bits; synchronises. */
+int snb_read_status(snb_context* ctx, int clear, int* flags);
+
+/* ---------------------------------------------------------------- exclusions */
+/* Uploads the excluded-atom CSR of MD_INFORMATION::non_bond_information
+ * (MD_core/nb.hpp:30-70): for atom i, h_excluded_list[h_excluded_list_start[i]
+ * .. + h_excluded_numbers[i]) lists its excluded partners (the native
+ * exclude file lists j > i).  The same arrays are what the reference passes as
+ * device pointers to NEIGHBOR_LIST::Update and the PME exclusion correction;
+ * those entry points below also accept them directly. */
+int snb_set_exclusions(snb_context* ctx, const int* h_excluded_list_start,
+                       const int* h_excluded_list, const int* h_excluded_numbers,
+                       int excluded_atom_numbers);
+
+/* ------------------------------------------------------------ neighbor list */
+/* NEIGHBOR_LIST::Initial (neighbor_list.cpp:729-832) with the controller keys
+ * resolved by the caller: neighbor_list_max_neighbor_numbers (default 1200),
+ * neighbor_list_max_atom_in_grid_numbers (150), neighbor_list_refresh_interval
+ * (0 = displacement criterion), neighbor_list_skin_permit (0.5).  Cells are
+ * (cutoff+skin)/2 wide; boxes with < 3 cells on an axis return
+ * SNB_ERR_BOX_TOO_SMALL. */
+int snb_nl_initial(snb_context* ctx, float cutoff, float skin, snb_ltmatrix3 cell,
+                   snb_ltmatrix3 rcell, int max_neighbor_numbers,
+                   int max_atom_in_grid_numbers, int refresh_interval,
+                   float skin_permit);
+
+/* NEIGHBOR_LIST::Update (neighbor_list.cpp:834-885).  update = 1 forces a
+ * rebuild (FORCED_UPDATE); otherwise the reference policy applies and the
+ * decision stays on the device (no host synchronisation).  atom_local may be
+ * NULL (identity); ghost_numbers must be 0.  The exclusion arrays may be NULL
+ * when snb_set_exclusions was called. */
+int snb_nl_update(snb_context* ctx, const int* d_atom_local, int local_atom_numbers,
+                  int ghost_numbers, const snb_vector* d_crd, snb_ltmatrix3 cell,
+                  snb_ltmatrix3 rcell, int step, int update,
+                  const int* d_excluded_list_start, const int* d_excluded_list,
+                  const int* d_excluded_numbers);
+
+/* The public half list (NEIGHBOR_LIST::d_nl, neighbor_list.h:42-45): an
+ * ATOM_GROUP per atom whose atom_serial points into a padded
+ * int[atom_numbers][max_neighbor_numbers] device table; row i holds j > i. */
+int snb_nl_device_list(snb_context* ctx, snb_atom_group** d_nl, int** d_serial_table,
+                       int* max_neighbor_numbers);
+/* Copies the row lengths (int[atom_numbers]) and the padded table
+ * (int[atom_numbers][max_neighbor_numbers]) of the public half list into
+ * caller device buffers (the reference's h_nl readback, neighbor_list.h:42). */
+int snb_nl_copy_rows(snb_context* ctx, int* d_counts, int* d_rows);
+/* Cell grid shape chosen by snb_nl_initial (NEIGHBOR_LIST::GRIDS Nx/Ny/Nz). */
+int snb_nl_grid_shape(snb_context* ctx, int* nx, int* ny, int* nz);
+
+/* --------------------------------------------------------------------- LJ */
+/* LENNARD_JONES_INFORMATION::Initial (Lennard_Jones_force.cpp:186-283) given
+ * the loaded tables: h_LJ_A/h_LJ_B are the in-memory pair tables
+ * (file value x12 and x6, xponge/load/native/lj.hpp:47,57), length
+ * atom_type_numbers*(atom_type_numbers+1)/2; h_atom_LJ_type per atom.  Also
+ * computes the long-range correction factor (:246-272) from a type histogram
+ * instead of the O(N^2) Total_C6_Get. */
+int snb_lj_initial(snb_context* ctx, int atom_type_numbers, const float* h_LJ_A,
+                   const float* h_LJ_B, const int* h_atom_LJ_type, float cutoff);
+
+/* LENNARD_JONES_INFORMATION::LJ_PME_Direct_Force_With_Atom_Energy_And_Virial
+ * (Lennard_Jones_force.cpp:359-417; declaration Lennard_Jones_force.h:138-144).
+ * LJ + erfc-screened Coulomb over the half list d_nl for pairs with r < cutoff.
+ * d_frc += forces; with need_atom_energy: d_atom_energy += LJ + Coulomb per
+ * atom and d_atom_direct_pme_energy = Coulomb per atom; with need_virial:
+ * d_atom_virial += per-atom virial (reference convention: -f (x) dr, with the
+ * off-diagonals symmetrised, vector.hpp:336-347).  Any output may be NULL
+ * (then kept in the fixed-point accumulators). */
+int snb_lj_pme_direct_force(snb_context* ctx, int atom_numbers, int local_atom_numbers,
+                            int solvent_numbers, int ghost_numbers,
+                            const snb_vector* d_crd, const float* d_charge,
+                            snb_vector* d_frc, snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
+                            const snb_atom_group* d_nl, float pme_beta,
+                            int need_atom_energy, float* d_atom_energy,
+                            int need_virial, snb_ltmatrix3* d_atom_virial,
+                            float* d_atom_direct_pme_energy);
+
+/* LENNARD_JONES_INFORMATION::Long_Range_Correction (Lennard_Jones_force.cpp:320-341):
+ * d_virial[0] diagonal += 2 f/V, d_potential[0] += f/V. */
+int snb_lj_long_range_correction(snb_context* ctx, int need_pressure, snb_ltmatrix3* d_virial,
+                                 int need_potential, float* d_potential, float volume);
+
+/* LENNARD_JONES_INFORMATION::Step_Print (Lennard_Jones_force.cpp:419-433):
+ * LJ_short = sum of per-atom LJ energies of the last energy call (exact
+ * int64 sum), LJ_long = f/V.  Synchronises. */
+int snb_lj_energy(snb_context* ctx, double* lj_short, double* lj_long);
+
+/* -------------------------------------------------------------------- PME */
+/* Particle_Mesh::Initial (PM_force.cpp:293-652) with the controller keys
+ * resolved by the caller: PM_Direct_Tolerance (1e-5), PM_fftx/y/z (<= 0 =
+ * automatic from box_length / PM_grid_spacing, common.cpp:639-659),
+ * PME_update_interval (1).  beta comes from the tolerance (:244-272). */
+int snb_pme_initial(snb_context* ctx, snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
+                    snb_vector box_length, float cutoff, float tolerance,
+                    int fftx, int ffty, int fftz, float grid_spacing,
+                    int update_interval);
+int snb_pme_parameters(snb_context* ctx, float* beta, int* fftx, int* ffty, int* fftz);
+
+/* Particle_Mesh::MPI_PME_Excluded_Force_With_Atom_Energy (PM_force.cpp:1816-1839,
+ * kernel :1743-1814) on one rank: removes the erf part of the Ewald
+ * interaction of every excluded pair.  atom_local/atom_local_id may be NULL;
+ * the exclusion arrays may be NULL when snb_set_exclusions was called. */
+int snb_pme_excluded_force(snb_context* ctx, int atom_numbers, const int* d_atom_local,
+                           const int* d_atom_local_id, const snb_vector* d_crd,
+                           snb_ltmatrix3 cell, snb_ltmatrix3 rcell, const float* d_charge,
+                           const int* d_excluded_list_start, const int* d_excluded_list,
+                           const int* d_excluded_numbers, snb_vector* d_frc,
+                           int need_energy, float* d_atom_energy, int need_virial,
+                           snb_ltmatrix3* d_atom_virial);
+
+/* Particle_Mesh::PME_Reciprocal_Force_With_Energy_And_Virial (PM_force.cpp:1067-1165):
+ * B-spline spread, 3-D R2C FFT, influence-function convolution, C2R, force
+ * gather; with need_energy adds reciprocal + self + neutralising energy to
+ * d_potential[0]; with need_virial adds the reciprocal virial to d_virial[0]. */
+int snb_pme_reciprocal_force(snb_context* ctx, const snb_vector* d_crd, snb_ltmatrix3 cell,
+                             snb_ltmatrix3 rcell, const float* d_charge, snb_vector* d_frc,
+                             int need_virial, int need_energy, snb_ltmatrix3* d_virial,
+                             float* d_potential, int step);
+
+/* Particle_Mesh::Step_Print (PM_force.cpp:1841-1866): the four PM terms of
+ * the last energy evaluation (FP64).  Synchronises. */
+int snb_pme_energy(snb_context* ctx, double* direct, double* reciprocal, double* self,
+                   double* correction);
+/* Reciprocal virial of the last need_virial call (FP64, LTMatrix3 order). */
+int snb_pme_reciprocal_virial(snb_context* ctx, double* virial6);
+
+/* ---------------------------------------------------- fixed-point accumulators */
+/* d_frc[i] += accumulated force, then clears the accumulator. */
+int snb_flush_forces(snb_context* ctx, snb_vector* d_frc);
+/* Copies the int64 force accumulators (SoA x[n] | y[n] | z[n]) to d_out. */
+int snb_copy_fixed_forces(snb_context* ctx, int64_t* d_out);
+int snb_clear_fixed_forces(snb_context* ctx);
+/* Copies the int64 per-atom accumulators: which = 0 LJ energy, 1 direct
+ * Coulomb energy, 2 exclusion-correction energy (n each), 3 virial (6 x n, SoA). */
+int snb_copy_fixed_atom_terms(snb_context* ctx, int which, int64_t* d_out);
+/* Copies the PME real-space grids of the last reciprocal call (float,
+ * [fftx][ffty][fftz]): which = 0 charge grid Q, 1 potential grid (C2R output). */
+int snb_copy_pme_grid(snb_context* ctx, int which, float* d_out);
+/* SPONGE_FFT_WRAPPER::R2C / C2R (utils/fft.hpp:93-109) on the PME grid of
+ * this context, with the library's own radix kernels: unnormalised 3-D
+ * real-to-complex transform of float[fftx][ffty][fftz] into complex
+ * float2[fftx][ffty][fftz/2+1] (exponent sign -1), and the inverse (+1). */
+int snb_fft_r2c(snb_context* ctx, const float* d_real, float* d_complex);
+int snb_fft_c2r(snb_context* ctx, const float* d_complex, float* d_real);
+
+/* ------------------------------------------------------------ profiling */
+/* Kernel timing with HIP events recorded on the context stream around each
+ * launch of a kernel group (no counterpart in the reference, whose
+ * TIME_RECORDER synchronises the device, utils/control/time_recorder.hpp:9-43):
+ * SNB_PROF_PAIR = the LJ + direct Coulomb kernel alone, SNB_PROF_NL = the
+ * neighbor-list update (check + rebuild kernels), SNB_PROF_PME = the
+ * reciprocal pipeline (prep .. gather), SNB_PROF_EXCL = the exclusion
+ * correction kernel.  snb_profile_read synchronises and returns the summed
+ * milliseconds and launch count since the last reset. */
+#define SNB_PROF_PAIR 0
+#define SNB_PROF_NL 1
+#define SNB_PROF_PME 2
+#define SNB_PROF_EXCL 3
+int snb_profile_enable(snb_context* ctx, int enable);
+int snb_profile_read(snb_context* ctx, int group, int reset, double* total_ms, int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPONGE_NB_H */

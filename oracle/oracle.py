@@ -1,0 +1,331 @@
+"""ctypes front end of the CPU oracle (oracle/snb_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product library never
+imports it.
+
+Arrays follow the reference's conventions: coordinates float32[n, 3]
+(VECTOR AoS), cell/rcell as LTMatrix3 float32[6] = {a11, a21, a22, a31, a32,
+a33}, charges already multiplied by 18.2223, LJ A/B tables already x12 / x6
+(the in-memory form of Xponge::Native_Load_LJ, xponge/load/native/lj.hpp:47,57).
+Fixed-point accumulators are int64 at scale 2^32, SoA by component.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libsnb_oracle.so")
+FIXED_SCALE = 2.0 ** 32
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_f = ctypes.c_float
+_d = ctypes.c_double
+_i = ctypes.c_int
+_p = ctypes.c_void_p
+
+
+def _declare(L):
+    L.orc_get_beta.restype = _f
+    L.orc_get_beta.argtypes = [_f, _f]
+    L.orc_fft_size.restype = _i
+    L.orc_fft_size.argtypes = [_f]
+    L.orc_check_2357.restype = _i
+    L.orc_check_2357.argtypes = [_i]
+    L.orc_bspline_moduli_f64.argtypes = [_i, _p]
+    L.orc_bc_f64.argtypes = [_i, _i, _i, _p, _d, _d, _p]
+    L.orc_nl_build.restype = _i
+    L.orc_nl_build.argtypes = [_i, _p, _p, _p, _f, _f, _p, _p, _p, _i, _p, _p]
+    L.orc_pair_f64.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p, _f, _f, _p, _p, _i,
+                               _p, _p, _p, _p]
+    L.orc_pair_twin.argtypes = L.orc_pair_f64.argtypes
+    L.orc_excl_f64.argtypes = [_i, _p, _p, _p, _p, _f, _p, _p, _p, _p, _p, _p]
+    L.orc_excl_twin.argtypes = L.orc_excl_f64.argtypes
+    L.orc_fft3d_f64.argtypes = [_p, _i, _i, _i, _i]
+    L.orc_dft3d_naive_f64.argtypes = [_p, _p, _i, _i, _i, _i]
+    L.orc_pme_f64.argtypes = [_i, _p, _p, _p, _i, _i, _i, _f, _d, _p, _p, _p, _p, _p]
+    L.orc_pme_spread_twin.argtypes = [_i, _p, _p, _p, _i, _i, _i, _p]
+    L.orc_pme_gather_twin.argtypes = [_i, _p, _p, _p, _i, _i, _i, _p, _f, _p]
+    L.orc_ewald_f64.argtypes = [_i, _p, _p, _p, _d, _d, _i, _p, _p, _p, _p]
+    L.orc_lj_c6_total.restype = _d
+    L.orc_lj_c6_total.argtypes = [_i, _p, _i, _p]
+    for name in ("orc_twin_expf", "orc_twin_erfc", "orc_twin_erf_minus_gauss",
+                 "orc_twin_rsqrtf"):
+        getattr(L, name).restype = _f
+        getattr(L, name).argtypes = [_f]
+    L.orc_to_fixed.restype = ctypes.c_int64
+    L.orc_to_fixed.argtypes = [_f]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _c(a, dt):
+    return None if a is None else np.ascontiguousarray(a, dtype=dt)
+
+
+# --------------------------------------------------------------------------
+# scalar host precompute
+# --------------------------------------------------------------------------
+def get_beta(cutoff, tolerance):
+    return float(lib().orc_get_beta(cutoff, tolerance))
+
+
+def fft_size(length):
+    return int(lib().orc_fft_size(length))
+
+
+def check_2357(n):
+    return bool(lib().orc_check_2357(n))
+
+
+def bspline_moduli(K):
+    out = np.zeros(K, np.float64)
+    lib().orc_bspline_moduli_f64(K, _ptr(out))
+    return out
+
+
+def bc_table(fftx, ffty, fftz, rcell, beta, volume):
+    out = np.zeros((fftx, ffty, fftz // 2 + 1), np.float64)
+    rc = _c(rcell, np.float32)
+    lib().orc_bc_f64(fftx, ffty, fftz, _ptr(rc), beta, volume, _ptr(out))
+    return out
+
+
+# --------------------------------------------------------------------------
+# neighbor list
+# --------------------------------------------------------------------------
+def nl_build(crd, cell, rcell, cutoff, skin, excl=None, max_nb=1200):
+    """Returns (counts int32[n], rows int32[n, max_nb], overflow bool)."""
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    counts = np.zeros(n, np.int32)
+    rows = np.zeros((n, max_nb), np.int32)
+    es, en, el = (None, None, None) if excl is None else excl_arrays(excl)
+    ov = lib().orc_nl_build(n, _ptr(crd), _ptr(_c(cell, np.float32)),
+                            _ptr(_c(rcell, np.float32)), cutoff, skin, _ptr(es),
+                            _ptr(en), _ptr(el), max_nb, _ptr(counts), _ptr(rows))
+    return counts, rows, bool(ov)
+
+
+def excl_arrays(excl):
+    """excl = (start, numbers, list) int32 arrays (the reference CSR)."""
+    start, numbers, lst = excl
+    lst = _c(lst, np.int32)
+    if lst.size == 0:
+        lst = np.zeros(1, np.int32)
+    return _c(start, np.int32), _c(numbers, np.int32), lst
+
+
+# --------------------------------------------------------------------------
+# LJ + direct Coulomb
+# --------------------------------------------------------------------------
+def pair_f64(crd, charge, ltype, lj_a, lj_b, cell, rcell, cutoff, beta, counts,
+             rows, virial=True):
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    frc = np.zeros((n, 3), np.float64)
+    e_lj = np.zeros(n, np.float64)
+    e_c = np.zeros(n, np.float64)
+    vir = np.zeros((n, 6), np.float64) if virial else None
+    rows = _c(rows, np.int32)
+    lib().orc_pair_f64(n, _ptr(crd), _ptr(_c(charge, np.float32)),
+                       _ptr(_c(ltype, np.int32)), _ptr(_c(lj_a, np.float32)),
+                       _ptr(_c(lj_b, np.float32)), _ptr(_c(cell, np.float32)),
+                       _ptr(_c(rcell, np.float32)), cutoff, beta,
+                       _ptr(_c(counts, np.int32)), _ptr(rows), rows.shape[1],
+                       _ptr(frc), _ptr(e_lj), _ptr(e_c), _ptr(vir))
+    return frc, e_lj, e_c, vir
+
+
+def pair_twin(crd, charge, ltype, lj_a, lj_b, cell, rcell, cutoff, beta, counts,
+              rows, virial=True):
+    """int64 accumulators: frc[3, n], e_lj[n], e_c[n], vir[6, n]."""
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    frc = np.zeros((3, n), np.int64)
+    e_lj = np.zeros(n, np.int64)
+    e_c = np.zeros(n, np.int64)
+    vir = np.zeros((6, n), np.int64) if virial else None
+    rows = _c(rows, np.int32)
+    lib().orc_pair_twin(n, _ptr(crd), _ptr(_c(charge, np.float32)),
+                        _ptr(_c(ltype, np.int32)), _ptr(_c(lj_a, np.float32)),
+                        _ptr(_c(lj_b, np.float32)), _ptr(_c(cell, np.float32)),
+                        _ptr(_c(rcell, np.float32)), cutoff, beta,
+                        _ptr(_c(counts, np.int32)), _ptr(rows), rows.shape[1],
+                        _ptr(frc), _ptr(e_lj), _ptr(e_c), _ptr(vir))
+    return frc, e_lj, e_c, vir
+
+
+# --------------------------------------------------------------------------
+# exclusion correction
+# --------------------------------------------------------------------------
+def excl_f64(crd, charge, cell, rcell, beta, excl, virial=True):
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    frc = np.zeros((n, 3), np.float64)
+    e = np.zeros(n, np.float64)
+    vir = np.zeros((n, 6), np.float64) if virial else None
+    es, en, el = excl_arrays(excl)
+    lib().orc_excl_f64(n, _ptr(crd), _ptr(_c(charge, np.float32)),
+                       _ptr(_c(cell, np.float32)), _ptr(_c(rcell, np.float32)), beta,
+                       _ptr(es), _ptr(en), _ptr(el), _ptr(frc), _ptr(e), _ptr(vir))
+    return frc, e, vir
+
+
+def excl_twin(crd, charge, cell, rcell, beta, excl, virial=True):
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    frc = np.zeros((3, n), np.int64)
+    e = np.zeros(n, np.int64)
+    vir = np.zeros((6, n), np.int64) if virial else None
+    es, en, el = excl_arrays(excl)
+    lib().orc_excl_twin(n, _ptr(crd), _ptr(_c(charge, np.float32)),
+                        _ptr(_c(cell, np.float32)), _ptr(_c(rcell, np.float32)), beta,
+                        _ptr(es), _ptr(en), _ptr(el), _ptr(frc), _ptr(e), _ptr(vir))
+    return frc, e, vir
+
+
+# --------------------------------------------------------------------------
+# FFT and PME
+# --------------------------------------------------------------------------
+def fft3d(a, sign=-1):
+    a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    n0, n1, n2 = a.shape
+    lib().orc_fft3d_f64(_ptr(a), n0, n1, n2, sign)
+    return a
+
+
+def dft3d_naive(a, sign=-1):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    out = np.zeros_like(a)
+    n0, n1, n2 = a.shape
+    lib().orc_dft3d_naive_f64(_ptr(a), _ptr(out), n0, n1, n2, sign)
+    return out
+
+
+def pme_f64(crd, charge, rcell, grid, beta, volume, want_grids=False):
+    """Returns (frc double[n,3], e_rec, vir[6], Q, phi) (grids only on request)."""
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    fx, fy, fz = grid
+    frc = np.zeros((n, 3), np.float64)
+    e = np.zeros(1, np.float64)
+    vir = np.zeros(6, np.float64)
+    q = np.zeros(grid, np.float64) if want_grids else None
+    phi = np.zeros(grid, np.float64) if want_grids else None
+    lib().orc_pme_f64(n, _ptr(crd), _ptr(_c(charge, np.float32)),
+                      _ptr(_c(rcell, np.float32)), fx, fy, fz, beta, volume,
+                      _ptr(frc), _ptr(e), _ptr(vir), _ptr(q), _ptr(phi))
+    return frc, float(e[0]), vir, q, phi
+
+
+def pme_spread_twin(crd, charge, rcell, grid):
+    crd = _c(crd, np.float32)
+    q = np.zeros(grid, np.int64)
+    lib().orc_pme_spread_twin(crd.shape[0], _ptr(crd), _ptr(_c(charge, np.float32)),
+                              _ptr(_c(rcell, np.float32)), grid[0], grid[1], grid[2],
+                              _ptr(q))
+    return q
+
+
+def pme_gather_twin(crd, charge, rcell, grid, phi, update_interval=1.0):
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    frc = np.zeros((3, n), np.int64)
+    lib().orc_pme_gather_twin(n, _ptr(crd), _ptr(_c(charge, np.float32)),
+                              _ptr(_c(rcell, np.float32)), grid[0], grid[1], grid[2],
+                              _ptr(_c(phi, np.float32)), update_interval, _ptr(frc))
+    return frc
+
+
+def ewald_f64(crd, q, box, beta, rcut, kmax):
+    crd = _c(crd, np.float64)
+    n = crd.shape[0]
+    frc = np.zeros((n, 3), np.float64)
+    ed, er, es = (np.zeros(1), np.zeros(1), np.zeros(1))
+    lib().orc_ewald_f64(n, _ptr(crd), _ptr(_c(q, np.float64)), _ptr(_c(box, np.float64)),
+                        beta, rcut, kmax, _ptr(frc), _ptr(ed), _ptr(er), _ptr(es))
+    return frc, float(ed[0]), float(er[0]), float(es[0])
+
+
+def lj_c6_total(ltype, ntypes, lj_b):
+    t = _c(ltype, np.int32)
+    return float(lib().orc_lj_c6_total(t.size, _ptr(t), ntypes, _ptr(_c(lj_b, np.float32))))
+
+
+# --------------------------------------------------------------------------
+# assembled terms (mdout columns), following the reference's Step_Print
+# --------------------------------------------------------------------------
+def self_energy(charge, beta, volume):
+    """PM_force.cpp:1146-1159: -beta/sqrt(pi) sum q^2 - pi/(2 beta^2 V) (sum q)^2."""
+    q = np.asarray(charge, np.float64)
+    return (-beta / np.sqrt(np.pi) * np.sum(q * q)
+            - 0.5 * np.pi / (beta * beta * volume) * np.sum(q) ** 2)
+
+
+def lj_long_range_factor(ltype, ntypes, lj_b, cutoff):
+    """Lennard_Jones_force.cpp:246-272: float(total C6) * (-2/3 pi / rc^3 / 6)."""
+    total = np.float32(lj_c6_total(ltype, ntypes, lj_b))
+    f = np.float32(-2.0) / np.float32(3.0) * np.float32(3.1415926535897932)
+    for _ in range(3):
+        f = f / np.float32(cutoff)
+    f = f / np.float32(6.0)
+    return float(total * f)
+
+
+def nonbonded_terms_f64(sysd, cutoff=None, skin=2.0, tolerance=1e-5, grid=None,
+                        want_forces=True):
+    """All mdout terms of the path for a system dict (see sponge_amd.systems):
+    LJ_short, LJ_long, PM_direct, PM_reciprocal, PM_self, PM_correction, PM,
+    plus total nonbonded forces (double[n,3]) and virial (reference convention)."""
+    crd, q = sysd["crd"], sysd["charge"]
+    cell, rcell = sysd["cell"], sysd["rcell"]
+    box = sysd["box"]
+    rc = float(sysd.get("cutoff", 10.0) if cutoff is None else cutoff)
+    beta = get_beta(rc, tolerance)
+    if grid is None:
+        grid = tuple(fft_size(float(b)) for b in box)
+    vol = float(np.float32(cell[0]) * np.float32(cell[2]) * np.float32(cell[5]))
+    excl = (sysd["excl_start"], sysd["excl_numbers"], sysd["excl_list"])
+    counts, rows, ov = nl_build(crd, cell, rcell, rc, skin, excl,
+                                max_nb=sysd.get("max_nb", 1200))
+    f_pair, e_lj, e_c, v_pair = pair_f64(crd, q, sysd["lj_type"], sysd["lj_a"],
+                                         sysd["lj_b"], cell, rcell, rc, beta, counts,
+                                         rows)
+    f_ex, e_ex, v_ex = excl_f64(crd, q, cell, rcell, beta, excl)
+    f_rec, e_rec, v_rec, _, _ = pme_f64(crd, q, rcell, grid, beta, vol)
+    e_self = self_energy(q, beta, vol)
+    lrf = lj_long_range_factor(sysd["lj_type"], sysd["lj_ntypes"], sysd["lj_b"], rc)
+    out = dict(beta=beta, grid=grid, volume=vol, nl_overflow=ov,
+               nl_counts=counts, nl_rows=rows,
+               LJ_short=float(e_lj.sum()), LJ_long=lrf / vol,
+               PM_direct=float(e_c.sum()), PM_reciprocal=e_rec, PM_self=e_self,
+               PM_correction=float(e_ex.sum()))
+    out["LJ"] = out["LJ_short"] + out["LJ_long"]
+    out["PM"] = out["PM_direct"] + e_rec + e_self + out["PM_correction"]
+    if want_forces:
+        out["frc"] = f_pair + f_ex + f_rec
+        out["frc_pair"], out["frc_excl"], out["frc_recip"] = f_pair, f_ex, f_rec
+        out["virial_atoms"] = v_pair + v_ex
+        out["virial_recip"] = v_rec
+    return out

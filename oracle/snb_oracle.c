@@ -1,0 +1,1052 @@
+/*
+ * snb_oracle.c -- CPU restatement of SPONGE's nonbonded force path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this file; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, and only as the checker.
+ *
+ * Two layers, both plain C (gcc, -ffp-contract=off, no fast-math):
+ *
+ *  FP64 layer (orc_*_f64): the reference algorithm evaluated in double
+ *  precision from the same float inputs.  It is the accuracy anchor for the
+ *  5e-5 relative-RMS force bound and the 1e-6 energy/virial bound.
+ *
+ *  FP32 twin (orc_*_twin): the float operation sequence the HIP kernels
+ *  execute, written independently here, with every per-pair / per-point term
+ *  rounded to int64 fixed point (scale 2^32) before any summation.  Because
+ *  integer sums are associative, the GPU accumulators must equal these
+ *  bit-for-bit whatever the summation order on the device.
+ *
+ * Reference citations are paths inside yuhaosimba/SPONGE2.0 (file:line).
+ *
+ * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ *   orc_get_beta            survey-measured reference values (rc 8, 12) + erfc(beta rc)/rc = tol
+ *   orc_fft_size            reference grid sizes for the fixture boxes + 2/3/5/7-smoothness
+ *   orc_bspline_moduli_f64  closed form |sum M4(k+1) e^{2 pi i m k/K}|^-2 at m=0, K/2
+ *   orc_nl_build            brute-force set definition itself; invariants (no dup, j>i, d<rc+skin)
+ *   orc_pair_f64            two-atom closed forms; GROMACS fixture LJ_sr within the reference's test tolerance
+ *   orc_pme_f64             exact Ewald sum (orc_ewald_f64) on small boxes; NaCl Madelung constant;
+ *                           GROMACS fixture coulomb_sr+coulomb_recip within the reference's tolerance
+ *   orc_fft3d_f64           naive DFT on small grids
+ *   orc_excl_f64            closed form erf; E_excl + E_direct(no cutoff) = bare Coulomb
+ *   orc_*_twin              the matching orc_*_f64 within float tolerance
+ *   orc_ewald_f64           textbook Madelung constant of rock salt (1.747565)
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------ */
+/* constants                                                                 */
+/* ------------------------------------------------------------------------ */
+#define ORC_PI_D 3.14159265358979323846
+/* PM_force.cpp:15-17 and Lennard_Jones_force.h:7 */
+#define ORC_PI_F 3.1415926f
+#define ORC_TWO_OVER_SQRT_PI_F 1.1283791670218446f
+#define ORC_FIXED_SCALE 4294967296.0 /* 2^32 */
+
+/* ------------------------------------------------------------------------ */
+/* small helpers                                                             */
+/* ------------------------------------------------------------------------ */
+static inline float f_from_bits(uint32_t u)
+{
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+static inline uint32_t bits_from_f(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+
+/* round(f * 2^32) to nearest-even, exact for |f| < 2^19, via the 1.5*2^52
+ * magic constant: the low mantissa bits of the sum hold the integer. */
+static inline int64_t to_fixed(float f)
+{
+    double d = fma((double)f, ORC_FIXED_SCALE, 6755399441055744.0);
+    uint64_t u;
+    memcpy(&u, &d, 8);
+    return (int64_t)u - (int64_t)0x4338000000000000LL;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP32 math written from published methods (twin of the kernels' math)     */
+/* ------------------------------------------------------------------------ */
+/* exp(y) for y in [-87, 0]: Cody-Waite reduction y = n ln2 + r with ln2 split
+ * into a 16-bit head and a tail, then a degree-6 polynomial for e^r fitted
+ * (weighted least squares, Lawson iteration) on [-ln2/2, ln2/2], then 2^n
+ * built directly in the exponent field. */
+static inline float twin_expf(float y)
+{
+    y = fmaxf(y, -87.0f);
+    float n = rintf(y * 1.44269504088896341f);
+    float r = fmaf(-n, 0.693145751953125f, y);
+    r = fmaf(-n, 1.428606765330187045e-06f, r);
+    float p = 0.0013836846f;
+    p = fmaf(p, r, 0.008374816f);
+    p = fmaf(p, r, 0.041668225f);
+    p = fmaf(p, r, 0.1666642f);
+    p = fmaf(p, r, 0.4999999f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    int32_t ni = (int32_t)n;
+    return p * f_from_bits((uint32_t)(ni + 127) << 23);
+}
+
+/* erfc(x) = t exp(-x^2) P(t), t = 1/(1 + 0.3275911 x) (the substitution of
+ * Abramowitz & Stegun 7.1.26) with a degree-7 P fitted on x in [0, 5] for
+ * minimum relative error (max 4.5e-8 in exact arithmetic). ex = exp(-x^2). */
+static inline float twin_erfc_core(float x, float ex)
+{
+    float t = 1.0f / fmaf(0.3275911f, x, 1.0f);
+    float q = -0.1413204f;
+    q = fmaf(q, t, 0.49561852f);
+    q = fmaf(q, t, -0.39975524f);
+    q = fmaf(q, t, 0.44215718f);
+    q = fmaf(q, t, 0.0353266f);
+    q = fmaf(q, t, 0.20112583f);
+    q = fmaf(q, t, 0.18191357f);
+    q = fmaf(q, t, 0.18493392f);
+    return (t * q) * ex;
+}
+
+/* h(x) = erf(x) - (2/sqrt(pi)) x exp(-x^2), needed without cancellation by
+ * the exclusion correction.  For x < 0.5 the Maclaurin series
+ * h = (2/sqrt(pi)) sum_{n>=1} (-1)^{n+1} 2n x^{2n+1} / (n! (2n+1)); above,
+ * erf = 1 - erfc. */
+static inline float twin_erf_minus_gauss(float x, float ex)
+{
+    if (x < 0.5f)
+    {
+        float x2 = x * x;
+        float s = -2.6339383e-05f;          /* n = 8 */
+        s = fmaf(s, x2, 2.08959105e-04f);   /* n = 7 */
+        s = fmaf(s, x2, -1.44663996e-03f);  /* n = 6 */
+        s = fmaf(s, x2, 8.54832702e-03f);   /* n = 5 */
+        s = fmaf(s, x2, -4.1791821e-02f);   /* n = 4 */
+        s = fmaf(s, x2, 1.61197024e-01f);   /* n = 3 */
+        s = fmaf(s, x2, -4.51351667e-01f);  /* n = 2 */
+        s = fmaf(s, x2, 7.52252778e-01f);   /* n = 1 */
+        return (s * x2) * x;
+    }
+    float erfv = 1.0f - twin_erfc_core(x, ex);
+    return erfv - (x * ORC_TWO_OVER_SQRT_PI_F) * ex;
+}
+
+/* 1/sqrt(x) for normal x > 0: integer seed 0x5f375a86 - (bits >> 1)
+ * (Lomont, "Fast inverse square root", 2003) refined by three Newton steps
+ * y <- y (1.5 - (x/2) y^2); only IEEE mul/fma, so it is reproducible. */
+static inline float twin_rsqrtf(float x)
+{
+    float y = f_from_bits(0x5f375a86u - (bits_from_f(x) >> 1));
+    float h = 0.5f * x;
+    y = y * fmaf(-(h * y), y, 1.5f);
+    y = y * fmaf(-(h * y), y, 1.5f);
+    y = y * fmaf(-(h * y), y, 1.5f);
+    return y;
+}
+
+/* ------------------------------------------------------------------------ */
+/* periodic displacement (utils/vector.hpp:316-327), float and double        */
+/* cell/rcell are LTMatrix3 {a11, a21, a22, a31, a32, a33}                   */
+/* ------------------------------------------------------------------------ */
+static inline void pbc_disp_f(const float* a, const float* b, const float* cell,
+                              const float* rcell, float* d)
+{
+    float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    float sx = (dx * rcell[0] + dy * rcell[1]) + dz * rcell[3];
+    float sy = dy * rcell[2] + dz * rcell[4];
+    float sz = dz * rcell[5];
+    float kx = floorf(sx + 0.5f), ky = floorf(sy + 0.5f), kz = floorf(sz + 0.5f);
+    d[0] = dx - ((kx * cell[0] + ky * cell[1]) + kz * cell[3]);
+    d[1] = dy - (ky * cell[2] + kz * cell[4]);
+    d[2] = dz - kz * cell[5];
+}
+
+static inline void pbc_disp_d(const float* a, const float* b, const float* cell,
+                              const float* rcell, double* d)
+{
+    double dx = (double)a[0] - b[0], dy = (double)a[1] - b[1],
+           dz = (double)a[2] - b[2];
+    double sx = dx * rcell[0] + dy * rcell[1] + dz * rcell[3];
+    double sy = dy * rcell[2] + dz * rcell[4];
+    double sz = dz * rcell[5];
+    double kx = floor(sx + 0.5), ky = floor(sy + 0.5), kz = floor(sz + 0.5);
+    d[0] = dx - (kx * cell[0] + ky * cell[1] + kz * cell[3]);
+    d[1] = dy - (ky * cell[2] + kz * cell[4]);
+    d[2] = dz - kz * cell[5];
+}
+
+/* Lennard_Jones_force.h:18-27: triangular type-pair index */
+static inline int lj_pair_index(int a, int b)
+{
+    int hi = a > b ? a : b, lo = a > b ? b : a;
+    return ((hi * (hi + 1)) >> 1) + lo;
+}
+
+/* neighbor_list.cpp:618-637: the reference tests membership only when j lies
+ * between the first and the last entry of i's exclusion list. */
+static int is_excluded(int i, int j, const int* excl_start, const int* excl_num,
+                       const int* excl_list)
+{
+    int n = excl_num[i];
+    if (n <= 0) return 0;
+    int s = excl_start[i];
+    int lo = excl_list[s], hi = excl_list[s + n - 1];
+    if (j < lo || j > hi) return 0;
+    for (int k = s; k < s + n; k++)
+        if (excl_list[k] == j) return 1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* host precompute                                                           */
+/* ------------------------------------------------------------------------ */
+/* PM_force.cpp:244-272: doubling then 50 bisection steps, in float. */
+ORC_EXPORT float orc_get_beta(float cutoff, float tolerance)
+{
+    float beta = 0.0f, low, high, tempf;
+    int ilow, ihigh;
+    high = 1.0f;
+    ihigh = 1;
+    while (1)
+    {
+        tempf = erfcf(high * cutoff) / cutoff;
+        if (tempf <= tolerance) break;
+        high *= 2;
+        ihigh++;
+    }
+    ihigh += 50;
+    low = 0.0f;
+    for (ilow = 1; ilow < ihigh; ilow++)
+    {
+        beta = (low + high) / 2;
+        tempf = erfcf(beta * cutoff) / cutoff;
+        if (tempf >= tolerance)
+            low = beta;
+        else
+            high = beta;
+    }
+    return beta;
+}
+
+/* common.cpp:521-557 */
+ORC_EXPORT int orc_check_2357(int number)
+{
+    static const int primes[4] = {2, 3, 5, 7};
+    if (number <= 0) return 0;
+    for (int p = 0; p < 4; p++)
+        while (number % primes[p] == 0) number /= primes[p];
+    return number == 1;
+}
+
+/* common.cpp:639-659 */
+ORC_EXPORT int orc_fft_size(float length)
+{
+    int tempi = (int)ceil(length + 3) >> 2 << 2;
+    if (tempi >= 60 && tempi <= 68)
+        tempi = 64;
+    else if (tempi >= 120 && tempi <= 136)
+        tempi = 128;
+    else if (tempi >= 240 && tempi <= 272)
+        tempi = 256;
+    else if (tempi >= 480 && tempi <= 544)
+        tempi = 512;
+    else if (tempi >= 960 && tempi <= 1088)
+        tempi = 1024;
+    while (!orc_check_2357(tempi)) tempi += 4;
+    return tempi;
+}
+
+/* PM_force.cpp:27-62 (order 4): b(m) = |e^{2 pi i 3 m/K} /
+ * sum_{k=0}^{2} M4(k+1) e^{2 pi i m k/K}|^2 with M4(1)=1/6, M4(2)=2/3,
+ * M4(3)=1/6 (the cardinal cubic B-spline at the integers). */
+ORC_EXPORT void orc_bspline_moduli_f64(int K, double* out)
+{
+    static const double M4[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
+    for (int m = 0; m < K; m++)
+    {
+        double re = 0, im = 0;
+        for (int k = 0; k < 3; k++)
+        {
+            double a = 2.0 * ORC_PI_D * m * k / K;
+            re += M4[k] * cos(a);
+            im += M4[k] * sin(a);
+        }
+        out[m] = 1.0 / (re * re + im * im);
+    }
+}
+
+/* PM_force.cpp:544-612: BC over the half spectrum kz <= fftz/2, index
+ * kx*ffty*(fftz/2+1) + ky*(fftz/2+1) + kz; m = MultiplyTranspose(k', rcell). */
+ORC_EXPORT void orc_bc_f64(int fx, int fy, int fz, const float* rcell,
+                           double beta, double volume, double* bc)
+{
+    double* b1 = (double*)malloc(sizeof(double) * fx);
+    double* b2 = (double*)malloc(sizeof(double) * fy);
+    double* b3 = (double*)malloc(sizeof(double) * fz);
+    orc_bspline_moduli_f64(fx, b1);
+    orc_bspline_moduli_f64(fy, b2);
+    orc_bspline_moduli_f64(fz, b3);
+    int fc = fz / 2 + 1;
+    double mpre = ORC_PI_D * ORC_PI_D / (beta * beta);
+    for (int kx = 0; kx < fx; kx++)
+    {
+        double kxr = kx > fx / 2 ? kx - fx : kx;
+        for (int ky = 0; ky < fy; ky++)
+        {
+            double kyr = ky > fy / 2 ? ky - fy : ky;
+            for (int kz = 0; kz < fc; kz++)
+            {
+                double mx = kxr * rcell[0];
+                double my = kxr * rcell[1] + kyr * rcell[2];
+                double mz = kxr * rcell[3] + kyr * rcell[4] + kz * rcell[5];
+                double msq = mx * mx + my * my + mz * mz;
+                size_t idx = ((size_t)kx * fy + ky) * fc + kz;
+                if (kx + ky + kz == 0)
+                    bc[idx] = 0.0;
+                else
+                    bc[idx] = exp(-mpre * msq) / (ORC_PI_D * msq * volume) *
+                              b1[kx] * b2[ky] * b3[kz];
+            }
+        }
+    }
+    free(b1);
+    free(b2);
+    free(b3);
+}
+
+/* ------------------------------------------------------------------------ */
+/* neighbor list (neighbor_list.cpp:206-275, :495-591, :594-665)            */
+/* ------------------------------------------------------------------------ */
+/* The reference's half list as a set: j in row i iff j > i, the float
+ * displacement of Get_Periodic_Displacement(crd_i, crd_j) has
+ * (x*x + y*y) + z*z < (0.5(rc+skin))^2 * 4, and j is not excluded from i.
+ * Rows come out sorted ascending.  Brute force O(N^2).
+ * Returns 1 if any row overflowed max_nb (the row is then truncated). */
+ORC_EXPORT int orc_nl_build(int n, const float* crd, const float* cell,
+                            const float* rcell, float cutoff, float skin,
+                            const int* excl_start, const int* excl_num,
+                            const int* excl_list, int max_nb, int* counts,
+                            int* rows)
+{
+    float g = 0.5f * (cutoff + skin);
+    float cs2 = g * g * 4.0f;
+    int overflow = 0;
+    for (int i = 0; i < n; i++)
+    {
+        int c = 0;
+        for (int j = i + 1; j < n; j++)
+        {
+            float d[3];
+            pbc_disp_f(crd + 3 * i, crd + 3 * j, cell, rcell, d);
+            float d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+            if (!(d2 < cs2)) continue;
+            if (excl_num && is_excluded(i, j, excl_start, excl_num, excl_list))
+                continue;
+            if (c < max_nb)
+                rows[(size_t)i * max_nb + c] = j;
+            else
+                overflow = 1;
+            c++;
+        }
+        counts[i] = c < max_nb ? c : max_nb;
+    }
+    return overflow;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LJ + direct Coulomb (Lennard_Jones_force.cpp:43-136, Lennard_Jones_force.h:43-73) */
+/* ------------------------------------------------------------------------ */
+/* FP64: per pair (i row atom, j entry), dr = pbc(r_j - r_i); if r < rc:
+ *   F/r = (B - A r^-6) r^-8 - qq r^-3 (2 beta r/sqrt(pi) e^{-beta^2 r^2} + erfc(beta r))
+ *   f = (F/r) dr; frc[i] += f; frc[j] -= f;   virial[i] -= f (x) dr (symmetrised)
+ *   e_lj[i] += A/12 r^-12 - B/6 r^-6;   e_c[i] += qq erfc(beta r)/r
+ * A, B are the in-memory tables (file values x12 and x6). */
+ORC_EXPORT void orc_pair_f64(int n, const float* crd, const float* charge,
+                             const int* type, const float* lj_a,
+                             const float* lj_b, const float* cell,
+                             const float* rcell, float cutoff, float beta,
+                             const int* counts, const int* rows, int max_nb,
+                             double* frc, double* e_lj, double* e_c,
+                             double* virial)
+{
+    double b = beta, rc = cutoff;
+    for (int i = 0; i < n; i++)
+    {
+        for (int k = 0; k < counts[i]; k++)
+        {
+            int j = rows[(size_t)i * max_nb + k];
+            double d[3];
+            pbc_disp_d(crd + 3 * j, crd + 3 * i, cell, rcell, d);
+            double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            double r = sqrt(r2);
+            if (!(r < rc)) continue;
+            int p = lj_pair_index(type[i], type[j]);
+            double A = lj_a[p], B = lj_b[p];
+            double ri2 = 1.0 / r2, ri6 = ri2 * ri2 * ri2;
+            double qq = (double)charge[i] * charge[j];
+            double x = b * r;
+            double ex = exp(-x * x);
+            double fc = qq * ri2 / r * (2.0 / sqrt(ORC_PI_D) * x * ex + erfc(x));
+            double fabs_ = (B - A * ri6) * ri6 * ri2 - fc;
+            double f[3] = {fabs_ * d[0], fabs_ * d[1], fabs_ * d[2]};
+            for (int c = 0; c < 3; c++)
+            {
+                frc[3 * i + c] += f[c];
+                frc[3 * j + c] -= f[c];
+            }
+            if (e_lj) e_lj[i] += (A / 12.0 * ri6 - B / 6.0) * ri6;
+            if (e_c) e_c[i] += qq * erfc(x) / r;
+            if (virial)
+            {
+                double* v = virial + 6 * i;
+                v[0] -= f[0] * d[0];
+                v[1] -= f[0] * d[1] + f[1] * d[0];
+                v[2] -= f[1] * d[1];
+                v[3] -= f[0] * d[2] + f[2] * d[0];
+                v[4] -= f[1] * d[2] + f[2] * d[1];
+                v[5] -= f[2] * d[2];
+            }
+        }
+    }
+}
+
+/* FP32 twin of the pair kernel.  Accumulators (+=) are int64 at scale 2^32:
+ * frc[3n] (SoA x|y|z), e_lj[n], e_c[n], vir[6n] (SoA by component). */
+ORC_EXPORT void orc_pair_twin(int n, const float* crd, const float* charge,
+                              const int* type, const float* lj_a,
+                              const float* lj_b, const float* cell,
+                              const float* rcell, float cutoff, float beta,
+                              const int* counts, const int* rows, int max_nb,
+                              int64_t* frc, int64_t* e_lj, int64_t* e_c,
+                              int64_t* vir)
+{
+    float rc2 = cutoff * cutoff;
+    for (int i = 0; i < n; i++)
+    {
+        const float* ri = crd + 3 * i;
+        float qi = charge[i];
+        int ti = type[i];
+        for (int k = 0; k < counts[i]; k++)
+        {
+            int j = rows[(size_t)i * max_nb + k];
+            float d[3];
+            pbc_disp_f(crd + 3 * j, ri, cell, rcell, d);
+            float r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+            if (!(r2 < rc2)) continue;
+            int p = lj_pair_index(ti, type[j]);
+            float A = lj_a[p], B = lj_b[p];
+            float rinv = twin_rsqrtf(r2);
+            float r = r2 * rinv;
+            float rinv2 = rinv * rinv;
+            float rinv6 = (rinv2 * rinv2) * rinv2;
+            float flj = ((B - A * rinv6) * rinv6) * rinv2;
+            float x = beta * r;
+            float ex = twin_expf(-(x * x));
+            float erfcv = twin_erfc_core(x, ex);
+            float qq = qi * charge[j];
+            float fc = (qq * (rinv2 * rinv)) *
+                       ((x * ORC_TWO_OVER_SQRT_PI_F) * ex + erfcv);
+            float fabs_ = flj - fc;
+            float fx = fabs_ * d[0], fy = fabs_ * d[1], fz = fabs_ * d[2];
+            int64_t ix = to_fixed(fx), iy = to_fixed(fy), iz = to_fixed(fz);
+            frc[i] += ix;
+            frc[n + i] += iy;
+            frc[2 * n + i] += iz;
+            frc[j] -= ix;
+            frc[n + j] -= iy;
+            frc[2 * n + j] -= iz;
+            if (e_lj)
+            {
+                float elj = ((A * 0.083333333f) * rinv6 - B * 0.166666667f) * rinv6;
+                e_lj[i] += to_fixed(elj);
+            }
+            if (e_c) e_c[i] += to_fixed((qq * erfcv) * rinv);
+            if (vir)
+            {
+                vir[0 * (size_t)n + i] += to_fixed(-(fx * d[0]));
+                vir[1 * (size_t)n + i] += to_fixed(-(fx * d[1] + fy * d[0]));
+                vir[2 * (size_t)n + i] += to_fixed(-(fy * d[1]));
+                vir[3 * (size_t)n + i] += to_fixed(-(fx * d[2] + fz * d[0]));
+                vir[4 * (size_t)n + i] += to_fixed(-(fy * d[2] + fz * d[1]));
+                vir[5 * (size_t)n + i] += to_fixed(-(fz * d[2]));
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* PME exclusion correction (PM_force.cpp:1743-1814), factor 1 (one rank)   */
+/* ------------------------------------------------------------------------ */
+/* For j in excl(i): dr = pbc(r_j - r_i), x = beta r,
+ *   f = qq (erf(x) - 2x/sqrt(pi) e^{-x^2}) / r^3 * dr  (force on i, -f on j)
+ *   e[i] -= qq erf(x)/r;   virial[i] -= f (x) dr                            */
+ORC_EXPORT void orc_excl_f64(int n, const float* crd, const float* charge,
+                             const float* cell, const float* rcell, float beta,
+                             const int* excl_start, const int* excl_num,
+                             const int* excl_list, double* frc, double* e,
+                             double* virial)
+{
+    double b = beta;
+    for (int i = 0; i < n; i++)
+    {
+        for (int k = excl_start[i]; k < excl_start[i] + excl_num[i]; k++)
+        {
+            int j = excl_list[k];
+            double d[3];
+            pbc_disp_d(crd + 3 * j, crd + 3 * i, cell, rcell, d);
+            double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            double r = sqrt(r2);
+            double x = b * r;
+            double qq = (double)charge[i] * charge[j];
+            double h = erf(x) - 2.0 / sqrt(ORC_PI_D) * x * exp(-x * x);
+            double fa = qq * h / (r2 * r);
+            double f[3] = {fa * d[0], fa * d[1], fa * d[2]};
+            for (int c = 0; c < 3; c++)
+            {
+                frc[3 * i + c] += f[c];
+                frc[3 * j + c] -= f[c];
+            }
+            if (e) e[i] -= qq * erf(x) / r;
+            if (virial)
+            {
+                double* v = virial + 6 * i;
+                v[0] -= f[0] * d[0];
+                v[1] -= f[0] * d[1] + f[1] * d[0];
+                v[2] -= f[1] * d[1];
+                v[3] -= f[0] * d[2] + f[2] * d[0];
+                v[4] -= f[1] * d[2] + f[2] * d[1];
+                v[5] -= f[2] * d[2];
+            }
+        }
+    }
+}
+
+ORC_EXPORT void orc_excl_twin(int n, const float* crd, const float* charge,
+                              const float* cell, const float* rcell, float beta,
+                              const int* excl_start, const int* excl_num,
+                              const int* excl_list, int64_t* frc, int64_t* e,
+                              int64_t* vir)
+{
+    for (int i = 0; i < n; i++)
+    {
+        float qi = charge[i];
+        for (int k = excl_start[i]; k < excl_start[i] + excl_num[i]; k++)
+        {
+            int j = excl_list[k];
+            float d[3];
+            pbc_disp_f(crd + 3 * j, crd + 3 * i, cell, rcell, d);
+            float r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+            float rinv = twin_rsqrtf(r2);
+            float r = r2 * rinv;
+            float x = beta * r;
+            float ex = twin_expf(-(x * x));
+            float h = twin_erf_minus_gauss(x, ex);
+            float qq = qi * charge[j];
+            float fa = (qq * h) * ((rinv * rinv) * rinv);
+            float fx = fa * d[0], fy = fa * d[1], fz = fa * d[2];
+            int64_t ix = to_fixed(fx), iy = to_fixed(fy), iz = to_fixed(fz);
+            frc[i] += ix;
+            frc[n + i] += iy;
+            frc[2 * n + i] += iz;
+            frc[j] -= ix;
+            frc[n + j] -= iy;
+            frc[2 * n + j] -= iz;
+            if (e)
+            {
+                float erfv = h + (x * ORC_TWO_OVER_SQRT_PI_F) * ex;
+                e[i] += to_fixed(-((qq * erfv) * rinv));
+            }
+            if (vir)
+            {
+                vir[0 * (size_t)n + i] += to_fixed(-(fx * d[0]));
+                vir[1 * (size_t)n + i] += to_fixed(-(fx * d[1] + fy * d[0]));
+                vir[2 * (size_t)n + i] += to_fixed(-(fy * d[1]));
+                vir[3 * (size_t)n + i] += to_fixed(-(fx * d[2] + fz * d[0]));
+                vir[4 * (size_t)n + i] += to_fixed(-(fy * d[2] + fz * d[1]));
+                vir[5 * (size_t)n + i] += to_fixed(-(fz * d[2]));
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* FFT (double): recursive mixed-radix Cooley-Tukey, any length              */
+/* ------------------------------------------------------------------------ */
+typedef struct
+{
+    double re, im;
+} cplx;
+
+/* out[k] = sum_j in[j*stride] exp(sign 2 pi i j k / n); root[t] holds
+ * exp(sign 2 pi i t / n0) for the top-level length n0 (n divides n0). */
+static void fft_rec(const cplx* in, int stride, cplx* out, int n, int n0,
+                    const cplx* root, cplx* scratch)
+{
+    if (n == 1)
+    {
+        out[0] = in[0];
+        return;
+    }
+    int p = 2;
+    while (n % p) p++;
+    int m = n / p;
+    int step = n0 / n;
+    /* sub-transforms of the p decimated sequences, written to scratch */
+    for (int r = 0; r < p; r++)
+        fft_rec(in + r * stride, stride * p, scratch + r * m, m, n0, root,
+                out /* out used as scratch by the child */);
+    for (int k = 0; k < n; k++)
+    {
+        double sr = 0, si = 0;
+        for (int r = 0; r < p; r++)
+        {
+            cplx v = scratch[r * m + (k % m)];
+            cplx w = root[((size_t)r * k % n) * step];
+            sr += v.re * w.re - v.im * w.im;
+            si += v.re * w.im + v.im * w.re;
+        }
+        out[k].re = sr;
+        out[k].im = si;
+    }
+}
+
+static void fft1d(cplx* data, int n, int stride, const cplx* root, cplx* work)
+{
+    cplx* in = work;
+    cplx* out = work + n;
+    cplx* scratch = work + 2 * n;
+    for (int j = 0; j < n; j++) in[j] = data[(size_t)j * stride];
+    fft_rec(in, 1, out, n, n, root, scratch);
+    for (int j = 0; j < n; j++) data[(size_t)j * stride] = out[j];
+}
+
+static cplx* make_roots(int n, int sign)
+{
+    cplx* r = (cplx*)malloc(sizeof(cplx) * n);
+    for (int t = 0; t < n; t++)
+    {
+        double a = sign * 2.0 * ORC_PI_D * (double)t / n;
+        r[t].re = cos(a);
+        r[t].im = sin(a);
+    }
+    return r;
+}
+
+/* In-place complex 3-D DFT of a [n0][n1][n2] array (last index fastest),
+ * exponent sign `sign` (-1 forward as in FFTW), unnormalised. */
+ORC_EXPORT void orc_fft3d_f64(double* data_ri, int n0, int n1, int n2, int sign)
+{
+    cplx* a = (cplx*)data_ri;
+    int nmax = n0 > n1 ? n0 : n1;
+    nmax = nmax > n2 ? nmax : n2;
+    cplx* work = (cplx*)malloc(sizeof(cplx) * 3 * nmax);
+    cplx* r2 = make_roots(n2, sign);
+    for (int i = 0; i < n0; i++)
+        for (int j = 0; j < n1; j++) fft1d(a + ((size_t)i * n1 + j) * n2, n2, 1, r2, work);
+    free(r2);
+    cplx* r1 = make_roots(n1, sign);
+    for (int i = 0; i < n0; i++)
+        for (int k = 0; k < n2; k++) fft1d(a + (size_t)i * n1 * n2 + k, n1, n2, r1, work);
+    free(r1);
+    cplx* r0 = make_roots(n0, sign);
+    for (int j = 0; j < n1; j++)
+        for (int k = 0; k < n2; k++) fft1d(a + (size_t)j * n2 + k, n0, n1 * n2, r0, work);
+    free(r0);
+    free(work);
+}
+
+/* Naive O(N^2) DFT of the same array, to pin orc_fft3d_f64 on small grids. */
+ORC_EXPORT void orc_dft3d_naive_f64(const double* in_ri, double* out_ri, int n0,
+                                    int n1, int n2, int sign)
+{
+    const cplx* in = (const cplx*)in_ri;
+    cplx* out = (cplx*)out_ri;
+    for (int k0 = 0; k0 < n0; k0++)
+        for (int k1 = 0; k1 < n1; k1++)
+            for (int k2 = 0; k2 < n2; k2++)
+            {
+                double sr = 0, si = 0;
+                for (int x0 = 0; x0 < n0; x0++)
+                    for (int x1 = 0; x1 < n1; x1++)
+                        for (int x2 = 0; x2 < n2; x2++)
+                        {
+                            double a = sign * 2.0 * ORC_PI_D *
+                                       ((double)k0 * x0 / n0 + (double)k1 * x1 / n1 +
+                                        (double)k2 * x2 / n2);
+                            cplx v = in[((size_t)x0 * n1 + x1) * n2 + x2];
+                            sr += v.re * cos(a) - v.im * sin(a);
+                            si += v.re * sin(a) + v.im * cos(a);
+                        }
+                out[((size_t)k0 * n1 + k1) * n2 + k2].re = sr;
+                out[((size_t)k0 * n1 + k1) * n2 + k2].im = si;
+            }
+}
+
+/* ------------------------------------------------------------------------ */
+/* PME reciprocal (PM_force.cpp:697-916, :1021-1165)                         */
+/* ------------------------------------------------------------------------ */
+/* B-spline coefficient tables, PM_force.cpp:18-24 */
+static const double BS_MA[4] = {1.0 / 6.0, -0.5, 0.5, -1.0 / 6.0};
+static const double BS_MB[4] = {0, 0.5, -1, 0.5};
+static const double BS_MC[4] = {0, 0.5, 0, -0.5};
+static const double BS_MD[4] = {0, 1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0};
+static const double BS_DMA[4] = {0.5, -1.5, 1.5, -0.5};
+static const double BS_DMB[4] = {0, 1, -2, 1};
+static const double BS_DMC[4] = {0, 0.5, 0, -0.5};
+
+/* PM_force.cpp:706-737: base grid point u and fraction fr, in float exactly
+ * as the reference (fractional coordinate crd*rcell wrapped to [0,1)). */
+static void atom_near_f(const float* r, const float* rcell, int fx, int fy,
+                        int fz, int* u, float* fr)
+{
+    float f[3];
+    f[0] = (r[0] * rcell[0] + r[1] * rcell[1]) + r[2] * rcell[3];
+    f[1] = r[1] * rcell[2] + r[2] * rcell[4];
+    f[2] = r[2] * rcell[5];
+    for (int c = 0; c < 3; c++) f[c] = f[c] - floorf(f[c]);
+    if (!isfinite(f[0]) || !isfinite(f[1]) || !isfinite(f[2]))
+        f[0] = f[1] = f[2] = 0.0f;
+    int dims[3] = {fx, fy, fz};
+    for (int c = 0; c < 3; c++)
+    {
+        float s = f[c] * (float)dims[c];
+        int t = (int)s;
+        t = t < 0 ? 0 : (t < dims[c] ? t : dims[c] - 1);
+        float q = s - (float)t;
+        fr[c] = q - floorf(q);
+        u[c] = t;
+    }
+}
+
+static inline int wrapi(int a, int n)
+{
+    a %= n;
+    return a < 0 ? a + n : a;
+}
+
+/* FP64 reciprocal pipeline.  Outputs: frc += update_interval * F (double[3n]),
+ * *e_rec = 0.5 sum Q phi, vir[6] = the reference's PME_Sum_Virial result
+ * (-sum_half w |FQ|^2 VirialBC with w = 0.5 on kz = 0 and kz = fftz/2),
+ * and optionally the real-space grids Q and phi (double[Nall]). */
+ORC_EXPORT void orc_pme_f64(int n, const float* crd, const float* charge,
+                            const float* rcell, int fx, int fy, int fz,
+                            float beta, double volume, double* frc,
+                            double* e_rec, double* vir, double* q_out,
+                            double* phi_out)
+{
+    size_t nall = (size_t)fx * fy * fz;
+    cplx* grid = (cplx*)calloc(nall, sizeof(cplx));
+    int* u = (int*)malloc(sizeof(int) * 3 * n);
+    double* frd = (double*)malloc(sizeof(double) * 3 * n);
+    for (int i = 0; i < n; i++)
+    {
+        float frf[3];
+        atom_near_f(crd + 3 * i, rcell, fx, fy, fz, u + 3 * i, frf);
+        for (int c = 0; c < 3; c++) frd[3 * i + c] = frf[c];
+        double w[3][4];
+        for (int c = 0; c < 3; c++)
+        {
+            double t = frd[3 * i + c];
+            for (int k = 0; k < 4; k++)
+                w[c][k] = ((BS_MA[k] * t + BS_MB[k]) * t + BS_MC[k]) * t + BS_MD[k];
+        }
+        for (int kx = 0; kx < 4; kx++)
+            for (int ky = 0; ky < 4; ky++)
+                for (int kz = 0; kz < 4; kz++)
+                {
+                    size_t idx = ((size_t)wrapi(u[3 * i] - kx, fx) * fy +
+                                  wrapi(u[3 * i + 1] - ky, fy)) * fz +
+                                 wrapi(u[3 * i + 2] - kz, fz);
+                    grid[idx].re += charge[i] * w[0][kx] * w[1][ky] * w[2][kz];
+                }
+    }
+    double* q = (double*)malloc(sizeof(double) * nall);
+    for (size_t p = 0; p < nall; p++) q[p] = grid[p].re;
+    orc_fft3d_f64((double*)grid, fx, fy, fz, -1);
+    /* convolution over the full spectrum with BC from the half spectrum */
+    int fc = fz / 2 + 1;
+    double* bc = (double*)malloc(sizeof(double) * (size_t)fx * fy * fc);
+    orc_bc_f64(fx, fy, fz, rcell, beta, volume, bc);
+    double mpre = ORC_PI_D * ORC_PI_D / ((double)beta * beta);
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int kx = 0; kx < fx; kx++)
+        for (int ky = 0; ky < fy; ky++)
+            for (int kz = 0; kz < fz; kz++)
+            {
+                size_t idx = ((size_t)kx * fy + ky) * fz + kz;
+                int hz = kz, hx = kx, hy = ky;
+                if (kz >= fc)
+                { /* conjugate partner in the half spectrum */
+                    hz = fz - kz;
+                    hx = (fx - kx) % fx;
+                    hy = (fy - ky) % fy;
+                }
+                double b = bc[((size_t)hx * fy + hy) * fc + hz];
+                if (kz < fc && b != 0.0)
+                {
+                    /* PM_force.cpp:1021-1065 virial over the half spectrum */
+                    double w = (kz == 0 || kz == fc - 1) ? 0.5 : 1.0;
+                    double a2 = grid[idx].re * grid[idx].re + grid[idx].im * grid[idx].im;
+                    double kxr = kx > fx / 2 ? kx - fx : kx;
+                    double kyr = ky > fy / 2 ? ky - fy : ky;
+                    double mx = kxr * rcell[0];
+                    double my = kxr * rcell[1] + kyr * rcell[2];
+                    double mz = kxr * rcell[3] + kyr * rcell[4] + kz * rcell[5];
+                    double msq = mx * mx + my * my + mz * mz;
+                    double cf = 2.0 / msq * (1.0 + mpre * msq);
+                    double hb = 0.5 * b * w * a2;
+                    v[0] -= hb * (1.0 - cf * mx * mx);
+                    v[1] -= hb * (0.0 - cf * my * mx);
+                    v[2] -= hb * (1.0 - cf * my * my);
+                    v[3] -= hb * (0.0 - cf * mz * mx);
+                    v[4] -= hb * (0.0 - cf * mz * my);
+                    v[5] -= hb * (1.0 - cf * mz * mz);
+                }
+                grid[idx].re *= b;
+                grid[idx].im *= b;
+            }
+    orc_fft3d_f64((double*)grid, fx, fy, fz, +1);
+    double e = 0;
+    for (size_t p = 0; p < nall; p++) e += q[p] * grid[p].re;
+    *e_rec = 0.5 * e;
+    if (vir)
+        for (int c = 0; c < 6; c++) vir[c] = v[c];
+    if (q_out) memcpy(q_out, q, sizeof(double) * nall);
+    if (phi_out)
+        for (size_t p = 0; p < nall; p++) phi_out[p] = grid[p].re;
+    /* gather (PM_force.cpp:826-890) */
+    for (int i = 0; i < n; i++)
+    {
+        double w[3][4], dw[3][4];
+        for (int c = 0; c < 3; c++)
+        {
+            double t = frd[3 * i + c];
+            for (int k = 0; k < 4; k++)
+            {
+                w[c][k] = ((BS_MA[k] * t + BS_MB[k]) * t + BS_MC[k]) * t + BS_MD[k];
+                dw[c][k] = (BS_DMA[k] * t + BS_DMB[k]) * t + BS_DMC[k];
+            }
+        }
+        double s[3] = {0, 0, 0};
+        for (int kx = 0; kx < 4; kx++)
+            for (int ky = 0; ky < 4; ky++)
+                for (int kz = 0; kz < 4; kz++)
+                {
+                    size_t idx = ((size_t)wrapi(u[3 * i] - kx, fx) * fy +
+                                  wrapi(u[3 * i + 1] - ky, fy)) * fz +
+                                 wrapi(u[3 * i + 2] - kz, fz);
+                    double ph = grid[idx].re;
+                    s[0] += ph * dw[0][kx] * w[1][ky] * w[2][kz];
+                    s[1] += ph * w[0][kx] * dw[1][ky] * w[2][kz];
+                    s[2] += ph * w[0][kx] * w[1][ky] * dw[2][kz];
+                }
+        double t0 = s[0] * fx, t1 = s[1] * fy, t2 = s[2] * fz;
+        double q_ = -(double)charge[i];
+        frc[3 * i + 0] += q_ * (t0 * rcell[0]);
+        frc[3 * i + 1] += q_ * (t0 * rcell[1] + t1 * rcell[2]);
+        frc[3 * i + 2] += q_ * (t0 * rcell[3] + t1 * rcell[4] + t2 * rcell[5]);
+    }
+    free(bc);
+    free(q);
+    free(grid);
+    free(u);
+    free(frd);
+}
+
+/* FP32 twin of the spread: the Q grid as int64 fixed point (sum of
+ * round(((q*wx)*wy)*wz * 2^32) over atoms), weights evaluated in float with
+ * the reference's op order (PM_force.cpp:776-805). */
+static inline float bs_w_f(int k, float t)
+{
+    static const float MA[4] = {1.0f / 6.0f, -0.5f, 0.5f, -1.0f / 6.0f};
+    static const float MB[4] = {0.0f, 0.5f, -1.0f, 0.5f};
+    static const float MC[4] = {0.0f, 0.5f, 0.0f, -0.5f};
+    static const float MD[4] = {0.0f, 1.0f / 6.0f, 4.0f / 6.0f, 1.0f / 6.0f};
+    float t2 = t * t;
+    return ((MA[k] * t) * t2 + MB[k] * t2) + MC[k] * t + MD[k];
+}
+static inline float bs_dw_f(int k, float t)
+{
+    static const float DMA[4] = {0.5f, -1.5f, 1.5f, -0.5f};
+    static const float DMB[4] = {0.0f, 1.0f, -2.0f, 1.0f};
+    static const float DMC[4] = {0.0f, 0.5f, 0.0f, -0.5f};
+    float t2 = t * t;
+    return (DMA[k] * t2 + DMB[k] * t) + DMC[k];
+}
+
+ORC_EXPORT void orc_pme_spread_twin(int n, const float* crd, const float* charge,
+                                    const float* rcell, int fx, int fy, int fz,
+                                    int64_t* qgrid)
+{
+    for (int i = 0; i < n; i++)
+    {
+        int u[3];
+        float fr[3];
+        atom_near_f(crd + 3 * i, rcell, fx, fy, fz, u, fr);
+        float w[3][4];
+        for (int c = 0; c < 3; c++)
+            for (int k = 0; k < 4; k++) w[c][k] = bs_w_f(k, fr[c]);
+        for (int kx = 0; kx < 4; kx++)
+            for (int ky = 0; ky < 4; ky++)
+                for (int kz = 0; kz < 4; kz++)
+                {
+                    size_t idx = ((size_t)wrapi(u[0] - kx, fx) * fy +
+                                  wrapi(u[1] - ky, fy)) * fz + wrapi(u[2] - kz, fz);
+                    float v = ((charge[i] * w[0][kx]) * w[1][ky]) * w[2][kz];
+                    qgrid[idx] += to_fixed(v);
+                }
+    }
+}
+
+/* FP32 twin of the gather: given the float potential grid phi (the C2R
+ * output), per atom sum in stencil order kx, ky, kz (outer to inner)
+ *   s_x = fma(phi, (dwx*wy)*wz, s_x) etc., then t = s * fft dims,
+ *   F = -q * MultiplyTranspose(t, rcell) * update_interval, to fixed. */
+ORC_EXPORT void orc_pme_gather_twin(int n, const float* crd, const float* charge,
+                                    const float* rcell, int fx, int fy, int fz,
+                                    const float* phi, float update_interval,
+                                    int64_t* frc)
+{
+    for (int i = 0; i < n; i++)
+    {
+        int u[3];
+        float fr[3];
+        atom_near_f(crd + 3 * i, rcell, fx, fy, fz, u, fr);
+        float w[3][4], dw[3][4];
+        for (int c = 0; c < 3; c++)
+            for (int k = 0; k < 4; k++)
+            {
+                w[c][k] = bs_w_f(k, fr[c]);
+                dw[c][k] = bs_dw_f(k, fr[c]);
+            }
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        for (int kx = 0; kx < 4; kx++)
+            for (int ky = 0; ky < 4; ky++)
+                for (int kz = 0; kz < 4; kz++)
+                {
+                    size_t idx = ((size_t)wrapi(u[0] - kx, fx) * fy +
+                                  wrapi(u[1] - ky, fy)) * fz + wrapi(u[2] - kz, fz);
+                    float ph = phi[idx];
+                    sx = fmaf(ph, (dw[0][kx] * w[1][ky]) * w[2][kz], sx);
+                    sy = fmaf(ph, (w[0][kx] * dw[1][ky]) * w[2][kz], sy);
+                    sz = fmaf(ph, (w[0][kx] * w[1][ky]) * dw[2][kz], sz);
+                }
+        float tx = sx * (float)fx, ty = sy * (float)fy, tz = sz * (float)fz;
+        float mq = -charge[i] * update_interval;
+        float gx = tx * rcell[0];
+        float gy = tx * rcell[1] + ty * rcell[2];
+        float gz = (tx * rcell[3] + ty * rcell[4]) + tz * rcell[5];
+        frc[i] += to_fixed(mq * gx);
+        frc[n + i] += to_fixed(mq * gy);
+        frc[2 * n + i] += to_fixed(mq * gz);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* exact Ewald sum (validation of the PME layer on small boxes)              */
+/* ------------------------------------------------------------------------ */
+/* Orthorhombic box L[3]; E = E_dir + E_rec + E_self with
+ * E_dir = sum_{i<j, images n, |r|<rcut} qq erfc(beta r)/r (no exclusions),
+ * E_rec = 1/(2 pi V) sum_{m != 0, |m|<=kmax} exp(-pi^2 m^2/beta^2)/m^2 |S(m)|^2,
+ * E_self = -beta/sqrt(pi) sum q^2.  Forces likewise.  All double. */
+ORC_EXPORT void orc_ewald_f64(int n, const double* crd, const double* q,
+                              const double* L, double beta, double rcut,
+                              int kmax, double* frc, double* e_dir,
+                              double* e_rec, double* e_self)
+{
+    double V = L[0] * L[1] * L[2];
+    double ed = 0, er = 0, es = 0;
+    int nim[3];
+    for (int c = 0; c < 3; c++) nim[c] = (int)ceil(rcut / L[c]) + 1;
+    for (int i = 0; i < n; i++)
+    {
+        es -= beta / sqrt(ORC_PI_D) * q[i] * q[i];
+        for (int j = 0; j < n; j++)
+            for (int a = -nim[0]; a <= nim[0]; a++)
+                for (int b = -nim[1]; b <= nim[1]; b++)
+                    for (int c = -nim[2]; c <= nim[2]; c++)
+                    {
+                        if (i == j && a == 0 && b == 0 && c == 0) continue;
+                        double d[3] = {crd[3 * j] - crd[3 * i] + a * L[0],
+                                       crd[3 * j + 1] - crd[3 * i + 1] + b * L[1],
+                                       crd[3 * j + 2] - crd[3 * i + 2] + c * L[2]};
+                        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                        double r = sqrt(r2);
+                        if (r >= rcut) continue;
+                        double x = beta * r;
+                        ed += 0.5 * q[i] * q[j] * erfc(x) / r;
+                        double fa = -q[i] * q[j] *
+                                    (2.0 / sqrt(ORC_PI_D) * x * exp(-x * x) + erfc(x)) /
+                                    (r2 * r);
+                        for (int t = 0; t < 3; t++) frc[3 * i + t] += fa * d[t];
+                    }
+    }
+    for (int a = -kmax; a <= kmax; a++)
+        for (int b = -kmax; b <= kmax; b++)
+            for (int c = -kmax; c <= kmax; c++)
+            {
+                if (a == 0 && b == 0 && c == 0) continue;
+                double m[3] = {a / L[0], b / L[1], c / L[2]};
+                double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+                double pref = exp(-ORC_PI_D * ORC_PI_D * m2 / (beta * beta)) /
+                              (2.0 * ORC_PI_D * V * m2);
+                double sr = 0, si = 0;
+                for (int i = 0; i < n; i++)
+                {
+                    double ph = 2.0 * ORC_PI_D *
+                                (m[0] * crd[3 * i] + m[1] * crd[3 * i + 1] +
+                                 m[2] * crd[3 * i + 2]);
+                    sr += q[i] * cos(ph);
+                    si += q[i] * sin(ph);
+                }
+                er += pref * (sr * sr + si * si);
+                for (int i = 0; i < n; i++)
+                {
+                    double ph = 2.0 * ORC_PI_D *
+                                (m[0] * crd[3 * i] + m[1] * crd[3 * i + 1] +
+                                 m[2] * crd[3 * i + 2]);
+                    /* F_i = -dE/dr_i = pref * 2 * q_i * 2 pi m (sin ph Sr - cos ph Si) */
+                    double g = pref * 2.0 * q[i] * 2.0 * ORC_PI_D *
+                               (sin(ph) * sr - cos(ph) * si);
+                    for (int t = 0; t < 3; t++) frc[3 * i + t] += g * m[t];
+                }
+            }
+    *e_dir = ed;
+    *e_rec = er;
+    *e_self = es;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LJ long-range correction (Lennard_Jones_force.cpp:146-184, :246-272)     */
+/* ------------------------------------------------------------------------ */
+/* sum over all ordered pairs (i, j), i == j included, of B_ij, via a type
+ * histogram; returns the double total (the reference rounds it to float). */
+ORC_EXPORT double orc_lj_c6_total(int n, const int* type, int ntypes,
+                                  const float* lj_b)
+{
+    double* hist = (double*)calloc(ntypes, sizeof(double));
+    for (int i = 0; i < n; i++) hist[type[i]] += 1.0;
+    double s = 0;
+    for (int a = 0; a < ntypes; a++)
+        for (int b = 0; b < ntypes; b++)
+            s += hist[a] * hist[b] * lj_b[lj_pair_index(a, b)];
+    free(hist);
+    return s;
+}
+
+/* exposes the twin's scalar math for unit tests */
+ORC_EXPORT float orc_twin_expf(float y) { return twin_expf(y); }
+ORC_EXPORT float orc_twin_erfc(float x) { return twin_erfc_core(x, twin_expf(-(x * x))); }
+ORC_EXPORT float orc_twin_erf_minus_gauss(float x)
+{
+    return twin_erf_minus_gauss(x, twin_expf(-(x * x)));
+}
+ORC_EXPORT float orc_twin_rsqrtf(float x) { return twin_rsqrtf(x); }
+ORC_EXPORT int64_t orc_to_fixed(float f) { return to_fixed(f); }

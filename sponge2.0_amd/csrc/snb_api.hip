@@ -1,0 +1,805 @@
+// snb_api.hip -- the extern "C" entry points of libsponge_nb (include/sponge_nb.h).
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "snb_internal.h"
+
+namespace
+{
+template <class T>
+int dalloc(T** p, size_t count)
+{
+    if (count == 0) count = 1;
+    if (hipMalloc((void**)p, sizeof(T) * count) != hipSuccess) return SNB_ERR_OUT_OF_MEMORY;
+    return SNB_OK;
+}
+
+template <class T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+struct DeviceGuard
+{
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int cur;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// PM_force.cpp:244-272: doubling, then 50 bisection steps, in float with the
+// host libm erfc.
+float get_beta(float cutoff, float tolerance)
+{
+    float beta = 0.0f, low, high = 1.0f, tempf;
+    int ihigh = 1;
+    while (1)
+    {
+        tempf = erfcf(high * cutoff) / cutoff;
+        if (tempf <= tolerance) break;
+        high *= 2;
+        ihigh++;
+        if (ihigh > 64) break;
+    }
+    ihigh += 50;
+    low = 0.0f;
+    for (int ilow = 1; ilow < ihigh; ilow++)
+    {
+        beta = (low + high) / 2;
+        tempf = erfcf(beta * cutoff) / cutoff;
+        if (tempf >= tolerance)
+            low = beta;
+        else
+            high = beta;
+    }
+    return beta;
+}
+
+// common.cpp:521-557, :639-659
+bool smooth_2357(int v)
+{
+    if (v <= 0) return false;
+    const int p[4] = {2, 3, 5, 7};
+    for (int k = 0; k < 4; k++)
+        while (v % p[k] == 0) v /= p[k];
+    return v == 1;
+}
+
+int fft_size(float length)
+{
+    int t = (int)ceil(length + 3) >> 2 << 2;
+    if (t >= 60 && t <= 68)
+        t = 64;
+    else if (t >= 120 && t <= 136)
+        t = 128;
+    else if (t >= 240 && t <= 272)
+        t = 256;
+    else if (t >= 480 && t <= 544)
+        t = 512;
+    else if (t >= 960 && t <= 1088)
+        t = 1024;
+    while (!smooth_2357(t)) t += 4;
+    return t;
+}
+
+bool orthorhombic(snb_ltmatrix3 c)
+{
+    return c.a21 == 0.0f && c.a31 == 0.0f && c.a32 == 0.0f;
+}
+
+void free_nl(SnbNl& L)
+{
+    dfree(L.cell_count);
+    dfree(L.cell_start);
+    dfree(L.atom_cell);
+    dfree(L.atom_slot);
+    dfree(L.sorted_id);
+    dfree(L.sorted_crd);
+    dfree(L.serial);
+    dfree(L.dnl);
+    dfree(L.need);
+    dfree(L.old_crd);
+    L.init = false;
+}
+
+void free_pme(SnbPme& P)
+{
+    dfree(P.tw_x);
+    dfree(P.tw_y);
+    dfree(P.tw_z);
+    dfree(P.bc);
+    dfree(P.q);
+    dfree(P.phi);
+    dfree(P.spec);
+    dfree(P.atom_u);
+    dfree(P.atom_fr);
+    dfree(P.brick_count);
+    dfree(P.brick_start);
+    dfree(P.atom_slot);
+    dfree(P.brick_atoms);
+    dfree(P.partials);
+    P.init = false;
+}
+
+const int* ex_or(const int* given, const int* own) { return given ? given : own; }
+
+// resolves the recorded event pairs of one group into done_ms / done_n
+int prof_collect(snb_context* c, int g)
+{
+    SnbProf& P = c->prof;
+    for (int k = 0; k < P.count[g]; k++)
+    {
+        hipEvent_t* e = P.ev + ((size_t)g * SNB_PROF_CAP + k) * 2;
+        float ms = 0.0f;
+        SNB_CHECK(hipEventElapsedTime(&ms, e[0], e[1]));
+        P.done_ms[g] += ms;
+        P.done_n[g] += 1;
+    }
+    P.count[g] = 0;
+    return SNB_OK;
+}
+}  // namespace
+
+void snb_prof_begin(snb_context* c, int g)
+{
+    SnbProf& P = c->prof;
+    if (!P.on) return;
+    if (P.count[g] >= SNB_PROF_CAP)
+    {
+        (void)hipStreamSynchronize(c->stream);
+        (void)prof_collect(c, g);
+    }
+    (void)hipEventRecord(P.ev[((size_t)g * SNB_PROF_CAP + P.count[g]) * 2], c->stream);
+}
+
+void snb_prof_end(snb_context* c, int g)
+{
+    SnbProf& P = c->prof;
+    if (!P.on) return;
+    (void)hipEventRecord(P.ev[((size_t)g * SNB_PROF_CAP + P.count[g]) * 2 + 1], c->stream);
+    P.count[g] += 1;
+}
+
+extern "C" {
+
+int snb_abi_version(void) { return 1; }
+
+const char* snb_status_string(int code)
+{
+    switch (code)
+    {
+        case SNB_OK: return "ok";
+        case SNB_ERR_ARGUMENT: return "invalid argument";
+        case SNB_ERR_HIP: return "HIP runtime error";
+        case SNB_ERR_BOX_TOO_SMALL: return "the box is too small for the neighbor-list cell grid";
+        case SNB_ERR_UNSUPPORTED: return "unsupported configuration";
+        case SNB_ERR_NOT_INITIALIZED: return "module not initialised";
+        case SNB_ERR_OUT_OF_MEMORY: return "device allocation failed";
+        case SNB_ERR_DEVICE_STATUS: return "a device status flag is set";
+        default: return "unknown status";
+    }
+}
+
+int snb_context_create(snb_context** out, int device, void* hip_stream, int atom_numbers)
+{
+    if (!out || atom_numbers < 0) return SNB_ERR_ARGUMENT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return SNB_ERR_HIP;
+    DeviceGuard g(device);
+    snb_context* c = new snb_context();
+    c->device = device;
+    c->stream = (hipStream_t)hip_stream;
+    c->n = atom_numbers;
+    int n = atom_numbers;
+    int rc = SNB_OK;
+    if (!rc) rc = dalloc(&c->atom4, n);
+    if (!rc) rc = dalloc(&c->type, n);
+    if (!rc) rc = dalloc(&c->status, 1);
+    if (!rc) rc = dalloc(&c->fx, 3 * (size_t)n);
+    if (!rc) rc = dalloc(&c->e_atom, 3 * (size_t)n);
+    if (!rc) rc = dalloc(&c->vir_atom, 6 * (size_t)n);
+    if (!rc) rc = dalloc(&c->scal, SNB_S_COUNT);
+    if (!rc) rc = dalloc(&c->dscal, SNB_D_COUNT);
+    if (rc)
+    {
+        snb_context_destroy(c);
+        return rc;
+    }
+    hipStream_t s = c->stream;
+    snb_launch_fill_int(s, c->status, 1, 0);
+    snb_launch_fill_int(s, c->type, n, 0);
+    snb_launch_fill_i64(s, c->fx, 3 * (size_t)n);
+    snb_launch_fill_i64(s, c->e_atom, 3 * (size_t)n);
+    snb_launch_fill_i64(s, c->vir_atom, 6 * (size_t)n);
+    snb_launch_fill_i64(s, c->scal, SNB_S_COUNT);
+    snb_launch_fill_i64(s, (int64_t*)c->dscal, SNB_D_COUNT);
+    if (hipStreamSynchronize(s) != hipSuccess)
+    {
+        snb_context_destroy(c);
+        return SNB_ERR_HIP;
+    }
+    *out = c;
+    return SNB_OK;
+}
+
+int snb_context_destroy(snb_context* c)
+{
+    if (!c) return SNB_OK;
+    DeviceGuard g(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    dfree(c->atom4);
+    dfree(c->type);
+    dfree(c->status);
+    dfree(c->fx);
+    dfree(c->e_atom);
+    dfree(c->vir_atom);
+    dfree(c->scal);
+    dfree(c->dscal);
+    dfree(c->ex_start);
+    dfree(c->ex_list);
+    dfree(c->ex_num);
+    free_nl(c->nl);
+    free_pme(c->pme);
+    if (c->prof.ev)
+    {
+        for (size_t k = 0; k < (size_t)SNB_PROF_GROUPS * SNB_PROF_CAP * 2; k++)
+            (void)hipEventDestroy(c->prof.ev[k]);
+        free(c->prof.ev);
+    }
+    dfree(c->lj.a);
+    dfree(c->lj.b);
+    delete c;
+    return SNB_OK;
+}
+
+int snb_set_stream(snb_context* c, void* hip_stream)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    c->stream = (hipStream_t)hip_stream;
+    return SNB_OK;
+}
+
+int snb_synchronize(snb_context* c)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SNB_CHECK(hipStreamSynchronize(c->stream));
+    return SNB_OK;
+}
+
+int snb_read_status(snb_context* c, int clear, int* flags)
+{
+    if (!c || !flags) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SNB_CHECK(hipStreamSynchronize(c->stream));
+    SNB_CHECK(hipMemcpy(flags, c->status, sizeof(int), hipMemcpyDeviceToHost));
+    if (clear)
+    {
+        snb_launch_fill_int(c->stream, c->status, 1, 0);
+        SNB_CHECK(hipStreamSynchronize(c->stream));
+    }
+    return SNB_OK;
+}
+
+int snb_set_exclusions(snb_context* c, const int* h_start, const int* h_list, const int* h_num,
+                       int total)
+{
+    if (!c || total < 0 || (c->n > 0 && (!h_start || !h_num)) || (total > 0 && !h_list))
+        return SNB_ERR_ARGUMENT;
+    // validate: every listed partner is an atom; per-atom ranges inside the list
+    for (int i = 0; i < c->n; i++)
+    {
+        if (h_num[i] < 0 || h_start[i] < 0 || h_start[i] + h_num[i] > total)
+            return SNB_ERR_ARGUMENT;
+        for (int k = h_start[i]; k < h_start[i] + h_num[i]; k++)
+            if (h_list[k] < 0 || h_list[k] >= c->n) return SNB_ERR_ARGUMENT;
+    }
+    DeviceGuard g(c->device);
+    dfree(c->ex_start);
+    dfree(c->ex_list);
+    dfree(c->ex_num);
+    int rc = dalloc(&c->ex_start, c->n);
+    if (!rc) rc = dalloc(&c->ex_list, total);
+    if (!rc) rc = dalloc(&c->ex_num, c->n);
+    if (rc) return rc;
+    if (c->n > 0)
+    {
+        SNB_CHECK(hipMemcpy(c->ex_start, h_start, sizeof(int) * c->n, hipMemcpyHostToDevice));
+        SNB_CHECK(hipMemcpy(c->ex_num, h_num, sizeof(int) * c->n, hipMemcpyHostToDevice));
+    }
+    if (total > 0)
+        SNB_CHECK(hipMemcpy(c->ex_list, h_list, sizeof(int) * total, hipMemcpyHostToDevice));
+    c->ex_total = total;
+    c->has_excl = true;
+    return SNB_OK;
+}
+
+// ----------------------------------------------------------------- neighbor list
+int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
+                   snb_ltmatrix3 rcell, int max_neighbor_numbers, int max_atom_in_grid_numbers,
+                   int refresh_interval, float skin_permit)
+{
+    (void)rcell;
+    if (!c || !(cutoff > 0.0f) || !(skin >= 0.0f) || max_neighbor_numbers <= 0 ||
+        max_atom_in_grid_numbers <= 0)
+        return SNB_ERR_ARGUMENT;
+    if (!orthorhombic(cell)) return SNB_ERR_UNSUPPORTED;
+    DeviceGuard g(c->device);
+    SnbNl& L = c->nl;
+    free_nl(L);
+    L.cutoff = cutoff;
+    L.skin = skin;
+    // neighbor_list.cpp:801-803, :705 -- the same float expressions
+    L.grid_length = (cutoff + skin) * 0.5f;
+    L.cs2 = L.grid_length * L.grid_length * 4.0f;
+    L.max_nb = max_neighbor_numbers;
+    L.max_in_cell = max_atom_in_grid_numbers;
+    L.refresh_interval = refresh_interval;
+    L.skin_permit = skin_permit;
+    // GRIDS::Initial (neighbor_list.cpp:58-60)
+    L.nx = (int)floorf(cell.a11 / L.grid_length);
+    L.ny = (int)floorf(cell.a22 / L.grid_length);
+    L.nz = (int)floorf(cell.a33 / L.grid_length);
+    if (L.nx <= 2 || L.ny <= 2 || L.nz <= 2) return SNB_ERR_BOX_TOO_SMALL;
+    L.ncell = L.nx * L.ny * L.nz;
+    int n = c->n;
+    int rc = SNB_OK;
+    if (!rc) rc = dalloc(&L.cell_count, L.ncell + 1);
+    if (!rc) rc = dalloc(&L.cell_start, L.ncell + 1);
+    if (!rc) rc = dalloc(&L.atom_cell, n);
+    if (!rc) rc = dalloc(&L.atom_slot, n);
+    if (!rc) rc = dalloc(&L.sorted_id, n);
+    if (!rc) rc = dalloc(&L.sorted_crd, n);
+    if (!rc) rc = dalloc(&L.serial, (size_t)n * L.max_nb);
+    if (!rc) rc = dalloc(&L.dnl, n);
+    if (!rc) rc = dalloc(&L.need, 1);
+    if (!rc) rc = dalloc(&L.old_crd, n);
+    if (rc)
+    {
+        free_nl(L);
+        return rc;
+    }
+    // ATOM_GROUP rows point into the padded table (neighbor_list.cpp:764-771)
+    std::vector<snb_atom_group> h(n > 0 ? n : 1);
+    for (int i = 0; i < n; i++)
+    {
+        h[i].atom_numbers = 0;
+        h[i].ghost_numbers = 0;
+        h[i].atom_serial = L.serial + (size_t)i * L.max_nb;
+    }
+    if (n > 0) SNB_CHECK(hipMemcpy(L.dnl, h.data(), sizeof(snb_atom_group) * n, hipMemcpyHostToDevice));
+    snb_launch_fill_int(c->stream, L.need, 1, 1);
+    SNB_CHECK(hipStreamSynchronize(c->stream));
+    L.init = true;
+    return SNB_OK;
+}
+
+int snb_nl_update(snb_context* c, const int* d_atom_local, int local_atom_numbers,
+                  int ghost_numbers, const snb_vector* d_crd, snb_ltmatrix3 cell,
+                  snb_ltmatrix3 rcell, int step, int update, const int* d_ex_start,
+                  const int* d_ex_list, const int* d_ex_num)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->nl.init) return SNB_ERR_NOT_INITIALIZED;
+    if (ghost_numbers != 0 || d_atom_local) return SNB_ERR_UNSUPPORTED;
+    if (local_atom_numbers != c->n || (c->n > 0 && !d_crd)) return SNB_ERR_ARGUMENT;
+    if (!orthorhombic(cell)) return SNB_ERR_UNSUPPORTED;
+    if (c->n == 0) return SNB_OK;
+    DeviceGuard g(c->device);
+    SnbBox box = snb_box(cell, rcell);
+    const int* es = ex_or(d_ex_start, c->ex_start);
+    const int* el = ex_or(d_ex_list, c->ex_list);
+    const int* en = ex_or(d_ex_num, c->ex_num);
+    if (!d_ex_num && !c->has_excl) es = el = en = nullptr;
+    return snb_launch_nl_update(c, d_crd, box, update ? 1 : 0, step, es, el, en);
+}
+
+int snb_nl_device_list(snb_context* c, snb_atom_group** d_nl, int** d_serial, int* max_nb)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->nl.init) return SNB_ERR_NOT_INITIALIZED;
+    if (d_nl) *d_nl = c->nl.dnl;
+    if (d_serial) *d_serial = c->nl.serial;
+    if (max_nb) *max_nb = c->nl.max_nb;
+    return SNB_OK;
+}
+
+__global__ void k_nl_counts(int n, const snb_atom_group* __restrict__ nl, int* __restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = nl[i].atom_numbers;
+}
+
+int snb_nl_copy_rows(snb_context* c, int* d_counts, int* d_rows)
+{
+    if (!c || !d_counts || !d_rows) return SNB_ERR_ARGUMENT;
+    if (!c->nl.init) return SNB_ERR_NOT_INITIALIZED;
+    if (c->n == 0) return SNB_OK;
+    DeviceGuard g(c->device);
+    hipLaunchKernelGGL(k_nl_counts, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n,
+                       c->nl.dnl, d_counts);
+    SNB_CHECK(hipGetLastError());
+    SNB_CHECK(hipMemcpyAsync(d_rows, c->nl.serial, sizeof(int) * (size_t)c->n * c->nl.max_nb,
+                             hipMemcpyDeviceToDevice, c->stream));
+    return SNB_OK;
+}
+
+int snb_nl_grid_shape(snb_context* c, int* nx, int* ny, int* nz)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->nl.init) return SNB_ERR_NOT_INITIALIZED;
+    if (nx) *nx = c->nl.nx;
+    if (ny) *ny = c->nl.ny;
+    if (nz) *nz = c->nl.nz;
+    return SNB_OK;
+}
+
+// ----------------------------------------------------------------------- LJ
+int snb_lj_initial(snb_context* c, int ntypes, const float* h_a, const float* h_b,
+                   const int* h_type, float cutoff)
+{
+    if (!c || ntypes <= 0 || !h_a || !h_b || (c->n > 0 && !h_type) || !(cutoff > 0.0f))
+        return SNB_ERR_ARGUMENT;
+    for (int i = 0; i < c->n; i++)
+        if (h_type[i] < 0 || h_type[i] >= ntypes) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SnbLj& J = c->lj;
+    dfree(J.a);
+    dfree(J.b);
+    size_t npair = (size_t)ntypes * (ntypes + 1) / 2;
+    int rc = dalloc(&J.a, npair);
+    if (!rc) rc = dalloc(&J.b, npair);
+    if (rc) return rc;
+    SNB_CHECK(hipMemcpy(J.a, h_a, sizeof(float) * npair, hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemcpy(J.b, h_b, sizeof(float) * npair, hipMemcpyHostToDevice));
+    if (c->n > 0) SNB_CHECK(hipMemcpy(c->type, h_type, sizeof(int) * c->n, hipMemcpyHostToDevice));
+    J.ntypes = ntypes;
+    J.cutoff = cutoff;
+    // Total_C6_Get (Lennard_Jones_force.cpp:146-184) = sum over all ordered
+    // atom pairs, i == j included, of B_ij; from the type histogram.
+    std::vector<double> hist(ntypes, 0.0);
+    for (int i = 0; i < c->n; i++) hist[h_type[i]] += 1.0;
+    double tot = 0.0;
+    for (int a = 0; a < ntypes; a++)
+    {
+        if (hist[a] == 0.0) continue;
+        for (int b = 0; b < ntypes; b++)
+        {
+            if (hist[b] == 0.0) continue;
+            int hi = a > b ? a : b, lo = a > b ? b : a;
+            tot += hist[a] * hist[b] * (double)h_b[(size_t)hi * (hi + 1) / 2 + lo];
+        }
+    }
+    J.c6_total = tot;
+    // Lennard_Jones_force.cpp:267-268 in float
+    float f = (float)tot;
+    f *= -2.0f / 3.0f * 3.1415926535897932f / cutoff / cutoff / cutoff / 6.0f;
+    J.long_range_factor = f;
+    J.init = true;
+    return SNB_OK;
+}
+
+int snb_lj_pme_direct_force(snb_context* c, int atom_numbers, int local_atom_numbers,
+                            int solvent_numbers, int ghost_numbers, const snb_vector* d_crd,
+                            const float* d_charge, snb_vector* d_frc, snb_ltmatrix3 cell,
+                            snb_ltmatrix3 rcell, const snb_atom_group* d_nl, float pme_beta,
+                            int need_atom_energy, float* d_atom_energy, int need_virial,
+                            snb_ltmatrix3* d_atom_virial, float* d_atom_direct_pme_energy)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->lj.init) return SNB_ERR_NOT_INITIALIZED;
+    if (solvent_numbers != 0 || ghost_numbers != 0) return SNB_ERR_UNSUPPORTED;
+    if (atom_numbers != c->n || local_atom_numbers != c->n) return SNB_ERR_ARGUMENT;
+    if (c->n == 0) return SNB_OK;
+    if (!d_crd || !d_charge || !d_nl) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SnbBox box = snb_box(cell, rcell);
+    int rc = snb_launch_pack_atoms(c, d_crd, d_charge);
+    if (rc) return rc;
+    if (need_atom_energy)
+    {
+        // the per-atom LJ and direct arrays restart each energy call
+        // (Lennard_Jones_force.cpp:376-382)
+        if ((rc = snb_launch_fill_i64(c->stream, c->e_atom, 2 * (size_t)c->n))) return rc;
+    }
+    rc = snb_launch_lj_direct(c, d_nl, box, pme_beta, need_atom_energy, need_virial);
+    if (rc) return rc;
+    if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
+    if (need_atom_energy)
+    {
+        if ((rc = snb_launch_sum_fixed(c, c->e_atom, c->n, SNB_S_E_LJ))) return rc;
+        if ((rc = snb_launch_sum_fixed(c, c->e_atom + c->n, c->n, SNB_S_E_DIRECT))) return rc;
+        if ((rc = snb_launch_flush_atom_energy(c, d_atom_energy, d_atom_direct_pme_energy, 3)))
+            return rc;
+    }
+    if (need_virial && d_atom_virial && (rc = snb_launch_flush_virial(c, d_atom_virial)))
+        return rc;
+    return SNB_OK;
+}
+
+int snb_lj_long_range_correction(snb_context* c, int need_pressure, snb_ltmatrix3* d_virial,
+                                 int need_potential, float* d_potential, float volume)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->lj.init) return SNB_ERR_NOT_INITIALIZED;
+    if (!(volume > 0.0f)) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    float f = c->lj.long_range_factor;
+    c->lj.last_long_energy = need_potential ? (double)(f / volume) : c->lj.last_long_energy;
+    snb_ltmatrix3* v = need_pressure ? d_virial : nullptr;
+    float* p = need_potential ? d_potential : nullptr;
+    if (!v && !p) return SNB_OK;
+    return snb_launch_lj_long_range(c->stream, v, p, 2 * f / volume, f / volume);
+}
+
+int snb_lj_energy(snb_context* c, double* lj_short, double* lj_long)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->lj.init) return SNB_ERR_NOT_INITIALIZED;
+    DeviceGuard g(c->device);
+    SNB_CHECK(hipStreamSynchronize(c->stream));
+    int64_t v = 0;
+    SNB_CHECK(hipMemcpy(&v, c->scal + SNB_S_E_LJ, sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (lj_short) *lj_short = snb_fixed_to_double(v);
+    if (lj_long) *lj_long = c->lj.last_long_energy;
+    return SNB_OK;
+}
+
+// ---------------------------------------------------------------------- PME
+int snb_pme_initial(snb_context* c, snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
+                    snb_vector box_length, float cutoff, float tolerance, int fftx, int ffty,
+                    int fftz, float grid_spacing, int update_interval)
+{
+    if (!c || !(cutoff > 0.0f) || !(tolerance > 0.0f) || update_interval <= 0)
+        return SNB_ERR_ARGUMENT;
+    if (grid_spacing <= 0.0f) grid_spacing = 1.0f;
+    DeviceGuard g(c->device);
+    SnbPme& P = c->pme;
+    free_pme(P);
+    P.cutoff = cutoff;
+    P.tolerance = tolerance;
+    P.update_interval = update_interval;
+    P.fx = fftx > 0 ? fftx : fft_size(box_length.x / grid_spacing);
+    P.fy = ffty > 0 ? ffty : fft_size(box_length.y / grid_spacing);
+    P.fz = fftz > 0 ? fftz : fft_size(box_length.z / grid_spacing);
+    P.beta = get_beta(cutoff, tolerance);
+    // PM_force.cpp:382, :409 (volume from the cell diagonal)
+    P.volume = (double)(cell.a11 * cell.a22 * cell.a33);
+    P.neutralizing_factor = (float)(-0.5 * M_PI / ((double)P.beta * P.beta * P.volume));
+    SnbBox box = snb_box(cell, rcell);
+    int rc = snb_pme_setup_tables(c, box);
+    if (rc)
+    {
+        free_pme(P);
+        return rc;
+    }
+    SNB_CHECK(hipStreamSynchronize(c->stream));
+    P.init = true;
+    return SNB_OK;
+}
+
+int snb_pme_parameters(snb_context* c, float* beta, int* fx, int* fy, int* fz)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->pme.init) return SNB_ERR_NOT_INITIALIZED;
+    if (beta) *beta = c->pme.beta;
+    if (fx) *fx = c->pme.fx;
+    if (fy) *fy = c->pme.fy;
+    if (fz) *fz = c->pme.fz;
+    return SNB_OK;
+}
+
+int snb_pme_excluded_force(snb_context* c, int atom_numbers, const int* d_atom_local,
+                           const int* d_atom_local_id, const snb_vector* d_crd,
+                           snb_ltmatrix3 cell, snb_ltmatrix3 rcell, const float* d_charge,
+                           const int* d_ex_start, const int* d_ex_list, const int* d_ex_num,
+                           snb_vector* d_frc, int need_energy, float* d_atom_energy,
+                           int need_virial, snb_ltmatrix3* d_atom_virial)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->pme.init) return SNB_ERR_NOT_INITIALIZED;
+    if (d_atom_local || d_atom_local_id) return SNB_ERR_UNSUPPORTED;
+    if (atom_numbers != c->n) return SNB_ERR_ARGUMENT;
+    if (c->n == 0) return SNB_OK;
+    if (!d_crd || !d_charge) return SNB_ERR_ARGUMENT;
+    const int* es = ex_or(d_ex_start, c->ex_start);
+    const int* el = ex_or(d_ex_list, c->ex_list);
+    const int* en = ex_or(d_ex_num, c->ex_num);
+    if (!d_ex_num && !c->has_excl) return SNB_ERR_NOT_INITIALIZED;
+    DeviceGuard g(c->device);
+    SnbBox box = snb_box(cell, rcell);
+    int rc = snb_launch_pack_atoms(c, d_crd, d_charge);
+    if (rc) return rc;
+    if (need_energy && (rc = snb_launch_fill_i64(c->stream, c->e_atom + 2 * (size_t)c->n, c->n)))
+        return rc;
+    rc = snb_launch_excl(c, c->atom4, box, c->pme.beta, es, el, en, need_energy, need_virial);
+    if (rc) return rc;
+    if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
+    if (need_energy)
+    {
+        if ((rc = snb_launch_sum_fixed(c, c->e_atom + 2 * (size_t)c->n, c->n, SNB_S_E_CORR)))
+            return rc;
+        if ((rc = snb_launch_flush_atom_energy(c, d_atom_energy, nullptr, 4))) return rc;
+    }
+    if (need_virial && d_atom_virial && (rc = snb_launch_flush_virial(c, d_atom_virial)))
+        return rc;
+    return SNB_OK;
+}
+
+int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatrix3 cell,
+                             snb_ltmatrix3 rcell, const float* d_charge, snb_vector* d_frc,
+                             int need_virial, int need_energy, snb_ltmatrix3* d_virial,
+                             float* d_potential, int step)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->pme.init) return SNB_ERR_NOT_INITIALIZED;
+    if (c->n > 0 && (!d_crd || !d_charge)) return SNB_ERR_ARGUMENT;
+    SnbPme& P = c->pme;
+    for (int k = 0; k < 6; k++)
+        if ((&rcell.a11)[k] != P.rcell[k]) return SNB_ERR_UNSUPPORTED;  // box change: re-initialise
+    DeviceGuard g(c->device);
+    SnbBox box = snb_box(cell, rcell);
+    int rc;
+    // PM_force.cpp:1080-1137: the reciprocal force (scaled by update_interval)
+    // and virial exist only on steps with step % update_interval == 0; on
+    // other steps only the energy is evaluated (here from the current
+    // coordinates, the reference reuses the grids of the last update step).
+    bool do_force = (step % P.update_interval) == 0;
+    if (!do_force) need_virial = 0;
+    if (c->n > 0 && (rc = snb_launch_pack_atoms(c, d_crd, d_charge))) return rc;
+    if (need_energy && (rc = snb_launch_charge_sums(c, d_charge))) return rc;
+    if (do_force || need_energy)
+    {
+        if ((rc = snb_launch_pme_recip(c, box, need_energy, need_virial, do_force ? 1 : 0)))
+            return rc;
+        P.have_energy = need_energy != 0;
+    }
+    if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
+    if (need_energy && d_potential && (rc = snb_launch_pme_energy_to_potential(c, d_potential)))
+        return rc;
+    if (need_virial && d_virial && (rc = snb_launch_pme_virial_to(c, d_virial))) return rc;
+    return SNB_OK;
+}
+
+int snb_pme_energy(snb_context* c, double* direct, double* reciprocal, double* self,
+                   double* correction)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SNB_CHECK(hipStreamSynchronize(c->stream));
+    int64_t s[SNB_S_COUNT];
+    double d[SNB_D_COUNT];
+    SNB_CHECK(hipMemcpy(s, c->scal, sizeof(s), hipMemcpyDeviceToHost));
+    SNB_CHECK(hipMemcpy(d, c->dscal, sizeof(d), hipMemcpyDeviceToHost));
+    if (direct) *direct = snb_fixed_to_double(s[SNB_S_E_DIRECT]);
+    if (correction) *correction = snb_fixed_to_double(s[SNB_S_E_CORR]);
+    if (reciprocal) *reciprocal = d[SNB_D_E_REC];
+    if (self) *self = d[SNB_D_E_SELF];
+    return SNB_OK;
+}
+
+int snb_pme_reciprocal_virial(snb_context* c, double* v6)
+{
+    if (!c || !v6) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SNB_CHECK(hipStreamSynchronize(c->stream));
+    double d[SNB_D_COUNT];
+    SNB_CHECK(hipMemcpy(d, c->dscal, sizeof(d), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 6; k++) v6[k] = d[SNB_D_VIR0 + k];
+    return SNB_OK;
+}
+
+// ------------------------------------------------------- fixed-point helpers
+int snb_flush_forces(snb_context* c, snb_vector* d_frc)
+{
+    if (!c || (!d_frc && c->n > 0)) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    return snb_launch_flush_forces(c, d_frc);
+}
+
+int snb_copy_fixed_forces(snb_context* c, int64_t* d_out)
+{
+    if (!c || !d_out) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SNB_CHECK(hipMemcpyAsync(d_out, c->fx, sizeof(int64_t) * 3 * (size_t)c->n,
+                             hipMemcpyDeviceToDevice, c->stream));
+    return SNB_OK;
+}
+
+int snb_clear_fixed_forces(snb_context* c)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    return snb_launch_fill_i64(c->stream, c->fx, 3 * (size_t)c->n);
+}
+
+int snb_copy_fixed_atom_terms(snb_context* c, int which, int64_t* d_out)
+{
+    if (!c || !d_out || which < 0 || which > 3) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    const int64_t* src = which < 3 ? c->e_atom + (size_t)which * c->n : c->vir_atom;
+    size_t cnt = which < 3 ? (size_t)c->n : 6 * (size_t)c->n;
+    SNB_CHECK(hipMemcpyAsync(d_out, src, sizeof(int64_t) * cnt, hipMemcpyDeviceToDevice,
+                             c->stream));
+    return SNB_OK;
+}
+
+int snb_copy_pme_grid(snb_context* c, int which, float* d_out)
+{
+    if (!c || !d_out || which < 0 || which > 1) return SNB_ERR_ARGUMENT;
+    if (!c->pme.init) return SNB_ERR_NOT_INITIALIZED;
+    DeviceGuard g(c->device);
+    size_t nall = (size_t)c->pme.fx * c->pme.fy * c->pme.fz;
+    SNB_CHECK(hipMemcpyAsync(d_out, which == 0 ? c->pme.q : c->pme.phi, sizeof(float) * nall,
+                             hipMemcpyDeviceToDevice, c->stream));
+    return SNB_OK;
+}
+
+int snb_fft_r2c(snb_context* c, const float* d_real, float* d_complex)
+{
+    if (!c || !d_real || !d_complex) return SNB_ERR_ARGUMENT;
+    if (!c->pme.init) return SNB_ERR_NOT_INITIALIZED;
+    DeviceGuard g(c->device);
+    return snb_fft_forward_full(c, d_real, (float2*)d_complex);
+}
+
+int snb_fft_c2r(snb_context* c, const float* d_complex, float* d_real)
+{
+    if (!c || !d_real || !d_complex) return SNB_ERR_ARGUMENT;
+    if (!c->pme.init) return SNB_ERR_NOT_INITIALIZED;
+    DeviceGuard g(c->device);
+    return snb_launch_fft_c2r(c, (const float2*)d_complex, d_real);
+}
+
+int snb_profile_enable(snb_context* c, int enable)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SnbProf& P = c->prof;
+    if (enable && !P.ev)
+    {
+        size_t m = (size_t)SNB_PROF_GROUPS * SNB_PROF_CAP * 2;
+        P.ev = (hipEvent_t*)calloc(m, sizeof(hipEvent_t));
+        if (!P.ev) return SNB_ERR_OUT_OF_MEMORY;
+        for (size_t k = 0; k < m; k++) SNB_CHECK(hipEventCreate(&P.ev[k]));
+    }
+    P.on = enable != 0;
+    return SNB_OK;
+}
+
+int snb_profile_read(snb_context* c, int group, int reset, double* total_ms, int* launches)
+{
+    if (!c || group < 0 || group >= SNB_PROF_GROUPS) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SNB_CHECK(hipStreamSynchronize(c->stream));
+    SnbProf& P = c->prof;
+    if (P.ev)
+    {
+        int rc = prof_collect(c, group);
+        if (rc) return rc;
+    }
+    if (total_ms) *total_ms = P.done_ms[group];
+    if (launches) *launches = P.done_n[group];
+    if (reset)
+    {
+        P.done_ms[group] = 0.0;
+        P.done_n[group] = 0;
+    }
+    return SNB_OK;
+}
+
+}  // extern "C"

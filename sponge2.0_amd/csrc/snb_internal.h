@@ -1,0 +1,206 @@
+// snb_internal.h -- context layout and kernel launchers of libsponge_nb.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sponge_nb.h"
+#include "snb_math.h"
+
+// Scalar accumulator slots (int64 fixed point, device, zeroed per call that
+// produces them).
+enum SnbScalarSlot
+{
+    SNB_S_E_LJ = 0,      // sum of per-atom LJ energy
+    SNB_S_E_DIRECT = 1,  // sum of per-atom direct Coulomb energy
+    SNB_S_E_CORR = 2,    // sum of per-atom exclusion-correction energy
+    SNB_S_Q2 = 3,        // sum q^2
+    SNB_S_Q = 4,         // sum q
+    SNB_S_COUNT = 8
+};
+
+// Double scalar slots written by the PME reduction.
+enum SnbDoubleSlot
+{
+    SNB_D_E_REC = 0,
+    SNB_D_VIR0 = 1,  // 6 slots: reciprocal virial a11..a33
+    SNB_D_E_SELF = 7,
+    SNB_D_COUNT = 8
+};
+
+#define SNB_FFT_MAX_STAGES 16
+#define SNB_FFT_MAX_N 1024
+
+struct SnbFftPlan
+{
+    int n;
+    int nstages;
+    int radix[SNB_FFT_MAX_STAGES];
+};
+
+struct SnbNl
+{
+    bool init = false;
+    float cutoff = 10.0f, skin = 2.0f, grid_length = 6.0f, cs2 = 144.0f;
+    float skin_permit = 0.5f;
+    int refresh_interval = 0;
+    int max_nb = 1200, max_in_cell = 150;
+    int nx = 0, ny = 0, nz = 0, ncell = 0;
+    int* cell_count = nullptr;   // ncell + 1
+    int* cell_start = nullptr;   // ncell + 1 (exclusive scan)
+    int* atom_cell = nullptr;    // n
+    int* atom_slot = nullptr;    // n
+    int* sorted_id = nullptr;    // n (scatter order)
+    float4* sorted_crd = nullptr;  // n: x, y, z, bits(atom id), sorted by cell then id
+    int* serial = nullptr;       // n * max_nb
+    snb_atom_group* dnl = nullptr;  // n
+    int* need = nullptr;         // rebuild flag (device)
+    snb_vector* old_crd = nullptr;
+};
+
+struct SnbLj
+{
+    bool init = false;
+    int ntypes = 0;
+    float cutoff = 10.0f;
+    float* a = nullptr;  // pair tables
+    float* b = nullptr;
+    double c6_total = 0.0;
+    float long_range_factor = 0.0f;
+    double last_long_energy = 0.0;
+};
+
+struct SnbPme
+{
+    bool init = false;
+    int fx = 0, fy = 0, fz = 0, fc = 0;
+    int bx = 0, by = 0, bz = 0, nbx = 0, nby = 0, nbz = 0, nbrick = 0;
+    float beta = 0.0f, tolerance = 1e-5f, cutoff = 10.0f;
+    int update_interval = 1;
+    double volume = 0.0;
+    float neutralizing_factor = 0.0f;
+    SnbFftPlan plan_x, plan_y, plan_z;
+    float2* tw_x = nullptr;  // e^{-2 pi i k/n}
+    float2* tw_y = nullptr;
+    float2* tw_z = nullptr;
+    float* bc = nullptr;      // [fx][fy][fc]
+    float* q = nullptr;       // [fx][fy][fz]
+    float* phi = nullptr;     // [fx][fy][fz]
+    float2* spec = nullptr;   // [fx][fy][fc]
+    int4* atom_u = nullptr;   // ux, uy, uz, brick
+    float4* atom_fr = nullptr;  // frx, fry, frz, q
+    int* brick_count = nullptr;  // nbrick + 1
+    int* brick_start = nullptr;  // nbrick + 1
+    int* atom_slot = nullptr;
+    int* brick_atoms = nullptr;
+    double* partials = nullptr;  // conv blocks * 8
+    int conv_blocks = 0;
+    float rcell[6];
+    bool have_energy = false;
+};
+
+#define SNB_PROF_GROUPS 4
+#define SNB_PROF_CAP 4096
+
+// Event pairs recorded around kernel groups while profiling is enabled.
+struct SnbProf
+{
+    bool on = false;
+    hipEvent_t* ev = nullptr;  // SNB_PROF_GROUPS * SNB_PROF_CAP * 2
+    int count[SNB_PROF_GROUPS] = {0, 0, 0, 0};
+    double done_ms[SNB_PROF_GROUPS] = {0, 0, 0, 0};
+    int done_n[SNB_PROF_GROUPS] = {0, 0, 0, 0};
+};
+
+struct snb_context
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n = 0;
+    float4* atom4 = nullptr;  // x, y, z, q
+    int* type = nullptr;
+    int* status = nullptr;
+    int64_t* fx = nullptr;       // 3n, SoA
+    int64_t* e_atom = nullptr;   // 3n: LJ | direct | correction
+    int64_t* vir_atom = nullptr; // 6n, SoA
+    int64_t* scal = nullptr;     // SNB_S_COUNT
+    double* dscal = nullptr;     // SNB_D_COUNT
+    int* scan_tmp = nullptr;     // scratch for scans
+    int scan_tmp_len = 0;
+    // exclusions
+    int* ex_start = nullptr;
+    int* ex_list = nullptr;
+    int* ex_num = nullptr;
+    int ex_total = 0;
+    bool has_excl = false;
+    SnbNl nl;
+    SnbLj lj;
+    SnbPme pme;
+    SnbProf prof;
+};
+
+// Profiling brackets (no-ops unless snb_profile_enable was called).
+void snb_prof_begin(snb_context* ctx, int group);
+void snb_prof_end(snb_context* ctx, int group);
+
+// ---------------------------------------------------------------------------
+// launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+#define SNB_CHECK(x)                              \
+    do                                            \
+    {                                             \
+        hipError_t e_ = (x);                      \
+        if (e_ != hipSuccess) return SNB_ERR_HIP; \
+    } while (0)
+
+struct SnbBox
+{
+    float c[6];   // cell a11 a21 a22 a31 a32 a33
+    float rc[6];  // rcell
+};
+
+static inline SnbBox snb_box(snb_ltmatrix3 cell, snb_ltmatrix3 rcell)
+{
+    SnbBox b;
+    b.c[0] = cell.a11; b.c[1] = cell.a21; b.c[2] = cell.a22;
+    b.c[3] = cell.a31; b.c[4] = cell.a32; b.c[5] = cell.a33;
+    b.rc[0] = rcell.a11; b.rc[1] = rcell.a21; b.rc[2] = rcell.a22;
+    b.rc[3] = rcell.a31; b.rc[4] = rcell.a32; b.rc[5] = rcell.a33;
+    return b;
+}
+
+// common (snb_common.hip)
+int snb_launch_pack_atoms(snb_context* ctx, const snb_vector* crd, const float* charge);
+int snb_launch_fill_int(hipStream_t s, int* p, int n, int v);
+int snb_launch_fill_i64(hipStream_t s, int64_t* p, size_t n);
+int snb_launch_exclusive_scan(hipStream_t s, const int* in, int* out, int n);
+int snb_launch_flush_forces(snb_context* ctx, snb_vector* frc);
+int snb_launch_flush_atom_energy(snb_context* ctx, float* atom_energy, float* direct_energy,
+                                 int which_mask);
+int snb_launch_flush_virial(snb_context* ctx, snb_ltmatrix3* atom_virial);
+int snb_launch_sum_fixed(snb_context* ctx, const int64_t* arr, int n, int slot);
+int snb_launch_charge_sums(snb_context* ctx, const float* charge);
+int snb_launch_add_scalar(hipStream_t s, float* dst, float v);
+int snb_launch_lj_long_range(hipStream_t s, snb_ltmatrix3* vir, float* pot, float vir_add,
+                             float pot_add);
+
+// neighbor list (snb_nl.hip)
+int snb_launch_nl_update(snb_context* ctx, const snb_vector* crd, const SnbBox& box, int forced,
+                         int step, const int* ex_start, const int* ex_list, const int* ex_num);
+
+// pair kernel (snb_pair.hip)
+int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBox& box,
+                         float beta, int need_energy, int need_virial);
+int snb_launch_excl(snb_context* ctx, const float4* atoms, const SnbBox& box, float beta,
+                    const int* ex_start, const int* ex_list, const int* ex_num,
+                    int need_energy, int need_virial);
+
+// PME (snb_pme.hip)
+int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box);
+int snb_launch_pme_recip(snb_context* ctx, const SnbBox& box, int need_energy, int need_virial,
+                         int with_force);
+int snb_fft_forward_full(snb_context* ctx, const float* real, float2* cplx);
+int snb_launch_pme_energy_to_potential(snb_context* ctx, float* potential);
+int snb_launch_pme_virial_to(snb_context* ctx, snb_ltmatrix3* vir);
+int snb_fft_make_plan(int n, SnbFftPlan* plan);
+int snb_launch_fft_r2c(snb_context* ctx, const float* real, float2* cplx);
+int snb_launch_fft_c2r(snb_context* ctx, const float2* cplx, float* real);

@@ -1,0 +1,137 @@
+// snb_math.h -- the float arithmetic of the nonbonded path, host+device.
+//
+// Every function here uses only IEEE-defined operations (add, sub, mul,
+// correctly rounded div, fma, floor, rint, integer bit manipulation), so the
+// same source evaluated on gfx950 and on a CPU gives the same bits.  The
+// library is built with -ffp-contract=off; an fma appears only where it is
+// written.  Per-pair / per-point terms are rounded to int64 fixed point
+// (scale 2^32) before they are summed, which makes every accumulator
+// independent of summation order.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define SNB_HD __host__ __device__ __forceinline__
+#else
+#define SNB_HD static inline
+#endif
+
+#define SNB_FIXED_SCALE 4294967296.0          // 2^32
+#define SNB_FIXED_INV 2.3283064365386963e-10  // 2^-32
+#define SNB_TWO_OVER_SQRT_PI 1.1283791670218446f  // Lennard_Jones_force.h:7
+
+SNB_HD float snb_f_from_bits(uint32_t u) { return __builtin_bit_cast(float, u); }
+SNB_HD uint32_t snb_bits_from_f(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+// round(f * 2^32), ties to even, exact for |f| < 2^19: adding 1.5*2^52 puts
+// the integer in the low mantissa bits of a double in [2^52, 2^53).
+SNB_HD int64_t snb_to_fixed(float f)
+{
+    double d = __builtin_fma((double)f, SNB_FIXED_SCALE, 6755399441055744.0);
+    return (int64_t)__builtin_bit_cast(uint64_t, d) - (int64_t)0x4338000000000000LL;
+}
+
+SNB_HD float snb_fixed_to_float(int64_t v) { return (float)((double)v * SNB_FIXED_INV); }
+SNB_HD double snb_fixed_to_double(int64_t v) { return (double)v * SNB_FIXED_INV; }
+
+// exp(y), y in [-87, 0]: Cody-Waite reduction y = n ln2 + r (ln2 split into
+// a 16-bit head and a tail so n*head is exact), degree-6 polynomial for e^r
+// fitted by weighted least squares (Lawson iteration) on [-ln2/2, ln2/2],
+// then 2^n assembled in the exponent field.
+SNB_HD float snb_expf(float y)
+{
+    y = __builtin_fmaxf(y, -87.0f);
+    float n = __builtin_rintf(y * 1.44269504088896341f);
+    float r = __builtin_fmaf(-n, 0.693145751953125f, y);
+    r = __builtin_fmaf(-n, 1.428606765330187045e-06f, r);
+    float p = 0.0013836846f;
+    p = __builtin_fmaf(p, r, 0.008374816f);
+    p = __builtin_fmaf(p, r, 0.041668225f);
+    p = __builtin_fmaf(p, r, 0.1666642f);
+    p = __builtin_fmaf(p, r, 0.4999999f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    int32_t ni = (int32_t)n;
+    return p * snb_f_from_bits((uint32_t)(ni + 127) << 23);
+}
+
+// erfc(x), x >= 0, given ex = exp(-x^2): t = 1/(1 + 0.3275911 x) (the
+// substitution of Abramowitz & Stegun 7.1.26) and a degree-7 polynomial in t
+// fitted on [0, 5] for minimum relative error (4.5e-8 in exact arithmetic).
+SNB_HD float snb_erfc_core(float x, float ex)
+{
+    float t = 1.0f / __builtin_fmaf(0.3275911f, x, 1.0f);
+    float q = -0.1413204f;
+    q = __builtin_fmaf(q, t, 0.49561852f);
+    q = __builtin_fmaf(q, t, -0.39975524f);
+    q = __builtin_fmaf(q, t, 0.44215718f);
+    q = __builtin_fmaf(q, t, 0.0353266f);
+    q = __builtin_fmaf(q, t, 0.20112583f);
+    q = __builtin_fmaf(q, t, 0.18191357f);
+    q = __builtin_fmaf(q, t, 0.18493392f);
+    return (t * q) * ex;
+}
+
+// h(x) = erf(x) - (2/sqrt(pi)) x exp(-x^2) without cancellation: Maclaurin
+// series h = (2/sqrt(pi)) sum_{n>=1} (-1)^{n+1} 2n x^{2n+1} / (n! (2n+1)) for
+// x < 0.5 (8 terms), erf = 1 - erfc above.
+SNB_HD float snb_erf_minus_gauss(float x, float ex)
+{
+    if (x < 0.5f)
+    {
+        float x2 = x * x;
+        float s = -2.6339383e-05f;
+        s = __builtin_fmaf(s, x2, 2.08959105e-04f);
+        s = __builtin_fmaf(s, x2, -1.44663996e-03f);
+        s = __builtin_fmaf(s, x2, 8.54832702e-03f);
+        s = __builtin_fmaf(s, x2, -4.1791821e-02f);
+        s = __builtin_fmaf(s, x2, 1.61197024e-01f);
+        s = __builtin_fmaf(s, x2, -4.51351667e-01f);
+        s = __builtin_fmaf(s, x2, 7.52252778e-01f);
+        return (s * x2) * x;
+    }
+    float erfv = 1.0f - snb_erfc_core(x, ex);
+    return erfv - (x * SNB_TWO_OVER_SQRT_PI) * ex;
+}
+
+// 1/sqrt(x), x > 0 normal: seed 0x5f375a86 - (bits >> 1) (Lomont, "Fast
+// inverse square root", 2003), three Newton steps y <- y (1.5 - (x/2) y^2).
+SNB_HD float snb_rsqrtf(float x)
+{
+    float y = snb_f_from_bits(0x5f375a86u - (snb_bits_from_f(x) >> 1));
+    float h = 0.5f * x;
+    y = y * __builtin_fmaf(-(h * y), y, 1.5f);
+    y = y * __builtin_fmaf(-(h * y), y, 1.5f);
+    y = y * __builtin_fmaf(-(h * y), y, 1.5f);
+    return y;
+}
+
+// Triangular LJ type-pair index (Lennard_Jones_force.h:18-27).
+SNB_HD int snb_lj_pair_index(int a, int b)
+{
+    int hi = a > b ? a : b, lo = a > b ? b : a;
+    return ((hi * (hi + 1)) >> 1) + lo;
+}
+
+// Cubic B-spline weight w_k(t) and derivative for stencil offset k, with the
+// reference's coefficient tables and evaluation order (PM_force.cpp:18-24,
+// :776-805, :859-880).
+SNB_HD float snb_bspline_w(int k, float t)
+{
+    const float ma = k == 0 ? 1.0f / 6.0f : (k == 1 ? -0.5f : (k == 2 ? 0.5f : -1.0f / 6.0f));
+    const float mb = k == 0 ? 0.0f : (k == 1 ? 0.5f : (k == 2 ? -1.0f : 0.5f));
+    const float mc = k == 0 ? 0.0f : (k == 1 ? 0.5f : (k == 2 ? 0.0f : -0.5f));
+    const float md = k == 0 ? 0.0f : (k == 1 ? 1.0f / 6.0f : (k == 2 ? 4.0f / 6.0f : 1.0f / 6.0f));
+    float t2 = t * t;
+    return ((ma * t) * t2 + mb * t2) + mc * t + md;
+}
+
+SNB_HD float snb_bspline_dw(int k, float t)
+{
+    const float da = k == 0 ? 0.5f : (k == 1 ? -1.5f : (k == 2 ? 1.5f : -0.5f));
+    const float db = k == 0 ? 0.0f : (k == 1 ? 1.0f : (k == 2 ? -2.0f : 1.0f));
+    const float dc = k == 0 ? 0.0f : (k == 1 ? 0.5f : (k == 2 ? 0.0f : -0.5f));
+    float t2 = t * t;
+    return (da * t2 + db * t) + dc;
+}

@@ -1,0 +1,901 @@
+// snb_pme.hip -- smooth PME reciprocal space on gfx950 (stands in for
+// Particle_Mesh::PME_Reciprocal_Force_With_Energy_And_Virial,
+// PM_force.cpp:1067-1165).
+//
+// Pipeline per call (all on the context stream, no host synchronisation):
+//   prep    per atom: base grid point u, spline fraction, brick id, counting
+//           sort of atoms by brick (PME_Atom_Near, PM_force.cpp:697-759);
+//   spread  one workgroup per grid brick: the atoms of the brick and of its
+//           +1 neighbours deposit their 4x4x4 B-spline stencils into an int64
+//           fixed-point LDS tile, which is written once (no global atomics,
+//           result independent of atom order; PME_Q_Spread :762-811);
+//   FFT     5 passes of LDS-resident line transforms (radix 4/2/3/5/7
+//           Stockham): z R2C, y, then x forward + influence-function product +
+//           energy/virial accumulation + x inverse fused in one kernel, y
+//           inverse, z C2R (hipFFT R2C/C2R + PME_BCFQ + PME_Sum_Virial,
+//           :1100-1122, :814-823, :1021-1065);
+//   gather  one thread per atom sums the 64 stencil points of the potential
+//           grid in a fixed order (PME_Final :826-890) and adds the force in
+//           fixed point.
+#include <math.h>
+
+#include <vector>
+
+#include "snb_internal.h"
+
+// ---------------------------------------------------------------------------
+// FFT plans and line transforms
+// ---------------------------------------------------------------------------
+int snb_fft_make_plan(int n, SnbFftPlan* plan)
+{
+    if (n < 1 || n > SNB_FFT_MAX_N) return SNB_ERR_UNSUPPORTED;
+    plan->n = n;
+    plan->nstages = 0;
+    int m = n;
+    const int order[5] = {4, 2, 3, 5, 7};
+    for (int k = 0; k < 5; k++)
+        while (m % order[k] == 0)
+        {
+            if (plan->nstages >= SNB_FFT_MAX_STAGES) return SNB_ERR_UNSUPPORTED;
+            plan->radix[plan->nstages++] = order[k];
+            m /= order[k];
+        }
+    if (m != 1) return SNB_ERR_UNSUPPORTED;
+    return SNB_OK;
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by s*i, s = +-1
+__device__ __forceinline__ float2 cmul_si(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
+
+// In-register DFT of length R with exponent sign s (X_m = sum v_r e^{s 2 pi i r m / R}).
+template <int R>
+__device__ __forceinline__ void small_dft(float2* v, float s);
+
+template <>
+__device__ __forceinline__ void small_dft<2>(float2* v, float s)
+{
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <>
+__device__ __forceinline__ void small_dft<4>(float2* v, float s)
+{
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = cmul_si(csub(v[1], v[3]), s);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+}
+
+template <>
+__device__ __forceinline__ void small_dft<3>(float2* v, float s)
+{
+    const float c = -0.5f, sn = 0.86602540378443865f;
+    float2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    float2 m = make_float2(v[0].x + c * t1.x, v[0].y + c * t1.y);
+    float2 r = cmul_si(make_float2(sn * t2.x, sn * t2.y), s);
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+}
+
+template <>
+__device__ __forceinline__ void small_dft<5>(float2* v, float s)
+{
+    const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+    const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+    float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    float2 a0 = v[0];
+    float2 m1 = make_float2(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
+    float2 m2 = make_float2(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
+    float2 r1 = cmul_si(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), s);
+    float2 r2 = cmul_si(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), s);
+    v[0] = cadd(a0, cadd(t1, t2));
+    v[1] = cadd(m1, r1);
+    v[4] = csub(m1, r1);
+    v[2] = cadd(m2, r2);
+    v[3] = csub(m2, r2);
+}
+
+template <>
+__device__ __forceinline__ void small_dft<7>(float2* v, float s)
+{
+    const float C[7] = {1.0f, 0.62348980185873353f, -0.22252093395631440f,
+                        -0.90096886790241913f, -0.90096886790241913f,
+                        -0.22252093395631440f, 0.62348980185873353f};
+    const float S[7] = {0.0f, 0.78183148246802981f, 0.97492791218182361f,
+                        0.43388373911755812f, -0.43388373911755812f,
+                        -0.97492791218182361f, -0.78183148246802981f};
+    float2 out[7];
+    for (int m = 0; m < 7; m++)
+    {
+        float2 acc = v[0];
+        for (int r = 1; r < 7; r++)
+        {
+            int q = (r * m) % 7;
+            float2 w = make_float2(C[q], s * S[q]);
+            acc = cadd(acc, cmul(v[r], w));
+        }
+        out[m] = acc;
+    }
+    for (int m = 0; m < 7; m++) v[m] = out[m];
+}
+
+// One Stockham stage for `lines` lines of length n stored contiguously
+// (stride n) in src, result in dst.  tw = e^{-2 pi i k/n} table in LDS.
+template <int R>
+__device__ void stockham_stage(const float2* src, float2* dst, int n, int lines, int ns,
+                               const float2* tw, float s)
+{
+    const int nr = n / R;
+    const int total = lines * nr;
+    for (int t = threadIdx.x; t < total; t += blockDim.x)
+    {
+        int line = t / nr, j = t - line * nr;
+        const float2* in = src + line * n;
+        float2* out = dst + line * n;
+        int k = j % ns;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = in[j + r * nr];
+        if (ns > 1)
+        {
+            int step = n / (ns * R);  // twiddle w^{r k}, w = e^{s 2 pi i/(ns R)}
+#pragma unroll
+            for (int r = 1; r < R; r++)
+            {
+                float2 w = tw[(r * k * step) % n];
+                w.y *= -s;  // table holds e^{-i..}; s = -1 keeps it, s = +1 conjugates
+                v[r] = cmul(v[r], w);
+            }
+        }
+        small_dft<R>(v, s);
+        int idx = (j / ns) * ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) out[idx + r * ns] = v[r];
+    }
+}
+
+// Transforms `lines` contiguous lines in a (ping-pong with b); returns the
+// buffer holding the result.
+__device__ float2* fft_lines(float2* a, float2* b, const SnbFftPlan& plan, int lines,
+                             const float2* tw, float s)
+{
+    int ns = 1;
+    float2* src = a;
+    float2* dst = b;
+    for (int st = 0; st < plan.nstages; st++)
+    {
+        int R = plan.radix[st];
+        switch (R)
+        {
+            case 4: stockham_stage<4>(src, dst, plan.n, lines, ns, tw, s); break;
+            case 2: stockham_stage<2>(src, dst, plan.n, lines, ns, tw, s); break;
+            case 3: stockham_stage<3>(src, dst, plan.n, lines, ns, tw, s); break;
+            case 5: stockham_stage<5>(src, dst, plan.n, lines, ns, tw, s); break;
+            default: stockham_stage<7>(src, dst, plan.n, lines, ns, tw, s); break;
+        }
+        __syncthreads();
+        ns *= R;
+        float2* t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
+__device__ void load_twiddles(float2* dst, const float2* src, int n)
+{
+    for (int k = threadIdx.x; k < n; k += blockDim.x) dst[k] = src[k];
+}
+
+static int lines_per_block(int n)
+{
+    // two ping-pong buffers of lines*n float2 within ~48 KB of LDS
+    int l = 49152 / (16 * n);
+    if (l > 64) l = 64;
+    if (l < 1) l = 1;
+    return l;
+}
+
+// z pass forward: real rows [nrow][fz] -> half spectra [nrow][fc]
+__global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ real,
+                                                   float2* __restrict__ spec, int nrow,
+                                                   SnbFftPlan plan, int fc,
+                                                   const float2* __restrict__ twg, int L)
+{
+    extern __shared__ float2 smem[];
+    int n = plan.n;
+    float2* tw = smem;
+    float2* a = smem + n;
+    float2* b = a + L * n;
+    load_twiddles(tw, twg, n);
+    int row0 = blockIdx.x * L;
+    int lines = min(L, nrow - row0);
+    for (int t = threadIdx.x; t < lines * n; t += blockDim.x)
+        a[t] = make_float2(real[(size_t)row0 * n + t], 0.0f);
+    __syncthreads();
+    float2* res = fft_lines(a, b, plan, lines, tw, -1.0f);
+    for (int t = threadIdx.x; t < lines * fc; t += blockDim.x)
+    {
+        int l = t / fc, k = t - l * fc;
+        spec[(size_t)(row0 + l) * fc + k] = res[l * n + k];
+    }
+}
+
+// z pass inverse: half spectra [nrow][fc] -> real rows [nrow][fz]
+__global__ void __launch_bounds__(256) k_fft_z_c2r(const float2* __restrict__ spec,
+                                                   float* __restrict__ real, int nrow,
+                                                   SnbFftPlan plan, int fc,
+                                                   const float2* __restrict__ twg, int L)
+{
+    extern __shared__ float2 smem[];
+    int n = plan.n;
+    float2* tw = smem;
+    float2* a = smem + n;
+    float2* b = a + L * n;
+    load_twiddles(tw, twg, n);
+    int row0 = blockIdx.x * L;
+    int lines = min(L, nrow - row0);
+    for (int t = threadIdx.x; t < lines * n; t += blockDim.x)
+    {
+        int l = t / n, k = t - l * n;
+        float2 v;
+        if (k < fc)
+            v = spec[(size_t)(row0 + l) * fc + k];
+        else
+        {
+            v = spec[(size_t)(row0 + l) * fc + (n - k)];
+            v.y = -v.y;
+        }
+        a[t] = v;
+    }
+    __syncthreads();
+    float2* res = fft_lines(a, b, plan, lines, tw, 1.0f);
+    for (int t = threadIdx.x; t < lines * n; t += blockDim.x)
+        real[(size_t)row0 * n + t] = res[t].x;
+}
+
+// y pass over spec[fx][fy][fc]: block = (x, chunk of W consecutive kz).
+__global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ spec, int fx, int fy,
+                                               int fc, SnbFftPlan plan,
+                                               const float2* __restrict__ twg, int W, float s)
+{
+    extern __shared__ float2 smem[];
+    int n = plan.n;  // == fy
+    float2* tw = smem;
+    float2* a = smem + n;
+    float2* b = a + W * n;
+    load_twiddles(tw, twg, n);
+    int nchunk = (fc + W - 1) / W;
+    int x = blockIdx.x / nchunk;
+    int k0 = (blockIdx.x - x * nchunk) * W;
+    int w = min(W, fc - k0);
+    for (int t = threadIdx.x; t < n * w; t += blockDim.x)
+    {
+        int y = t / w, l = t - y * w;
+        a[l * n + y] = spec[((size_t)x * fy + y) * fc + k0 + l];
+    }
+    __syncthreads();
+    float2* res = fft_lines(a, b, plan, w, tw, s);
+    for (int t = threadIdx.x; t < n * w; t += blockDim.x)
+    {
+        int y = t / w, l = t - y * w;
+        spec[((size_t)x * fy + y) * fc + k0 + l] = res[l * n + y];
+    }
+}
+
+// x pass with the convolution: block = (y, chunk of W consecutive kz).
+// Forward FFT along x, then per element (kx, y, kz):
+//   energy  += 1/2 * wfull * BC |F|^2        (wfull = 1 on kz = 0 and the
+//                                               Nyquist plane, else 2)
+//   virial  -= wref * 1/2 BC |F|^2 (delta - 2 (1 + pi^2 m^2/beta^2) m m/m^2)
+//             (wref = 0.5 on kz = 0 and kz = fz/2, else 1: PM_force.cpp:1033-1037)
+//   F *= BC
+// then inverse FFT along x.  Per-block partial sums go to partials[block][8].
+template <bool NEED_EV>
+__global__ void __launch_bounds__(256) k_fft_x_conv(
+    float2* __restrict__ spec, const float* __restrict__ bc, int fx, int fy, int fz, int fc,
+    SnbFftPlan plan, const float2* __restrict__ twg, int W, double r11, double r21,
+    double r22, double r31, double r32, double r33, double mpre, double* __restrict__ partials)
+{
+    extern __shared__ float2 smem[];
+    int n = plan.n;  // == fx
+    float2* tw = smem;
+    float2* a = smem + n;
+    float2* b = a + W * n;
+    load_twiddles(tw, twg, n);
+    int nchunk = (fc + W - 1) / W;
+    int y = blockIdx.x / nchunk;
+    int k0 = (blockIdx.x - y * nchunk) * W;
+    int w = min(W, fc - k0);
+    for (int t = threadIdx.x; t < n * w; t += blockDim.x)
+    {
+        int x = t / w, l = t - x * w;
+        a[l * n + x] = spec[((size_t)x * fy + y) * fc + k0 + l];
+    }
+    __syncthreads();
+    float2* res = fft_lines(a, b, plan, w, tw, -1.0f);
+    double e = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+    for (int t = threadIdx.x; t < n * w; t += blockDim.x)
+    {
+        int l = t / n, kx = t - l * n;
+        int kz = k0 + l;
+        float bcv = bc[((size_t)kx * fy + y) * fc + kz];
+        float2 f = res[t];
+        if (NEED_EV && bcv != 0.0f)
+        {
+            double a2 = (double)f.x * f.x + (double)f.y * f.y;
+            double wfull = (kz == 0 || (2 * kz == fz)) ? 1.0 : 2.0;
+            e += 0.5 * wfull * (double)bcv * a2;
+            double wref = (kz == 0 || kz == fc - 1) ? 0.5 : 1.0;
+            double kxr = kx > fx / 2 ? kx - fx : kx;
+            double kyr = y > fy / 2 ? y - fy : y;
+            double mx = kxr * r11;
+            double my = kxr * r21 + kyr * r22;
+            double mz = kxr * r31 + kyr * r32 + kz * r33;
+            double msq = mx * mx + my * my + mz * mz;
+            double cf = 2.0 / msq * (1.0 + mpre * msq);
+            double hb = wref * 0.5 * (double)bcv * a2;
+            v0 -= hb * (1.0 - cf * mx * mx);
+            v1 -= hb * (0.0 - cf * my * mx);
+            v2 -= hb * (1.0 - cf * my * my);
+            v3 -= hb * (0.0 - cf * mz * mx);
+            v4 -= hb * (0.0 - cf * mz * my);
+            v5 -= hb * (1.0 - cf * mz * mz);
+        }
+        res[t] = make_float2(f.x * bcv, f.y * bcv);
+    }
+    __syncthreads();
+    float2* other = (res == a) ? b : a;
+    float2* res2 = fft_lines(res, other, plan, w, tw, 1.0f);
+    for (int t = threadIdx.x; t < n * w; t += blockDim.x)
+    {
+        int x = t / w, l = t - x * w;
+        spec[((size_t)x * fy + y) * fc + k0 + l] = res2[l * n + x];
+    }
+    if (NEED_EV)
+    {
+        __shared__ double red[8][4];
+        double vals[7] = {e, v0, v1, v2, v3, v4, v5};
+        int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int c = 0; c < 7; c++)
+        {
+            double v = vals[c];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) red[c][wid] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 7)
+        {
+            double s = 0;
+            for (int q = 0; q < (int)(blockDim.x >> 6); q++) s += red[threadIdx.x][q];
+            partials[(size_t)blockIdx.x * 8 + threadIdx.x] = s;
+        }
+    }
+}
+
+// Fixed-order reduction of the per-block partials into dscal[0..6] and the
+// self + neutralising energy into dscal[7] (PM_force.cpp:1146-1159).
+__global__ void k_pme_finalize(const double* __restrict__ partials, int nblocks,
+                               double* __restrict__ dscal, const int64_t* __restrict__ scal,
+                               double beta, double nfactor, int want_self)
+{
+    int c = threadIdx.x;
+    if (c < 7)
+    {
+        double s = 0;
+        for (int k = 0; k < nblocks; k++) s += partials[(size_t)k * 8 + c];
+        dscal[c] = s;
+    }
+    if (c == 7 && want_self)
+    {
+        double q2 = snb_fixed_to_double(scal[SNB_S_Q2]);
+        double q1 = snb_fixed_to_double(scal[SNB_S_Q]);
+        dscal[7] = -beta / sqrt(3.14159265358979323846) * q2 + nfactor * q1 * q1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// atoms -> bricks, spread, gather
+// ---------------------------------------------------------------------------
+// PME_Atom_Near (PM_force.cpp:706-737) in the reference's float operations.
+__device__ __forceinline__ void atom_near(float4 r, const float* rc, int fx, int fy, int fz,
+                                          int& ux, int& uy, int& uz, float& frx, float& fry,
+                                          float& frz)
+{
+    float f0 = (r.x * rc[0] + r.y * rc[1]) + r.z * rc[3];
+    float f1 = r.y * rc[2] + r.z * rc[4];
+    float f2 = r.z * rc[5];
+    f0 = f0 - floorf(f0);
+    f1 = f1 - floorf(f1);
+    f2 = f2 - floorf(f2);
+    if (!(__builtin_isfinite(f0) && __builtin_isfinite(f1) && __builtin_isfinite(f2)))
+        f0 = f1 = f2 = 0.0f;
+    float s;
+    float q;
+    s = f0 * (float)fx;
+    ux = (int)s;
+    ux = ux < 0 ? 0 : (ux < fx ? ux : fx - 1);
+    q = s - (float)ux;
+    frx = q - floorf(q);
+    s = f1 * (float)fy;
+    uy = (int)s;
+    uy = uy < 0 ? 0 : (uy < fy ? uy : fy - 1);
+    q = s - (float)uy;
+    fry = q - floorf(q);
+    s = f2 * (float)fz;
+    uz = (int)s;
+    uz = uz < 0 ? 0 : (uz < fz ? uz : fz - 1);
+    q = s - (float)uz;
+    frz = q - floorf(q);
+}
+
+struct SnbRc
+{
+    float v[6];
+};
+
+__global__ void k_pme_prep(int n, const float4* __restrict__ atoms, SnbRc rc, int fx, int fy,
+                           int fz, int bx, int by, int bz, int nby, int nbz,
+                           int4* __restrict__ atom_u, float4* __restrict__ atom_fr,
+                           int* __restrict__ slot, int* __restrict__ cnt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 r = atoms[i];
+    int ux, uy, uz;
+    float a, b, c;
+    atom_near(r, rc.v, fx, fy, fz, ux, uy, uz, a, b, c);
+    int br = ((ux / bx) * nby + uy / by) * nbz + uz / bz;
+    atom_u[i] = make_int4(ux, uy, uz, br);
+    atom_fr[i] = make_float4(a, b, c, r.w);
+    slot[i] = atomicAdd(&cnt[br], 1);
+}
+
+__global__ void k_pme_scatter(int n, const int4* __restrict__ atom_u, const int* __restrict__ slot,
+                              const int* __restrict__ start, int* __restrict__ brick_atoms)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    brick_atoms[start[atom_u[i].w] + slot[i]] = i;
+}
+
+__device__ __forceinline__ int wrap_axis(int v, int n) { return v < 0 ? v + n : v; }
+
+// One workgroup per brick of bx*by*bz grid points.  An atom with base point u
+// touches points u-3..u on each axis, so a brick receives from atoms whose
+// base lies in the brick itself or in the next brick on each axis (bricks are
+// >= 4 points wide): 8 source bricks.
+__global__ void __launch_bounds__(256) k_pme_spread(
+    const int4* __restrict__ atom_u, const float4* __restrict__ atom_fr,
+    const int* __restrict__ start, const int* __restrict__ brick_atoms, int fx, int fy, int fz,
+    int bx, int by, int bz, int nbx, int nby, int nbz, float* __restrict__ q)
+{
+    extern __shared__ long long acc[];
+    const int npt = bx * by * bz;
+    for (int t = threadIdx.x; t < npt; t += blockDim.x) acc[t] = 0;
+    __syncthreads();
+    const int B = blockIdx.x;
+    const int Bz = B % nbz, By = (B / nbz) % nby, Bx = B / (nbz * nby);
+    const int ox = Bx * bx, oy = By * by, oz = Bz * bz;
+    for (int s = 0; s < 8; s++)
+    {
+        int sx = s >> 2, sy = (s >> 1) & 1, sz = s & 1;
+        if ((sx && nbx == 1) || (sy && nby == 1) || (sz && nbz == 1)) continue;
+        int sb = (((Bx + sx) % nbx) * nby + (By + sy) % nby) * nbz + (Bz + sz) % nbz;
+        int a0 = start[sb], a1 = start[sb + 1];
+        for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x)
+        {
+            int i = brick_atoms[a];
+            int4 u = atom_u[i];
+            float4 fr = atom_fr[i];
+            float wx[4], wy[4], wz[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+            {
+                wx[k] = snb_bspline_w(k, fr.x);
+                wy[k] = snb_bspline_w(k, fr.y);
+                wz[k] = snb_bspline_w(k, fr.z);
+            }
+#pragma unroll
+            for (int kx = 0; kx < 4; kx++)
+            {
+                int lx = wrap_axis(u.x - kx, fx) - ox;
+                if ((unsigned)lx >= (unsigned)bx) continue;
+                float qx = fr.w * wx[kx];
+#pragma unroll
+                for (int ky = 0; ky < 4; ky++)
+                {
+                    int ly = wrap_axis(u.y - ky, fy) - oy;
+                    if ((unsigned)ly >= (unsigned)by) continue;
+                    float qxy = qx * wy[ky];
+#pragma unroll
+                    for (int kz = 0; kz < 4; kz++)
+                    {
+                        int lz = wrap_axis(u.z - kz, fz) - oz;
+                        if ((unsigned)lz >= (unsigned)bz) continue;
+                        atomicAdd((unsigned long long*)&acc[(lx * by + ly) * bz + lz],
+                                  (unsigned long long)snb_to_fixed(qxy * wz[kz]));
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < npt; t += blockDim.x)
+    {
+        int lz = t % bz, ly = (t / bz) % by, lx = t / (bz * by);
+        q[((size_t)(ox + lx) * fy + (oy + ly)) * fz + (oz + lz)] = snb_fixed_to_float(acc[t]);
+    }
+}
+
+// PME_Final + device_add_force (PM_force.cpp:826-890, :1011-1019): one thread
+// per atom, stencil order kx, ky, kz (outer to inner), s = fma(phi, g, s).
+__global__ void __launch_bounds__(256) k_pme_gather(int n, const int4* __restrict__ atom_u,
+                                                    const float4* __restrict__ atom_fr,
+                                                    const float* __restrict__ phi, int fx,
+                                                    int fy, int fz, SnbRc rc, float upd,
+                                                    int64_t* __restrict__ frc)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int4 u = atom_u[i];
+    float4 fr = atom_fr[i];
+    float wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+    {
+        wx[k] = snb_bspline_w(k, fr.x);
+        wy[k] = snb_bspline_w(k, fr.y);
+        wz[k] = snb_bspline_w(k, fr.z);
+        dx[k] = snb_bspline_dw(k, fr.x);
+        dy[k] = snb_bspline_dw(k, fr.y);
+        dz[k] = snb_bspline_dw(k, fr.z);
+    }
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll
+    for (int kx = 0; kx < 4; kx++)
+    {
+        int gx = wrap_axis(u.x - kx, fx);
+#pragma unroll
+        for (int ky = 0; ky < 4; ky++)
+        {
+            int gy = wrap_axis(u.y - ky, fy);
+            const float* rowp = phi + ((size_t)gx * fy + gy) * fz;
+#pragma unroll
+            for (int kz = 0; kz < 4; kz++)
+            {
+                int gz = wrap_axis(u.z - kz, fz);
+                float ph = rowp[gz];
+                sx = __builtin_fmaf(ph, (dx[kx] * wy[ky]) * wz[kz], sx);
+                sy = __builtin_fmaf(ph, (wx[kx] * dy[ky]) * wz[kz], sy);
+                sz = __builtin_fmaf(ph, (wx[kx] * wy[ky]) * dz[kz], sz);
+            }
+        }
+    }
+    float tx = sx * (float)fx, ty = sy * (float)fy, tz = sz * (float)fz;
+    float mq = -fr.w * upd;
+    float gx = tx * rc.v[0];
+    float gy = tx * rc.v[1] + ty * rc.v[2];
+    float gz = (tx * rc.v[3] + ty * rc.v[4]) + tz * rc.v[5];
+    frc[i] += snb_to_fixed(mq * gx);
+    frc[n + i] += snb_to_fixed(mq * gy);
+    frc[2 * n + i] += snb_to_fixed(mq * gz);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static double bspline_modulus(int m, int K)
+{
+    // PM_force.cpp:27-62, order 4: |sum_{k=0}^{2} M4(k+1) e^{2 pi i m k/K}|^-2
+    const double M4[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
+    double re = 0, im = 0;
+    for (int k = 0; k < 3; k++)
+    {
+        double a = 2.0 * M_PI * m * k / K;
+        re += M4[k] * cos(a);
+        im += M4[k] * sin(a);
+    }
+    return 1.0 / (re * re + im * im);
+}
+
+static int brick_dim(int n)
+{
+    // a divisor of n in [4, 16], nearest to 8; the whole axis if n <= 16
+    if (n <= 16) return n;
+    int best = 0;
+    for (int d = 4; d <= 16; d++)
+        if (n % d == 0 && (best == 0 || abs(d - 8) < abs(best - 8))) best = d;
+    return best;
+}
+
+static int upload_twiddles(int n, float2** dst)
+{
+    std::vector<float2> h(n);
+    for (int k = 0; k < n; k++)
+    {
+        double a = -2.0 * M_PI * k / n;
+        h[k] = make_float2((float)cos(a), (float)sin(a));
+    }
+    SNB_CHECK(hipMalloc(dst, sizeof(float2) * n));
+    SNB_CHECK(hipMemcpy(*dst, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    return SNB_OK;
+}
+
+// lines per workgroup for the y and x passes (a chunk of consecutive kz)
+static int fft_w(int fc, int n)
+{
+    int w = lines_per_block(n);
+    if (w > 16) w = 16;
+    return fc < w ? fc : w;
+}
+
+int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box)
+{
+    SnbPme& P = ctx->pme;
+    int rc;
+    if ((rc = snb_fft_make_plan(P.fx, &P.plan_x))) return rc;
+    if ((rc = snb_fft_make_plan(P.fy, &P.plan_y))) return rc;
+    if ((rc = snb_fft_make_plan(P.fz, &P.plan_z))) return rc;
+    P.fc = P.fz / 2 + 1;
+    P.bx = brick_dim(P.fx);
+    P.by = brick_dim(P.fy);
+    P.bz = brick_dim(P.fz);
+    if (!P.bx || !P.by || !P.bz) return SNB_ERR_UNSUPPORTED;
+    if ((P.bx < 4 && P.bx != P.fx) || (P.by < 4 && P.by != P.fy) || (P.bz < 4 && P.bz != P.fz))
+        return SNB_ERR_UNSUPPORTED;
+    P.nbx = P.fx / P.bx;
+    P.nby = P.fy / P.by;
+    P.nbz = P.fz / P.bz;
+    P.nbrick = P.nbx * P.nby * P.nbz;
+    for (int c = 0; c < 6; c++) P.rcell[c] = box.rc[c];
+    size_t nall = (size_t)P.fx * P.fy * P.fz;
+    size_t nfft = (size_t)P.fx * P.fy * P.fc;
+    int n = ctx->n;
+    if ((rc = upload_twiddles(P.fx, &P.tw_x))) return rc;
+    if ((rc = upload_twiddles(P.fy, &P.tw_y))) return rc;
+    if ((rc = upload_twiddles(P.fz, &P.tw_z))) return rc;
+    SNB_CHECK(hipMalloc(&P.bc, sizeof(float) * nfft));
+    SNB_CHECK(hipMalloc(&P.q, sizeof(float) * nall));
+    SNB_CHECK(hipMalloc(&P.phi, sizeof(float) * nall));
+    SNB_CHECK(hipMalloc(&P.spec, sizeof(float2) * nfft));
+    SNB_CHECK(hipMalloc(&P.atom_u, sizeof(int4) * (n > 0 ? n : 1)));
+    SNB_CHECK(hipMalloc(&P.atom_fr, sizeof(float4) * (n > 0 ? n : 1)));
+    SNB_CHECK(hipMalloc(&P.atom_slot, sizeof(int) * (n > 0 ? n : 1)));
+    SNB_CHECK(hipMalloc(&P.brick_atoms, sizeof(int) * (n > 0 ? n : 1)));
+    SNB_CHECK(hipMalloc(&P.brick_count, sizeof(int) * (P.nbrick + 1)));
+    SNB_CHECK(hipMalloc(&P.brick_start, sizeof(int) * (P.nbrick + 1)));
+    int W = fft_w(P.fc, P.fx);
+    P.conv_blocks = P.fy * ((P.fc + W - 1) / W);
+    SNB_CHECK(hipMalloc(&P.partials, sizeof(double) * 8 * P.conv_blocks));
+    // influence function BC (PM_force.cpp:544-612) in double, stored as float
+    std::vector<double> b1(P.fx), b2(P.fy), b3(P.fz);
+    for (int k = 0; k < P.fx; k++) b1[k] = bspline_modulus(k, P.fx);
+    for (int k = 0; k < P.fy; k++) b2[k] = bspline_modulus(k, P.fy);
+    for (int k = 0; k < P.fz; k++) b3[k] = bspline_modulus(k, P.fz);
+    std::vector<float> h(nfft);
+    double beta = P.beta;
+    double mpre = M_PI * M_PI / (beta * beta);
+    for (int kx = 0; kx < P.fx; kx++)
+    {
+        double kxr = kx > P.fx / 2 ? kx - P.fx : kx;
+        for (int ky = 0; ky < P.fy; ky++)
+        {
+            double kyr = ky > P.fy / 2 ? ky - P.fy : ky;
+            for (int kz = 0; kz < P.fc; kz++)
+            {
+                double mx = kxr * box.rc[0];
+                double my = kxr * box.rc[1] + kyr * box.rc[2];
+                double mz = kxr * box.rc[3] + kyr * box.rc[4] + kz * box.rc[5];
+                double msq = mx * mx + my * my + mz * mz;
+                size_t idx = ((size_t)kx * P.fy + ky) * P.fc + kz;
+                h[idx] = (kx + ky + kz == 0)
+                             ? 0.0f
+                             : (float)(exp(-mpre * msq) / (M_PI * msq * P.volume) * b1[kx] *
+                                       b2[ky] * b3[kz]);
+            }
+        }
+    }
+    SNB_CHECK(hipMemcpy(P.bc, h.data(), sizeof(float) * nfft, hipMemcpyHostToDevice));
+    return SNB_OK;
+}
+
+static size_t lds_lines(int n, int L) { return sizeof(float2) * ((size_t)n + 2 * (size_t)L * n); }
+
+int snb_launch_fft_r2c(snb_context* ctx, const float* real, float2* cplx)
+{
+    SnbPme& P = ctx->pme;
+    hipStream_t s = ctx->stream;
+    int nrow = P.fx * P.fy;
+    int Lz = lines_per_block(P.fz);
+    hipLaunchKernelGGL(k_fft_z_r2c, dim3((nrow + Lz - 1) / Lz), dim3(256), lds_lines(P.fz, Lz),
+                       s, real, cplx, nrow, P.plan_z, P.fc, P.tw_z, Lz);
+    int W = fft_w(P.fc, P.fy);
+    int nchunk = (P.fc + W - 1) / W;
+    hipLaunchKernelGGL(k_fft_y, dim3(P.fx * nchunk), dim3(256), lds_lines(P.fy, W), s, cplx,
+                       P.fx, P.fy, P.fc, P.plan_y, P.tw_y, W, -1.0f);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
+// x-direction plain transform (no convolution), used by the FFT wrappers.
+__global__ void __launch_bounds__(256) k_fft_x(float2* __restrict__ spec, int fx, int fy, int fc,
+                                               SnbFftPlan plan, const float2* __restrict__ twg,
+                                               int W, float s)
+{
+    extern __shared__ float2 smem[];
+    int n = plan.n;
+    float2* tw = smem;
+    float2* a = smem + n;
+    float2* b = a + W * n;
+    load_twiddles(tw, twg, n);
+    int nchunk = (fc + W - 1) / W;
+    int y = blockIdx.x / nchunk;
+    int k0 = (blockIdx.x - y * nchunk) * W;
+    int w = min(W, fc - k0);
+    for (int t = threadIdx.x; t < n * w; t += blockDim.x)
+    {
+        int x = t / w, l = t - x * w;
+        a[l * n + x] = spec[((size_t)x * fy + y) * fc + k0 + l];
+    }
+    __syncthreads();
+    float2* res = fft_lines(a, b, plan, w, tw, s);
+    for (int t = threadIdx.x; t < n * w; t += blockDim.x)
+    {
+        int x = t / w, l = t - x * w;
+        spec[((size_t)x * fy + y) * fc + k0 + l] = res[l * n + x];
+    }
+}
+
+static int fft_x_plain(snb_context* ctx, float2* cplx, float sign)
+{
+    SnbPme& P = ctx->pme;
+    int W = fft_w(P.fc, P.fx);
+    int nchunk = (P.fc + W - 1) / W;
+    hipLaunchKernelGGL(k_fft_x, dim3(P.fy * nchunk), dim3(256), lds_lines(P.fx, W), ctx->stream,
+                       cplx, P.fx, P.fy, P.fc, P.plan_x, P.tw_x, W, sign);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
+int snb_launch_fft_c2r(snb_context* ctx, const float2* cplx_in, float* real)
+{
+    SnbPme& P = ctx->pme;
+    hipStream_t s = ctx->stream;
+    size_t nfft = (size_t)P.fx * P.fy * P.fc;
+    SNB_CHECK(hipMemcpyAsync(P.spec, cplx_in, sizeof(float2) * nfft, hipMemcpyDeviceToDevice, s));
+    int rc = fft_x_plain(ctx, P.spec, 1.0f);
+    if (rc) return rc;
+    int W = fft_w(P.fc, P.fy);
+    int nchunk = (P.fc + W - 1) / W;
+    hipLaunchKernelGGL(k_fft_y, dim3(P.fx * nchunk), dim3(256), lds_lines(P.fy, W), s, P.spec,
+                       P.fx, P.fy, P.fc, P.plan_y, P.tw_y, W, 1.0f);
+    int nrow = P.fx * P.fy;
+    int Lz = lines_per_block(P.fz);
+    hipLaunchKernelGGL(k_fft_z_c2r, dim3((nrow + Lz - 1) / Lz), dim3(256), lds_lines(P.fz, Lz),
+                       s, P.spec, real, nrow, P.plan_z, P.fc, P.tw_z, Lz);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
+int snb_fft_forward_full(snb_context* ctx, const float* real, float2* cplx)
+{
+    int rc = snb_launch_fft_r2c(ctx, real, cplx);
+    if (rc) return rc;
+    return fft_x_plain(ctx, cplx, -1.0f);
+}
+
+int snb_launch_pme_recip(snb_context* ctx, const SnbBox& box, int need_energy, int need_virial,
+                         int with_force)
+{
+    SnbPme& P = ctx->pme;
+    hipStream_t s = ctx->stream;
+    int n = ctx->n;
+    int rc;
+    SnbRc r6;
+    for (int c = 0; c < 6; c++) r6.v[c] = box.rc[c];
+    snb_prof_begin(ctx, SNB_PROF_PME);
+    if (n > 0)
+    {
+        if ((rc = snb_launch_fill_int(s, P.brick_count, P.nbrick + 1, 0))) return rc;
+        hipLaunchKernelGGL(k_pme_prep, dim3((n + 255) / 256), dim3(256), 0, s, n, ctx->atom4, r6,
+                           P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nby, P.nbz, P.atom_u,
+                           P.atom_fr, P.atom_slot, P.brick_count);
+        if ((rc = snb_launch_exclusive_scan(s, P.brick_count, P.brick_start, P.nbrick))) return rc;
+        hipLaunchKernelGGL(k_pme_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, P.atom_u,
+                           P.atom_slot, P.brick_start, P.brick_atoms);
+    }
+    else
+    {
+        if ((rc = snb_launch_fill_int(s, P.brick_start, P.nbrick + 1, 0))) return rc;
+    }
+    size_t lds = sizeof(long long) * (size_t)P.bx * P.by * P.bz;
+    hipLaunchKernelGGL(k_pme_spread, dim3(P.nbrick), dim3(256), lds, s, P.atom_u, P.atom_fr,
+                       P.brick_start, P.brick_atoms, P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nbx,
+                       P.nby, P.nbz, P.q);
+    SNB_CHECK(hipGetLastError());
+    if ((rc = snb_launch_fft_r2c(ctx, P.q, P.spec))) return rc;
+    int W = fft_w(P.fc, P.fx);
+    int nchunk = (P.fc + W - 1) / W;
+    double mpre = M_PI * M_PI / ((double)P.beta * P.beta);
+    bool ev = need_energy || need_virial;
+    if (ev)
+        hipLaunchKernelGGL((k_fft_x_conv<true>), dim3(P.fy * nchunk), dim3(256),
+                           lds_lines(P.fx, W), s, P.spec, P.bc, P.fx, P.fy, P.fz, P.fc, P.plan_x,
+                           P.tw_x, W, (double)box.rc[0], (double)box.rc[1], (double)box.rc[2],
+                           (double)box.rc[3], (double)box.rc[4], (double)box.rc[5], mpre,
+                           P.partials);
+    else
+        hipLaunchKernelGGL((k_fft_x_conv<false>), dim3(P.fy * nchunk), dim3(256),
+                           lds_lines(P.fx, W), s, P.spec, P.bc, P.fx, P.fy, P.fz, P.fc, P.plan_x,
+                           P.tw_x, W, (double)box.rc[0], (double)box.rc[1], (double)box.rc[2],
+                           (double)box.rc[3], (double)box.rc[4], (double)box.rc[5], mpre,
+                           P.partials);
+    SNB_CHECK(hipGetLastError());
+    int Wy = fft_w(P.fc, P.fy);
+    int nchunky = (P.fc + Wy - 1) / Wy;
+    hipLaunchKernelGGL(k_fft_y, dim3(P.fx * nchunky), dim3(256), lds_lines(P.fy, Wy), s, P.spec,
+                       P.fx, P.fy, P.fc, P.plan_y, P.tw_y, Wy, 1.0f);
+    int nrow = P.fx * P.fy;
+    int Lz = lines_per_block(P.fz);
+    hipLaunchKernelGGL(k_fft_z_c2r, dim3((nrow + Lz - 1) / Lz), dim3(256), lds_lines(P.fz, Lz),
+                       s, P.spec, P.phi, nrow, P.plan_z, P.fc, P.tw_z, Lz);
+    SNB_CHECK(hipGetLastError());
+    if (n > 0 && with_force)
+        hipLaunchKernelGGL(k_pme_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, P.atom_u,
+                           P.atom_fr, P.phi, P.fx, P.fy, P.fz, r6, (float)P.update_interval,
+                           ctx->fx);
+    snb_prof_end(ctx, SNB_PROF_PME);
+    SNB_CHECK(hipGetLastError());
+    if (ev)
+    {
+        double nfactor = -0.5 * M_PI / ((double)P.beta * P.beta * P.volume);
+        hipLaunchKernelGGL(k_pme_finalize, dim3(1), dim3(64), 0, s, P.partials, P.conv_blocks,
+                           ctx->dscal, ctx->scal, (double)P.beta, nfactor, need_energy ? 1 : 0);
+        SNB_CHECK(hipGetLastError());
+    }
+    return SNB_OK;
+}
+
+// d_potential[0] += E_rec + E_self (PME_Add_Energy_To_Potential, PM_force.cpp:1004-1009)
+__global__ void k_pme_add_potential(const double* __restrict__ dscal, float* pot)
+{
+    pot[0] += (float)(dscal[SNB_D_E_REC] + dscal[SNB_D_E_SELF]);
+}
+
+int snb_launch_pme_energy_to_potential(snb_context* ctx, float* potential)
+{
+    hipLaunchKernelGGL(k_pme_add_potential, dim3(1), dim3(1), 0, ctx->stream, ctx->dscal,
+                       potential);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
+__global__ void k_pme_add_virial(const double* __restrict__ dscal, snb_ltmatrix3* vir)
+{
+    vir[0].a11 += (float)dscal[SNB_D_VIR0 + 0];
+    vir[0].a21 += (float)dscal[SNB_D_VIR0 + 1];
+    vir[0].a22 += (float)dscal[SNB_D_VIR0 + 2];
+    vir[0].a31 += (float)dscal[SNB_D_VIR0 + 3];
+    vir[0].a32 += (float)dscal[SNB_D_VIR0 + 4];
+    vir[0].a33 += (float)dscal[SNB_D_VIR0 + 5];
+}
+
+int snb_launch_pme_virial_to(snb_context* ctx, snb_ltmatrix3* vir)
+{
+    hipLaunchKernelGGL(k_pme_add_virial, dim3(1), dim3(1), 0, ctx->stream, ctx->dscal, vir);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}

@@ -1,0 +1,337 @@
+"""ctypes binding of libsponge_nb.so (C ABI: include/sponge_nb.h) plus a
+small host driver of the nonbonded path on torch-allocated device memory.
+
+torch is used only for device allocation and the stream handle; all compute
+runs in the library's HIP kernels.  If the library is missing, loading fails
+with an error -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.abspath(os.path.join(_HERE, "..", ".."))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsponge_nb.so")
+HEADER_PATH = os.path.abspath(os.path.join(PKG_ROOT, "..", "include", "sponge_nb.h"))
+FIXED_SCALE = 2.0 ** 32
+
+SNB_OK = 0
+STATUS_NAMES = {1: "invalid argument", 2: "HIP runtime error", 3: "box too small",
+                4: "unsupported configuration", 5: "module not initialised",
+                6: "device allocation failed", 7: "device status flag set"}
+FLAG_NL_OVERFLOW, FLAG_CELL_OVERFLOW, FLAG_NONFINITE_CRD = 1, 2, 4
+
+
+class SnbError(RuntimeError):
+    def __init__(self, code, where):
+        super().__init__("%s failed: %s (%d)" % (where, STATUS_NAMES.get(code, "?"), code))
+        self.code = code
+
+
+class LTMatrix3(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("a11", "a21", "a22", "a31", "a32", "a33")]
+
+    @classmethod
+    def of(cls, a6):
+        return cls(*[float(x) for x in np.asarray(a6, np.float32)])
+
+
+class Vector(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float), ("z", ctypes.c_float)]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_SIGS = {
+    "snb_abi_version": ([], _I),
+    "snb_status_string": ([_I], ctypes.c_char_p),
+    "snb_context_create": ([ctypes.POINTER(_P), _I, _P, _I], _I),
+    "snb_context_destroy": ([_P], _I),
+    "snb_set_stream": ([_P, _P], _I),
+    "snb_synchronize": ([_P], _I),
+    "snb_read_status": ([_P, _I, ctypes.POINTER(_I)], _I),
+    "snb_set_exclusions": ([_P, _P, _P, _P, _I], _I),
+    "snb_nl_initial": ([_P, _F, _F, LTMatrix3, LTMatrix3, _I, _I, _I, _F], _I),
+    "snb_nl_update": ([_P, _P, _I, _I, _P, LTMatrix3, LTMatrix3, _I, _I, _P, _P, _P], _I),
+    "snb_nl_device_list": ([_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_I)], _I),
+    "snb_nl_copy_rows": ([_P, _P, _P], _I),
+    "snb_nl_grid_shape": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
+    "snb_lj_initial": ([_P, _I, _P, _P, _P, _F], _I),
+    "snb_lj_pme_direct_force": ([_P, _I, _I, _I, _I, _P, _P, _P, LTMatrix3, LTMatrix3, _P, _F,
+                                 _I, _P, _I, _P, _P], _I),
+    "snb_lj_long_range_correction": ([_P, _I, _P, _I, _P, _F], _I),
+    "snb_lj_energy": ([_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], _I),
+    "snb_pme_initial": ([_P, LTMatrix3, LTMatrix3, Vector, _F, _F, _I, _I, _I, _F, _I], _I),
+    "snb_pme_parameters": ([_P, ctypes.POINTER(_F), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                            ctypes.POINTER(_I)], _I),
+    "snb_pme_excluded_force": ([_P, _I, _P, _P, _P, LTMatrix3, LTMatrix3, _P, _P, _P, _P, _P,
+                                _I, _P, _I, _P], _I),
+    "snb_pme_reciprocal_force": ([_P, _P, LTMatrix3, LTMatrix3, _P, _P, _I, _I, _P, _P, _I], _I),
+    "snb_pme_energy": ([_P] + [ctypes.POINTER(ctypes.c_double)] * 4, _I),
+    "snb_pme_reciprocal_virial": ([_P, _P], _I),
+    "snb_flush_forces": ([_P, _P], _I),
+    "snb_copy_fixed_forces": ([_P, _P], _I),
+    "snb_clear_fixed_forces": ([_P], _I),
+    "snb_copy_fixed_atom_terms": ([_P, _I, _P], _I),
+    "snb_copy_pme_grid": ([_P, _I, _P], _I),
+    "snb_fft_r2c": ([_P, _P, _P], _I),
+    "snb_fft_c2r": ([_P, _P, _P], _I),
+    "snb_profile_enable": ([_P, _I], _I),
+    "snb_profile_read": ([_P, _I, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)], _I),
+}
+PROF_PAIR, PROF_NL, PROF_PME, PROF_EXCL = 0, 1, 2, 3
+
+_lib = None
+
+
+def load():
+    """Loads the in-tree library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        # torch's HIP runtime (SONAME libamdhip64.so.7) must be the one the
+        # library binds to: import torch before dlopen so there is one runtime.
+        import torch  # noqa: F401
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libsponge_nb.so is not built (%s); run "
+                               "__graft_entry__.build() or make -C sponge2.0_amd" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc, where):
+    if rc != SNB_OK:
+        raise SnbError(rc, where)
+
+
+def header_symbols():
+    """Names of the functions declared in include/sponge_nb.h."""
+    import re
+    txt = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(snb_\w+)\s*\(", txt, re.M)))
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Nonbonded:
+    """One system on one GPU: the three reference modules behind one context.
+
+    sysd is a system dict (sponge_amd.systems / native_io.system_from_prefix).
+    Device arrays are torch tensors on `device`; the caller-owned buffers of
+    the reference (crd, charge, frc, per-atom energy and virial) are held here
+    so tests and the bench can drive the path like the reference's main loop.
+    """
+
+    def __init__(self, sysd, cutoff=None, skin=2.0, tolerance=1e-5, grid=None,
+                 max_neighbor_numbers=1200, max_atom_in_grid_numbers=150,
+                 refresh_interval=0, skin_permit=0.5, update_interval=1, device=0,
+                 stream=None):
+        import torch
+
+        self.torch = torch
+        L = load()
+        self.L = L
+        self.n = int(sysd["n"])
+        self.dev = torch.device("cuda", device)
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.dev)
+        self.cutoff = float(sysd.get("cutoff", 10.0) if cutoff is None else cutoff)
+        self.cell = LTMatrix3.of(sysd["cell"])
+        self.rcell = LTMatrix3.of(sysd["rcell"])
+        self.box = np.asarray(sysd["box"], np.float32)
+        ctx = ctypes.c_void_p()
+        check(L.snb_context_create(ctypes.byref(ctx), device,
+                                   ctypes.c_void_p(self.stream.cuda_stream), self.n),
+              "snb_context_create")
+        self.ctx = ctx
+        es = np.ascontiguousarray(sysd["excl_start"], np.int32)
+        en = np.ascontiguousarray(sysd["excl_numbers"], np.int32)
+        el = np.ascontiguousarray(sysd["excl_list"], np.int32)
+        if el.size == 0:
+            el = np.zeros(1, np.int32)
+        check(L.snb_set_exclusions(ctx, es.ctypes.data, el.ctypes.data, en.ctypes.data,
+                                   int(en.sum())), "snb_set_exclusions")
+        check(L.snb_nl_initial(ctx, self.cutoff, skin, self.cell, self.rcell,
+                               max_neighbor_numbers, max_atom_in_grid_numbers,
+                               refresh_interval, skin_permit), "snb_nl_initial")
+        a = np.ascontiguousarray(sysd["lj_a"], np.float32)
+        b = np.ascontiguousarray(sysd["lj_b"], np.float32)
+        t = np.ascontiguousarray(sysd["lj_type"], np.int32)
+        check(L.snb_lj_initial(ctx, int(sysd["lj_ntypes"]), a.ctypes.data, b.ctypes.data,
+                               t.ctypes.data, self.cutoff), "snb_lj_initial")
+        g = grid if grid is not None else sysd.get("pme_grid", (0, 0, 0))
+        check(L.snb_pme_initial(ctx, self.cell, self.rcell, Vector(*[float(x) for x in self.box]),
+                                self.cutoff, tolerance, int(g[0]), int(g[1]), int(g[2]), 1.0,
+                                update_interval), "snb_pme_initial")
+        beta = ctypes.c_float()
+        fx, fy, fz = _I(), _I(), _I()
+        check(L.snb_pme_parameters(ctx, ctypes.byref(beta), ctypes.byref(fx), ctypes.byref(fy),
+                                   ctypes.byref(fz)), "snb_pme_parameters")
+        self.beta = beta.value
+        self.grid = (fx.value, fy.value, fz.value)
+        self.volume = float(np.float32(sysd["cell"][0]) * np.float32(sysd["cell"][2]) *
+                            np.float32(sysd["cell"][5]))
+        nl = ctypes.c_void_p()
+        ser = ctypes.c_void_p()
+        mx = _I()
+        check(L.snb_nl_device_list(ctx, ctypes.byref(nl), ctypes.byref(ser), ctypes.byref(mx)),
+              "snb_nl_device_list")
+        self.d_nl, self.d_serial, self.max_nb = nl, ser, mx.value
+        n = self.n
+        f32 = torch.float32
+        self.crd = torch.as_tensor(np.asarray(sysd["crd"], np.float32)).to(self.dev).contiguous()
+        self.charge = torch.as_tensor(np.asarray(sysd["charge"], np.float32)).to(self.dev)
+        self.frc = torch.zeros((n, 3), dtype=f32, device=self.dev)
+        self.atom_energy = torch.zeros(n, dtype=f32, device=self.dev)
+        self.direct_energy = torch.zeros(n, dtype=f32, device=self.dev)
+        self.atom_virial = torch.zeros((n, 6), dtype=f32, device=self.dev)
+        self.virial = torch.zeros(6, dtype=f32, device=self.dev)   # d_virial[0]
+        self.potential = torch.zeros(1, dtype=f32, device=self.dev)
+
+    def close(self):
+        if self.ctx is not None:
+            self.L.snb_context_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reference module calls --------------------------------------------
+    def nl_update(self, step=0, forced=True):
+        check(self.L.snb_nl_update(self.ctx, None, self.n, 0, _ptr(self.crd), self.cell,
+                                   self.rcell, step, 1 if forced else 0, None, None, None),
+              "snb_nl_update")
+
+    def lj_direct(self, need_energy=False, need_virial=False, frc=True):
+        check(self.L.snb_lj_pme_direct_force(
+            self.ctx, self.n, self.n, 0, 0, _ptr(self.crd), _ptr(self.charge),
+            _ptr(self.frc) if frc else None, self.cell, self.rcell, self.d_nl, self.beta,
+            1 if need_energy else 0, _ptr(self.atom_energy) if frc else None,
+            1 if need_virial else 0, _ptr(self.atom_virial) if frc else None,
+            _ptr(self.direct_energy) if frc else None), "snb_lj_pme_direct_force")
+
+    def excluded(self, need_energy=False, need_virial=False, frc=True):
+        check(self.L.snb_pme_excluded_force(
+            self.ctx, self.n, None, None, _ptr(self.crd), self.cell, self.rcell,
+            _ptr(self.charge), None, None, None, _ptr(self.frc) if frc else None,
+            1 if need_energy else 0, _ptr(self.atom_energy) if frc else None,
+            1 if need_virial else 0, _ptr(self.atom_virial) if frc else None),
+            "snb_pme_excluded_force")
+
+    def reciprocal(self, step=0, need_energy=False, need_virial=False, frc=True):
+        check(self.L.snb_pme_reciprocal_force(
+            self.ctx, _ptr(self.crd), self.cell, self.rcell, _ptr(self.charge),
+            _ptr(self.frc) if frc else None, 1 if need_virial else 0, 1 if need_energy else 0,
+            _ptr(self.virial), _ptr(self.potential), step), "snb_pme_reciprocal_force")
+
+    def long_range(self, need_energy=False, need_virial=False):
+        check(self.L.snb_lj_long_range_correction(
+            self.ctx, 1 if need_virial else 0, _ptr(self.virial), 1 if need_energy else 0,
+            _ptr(self.potential), self.volume), "snb_lj_long_range_correction")
+
+    def flush(self):
+        check(self.L.snb_flush_forces(self.ctx, _ptr(self.frc)), "snb_flush_forces")
+
+    def zero_outputs(self):
+        self.frc.zero_()
+        self.atom_energy.zero_()
+        self.atom_virial.zero_()
+        self.virial.zero_()
+        self.potential.zero_()
+
+    def step(self, step=0, need_energy=False, need_virial=False, forced_nl=False, fused=True):
+        """One force evaluation of the path in the reference's order
+        (main.cpp:307-334, :371-396, :474-478).  fused=True keeps every term
+        in the fixed-point accumulators and converts once at the end."""
+        check(self.L.snb_nl_update(self.ctx, None, self.n, 0, _ptr(self.crd), self.cell,
+                                   self.rcell, step, 1 if forced_nl else 0, None, None, None),
+              "snb_nl_update")
+        self.excluded(need_energy, need_virial, frc=not fused)
+        self.lj_direct(need_energy, need_virial, frc=not fused)
+        if need_energy or need_virial:
+            self.long_range(need_energy, need_virial)
+        self.reciprocal(step, need_energy, need_virial, frc=not fused)
+        if fused:
+            self.flush()
+
+    # -- read-back helpers ----------------------------------------------------
+    def sync(self):
+        check(self.L.snb_synchronize(self.ctx), "snb_synchronize")
+
+    def status(self, clear=False):
+        f = _I()
+        check(self.L.snb_read_status(self.ctx, 1 if clear else 0, ctypes.byref(f)),
+              "snb_read_status")
+        return f.value
+
+    def fixed_forces(self):
+        t = self.torch.empty(3 * self.n, dtype=self.torch.int64, device=self.dev)
+        check(self.L.snb_copy_fixed_forces(self.ctx, _ptr(t)), "snb_copy_fixed_forces")
+        self.sync()
+        return t.view(3, self.n).cpu().numpy()
+
+    def clear_fixed_forces(self):
+        check(self.L.snb_clear_fixed_forces(self.ctx), "snb_clear_fixed_forces")
+
+    def fixed_atom_terms(self, which):
+        cnt = 6 * self.n if which == 3 else self.n
+        t = self.torch.empty(cnt, dtype=self.torch.int64, device=self.dev)
+        check(self.L.snb_copy_fixed_atom_terms(self.ctx, which, _ptr(t)),
+              "snb_copy_fixed_atom_terms")
+        self.sync()
+        a = t.cpu().numpy()
+        return a.reshape(6, self.n) if which == 3 else a
+
+    def pme_grid(self, which):
+        t = self.torch.empty(self.grid, dtype=self.torch.float32, device=self.dev)
+        check(self.L.snb_copy_pme_grid(self.ctx, which, _ptr(t)), "snb_copy_pme_grid")
+        self.sync()
+        return t.cpu().numpy()
+
+    def nl_rows(self):
+        """(counts int32[n], rows int32[n, max_nb]) of the public half list."""
+        torch = self.torch
+        counts = torch.empty(self.n, dtype=torch.int32, device=self.dev)
+        rows = torch.empty((self.n, self.max_nb), dtype=torch.int32, device=self.dev)
+        check(self.L.snb_nl_copy_rows(self.ctx, _ptr(counts), _ptr(rows)), "snb_nl_copy_rows")
+        self.sync()
+        return counts.cpu().numpy(), rows.cpu().numpy()
+
+    def profile(self, enable=True):
+        check(self.L.snb_profile_enable(self.ctx, 1 if enable else 0), "snb_profile_enable")
+
+    def profile_read(self, group, reset=True):
+        ms = ctypes.c_double()
+        cnt = _I()
+        check(self.L.snb_profile_read(self.ctx, group, 1 if reset else 0, ctypes.byref(ms),
+                                      ctypes.byref(cnt)), "snb_profile_read")
+        return ms.value, cnt.value
+
+    def energies(self):
+        d = [ctypes.c_double() for _ in range(4)]
+        check(self.L.snb_pme_energy(self.ctx, *[ctypes.byref(x) for x in d]), "snb_pme_energy")
+        ls, ll = ctypes.c_double(), ctypes.c_double()
+        check(self.L.snb_lj_energy(self.ctx, ctypes.byref(ls), ctypes.byref(ll)), "snb_lj_energy")
+        out = dict(LJ_short=ls.value, LJ_long=ll.value, PM_direct=d[0].value,
+                   PM_reciprocal=d[1].value, PM_self=d[2].value, PM_correction=d[3].value)
+        out["LJ"] = out["LJ_short"] + out["LJ_long"]
+        out["PM"] = out["PM_direct"] + out["PM_reciprocal"] + out["PM_self"] + out["PM_correction"]
+        return out
+
+    def reciprocal_virial(self):
+        v = (ctypes.c_double * 6)()
+        check(self.L.snb_pme_reciprocal_virial(self.ctx, ctypes.cast(v, ctypes.c_void_p)),
+              "snb_pme_reciprocal_virial")
+        return np.array(v[:])

@@ -1,0 +1,147 @@
+"""SPONGE native text formats: readers and writers.
+
+The formats are the ones Xponge's native loaders read
+(xponge/load/native/lj.hpp:8-69, xponge/load/native/md_core.hpp:8-284):
+
+  LJ        ``N T`` / A lower triangle (T(T+1)/2) / B lower triangle / type per atom.
+            The loader multiplies A by 12 and B by 6 (lj.hpp:47, :57).
+  charge    ``N`` / one charge per atom (already x18.2223).
+  mass      ``N`` / one mass per atom.
+  coordinate ``N [t0]`` / xyz per atom / ``Lx Ly Lz`` / ``alpha beta gamma``.
+  exclude   ``N total`` / per atom ``count j1 j2 ...``.
+  residue   ``N R`` / atoms per residue.
+
+``system_from_prefix`` assembles the arrays the nonbonded path needs, with the
+box converted to the reference's LTMatrix3 cell/rcell (MD_core/pbc.hpp:66-101).
+"""
+import os
+
+import numpy as np
+
+CHARGE_SCALE = 18.2223  # common.h:132
+
+
+def _tokens(path):
+    with open(path) as f:
+        return f.read().split()
+
+
+def read_lj(path):
+    t = _tokens(path)
+    n, nt = int(t[0]), int(t[1])
+    npair = nt * (nt + 1) // 2
+    a = np.array(t[2:2 + npair], dtype=np.float32) * np.float32(12.0)
+    b = np.array(t[2 + npair:2 + 2 * npair], dtype=np.float32) * np.float32(6.0)
+    types = np.array(t[2 + 2 * npair:2 + 2 * npair + n], dtype=np.int32)
+    return n, nt, a, b, types
+
+
+def read_scalar_list(path, dtype=np.float32):
+    t = _tokens(path)
+    n = int(t[0])
+    return np.array(t[1:1 + n], dtype=dtype)
+
+
+def read_coordinate(path):
+    with open(path) as f:
+        first = f.readline().split()
+        n = int(first[0])
+        rest = f.read().split()
+    xyz = np.array(rest[:3 * n], dtype=np.float32).reshape(n, 3)
+    box = np.array(rest[3 * n:3 * n + 3], dtype=np.float32)
+    ang = np.array(rest[3 * n + 3:3 * n + 6], dtype=np.float32)
+    return xyz, box, ang
+
+
+def read_exclude(path):
+    t = _tokens(path)
+    n, total = int(t[0]), int(t[1])
+    pos = 2
+    numbers = np.zeros(n, np.int32)
+    start = np.zeros(n, np.int32)
+    lst = []
+    for i in range(n):
+        c = int(t[pos])
+        pos += 1
+        start[i] = len(lst)
+        numbers[i] = c
+        lst.extend(int(x) for x in t[pos:pos + c])
+        pos += c
+    if len(lst) != total:
+        raise ValueError("exclude_in_file: total does not match the lists")
+    return start, numbers, np.array(lst, dtype=np.int32)
+
+
+def box_to_cell(box, angle=(90.0, 90.0, 90.0)):
+    """MD_core/pbc.hpp:66-101 in float32: returns (cell, rcell) LTMatrix3."""
+    f = np.float32
+    d2r = f(0.0174532925)
+    a, b, c = (f(x) for x in box)
+    al, be, ga = (f(x) * d2r for x in angle)
+    cal, cbe, cga = np.cos(al), np.cos(be), np.cos(ga)
+    sga = np.sin(ga)
+    za = np.sqrt(f(1) - cal * cal - cbe * cbe - cga * cga + f(2) * cal * cbe * cga)
+    cell = np.array([a, b * cga, b * sga, c * cbe, c / sga * (cal - cbe * cga),
+                     c / sga * za], dtype=np.float32)
+    rcell = np.zeros(6, np.float32)
+    rcell[0] = f(1) / cell[0]
+    rcell[2] = f(1) / cell[2]
+    rcell[5] = f(1) / cell[5]
+    rcell[1] = -rcell[0] / np.tan(ga)
+    rcell[3] = (cal / np.tan(ga) - cbe / sga) / za / a
+    rcell[4] = (cbe / np.tan(ga) - cal / sga) / za / b
+    for m in (cell, rcell):
+        for k in (1, 3, 4):
+            if abs(m[k]) < 1e-3:
+                m[k] = 0.0
+    return cell, rcell
+
+
+def system_from_prefix(directory, prefix):
+    """Loads ``<prefix>_{LJ,charge,coordinate,exclude}.txt`` (and mass when
+    present) into the dict the path uses."""
+    p = lambda name: os.path.join(directory, "%s_%s.txt" % (prefix, name))
+    n, nt, a, b, types = read_lj(p("LJ"))
+    crd, box, ang = read_coordinate(p("coordinate"))
+    charge = read_scalar_list(p("charge"))
+    es, en, el = read_exclude(p("exclude"))
+    cell, rcell = box_to_cell(box, ang)
+    d = dict(n=n, crd=crd, charge=charge, lj_ntypes=nt, lj_a=a, lj_b=b, lj_type=types,
+             box=box, angle=ang, cell=cell, rcell=rcell, excl_start=es,
+             excl_numbers=en, excl_list=el)
+    if os.path.exists(p("mass")):
+        d["mass"] = read_scalar_list(p("mass"))
+    return d
+
+
+# ---------------------------------------------------------------------------
+# writers (used by the synthetic-system generators)
+# ---------------------------------------------------------------------------
+def write_system(directory, prefix, sysd):
+    os.makedirs(directory, exist_ok=True)
+    p = lambda name: os.path.join(directory, "%s_%s.txt" % (prefix, name))
+    n = int(sysd["n"])
+    nt = int(sysd["lj_ntypes"])
+    with open(p("LJ"), "w") as f:
+        f.write("%d %d\n" % (n, nt))
+        np.savetxt(f, (sysd["lj_a"] / np.float32(12.0)).reshape(-1, 1), fmt="%.9e")
+        np.savetxt(f, (sysd["lj_b"] / np.float32(6.0)).reshape(-1, 1), fmt="%.9e")
+        np.savetxt(f, sysd["lj_type"].reshape(-1, 1), fmt="%d")
+    with open(p("charge"), "w") as f:
+        f.write("%d\n" % n)
+        np.savetxt(f, sysd["charge"].reshape(-1, 1), fmt="%.9e")
+    if "mass" in sysd:
+        with open(p("mass"), "w") as f:
+            f.write("%d\n" % n)
+            np.savetxt(f, sysd["mass"].reshape(-1, 1), fmt="%.6f")
+    with open(p("coordinate"), "w") as f:
+        f.write("%d\n" % n)
+        np.savetxt(f, sysd["crd"], fmt="%.7f")
+        f.write("%.7f %.7f %.7f\n" % tuple(sysd["box"]))
+        f.write("%.7f %.7f %.7f\n" % tuple(sysd.get("angle", (90.0, 90.0, 90.0))))
+    with open(p("exclude"), "w") as f:
+        es, en, el = sysd["excl_start"], sysd["excl_numbers"], sysd["excl_list"]
+        f.write("%d %d\n" % (n, int(en.sum())))
+        for i in range(n):
+            js = el[es[i]:es[i] + en[i]]
+            f.write(" ".join([str(int(en[i]))] + [str(int(j)) for j in js]) + "\n")

@@ -1,0 +1,305 @@
+"""GPU parity of the HIP path against the CPU oracle.
+
+Bit-exact: neighbor-list sets (vs the brute-force restatement of the
+reference's criterion) and every int64 fixed-point accumulator (vs the FP32
+twin given the same inputs).  Tolerance: forces 5e-5 relative RMS and
+energies/virials 1e-6 relative against the FP64 oracle; the reference's own
+GROMACS-comparison tolerances on the committed fixture.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, gromacs_case, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 5e-5        # per-atom force, relative RMS vs FP64
+ENERGY_TOL = 1e-6       # energy terms, relative vs FP64
+
+
+def _nb(sysd, **kw):
+    from sponge_amd.lib import Nonbonded
+    return Nonbonded(sysd, **kw)
+
+
+def _excl(sysd):
+    return (sysd["excl_start"], sysd["excl_numbers"], sysd["excl_list"])
+
+
+def _sorted_rows(counts, rows):
+    return [np.sort(rows[i, :counts[i]]) for i in range(len(counts))]
+
+
+SYSTEMS = ["tip3p_1011", "gromacs_iter0", "water_9k", "small_water"]
+
+
+def _get(request, name):
+    return request.getfixturevalue(name)
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_neighbor_list_sets_equal_oracle(request, name):
+    import oracle
+    s = _get(request, name)
+    nb = _nb(s)
+    print("case", name, "n", s["n"], flush=True)
+    nb.nl_update(forced=True)
+    gc, gr = nb.nl_rows()
+    assert nb.status() == 0
+    oc, orows, ov = oracle.nl_build(s["crd"], s["cell"], s["rcell"], nb.cutoff, 2.0, _excl(s),
+                                    max_nb=nb.max_nb)
+    assert not ov
+    np.testing.assert_array_equal(gc, oc)
+    for i, (a, b) in enumerate(zip(_sorted_rows(gc, gr), _sorted_rows(oc, orows))):
+        assert np.array_equal(a, b), "row %d differs" % i
+    nb.close()
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+@pytest.mark.parametrize("energy,virial", [(False, False), (True, True)])
+def test_pair_kernel_bit_exact_with_twin(request, name, energy, virial):
+    import oracle
+    s = _get(request, name)
+    nb = _nb(s)
+    print("case", name, energy, virial, flush=True)
+    nb.nl_update(forced=True)
+    counts, rows = nb.nl_rows()
+    nb.lj_direct(need_energy=energy, need_virial=virial, frc=False)
+    g_f = nb.fixed_forces()
+    t_f, t_elj, t_ec, t_v = oracle.pair_twin(s["crd"], s["charge"], s["lj_type"], s["lj_a"],
+                                             s["lj_b"], s["cell"], s["rcell"], nb.cutoff,
+                                             nb.beta, counts, rows, virial=True)
+    np.testing.assert_array_equal(g_f, t_f)
+    # every pair adds +f to i and -f to j: the fixed-point total is exactly 0
+    assert np.all(g_f.sum(axis=1) == 0)
+    if energy:
+        np.testing.assert_array_equal(nb.fixed_atom_terms(0), t_elj)
+        np.testing.assert_array_equal(nb.fixed_atom_terms(1), t_ec)
+    if virial:
+        np.testing.assert_array_equal(nb.fixed_atom_terms(3), t_v)
+    nb.close()
+
+
+@pytest.mark.parametrize("name", ["gromacs_iter0", "water_9k"])
+def test_exclusion_correction_bit_exact_with_twin(request, name):
+    import oracle
+    s = _get(request, name)
+    nb = _nb(s)
+    nb.excluded(need_energy=True, need_virial=True, frc=False)
+    g_f = nb.fixed_forces()
+    t_f, t_e, t_v = oracle.excl_twin(s["crd"], s["charge"], s["cell"], s["rcell"], nb.beta,
+                                     _excl(s))
+    np.testing.assert_array_equal(g_f, t_f)
+    np.testing.assert_array_equal(nb.fixed_atom_terms(2), t_e)
+    np.testing.assert_array_equal(nb.fixed_atom_terms(3), t_v)
+    nb.close()
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_pme_spread_and_gather_bit_exact_with_twin(request, name):
+    import oracle
+    s = _get(request, name)
+    nb = _nb(s)
+    nb.reciprocal(step=0, frc=False)
+    q_gpu = nb.pme_grid(0)
+    q_twin = oracle.pme_spread_twin(s["crd"], s["charge"], s["rcell"], nb.grid)
+    q_twin_f = (q_twin.astype(np.float64) * 2.0 ** -32).astype(np.float32)
+    np.testing.assert_array_equal(q_gpu.view(np.int32), q_twin_f.view(np.int32))
+    phi = nb.pme_grid(1)
+    g_f = nb.fixed_forces()
+    t_f = oracle.pme_gather_twin(s["crd"], s["charge"], s["rcell"], nb.grid, phi)
+    np.testing.assert_array_equal(g_f, t_f)
+    nb.close()
+
+
+@pytest.mark.parametrize("grid", [(48, 48, 48), (24, 24, 20), (28, 40, 36), (72, 72, 72),
+                                  (96, 96, 96), (30, 42, 56)])
+def test_fft_matches_float64_dft_and_rocfft(water_9k, grid):
+    """The library's radix-4/2/3/5/7 R2C/C2R against numpy's float64 FFT and,
+    as the independent cross-check, torch.fft (rocFFT)."""
+    import ctypes
+
+    import torch
+    from sponge_amd import lib
+    nb = _nb(water_9k, grid=grid)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(grid).astype(np.float32)
+    xd = torch.as_tensor(x).cuda()
+    fc = grid[2] // 2 + 1
+    spec = torch.empty(grid[0] * grid[1] * fc * 2, dtype=torch.float32, device="cuda")
+    lib.check(nb.L.snb_fft_r2c(nb.ctx, ctypes.c_void_p(xd.data_ptr()),
+                               ctypes.c_void_p(spec.data_ptr())), "snb_fft_r2c")
+    nb.sync()
+    got = spec.view(grid[0], grid[1], fc, 2).cpu().numpy()
+    got = got[..., 0] + 1j * got[..., 1]
+    ref = np.fft.rfftn(x.astype(np.float64))
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err < 2e-6, err
+    roc = torch.fft.rfftn(xd).cpu().numpy()
+    assert np.max(np.abs(got - roc)) / np.max(np.abs(ref)) < 4e-6
+    back = torch.empty(grid, dtype=torch.float32, device="cuda")
+    lib.check(nb.L.snb_fft_c2r(nb.ctx, ctypes.c_void_p(spec.data_ptr()),
+                               ctypes.c_void_p(back.data_ptr())), "snb_fft_c2r")
+    nb.sync()
+    nall = grid[0] * grid[1] * grid[2]
+    rt = back.cpu().numpy() / nall
+    assert np.max(np.abs(rt - x)) < 2e-5
+    nb.close()
+
+
+def _full_terms(nb, step=0):
+    nb.zero_outputs()
+    nb.step(step=step, need_energy=True, need_virial=True, forced_nl=True, fused=True)
+    nb.sync()
+    e = nb.energies()
+    return e, nb.frc.cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("name", ["tip3p_1011", "gromacs_iter0", "water_9k"])
+def test_forces_and_energies_vs_fp64_oracle(request, name):
+    import oracle
+    s = _get(request, name)
+    nb = _nb(s)
+    e, f = _full_terms(nb)
+    ref = oracle.nonbonded_terms_f64(s, cutoff=nb.cutoff, grid=nb.grid)
+    err = rel_rms(f, ref["frc"])
+    print("case", name, "force rel RMS", err, flush=True)
+    assert err < FORCE_TOL
+    assert abs(e["LJ_short"] - ref["LJ_short"]) <= ENERGY_TOL * abs(ref["LJ_short"])
+    # the PM components cancel heavily (self ~ -correction); each is held to
+    # 1e-6 of the printed PM total, and PM itself to 1e-6 relative
+    for k in ("PM_direct", "PM_correction", "PM_self", "PM_reciprocal"):
+        print(k, e[k], ref[k], (e[k] - ref[k]) / abs(ref[k]), flush=True)
+        assert abs(e[k] - ref[k]) <= ENERGY_TOL * abs(ref["PM"]), (k, e[k], ref[k])
+    assert abs(e["PM"] - ref["PM"]) <= ENERGY_TOL * abs(ref["PM"])
+    assert abs(e["LJ_long"] - ref["LJ_long"]) <= 1e-6 * abs(ref["LJ_long"]) + 1e-9
+    # reciprocal virial (reference PME_Sum_Virial convention)
+    v = nb.reciprocal_virial()
+    scale = np.max(np.abs(ref["virial_recip"]))
+    assert np.max(np.abs(v - ref["virial_recip"])) <= 1e-5 * scale
+    # per-atom virial of the direct + exclusion terms (fixed point, flushed to float)
+    av = nb.atom_virial.cpu().numpy().astype(np.float64)
+    assert rel_rms(av, ref["virial_atoms"]) < FORCE_TOL
+    nb.close()
+
+
+@pytest.mark.parametrize("it", [0, 1, 2])
+def test_gromacs_fixture_within_reference_tolerances(it):
+    """benchmarks/comparison/tests/gromacs/tests/comp_alanine_dipeptide.py:106-115:
+    |LJ_short - lj_sr| <= 25 and |PM - (coulomb_sr + coulomb_recip)| <= 2.0."""
+    s = gromacs_case(it)
+    ref = json.load(open(os.path.join(GOLDEN, "gromacs_charmm_tip3p", "reference_terms.json")))
+    entry = [x for x in ref["entries"] if x["iteration"] == it][0]
+    nb = _nb(s, cutoff=12.0)
+    e, _ = _full_terms(nb)
+    print("iter", it, "LJ_short", e["LJ_short"], "PM", e["PM"], flush=True)
+    assert abs(e["LJ_short"] - entry["lj_sr"]) <= 25.0
+    assert abs(e["PM"] - (entry["coulomb_sr"] + entry["coulomb_recip"])) <= 2.0
+    nb.close()
+
+
+def test_reference_cpu_values_on_fixture(gromacs_iter0, tip3p_1011):
+    """Values the reference's own CPU build printed for the committed inputs
+    (recorded when the reference was probed): GROMACS iter0 LJ_short 1237.962,
+    PM -7437.531 (rc 12); TIP3P 1011 LJ_short 508.1525, PM -3712.9453 (rc 8).
+    Agreement to the reference's float precision (~1e-6 relative)."""
+    for s, rc, lj, pm in ((gromacs_iter0, 12.0, 1237.962, -7437.531),
+                          (tip3p_1011, 8.0, 508.1525, -3712.9453)):
+        nb = _nb(s, cutoff=rc)
+        e, _ = _full_terms(nb)
+        assert abs(e["LJ_short"] - lj) < 2e-6 * abs(lj) + 1e-3
+        assert abs(e["PM"] - pm) < 2e-6 * abs(pm) + 1e-3
+        nb.close()
+
+
+def test_determinism_two_runs_identical(water_9k):
+    nb = _nb(water_9k)
+    outs = []
+    for _ in range(2):
+        nb.clear_fixed_forces()
+        nb.nl_update(forced=True)
+        nb.excluded(True, True, frc=False)
+        nb.lj_direct(True, True, frc=False)
+        nb.reciprocal(0, True, True, frc=False)
+        outs.append((nb.fixed_forces(), nb.energies(), nb.nl_rows()))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
+    np.testing.assert_array_equal(outs[0][2][0], outs[1][2][0])
+    np.testing.assert_array_equal(outs[0][2][1], outs[1][2][1])
+    nb.close()
+
+
+def test_translation_invariance_pme(water_9k):
+    """Shifting every atom by a lattice-commensurate vector leaves the grid
+    relation unchanged: PME energy is invariant to a shift by whole grid cells."""
+    s = dict(water_9k)
+    nb = _nb(s)
+    e0, f0 = _full_terms(nb)
+    nb.close()
+    h = np.float32(s["box"][0]) / np.float32(48.0) * np.float32(3.0)
+    s2 = dict(s)
+    s2["crd"] = (s["crd"] + h).astype(np.float32)
+    nb = _nb(s2)
+    e1, f1 = _full_terms(nb)
+    nb.close()
+    assert abs(e1["PM_reciprocal"] - e0["PM_reciprocal"]) < 1e-4 * abs(e0["PM"])
+    assert rel_rms(f1, f0) < 1e-4
+
+
+def test_nonfinite_coordinate_raises_flag(small_water):
+    s = dict(small_water)
+    crd = s["crd"].copy()
+    crd[5, 1] = np.nan
+    s["crd"] = crd
+    nb = _nb(s)
+    nb.reciprocal(0, frc=False)   # PME treats the atom as frac 0 (PM_force.cpp:710-714)
+    nb.lj_direct(frc=False)        # packing raises the flag
+    from sponge_amd.lib import FLAG_NONFINITE_CRD
+    assert nb.status(clear=True) & FLAG_NONFINITE_CRD
+    assert nb.status() == 0
+    nb.close()
+
+
+def test_neighbor_overflow_flag_and_truncation(small_water):
+    from sponge_amd.lib import FLAG_NL_OVERFLOW
+    nb = _nb(small_water, max_neighbor_numbers=16)
+    nb.nl_update(forced=True)
+    counts, rows = nb.nl_rows()
+    assert counts.max() == 16
+    assert nb.status() & FLAG_NL_OVERFLOW
+    nb.close()
+
+
+def test_box_too_small_is_an_error(small_water):
+    from sponge_amd.lib import SnbError
+    with pytest.raises(SnbError) as ei:
+        _nb(small_water, cutoff=12.0)   # (12+2)/2 = 7 A cells, 18.7 A box -> 2 cells
+    assert ei.value.code == 3
+
+
+def test_conditional_rebuild_policy(small_water):
+    """refresh_interval 0: rebuild only when an atom moved more than
+    skin * skin_permit (neighbor_list.cpp:166-180); the stale list is kept
+    otherwise."""
+    import torch
+    nb = _nb(small_water, refresh_interval=0)
+    nb.nl_update(forced=True)
+    c0, r0 = nb.nl_rows()
+    # move one atom by 0.9 A (< 1 A): no rebuild, rows unchanged
+    nb.crd[0, 0] += 0.9
+    nb.nl_update(step=1, forced=False)
+    c1, r1 = nb.nl_rows()
+    np.testing.assert_array_equal(c0, c1)
+    # move it by 1.1 A in total from the rebuild point: rebuild
+    nb.crd[0, 0] += 0.2
+    nb.nl_update(step=2, forced=False)
+    c2, _ = nb.nl_rows()
+    import oracle
+    s = small_water
+    oc, _, _ = oracle.nl_build(nb.crd.cpu().numpy(), s["cell"], s["rcell"], nb.cutoff, 2.0,
+                               _excl(s), max_nb=nb.max_nb)
+    np.testing.assert_array_equal(c2, oc)
+    nb.close()

@@ -1,0 +1,385 @@
+"""Pins of the CPU oracle (oracle/snb_oracle.c), CPU only.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a sign, an index, a transposed operand) fails it: known answers printed
+by the reference for its own fixtures, textbook constants, closed forms and
+brute force, and the float twin against the FP64 layer.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy.special import erfc as sp_erfc
+
+import oracle
+from conftest import GOLDEN, gromacs_case, rel_rms
+
+
+# --------------------------------------------------------------------- beta
+@pytest.mark.parametrize("rc,beta", [(8.0, 0.348639), (12.0, 0.226635), (10.0, 0.275106),
+                                     (9.0, 0.307676)])
+def test_beta_known_values(rc, beta):
+    """PM_force.cpp:244-272: reference values (rc 8 and 12 printed by the
+    reference's CPU build, rc 9 and 10 from a float emulation of it)."""
+    assert abs(oracle.get_beta(rc, 1e-5) - beta) < 2e-6
+
+
+@pytest.mark.parametrize("rc", [8.0, 9.0, 10.0, 12.0])
+def test_beta_solves_tolerance_equation(rc):
+    b = oracle.get_beta(rc, 1e-5)
+    assert abs(sp_erfc(b * rc) / rc - 1e-5) < 1e-9
+
+
+# ----------------------------------------------------------------- grid size
+@pytest.mark.parametrize("length,size", [(44.8, 48), (96.5, 100), (215.24, 216), (24.64, 28),
+                                         (79.3, 80), (70.0, 72), (21.8359051, 24),
+                                         (23.3691196, 24), (19.9797707, 20), (62.0, 64),
+                                         (125.0, 128)])
+def test_fft_size_reference_rule(length, size):
+    """common.cpp:639-659 (values the reference printed: 72^3 for the 70 A
+    GROMACS box, 24x24x20 for the TIP3P box)."""
+    n = oracle.fft_size(length)
+    assert n == size
+    assert oracle.check_2357(n) and n % 4 == 0
+
+
+def test_check_2357():
+    assert [oracle.check_2357(x) for x in (1, 2, 49, 210, 11, 22, 26)] == \
+        [True, True, True, True, False, False, False]
+
+
+# ------------------------------------------------------- B-spline machinery
+def test_bspline_moduli_closed_form():
+    """order 4: b(0) = 1 (the weights sum to 1), b(K/2) = 9 (|1/6 - 2/3 + 1/6| = 1/3)."""
+    b = oracle.bspline_moduli(48)
+    assert abs(b[0] - 1.0) < 1e-14
+    assert abs(b[24] - 9.0) < 1e-12
+    m = np.arange(48)
+    w = np.exp(2j * np.pi * np.outer(m, np.arange(3)) / 48) @ np.array([1 / 6, 2 / 3, 1 / 6])
+    np.testing.assert_allclose(b, 1.0 / np.abs(w) ** 2, rtol=1e-13)
+
+
+def test_bc_zero_mode_and_symmetry():
+    rcell = np.array([1 / 30.0, 0, 1 / 30.0, 0, 0, 1 / 30.0], np.float32)
+    bc = oracle.bc_table(24, 24, 24, rcell, 0.3, 27000.0)
+    assert bc[0, 0, 0] == 0.0
+    assert np.all(bc.ravel()[1:] > 0)
+    np.testing.assert_allclose(bc[1, 2, 3], bc[23, 22, 3], rtol=1e-13)  # k and -k (kz kept)
+
+
+# ------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("shape", [(4, 6, 5), (8, 3, 7), (6, 10, 9)])
+def test_fft_matches_naive_dft(shape):
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    np.testing.assert_allclose(oracle.fft3d(x), oracle.dft3d_naive(x), atol=1e-10)
+    np.testing.assert_allclose(oracle.fft3d(x, +1), oracle.dft3d_naive(x, +1), atol=1e-10)
+
+
+# ----------------------------------------------------------------- Ewald
+def _rock_salt(cells, a=1.0):
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    na = base
+    cl = base + np.array([0.5, 0, 0])
+    pos, q = [], []
+    for i in range(cells):
+        for j in range(cells):
+            for k in range(cells):
+                off = np.array([i, j, k], float)
+                pos += list((na + off) * a)
+                q += [1.0] * 4
+                pos += list((cl + off) * a)
+                q += [-1.0] * 4
+    return np.array(pos), np.array(q), cells * a
+
+
+def test_ewald_madelung_constant_rock_salt():
+    """Textbook: E per ion pair = -M q^2 / r0 with M = 1.747565 (r0 = a/2)."""
+    pos, q, L = _rock_salt(3, a=2.0)
+    _, ed, er, es = oracle.ewald_f64(pos, q, np.array([L, L, L]), beta=1.5, rcut=2.99, kmax=14)
+    e = ed + er + es
+    m = -e / (len(q) / 2) * 1.0   # r0 = 1
+    assert abs(m - 1.747565) < 1e-5
+
+
+def test_pme_f64_matches_exact_ewald():
+    """The FP64 PME reciprocal + direct + self reproduces the exact Ewald sum of
+    a random neutral box (no exclusions), energies and forces."""
+    rng = np.random.default_rng(5)
+    n, L = 60, 20.0
+    pos = rng.uniform(0, L, (n, 3))
+    q = rng.normal(0, 1, n)
+    q -= q.mean()
+    beta, rc = 0.35, 9.5
+    f_ew, ed, er, es = oracle.ewald_f64(pos, q, np.array([L, L, L]), beta, rc, kmax=14)
+    rcell = np.array([1 / L, 0, 1 / L, 0, 0, 1 / L], np.float32)
+    crd = pos.astype(np.float32)
+    qf = q.astype(np.float32)
+    f_rec, e_rec, _, _, _ = oracle.pme_f64(crd, qf, rcell, (48, 48, 48), beta, L ** 3)
+    # 4th-order B-splines on a 0.42 A grid: ~1e-4 relative reciprocal error
+    assert abs(e_rec - er) < 2e-3 * abs(er) + 1e-3
+    # reciprocal forces: Ewald total minus its direct part
+    f_dir = np.zeros((n, 3))
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            for a in (-1, 0, 1):
+                for b in (-1, 0, 1):
+                    for c in (-1, 0, 1):
+                        d = pos[j] - pos[i] + np.array([a, b, c]) * L
+                        r = np.linalg.norm(d)
+                        if r >= rc:
+                            continue
+                        x = beta * r
+                        fa = -q[i] * q[j] * (2 / math.sqrt(math.pi) * x * math.exp(-x * x) +
+                                             math.erfc(x)) / r ** 3
+                        f_dir[i] += fa * d
+    assert rel_rms(f_rec, f_ew - f_dir) < 2e-3
+
+
+def test_pme_translation_invariance_and_zero_net_force():
+    rng = np.random.default_rng(9)
+    n, L = 80, 24.0
+    pos = rng.uniform(0, L, (n, 3)).astype(np.float32)
+    q = rng.normal(0, 1, n).astype(np.float32)
+    q -= q.mean()
+    rcell = np.array([1 / L, 0, 1 / L, 0, 0, 1 / L], np.float32)
+    f0, e0, _, _, _ = oracle.pme_f64(pos, q, rcell, (24, 24, 24), 0.3, L ** 3)
+    f1, e1, _, _, _ = oracle.pme_f64((pos + np.float32(L / 24 * 5)).astype(np.float32), q,
+                                     rcell, (24, 24, 24), 0.3, L ** 3)
+    assert abs(e1 - e0) < 1e-6 * abs(e0)
+    assert rel_rms(f1, f0) < 1e-4
+    # smooth PME does not conserve momentum exactly (the B-spline gather is not
+    # pairwise antisymmetric); the net force stays a small fraction
+    assert np.abs(f0.sum(0)).max() < 1e-2 * np.abs(f0).max()
+
+
+def test_bspline_partition_of_unity():
+    """sum_k w_k(t) = 1 and sum_k w_k'(t) = 0 -> a uniform charge sheet gives a
+    spread grid summing to the total charge."""
+    L = 16.0
+    rcell = np.array([1 / L, 0, 1 / L, 0, 0, 1 / L], np.float32)
+    rng = np.random.default_rng(1)
+    pos = rng.uniform(0, L, (30, 3)).astype(np.float32)
+    q = rng.normal(0, 1, 30).astype(np.float32)
+    _, _, _, Q, _ = oracle.pme_f64(pos, q, rcell, (16, 16, 16), 0.3, L ** 3, want_grids=True)
+    assert abs(Q.sum() - q.astype(np.float64).sum()) < 1e-5
+
+
+# ------------------------------------------------------------ pair math
+def _two_atoms(r, qa=1.3, qb=-0.7, sig=3.2, eps=0.2, L=40.0):
+    crd = np.array([[5.0, 5.0, 5.0], [5.0 + r, 5.0, 5.0]], np.float32)
+    a = np.array([4 * eps * sig ** 12 * 12], np.float32)
+    b = np.array([4 * eps * sig ** 6 * 6], np.float32)
+    cell = np.array([L, 0, L, 0, 0, L], np.float32)
+    rcell = np.array([1 / L, 0, 1 / L, 0, 0, 1 / L], np.float32)
+    return crd, np.array([qa, qb], np.float32), np.zeros(2, np.int32), a, b, cell, rcell
+
+
+@pytest.mark.parametrize("r", [2.5, 3.5, 5.0, 9.0])
+def test_pair_two_atom_closed_form(r):
+    """Lennard_Jones_force.h:43-73: E = A/12 r^-12 - B/6 r^-6 + qq erfc(br)/r and
+    F_i = -dE/dr_i, checked against a central difference of the closed form."""
+    crd, q, t, A, B, cell, rcell = _two_atoms(r)
+    beta = 0.3
+    counts = np.array([1, 0], np.int32)
+    rows = np.array([[1], [0]], np.int32)
+    f, elj, ec, vir = oracle.pair_f64(crd, q, t, A, B, cell, rcell, 10.0, beta, counts, rows)
+    rr = float(np.float32(5.0 + r) - np.float32(5.0))
+    qq = float(q[0]) * float(q[1])
+
+    def energy(x):
+        return float(A[0]) / 12 * x ** -12 - float(B[0]) / 6 * x ** -6 + qq * math.erfc(beta * x) / x
+    de = (energy(rr + 1e-6) - energy(rr - 1e-6)) / 2e-6
+    assert abs(elj[0] + ec[0] - energy(rr)) < 1e-8 * max(1, abs(energy(rr)))
+    # force on atom 0 along +x is +dE/dr (moving atom 0 right shortens r)
+    assert abs(f[0, 0] - de) < 1e-5 * max(1.0, abs(de))
+    assert abs(f[0, 0] + f[1, 0]) < 1e-12
+    # virial (reference convention -f (x) dr): a11 = -f0x * (x1 - x0)
+    assert abs(vir[0, 0] - (-f[0, 0] * rr)) < 1e-9 * max(1, abs(vir[0, 0]))
+
+
+def test_pair_cutoff_excludes_far_pairs():
+    crd, q, t, A, B, cell, rcell = _two_atoms(10.5)
+    f, elj, ec, _ = oracle.pair_f64(crd, q, t, A, B, cell, rcell, 10.0, 0.3,
+                                    np.array([1, 0], np.int32), np.array([[1], [0]], np.int32))
+    assert np.all(f == 0) and elj[0] == 0 and ec[0] == 0
+
+
+def test_pair_minimum_image_across_boundary():
+    L = 30.0
+    crd = np.array([[0.5, 1.0, 1.0], [29.0, 1.0, 1.0]], np.float32)   # 1.5 A apart via the wall
+    _, q, t, A, B, cell, rcell = _two_atoms(1.0, L=L)
+    f, elj, _, _ = oracle.pair_f64(crd, q, t, A, B, cell, rcell, 10.0, 0.3,
+                                   np.array([1, 0], np.int32), np.array([[1], [0]], np.int32))
+    crd2 = np.array([[0.5, 1.0, 1.0], [-1.0, 1.0, 1.0]], np.float32)
+    f2, elj2, _, _ = oracle.pair_f64(crd2, q, t, A, B, cell, rcell, 10.0, 0.3,
+                                     np.array([1, 0], np.int32), np.array([[1], [0]], np.int32))
+    np.testing.assert_allclose(f, f2, rtol=1e-6)
+    assert abs(elj[0] - elj2[0]) < 1e-6 * abs(elj[0])
+
+
+# --------------------------------------------------------- exclusion term
+def test_exclusion_plus_direct_equals_bare_coulomb():
+    """E_excl(pair) + E_direct(pair, no cutoff) = -qq erf/r + qq erfc/r: for an
+    excluded pair the correction removes exactly the erf part, so
+    E_excl + E_direct + qq/r * ... : (-qq erf(x) + qq erfc(x))/r + qq erf(x)/r*2 = qq/r."""
+    crd, q, t, A, B, cell, rcell = _two_atoms(1.6)
+    A[:] = 0
+    B[:] = 0
+    beta = 0.3
+    excl = (np.array([0, 1], np.int32), np.array([1, 0], np.int32), np.array([1], np.int32))
+    f_ex, e_ex, _ = oracle.excl_f64(crd, q, cell, rcell, beta, excl)
+    f_d, _, ec, _ = oracle.pair_f64(crd, q, t, A, B, cell, rcell, 10.0, beta,
+                                    np.array([1, 0], np.int32), np.array([[1], [0]], np.int32))
+    r = float(np.float32(6.6) - np.float32(5.0))
+    qq = float(q[0]) * float(q[1])
+    beta = float(np.float32(beta))   # the oracle takes beta as float
+    assert abs((e_ex[0] + ec[0]) - qq * (math.erfc(beta * r) - math.erf(beta * r)) / r) < 1e-10
+    # forces on atom 0 along x: the direct (erfc) part plus the erf part that the
+    # reciprocal sum carries is the bare Coulomb force, and the exclusion
+    # correction removes exactly that erf part
+    x = beta * r
+    f_erf = -qq * (math.erf(x) - 2 / math.sqrt(math.pi) * x * math.exp(-x * x)) / r ** 2
+    f_bare = -qq / r ** 2
+    assert abs((f_d[0, 0] + f_erf) - f_bare) < 1e-9 * abs(f_bare)
+    assert abs(f_ex[0, 0] + f_erf) < 1e-9 * abs(f_bare)
+
+
+# ------------------------------------------------------- neighbor list
+def test_nl_brute_force_invariants(small_water):
+    s = small_water
+    excl = (s["excl_start"], s["excl_numbers"], s["excl_list"])
+    counts, rows, ov = oracle.nl_build(s["crd"], s["cell"], s["rcell"], 8.0, 2.0, excl)
+    assert not ov
+    L = s["box"][0]
+    for i in range(s["n"]):
+        r = rows[i, :counts[i]]
+        assert np.all(r > i) and len(np.unique(r)) == len(r)
+        d = s["crd"][r] - s["crd"][i]
+        d -= np.round(d / L) * L
+        assert np.all(np.einsum("ij,ij->i", d, d) < 100.0 + 1e-3)
+    # exclusions removed: water i's H's are never in the O row
+    for w in range(0, s["n"], 3):
+        assert not np.isin([w + 1, w + 2], rows[w, :counts[w]]).any()
+
+
+def test_nl_overflow_truncates(small_water):
+    s = small_water
+    counts, rows, ov = oracle.nl_build(s["crd"], s["cell"], s["rcell"], 8.0, 2.0, None, max_nb=8)
+    assert ov and counts.max() == 8
+
+
+# ---------------------------------------------------- twin vs FP64 layer
+def test_twin_math_error_bounds():
+    # erfc on beta*r <= 3.5 (beta*rc = 2.75 at rc 10 A, tol 1e-5); the float
+    # rounding of the argument -x*x adds ~x^2 * 2^-24 relative error
+    x = np.linspace(0, 3.5, 7001).astype(np.float32)
+    got = np.array([lib_call(oracle.lib().orc_twin_erfc, v) for v in x[::10]])
+    ref = sp_erfc(x[::10].astype(np.float64))
+    assert np.max(np.abs(got - ref) / ref) < 1e-6
+    y = -np.linspace(0, 30, 301).astype(np.float32)
+    ge = np.array([lib_call(oracle.lib().orc_twin_expf, v) for v in y])
+    assert np.max(np.abs(ge - np.exp(y.astype(np.float64))) / np.exp(y.astype(np.float64))) < 3e-7
+    r2 = np.geomspace(0.5, 200.0, 400).astype(np.float32)
+    gr = np.array([lib_call(oracle.lib().orc_twin_rsqrtf, v) for v in r2])
+    assert np.max(np.abs(gr * np.sqrt(r2.astype(np.float64)) - 1)) < 3e-7
+    xs = np.linspace(0.0, 2.0, 401).astype(np.float32)
+    gh = np.array([lib_call(oracle.lib().orc_twin_erf_minus_gauss, v) for v in xs])
+    from scipy.special import erf
+    xd = xs.astype(np.float64)
+    hr = erf(xd) - 2 / np.sqrt(np.pi) * xd * np.exp(-xd * xd)
+    nz = xd > 0.05
+    assert np.max(np.abs(gh[nz] - hr[nz]) / hr[nz]) < 2e-6
+
+
+def lib_call(fn, v):
+    return float(fn(float(v)))
+
+
+def test_fixed_point_rounding():
+    L = oracle.lib()
+    assert L.orc_to_fixed(1.0) == 2 ** 32
+    assert L.orc_to_fixed(-1.5) == -3 * 2 ** 31
+    assert L.orc_to_fixed(2.0 ** -33) == 0            # half an LSB rounds to even (0)
+    assert L.orc_to_fixed(3 * 2.0 ** -33) == 2        # 1.5 LSB rounds to even (2)
+
+
+def test_twin_pair_and_exclusion_match_fp64(water_9k):
+    s = water_9k
+    excl = (s["excl_start"], s["excl_numbers"], s["excl_list"])
+    beta = oracle.get_beta(10.0, 1e-5)
+    counts, rows, _ = oracle.nl_build(s["crd"], s["cell"], s["rcell"], 10.0, 2.0, excl)
+    tf, telj, tec, tv = oracle.pair_twin(s["crd"], s["charge"], s["lj_type"], s["lj_a"],
+                                         s["lj_b"], s["cell"], s["rcell"], 10.0, beta, counts,
+                                         rows)
+    df, delj, dec, dv = oracle.pair_f64(s["crd"], s["charge"], s["lj_type"], s["lj_a"],
+                                        s["lj_b"], s["cell"], s["rcell"], 10.0, beta, counts,
+                                        rows)
+    assert rel_rms(tf.T / 2 ** 32, df) < 2e-6
+    assert abs(telj.sum() / 2 ** 32 - delj.sum()) < 1e-6 * abs(delj.sum())
+    # the direct Coulomb sum of neutral waters cancels ~1e3-fold: float per-pair
+    # rounding (~1e-7 of each term, random) leaves ~1e-5 of the net sum
+    assert abs(tec.sum() / 2 ** 32 - dec.sum()) < 3e-5 * abs(dec.sum())
+    assert rel_rms(tv.T / 2 ** 32, dv) < 2e-6
+    assert np.all(tf.sum(axis=1) == 0)
+    ef, ee, ev = oracle.excl_twin(s["crd"], s["charge"], s["cell"], s["rcell"], beta, excl)
+    df2, de2, dv2 = oracle.excl_f64(s["crd"], s["charge"], s["cell"], s["rcell"], beta, excl)
+    assert rel_rms(ef.T / 2 ** 32, df2) < 2e-6
+    assert abs(ee.sum() / 2 ** 32 - de2.sum()) < 1e-6 * abs(de2.sum())
+
+
+def test_twin_spread_gather_match_fp64(water_9k):
+    s = water_9k
+    beta = oracle.get_beta(10.0, 1e-5)
+    L = float(s["box"][0])
+    vol = float(np.float32(L) ** 3)
+    f, e, _, Q, phi = oracle.pme_f64(s["crd"], s["charge"], s["rcell"], (48, 48, 48), beta, vol,
+                                     want_grids=True)
+    qt = oracle.pme_spread_twin(s["crd"], s["charge"], s["rcell"], (48, 48, 48))
+    assert np.max(np.abs(qt / 2 ** 32 - Q)) < 1e-5 * np.max(np.abs(Q))
+    ft = oracle.pme_gather_twin(s["crd"], s["charge"], s["rcell"], (48, 48, 48),
+                                phi.astype(np.float32))
+    assert rel_rms(ft.T / 2 ** 32, f) < 1e-6
+
+
+# ----------------------------------------------- the reference's fixtures
+def test_oracle_reproduces_reference_cpu_output(gromacs_iter0, tip3p_1011):
+    """Numbers the reference's own CPU build printed for its committed inputs:
+    GROMACS CHARMM TIP3P iter0 (rc 12): LJ_short 1237.962, PM -7437.531;
+    TIP3P water 1011 atoms (rc 8): LJ_short 508.1525, PM -3712.9453.  The
+    reference runs in FP32 with MKL FFT, so agreement is ~1e-6 relative."""
+    r = oracle.nonbonded_terms_f64(gromacs_iter0, cutoff=12.0)
+    assert r["grid"] == (72, 72, 72) and abs(r["beta"] - 0.226635) < 2e-6
+    assert abs(r["LJ_short"] - 1237.962) < 1e-3
+    assert abs(r["PM"] - (-7437.531)) < 7.5e-3
+    r = oracle.nonbonded_terms_f64(tip3p_1011, cutoff=8.0)
+    assert r["grid"] == (24, 24, 20)
+    assert abs(r["LJ_short"] - 508.1525) < 1e-3
+    assert abs(r["PM"] - (-3712.9453)) < 3.7e-3
+
+
+@pytest.mark.parametrize("it", [0, 1, 2])
+def test_oracle_within_reference_gromacs_tolerances(it):
+    """benchmarks/comparison/tests/gromacs/tests/comp_alanine_dipeptide.py:111-112."""
+    s = gromacs_case(it)
+    ref = json.load(open(os.path.join(GOLDEN, "gromacs_charmm_tip3p", "reference_terms.json")))
+    entry = [x for x in ref["entries"] if x["iteration"] == it][0]
+    r = oracle.nonbonded_terms_f64(s, cutoff=12.0)
+    assert abs(r["LJ_short"] - entry["lj_sr"]) <= 25.0
+    assert abs(r["PM"] - (entry["coulomb_sr"] + entry["coulomb_recip"])) <= 2.0
+
+
+def test_lj_long_range_factor_histogram_equals_double_loop(tip3p_1011):
+    s = tip3p_1011
+    t = s["lj_type"]
+    b = s["lj_b"]
+    tot = 0.0
+    for i in range(s["lj_ntypes"]):
+        for j in range(s["lj_ntypes"]):
+            hi, lo = max(i, j), min(i, j)
+            tot += np.sum(t == i) * np.sum(t == j) * float(b[hi * (hi + 1) // 2 + lo])
+    assert abs(oracle.lj_c6_total(t, s["lj_ntypes"], b) - tot) < 1e-9 * abs(tot)
