@@ -195,6 +195,12 @@ int snb_pme_reciprocal_virial(snb_context* ctx, double* virial6);
 /* ---------------------------------------------------- fixed-point accumulators */
 /* d_frc[i] += accumulated force, then clears the accumulator. */
 int snb_flush_forces(snb_context* ctx, snb_vector* d_frc);
+/* For outputs deferred with NULL: d_atom_energy[i] += the per-atom energies of
+ * the last energy calls (LJ + direct Coulomb + exclusion correction) and
+ * d_atom_direct_pme_energy[i] = the direct Coulomb part; either may be NULL. */
+int snb_flush_atom_energy(snb_context* ctx, float* d_atom_energy, float* d_atom_direct_pme_energy);
+/* d_atom_virial[i] += the accumulated per-atom virial, then clears it. */
+int snb_flush_atom_virial(snb_context* ctx, snb_ltmatrix3* d_atom_virial);
 /* Copies the int64 force accumulators (SoA x[n] | y[n] | z[n]) to d_out. */
 int snb_copy_fixed_forces(snb_context* ctx, int64_t* d_out);
 int snb_clear_fixed_forces(snb_context* ctx);

@@ -67,8 +67,10 @@ def _declare(L):
     L.orc_ewald_f64.argtypes = [_i, _p, _p, _p, _d, _d, _i, _p, _p, _p, _p]
     L.orc_lj_c6_total.restype = _d
     L.orc_lj_c6_total.argtypes = [_i, _p, _i, _p]
-    for name in ("orc_twin_expf", "orc_twin_erfc", "orc_twin_erf_minus_gauss",
-                 "orc_twin_rsqrtf"):
+    L.orc_twin_erf_parts.argtypes = [_d, ctypes.POINTER(_d), ctypes.POINTER(_d)]
+    L.orc_twin_exp_d.restype = _d
+    L.orc_twin_exp_d.argtypes = [_d]
+    for name in ("orc_twin_expf", "orc_twin_erfc", "orc_twin_rsqrtf"):
         getattr(L, name).restype = _f
         getattr(L, name).argtypes = [_f]
     L.orc_to_fixed.restype = ctypes.c_int64

@@ -115,29 +115,6 @@ static inline float twin_erfc_core(float x, float ex)
     return (t * q) * ex;
 }
 
-/* h(x) = erf(x) - (2/sqrt(pi)) x exp(-x^2), needed without cancellation by
- * the exclusion correction.  For x < 0.5 the Maclaurin series
- * h = (2/sqrt(pi)) sum_{n>=1} (-1)^{n+1} 2n x^{2n+1} / (n! (2n+1)); above,
- * erf = 1 - erfc. */
-static inline float twin_erf_minus_gauss(float x, float ex)
-{
-    if (x < 0.5f)
-    {
-        float x2 = x * x;
-        float s = -2.6339383e-05f;          /* n = 8 */
-        s = fmaf(s, x2, 2.08959105e-04f);   /* n = 7 */
-        s = fmaf(s, x2, -1.44663996e-03f);  /* n = 6 */
-        s = fmaf(s, x2, 8.54832702e-03f);   /* n = 5 */
-        s = fmaf(s, x2, -4.1791821e-02f);   /* n = 4 */
-        s = fmaf(s, x2, 1.61197024e-01f);   /* n = 3 */
-        s = fmaf(s, x2, -4.51351667e-01f);  /* n = 2 */
-        s = fmaf(s, x2, 7.52252778e-01f);   /* n = 1 */
-        return (s * x2) * x;
-    }
-    float erfv = 1.0f - twin_erfc_core(x, ex);
-    return erfv - (x * ORC_TWO_OVER_SQRT_PI_F) * ex;
-}
-
 /* 1/sqrt(x) for normal x > 0: integer seed 0x5f375a86 - (bits >> 1)
  * (Lomont, "Fast inverse square root", 2003) refined by three Newton steps
  * y <- y (1.5 - (x/2) y^2); only IEEE mul/fma, so it is reproducible. */
@@ -149,6 +126,51 @@ static inline float twin_rsqrtf(float x)
     y = y * fmaf(-(h * y), y, 1.5f);
     y = y * fmaf(-(h * y), y, 1.5f);
     return y;
+}
+
+/* FP64 twin pieces: exp by Cody-Waite (33-bit ln2 head) + degree-13 Taylor;
+ * erf and h = erf - 2x/sqrt(pi) e^{-x^2} from the positive series
+ * erf = 2/sqrt(pi) x e^{-x^2} sum_{n>=0} (2x^2)^n/(2n+1)!! (A&S 7.1.6). */
+static inline double twin_exp_d(double y)
+{
+    static const double inv_fact[14] = {1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0,
+                                        1.0 / 720.0, 1.0 / 5040.0, 1.0 / 40320.0,
+                                        1.0 / 362880.0, 1.0 / 3628800.0, 1.0 / 39916800.0,
+                                        1.0 / 479001600.0, 1.0 / 6227020800.0};
+    y = fmax(y, -700.0);
+    double n = rint(y * 1.4426950408889634);
+    double r = fma(-n, 0.6931471806019545, y);
+    r = fma(-n, -4.2009150726810846e-11, r);
+    double p = inv_fact[13];
+    for (int k = 12; k >= 0; k--) p = fma(p, r, inv_fact[k]);
+    int64_t ni = (int64_t)n;
+    uint64_t b = (uint64_t)(ni + 1023) << 52;
+    double s;
+    memcpy(&s, &b, 8);
+    return p * s;
+}
+
+static inline void twin_erf_parts_d(double x, double* erf_out, double* h_out)
+{
+    double g = 1.1283791670955126 * x * twin_exp_d(-(x * x));
+    double a = 2.0 * x * x;
+    double t = 1.0, s = 0.0;
+    for (int n = 1; n < 80; n++)
+    {
+        t = t * (a / (double)(2 * n + 1));
+        s = s + t;
+        if (t <= 1e-17 * s) break;
+    }
+    *h_out = g * s;
+    *erf_out = g * s + g;
+}
+
+static inline int64_t to_fixed_d(double d)
+{
+    double v = fma(d, ORC_FIXED_SCALE, 6755399441055744.0);
+    uint64_t u;
+    memcpy(&u, &v, 8);
+    return (int64_t)u - (int64_t)0x4338000000000000LL;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -462,12 +484,23 @@ ORC_EXPORT void orc_pair_twin(int n, const float* crd, const float* charge,
             frc[j] -= ix;
             frc[n + j] -= iy;
             frc[2 * n + j] -= iz;
-            if (e_lj)
+            if (e_lj || e_c)
             {
-                float elj = ((A * 0.083333333f) * rinv6 - B * 0.166666667f) * rinv6;
-                e_lj[i] += to_fixed(elj);
+                /* energies in FP64 from the float displacement */
+                double ddx = d[0], ddy = d[1], ddz = d[2];
+                double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                double rd = sqrt(r2d);
+                double rid = 1.0 / rd;
+                double ri2 = rid * rid;
+                double ri6 = (ri2 * ri2) * ri2;
+                if (e_lj)
+                    e_lj[i] += to_fixed_d((((double)A * (1.0 / 12.0)) * ri6 -
+                                           (double)B * (1.0 / 6.0)) * ri6);
+                double erfd, hd;
+                twin_erf_parts_d((double)beta * rd, &erfd, &hd);
+                if (e_c)
+                    e_c[i] += to_fixed_d((((double)qi * (double)charge[j]) * (1.0 - erfd)) * rid);
             }
-            if (e_c) e_c[i] += to_fixed((qq * erfcv) * rinv);
             if (vir)
             {
                 vir[0 * (size_t)n + i] += to_fixed(-(fx * d[0]));
@@ -542,35 +575,32 @@ ORC_EXPORT void orc_excl_twin(int n, const float* crd, const float* charge,
             int j = excl_list[k];
             float d[3];
             pbc_disp_f(crd + 3 * j, crd + 3 * i, cell, rcell, d);
-            float r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
-            float rinv = twin_rsqrtf(r2);
-            float r = r2 * rinv;
-            float x = beta * r;
-            float ex = twin_expf(-(x * x));
-            float h = twin_erf_minus_gauss(x, ex);
-            float qq = qi * charge[j];
-            float fa = (qq * h) * ((rinv * rinv) * rinv);
-            float fx = fa * d[0], fy = fa * d[1], fz = fa * d[2];
-            int64_t ix = to_fixed(fx), iy = to_fixed(fy), iz = to_fixed(fz);
+            /* FP64 from the float displacement */
+            double ddx = d[0], ddy = d[1], ddz = d[2];
+            double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
+            double rd = sqrt(r2d);
+            double rid = 1.0 / rd;
+            double erfd, hd;
+            twin_erf_parts_d((double)beta * rd, &erfd, &hd);
+            double qq = (double)qi * (double)charge[j];
+            double fa = (qq * hd) * ((rid * rid) * rid);
+            double fx = fa * ddx, fy = fa * ddy, fz = fa * ddz;
+            int64_t ix = to_fixed_d(fx), iy = to_fixed_d(fy), iz = to_fixed_d(fz);
             frc[i] += ix;
             frc[n + i] += iy;
             frc[2 * n + i] += iz;
             frc[j] -= ix;
             frc[n + j] -= iy;
             frc[2 * n + j] -= iz;
-            if (e)
-            {
-                float erfv = h + (x * ORC_TWO_OVER_SQRT_PI_F) * ex;
-                e[i] += to_fixed(-((qq * erfv) * rinv));
-            }
+            if (e) e[i] += to_fixed_d(-((qq * erfd) * rid));
             if (vir)
             {
-                vir[0 * (size_t)n + i] += to_fixed(-(fx * d[0]));
-                vir[1 * (size_t)n + i] += to_fixed(-(fx * d[1] + fy * d[0]));
-                vir[2 * (size_t)n + i] += to_fixed(-(fy * d[1]));
-                vir[3 * (size_t)n + i] += to_fixed(-(fx * d[2] + fz * d[0]));
-                vir[4 * (size_t)n + i] += to_fixed(-(fy * d[2] + fz * d[1]));
-                vir[5 * (size_t)n + i] += to_fixed(-(fz * d[2]));
+                vir[0 * (size_t)n + i] += to_fixed_d(-(fx * ddx));
+                vir[1 * (size_t)n + i] += to_fixed_d(-(fx * ddy + fy * ddx));
+                vir[2 * (size_t)n + i] += to_fixed_d(-(fy * ddy));
+                vir[3 * (size_t)n + i] += to_fixed_d(-(fx * ddz + fz * ddx));
+                vir[4 * (size_t)n + i] += to_fixed_d(-(fy * ddz + fz * ddy));
+                vir[5 * (size_t)n + i] += to_fixed_d(-(fz * ddz));
             }
         }
     }
@@ -1044,9 +1074,10 @@ ORC_EXPORT double orc_lj_c6_total(int n, const int* type, int ntypes,
 /* exposes the twin's scalar math for unit tests */
 ORC_EXPORT float orc_twin_expf(float y) { return twin_expf(y); }
 ORC_EXPORT float orc_twin_erfc(float x) { return twin_erfc_core(x, twin_expf(-(x * x))); }
-ORC_EXPORT float orc_twin_erf_minus_gauss(float x)
+ORC_EXPORT void orc_twin_erf_parts(double x, double* erf_out, double* h_out)
 {
-    return twin_erf_minus_gauss(x, twin_expf(-(x * x)));
+    twin_erf_parts_d(x, erf_out, h_out);
 }
+ORC_EXPORT double orc_twin_exp_d(double y) { return twin_exp_d(y); }
 ORC_EXPORT float orc_twin_rsqrtf(float x) { return twin_rsqrtf(x); }
 ORC_EXPORT int64_t orc_to_fixed(float f) { return to_fixed(f); }

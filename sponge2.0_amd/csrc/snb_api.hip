@@ -405,7 +405,10 @@ int snb_nl_update(snb_context* c, const int* d_atom_local, int local_atom_number
     const int* el = ex_or(d_ex_list, c->ex_list);
     const int* en = ex_or(d_ex_num, c->ex_num);
     if (!d_ex_num && !c->has_excl) es = el = en = nullptr;
-    return snb_launch_nl_update(c, d_crd, box, update ? 1 : 0, step, es, el, en);
+    int forced = (update || !c->nl.built) ? 1 : 0;
+    int rc = snb_launch_nl_update(c, d_crd, box, forced, step, es, el, en);
+    if (!rc) c->nl.built = true;
+    return rc;
 }
 
 int snb_nl_device_list(snb_context* c, snb_atom_group** d_nl, int** d_serial, int* max_nb)
@@ -709,6 +712,20 @@ int snb_flush_forces(snb_context* c, snb_vector* d_frc)
     if (!c || (!d_frc && c->n > 0)) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
     return snb_launch_flush_forces(c, d_frc);
+}
+
+int snb_flush_atom_energy(snb_context* c, float* d_atom_energy, float* d_direct)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    return snb_launch_flush_atom_energy(c, d_atom_energy, d_direct, 7);
+}
+
+int snb_flush_atom_virial(snb_context* c, snb_ltmatrix3* d_atom_virial)
+{
+    if (!c || (!d_atom_virial && c->n > 0)) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    return snb_launch_flush_virial(c, d_atom_virial);
 }
 
 int snb_copy_fixed_forces(snb_context* c, int64_t* d_out)

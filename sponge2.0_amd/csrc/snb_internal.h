@@ -40,6 +40,7 @@ struct SnbFftPlan
 struct SnbNl
 {
     bool init = false;
+    bool built = false;  // the first update always rebuilds (main.cpp:571-575)
     float cutoff = 10.0f, skin = 2.0f, grid_length = 6.0f, cs2 = 144.0f;
     float skin_permit = 0.5f;
     int refresh_interval = 0;

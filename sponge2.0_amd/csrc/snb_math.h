@@ -73,28 +73,6 @@ SNB_HD float snb_erfc_core(float x, float ex)
     return (t * q) * ex;
 }
 
-// h(x) = erf(x) - (2/sqrt(pi)) x exp(-x^2) without cancellation: Maclaurin
-// series h = (2/sqrt(pi)) sum_{n>=1} (-1)^{n+1} 2n x^{2n+1} / (n! (2n+1)) for
-// x < 0.5 (8 terms), erf = 1 - erfc above.
-SNB_HD float snb_erf_minus_gauss(float x, float ex)
-{
-    if (x < 0.5f)
-    {
-        float x2 = x * x;
-        float s = -2.6339383e-05f;
-        s = __builtin_fmaf(s, x2, 2.08959105e-04f);
-        s = __builtin_fmaf(s, x2, -1.44663996e-03f);
-        s = __builtin_fmaf(s, x2, 8.54832702e-03f);
-        s = __builtin_fmaf(s, x2, -4.1791821e-02f);
-        s = __builtin_fmaf(s, x2, 1.61197024e-01f);
-        s = __builtin_fmaf(s, x2, -4.51351667e-01f);
-        s = __builtin_fmaf(s, x2, 7.52252778e-01f);
-        return (s * x2) * x;
-    }
-    float erfv = 1.0f - snb_erfc_core(x, ex);
-    return erfv - (x * SNB_TWO_OVER_SQRT_PI) * ex;
-}
-
 // 1/sqrt(x), x > 0 normal: seed 0x5f375a86 - (bits >> 1) (Lomont, "Fast
 // inverse square root", 2003), three Newton steps y <- y (1.5 - (x/2) y^2).
 SNB_HD float snb_rsqrtf(float x)
@@ -105,6 +83,64 @@ SNB_HD float snb_rsqrtf(float x)
     y = y * __builtin_fmaf(-(h * y), y, 1.5f);
     y = y * __builtin_fmaf(-(h * y), y, 1.5f);
     return y;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 pieces (energy steps and the exclusion correction)
+// ---------------------------------------------------------------------------
+SNB_HD double snb_f64_from_bits(uint64_t u) { return __builtin_bit_cast(double, u); }
+
+// exp(y), y in [-700, 0]: Cody-Waite y = n ln2 + r with ln2 split into a
+// 33-bit head and a tail, Taylor polynomial of degree 13 for e^r on
+// |r| <= ln2/2 (truncation error < 5e-18), 2^n in the exponent field.
+SNB_HD double snb_exp_d(double y)
+{
+    y = __builtin_fmax(y, -700.0);
+    double n = __builtin_rint(y * 1.4426950408889634);
+    double r = __builtin_fma(-n, 0.6931471806019545, y);
+    r = __builtin_fma(-n, -4.2009150726810846e-11, r);
+    double p = 1.0 / 6227020800.0;               // 1/13!
+    p = __builtin_fma(p, r, 1.0 / 479001600.0);  // 1/12!
+    p = __builtin_fma(p, r, 1.0 / 39916800.0);
+    p = __builtin_fma(p, r, 1.0 / 3628800.0);
+    p = __builtin_fma(p, r, 1.0 / 362880.0);
+    p = __builtin_fma(p, r, 1.0 / 40320.0);
+    p = __builtin_fma(p, r, 1.0 / 5040.0);
+    p = __builtin_fma(p, r, 1.0 / 720.0);
+    p = __builtin_fma(p, r, 1.0 / 120.0);
+    p = __builtin_fma(p, r, 1.0 / 24.0);
+    p = __builtin_fma(p, r, 1.0 / 6.0);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    int64_t ni = (int64_t)n;
+    return p * snb_f64_from_bits((uint64_t)(ni + 1023) << 52);
+}
+
+// erf(x) and h(x) = erf(x) - (2/sqrt(pi)) x e^{-x^2} for 0 <= x <= 4.9 from
+// the all-positive series erf(x) = (2/sqrt(pi)) x e^{-x^2}
+// sum_{n>=0} (2x^2)^n / (1*3*...*(2n+1)) (Abramowitz & Stegun 7.1.6): the
+// n >= 1 part is h, so neither quantity suffers cancellation.
+SNB_HD void snb_erf_parts_d(double x, double* erf_out, double* h_out)
+{
+    double g = 1.1283791670955126 * x * snb_exp_d(-(x * x));
+    double a = 2.0 * x * x;
+    double t = 1.0, s = 0.0;
+    for (int n = 1; n < 80; n++)
+    {
+        t = t * (a / (double)(2 * n + 1));
+        s = s + t;
+        if (t <= 1e-17 * s) break;
+    }
+    *h_out = g * s;
+    *erf_out = g * s + g;
+}
+
+// round(d * 2^32) to nearest-even for |d| < 2^19.
+SNB_HD int64_t snb_to_fixed_d(double d)
+{
+    double v = __builtin_fma(d, SNB_FIXED_SCALE, 6755399441055744.0);
+    return (int64_t)__builtin_bit_cast(uint64_t, v) - (int64_t)0x4338000000000000LL;
 }
 
 // Triangular LJ type-pair index (Lennard_Jones_force.h:18-27).

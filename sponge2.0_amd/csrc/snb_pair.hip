@@ -82,8 +82,18 @@ __global__ void __launch_bounds__(256) k_lj_direct(
             atomic_add_i64(fx + 2 * n + j, -iz);
             if (NEED_ENERGY)
             {
-                elj += snb_to_fixed(((A * 0.083333333f) * rinv6 - B * 0.166666667f) * rinv6);
-                ec += snb_to_fixed((qq * erfcv) * rinv);
+                // energies in FP64 from the (exact) float displacement
+                double ddx = dx, ddy = dy, ddz = dz;
+                double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                double rd = __builtin_sqrt(r2d);
+                double rid = 1.0 / rd;
+                double ri2 = rid * rid;
+                double ri6 = (ri2 * ri2) * ri2;
+                elj += snb_to_fixed_d((((double)A * (1.0 / 12.0)) * ri6 - (double)B * (1.0 / 6.0)) *
+                                      ri6);
+                double erfd, hd;
+                snb_erf_parts_d((double)beta * rd, &erfd, &hd);
+                ec += snb_to_fixed_d((((double)ai.w * (double)aj.w) * (1.0 - erfd)) * rid);
             }
             if (NEED_VIRIAL)
             {
@@ -160,8 +170,9 @@ int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBo
 
 // MPI_PME_Excluded_Force_With_Atom_Energy_Correction (PM_force.cpp:1743-1814)
 // with exclude_factor 1: for j in excl(i), f = qq (erf(x) - 2x/sqrt(pi)
-// e^{-x^2}) r^-3 dr on i and -f on j, energy -qq erf(x)/r and virial to i.
-// One thread per atom i; i-sums in registers, j-terms by integer atomics.
+// e^{-x^2}) r^-3 dr on i and -f on j, energy -qq erf(x)/r and virial to i,
+// evaluated in FP64.  One thread per atom i; i-sums in registers, j-terms by
+// integer atomics (a few per atom).
 template <bool NEED_ENERGY, bool NEED_VIRIAL>
 __global__ void __launch_bounds__(256) k_excl(int n, const float4* __restrict__ atoms,
                                               SnbBox box, float beta,
@@ -186,35 +197,33 @@ __global__ void __launch_bounds__(256) k_excl(int n, const float4* __restrict__ 
         float4 aj = atoms[j];
         float dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
         pbc_disp(dx, dy, dz, box);
-        float r2 = (dx * dx + dy * dy) + dz * dz;
-        float rinv = snb_rsqrtf(r2);
-        float r = r2 * rinv;
-        float x = beta * r;
-        float ex = snb_expf(-(x * x));
-        float h = snb_erf_minus_gauss(x, ex);
-        float qq = ai.w * aj.w;
-        float fa = (qq * h) * ((rinv * rinv) * rinv);
-        float fxv = fa * dx, fyv = fa * dy, fzv = fa * dz;
-        int64_t ix = snb_to_fixed(fxv), iy = snb_to_fixed(fyv), iz = snb_to_fixed(fzv);
+        // FP64 from the float displacement: the erf part of an excluded pair is
+        // a small difference of large terms in float arithmetic
+        double ddx = dx, ddy = dy, ddz = dz;
+        double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
+        double rd = __builtin_sqrt(r2d);
+        double rid = 1.0 / rd;
+        double erfd, hd;
+        snb_erf_parts_d((double)beta * rd, &erfd, &hd);
+        double qq = (double)ai.w * (double)aj.w;
+        double fa = (qq * hd) * ((rid * rid) * rid);
+        double fxv = fa * ddx, fyv = fa * ddy, fzv = fa * ddz;
+        int64_t ix = snb_to_fixed_d(fxv), iy = snb_to_fixed_d(fyv), iz = snb_to_fixed_d(fzv);
         sx += ix;
         sy += iy;
         sz += iz;
         atomic_add_i64(fx + j, -ix);
         atomic_add_i64(fx + n + j, -iy);
         atomic_add_i64(fx + 2 * n + j, -iz);
-        if (NEED_ENERGY)
-        {
-            float erfv = h + (x * SNB_TWO_OVER_SQRT_PI) * ex;
-            e += snb_to_fixed(-((qq * erfv) * rinv));
-        }
+        if (NEED_ENERGY) e += snb_to_fixed_d(-((qq * erfd) * rid));
         if (NEED_VIRIAL)
         {
-            v0 += snb_to_fixed(-(fxv * dx));
-            v1 += snb_to_fixed(-(fxv * dy + fyv * dx));
-            v2 += snb_to_fixed(-(fyv * dy));
-            v3 += snb_to_fixed(-(fxv * dz + fzv * dx));
-            v4 += snb_to_fixed(-(fyv * dz + fzv * dy));
-            v5 += snb_to_fixed(-(fzv * dz));
+            v0 += snb_to_fixed_d(-(fxv * ddx));
+            v1 += snb_to_fixed_d(-(fxv * ddy + fyv * ddx));
+            v2 += snb_to_fixed_d(-(fyv * ddy));
+            v3 += snb_to_fixed_d(-(fxv * ddz + fzv * ddx));
+            v4 += snb_to_fixed_d(-(fyv * ddz + fzv * ddy));
+            v5 += snb_to_fixed_d(-(fzv * ddz));
         }
     }
     atomic_add_i64(fx + i, sx);

@@ -72,6 +72,8 @@ _SIGS = {
     "snb_pme_energy": ([_P] + [ctypes.POINTER(ctypes.c_double)] * 4, _I),
     "snb_pme_reciprocal_virial": ([_P, _P], _I),
     "snb_flush_forces": ([_P, _P], _I),
+    "snb_flush_atom_energy": ([_P, _P, _P], _I),
+    "snb_flush_atom_virial": ([_P, _P], _I),
     "snb_copy_fixed_forces": ([_P, _P], _I),
     "snb_clear_fixed_forces": ([_P], _I),
     "snb_copy_fixed_atom_terms": ([_P, _I, _P], _I),
@@ -265,6 +267,13 @@ class Nonbonded:
         self.reciprocal(step, need_energy, need_virial, frc=not fused)
         if fused:
             self.flush()
+            if need_energy:
+                check(self.L.snb_flush_atom_energy(self.ctx, _ptr(self.atom_energy),
+                                                   _ptr(self.direct_energy)),
+                      "snb_flush_atom_energy")
+            if need_virial:
+                check(self.L.snb_flush_atom_virial(self.ctx, _ptr(self.atom_virial)),
+                      "snb_flush_atom_virial")
 
     # -- read-back helpers ----------------------------------------------------
     def sync(self):

@@ -131,18 +131,24 @@ def _chain_walk(rng, n_beads, box, bond, placed_tree_pts, min_dist=3.0, angle_de
     except its three predecessors."""
     from scipy.spatial import cKDTree
 
-    pts = []
     cos_a = np.cos(np.deg2rad(180.0 - angle_deg))
     others = list(placed_tree_pts)
     tree = cKDTree(np.array(others) % box, boxsize=box) if others else None
-    start = rng.uniform(0, box, 3)
-    pts.append(start)
-    d = rng.normal(size=3)
-    d /= np.linalg.norm(d)
-    pts.append(start + bond * d)
-    recent = []
-    for k in range(2, n_beads):
-        for _attempt in range(200):
+
+    def clear_of_others(p):
+        return tree is None or tree.query(p % box)[0] >= min_dist
+
+    while True:
+        start = rng.uniform(0, box, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        if clear_of_others(start) and clear_of_others(start + bond * d):
+            break
+    pts = [start, start + bond * d]
+    failures = 0
+    while len(pts) < n_beads:
+        placed = False
+        for _attempt in range(100):
             prev = pts[-1] - pts[-2]
             prev /= np.linalg.norm(prev)
             # random direction at the fixed angle to the previous bond
@@ -150,17 +156,21 @@ def _chain_walk(rng, n_beads, box, bond, placed_tree_pts, min_dist=3.0, angle_de
             perp /= np.linalg.norm(perp)
             new_dir = cos_a * prev + np.sqrt(1 - cos_a ** 2) * perp
             cand = pts[-1] + bond * new_dir
-            ok = True
-            own = np.array(pts[:-3]) if len(pts) > 3 else None
-            if own is not None and len(own):
-                dd = (own - cand + box / 2) % box - box / 2
+            if len(pts) > 3:
+                dd = (np.array(pts[:-3]) - cand + box / 2) % box - box / 2
                 if np.min(np.einsum("ij,ij->i", dd, dd)) < min_dist ** 2:
-                    ok = False
-            if ok and tree is not None and tree.query(cand % box)[0] < min_dist:
-                ok = False
-            if ok:
-                break
-        pts.append(cand)
+                    continue
+            if not clear_of_others(cand):
+                continue
+            pts.append(cand)
+            placed = True
+            break
+        if not placed:
+            # trapped: back off a few beads and regrow
+            failures += 1
+            if failures > 1000:
+                raise RuntimeError("chain growth failed")
+            del pts[max(2, len(pts) - 5):]
     return np.array(pts)
 
 
@@ -207,8 +217,26 @@ def mixed_system(n_waters=30000, n_chains=3, beads_per_chain=1000, box=96.5,
             for d in (1, 2, 3):
                 if k + d < beads_per_chain:
                     pairs.append((b0 + k, b0 + k + d))
-    return _finish(crd, charge, ltype, sig, eps, mass, pairs, box, cutoff,
-                   n_waters=n_waters, n_beads=nb)
+    out = _finish(crd, charge, ltype, sig, eps, mass, pairs, box, cutoff,
+                  n_waters=n_waters, n_beads=nb)
+    # No LJ site (water O, bead) within 2.2 A of another non-excluded LJ site,
+    # and no two atoms of different molecules within 0.6 A (random water
+    # orientations on the lattice can bring two H's to ~0.8 A; H carries no LJ).
+    mol = np.concatenate([np.repeat(np.arange(n_waters), 3),
+                          n_waters + np.arange(nb) // beads_per_chain])
+    pos = np.concatenate([np.zeros(nw, int), np.arange(nb) % beads_per_chain])
+    lj_site = np.concatenate([np.tile([True, False, False], n_waters), np.ones(nb, bool)])
+    c = out["crd"] % box
+    t = cKDTree(c, boxsize=box)
+    close = t.query_pairs(2.2, output_type="ndarray")
+    a, b = close[:, 0], close[:, 1]
+    excluded = (mol[a] == mol[b]) & ((a < nw) | (np.abs(pos[a] - pos[b]) <= 3))
+    d = (c[a] - c[b] + box / 2) % box - box / 2
+    r = np.sqrt(np.einsum("ij,ij->i", d, d))
+    bad = ~excluded & ((lj_site[a] & lj_site[b]) | (r < 0.6))
+    if bad.any():
+        raise RuntimeError("generator produced overlapping atoms")
+    return out
 
 
 def config_cpu_9k(seed=20260217):

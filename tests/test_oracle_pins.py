@@ -287,13 +287,21 @@ def test_twin_math_error_bounds():
     r2 = np.geomspace(0.5, 200.0, 400).astype(np.float32)
     gr = np.array([lib_call(oracle.lib().orc_twin_rsqrtf, v) for v in r2])
     assert np.max(np.abs(gr * np.sqrt(r2.astype(np.float64)) - 1)) < 3e-7
-    xs = np.linspace(0.0, 2.0, 401).astype(np.float32)
-    gh = np.array([lib_call(oracle.lib().orc_twin_erf_minus_gauss, v) for v in xs])
+    # FP64 pieces of the energy / exclusion path
+    import ctypes
     from scipy.special import erf
-    xd = xs.astype(np.float64)
-    hr = erf(xd) - 2 / np.sqrt(np.pi) * xd * np.exp(-xd * xd)
-    nz = xd > 0.05
-    assert np.max(np.abs(gh[nz] - hr[nz]) / hr[nz]) < 2e-6
+    L = oracle.lib()
+    for xv in np.linspace(0.0, 4.5, 181):
+        e_, h_ = ctypes.c_double(), ctypes.c_double()
+        L.orc_twin_erf_parts(float(xv), ctypes.byref(e_), ctypes.byref(h_))
+        ref_e = erf(xv)
+        ref_h = ref_e - 2 / np.sqrt(np.pi) * xv * np.exp(-xv * xv)
+        assert abs(e_.value - ref_e) <= 2e-15 * ref_e + 1e-300
+        # (the scipy reference itself cancels for small x: absolute bound there)
+        if xv > 0:
+            assert abs(h_.value - ref_h) <= 1e-14 * ref_h + 4e-17
+    for yv in np.linspace(-40.0, 0.0, 161):
+        assert abs(L.orc_twin_exp_d(float(yv)) - np.exp(yv)) <= 3e-16 * np.exp(yv)
 
 
 def lib_call(fn, v):
