@@ -113,6 +113,27 @@ int snb_nl_device_list(snb_context* ctx, snb_atom_group** d_nl, int** d_serial_t
  * (int[atom_numbers][max_neighbor_numbers]) of the public half list into
  * caller device buffers (the reference's h_nl readback, neighbor_list.h:42). */
 int snb_nl_copy_rows(snb_context* ctx, int* d_counts, int* d_rows);
+/* Whether snb_nl_update also writes the public list above (default 1, the
+ * reference's behaviour).  The library's LJ kernel does not read it: with
+ * every axis >= 5 cells it consumes an internal cell-ordered list of 16-bit
+ * window slots with the same pair set.  Callers that never read d_nl may
+ * switch the export off (no counterpart in the reference). */
+int snb_nl_set_export(snb_context* ctx, int enable);
+/* The internal list in atom ids: d_counts[i] and the row
+ * d_rows[i * max_neighbor_numbers ..] of every pair whose internal row owner
+ * is atom i (each pair once; owner = the atom whose cell precedes by the
+ * forward cell rule).  Equals the public list when the windowed path is off. */
+int snb_nl_copy_internal(snb_context* ctx, int* d_counts, int* d_rows);
+/* Per-atom energy and virial of a pair (atom_energy, atom_direct_cf_energy,
+ * atom_virial of snb_lj_pme_direct_force): by default they go to the lower atom
+ * id, the reference's row atom, so the per-atom arrays equal the reference's.
+ * enable = 1 keeps them with the windowed list's row owner instead: the same
+ * totals with fewer atomics on energy/virial steps (no counterpart in the
+ * reference). */
+int snb_lj_set_owner_attribution(snb_context* ctx, int enable);
+/* Geometry of the windowed pair path (windowed = 0 when not used). */
+int snb_nl_window_info(snb_context* ctx, int* windowed, int* a, int* b, int* s, int* wcap,
+                       int* nblock);
 /* Cell grid shape chosen by snb_nl_initial (NEIGHBOR_LIST::GRIDS Nx/Ny/Nz). */
 int snb_nl_grid_shape(snb_context* ctx, int* nx, int* ny, int* nz);
 

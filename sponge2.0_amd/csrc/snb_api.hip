@@ -111,7 +111,63 @@ void free_nl(SnbNl& L)
     dfree(L.dnl);
     dfree(L.need);
     dfree(L.old_crd);
+    dfree(L.order);
+    dfree(L.sorted_cell);
+    dfree(L.nl16);
+    dfree(L.row_count);
+    dfree(L.win_size);
+    dfree(L.stage_base);
+    dfree(L.stage);
+    dfree(L.inv_count);
+    dfree(L.inv_list);
+    dfree(L.pub_count);
+    L.windowed = false;
     L.init = false;
+}
+
+// Chooses the block of the windowed pair path (snb_window.h): the largest
+// owned block whose expected window fills at most 75% of the LDS capacity.
+// Needs >= 5 cells on every axis (unique forward displacements).
+bool choose_window(int nx, int ny, int nz, double atoms_per_cell, int max_lds, SnbWin* W)
+{
+    if (nx < 5 || ny < 5 || nz < 5) return false;
+    // LDS per slot: 3 x int64 accumulator + int slot map; plus two range tables
+    int wcap = (max_lds - (int)sizeof(int) * (2 * SNB_WIN_MAX_RANGES + 1)) / 28;
+    if (wcap > 65535) wcap = 65535;
+    static const int cand[][3] = {{2, 2, 8}, {2, 2, 6}, {2, 2, 4}, {2, 2, 3}, {2, 2, 2},
+                                  {1, 2, 4}, {1, 1, 4}, {1, 1, 2}, {1, 1, 1}};
+    for (const auto& c : cand)
+    {
+        int a = c[0], b = c[1], s = c[2];
+        if (a > nx - 2 || b > ny - 4 || s > nz - 4) continue;
+        if (2 * (a + 2) * (b + 4) > SNB_WIN_MAX_RANGES) continue;
+        double expect = (double)(a + 2) * (b + 4) * (s + 4) * atoms_per_cell;
+        if (expect > 0.75 * wcap) continue;
+        W->nx = nx;
+        W->ny = ny;
+        W->nz = nz;
+        W->a = a;
+        W->b = b;
+        W->s = s;
+        W->nbx = (nx + a - 1) / a;
+        W->nby = (ny + b - 1) / b;
+        W->nbz = (nz + s - 1) / s;
+        W->nblock = W->nbx * W->nby * W->nbz;
+        W->wu = a + 2;
+        W->wv = b + 4;
+        W->wz = s + 4;
+        W->nrange = 2 * W->wu * W->wv;
+        W->wcap = wcap;
+        return true;
+    }
+    return false;
+}
+
+// blocks whose window can contain one cell, per axis (+1 when a partial last
+// block makes origins closer than the block size across the wrap)
+int window_cover(int extent, int size, int n)
+{
+    return (extent + size - 1) / size + (n % size ? 1 : 0);
 }
 
 void free_pme(SnbPme& P)
@@ -202,6 +258,13 @@ int snb_context_create(snb_context** out, int device, void* hip_stream, int atom
     DeviceGuard g(device);
     snb_context* c = new snb_context();
     c->device = device;
+    {
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) ==
+                hipSuccess &&
+            lds > 0)
+            c->max_lds = lds;
+    }
     c->stream = (hipStream_t)hip_stream;
     c->n = atom_numbers;
     int n = atom_numbers;
@@ -212,6 +275,10 @@ int snb_context_create(snb_context** out, int device, void* hip_stream, int atom
     if (!rc) rc = dalloc(&c->fx, 3 * (size_t)n);
     if (!rc) rc = dalloc(&c->e_atom, 3 * (size_t)n);
     if (!rc) rc = dalloc(&c->vir_atom, 6 * (size_t)n);
+    if (!rc) rc = dalloc(&c->satom, n);
+    if (!rc) rc = dalloc(&c->stype, n);
+    if (!rc) rc = dalloc(&c->e_sorted, 2 * (size_t)n);
+    if (!rc) rc = dalloc(&c->v_sorted, 6 * (size_t)n);
     if (!rc) rc = dalloc(&c->scal, SNB_S_COUNT);
     if (!rc) rc = dalloc(&c->dscal, SNB_D_COUNT);
     if (rc)
@@ -247,6 +314,10 @@ int snb_context_destroy(snb_context* c)
     dfree(c->fx);
     dfree(c->e_atom);
     dfree(c->vir_atom);
+    dfree(c->satom);
+    dfree(c->stype);
+    dfree(c->e_sorted);
+    dfree(c->v_sorted);
     dfree(c->scal);
     dfree(c->dscal);
     dfree(c->ex_start);
@@ -368,10 +439,34 @@ int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
     if (!rc) rc = dalloc(&L.dnl, n);
     if (!rc) rc = dalloc(&L.need, 1);
     if (!rc) rc = dalloc(&L.old_crd, n);
+    if (!rc) rc = dalloc(&L.order, n);
+    if (!rc) rc = dalloc(&L.sorted_cell, n);
     if (rc)
     {
         free_nl(L);
         return rc;
+    }
+    L.windowed = choose_window(L.nx, L.ny, L.nz, (double)n / L.ncell, c->max_lds, &L.win);
+    if (L.windowed)
+    {
+        const SnbWin& W = L.win;
+        L.invmax = window_cover(W.wu, W.a, W.nx) * window_cover(W.wv, W.b, W.ny) *
+                   window_cover(W.wz, W.s, W.nz);
+        if (!rc) rc = dalloc(&L.nl16, (size_t)n * L.max_nb);
+        if (!rc) rc = dalloc(&L.row_count, n);
+        if (!rc) rc = dalloc(&L.win_size, W.nblock + 1);
+        if (!rc) rc = dalloc(&L.stage_base, W.nblock + 1);
+        // every atom sits in at most invmax windows
+        if (!rc) rc = dalloc(&L.stage, 3 * (size_t)n * L.invmax);
+        if (!rc) rc = dalloc(&L.inv_count, n);
+        if (!rc) rc = dalloc(&L.inv_list, (size_t)n * L.invmax);
+        if (!rc) rc = dalloc(&L.pub_count, n);
+        if (rc)
+        {
+            free_nl(L);
+            return rc;
+        }
+        snb_launch_fill_int(c->stream, L.row_count, n, 0);
     }
     // ATOM_GROUP rows point into the padded table (neighbor_list.cpp:764-771)
     std::vector<snb_atom_group> h(n > 0 ? n : 1);
@@ -439,6 +534,45 @@ int snb_nl_copy_rows(snb_context* c, int* d_counts, int* d_rows)
     SNB_CHECK(hipMemcpyAsync(d_rows, c->nl.serial, sizeof(int) * (size_t)c->n * c->nl.max_nb,
                              hipMemcpyDeviceToDevice, c->stream));
     return SNB_OK;
+}
+
+int snb_lj_set_owner_attribution(snb_context* c, int enable)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    c->lj.owner_attribution = enable != 0;
+    return SNB_OK;
+}
+
+int snb_nl_set_export(snb_context* c, int enable)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    c->nl.export_public = enable != 0;
+    return SNB_OK;
+}
+
+int snb_nl_window_info(snb_context* c, int* windowed, int* a, int* b, int* s, int* wcap,
+                       int* nblock)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->nl.init) return SNB_ERR_NOT_INITIALIZED;
+    const SnbWin& W = c->nl.win;
+    bool on = c->nl.windowed;
+    if (windowed) *windowed = on ? 1 : 0;
+    if (a) *a = on ? W.a : 0;
+    if (b) *b = on ? W.b : 0;
+    if (s) *s = on ? W.s : 0;
+    if (wcap) *wcap = on ? W.wcap : 0;
+    if (nblock) *nblock = on ? W.nblock : 0;
+    return SNB_OK;
+}
+
+int snb_nl_copy_internal(snb_context* c, int* d_counts, int* d_rows)
+{
+    if (!c || !d_counts || !d_rows) return SNB_ERR_ARGUMENT;
+    if (!c->nl.init) return SNB_ERR_NOT_INITIALIZED;
+    DeviceGuard g(c->device);
+    if (!c->nl.windowed) return snb_nl_copy_rows(c, d_counts, d_rows);
+    return snb_launch_nl_copy_internal(c, d_counts, d_rows);
 }
 
 int snb_nl_grid_shape(snb_context* c, int* nx, int* ny, int* nz)
@@ -511,15 +645,23 @@ int snb_lj_pme_direct_force(snb_context* c, int atom_numbers, int local_atom_num
     if (!d_crd || !d_charge || !d_nl) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
     SnbBox box = snb_box(cell, rcell);
-    int rc = snb_launch_pack_atoms(c, d_crd, d_charge);
-    if (rc) return rc;
+    int rc;
     if (need_atom_energy)
     {
         // the per-atom LJ and direct arrays restart each energy call
         // (Lennard_Jones_force.cpp:376-382)
         if ((rc = snb_launch_fill_i64(c->stream, c->e_atom, 2 * (size_t)c->n))) return rc;
     }
-    rc = snb_launch_lj_direct(c, d_nl, box, pme_beta, need_atom_energy, need_virial);
+    // The library's own list goes through the windowed kernel (same pair set,
+    // internal row layout); any other list through the row kernel.
+    if (c->nl.init && c->nl.windowed && c->nl.built && d_nl == c->nl.dnl)
+        rc = snb_launch_lj_window(c, d_crd, d_charge, box, pme_beta, need_atom_energy,
+                                  need_virial);
+    else
+    {
+        if ((rc = snb_launch_pack_atoms(c, d_crd, d_charge))) return rc;
+        rc = snb_launch_lj_direct(c, d_nl, box, pme_beta, need_atom_energy, need_virial);
+    }
     if (rc) return rc;
     if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
     if (need_atom_energy)

@@ -202,9 +202,9 @@ __global__ void k_charge_sums(int n, const float* __restrict__ q, int64_t* __res
     int64_t s2 = 0, s1 = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
-        float v = q[i];
-        s2 += snb_to_fixed(v * v);
-        s1 += snb_to_fixed(v);
+        double v = q[i];
+        s2 += snb_to_fixed_d(v * v);  // exact product of two floats
+        s1 += snb_to_fixed_d(v);
     }
     for (int off = 32; off > 0; off >>= 1)
     {

@@ -5,6 +5,7 @@
 
 #include "../../include/sponge_nb.h"
 #include "snb_math.h"
+#include "snb_window.h"
 
 // Scalar accumulator slots (int64 fixed point, device, zeroed per call that
 // produces them).
@@ -52,10 +53,26 @@ struct SnbNl
     int* atom_slot = nullptr;    // n
     int* sorted_id = nullptr;    // n (scatter order)
     float4* sorted_crd = nullptr;  // n: x, y, z, bits(atom id), sorted by cell then id
+    int* order = nullptr;        // n: atom id at sorted position s
+    int* sorted_cell = nullptr;  // n: cell of sorted position s
     int* serial = nullptr;       // n * max_nb
     snb_atom_group* dnl = nullptr;  // n
     int* need = nullptr;         // rebuild flag (device)
     snb_vector* old_crd = nullptr;
+    // windowed pair path (every axis >= 5 cells)
+    bool windowed = false;
+    bool export_public = true;   // also write the reference-layout list at rebuild
+    SnbWin win;
+    uint16_t* nl16 = nullptr;    // n * max_nb window slots, rows by sorted position
+    int* row_count = nullptr;    // n
+    int* win_size = nullptr;     // nblock + 1
+    int* stage_base = nullptr;   // nblock + 1 (exclusive scan of win_size)
+    long long* stage = nullptr;  // nblock * wcap * 3
+    int* inv_count = nullptr;    // n
+    int* inv_list = nullptr;     // n * invmax: stage slots of each sorted atom
+    int invmax = 0;
+    int* pub_count = nullptr;    // n (export scratch)
+    size_t win_lds_set = 0;      // dynamic LDS granted to the pair kernel
 };
 
 struct SnbLj
@@ -68,6 +85,9 @@ struct SnbLj
     double c6_total = 0.0;
     float long_range_factor = 0.0f;
     double last_long_energy = 0.0;
+    // per-atom energy/virial of a pair: false = the lower atom id (the
+    // reference's row atom), true = the windowed list's row owner (cheaper)
+    bool owner_attribution = false;
 };
 
 struct SnbPme
@@ -119,6 +139,11 @@ struct snb_context
     int n = 0;
     float4* atom4 = nullptr;  // x, y, z, q
     int* type = nullptr;
+    float4* satom = nullptr;     // x, y, z, q in sorted (cell) order
+    int* stype = nullptr;        // LJ type in sorted order
+    int64_t* e_sorted = nullptr; // 2n: LJ | direct energy of row owners, sorted order
+    int64_t* v_sorted = nullptr; // 6n: virial of row owners, sorted order
+    int max_lds = 65536;         // dynamic LDS bytes per workgroup the device allows
     int* status = nullptr;
     int64_t* fx = nullptr;       // 3n, SoA
     int64_t* e_atom = nullptr;   // 3n: LJ | direct | correction
@@ -187,6 +212,13 @@ int snb_launch_lj_long_range(hipStream_t s, snb_ltmatrix3* vir, float* pot, floa
 // neighbor list (snb_nl.hip)
 int snb_launch_nl_update(snb_context* ctx, const snb_vector* crd, const SnbBox& box, int forced,
                          int step, const int* ex_start, const int* ex_list, const int* ex_num);
+
+// windowed path (snb_nlwin.hip, snb_pair.hip)
+int snb_launch_nl_window(snb_context* ctx, const SnbBox& box, const int* ex_start,
+                         const int* ex_list, const int* ex_num);
+int snb_launch_nl_copy_internal(snb_context* ctx, int* counts, int* rows);
+int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* charge,
+                         const SnbBox& box, float beta, int need_energy, int need_virial);
 
 // pair kernel (snb_pair.hip)
 int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBox& box,

@@ -87,15 +87,17 @@ __global__ void k_nl_scatter(const int* __restrict__ need, int n, const int* __r
 }
 
 // One wave per cell: rank the cell's atoms by id (O(m^2) with m ~ 20-60) and
-// write ids + coordinates in id order.  Also flags cells above the
-// reference's per-cell capacity (max_atom_in_grid_numbers) for parity of the
-// overflow report; nothing is truncated here.
+// write ids, cell numbers and coordinates in id order.  Also flags cells above
+// the reference's per-cell capacity (max_atom_in_grid_numbers) for parity of
+// the overflow report; nothing is truncated here.
 #define SNB_NL_SORT_MAX 1024
 __global__ void __launch_bounds__(64) k_nl_cell_sort(const int* __restrict__ need, int ncell,
                                                      const int* __restrict__ start,
                                                      const int* __restrict__ sorted_id,
                                                      const snb_vector* __restrict__ crd,
                                                      float4* __restrict__ sorted_crd,
+                                                     int* __restrict__ order,
+                                                     int* __restrict__ sorted_cell,
                                                      int max_in_cell, int* __restrict__ status)
 {
     if (need[0] == 0) return;
@@ -107,12 +109,14 @@ __global__ void __launch_bounds__(64) k_nl_cell_sort(const int* __restrict__ nee
     if (m > max_in_cell && lane == 0) atomicOr(status, SNB_FLAG_CELL_OVERFLOW);
     if (m > SNB_NL_SORT_MAX)
     {
-        // keep scatter order (still a correct set); flag it
+        // keep scatter order (still a correct set)
         for (int k = lane; k < m; k += 64)
         {
             int id = sorted_id[b + k];
             snb_vector r = crd[id];
             sorted_crd[b + k] = make_float4(r.x, r.y, r.z, __int_as_float(id));
+            order[b + k] = id;
+            sorted_cell[b + k] = c;
         }
         return;
     }
@@ -125,6 +129,8 @@ __global__ void __launch_bounds__(64) k_nl_cell_sort(const int* __restrict__ nee
         for (int t = 0; t < m; t++) rank += ids[t] < id;
         snb_vector r = crd[id];
         sorted_crd[b + rank] = make_float4(r.x, r.y, r.z, __int_as_float(id));
+        order[b + rank] = id;
+        sorted_cell[b + rank] = c;
     }
 }
 
@@ -286,11 +292,18 @@ int snb_launch_nl_update(snb_context* ctx, const snb_vector* crd, const SnbBox& 
     hipLaunchKernelGGL(k_nl_scatter, dim3((n + 255) / 256), dim3(256), 0, s, L.need, n,
                        L.atom_cell, L.atom_slot, L.cell_start, L.sorted_id);
     hipLaunchKernelGGL(k_nl_cell_sort, dim3(L.ncell), dim3(64), 0, s, L.need, L.ncell,
-                       L.cell_start, L.sorted_id, crd, L.sorted_crd, L.max_in_cell,
-                       ctx->status);
-    hipLaunchKernelGGL(k_nl_search, dim3((n + 3) / 4), dim3(256), 0, s, L.need, n, crd,
-                       L.atom_cell, L.cell_start, L.sorted_crd, box, L.cs2, L.nx, L.ny, L.nz,
-                       ex_start, ex_list, ex_num, L.max_nb, L.serial, L.dnl, ctx->status);
+                       L.cell_start, L.sorted_id, crd, L.sorted_crd, L.order, L.sorted_cell,
+                       L.max_in_cell, ctx->status);
+    if (L.windowed)
+    {
+        rc = snb_launch_nl_window(ctx, box, ex_start, ex_list, ex_num);
+        if (rc) return rc;
+    }
+    else
+        hipLaunchKernelGGL(k_nl_search, dim3((n + 3) / 4), dim3(256), 0, s, L.need, n, crd,
+                           L.atom_cell, L.cell_start, L.sorted_crd, box, L.cs2, L.nx, L.ny,
+                           L.nz, ex_start, ex_list, ex_num, L.max_nb, L.serial, L.dnl,
+                           ctx->status);
     snb_prof_end(ctx, SNB_PROF_NL);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;

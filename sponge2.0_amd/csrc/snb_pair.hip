@@ -265,3 +265,365 @@ int snb_launch_excl(snb_context* ctx, const float4* atoms, const SnbBox& box, fl
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
+
+// ---------------------------------------------------------------------------
+// windowed path
+// ---------------------------------------------------------------------------
+// Packs coordinates and charges in sorted (cell) order: satom[s] = (crd, q) of
+// atom order[s]; raises the non-finite flag.
+__global__ void k_pack_sorted(int n, const int* __restrict__ order,
+                              const snb_vector* __restrict__ crd, const float* __restrict__ charge,
+                              float4* __restrict__ satom, int* __restrict__ status)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int id = order[s];
+    snb_vector r = crd[id];
+    satom[s] = make_float4(r.x, r.y, r.z, charge[id]);
+    if (!(__builtin_isfinite(r.x) && __builtin_isfinite(r.y) && __builtin_isfinite(r.z)))
+        atomicOr(status, SNB_FLAG_NONFINITE_CRD);
+}
+
+// One workgroup per block of a x b columns x S z-cells (snb_window.h).  The
+// workgroup keeps int64 force accumulators for every atom of its window in
+// LDS; a wavefront evaluates one row at a time (64 lanes over its entries),
+// adds -f to the j slot (ds_add_u64) and, after a wave reduction, +sum f to
+// the row's own slot.  At the end the accumulators go to the stage buffer;
+// k_stage_gather sums each atom's stage entries.  Windows larger than the LDS
+// capacity fall back to integer atomics on the global accumulators.
+// Per-atom energies and virials go, as in the reference, to the lower atom id
+// of the pair: row-owner sums when the owner is the lower id, integer atomics
+// on the id-ordered arrays otherwise (energy/virial steps only); with
+// ref_attr == 0 every pair's terms stay with the row owner.
+template <bool NEED_ENERGY, bool NEED_VIRIAL>
+__global__ void __launch_bounds__(1024) k_pair_win(
+    int n, SnbWin W, const int* __restrict__ start, const int* __restrict__ sorted_cell,
+    const int* __restrict__ row_count, const uint16_t* __restrict__ nl16, int max_nb,
+    const float4* __restrict__ satom, const int* __restrict__ stype,
+    const int* __restrict__ order, const float* __restrict__ lj_a,
+    const float* __restrict__ lj_b, SnbBox box, float rc2, float beta,
+    const int* __restrict__ stage_base, long long* __restrict__ stage, size_t stage_cap,
+    int64_t* __restrict__ fx, int64_t* __restrict__ e_sorted, int64_t* __restrict__ v_sorted,
+    int ref_attr, int64_t* __restrict__ e_atom, int64_t* __restrict__ vir)
+{
+    extern __shared__ long long smem[];
+    long long* acc = smem;                          // [3][wcap]
+    int* slot_s = (int*)(smem + 3 * (size_t)W.wcap);  // [wcap]
+    int* rfirst = slot_s + W.wcap;                  // [128]
+    int* rpref = rfirst + SNB_WIN_MAX_RANGES;       // [129]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nwave = blockDim.x >> 6;
+    const int w = blockIdx.x;
+    int x0, y0, z0;
+    snb_win_block(W, w, x0, y0, z0);
+    if (tid < SNB_WIN_MAX_RANGES)
+    {
+        int f = 0, sz = 0;
+        if (tid < W.nrange)
+        {
+            int fc, nc;
+            snb_win_range_cells(W, x0, y0, z0, tid, fc, nc);
+            f = start[fc];
+            sz = nc > 0 ? start[fc + nc] - f : 0;
+        }
+        rfirst[tid] = f;
+        rpref[tid + 1] = sz;
+    }
+    __syncthreads();
+    if (tid == 0)
+    {
+        rpref[0] = 0;
+        for (int k = 0; k < SNB_WIN_MAX_RANGES; k++) rpref[k + 1] += rpref[k];
+    }
+    __syncthreads();
+    const int total = rpref[SNB_WIN_MAX_RANGES];
+    const bool ov = total > W.wcap;
+    if (!ov)
+    {
+        for (int k = 0; k < W.nrange; k++)
+        {
+            int b0 = rpref[k], cnt = rpref[k + 1] - b0, f = rfirst[k];
+            for (int t = tid; t < cnt; t += blockDim.x) slot_s[b0 + t] = f + t;
+        }
+        for (int t = tid; t < 3 * W.wcap; t += blockDim.x) acc[t] = 0;
+    }
+    __syncthreads();
+    for (int col = 0; col < W.a * W.b; col++)
+    {
+        int ua = col / W.b, vb = col - ua * W.b;
+        if (x0 + ua >= W.nx || y0 + vb >= W.ny) continue;
+        int colbase = ((x0 + ua) * W.ny + (y0 + vb)) * W.nz;
+        int z1 = z0 + W.s < W.nz ? z0 + W.s : W.nz;
+        int r0 = start[colbase + z0], r1 = start[colbase + z1];
+        for (int si = r0 + wave; si < r1; si += nwave)
+        {
+            const int cnt = row_count[si];
+            const uint16_t* e16 = nl16 + (size_t)si * max_nb;
+            const float4 ai = satom[si];
+            const int ti = stype[si];
+            // reference attribution: a pair's energy and virial belong to the
+            // lower atom id (the reference's row atom)
+            const int id_i = (NEED_ENERGY || NEED_VIRIAL) && ref_attr ? order[si] : 0;
+            int64_t sx = 0, sy = 0, sz = 0, elj = 0, ec = 0;
+            int64_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+            for (int k = lane; k < cnt; k += 64)
+            {
+                const int slot = e16[k];
+                int sj;
+                if (!ov)
+                    sj = slot_s[slot];
+                else
+                {
+                    int lo = 0, hi = W.nrange - 1;
+                    while (lo < hi)
+                    {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (rpref[mid] <= slot)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    sj = rfirst[lo] + (slot - rpref[lo]);
+                }
+                const float4 aj = satom[sj];
+                float dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
+                pbc_disp(dx, dy, dz, box);
+                float r2 = (dx * dx + dy * dy) + dz * dz;
+                if (!(r2 < rc2)) continue;
+                const int p = snb_lj_pair_index(ti, stype[sj]);
+                const float A = lj_a[p], B = lj_b[p];
+                float rinv = snb_rsqrtf(r2);
+                float r = r2 * rinv;
+                float rinv2 = rinv * rinv;
+                float rinv6 = (rinv2 * rinv2) * rinv2;
+                float flj = ((B - A * rinv6) * rinv6) * rinv2;
+                float x = beta * r;
+                float ex = snb_expf(-(x * x));
+                float erfcv = snb_erfc_core(x, ex);
+                float qq = ai.w * aj.w;
+                float fc = (qq * (rinv2 * rinv)) * ((x * SNB_TWO_OVER_SQRT_PI) * ex + erfcv);
+                float fa = flj - fc;
+                float fxv = fa * dx, fyv = fa * dy, fzv = fa * dz;
+                int64_t ix = snb_to_fixed(fxv), iy = snb_to_fixed(fyv), iz = snb_to_fixed(fzv);
+                sx += ix;
+                sy += iy;
+                sz += iz;
+                if (!ov)
+                {
+                    atomicAdd((unsigned long long*)&acc[slot], (unsigned long long)(-ix));
+                    atomicAdd((unsigned long long*)&acc[W.wcap + slot], (unsigned long long)(-iy));
+                    atomicAdd((unsigned long long*)&acc[2 * W.wcap + slot],
+                              (unsigned long long)(-iz));
+                }
+                else
+                {
+                    int idj = order[sj];
+                    atomic_add_i64(fx + idj, -ix);
+                    atomic_add_i64(fx + n + idj, -iy);
+                    atomic_add_i64(fx + 2 * n + idj, -iz);
+                }
+                int id_j = -1;
+                if ((NEED_ENERGY || NEED_VIRIAL) && ref_attr)
+                {
+                    id_j = order[sj];
+                    if (id_j > id_i) id_j = -1;
+                }
+                if (NEED_ENERGY)
+                {
+                    double ddx = dx, ddy = dy, ddz = dz;
+                    double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                    double rd = __builtin_sqrt(r2d);
+                    double rid = 1.0 / rd;
+                    double ri2 = rid * rid;
+                    double ri6 = (ri2 * ri2) * ri2;
+                    int64_t pl = snb_to_fixed_d((((double)A * (1.0 / 12.0)) * ri6 -
+                                                 (double)B * (1.0 / 6.0)) * ri6);
+                    double erfd, hd;
+                    snb_erf_parts_d((double)beta * rd, &erfd, &hd);
+                    int64_t pc =
+                        snb_to_fixed_d((((double)ai.w * (double)aj.w) * (1.0 - erfd)) * rid);
+                    if (id_j >= 0)
+                    {
+                        atomic_add_i64(e_atom + id_j, pl);
+                        atomic_add_i64(e_atom + n + id_j, pc);
+                    }
+                    else
+                    {
+                        elj += pl;
+                        ec += pc;
+                    }
+                }
+                if (NEED_VIRIAL)
+                {
+                    int64_t p0 = snb_to_fixed(-(fxv * dx));
+                    int64_t p1 = snb_to_fixed(-(fxv * dy + fyv * dx));
+                    int64_t p2 = snb_to_fixed(-(fyv * dy));
+                    int64_t p3 = snb_to_fixed(-(fxv * dz + fzv * dx));
+                    int64_t p4 = snb_to_fixed(-(fyv * dz + fzv * dy));
+                    int64_t p5 = snb_to_fixed(-(fzv * dz));
+                    if (id_j >= 0)
+                    {
+                        atomic_add_i64(vir + id_j, p0);
+                        atomic_add_i64(vir + (size_t)n + id_j, p1);
+                        atomic_add_i64(vir + (size_t)2 * n + id_j, p2);
+                        atomic_add_i64(vir + (size_t)3 * n + id_j, p3);
+                        atomic_add_i64(vir + (size_t)4 * n + id_j, p4);
+                        atomic_add_i64(vir + (size_t)5 * n + id_j, p5);
+                    }
+                    else
+                    {
+                        v0 += p0; v1 += p1; v2 += p2; v3 += p3; v4 += p4; v5 += p5;
+                    }
+                }
+            }
+            sx = wave_sum(sx);
+            sy = wave_sum(sy);
+            sz = wave_sum(sz);
+            if (NEED_ENERGY)
+            {
+                elj = wave_sum(elj);
+                ec = wave_sum(ec);
+            }
+            if (NEED_VIRIAL)
+            {
+                v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2);
+                v3 = wave_sum(v3); v4 = wave_sum(v4); v5 = wave_sum(v5);
+            }
+            if (lane == 0)
+            {
+                if (!ov)
+                {
+                    int cz = sorted_cell[si] % W.nz;
+                    int k = snb_win_range_of(W, z0, ua, vb + 2, cz - z0 + 2);
+                    int slot_i = rpref[k] + (si - rfirst[k]);
+                    atomicAdd((unsigned long long*)&acc[slot_i], (unsigned long long)sx);
+                    atomicAdd((unsigned long long*)&acc[W.wcap + slot_i], (unsigned long long)sy);
+                    atomicAdd((unsigned long long*)&acc[2 * W.wcap + slot_i],
+                              (unsigned long long)sz);
+                }
+                else
+                {
+                    int idi = order[si];
+                    atomic_add_i64(fx + idi, sx);
+                    atomic_add_i64(fx + n + idi, sy);
+                    atomic_add_i64(fx + 2 * n + idi, sz);
+                }
+                if (NEED_ENERGY)
+                {
+                    e_sorted[si] = elj;
+                    e_sorted[(size_t)n + si] = ec;
+                }
+                if (NEED_VIRIAL)
+                {
+                    v_sorted[si] = v0;
+                    v_sorted[(size_t)n + si] = v1;
+                    v_sorted[(size_t)2 * n + si] = v2;
+                    v_sorted[(size_t)3 * n + si] = v3;
+                    v_sorted[(size_t)4 * n + si] = v4;
+                    v_sorted[(size_t)5 * n + si] = v5;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int base = stage_base[w];
+    for (int t = tid; t < total; t += blockDim.x)
+    {
+        if ((size_t)(base + t) >= stage_cap) break;  // cannot happen: stage holds n x invmax
+        long long* dst = stage + 3 * (size_t)(base + t);
+        dst[0] = ov ? 0 : acc[t];
+        dst[1] = ov ? 0 : acc[W.wcap + t];
+        dst[2] = ov ? 0 : acc[2 * W.wcap + t];
+    }
+}
+
+// Per sorted atom: sum of its stage entries into the id-ordered accumulators
+// (plus the row-owner energies / virials on those steps).
+__global__ void k_stage_gather(int n, const int* __restrict__ inv_count,
+                               const int* __restrict__ inv_list, int invmax,
+                               const long long* __restrict__ stage, size_t stage_cap,
+                               const int* __restrict__ order,
+                               int64_t* __restrict__ fx, int need_energy, int need_virial,
+                               const int64_t* __restrict__ e_sorted,
+                               const int64_t* __restrict__ v_sorted, int64_t* __restrict__ e_atom,
+                               int64_t* __restrict__ vir)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int id = order[s];
+    int m = inv_count[s];
+    if (m > invmax) m = invmax;
+    int64_t sx = 0, sy = 0, sz = 0;
+    const int* lst = inv_list + (size_t)s * invmax;
+    for (int k = 0; k < m; k++)
+    {
+        if ((size_t)lst[k] >= stage_cap) continue;
+        const long long* p = stage + 3 * (size_t)lst[k];
+        sx += p[0];
+        sy += p[1];
+        sz += p[2];
+    }
+    fx[id] += sx;
+    fx[n + id] += sy;
+    fx[2 * n + id] += sz;
+    if (need_energy)
+    {
+        e_atom[id] += e_sorted[s];
+        e_atom[(size_t)n + id] += e_sorted[(size_t)n + s];
+    }
+    if (need_virial)
+        for (int c = 0; c < 6; c++) vir[(size_t)c * n + id] += v_sorted[(size_t)c * n + s];
+}
+
+int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* charge,
+                         const SnbBox& box, float beta, int need_energy, int need_virial)
+{
+    int n = ctx->n;
+    if (n <= 0) return SNB_OK;
+    SnbNl& L = ctx->nl;
+    const SnbWin& W = L.win;
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_pack_sorted, dim3((n + 255) / 256), dim3(256), 0, s, n, L.order, crd,
+                       charge, ctx->satom, ctx->status);
+    float rc2 = ctx->lj.cutoff * ctx->lj.cutoff;
+    size_t lds = sizeof(long long) * 3 * (size_t)W.wcap + sizeof(int) * (size_t)W.wcap +
+                 sizeof(int) * (2 * SNB_WIN_MAX_RANGES + 1);
+    if (L.win_lds_set != lds)
+    {
+        // dynamic LDS above 64 KiB is granted per kernel
+        SNB_CHECK(hipFuncSetAttribute((const void*)k_pair_win<true, true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        SNB_CHECK(hipFuncSetAttribute((const void*)k_pair_win<true, false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        SNB_CHECK(hipFuncSetAttribute((const void*)k_pair_win<false, true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        SNB_CHECK(hipFuncSetAttribute((const void*)k_pair_win<false, false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        L.win_lds_set = lds;
+    }
+    snb_prof_begin(ctx, SNB_PROF_PAIR);
+#define SNB_WIN_LAUNCH(E, V)                                                                    \
+    hipLaunchKernelGGL((k_pair_win<E, V>), dim3(W.nblock), dim3(1024), lds, s, n, W,            \
+                       L.cell_start, L.sorted_cell, L.row_count, L.nl16, L.max_nb, ctx->satom, \
+                       ctx->stype, L.order, ctx->lj.a, ctx->lj.b, box, rc2, beta,              \
+                       L.stage_base, L.stage, (size_t)n * L.invmax, ctx->fx, ctx->e_sorted,   \
+                       ctx->v_sorted, ctx->lj.owner_attribution ? 0 : 1, ctx->e_atom,          \
+                       ctx->vir_atom)
+    if (need_energy && need_virial)
+        SNB_WIN_LAUNCH(true, true);
+    else if (need_energy)
+        SNB_WIN_LAUNCH(true, false);
+    else if (need_virial)
+        SNB_WIN_LAUNCH(false, true);
+    else
+        SNB_WIN_LAUNCH(false, false);
+#undef SNB_WIN_LAUNCH
+    snb_prof_end(ctx, SNB_PROF_PAIR);
+    SNB_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_stage_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, L.inv_count,
+                       L.inv_list, L.invmax, L.stage, (size_t)n * L.invmax, L.order, ctx->fx,
+                       need_energy, need_virial,
+                       ctx->e_sorted, ctx->v_sorted, ctx->e_atom, ctx->vir_atom);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}

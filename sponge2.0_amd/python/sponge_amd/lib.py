@@ -57,6 +57,10 @@ _SIGS = {
     "snb_nl_update": ([_P, _P, _I, _I, _P, LTMatrix3, LTMatrix3, _I, _I, _P, _P, _P], _I),
     "snb_nl_device_list": ([_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_I)], _I),
     "snb_nl_copy_rows": ([_P, _P, _P], _I),
+    "snb_nl_set_export": ([_P, _I], _I),
+    "snb_lj_set_owner_attribution": ([_P, _I], _I),
+    "snb_nl_copy_internal": ([_P, _P, _P], _I),
+    "snb_nl_window_info": ([_P] + [ctypes.POINTER(_I)] * 6, _I),
     "snb_nl_grid_shape": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "snb_lj_initial": ([_P, _I, _P, _P, _P, _F], _I),
     "snb_lj_pme_direct_force": ([_P, _I, _I, _I, _I, _P, _P, _P, LTMatrix3, LTMatrix3, _P, _F,
@@ -317,6 +321,33 @@ class Nonbonded:
         check(self.L.snb_nl_copy_rows(self.ctx, _ptr(counts), _ptr(rows)), "snb_nl_copy_rows")
         self.sync()
         return counts.cpu().numpy(), rows.cpu().numpy()
+
+    def internal_rows(self):
+        """(counts, rows) of the list the pair kernel reads, by row-owner atom id."""
+        torch = self.torch
+        counts = torch.empty(self.n, dtype=torch.int32, device=self.dev)
+        rows = torch.empty((self.n, self.max_nb), dtype=torch.int32, device=self.dev)
+        check(self.L.snb_nl_copy_internal(self.ctx, _ptr(counts), _ptr(rows)),
+              "snb_nl_copy_internal")
+        self.sync()
+        return counts.cpu().numpy(), rows.cpu().numpy()
+
+    def set_export(self, enable):
+        """Whether rebuilds also write the reference-layout public list."""
+        check(self.L.snb_nl_set_export(self.ctx, 1 if enable else 0), "snb_nl_set_export")
+
+    def set_owner_attribution(self, enable):
+        """Per-atom pair energy/virial to the internal row owner (True) or to
+        the lower atom id as the reference does (False, default)."""
+        check(self.L.snb_lj_set_owner_attribution(self.ctx, 1 if enable else 0),
+              "snb_lj_set_owner_attribution")
+
+    def window_info(self):
+        v = [_I() for _ in range(6)]
+        check(self.L.snb_nl_window_info(self.ctx, *[ctypes.byref(x) for x in v]),
+              "snb_nl_window_info")
+        keys = ("windowed", "a", "b", "s", "wcap", "nblock")
+        return dict(zip(keys, [x.value for x in v]))
 
     def profile(self, enable=True):
         check(self.L.snb_profile_enable(self.ctx, 1 if enable else 0), "snb_profile_enable")

@@ -62,3 +62,12 @@ def rel_rms(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+@pytest.fixture(scope="session")
+def mixed_small():
+    """4,000 waters + 2 chains x 300 beads in a 52 A box (bead exclusions,
+    one LJ type per bead): the windowed pair path with a large type table."""
+    from sponge_amd import systems
+    return _cached("mix13k", lambda: systems.mixed_system(
+        n_waters=4000, n_chains=2, beads_per_chain=300, box=52.0, seed=11))

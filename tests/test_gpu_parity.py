@@ -59,14 +59,19 @@ def test_neighbor_list_sets_equal_oracle(request, name):
 
 
 @pytest.mark.parametrize("name", SYSTEMS)
-@pytest.mark.parametrize("energy,virial", [(False, False), (True, True)])
-def test_pair_kernel_bit_exact_with_twin(request, name, energy, virial):
+@pytest.mark.parametrize("energy,virial,owner", [(False, False, False), (True, True, False),
+                                                 (True, True, True)])
+def test_pair_kernel_bit_exact_with_twin(request, name, energy, virial, owner):
+    """Default: per-atom terms on the lower id, as the reference's rows hold
+    them (twin over the public list).  owner=True: per-atom terms on the row
+    owner of the internal list (twin over the internal rows)."""
     import oracle
     s = _get(request, name)
     nb = _nb(s)
-    print("case", name, energy, virial, flush=True)
+    print("case", name, energy, virial, owner, flush=True)
+    nb.set_owner_attribution(owner)
     nb.nl_update(forced=True)
-    counts, rows = nb.nl_rows()
+    counts, rows = nb.internal_rows() if owner else nb.nl_rows()
     nb.lj_direct(need_energy=energy, need_virial=virial, frc=False)
     g_f = nb.fixed_forces()
     t_f, t_elj, t_ec, t_v = oracle.pair_twin(s["crd"], s["charge"], s["lj_type"], s["lj_a"],
@@ -302,4 +307,129 @@ def test_conditional_rebuild_policy(small_water):
     oc, _, _ = oracle.nl_build(nb.crd.cpu().numpy(), s["cell"], s["rcell"], nb.cutoff, 2.0,
                                _excl(s), max_nb=nb.max_nb)
     np.testing.assert_array_equal(c2, oc)
+    nb.close()
+
+
+# --------------------------------------------------------------------------
+# windowed pair path (cell-ordered internal list, LDS force accumulators)
+# --------------------------------------------------------------------------
+def _pairs(counts, rows):
+    out = set()
+    for i in range(len(counts)):
+        for j in rows[i, :counts[i]]:
+            out.add((min(i, int(j)), max(i, int(j))))
+    return out
+
+
+@pytest.mark.parametrize("name,windowed", [("water_9k", 1), ("gromacs_iter0", 1),
+                                           ("mixed_small", 1), ("small_water", 0),
+                                           ("tip3p_1011", 0)])
+def test_window_path_selection_and_pair_set(request, name, windowed):
+    """Every axis >= 5 cells selects the windowed path; its internal rows hold
+    exactly the public half list's pairs, each once."""
+    s = _get(request, name)
+    nb = _nb(s)
+    info = nb.window_info()
+    print("case", name, info, flush=True)
+    assert info["windowed"] == windowed
+    nb.nl_update(forced=True)
+    ic, ir = nb.internal_rows()
+    pc, pr = nb.nl_rows()
+    assert nb.status() == 0
+    assert ic.sum() == pc.sum()
+    assert _pairs(ic, ir) == _pairs(pc, pr)
+    nb.close()
+
+
+def test_window_pair_bit_exact_mixed(mixed_small):
+    """Bead chains (exclusions, 602 LJ types) through the windowed kernel."""
+    import oracle
+    s = mixed_small
+    nb = _nb(s)
+    assert nb.window_info()["windowed"] == 1
+    nb.nl_update(forced=True)
+    counts, rows = nb.nl_rows()
+    nb.lj_direct(need_energy=True, need_virial=True, frc=False)
+    g_f = nb.fixed_forces()
+    t_f, t_elj, t_ec, t_v = oracle.pair_twin(s["crd"], s["charge"], s["lj_type"], s["lj_a"],
+                                             s["lj_b"], s["cell"], s["rcell"], nb.cutoff,
+                                             nb.beta, counts, rows, virial=True)
+    np.testing.assert_array_equal(g_f, t_f)
+    np.testing.assert_array_equal(nb.fixed_atom_terms(0), t_elj)
+    np.testing.assert_array_equal(nb.fixed_atom_terms(1), t_ec)
+    np.testing.assert_array_equal(nb.fixed_atom_terms(3), t_v)
+    oc, orows, _ = oracle.nl_build(s["crd"], s["cell"], s["rcell"], nb.cutoff, 2.0, _excl(s),
+                                   max_nb=nb.max_nb)
+    assert _pairs(counts, rows) == _pairs(oc, orows)
+    ic, ir = nb.internal_rows()
+    assert _pairs(ic, ir) == _pairs(oc, orows)
+    nb.close()
+
+
+def test_window_export_off_same_forces(water_9k):
+    """With the public list export switched off the pair forces are unchanged."""
+    outs = []
+    for export in (True, False):
+        nb = _nb(water_9k)
+        nb.set_export(export)
+        nb.nl_update(forced=True)
+        nb.lj_direct(True, True, frc=False)
+        outs.append((nb.fixed_forces(), nb.fixed_atom_terms(0), nb.fixed_atom_terms(3)))
+        nb.close()
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_window_neighbor_overflow_flag(water_9k):
+    from sponge_amd.lib import FLAG_NL_OVERFLOW
+    nb = _nb(water_9k, max_neighbor_numbers=16)
+    assert nb.window_info()["windowed"] == 1
+    nb.nl_update(forced=True)
+    counts, _ = nb.internal_rows()
+    assert counts.max() == 16
+    assert nb.status() & FLAG_NL_OVERFLOW
+    nb.close()
+
+
+def test_window_rows_follow_moving_atoms(water_9k):
+    """Between rebuilds atoms keep their cell slots: after a displacement the
+    forces still equal the twin on the (stale) internal rows."""
+    import oracle
+    s = water_9k
+    nb = _nb(s, refresh_interval=10)
+    nb.nl_update(step=0, forced=True)
+    rng = np.random.default_rng(3)
+    moved = (s["crd"] + rng.uniform(-0.3, 0.3, s["crd"].shape)).astype(np.float32)
+    import torch
+    nb.crd.copy_(torch.from_numpy(moved))
+    nb.nl_update(step=1, forced=False)   # (1+1) % 10 != 0: no rebuild
+    counts, rows = nb.internal_rows()
+    nb.clear_fixed_forces()
+    nb.lj_direct(frc=False)
+    t_f, _, _, _ = oracle.pair_twin(moved, s["charge"], s["lj_type"], s["lj_a"], s["lj_b"],
+                                    s["cell"], s["rcell"], nb.cutoff, nb.beta, counts, rows,
+                                    virial=False)
+    np.testing.assert_array_equal(nb.fixed_forces(), t_f)
+    nb.close()
+
+
+def test_window_93k_workload_pair_exact(request):
+    """The 93k-atom workload (30,000 waters + 3,000 beads, 96.5 A box): pair
+    forces bit-exact with the twin, fixed-point total exactly zero."""
+    import oracle
+    from sponge_amd import systems
+    s = systems.config_gpu_93k()
+    nb = _nb(s, grid=s["pme_grid"])
+    print("case 93k", nb.window_info(), flush=True)
+    nb.nl_update(forced=True)
+    counts, rows = nb.nl_rows()
+    nb.lj_direct(True, False, frc=False)
+    g_f = nb.fixed_forces()
+    assert nb.status() == 0
+    assert np.all(g_f.sum(axis=1) == 0)
+    t_f, t_elj, t_ec, _ = oracle.pair_twin(s["crd"], s["charge"], s["lj_type"], s["lj_a"],
+                                           s["lj_b"], s["cell"], s["rcell"], nb.cutoff,
+                                           nb.beta, counts, rows, virial=False)
+    np.testing.assert_array_equal(g_f, t_f)
+    np.testing.assert_array_equal(nb.fixed_atom_terms(0), t_elj)
     nb.close()
