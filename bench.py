@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU-baseline sample budget (0 disables)")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 256 MiB device copy (scales the HBM byte counters "
+                         "of a profiler run)")
     return ap.parse_args()
 
 
@@ -89,6 +92,17 @@ def accuracy(sysd, nb, counts, rows, pme, gpu_frc):
     return float(np.sqrt(((g - ref) ** 2).sum() / (ref ** 2).sum()))
 
 
+def pair_traffic(kernel):
+    """HBM bytes per launch of the pair kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes over this same workload (scaled by what the
+    counters read for a 256 MiB copy); None when no such profile is present."""
+    path = os.path.join(ROOT, "profiles", "k_pair_win_93k_mixed_traffic.json")
+    if kernel != "k_pair_win" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return float(json.load(f)["bytes_per_launch"])
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -107,14 +121,23 @@ def main():
     nb.set_export(False)
     torch.cuda.synchronize()
 
+    profiling = [False]
+
     def step(k):
         e = (k % args.energy_interval) == 0
+        # kernel averages are over force-only steps (the steady state); the
+        # energy/virial steps still count in ms_per_step
+        if profiling[0] and e:
+            nb.profile(False)
         nb.step(step=k, need_energy=e, need_virial=e, forced_nl=(k == 0), fused=True)
+        if profiling[0] and e:
+            nb.profile(True)
 
     for k in range(args.warmup):
         step(k)
     torch.cuda.synchronize()
     nb.profile(True)
+    profiling[0] = True
     for g in (lib.PROF_PAIR, lib.PROF_NL, lib.PROF_PME, lib.PROF_EXCL):
         nb.profile_read(g, reset=True)
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -132,8 +155,9 @@ def main():
             (("pair", lib.PROF_PAIR), ("nl", lib.PROF_NL), ("pme", lib.PROF_PME),
              ("excl", lib.PROF_EXCL))}
     nb.profile(False)
+    profiling[0] = False
     status = nb.status()
-    counts, rows = nb.internal_rows()
+    counts = nb.internal_counts()
     e_list = int(counts.sum())
     n = sysd["n"]
     ms_per_step = ms / args.steps
@@ -142,12 +166,21 @@ def main():
     pair_avg_s = pair_ms / max(pair_n, 1) / 1000.0
     alg_bytes = 4.0 * e_list + 44.0 * n
     achieved = alg_bytes / pair_avg_s / 1e9
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=nb.dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     # one more step with fresh outputs for the accuracy check
     nb.zero_outputs()
     nb.step(step=0, need_energy=True, need_virial=True, forced_nl=True, fused=True)
     torch.cuda.synchronize()
     gpu_frc = nb.frc.cpu().numpy()
-    counts, rows = nb.internal_rows()
+    counts, rows = (None, None)
+    if args.cpu_seconds > 0 or not args.no_accuracy:
+        counts, rows = nb.internal_rows()
     base, pme = (None, None)
     if args.cpu_seconds > 0:
         base, pme = cpu_baseline(sysd, nb, counts, rows, args.cpu_seconds)
@@ -174,7 +207,7 @@ def main():
                    "pme_grid": [96, 96, 96], "dt_fs": 2.0, "nl_entries": e_list,
                    "energy_interval": args.energy_interval},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": pair_traffic(pair_kernel),
                      "kernel": pair_kernel, "alg_bytes_per_launch": alg_bytes,
                      "avg_launch_us": pair_avg_s * 1e6, "launches": pair_n,
                      "pair_interactions_per_s": None},

@@ -94,8 +94,11 @@ int snb_nl_initial(snb_context* ctx, float cutoff, float skin, snb_ltmatrix3 cel
                    float skin_permit);
 
 /* NEIGHBOR_LIST::Update (neighbor_list.cpp:834-885).  update = 1 forces a
- * rebuild (FORCED_UPDATE); otherwise the reference policy applies and the
- * decision stays on the device (no host synchronisation).  atom_local may be
+ * rebuild (FORCED_UPDATE); otherwise the reference policy applies: with a
+ * positive refresh interval the step number decides on the host and a step
+ * without a rebuild launches nothing; with interval 0 the displacement test
+ * runs on the device and gates the rebuild there (no host synchronisation).
+ * atom_local may be
  * NULL (identity); ghost_numbers must be 0.  The exclusion arrays may be NULL
  * when snb_set_exclusions was called. */
 int snb_nl_update(snb_context* ctx, const int* d_atom_local, int local_atom_numbers,

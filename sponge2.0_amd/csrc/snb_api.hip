@@ -501,6 +501,12 @@ int snb_nl_update(snb_context* c, const int* d_atom_local, int local_atom_number
     const int* en = ex_or(d_ex_num, c->ex_num);
     if (!d_ex_num && !c->has_excl) es = el = en = nullptr;
     int forced = (update || !c->nl.built) ? 1 : 0;
+    // With a fixed refresh interval the host knows the decision
+    // (neighbor_list.cpp:843-856): a step that does not rebuild launches
+    // nothing.  The displacement criterion (interval 0) is decided on the
+    // device and gates the kernels there.
+    if (!forced && c->nl.refresh_interval > 0 && (step + 1) % c->nl.refresh_interval != 0)
+        return SNB_OK;
     int rc = snb_launch_nl_update(c, d_crd, box, forced, step, es, el, en);
     if (!rc) c->nl.built = true;
     return rc;

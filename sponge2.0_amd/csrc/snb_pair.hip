@@ -286,7 +286,8 @@ __global__ void k_pack_sorted(int n, const int* __restrict__ order,
 
 // One workgroup per block of a x b columns x S z-cells (snb_window.h).  The
 // workgroup keeps int64 force accumulators for every atom of its window in
-// LDS; a wavefront evaluates one row at a time (64 lanes over its entries),
+// LDS; a wavefront evaluates one row at a time (64 lanes over its entries,
+// two entries per lane in flight so the gathers of one overlap the other),
 // adds -f to the j slot (ds_add_u64) and, after a wave reduction, +sum f to
 // the row's own slot.  At the end the accumulators go to the stage buffer;
 // k_stage_gather sums each atom's stage entries.  Windows larger than the LDS
@@ -295,7 +296,154 @@ __global__ void k_pack_sorted(int n, const int* __restrict__ order,
 // of the pair: row-owner sums when the owner is the lower id, integer atomics
 // on the id-ordered arrays otherwise (energy/virial steps only); with
 // ref_attr == 0 every pair's terms stay with the row owner.
+
+// Periodic displacement; ORTHO drops the off-diagonal terms, which is the
+// same arithmetic when cell and rcell are diagonal (x + 0 == x and 0 * c == 0
+// change no bits that reach the result).
+template <bool ORTHO>
+__device__ __forceinline__ void win_disp(float& dx, float& dy, float& dz, const SnbBox& box)
+{
+    if (ORTHO)
+    {
+        float kx = floorf(dx * box.rc[0] + 0.5f);
+        float ky = floorf(dy * box.rc[2] + 0.5f);
+        float kz = floorf(dz * box.rc[5] + 0.5f);
+        dx = dx - kx * box.c[0];
+        dy = dy - ky * box.c[2];
+        dz = dz - kz * box.c[5];
+    }
+    else
+        pbc_disp(dx, dy, dz, box);
+}
+
+// Force factor of one pair (Lennard_Jones_force.cpp:73-97): LJ plus the
+// erfc-screened Coulomb term, float arithmetic of the library's own math.
+__device__ __forceinline__ float pair_force_factor(float r2, float A, float B, float qq,
+                                                   float beta)
+{
+    float rinv = snb_rsqrtf(r2);
+    float r = r2 * rinv;
+    float rinv2 = rinv * rinv;
+    float rinv6 = (rinv2 * rinv2) * rinv2;
+    float flj = ((B - A * rinv6) * rinv6) * rinv2;
+    float x = beta * r;
+    float ex = snb_expf(-(x * x));
+    float erfcv = snb_erfc_core(x, ex);
+    float fc = (qq * (rinv2 * rinv)) * ((x * SNB_TWO_OVER_SQRT_PI) * ex + erfcv);
+    return flj - fc;
+}
+
+struct WinRowSums
+{
+    int64_t sx, sy, sz, elj, ec, v0, v1, v2, v3, v4, v5;
+};
+
+// Everything one in-cutoff pair contributes: -f to the j accumulator (LDS
+// slot, or the global array in overflow mode), +f and the row owner's share
+// of energy/virial to the register sums.
 template <bool NEED_ENERGY, bool NEED_VIRIAL>
+__device__ __forceinline__ void win_pair(WinRowSums& S, float dx, float dy, float dz, float r2,
+                                         float A, float B, float qi, float qj, float beta,
+                                         int slot, int sj, bool ov, int wcap, long long* acc,
+                                         int n, const int* __restrict__ order,
+                                         int64_t* __restrict__ fx, int ref_attr, int id_i,
+                                         int64_t* __restrict__ e_atom, int64_t* __restrict__ vir)
+{
+    float fa = pair_force_factor(r2, A, B, qi * qj, beta);
+    float fxv = fa * dx, fyv = fa * dy, fzv = fa * dz;
+    int64_t ix = snb_to_fixed(fxv), iy = snb_to_fixed(fyv), iz = snb_to_fixed(fzv);
+    S.sx += ix;
+    S.sy += iy;
+    S.sz += iz;
+    if (!ov)
+    {
+        atomicAdd((unsigned long long*)&acc[slot], (unsigned long long)(-ix));
+        atomicAdd((unsigned long long*)&acc[wcap + slot], (unsigned long long)(-iy));
+        atomicAdd((unsigned long long*)&acc[2 * wcap + slot], (unsigned long long)(-iz));
+    }
+    else
+    {
+        int idj = order[sj];
+        atomic_add_i64(fx + idj, -ix);
+        atomic_add_i64(fx + n + idj, -iy);
+        atomic_add_i64(fx + 2 * n + idj, -iz);
+    }
+    if (NEED_ENERGY || NEED_VIRIAL)
+    {
+        int id_j = -1;
+        if (ref_attr)
+        {
+            id_j = order[sj];
+            if (id_j > id_i) id_j = -1;
+        }
+        if (NEED_ENERGY)
+        {
+            double ddx = dx, ddy = dy, ddz = dz;
+            double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
+            double rd = __builtin_sqrt(r2d);
+            double rid = 1.0 / rd;
+            double ri2 = rid * rid;
+            double ri6 = (ri2 * ri2) * ri2;
+            int64_t pl = snb_to_fixed_d((((double)A * (1.0 / 12.0)) * ri6 -
+                                         (double)B * (1.0 / 6.0)) * ri6);
+            double erfd, hd;
+            snb_erf_parts_d((double)beta * rd, &erfd, &hd);
+            int64_t pc = snb_to_fixed_d((((double)qi * (double)qj) * (1.0 - erfd)) * rid);
+            if (id_j >= 0)
+            {
+                atomic_add_i64(e_atom + id_j, pl);
+                atomic_add_i64(e_atom + n + id_j, pc);
+            }
+            else
+            {
+                S.elj += pl;
+                S.ec += pc;
+            }
+        }
+        if (NEED_VIRIAL)
+        {
+            int64_t p0 = snb_to_fixed(-(fxv * dx));
+            int64_t p1 = snb_to_fixed(-(fxv * dy + fyv * dx));
+            int64_t p2 = snb_to_fixed(-(fyv * dy));
+            int64_t p3 = snb_to_fixed(-(fxv * dz + fzv * dx));
+            int64_t p4 = snb_to_fixed(-(fyv * dz + fzv * dy));
+            int64_t p5 = snb_to_fixed(-(fzv * dz));
+            if (id_j >= 0)
+            {
+                atomic_add_i64(vir + id_j, p0);
+                atomic_add_i64(vir + (size_t)n + id_j, p1);
+                atomic_add_i64(vir + (size_t)2 * n + id_j, p2);
+                atomic_add_i64(vir + (size_t)3 * n + id_j, p3);
+                atomic_add_i64(vir + (size_t)4 * n + id_j, p4);
+                atomic_add_i64(vir + (size_t)5 * n + id_j, p5);
+            }
+            else
+            {
+                S.v0 += p0; S.v1 += p1; S.v2 += p2; S.v3 += p3; S.v4 += p4; S.v5 += p5;
+            }
+        }
+    }
+}
+
+// window slot -> sorted index: the LDS map, or a search of the range table
+// when the window did not fit
+__device__ __forceinline__ int win_slot_sorted(int slot, bool ov, const int* slot_s,
+                                               const int* rfirst, const int* rpref, int nrange)
+{
+    if (!ov) return slot_s[slot];
+    int lo = 0, hi = nrange - 1;
+    while (lo < hi)
+    {
+        int mid = (lo + hi + 1) >> 1;
+        if (rpref[mid] <= slot)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return rfirst[lo] + (slot - rpref[lo]);
+}
+
+template <bool NEED_ENERGY, bool NEED_VIRIAL, bool ORTHO>
 __global__ void __launch_bounds__(1024) k_pair_win(
     int n, SnbWin W, const int* __restrict__ start, const int* __restrict__ sorted_cell,
     const int* __restrict__ row_count, const uint16_t* __restrict__ nl16, int max_nb,
@@ -316,24 +464,39 @@ __global__ void __launch_bounds__(1024) k_pair_win(
     const int w = blockIdx.x;
     int x0, y0, z0;
     snb_win_block(W, w, x0, y0, z0);
+    // range table: sizes, then a wave-0 scan
+    int rsz = 0;
     if (tid < SNB_WIN_MAX_RANGES)
     {
-        int f = 0, sz = 0;
+        int f = 0;
         if (tid < W.nrange)
         {
             int fc, nc;
             snb_win_range_cells(W, x0, y0, z0, tid, fc, nc);
             f = start[fc];
-            sz = nc > 0 ? start[fc + nc] - f : 0;
+            rsz = nc > 0 ? start[fc + nc] - f : 0;
         }
         rfirst[tid] = f;
-        rpref[tid + 1] = sz;
+        rpref[tid + 1] = rsz;
     }
     __syncthreads();
-    if (tid == 0)
+    if (wave == 0)
     {
-        rpref[0] = 0;
-        for (int k = 0; k < SNB_WIN_MAX_RANGES; k++) rpref[k + 1] += rpref[k];
+        int a0 = rpref[lane + 1], a1 = rpref[lane + 65];
+        int i0 = a0, i1 = a1;
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            int u0 = __shfl_up(i0, off), u1 = __shfl_up(i1, off);
+            if (lane >= off)
+            {
+                i0 += u0;
+                i1 += u1;
+            }
+        }
+        int t0 = __shfl(i0, 63);
+        rpref[lane + 1] = i0;
+        rpref[lane + 65] = t0 + i1;
+        if (lane == 0) rpref[0] = 0;
     }
     __syncthreads();
     const int total = rpref[SNB_WIN_MAX_RANGES];
@@ -361,133 +524,46 @@ __global__ void __launch_bounds__(1024) k_pair_win(
             const uint16_t* e16 = nl16 + (size_t)si * max_nb;
             const float4 ai = satom[si];
             const int ti = stype[si];
-            // reference attribution: a pair's energy and virial belong to the
-            // lower atom id (the reference's row atom)
             const int id_i = (NEED_ENERGY || NEED_VIRIAL) && ref_attr ? order[si] : 0;
-            int64_t sx = 0, sy = 0, sz = 0, elj = 0, ec = 0;
-            int64_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-            for (int k = lane; k < cnt; k += 64)
+            WinRowSums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int k0 = 0; k0 < cnt; k0 += 128)
             {
-                const int slot = e16[k];
-                int sj;
-                if (!ov)
-                    sj = slot_s[slot];
-                else
-                {
-                    int lo = 0, hi = W.nrange - 1;
-                    while (lo < hi)
-                    {
-                        int mid = (lo + hi + 1) >> 1;
-                        if (rpref[mid] <= slot)
-                            lo = mid;
-                        else
-                            hi = mid - 1;
-                    }
-                    sj = rfirst[lo] + (slot - rpref[lo]);
-                }
-                const float4 aj = satom[sj];
-                float dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
-                pbc_disp(dx, dy, dz, box);
-                float r2 = (dx * dx + dy * dy) + dz * dz;
-                if (!(r2 < rc2)) continue;
-                const int p = snb_lj_pair_index(ti, stype[sj]);
-                const float A = lj_a[p], B = lj_b[p];
-                float rinv = snb_rsqrtf(r2);
-                float r = r2 * rinv;
-                float rinv2 = rinv * rinv;
-                float rinv6 = (rinv2 * rinv2) * rinv2;
-                float flj = ((B - A * rinv6) * rinv6) * rinv2;
-                float x = beta * r;
-                float ex = snb_expf(-(x * x));
-                float erfcv = snb_erfc_core(x, ex);
-                float qq = ai.w * aj.w;
-                float fc = (qq * (rinv2 * rinv)) * ((x * SNB_TWO_OVER_SQRT_PI) * ex + erfcv);
-                float fa = flj - fc;
-                float fxv = fa * dx, fyv = fa * dy, fzv = fa * dz;
-                int64_t ix = snb_to_fixed(fxv), iy = snb_to_fixed(fyv), iz = snb_to_fixed(fzv);
-                sx += ix;
-                sy += iy;
-                sz += iz;
-                if (!ov)
-                {
-                    atomicAdd((unsigned long long*)&acc[slot], (unsigned long long)(-ix));
-                    atomicAdd((unsigned long long*)&acc[W.wcap + slot], (unsigned long long)(-iy));
-                    atomicAdd((unsigned long long*)&acc[2 * W.wcap + slot],
-                              (unsigned long long)(-iz));
-                }
-                else
-                {
-                    int idj = order[sj];
-                    atomic_add_i64(fx + idj, -ix);
-                    atomic_add_i64(fx + n + idj, -iy);
-                    atomic_add_i64(fx + 2 * n + idj, -iz);
-                }
-                int id_j = -1;
-                if ((NEED_ENERGY || NEED_VIRIAL) && ref_attr)
-                {
-                    id_j = order[sj];
-                    if (id_j > id_i) id_j = -1;
-                }
-                if (NEED_ENERGY)
-                {
-                    double ddx = dx, ddy = dy, ddz = dz;
-                    double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
-                    double rd = __builtin_sqrt(r2d);
-                    double rid = 1.0 / rd;
-                    double ri2 = rid * rid;
-                    double ri6 = (ri2 * ri2) * ri2;
-                    int64_t pl = snb_to_fixed_d((((double)A * (1.0 / 12.0)) * ri6 -
-                                                 (double)B * (1.0 / 6.0)) * ri6);
-                    double erfd, hd;
-                    snb_erf_parts_d((double)beta * rd, &erfd, &hd);
-                    int64_t pc =
-                        snb_to_fixed_d((((double)ai.w * (double)aj.w) * (1.0 - erfd)) * rid);
-                    if (id_j >= 0)
-                    {
-                        atomic_add_i64(e_atom + id_j, pl);
-                        atomic_add_i64(e_atom + n + id_j, pc);
-                    }
-                    else
-                    {
-                        elj += pl;
-                        ec += pc;
-                    }
-                }
-                if (NEED_VIRIAL)
-                {
-                    int64_t p0 = snb_to_fixed(-(fxv * dx));
-                    int64_t p1 = snb_to_fixed(-(fxv * dy + fyv * dx));
-                    int64_t p2 = snb_to_fixed(-(fyv * dy));
-                    int64_t p3 = snb_to_fixed(-(fxv * dz + fzv * dx));
-                    int64_t p4 = snb_to_fixed(-(fyv * dz + fzv * dy));
-                    int64_t p5 = snb_to_fixed(-(fzv * dz));
-                    if (id_j >= 0)
-                    {
-                        atomic_add_i64(vir + id_j, p0);
-                        atomic_add_i64(vir + (size_t)n + id_j, p1);
-                        atomic_add_i64(vir + (size_t)2 * n + id_j, p2);
-                        atomic_add_i64(vir + (size_t)3 * n + id_j, p3);
-                        atomic_add_i64(vir + (size_t)4 * n + id_j, p4);
-                        atomic_add_i64(vir + (size_t)5 * n + id_j, p5);
-                    }
-                    else
-                    {
-                        v0 += p0; v1 += p1; v2 += p2; v3 += p3; v4 += p4; v5 += p5;
-                    }
-                }
+                // two entries per lane: issue both gathers before either pair
+                const int ka = k0 + lane, kb = ka + 64;
+                const int sla = ka < cnt ? (int)e16[ka] : -1;
+                const int slb = kb < cnt ? (int)e16[kb] : -1;
+                const int sja = sla >= 0 ? win_slot_sorted(sla, ov, slot_s, rfirst, rpref, W.nrange) : si;
+                const int sjb = slb >= 0 ? win_slot_sorted(slb, ov, slot_s, rfirst, rpref, W.nrange) : si;
+                const float4 aja = satom[sja], ajb = satom[sjb];
+                const int tja = stype[sja], tjb = stype[sjb];
+                const int pa = snb_lj_pair_index(ti, tja), pb = snb_lj_pair_index(ti, tjb);
+                const float Aa = lj_a[pa], Ba = lj_b[pa], Ab = lj_a[pb], Bb = lj_b[pb];
+                float dxa = aja.x - ai.x, dya = aja.y - ai.y, dza = aja.z - ai.z;
+                float dxb = ajb.x - ai.x, dyb = ajb.y - ai.y, dzb = ajb.z - ai.z;
+                win_disp<ORTHO>(dxa, dya, dza, box);
+                win_disp<ORTHO>(dxb, dyb, dzb, box);
+                const float r2a = (dxa * dxa + dya * dya) + dza * dza;
+                const float r2b = (dxb * dxb + dyb * dyb) + dzb * dzb;
+                if (sla >= 0 && r2a < rc2)
+                    win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxa, dya, dza, r2a, Aa, Ba, ai.w, aja.w,
+                                                       beta, sla, sja, ov, W.wcap, acc, n, order,
+                                                       fx, ref_attr, id_i, e_atom, vir);
+                if (slb >= 0 && r2b < rc2)
+                    win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxb, dyb, dzb, r2b, Ab, Bb, ai.w, ajb.w,
+                                                       beta, slb, sjb, ov, W.wcap, acc, n, order,
+                                                       fx, ref_attr, id_i, e_atom, vir);
             }
-            sx = wave_sum(sx);
-            sy = wave_sum(sy);
-            sz = wave_sum(sz);
+            int64_t sx = wave_sum(S.sx), sy = wave_sum(S.sy), sz = wave_sum(S.sz);
+            int64_t elj = 0, ec = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
             if (NEED_ENERGY)
             {
-                elj = wave_sum(elj);
-                ec = wave_sum(ec);
+                elj = wave_sum(S.elj);
+                ec = wave_sum(S.ec);
             }
             if (NEED_VIRIAL)
             {
-                v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2);
-                v3 = wave_sum(v3); v4 = wave_sum(v4); v5 = wave_sum(v5);
+                v0 = wave_sum(S.v0); v1 = wave_sum(S.v1); v2 = wave_sum(S.v2);
+                v3 = wave_sum(S.v3); v4 = wave_sum(S.v4); v5 = wave_sum(S.v5);
             }
             if (lane == 0)
             {
@@ -591,32 +667,45 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
     if (L.win_lds_set != lds)
     {
         // dynamic LDS above 64 KiB is granted per kernel
-        SNB_CHECK(hipFuncSetAttribute((const void*)k_pair_win<true, true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        SNB_CHECK(hipFuncSetAttribute((const void*)k_pair_win<true, false>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        SNB_CHECK(hipFuncSetAttribute((const void*)k_pair_win<false, true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        SNB_CHECK(hipFuncSetAttribute((const void*)k_pair_win<false, false>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const void* fns[8] = {
+            (const void*)k_pair_win<true, true, true>, (const void*)k_pair_win<true, false, true>,
+            (const void*)k_pair_win<false, true, true>, (const void*)k_pair_win<false, false, true>,
+            (const void*)k_pair_win<true, true, false>, (const void*)k_pair_win<true, false, false>,
+            (const void*)k_pair_win<false, true, false>,
+            (const void*)k_pair_win<false, false, false>};
+        for (const void* f : fns)
+            SNB_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         L.win_lds_set = lds;
     }
+    // diagonal cell and rcell: the displacement skips the zero terms
+    const bool ortho = box.c[1] == 0.0f && box.c[3] == 0.0f && box.c[4] == 0.0f &&
+                       box.rc[1] == 0.0f && box.rc[3] == 0.0f && box.rc[4] == 0.0f;
     snb_prof_begin(ctx, SNB_PROF_PAIR);
-#define SNB_WIN_LAUNCH(E, V)                                                                    \
-    hipLaunchKernelGGL((k_pair_win<E, V>), dim3(W.nblock), dim3(1024), lds, s, n, W,            \
+#define SNB_WIN_LAUNCH(E, V, O)                                                                 \
+    hipLaunchKernelGGL((k_pair_win<E, V, O>), dim3(W.nblock), dim3(1024), lds, s, n, W,         \
                        L.cell_start, L.sorted_cell, L.row_count, L.nl16, L.max_nb, ctx->satom, \
                        ctx->stype, L.order, ctx->lj.a, ctx->lj.b, box, rc2, beta,              \
                        L.stage_base, L.stage, (size_t)n * L.invmax, ctx->fx, ctx->e_sorted,   \
                        ctx->v_sorted, ctx->lj.owner_attribution ? 0 : 1, ctx->e_atom,          \
                        ctx->vir_atom)
-    if (need_energy && need_virial)
-        SNB_WIN_LAUNCH(true, true);
-    else if (need_energy)
-        SNB_WIN_LAUNCH(true, false);
-    else if (need_virial)
-        SNB_WIN_LAUNCH(false, true);
+#define SNB_WIN_LAUNCH_EV(O)                 \
+    if (need_energy && need_virial)          \
+        SNB_WIN_LAUNCH(true, true, O);       \
+    else if (need_energy)                    \
+        SNB_WIN_LAUNCH(true, false, O);      \
+    else if (need_virial)                    \
+        SNB_WIN_LAUNCH(false, true, O);      \
+    else                                     \
+        SNB_WIN_LAUNCH(false, false, O);
+    if (ortho)
+    {
+        SNB_WIN_LAUNCH_EV(true)
+    }
     else
-        SNB_WIN_LAUNCH(false, false);
+    {
+        SNB_WIN_LAUNCH_EV(false)
+    }
+#undef SNB_WIN_LAUNCH_EV
 #undef SNB_WIN_LAUNCH
     snb_prof_end(ctx, SNB_PROF_PAIR);
     SNB_CHECK(hipGetLastError());

@@ -332,6 +332,16 @@ class Nonbonded:
         self.sync()
         return counts.cpu().numpy(), rows.cpu().numpy()
 
+    def internal_counts(self):
+        """Row lengths of the internal list, by row-owner atom id."""
+        torch = self.torch
+        counts = torch.empty(self.n, dtype=torch.int32, device=self.dev)
+        rows = torch.empty((self.n, self.max_nb), dtype=torch.int32, device=self.dev)
+        check(self.L.snb_nl_copy_internal(self.ctx, _ptr(counts), _ptr(rows)),
+              "snb_nl_copy_internal")
+        self.sync()
+        return counts.cpu().numpy()
+
     def set_export(self, enable):
         """Whether rebuilds also write the reference-layout public list."""
         check(self.L.snb_nl_set_export(self.ctx, 1 if enable else 0), "snb_nl_set_export")
