@@ -39,6 +39,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU-baseline sample budget (0 disables)")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager launches instead of HIP-graph replays of each kind of step")
+    ap.add_argument("--profile-steps", type=int, default=200,
+                    help="eager steps, before the timed region, whose kernel-group times "
+                         "give kernel_avg_us and the roofline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (scales the HBM byte counters "
                          "of a profiler run)")
@@ -136,26 +141,48 @@ def main():
     for k in range(args.warmup):
         step(k)
     torch.cuda.synchronize()
+    # kernel-group times from eager launches (HIP events on the library stream)
+    k0 = args.warmup
     nb.profile(True)
     profiling[0] = True
     for g in (lib.PROF_PAIR, lib.PROF_NL, lib.PROF_PME, lib.PROF_EXCL):
         nb.profile_read(g, reset=True)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(nb.stream)
-    for k in range(args.warmup, args.warmup + args.steps):
+    for k in range(k0, k0 + args.profile_steps):
         step(k)
-    ev1.record(nb.stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = ev0.elapsed_time(ev1)
     prof = {name: nb.profile_read(g) for name, g in
             (("pair", lib.PROF_PAIR), ("nl", lib.PROF_NL), ("pme", lib.PROF_PME),
              ("excl", lib.PROF_EXCL))}
     nb.profile(False)
     profiling[0] = False
+    k0 += args.profile_steps
+
+    def kind(k):
+        return ((k + 1) % args.refresh == 0, (k % args.energy_interval) == 0)
+
+    graphs = {}
+    if not args.no_graph:
+        # one graph per kind of step: (rebuild, energy); captured with a step
+        # number of that kind, replayed for every step of that kind
+        for k in range(k0, k0 + max(args.refresh, args.energy_interval) + 1):
+            if kind(k) not in graphs:
+                rb, e = kind(k)
+                graphs[kind(k)] = nb.capture_step(step=k, need_energy=e, need_virial=e)
+        torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(nb.stream)
+    for k in range(k0, k0 + args.steps):
+        if graphs:
+            graphs[kind(k)].replay()
+        else:
+            step(k)
+    ev1.record(nb.stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_time(ev1)
     status = nb.status()
     counts = nb.internal_counts()
     e_list = int(counts.sum())
@@ -213,6 +240,7 @@ def main():
                      "pair_interactions_per_s": None},
         "kernel_avg_us": {k: v * 1000.0 for k, v in per.items()},
         "pair_window": nb.window_info(),
+        "launch": "hip-graph replay per step kind" if graphs else "eager",
         "force_rel_rms_vs_fp64": err,
         "status_flags": status,
         "wall_s_timed": wall,

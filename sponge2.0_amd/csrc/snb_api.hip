@@ -121,14 +121,30 @@ void free_nl(SnbNl& L)
     dfree(L.inv_count);
     dfree(L.inv_list);
     dfree(L.pub_count);
+    dfree(L.ex_rng);
+    dfree(L.block_work);
+    dfree(L.block_perm);
     L.windowed = false;
     L.init = false;
 }
 
 // Chooses the block of the windowed pair path (snb_window.h): the largest
-// owned block whose expected window fills at most 75% of the LDS capacity.
-// Needs >= 5 cells on every axis (unique forward displacements).
-bool choose_window(int nx, int ny, int nz, double atoms_per_cell, int max_lds, SnbWin* W)
+// owned block whose expected window fills at most 75% of the LDS capacity and
+// that still gives at least min_blocks blocks (two per compute unit, so the
+// heaviest-first launch order can even out dense and sparse regions); when no
+// block gives that many, the largest that fits.  Needs >= 5 cells on every
+// axis (unique forward displacements).
+bool choose_window_pass(int nx, int ny, int nz, double atoms_per_cell, int max_lds,
+                        int min_blocks, SnbWin* W);
+bool choose_window(int nx, int ny, int nz, double atoms_per_cell, int max_lds, int min_blocks,
+                   SnbWin* W)
+{
+    return choose_window_pass(nx, ny, nz, atoms_per_cell, max_lds, min_blocks, W) ||
+           choose_window_pass(nx, ny, nz, atoms_per_cell, max_lds, 1, W);
+}
+
+bool choose_window_pass(int nx, int ny, int nz, double atoms_per_cell, int max_lds,
+                        int min_blocks, SnbWin* W)
 {
     if (nx < 5 || ny < 5 || nz < 5) return false;
     // LDS per slot: 3 x int64 accumulator + int slot map; plus two range tables
@@ -143,6 +159,8 @@ bool choose_window(int nx, int ny, int nz, double atoms_per_cell, int max_lds, S
         if (2 * (a + 2) * (b + 4) > SNB_WIN_MAX_RANGES) continue;
         double expect = (double)(a + 2) * (b + 4) * (s + 4) * atoms_per_cell;
         if (expect > 0.75 * wcap) continue;
+        long nblk = (long)((nx + a - 1) / a) * ((ny + b - 1) / b) * ((nz + s - 1) / s);
+        if (nblk < min_blocks) continue;
         W->nx = nx;
         W->ny = ny;
         W->nz = nz;
@@ -264,6 +282,11 @@ int snb_context_create(snb_context** out, int device, void* hip_stream, int atom
                 hipSuccess &&
             lds > 0)
             c->max_lds = lds;
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) ==
+                hipSuccess &&
+            ncu > 0)
+            c->num_cu = ncu;
     }
     c->stream = (hipStream_t)hip_stream;
     c->n = atom_numbers;
@@ -446,7 +469,8 @@ int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
         free_nl(L);
         return rc;
     }
-    L.windowed = choose_window(L.nx, L.ny, L.nz, (double)n / L.ncell, c->max_lds, &L.win);
+    L.windowed = choose_window(L.nx, L.ny, L.nz, (double)n / L.ncell, c->max_lds, 2 * c->num_cu,
+                               &L.win);
     if (L.windowed)
     {
         const SnbWin& W = L.win;
@@ -461,6 +485,9 @@ int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
         if (!rc) rc = dalloc(&L.inv_count, n);
         if (!rc) rc = dalloc(&L.inv_list, (size_t)n * L.invmax);
         if (!rc) rc = dalloc(&L.pub_count, n);
+        if (!rc) rc = dalloc(&L.ex_rng, n);
+        if (!rc) rc = dalloc(&L.block_work, W.nblock);
+        if (!rc) rc = dalloc(&L.block_perm, W.nblock);
         if (rc)
         {
             free_nl(L);
@@ -773,11 +800,11 @@ int snb_pme_excluded_force(snb_context* c, int atom_numbers, const int* d_atom_l
     if (!d_ex_num && !c->has_excl) return SNB_ERR_NOT_INITIALIZED;
     DeviceGuard g(c->device);
     SnbBox box = snb_box(cell, rcell);
-    int rc = snb_launch_pack_atoms(c, d_crd, d_charge);
-    if (rc) return rc;
+    int rc;
     if (need_energy && (rc = snb_launch_fill_i64(c->stream, c->e_atom + 2 * (size_t)c->n, c->n)))
         return rc;
-    rc = snb_launch_excl(c, c->atom4, box, c->pme.beta, es, el, en, need_energy, need_virial);
+    rc = snb_launch_excl(c, d_crd, d_charge, box, c->pme.beta, es, el, en, need_energy,
+                         need_virial);
     if (rc) return rc;
     if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
     if (need_energy)
@@ -811,11 +838,11 @@ int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatr
     // coordinates, the reference reuses the grids of the last update step).
     bool do_force = (step % P.update_interval) == 0;
     if (!do_force) need_virial = 0;
-    if (c->n > 0 && (rc = snb_launch_pack_atoms(c, d_crd, d_charge))) return rc;
     if (need_energy && (rc = snb_launch_charge_sums(c, d_charge))) return rc;
     if (do_force || need_energy)
     {
-        if ((rc = snb_launch_pme_recip(c, box, need_energy, need_virial, do_force ? 1 : 0)))
+        if ((rc = snb_launch_pme_recip(c, d_crd, d_charge, box, need_energy, need_virial,
+                                       do_force ? 1 : 0)))
             return rc;
         P.have_energy = need_energy != 0;
     }

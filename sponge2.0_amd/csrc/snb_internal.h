@@ -72,6 +72,9 @@ struct SnbNl
     int* inv_list = nullptr;     // n * invmax: stage slots of each sorted atom
     int invmax = 0;
     int* pub_count = nullptr;    // n (export scratch)
+    int2* ex_rng = nullptr;      // n: first/last exclusion partner per atom id
+    int* block_work = nullptr;   // nblock: list entries owned by each block
+    int* block_perm = nullptr;   // nblock: launch order, heaviest block first
     size_t win_lds_set = 0;      // dynamic LDS granted to the pair kernel
 };
 
@@ -117,6 +120,7 @@ struct SnbPme
     int conv_blocks = 0;
     float rcell[6];
     bool have_energy = false;
+    size_t plane_lds_set = 0;  // dynamic LDS granted to the fused plane FFT kernels
 };
 
 #define SNB_PROF_GROUPS 4
@@ -144,6 +148,7 @@ struct snb_context
     int64_t* e_sorted = nullptr; // 2n: LJ | direct energy of row owners, sorted order
     int64_t* v_sorted = nullptr; // 6n: virial of row owners, sorted order
     int max_lds = 65536;         // dynamic LDS bytes per workgroup the device allows
+    int num_cu = 256;            // compute units of the device
     int* status = nullptr;
     int64_t* fx = nullptr;       // 3n, SoA
     int64_t* e_atom = nullptr;   // 3n: LJ | direct | correction
@@ -223,14 +228,15 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
 // pair kernel (snb_pair.hip)
 int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBox& box,
                          float beta, int need_energy, int need_virial);
-int snb_launch_excl(snb_context* ctx, const float4* atoms, const SnbBox& box, float beta,
+int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge,
+                    const SnbBox& box, float beta,
                     const int* ex_start, const int* ex_list, const int* ex_num,
                     int need_energy, int need_virial);
 
 // PME (snb_pme.hip)
 int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box);
-int snb_launch_pme_recip(snb_context* ctx, const SnbBox& box, int need_energy, int need_virial,
-                         int with_force);
+int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* charge,
+                         const SnbBox& box, int need_energy, int need_virial, int with_force);
 int snb_fft_forward_full(snb_context* ctx, const float* real, float2* cplx);
 int snb_launch_pme_energy_to_potential(snb_context* ctx, float* potential);
 int snb_launch_pme_virial_to(snb_context* ctx, snb_ltmatrix3* vir);

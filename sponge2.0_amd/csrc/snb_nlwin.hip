@@ -1,9 +1,9 @@
 // snb_nlwin.hip -- neighbor-list build for the windowed pair path.
 //
-// After the cell sort (snb_nl.hip), one wavefront per sorted atom s builds its
-// row: the candidates of its forward half-stencil (13 columns, runs of up to
-// five z-cells, see snb_window.h) are tested 64 at a time and the survivors
-// are stored as 16-bit slots of the owning block's window.  The same pair set
+// After the cell sort (snb_nl.hip), one workgroup per cell builds the rows of
+// its atoms: the candidates of the forward half-stencil (13 columns, runs of
+// up to five z-cells, see snb_window.h) are staged in LDS, tested 64 at a time
+// and the survivors are stored as 16-bit slots of the owning block's window.  The same pair set
 // as the reference's half list results -- every unordered pair within
 // rc + skin once, minus exclusions decided on the lower atom id as the
 // reference does (neighbor_list.cpp:594-665) -- only the row that holds a
@@ -79,140 +79,223 @@ __device__ __forceinline__ void win_block_of_cell(const SnbWin& W, int c, int& c
     z0 = (cz / W.s) * W.s;
 }
 
-// One wavefront (one 64-thread workgroup) per sorted atom s.
-__global__ void __launch_bounds__(64) k_nl_build_win(
-    const int* __restrict__ need, int n, SnbWin W, const int* __restrict__ start,
-    const int* __restrict__ sorted_cell, const float4* __restrict__ sorted_crd, SnbBox box,
-    float cs2, const int* __restrict__ ex_start, const int* __restrict__ ex_list,
-    const int* __restrict__ ex_num, int max_nb, uint16_t* __restrict__ nl16,
-    int* __restrict__ row_count, int* __restrict__ status)
+// Candidate capacity of one LDS chunk and the atoms per cell whose row counts
+// live in LDS (the reference caps a cell at max_atom_in_grid_numbers = 150).
+#define SNB_NLC_CAP 1792
+#define SNB_NLC_MAXA 256
+#define SNB_NLC_MAXRUN 26
+
+// One 512-thread workgroup per cell.  The candidates of the cell's forward
+// stencil -- 13 columns, z-runs split at the periodic wrap, at most 26 runs --
+// are staged in LDS once (in chunks of SNB_NLC_CAP) with their window slots,
+// and every atom of the cell tests them from LDS: wave w builds the rows of
+// atoms w, w + 8, ...  Candidate order is column, run, sorted index, so a row
+// lists its entries in that order; the first run is the cell itself, where
+// only larger sorted indices count.
+//
+// The distance is taken as q_i - q_j for the row atom i.  The reference takes
+// it from the higher id to the lower; the two agree bit for bit (negation is
+// exact) except where floorf(s + 0.5) rounds differently at |s| = k + 1/2,
+// i.e. where the minimum image is half a box away in that dimension.  With
+// >= 5 cells of >= (rc + skin)/2 per axis that is >= 1.25 (rc + skin), so
+// such a pair fails the test either way.
+template <bool ORTHO>
+__global__ void __launch_bounds__(512) k_nl_build_cell(
+    const int* __restrict__ need, SnbWin W, const int* __restrict__ start,
+    const float4* __restrict__ sorted_crd, SnbBox box, float cs2,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+    const int* __restrict__ ex_num, const int2* __restrict__ ex_rng, int max_nb,
+    uint16_t* __restrict__ nl16, int* __restrict__ row_count, int* __restrict__ status)
 {
     if (need[0] == 0) return;
     __shared__ int first[SNB_WIN_MAX_RANGES];
     __shared__ int pref[SNB_WIN_MAX_RANGES + 1];
-    const int s = blockIdx.x, lane = threadIdx.x;
-    int cx, cy, cz, x0, y0, z0;
-    win_block_of_cell(W, sorted_cell[s], cx, cy, cz, x0, y0, z0);
-    win_table_wave(W, start, x0, y0, z0, lane, first, pref);
+    __shared__ float4 cand[SNB_NLC_CAP];
+    __shared__ int2 cex[SNB_NLC_CAP];  // exclusion id window of each candidate
+    __shared__ uint16_t cslot[SNB_NLC_CAP];
+    __shared__ int run_off[SNB_NLC_MAXRUN + 1];
+    __shared__ int run_p0[SNB_NLC_MAXRUN];
+    __shared__ int run_slot0[SNB_NLC_MAXRUN];
+    __shared__ int nrun_s;
+    __shared__ int rowcnt[SNB_NLC_MAXA];
+    const int c = blockIdx.x;
+    const int cb = start[c], m = start[c + 1] - cb;
+    if (m == 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int cz = c % W.nz, cy = (c / W.nz) % W.ny, cx = c / (W.nz * W.ny);
+    const int x0 = (cx / W.a) * W.a, y0 = (cy / W.b) * W.b, z0 = (cz / W.s) * W.s;
+    if (wave == 0) win_table_wave(W, start, x0, y0, z0, lane, first, pref);
+    for (int a = tid; a < SNB_NLC_MAXA; a += blockDim.x) rowcnt[a] = 0;
     __syncthreads();
     if (pref[SNB_WIN_MAX_RANGES] > 65535)
     {
-        if (lane == 0)
-        {
-            atomicOr(status, SNB_FLAG_NL_OVERFLOW);
-            row_count[s] = 0;
-        }
+        // the window does not fit 16-bit slots: no rows, flagged
+        for (int a = tid; a < m; a += blockDim.x) row_count[cb + a] = 0;
+        if (tid == 0) atomicOr(status, SNB_FLAG_NL_OVERFLOW);
         return;
     }
-    const float4 qi = sorted_crd[s];
-    const int id_i = __float_as_int(qi.w);
-    uint16_t* row = nl16 + (size_t)s * max_nb;
-    const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int count = 0;
-    // forward columns: (0,0), (0,1), (0,2), (1,-2..2), (2,-2..2)
-    for (int col = 0; col < 13; col++)
+    if (tid == 0)
     {
-        int dx, dy;
-        if (col < 3)
+        int nr = 0, tot = 0;
+        for (int col = 0; col < 13; col++)
         {
-            dx = 0;
-            dy = col;
-        }
-        else
-        {
-            dx = 1 + (col - 3) / 5;
-            dy = (col - 3) % 5 - 2;
-        }
-        int u = (cx - x0) + dx, v = (cy - y0) + dy + 2;
-        int ccx = snb_wrap(cx + dx, W.nx), ccy = snb_wrap(cy + dy, W.ny);
-        int base = (ccx * W.ny + ccy) * W.nz;
-        int zlo = cz + (col == 0 ? 0 : -2), zhi = cz + 2;
-        // split the run [zlo, zhi] (unwrapped) at the periodic wrap
-        int za[2], zb[2], zu[2], nrun;
-        if (zlo < 0)
-        {
-            za[0] = zlo + W.nz; zb[0] = W.nz - 1; zu[0] = zlo;
-            za[1] = 0; zb[1] = zhi; zu[1] = 0;
-            nrun = 2;
-        }
-        else if (zhi >= W.nz)
-        {
-            za[0] = zlo; zb[0] = W.nz - 1; zu[0] = zlo;
-            za[1] = 0; zb[1] = zhi - W.nz; zu[1] = W.nz;
-            nrun = 2;
-        }
-        else
-        {
-            za[0] = zlo; zb[0] = zhi; zu[0] = zlo;
-            nrun = 1;
-        }
-        for (int r = 0; r < nrun; r++)
-        {
-            int k = snb_win_range_of(W, z0, u, v, zu[r] - z0 + 2);
-            int kfirst = first[k], kpref = pref[k];
-            int p0 = start[base + za[r]], p1 = start[base + zb[r] + 1];
-            if (col == 0 && r == 0) p0 = s + 1;  // same cell: larger sorted index only
-            for (int pb = p0; pb < p1; pb += 64)
+            int dx, dy;
+            if (col < 3)
             {
-                int p = pb + lane;
+                dx = 0;
+                dy = col;
+            }
+            else
+            {
+                dx = 1 + (col - 3) / 5;
+                dy = (col - 3) % 5 - 2;
+            }
+            int u = (cx - x0) + dx, v = (cy - y0) + dy + 2;
+            int base = (snb_wrap(cx + dx, W.nx) * W.ny + snb_wrap(cy + dy, W.ny)) * W.nz;
+            int zlo = cz + (col == 0 ? 0 : -2), zhi = cz + 2;
+            int za[2], zb[2], zu[2], nr2;
+            if (zlo < 0)
+            {
+                za[0] = zlo + W.nz; zb[0] = W.nz - 1; zu[0] = zlo;
+                za[1] = 0; zb[1] = zhi; zu[1] = 0;
+                nr2 = 2;
+            }
+            else if (zhi >= W.nz)
+            {
+                za[0] = zlo; zb[0] = W.nz - 1; zu[0] = zlo;
+                za[1] = 0; zb[1] = zhi - W.nz; zu[1] = W.nz;
+                nr2 = 2;
+            }
+            else
+            {
+                za[0] = zlo; zb[0] = zhi; zu[0] = zlo;
+                nr2 = 1;
+            }
+            for (int r = 0; r < nr2; r++)
+            {
+                int k = snb_win_range_of(W, z0, u, v, zu[r] - z0 + 2);
+                int p0 = start[base + za[r]], p1 = start[base + zb[r] + 1];
+                run_p0[nr] = p0;
+                run_slot0[nr] = pref[k] + (p0 - first[k]);
+                run_off[nr] = tot;
+                tot += p1 - p0;
+                nr++;
+            }
+        }
+        run_off[nr] = tot;
+        nrun_s = nr;
+    }
+    __syncthreads();
+    const int nrun = nrun_s, total = run_off[nrun];
+    const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int chunk0 = 0; chunk0 < total; chunk0 += SNB_NLC_CAP)
+    {
+        const int clen = total - chunk0 < SNB_NLC_CAP ? total - chunk0 : SNB_NLC_CAP;
+        for (int t = tid; t < clen; t += blockDim.x)
+        {
+            int g = chunk0 + t, r = 0;
+            while (run_off[r + 1] <= g) r++;
+            int off = g - run_off[r];
+            float4 q = sorted_crd[run_p0[r] + off];
+            cand[t] = q;
+            cex[t] = ex_num ? ex_rng[__float_as_int(q.w)] : make_int2(1, 0);
+            cslot[t] = (uint16_t)(run_slot0[r] + off);
+        }
+        __syncthreads();
+        for (int a = wave; a < m && a < SNB_NLC_MAXA; a += (int)(blockDim.x >> 6))
+        {
+            const int s = cb + a;
+            const float4 qi = sorted_crd[s];
+            const int id_i = __float_as_int(qi.w);
+            const int2 rng_i = ex_num ? ex_rng[id_i] : make_int2(1, 0);
+            uint16_t* row = nl16 + (size_t)s * max_nb;
+            int count = rowcnt[a];
+            for (int t0 = 0; t0 < clen; t0 += 64)
+            {
+                const int t = t0 + lane;
+                const int g = chunk0 + t;
                 bool ok = false;
-                if (p < p1)
+                // the cell's own atoms come first (g < m): larger sorted index only
+                if (t < clen && (g >= m || g > a))
                 {
-                    float4 qj = sorted_crd[p];
-                    // displacement from the higher id to the lower, as the
-                    // reference's row atom sees it (same rounding at the wrap)
-                    int id_j = __float_as_int(qj.w);
-                    float4 ql = id_i < id_j ? qi : qj, qh = id_i < id_j ? qj : qi;
-                    float ddx = ql.x - qh.x, ddy = ql.y - qh.y, ddz = ql.z - qh.z;
-                    float sx = (ddx * box.rc[0] + ddy * box.rc[1]) + ddz * box.rc[3];
-                    float sy = ddy * box.rc[2] + ddz * box.rc[4];
-                    float sz = ddz * box.rc[5];
-                    float kx = floorf(sx + 0.5f), ky = floorf(sy + 0.5f), kz = floorf(sz + 0.5f);
-                    ddx = ddx - ((kx * box.c[0] + ky * box.c[1]) + kz * box.c[3]);
-                    ddy = ddy - (ky * box.c[2] + kz * box.c[4]);
-                    ddz = ddz - kz * box.c[5];
+                    const float4 qj = cand[t];
+                    const int id_j = __float_as_int(qj.w);
+                    float ddx = qi.x - qj.x, ddy = qi.y - qj.y, ddz = qi.z - qj.z;
+                    if (ORTHO)
+                    {
+                        float kx = floorf(ddx * box.rc[0] + 0.5f);
+                        float ky = floorf(ddy * box.rc[2] + 0.5f);
+                        float kz = floorf(ddz * box.rc[5] + 0.5f);
+                        ddx = ddx - kx * box.c[0];
+                        ddy = ddy - ky * box.c[2];
+                        ddz = ddz - kz * box.c[5];
+                    }
+                    else
+                    {
+                        float sx = (ddx * box.rc[0] + ddy * box.rc[1]) + ddz * box.rc[3];
+                        float sy = ddy * box.rc[2] + ddz * box.rc[4];
+                        float sz = ddz * box.rc[5];
+                        float kx = floorf(sx + 0.5f), ky = floorf(sy + 0.5f),
+                              kz = floorf(sz + 0.5f);
+                        ddx = ddx - ((kx * box.c[0] + ky * box.c[1]) + kz * box.c[3]);
+                        ddy = ddy - (ky * box.c[2] + kz * box.c[4]);
+                        ddz = ddz - kz * box.c[5];
+                    }
                     ok = (ddx * ddx + ddy * ddy) + ddz * ddz < cs2;
-                    if (ok) ok = !excluded_pair(id_i, id_j, ex_start, ex_list, ex_num);
+                    // the lower id's list decides; its id window rules out
+                    // most pairs without touching the list
+                    const int2 rl = id_i < id_j ? rng_i : cex[t];
+                    const int hi = id_i < id_j ? id_j : id_i;
+                    if (ok && hi >= rl.x && hi <= rl.y)
+                        ok = !excluded_pair(id_i, id_j, ex_start, ex_list, ex_num);
                 }
-                uint64_t mask = __ballot(ok);
+                const uint64_t mask = __ballot(ok);
                 if (ok)
                 {
                     int pos = count + __popcll(mask & lower);
-                    if (pos < max_nb) row[pos] = (uint16_t)(kpref + (p - kfirst));
+                    if (pos < max_nb) row[pos] = cslot[t];
                 }
                 count += __popcll(mask);
             }
+            if (lane == 0) rowcnt[a] = count;
         }
+        __syncthreads();
     }
-    if (lane == 0)
+    for (int a = tid; a < m; a += blockDim.x)
     {
-        row_count[s] = count < max_nb ? count : max_nb;
-        if (count > max_nb) atomicOr(status, SNB_FLAG_NL_OVERFLOW);
+        if (a < SNB_NLC_MAXA)
+        {
+            int cnt = rowcnt[a];
+            row_count[cb + a] = cnt < max_nb ? cnt : max_nb;
+            if (cnt > max_nb) atomicOr(status, SNB_FLAG_NL_OVERFLOW);
+        }
+        else
+        {
+            row_count[cb + a] = 0;
+            atomicOr(status, SNB_FLAG_CELL_OVERFLOW);
+        }
     }
 }
 
-// Number of window atoms of every block.
-__global__ void k_win_sizes(const int* __restrict__ need, SnbWin W, const int* __restrict__ start,
-                            int* __restrict__ win_size)
+// Number of window atoms of every block (one wavefront per block).
+__global__ void __launch_bounds__(64) k_win_sizes(const int* __restrict__ need, SnbWin W,
+                                                  const int* __restrict__ start,
+                                                  int* __restrict__ win_size)
 {
     if (need[0] == 0) return;
-    int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= W.nblock) return;
+    __shared__ int first[SNB_WIN_MAX_RANGES];
+    __shared__ int pref[SNB_WIN_MAX_RANGES + 1];
+    const int w = blockIdx.x, lane = threadIdx.x;
     int x0, y0, z0;
     snb_win_block(W, w, x0, y0, z0);
-    int tot = 0;
-    for (int k = 0; k < W.nrange; k++)
-    {
-        int fc, nc;
-        snb_win_range_cells(W, x0, y0, z0, k, fc, nc);
-        if (nc > 0) tot += start[fc + nc] - start[fc];
-    }
-    win_size[w] = tot;
+    win_table_wave(W, start, x0, y0, z0, lane, first, pref);
+    __syncthreads();
+    if (lane == 0) win_size[w] = pref[SNB_WIN_MAX_RANGES];
 }
 
 // For every window slot of every block, record its stage position in the
 // list of the atom it holds (the per-step gather sums these).
-__global__ void __launch_bounds__(64) k_stage_inverse(const int* __restrict__ need, SnbWin W,
+__global__ void __launch_bounds__(256) k_stage_inverse(const int* __restrict__ need, SnbWin W,
                                                       const int* __restrict__ start,
                                                       const int* __restrict__ stage_base,
                                                       int* __restrict__ inv_count,
@@ -222,13 +305,13 @@ __global__ void __launch_bounds__(64) k_stage_inverse(const int* __restrict__ ne
     if (need[0] == 0) return;
     __shared__ int first[SNB_WIN_MAX_RANGES];
     __shared__ int pref[SNB_WIN_MAX_RANGES + 1];
-    int w = blockIdx.x, lane = threadIdx.x;
+    int w = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int x0, y0, z0;
     snb_win_block(W, w, x0, y0, z0);
-    win_table_wave(W, start, x0, y0, z0, lane, first, pref);
+    if (wave == 0) win_table_wave(W, start, x0, y0, z0, lane, first, pref);
     __syncthreads();
     int base = stage_base[w];
-    for (int k = 0; k < W.nrange; k++)
+    for (int k = wave; k < W.nrange; k += blockDim.x >> 6)
     {
         // ranges beyond nrange are empty, so pref[k + 1] is defined for k < nrange <= 128
         int cnt = pref[k + 1] - pref[k];
@@ -368,6 +451,95 @@ __global__ void __launch_bounds__(64) k_nl_copy_internal(int n, SnbWin W,
             order[win_slot_to_sorted(W, first, pref, nl16[(size_t)s * max_nb + k])];
 }
 
+// Per atom id: first and last entry of its exclusion list (an empty window
+// (1, 0) when it has none), the window the reference tests before searching
+// the list (neighbor_list.cpp:594-665).
+__global__ void k_ex_range(const int* __restrict__ need, int n, const int* __restrict__ ex_start,
+                           const int* __restrict__ ex_list, const int* __restrict__ ex_num,
+                           int2* __restrict__ ex_rng)
+{
+    if (need[0] == 0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int en = ex_num[i];
+    int2 r = make_int2(1, 0);
+    if (en > 0)
+    {
+        int es = ex_start[i];
+        r = make_int2(ex_list[es], ex_list[es + en - 1]);
+    }
+    ex_rng[i] = r;
+}
+
+// List entries owned by every block (one wavefront per block).
+__global__ void __launch_bounds__(64) k_block_work(const int* __restrict__ need, SnbWin W,
+                                                   const int* __restrict__ start,
+                                                   const int* __restrict__ row_count,
+                                                   int* __restrict__ work)
+{
+    if (need[0] == 0) return;
+    const int w = blockIdx.x, lane = threadIdx.x;
+    int x0, y0, z0;
+    snb_win_block(W, w, x0, y0, z0);
+    const int z1 = z0 + W.s < W.nz ? z0 + W.s : W.nz;
+    int sum = 0;
+    for (int col = 0; col < W.a * W.b; col++)
+    {
+        int ua = col / W.b, vb = col - ua * W.b;
+        if (x0 + ua >= W.nx || y0 + vb >= W.ny) continue;
+        int colbase = ((x0 + ua) * W.ny + (y0 + vb)) * W.nz;
+        for (int si = start[colbase + z0] + lane; si < start[colbase + z1]; si += 64)
+            sum += row_count[si];
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if (lane == 0) work[w] = sum;
+}
+
+// Launch order of the pair kernel's blocks: descending work (ties by block
+// index), one workgroup bitonic sort; identity when there are more blocks
+// than SNB_ORDER_CAP.
+#define SNB_ORDER_CAP 4096
+__global__ void __launch_bounds__(1024) k_block_order(const int* __restrict__ need, int nblock,
+                                                      const int* __restrict__ work,
+                                                      int* __restrict__ perm)
+{
+    if (need[0] == 0) return;
+    __shared__ long long key[SNB_ORDER_CAP];
+    if (nblock > SNB_ORDER_CAP)
+    {
+        for (int t = threadIdx.x; t < nblock; t += blockDim.x) perm[t] = t;
+        return;
+    }
+    int P = 1;
+    while (P < nblock) P <<= 1;
+    // key: -work in the high word, index in the low word; padding sorts last
+    for (int t = threadIdx.x; t < P; t += blockDim.x)
+        key[t] = t < nblock ? (-(long long)work[t]) * 4294967296ll + (long long)t
+                            : 0x7fffffffffffffffll;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1)
+        {
+            for (int t = threadIdx.x; t < P; t += blockDim.x)
+            {
+                int q = t ^ j;
+                if (q > t)
+                {
+                    long long a = key[t], b = key[q];
+                    bool up = (t & k) == 0;
+                    if ((a > b) == up)
+                    {
+                        key[t] = b;
+                        key[q] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int t = threadIdx.x; t < nblock; t += blockDim.x)
+        perm[t] = (int)(key[t] & 0xffffffffll);  // low word: the block index (>= 0)
+}
+
 // Clears p[0 .. n) on rebuild steps only.
 __global__ void k_fill_need(const int* __restrict__ need, int* __restrict__ p, int n)
 {
@@ -394,18 +566,32 @@ int snb_launch_nl_window(snb_context* ctx, const SnbBox& box, const int* ex_star
     const SnbWin& W = L.win;
     hipLaunchKernelGGL(k_win_types, dim3((n + 255) / 256), dim3(256), 0, s, L.need, n, L.order,
                        ctx->type, ctx->stype);
-    hipLaunchKernelGGL(k_nl_build_win, dim3(n), dim3(64), 0, s, L.need, n, W, L.cell_start,
-                       L.sorted_cell, L.sorted_crd, box, L.cs2, ex_start, ex_list, ex_num,
-                       L.max_nb, L.nl16, L.row_count, ctx->status);
-    hipLaunchKernelGGL(k_win_sizes, dim3((W.nblock + 255) / 256), dim3(256), 0, s, L.need, W,
-                       L.cell_start, L.win_size);
+    const bool ortho = box.c[1] == 0.0f && box.c[3] == 0.0f && box.c[4] == 0.0f &&
+                       box.rc[1] == 0.0f && box.rc[3] == 0.0f && box.rc[4] == 0.0f;
+    if (ex_num)
+        hipLaunchKernelGGL(k_ex_range, dim3((n + 255) / 256), dim3(256), 0, s, L.need, n,
+                           ex_start, ex_list, ex_num, L.ex_rng);
+    if (ortho)
+        hipLaunchKernelGGL(k_nl_build_cell<true>, dim3(L.ncell), dim3(512), 0, s, L.need, W,
+                           L.cell_start, L.sorted_crd, box, L.cs2, ex_start, ex_list, ex_num,
+                           L.ex_rng, L.max_nb, L.nl16, L.row_count, ctx->status);
+    else
+        hipLaunchKernelGGL(k_nl_build_cell<false>, dim3(L.ncell), dim3(512), 0, s, L.need, W,
+                           L.cell_start, L.sorted_crd, box, L.cs2, ex_start, ex_list, ex_num,
+                           L.ex_rng, L.max_nb, L.nl16, L.row_count, ctx->status);
+    hipLaunchKernelGGL(k_win_sizes, dim3(W.nblock), dim3(64), 0, s, L.need, W, L.cell_start,
+                       L.win_size);
+    hipLaunchKernelGGL(k_block_work, dim3(W.nblock), dim3(64), 0, s, L.need, W, L.cell_start,
+                       L.row_count, L.block_work);
+    hipLaunchKernelGGL(k_block_order, dim3(1), dim3(1024), 0, s, L.need, W.nblock, L.block_work,
+                       L.block_perm);
     SNB_CHECK(hipGetLastError());
     int rc = snb_launch_exclusive_scan(s, L.win_size, L.stage_base, W.nblock);
     if (rc) return rc;
     // the inverse lists persist between rebuilds: clear them on rebuilds only
     hipLaunchKernelGGL(k_fill_need, dim3((n + 255) / 256), dim3(256), 0, s, L.need, L.inv_count,
                        n);
-    hipLaunchKernelGGL(k_stage_inverse, dim3(W.nblock), dim3(64), 0, s, L.need, W, L.cell_start,
+    hipLaunchKernelGGL(k_stage_inverse, dim3(W.nblock), dim3(256), 0, s, L.need, W, L.cell_start,
                        L.stage_base, L.inv_count, L.inv_list, L.invmax, ctx->status);
     SNB_CHECK(hipGetLastError());
     if (L.export_public)

@@ -174,7 +174,8 @@ int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBo
 // evaluated in FP64.  One thread per atom i; i-sums in registers, j-terms by
 // integer atomics (a few per atom).
 template <bool NEED_ENERGY, bool NEED_VIRIAL>
-__global__ void __launch_bounds__(256) k_excl(int n, const float4* __restrict__ atoms,
+__global__ void __launch_bounds__(256) k_excl(int n, const snb_vector* __restrict__ crd,
+                                              const float* __restrict__ charge,
                                               SnbBox box, float beta,
                                               const int* __restrict__ ex_start,
                                               const int* __restrict__ ex_list,
@@ -188,13 +189,15 @@ __global__ void __launch_bounds__(256) k_excl(int n, const float4* __restrict__ 
     int cnt = ex_num[i];
     if (cnt <= 0) return;
     int s0 = ex_start[i];
-    float4 ai = atoms[i];
+    const snb_vector ri = crd[i];
+    const float4 ai = make_float4(ri.x, ri.y, ri.z, charge[i]);
     int64_t sx = 0, sy = 0, sz = 0, e = 0;
     int64_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     for (int k = s0; k < s0 + cnt; k++)
     {
         int j = ex_list[k];
-        float4 aj = atoms[j];
+        const snb_vector rj = crd[j];
+        const float4 aj = make_float4(rj.x, rj.y, rj.z, charge[j]);
         float dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
         pbc_disp(dx, dy, dz, box);
         // FP64 from the float displacement: the erf part of an excluded pair is
@@ -241,7 +244,8 @@ __global__ void __launch_bounds__(256) k_excl(int n, const float4* __restrict__ 
     }
 }
 
-int snb_launch_excl(snb_context* ctx, const float4* atoms, const SnbBox& box, float beta,
+int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge,
+                    const SnbBox& box, float beta,
                     const int* ex_start, const int* ex_list, const int* ex_num,
                     int need_energy, int need_virial)
 {
@@ -250,7 +254,7 @@ int snb_launch_excl(snb_context* ctx, const float4* atoms, const SnbBox& box, fl
     dim3 g((n + 255) / 256);
     snb_prof_begin(ctx, SNB_PROF_EXCL);
 #define SNB_EX_LAUNCH(E, V)                                                                  \
-    hipLaunchKernelGGL((k_excl<E, V>), g, dim3(256), 0, ctx->stream, n, atoms, box, beta,     \
+    hipLaunchKernelGGL((k_excl<E, V>), g, dim3(256), 0, ctx->stream, n, crd, charge, box, beta, \
                        ex_start, ex_list, ex_num, ctx->fx, ctx->e_atom, ctx->vir_atom)
     if (need_energy && need_virial)
         SNB_EX_LAUNCH(true, true);
@@ -452,7 +456,8 @@ __global__ void __launch_bounds__(1024) k_pair_win(
     const float* __restrict__ lj_b, SnbBox box, float rc2, float beta,
     const int* __restrict__ stage_base, long long* __restrict__ stage, size_t stage_cap,
     int64_t* __restrict__ fx, int64_t* __restrict__ e_sorted, int64_t* __restrict__ v_sorted,
-    int ref_attr, int64_t* __restrict__ e_atom, int64_t* __restrict__ vir)
+    int ref_attr, int64_t* __restrict__ e_atom, int64_t* __restrict__ vir,
+    const int* __restrict__ perm)
 {
     extern __shared__ long long smem[];
     long long* acc = smem;                          // [3][wcap]
@@ -461,7 +466,7 @@ __global__ void __launch_bounds__(1024) k_pair_win(
     int* rpref = rfirst + SNB_WIN_MAX_RANGES;       // [129]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nwave = blockDim.x >> 6;
-    const int w = blockIdx.x;
+    const int w = perm[blockIdx.x];  // heaviest blocks first
     int x0, y0, z0;
     snb_win_block(W, w, x0, y0, z0);
     // range table: sizes, then a wave-0 scan
@@ -687,7 +692,7 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
                        ctx->stype, L.order, ctx->lj.a, ctx->lj.b, box, rc2, beta,              \
                        L.stage_base, L.stage, (size_t)n * L.invmax, ctx->fx, ctx->e_sorted,   \
                        ctx->v_sorted, ctx->lj.owner_attribution ? 0 : 1, ctx->e_atom,          \
-                       ctx->vir_atom)
+                       ctx->vir_atom, L.block_perm)
 #define SNB_WIN_LAUNCH_EV(O)                 \
     if (need_energy && need_virial)          \
         SNB_WIN_LAUNCH(true, true, O);       \

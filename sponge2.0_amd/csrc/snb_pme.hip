@@ -9,11 +9,13 @@
 //           +1 neighbours deposit their 4x4x4 B-spline stencils into an int64
 //           fixed-point LDS tile, which is written once (no global atomics,
 //           result independent of atom order; PME_Q_Spread :762-811);
-//   FFT     5 passes of LDS-resident line transforms (radix 4/2/3/5/7
-//           Stockham): z R2C, y, then x forward + influence-function product +
-//           energy/virial accumulation + x inverse fused in one kernel, y
-//           inverse, z C2R (hipFFT R2C/C2R + PME_BCFQ + PME_Sum_Virial,
-//           :1100-1122, :814-823, :1021-1065);
+//   FFT     LDS-resident line transforms (radix 4/2/3/5/7 Stockham): one
+//           workgroup per x-plane does z R2C + y forward; then x forward +
+//           influence-function product + energy/virial accumulation + x
+//           inverse in one kernel; then per x-plane y inverse + z C2R
+//           (hipFFT R2C/C2R + PME_BCFQ + PME_Sum_Virial, :1100-1122,
+//           :814-823, :1021-1065).  Planes too large for LDS take separate z
+//           and y line passes;
 //   gather  one thread per atom sums the 64 stencil points of the potential
 //           grid in a fixed order (PME_Final :826-890) and adds the force in
 //           fixed point.
@@ -385,6 +387,136 @@ __global__ void __launch_bounds__(256) k_fft_x_conv(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused plane passes: one workgroup per x-plane does the z and y transforms
+// in LDS.  Two real z-rows travel as one complex row (row 2p as the real
+// part, row 2p+1 as the imaginary part) and are separated by the conjugate
+// symmetry of real transforms, which halves the z work.
+// LDS: tw_z[fz] | tw_y[fy] | A[cap] | B[cap], cap = max(ceil(fy/2)*fz, fc*fy).
+// ---------------------------------------------------------------------------
+__host__ __device__ static size_t plane_cap(int fy, int fz, int fc)
+{
+    size_t a = (size_t)((fy + 1) / 2) * fz, b = (size_t)fc * fy;
+    return a > b ? a : b;
+}
+
+static size_t plane_lds(int fy, int fz, int fc)
+{
+    return sizeof(float2) * ((size_t)fz + fy + 2 * plane_cap(fy, fz, fc));
+}
+
+// real[x][fy][fz] -> spec[x][fy][fc] (z R2C then y forward), exponent -1
+__global__ void __launch_bounds__(1024) k_fft_plane_fwd(const float* __restrict__ real,
+                                                        float2* __restrict__ spec, int fy,
+                                                        int fz, int fc, SnbFftPlan plan_z,
+                                                        SnbFftPlan plan_y,
+                                                        const float2* __restrict__ twzg,
+                                                        const float2* __restrict__ twyg)
+{
+    extern __shared__ float2 smem[];
+    const int cap = (int)plane_cap(fy, fz, fc);
+    float2* twz = smem;
+    float2* twy = twz + fz;
+    float2* A = twy + fy;
+    float2* B = A + cap;
+    load_twiddles(twz, twzg, fz);
+    load_twiddles(twy, twyg, fy);
+    const int x = blockIdx.x;
+    const int np = (fy + 1) / 2;
+    const float* plane = real + (size_t)x * fy * fz;
+    for (int t = threadIdx.x; t < np * fz; t += blockDim.x)
+    {
+        int p = t / fz, z = t - p * fz;
+        float re = plane[(size_t)(2 * p) * fz + z];
+        float im = (2 * p + 1 < fy) ? plane[(size_t)(2 * p + 1) * fz + z] : 0.0f;
+        A[t] = make_float2(re, im);
+    }
+    __syncthreads();
+    float2* res = fft_lines(A, B, plan_z, np, twz, -1.0f);
+    float2* O = (res == A) ? B : A;
+    // separate: Xa[k] = (Z[k] + conj Z[-k]) / 2, Xb[k] = (Z[k] - conj Z[-k]) / (2i);
+    // write as y-lines O[kz][y]
+    for (int t = threadIdx.x; t < np * fc; t += blockDim.x)
+    {
+        int p = t / fc, k = t - p * fc;
+        float2 zk = res[p * fz + k];
+        float2 zm = res[p * fz + (k == 0 ? 0 : fz - k)];
+        float2 xa = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        float2 xb = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
+        O[k * fy + 2 * p] = xa;
+        if (2 * p + 1 < fy) O[k * fy + 2 * p + 1] = xb;
+    }
+    __syncthreads();
+    float2* res2 = fft_lines(O, res, plan_y, fc, twy, -1.0f);
+    float2* out = spec + (size_t)x * fy * fc;
+    for (int t = threadIdx.x; t < fy * fc; t += blockDim.x)
+    {
+        int y = t / fc, k = t - y * fc;
+        out[t] = res2[k * fy + y];
+    }
+}
+
+// spec[x][fy][fc] -> real[x][fy][fz] (y inverse then z C2R), exponent +1
+__global__ void __launch_bounds__(1024) k_fft_plane_inv(const float2* __restrict__ spec,
+                                                        float* __restrict__ real, int fy,
+                                                        int fz, int fc, SnbFftPlan plan_z,
+                                                        SnbFftPlan plan_y,
+                                                        const float2* __restrict__ twzg,
+                                                        const float2* __restrict__ twyg)
+{
+    extern __shared__ float2 smem[];
+    const int cap = (int)plane_cap(fy, fz, fc);
+    float2* twz = smem;
+    float2* twy = twz + fz;
+    float2* A = twy + fy;
+    float2* B = A + cap;
+    load_twiddles(twz, twzg, fz);
+    load_twiddles(twy, twyg, fy);
+    const int x = blockIdx.x;
+    const int np = (fy + 1) / 2;
+    const float2* in = spec + (size_t)x * fy * fc;
+    for (int t = threadIdx.x; t < fy * fc; t += blockDim.x)
+    {
+        int y = t / fc, k = t - y * fc;
+        A[k * fy + y] = in[t];
+    }
+    __syncthreads();
+    float2* res = fft_lines(A, B, plan_y, fc, twy, 1.0f);
+    float2* O = (res == A) ? B : A;
+    // Hermitian extension of rows 2p and 2p+1 along z, packed as Ya + i Yb.
+    // The imaginary parts at kz = 0 and at the Nyquist plane are dropped,
+    // which is what taking the real part of a C2R output does.
+    const bool nyq = (fz % 2) == 0;
+    for (int t = threadIdx.x; t < np * fz; t += blockDim.x)
+    {
+        int p = t / fz, k = t - p * fz;
+        int kk = k < fc ? k : fz - k;
+        float2 a = res[kk * fy + 2 * p];
+        float2 b = (2 * p + 1 < fy) ? res[kk * fy + 2 * p + 1] : make_float2(0.0f, 0.0f);
+        if (k >= fc)
+        {
+            a.y = -a.y;
+            b.y = -b.y;
+        }
+        if (k == 0 || (nyq && 2 * k == fz))
+        {
+            a.y = 0.0f;
+            b.y = 0.0f;
+        }
+        O[t] = make_float2(a.x - b.y, a.y + b.x);
+    }
+    __syncthreads();
+    float2* res2 = fft_lines(O, res, plan_z, np, twz, 1.0f);
+    float* plane = real + (size_t)x * fy * fz;
+    for (int t = threadIdx.x; t < np * fz; t += blockDim.x)
+    {
+        int p = t / fz, z = t - p * fz;
+        float2 v = res2[t];
+        plane[(size_t)(2 * p) * fz + z] = v.x;
+        if (2 * p + 1 < fy) plane[(size_t)(2 * p + 1) * fz + z] = v.y;
+    }
+}
+
 // Fixed-order reduction of the per-block partials into dscal[0..6] and the
 // self + neutralising energy into dscal[7] (PM_force.cpp:1146-1159).
 __global__ void k_pme_finalize(const double* __restrict__ partials, int nblocks,
@@ -446,21 +578,39 @@ struct SnbRc
     float v[6];
 };
 
-__global__ void k_pme_prep(int n, const float4* __restrict__ atoms, SnbRc rc, int fx, int fy,
+__global__ void k_pme_prep(int n, const snb_vector* __restrict__ crd,
+                           const float* __restrict__ charge, SnbRc rc, int fx, int fy,
                            int fz, int bx, int by, int bz, int nby, int nbz,
                            int4* __restrict__ atom_u, float4* __restrict__ atom_fr,
                            int* __restrict__ slot, int* __restrict__ cnt)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float4 r = atoms[i];
+    const snb_vector ri = crd[i];
+    float4 r = make_float4(ri.x, ri.y, ri.z, charge[i]);
     int ux, uy, uz;
     float a, b, c;
     atom_near(r, rc.v, fx, fy, fz, ux, uy, uz, a, b, c);
     int br = ((ux / bx) * nby + uy / by) * nbz + uz / bz;
     atom_u[i] = make_int4(ux, uy, uz, br);
     atom_fr[i] = make_float4(a, b, c, r.w);
-    slot[i] = atomicAdd(&cnt[br], 1);
+    // one counter atomic per distinct brick of the wavefront (neighbouring
+    // ids are mostly in the same brick, so per-atom atomics would queue on
+    // one address)
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long todo = __ballot(1);
+    while (todo)
+    {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int b0 = __shfl(br, leader);
+        const unsigned long long same = __ballot(br == b0) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[b0], __popcll(same));
+        base = __shfl(base, leader);
+        if (br == b0 && ((todo >> lane) & 1ull)) slot[i] = base + __popcll(same & below);
+        todo &= ~same;
+    }
 }
 
 __global__ void k_pme_scatter(int n, const int4* __restrict__ atom_u, const int* __restrict__ slot,
@@ -714,10 +864,65 @@ int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box)
 
 static size_t lds_lines(int n, int L) { return sizeof(float2) * ((size_t)n + 2 * (size_t)L * n); }
 
+// Whether one x-plane of the grid fits the fused plane kernels' LDS; grants
+// the dynamic LDS on first use.
+static int plane_ok(snb_context* ctx, bool* ok)
+{
+    SnbPme& P = ctx->pme;
+    size_t lds = plane_lds(P.fy, P.fz, P.fc);
+    *ok = lds <= (size_t)ctx->max_lds;
+    if (*ok && P.plane_lds_set != lds)
+    {
+        SNB_CHECK(hipFuncSetAttribute((const void*)k_fft_plane_fwd,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        SNB_CHECK(hipFuncSetAttribute((const void*)k_fft_plane_inv,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        P.plane_lds_set = lds;
+    }
+    return SNB_OK;
+}
+
+// y inverse + z C2R: spec[fx][fy][fc] -> real[fx][fy][fz]
+static int fft_yz_inverse(snb_context* ctx, float2* cplx, float* real)
+{
+    SnbPme& P = ctx->pme;
+    hipStream_t s = ctx->stream;
+    bool ok;
+    int rc = plane_ok(ctx, &ok);
+    if (rc) return rc;
+    if (ok)
+    {
+        hipLaunchKernelGGL(k_fft_plane_inv, dim3(P.fx), dim3(1024), plane_lds(P.fy, P.fz, P.fc),
+                           s, cplx, real, P.fy, P.fz, P.fc, P.plan_z, P.plan_y, P.tw_z, P.tw_y);
+        SNB_CHECK(hipGetLastError());
+        return SNB_OK;
+    }
+    int W = fft_w(P.fc, P.fy);
+    int nchunk = (P.fc + W - 1) / W;
+    hipLaunchKernelGGL(k_fft_y, dim3(P.fx * nchunk), dim3(256), lds_lines(P.fy, W), s, cplx,
+                       P.fx, P.fy, P.fc, P.plan_y, P.tw_y, W, 1.0f);
+    int nrow = P.fx * P.fy;
+    int Lz = lines_per_block(P.fz);
+    hipLaunchKernelGGL(k_fft_z_c2r, dim3((nrow + Lz - 1) / Lz), dim3(256), lds_lines(P.fz, Lz),
+                       s, cplx, real, nrow, P.plan_z, P.fc, P.tw_z, Lz);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
 int snb_launch_fft_r2c(snb_context* ctx, const float* real, float2* cplx)
 {
     SnbPme& P = ctx->pme;
     hipStream_t s = ctx->stream;
+    bool ok;
+    int prc = plane_ok(ctx, &ok);
+    if (prc) return prc;
+    if (ok)
+    {
+        hipLaunchKernelGGL(k_fft_plane_fwd, dim3(P.fx), dim3(1024), plane_lds(P.fy, P.fz, P.fc),
+                           s, real, cplx, P.fy, P.fz, P.fc, P.plan_z, P.plan_y, P.tw_z, P.tw_y);
+        SNB_CHECK(hipGetLastError());
+        return SNB_OK;
+    }
     int nrow = P.fx * P.fy;
     int Lz = lines_per_block(P.fz);
     hipLaunchKernelGGL(k_fft_z_r2c, dim3((nrow + Lz - 1) / Lz), dim3(256), lds_lines(P.fz, Lz),
@@ -778,16 +983,7 @@ int snb_launch_fft_c2r(snb_context* ctx, const float2* cplx_in, float* real)
     SNB_CHECK(hipMemcpyAsync(P.spec, cplx_in, sizeof(float2) * nfft, hipMemcpyDeviceToDevice, s));
     int rc = fft_x_plain(ctx, P.spec, 1.0f);
     if (rc) return rc;
-    int W = fft_w(P.fc, P.fy);
-    int nchunk = (P.fc + W - 1) / W;
-    hipLaunchKernelGGL(k_fft_y, dim3(P.fx * nchunk), dim3(256), lds_lines(P.fy, W), s, P.spec,
-                       P.fx, P.fy, P.fc, P.plan_y, P.tw_y, W, 1.0f);
-    int nrow = P.fx * P.fy;
-    int Lz = lines_per_block(P.fz);
-    hipLaunchKernelGGL(k_fft_z_c2r, dim3((nrow + Lz - 1) / Lz), dim3(256), lds_lines(P.fz, Lz),
-                       s, P.spec, real, nrow, P.plan_z, P.fc, P.tw_z, Lz);
-    SNB_CHECK(hipGetLastError());
-    return SNB_OK;
+    return fft_yz_inverse(ctx, P.spec, real);
 }
 
 int snb_fft_forward_full(snb_context* ctx, const float* real, float2* cplx)
@@ -797,8 +993,8 @@ int snb_fft_forward_full(snb_context* ctx, const float* real, float2* cplx)
     return fft_x_plain(ctx, cplx, -1.0f);
 }
 
-int snb_launch_pme_recip(snb_context* ctx, const SnbBox& box, int need_energy, int need_virial,
-                         int with_force)
+int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* charge,
+                         const SnbBox& box, int need_energy, int need_virial, int with_force)
 {
     SnbPme& P = ctx->pme;
     hipStream_t s = ctx->stream;
@@ -810,7 +1006,7 @@ int snb_launch_pme_recip(snb_context* ctx, const SnbBox& box, int need_energy, i
     if (n > 0)
     {
         if ((rc = snb_launch_fill_int(s, P.brick_count, P.nbrick + 1, 0))) return rc;
-        hipLaunchKernelGGL(k_pme_prep, dim3((n + 255) / 256), dim3(256), 0, s, n, ctx->atom4, r6,
+        hipLaunchKernelGGL(k_pme_prep, dim3((n + 255) / 256), dim3(256), 0, s, n, crd, charge, r6,
                            P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nby, P.nbz, P.atom_u,
                            P.atom_fr, P.atom_slot, P.brick_count);
         if ((rc = snb_launch_exclusive_scan(s, P.brick_count, P.brick_start, P.nbrick))) return rc;
@@ -844,15 +1040,7 @@ int snb_launch_pme_recip(snb_context* ctx, const SnbBox& box, int need_energy, i
                            (double)box.rc[3], (double)box.rc[4], (double)box.rc[5], mpre,
                            P.partials);
     SNB_CHECK(hipGetLastError());
-    int Wy = fft_w(P.fc, P.fy);
-    int nchunky = (P.fc + Wy - 1) / Wy;
-    hipLaunchKernelGGL(k_fft_y, dim3(P.fx * nchunky), dim3(256), lds_lines(P.fy, Wy), s, P.spec,
-                       P.fx, P.fy, P.fc, P.plan_y, P.tw_y, Wy, 1.0f);
-    int nrow = P.fx * P.fy;
-    int Lz = lines_per_block(P.fz);
-    hipLaunchKernelGGL(k_fft_z_c2r, dim3((nrow + Lz - 1) / Lz), dim3(256), lds_lines(P.fz, Lz),
-                       s, P.spec, P.phi, nrow, P.plan_z, P.fc, P.tw_z, Lz);
-    SNB_CHECK(hipGetLastError());
+    if ((rc = fft_yz_inverse(ctx, P.spec, P.phi))) return rc;
     if (n > 0 && with_force)
         hipLaunchKernelGGL(k_pme_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, P.atom_u,
                            P.atom_fr, P.phi, P.fx, P.fy, P.fz, r6, (float)P.update_interval,

@@ -257,6 +257,35 @@ class Nonbonded:
         self.virial.zero_()
         self.potential.zero_()
 
+    def set_stream(self, stream):
+        """Points the library context at another torch stream."""
+        check(self.L.snb_set_stream(self.ctx, ctypes.c_void_p(stream.cuda_stream)),
+              "snb_set_stream")
+        self.stream = stream
+
+    def capture_step(self, **step_kwargs):
+        """Records one step() (same keyword arguments) as a HIP graph.
+
+        Capture runs on a side stream that the context is pointed at for the
+        duration; the graph is replayed with graph.replay() on torch's current
+        stream, which is the context's stream.  Host-side decisions inside the
+        step (neighbour-list rebuild or not, energy or not) are fixed at
+        capture: record one graph per kind of step.  The library issues no
+        memset nodes (its clears are kernels)."""
+        torch = self.torch
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(device=self.dev)
+        home = self.stream
+        torch.cuda.synchronize()
+        self.set_stream(side)
+        try:
+            with torch.cuda.graph(graph, stream=side):
+                self.step(**step_kwargs)
+        finally:
+            self.set_stream(home)
+        torch.cuda.synchronize()
+        return graph
+
     def step(self, step=0, need_energy=False, need_virial=False, forced_nl=False, fused=True):
         """One force evaluation of the path in the reference's order
         (main.cpp:307-334, :371-396, :474-478).  fused=True keeps every term

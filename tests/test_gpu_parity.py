@@ -433,3 +433,59 @@ def test_window_93k_workload_pair_exact(request):
     np.testing.assert_array_equal(g_f, t_f)
     np.testing.assert_array_equal(nb.fixed_atom_terms(0), t_elj)
     nb.close()
+
+
+# --------------------------------------------------------------------------
+# HIP graph replay of a whole step
+# --------------------------------------------------------------------------
+def test_graph_replay_matches_eager_every_replay(water_9k):
+    """A captured force step replays to the same float forces as the eager
+    step, on every replay, with eager work and synchronisations in between."""
+    import torch
+    nb = _nb(water_9k, refresh_interval=10)
+    nb.set_export(False)
+    nb.step(step=0, forced_nl=True)          # builds the list (step 0 rebuilds)
+    torch.cuda.synchronize()
+    nb.zero_outputs()
+    nb.step(step=1)                          # eager reference: no rebuild, no energy
+    torch.cuda.synchronize()
+    ref = nb.frc.cpu().numpy().copy()
+    graph = nb.capture_step(step=1)
+    for rep in range(4):
+        nb.zero_outputs()
+        graph.replay()
+        torch.cuda.synchronize()
+        got = nb.frc.cpu().numpy()
+        print("replay", rep, flush=True)
+        np.testing.assert_array_equal(got, ref)
+        # eager work between replays
+        nb.zero_outputs()
+        nb.step(step=1 + rep)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(nb.frc.cpu().numpy(), ref)
+    assert nb.status() == 0
+    nb.close()
+
+
+def test_graph_replay_of_rebuild_and_energy_steps(water_9k):
+    """Rebuild and energy steps captured as graphs give the eager results."""
+    import torch
+    nb = _nb(water_9k, refresh_interval=10)
+    nb.set_export(False)
+    nb.step(step=0, forced_nl=True)
+    torch.cuda.synchronize()
+    nb.zero_outputs()
+    nb.step(step=9, need_energy=True, need_virial=True)   # (9+1) % 10 == 0: rebuild
+    torch.cuda.synchronize()
+    ref_f = nb.frc.cpu().numpy().copy()
+    ref_e = nb.energies()
+    ref_v = nb.atom_virial.cpu().numpy().copy()
+    graph = nb.capture_step(step=9, need_energy=True, need_virial=True)
+    for rep in range(3):
+        nb.zero_outputs()
+        graph.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(nb.frc.cpu().numpy(), ref_f)
+        np.testing.assert_array_equal(nb.atom_virial.cpu().numpy(), ref_v)
+        assert nb.energies() == ref_e
+    nb.close()
