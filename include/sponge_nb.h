@@ -66,6 +66,16 @@ const char* snb_status_string(int code);
 int snb_context_create(snb_context** out, int device, void* hip_stream, int atom_numbers);
 int snb_context_destroy(snb_context* ctx);
 int snb_set_stream(snb_context* ctx, void* hip_stream);
+/* Runs the PME reciprocal pipeline on a second, library-owned stream beside
+ * the direct-space work (default 1).  It starts once the step's coordinates
+ * are final -- at the last snb_nl_update, or at the reciprocal call when no
+ * update came since the previous one -- and every later call that reads its
+ * results (force flush, copies, energies, status) waits for it on the
+ * context stream.  With overlap on, crd and charge must not change between a
+ * step's snb_nl_update and its snb_pme_reciprocal_force, as in the
+ * reference's step loop.  Results are identical either way (no counterpart
+ * in the reference, which runs everything on the null stream). */
+int snb_set_pme_overlap(snb_context* ctx, int enable);
 int snb_synchronize(snb_context* ctx);
 /* Reads (and with clear != 0, resets) the sticky SNB_FLAG_* bits; synchronises. */
 int snb_read_status(snb_context* ctx, int clear, int* flags);
@@ -134,6 +144,12 @@ int snb_nl_copy_internal(snb_context* ctx, int* d_counts, int* d_rows);
  * totals with fewer atomics on energy/virial steps (no counterpart in the
  * reference). */
 int snb_lj_set_owner_attribution(snb_context* ctx, int enable);
+/* LDS slots the windowed pair kernel reserves per block (default 1.3x the
+ * mean window, at most what the device's LDS holds).  Blocks whose window is
+ * larger take a slower path with global integer atomics; results are the same.
+ * A smaller reservation lets more work of the reciprocal stream share a
+ * compute unit (no counterpart in the reference). */
+int snb_nl_set_window_capacity(snb_context* ctx, int slots);
 /* Geometry of the windowed pair path (windowed = 0 when not used). */
 int snb_nl_window_info(snb_context* ctx, int* windowed, int* a, int* b, int* s, int* wcap,
                        int* nblock);

@@ -175,7 +175,11 @@ bool choose_window_pass(int nx, int ny, int nz, double atoms_per_cell, int max_l
         W->wv = b + 4;
         W->wz = s + 4;
         W->nrange = 2 * W->wu * W->wv;
-        W->wcap = wcap;
+        // request LDS for 1.3x the mean window (the densest windows of the
+        // test systems reach 1.15x; a larger one takes the overflow path) so
+        // that small kernels of the reciprocal stream can share the CU
+        int want = (int)(1.3 * expect) + 64;
+        W->wcap = want < wcap ? want : wcap;
         return true;
     }
     return false;
@@ -225,6 +229,14 @@ int prof_collect(snb_context* c, int g)
     return SNB_OK;
 }
 }  // namespace
+
+int snb_join_pme(snb_context* c)
+{
+    if (!c->pme_pending) return SNB_OK;
+    SNB_CHECK(hipStreamWaitEvent(c->stream, c->ev_pme, 0));
+    c->pme_pending = false;
+    return SNB_OK;
+}
 
 void snb_prof_begin(snb_context* c, int g)
 {
@@ -296,6 +308,13 @@ int snb_context_create(snb_context** out, int device, void* hip_stream, int atom
     if (!rc) rc = dalloc(&c->type, n);
     if (!rc) rc = dalloc(&c->status, 1);
     if (!rc) rc = dalloc(&c->fx, 3 * (size_t)n);
+    if (!rc) rc = dalloc(&c->fx_pme, 3 * (size_t)n);
+    if (!rc && hipStreamCreateWithFlags(&c->pme_stream, hipStreamNonBlocking) != hipSuccess)
+        rc = SNB_ERR_HIP;
+    if (!rc && hipEventCreateWithFlags(&c->ev_inputs, hipEventDisableTiming) != hipSuccess)
+        rc = SNB_ERR_HIP;
+    if (!rc && hipEventCreateWithFlags(&c->ev_pme, hipEventDisableTiming) != hipSuccess)
+        rc = SNB_ERR_HIP;
     if (!rc) rc = dalloc(&c->e_atom, 3 * (size_t)n);
     if (!rc) rc = dalloc(&c->vir_atom, 6 * (size_t)n);
     if (!rc) rc = dalloc(&c->satom, n);
@@ -313,6 +332,7 @@ int snb_context_create(snb_context** out, int device, void* hip_stream, int atom
     snb_launch_fill_int(s, c->status, 1, 0);
     snb_launch_fill_int(s, c->type, n, 0);
     snb_launch_fill_i64(s, c->fx, 3 * (size_t)n);
+    snb_launch_fill_i64(s, c->fx_pme, 3 * (size_t)n);
     snb_launch_fill_i64(s, c->e_atom, 3 * (size_t)n);
     snb_launch_fill_i64(s, c->vir_atom, 6 * (size_t)n);
     snb_launch_fill_i64(s, c->scal, SNB_S_COUNT);
@@ -331,6 +351,11 @@ int snb_context_destroy(snb_context* c)
     if (!c) return SNB_OK;
     DeviceGuard g(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->pme_stream) (void)hipStreamSynchronize(c->pme_stream);
+    if (c->ev_inputs) (void)hipEventDestroy(c->ev_inputs);
+    if (c->ev_pme) (void)hipEventDestroy(c->ev_pme);
+    if (c->pme_stream) (void)hipStreamDestroy(c->pme_stream);
+    dfree(c->fx_pme);
     dfree(c->atom4);
     dfree(c->type);
     dfree(c->status);
@@ -364,6 +389,17 @@ int snb_set_stream(snb_context* c, void* hip_stream)
 {
     if (!c) return SNB_ERR_ARGUMENT;
     c->stream = (hipStream_t)hip_stream;
+    c->inputs_marked = false;  // an inputs event recorded on the old stream is not reused
+    return SNB_OK;
+}
+
+int snb_set_pme_overlap(snb_context* c, int enable)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
+    c->overlap = enable != 0;
     return SNB_OK;
 }
 
@@ -371,6 +407,8 @@ int snb_synchronize(snb_context* c)
 {
     if (!c) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
     SNB_CHECK(hipStreamSynchronize(c->stream));
     return SNB_OK;
 }
@@ -379,6 +417,8 @@ int snb_read_status(snb_context* c, int clear, int* flags)
 {
     if (!c || !flags) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
     SNB_CHECK(hipStreamSynchronize(c->stream));
     SNB_CHECK(hipMemcpy(flags, c->status, sizeof(int), hipMemcpyDeviceToHost));
     if (clear)
@@ -527,6 +567,13 @@ int snb_nl_update(snb_context* c, const int* d_atom_local, int local_atom_number
     const int* el = ex_or(d_ex_list, c->ex_list);
     const int* en = ex_or(d_ex_num, c->ex_num);
     if (!d_ex_num && !c->has_excl) es = el = en = nullptr;
+    // the step's coordinates are final here: the reciprocal pipeline of this
+    // step may start from this point of the stream
+    if (c->overlap)
+    {
+        SNB_CHECK(hipEventRecord(c->ev_inputs, c->stream));
+        c->inputs_marked = true;
+    }
     int forced = (update || !c->nl.built) ? 1 : 0;
     // With a fixed refresh interval the host knows the decision
     // (neighbor_list.cpp:843-856): a step that does not rebuild launches
@@ -580,6 +627,21 @@ int snb_nl_set_export(snb_context* c, int enable)
 {
     if (!c) return SNB_ERR_ARGUMENT;
     c->nl.export_public = enable != 0;
+    return SNB_OK;
+}
+
+int snb_nl_set_window_capacity(snb_context* c, int slots)
+{
+    if (!c || slots < 64) return SNB_ERR_ARGUMENT;
+    if (!c->nl.init) return SNB_ERR_NOT_INITIALIZED;
+    if (!c->nl.windowed) return SNB_OK;
+    int most = (c->max_lds - (int)sizeof(int) * (2 * SNB_WIN_MAX_RANGES + 1)) / 28;
+    if (most > 65535) most = 65535;
+    DeviceGuard g(c->device);
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
+    SNB_CHECK(hipStreamSynchronize(c->stream));  // no pair launch in flight uses the old size
+    c->nl.win.wcap = slots < most ? slots : most;
     return SNB_OK;
 }
 
@@ -838,15 +900,36 @@ int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatr
     // coordinates, the reference reuses the grids of the last update step).
     bool do_force = (step % P.update_interval) == 0;
     if (!do_force) need_virial = 0;
-    if (need_energy && (rc = snb_launch_charge_sums(c, d_charge))) return rc;
     if (do_force || need_energy)
     {
-        if ((rc = snb_launch_pme_recip(c, d_crd, d_charge, box, need_energy, need_virial,
-                                       do_force ? 1 : 0)))
-            return rc;
+        // the pipeline reads only crd/charge and module buffers and writes
+        // fx_pme / dscal / the Q slots, so it may run beside the direct-space
+        // work: on pme_stream, after the step's inputs are final
+        hipStream_t home = c->stream;
+        if (c->overlap)
+        {
+            if ((rc = snb_join_pme(c))) return rc;  // the previous call's results are consumed
+            if (!c->inputs_marked) SNB_CHECK(hipEventRecord(c->ev_inputs, home));
+            SNB_CHECK(hipStreamWaitEvent(c->pme_stream, c->ev_inputs, 0));
+            c->inputs_marked = false;
+            c->stream = c->pme_stream;
+        }
+        rc = need_energy ? snb_launch_charge_sums(c, d_charge) : SNB_OK;
+        if (!rc)
+            rc = snb_launch_pme_recip(c, d_crd, d_charge, box, need_energy, need_virial,
+                                      do_force ? 1 : 0);
+        if (c->overlap)
+        {
+            c->stream = home;
+            if (!rc && hipEventRecord(c->ev_pme, c->pme_stream) != hipSuccess) rc = SNB_ERR_HIP;
+            if (!rc) c->pme_pending = true;
+        }
+        if (rc) return rc;
         P.have_energy = need_energy != 0;
     }
     if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
+    if ((d_potential && need_energy) || (d_virial && need_virial))
+        if ((rc = snb_join_pme(c))) return rc;
     if (need_energy && d_potential && (rc = snb_launch_pme_energy_to_potential(c, d_potential)))
         return rc;
     if (need_virial && d_virial && (rc = snb_launch_pme_virial_to(c, d_virial))) return rc;
@@ -858,6 +941,8 @@ int snb_pme_energy(snb_context* c, double* direct, double* reciprocal, double* s
 {
     if (!c) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
     SNB_CHECK(hipStreamSynchronize(c->stream));
     int64_t s[SNB_S_COUNT];
     double d[SNB_D_COUNT];
@@ -874,6 +959,8 @@ int snb_pme_reciprocal_virial(snb_context* c, double* v6)
 {
     if (!c || !v6) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
     SNB_CHECK(hipStreamSynchronize(c->stream));
     double d[SNB_D_COUNT];
     SNB_CHECK(hipMemcpy(d, c->dscal, sizeof(d), hipMemcpyDeviceToHost));
@@ -907,15 +994,18 @@ int snb_copy_fixed_forces(snb_context* c, int64_t* d_out)
 {
     if (!c || !d_out) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
-    SNB_CHECK(hipMemcpyAsync(d_out, c->fx, sizeof(int64_t) * 3 * (size_t)c->n,
-                             hipMemcpyDeviceToDevice, c->stream));
-    return SNB_OK;
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
+    return snb_launch_add_i64(c->stream, c->fx, c->fx_pme, d_out, 3 * (size_t)c->n);
 }
 
 int snb_clear_fixed_forces(snb_context* c)
 {
     if (!c) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
+    if ((rc = snb_launch_fill_i64(c->stream, c->fx_pme, 3 * (size_t)c->n))) return rc;
     return snb_launch_fill_i64(c->stream, c->fx, 3 * (size_t)c->n);
 }
 
@@ -935,6 +1025,8 @@ int snb_copy_pme_grid(snb_context* c, int which, float* d_out)
     if (!c || !d_out || which < 0 || which > 1) return SNB_ERR_ARGUMENT;
     if (!c->pme.init) return SNB_ERR_NOT_INITIALIZED;
     DeviceGuard g(c->device);
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
     size_t nall = (size_t)c->pme.fx * c->pme.fy * c->pme.fz;
     SNB_CHECK(hipMemcpyAsync(d_out, which == 0 ? c->pme.q : c->pme.phi, sizeof(float) * nall,
                              hipMemcpyDeviceToDevice, c->stream));
@@ -946,6 +1038,8 @@ int snb_fft_r2c(snb_context* c, const float* d_real, float* d_complex)
     if (!c || !d_real || !d_complex) return SNB_ERR_ARGUMENT;
     if (!c->pme.init) return SNB_ERR_NOT_INITIALIZED;
     DeviceGuard g(c->device);
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
     return snb_fft_forward_full(c, d_real, (float2*)d_complex);
 }
 
@@ -954,6 +1048,8 @@ int snb_fft_c2r(snb_context* c, const float* d_complex, float* d_real)
     if (!c || !d_real || !d_complex) return SNB_ERR_ARGUMENT;
     if (!c->pme.init) return SNB_ERR_NOT_INITIALIZED;
     DeviceGuard g(c->device);
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
     return snb_launch_fft_c2r(c, (const float2*)d_complex, d_real);
 }
 
@@ -977,6 +1073,8 @@ int snb_profile_read(snb_context* c, int group, int reset, double* total_ms, int
 {
     if (!c || group < 0 || group >= SNB_PROF_GROUPS) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
+    int rc0 = snb_join_pme(c);
+    if (rc0) return rc0;
     SNB_CHECK(hipStreamSynchronize(c->stream));
     SnbProf& P = c->prof;
     if (P.ev)

@@ -48,6 +48,22 @@ int snb_launch_fill_int(hipStream_t s, int* p, int n, int v)
     return SNB_OK;
 }
 
+__global__ void k_add_i64(const int64_t* __restrict__ a, const int64_t* __restrict__ b,
+                          int64_t* __restrict__ out, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] + b[i];
+}
+
+int snb_launch_add_i64(hipStream_t s, const int64_t* a, const int64_t* b, int64_t* out, size_t n)
+{
+    if (n == 0) return SNB_OK;
+    hipLaunchKernelGGL(k_add_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, out,
+                       n);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
 int snb_launch_fill_i64(hipStream_t s, int64_t* p, size_t n)
 {
     if (n == 0) return SNB_OK;
@@ -95,26 +111,32 @@ int snb_launch_exclusive_scan(hipStream_t s, const int* in, int* out, int n)
 }
 
 // frc[i] += fixed[i] * 2^-32 and clears the accumulator.
-__global__ void k_flush_forces(int n, int64_t* __restrict__ fx, snb_vector* __restrict__ frc)
+__global__ void k_flush_forces(int n, int64_t* __restrict__ fx, int64_t* __restrict__ fx_pme,
+                               snb_vector* __restrict__ frc)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     snb_vector f = frc[i];
-    f.x += snb_fixed_to_float(fx[i]);
-    f.y += snb_fixed_to_float(fx[n + i]);
-    f.z += snb_fixed_to_float(fx[2 * n + i]);
+    f.x += snb_fixed_to_float(fx[i] + fx_pme[i]);
+    f.y += snb_fixed_to_float(fx[n + i] + fx_pme[n + i]);
+    f.z += snb_fixed_to_float(fx[2 * n + i] + fx_pme[2 * n + i]);
     frc[i] = f;
     fx[i] = 0;
     fx[n + i] = 0;
     fx[2 * n + i] = 0;
+    fx_pme[i] = 0;
+    fx_pme[n + i] = 0;
+    fx_pme[2 * n + i] = 0;
 }
 
 int snb_launch_flush_forces(snb_context* ctx, snb_vector* frc)
 {
     int n = ctx->n;
     if (n <= 0) return SNB_OK;
+    int rc = snb_join_pme(ctx);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_flush_forces, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
-                       ctx->fx, frc);
+                       ctx->fx, ctx->fx_pme, frc);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }

@@ -150,7 +150,9 @@ struct snb_context
     int max_lds = 65536;         // dynamic LDS bytes per workgroup the device allows
     int num_cu = 256;            // compute units of the device
     int* status = nullptr;
-    int64_t* fx = nullptr;       // 3n, SoA
+    int64_t* fx = nullptr;       // 3n, SoA: direct-space and exclusion forces
+    int64_t* fx_pme = nullptr;   // 3n, SoA: reciprocal forces (own accumulator: the PME
+                                 // pipeline may run concurrently on pme_stream)
     int64_t* e_atom = nullptr;   // 3n: LJ | direct | correction
     int64_t* vir_atom = nullptr; // 6n, SoA
     int64_t* scal = nullptr;     // SNB_S_COUNT
@@ -163,11 +165,26 @@ struct snb_context
     int* ex_num = nullptr;
     int ex_total = 0;
     bool has_excl = false;
+    // Reciprocal pipeline on a second stream: it waits for ev_inputs (recorded
+    // on `stream` when the step's coordinates are final -- at the NL update,
+    // the first call of a step -- or at the PME call itself) and signals
+    // ev_pme; everything that reads PME results on `stream` waits for ev_pme
+    // first (snb_join_pme).
+    bool overlap = true;
+    hipStream_t pme_stream = nullptr;
+    hipEvent_t ev_inputs = nullptr;
+    hipEvent_t ev_pme = nullptr;
+    bool inputs_marked = false;
+    bool pme_pending = false;
     SnbNl nl;
     SnbLj lj;
     SnbPme pme;
     SnbProf prof;
 };
+
+// Makes `stream` wait for the reciprocal pipeline launched last (no-op when
+// none is pending).
+int snb_join_pme(snb_context* ctx);
 
 // Profiling brackets (no-ops unless snb_profile_enable was called).
 void snb_prof_begin(snb_context* ctx, int group);
@@ -203,8 +220,9 @@ static inline SnbBox snb_box(snb_ltmatrix3 cell, snb_ltmatrix3 rcell)
 int snb_launch_pack_atoms(snb_context* ctx, const snb_vector* crd, const float* charge);
 int snb_launch_fill_int(hipStream_t s, int* p, int n, int v);
 int snb_launch_fill_i64(hipStream_t s, int64_t* p, size_t n);
+int snb_launch_add_i64(hipStream_t s, const int64_t* a, const int64_t* b, int64_t* out, size_t n);
 int snb_launch_exclusive_scan(hipStream_t s, const int* in, int* out, int n);
-int snb_launch_flush_forces(snb_context* ctx, snb_vector* frc);
+int snb_launch_flush_forces(snb_context* ctx, snb_vector* frc);  // joins the PME stream
 int snb_launch_flush_atom_energy(snb_context* ctx, float* atom_energy, float* direct_energy,
                                  int which_mask);
 int snb_launch_flush_virial(snb_context* ctx, snb_ltmatrix3* atom_virial);

@@ -1044,7 +1044,7 @@ int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* c
     if (n > 0 && with_force)
         hipLaunchKernelGGL(k_pme_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, P.atom_u,
                            P.atom_fr, P.phi, P.fx, P.fy, P.fz, r6, (float)P.update_interval,
-                           ctx->fx);
+                           ctx->fx_pme);
     snb_prof_end(ctx, SNB_PROF_PME);
     SNB_CHECK(hipGetLastError());
     if (ev)

@@ -50,6 +50,7 @@ _SIGS = {
     "snb_context_create": ([ctypes.POINTER(_P), _I, _P, _I], _I),
     "snb_context_destroy": ([_P], _I),
     "snb_set_stream": ([_P, _P], _I),
+    "snb_set_pme_overlap": ([_P, _I], _I),
     "snb_synchronize": ([_P], _I),
     "snb_read_status": ([_P, _I, ctypes.POINTER(_I)], _I),
     "snb_set_exclusions": ([_P, _P, _P, _P, _I], _I),
@@ -61,6 +62,7 @@ _SIGS = {
     "snb_lj_set_owner_attribution": ([_P, _I], _I),
     "snb_nl_copy_internal": ([_P, _P, _P], _I),
     "snb_nl_window_info": ([_P] + [ctypes.POINTER(_I)] * 6, _I),
+    "snb_nl_set_window_capacity": ([_P, _I], _I),
     "snb_nl_grid_shape": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "snb_lj_initial": ([_P, _I, _P, _P, _P, _F], _I),
     "snb_lj_pme_direct_force": ([_P, _I, _I, _I, _I, _P, _P, _P, LTMatrix3, LTMatrix3, _P, _F,
@@ -257,6 +259,10 @@ class Nonbonded:
         self.virial.zero_()
         self.potential.zero_()
 
+    def set_pme_overlap(self, enable):
+        """PME reciprocal pipeline on the library's second stream (default on)."""
+        check(self.L.snb_set_pme_overlap(self.ctx, 1 if enable else 0), "snb_set_pme_overlap")
+
     def set_stream(self, stream):
         """Points the library context at another torch stream."""
         check(self.L.snb_set_stream(self.ctx, ctypes.c_void_p(stream.cuda_stream)),
@@ -276,6 +282,7 @@ class Nonbonded:
         graph = torch.cuda.CUDAGraph()
         side = torch.cuda.Stream(device=self.dev)
         home = self.stream
+        self.sync()  # joins the reciprocal stream: nothing outside the capture is pending
         torch.cuda.synchronize()
         self.set_stream(side)
         try:
@@ -380,6 +387,10 @@ class Nonbonded:
         the lower atom id as the reference does (False, default)."""
         check(self.L.snb_lj_set_owner_attribution(self.ctx, 1 if enable else 0),
               "snb_lj_set_owner_attribution")
+
+    def set_window_capacity(self, slots):
+        check(self.L.snb_nl_set_window_capacity(self.ctx, int(slots)),
+              "snb_nl_set_window_capacity")
 
     def window_info(self):
         v = [_I() for _ in range(6)]

@@ -489,3 +489,47 @@ def test_graph_replay_of_rebuild_and_energy_steps(water_9k):
         np.testing.assert_array_equal(nb.atom_virial.cpu().numpy(), ref_v)
         assert nb.energies() == ref_e
     nb.close()
+
+
+@pytest.mark.parametrize("slots", [64, 1500])
+def test_window_overflow_path_bit_exact(water_9k, slots):
+    """Blocks whose window exceeds the LDS reservation use global integer
+    atomics: the forces and per-atom terms are unchanged."""
+    import oracle
+    s = water_9k
+    nb = _nb(s)
+    nb.set_window_capacity(slots)
+    print("case slots", slots, nb.window_info(), flush=True)
+    nb.nl_update(forced=True)
+    counts, rows = nb.nl_rows()
+    nb.lj_direct(need_energy=True, need_virial=True, frc=False)
+    t_f, t_elj, t_ec, t_v = oracle.pair_twin(s["crd"], s["charge"], s["lj_type"], s["lj_a"],
+                                             s["lj_b"], s["cell"], s["rcell"], nb.cutoff,
+                                             nb.beta, counts, rows, virial=True)
+    np.testing.assert_array_equal(nb.fixed_forces(), t_f)
+    np.testing.assert_array_equal(nb.fixed_atom_terms(0), t_elj)
+    np.testing.assert_array_equal(nb.fixed_atom_terms(1), t_ec)
+    np.testing.assert_array_equal(nb.fixed_atom_terms(3), t_v)
+    nb.close()
+
+
+@pytest.mark.parametrize("overlap", [True, False])
+def test_pme_stream_overlap_same_results(water_9k, overlap):
+    """The reciprocal pipeline on its own stream gives the same fused forces,
+    energies and virials as on the context stream."""
+    import torch
+    res = []
+    for ov in (False, overlap):
+        nb = _nb(water_9k, refresh_interval=10)
+        nb.set_pme_overlap(ov)
+        nb.step(step=0, forced_nl=True, need_energy=True, need_virial=True)
+        nb.zero_outputs()
+        nb.step(step=1, need_energy=True, need_virial=True)
+        torch.cuda.synchronize()
+        res.append((nb.frc.cpu().numpy().copy(), nb.atom_virial.cpu().numpy().copy(),
+                    nb.energies(), nb.reciprocal_virial()))
+        nb.close()
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    np.testing.assert_array_equal(res[0][1], res[1][1])
+    assert res[0][2] == res[1][2]
+    assert list(res[0][3]) == list(res[1][3])
