@@ -51,10 +51,11 @@ def parse():
 
 
 def cpu_baseline(sysd, nb, counts, rows, budget_s):
-    """The CPU port (oracle/, 1 thread) on the same 93k workload: FP32 twin
-    pair kernel over the GPU-built list + FP32 twin exclusion correction +
-    FP64 PME reciprocal (spread, mixed-radix FFT, gather).  The list rebuild
-    is left out (it favours the CPU).  Runs whole steps until budget_s."""
+    """The CPU port (oracle/, OpenMP over the host's threads) on the same 93k
+    workload: FP32 twin pair kernel over the GPU-built list + FP32 twin
+    exclusion correction + FP64 PME reciprocal (spread, mixed-radix FFT,
+    gather).  The list rebuild is left out (it favours the CPU).  Runs whole
+    steps until budget_s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     s = sysd
@@ -73,9 +74,11 @@ def cpu_baseline(sysd, nb, counts, rows, budget_s):
         if el >= budget_s or steps >= 50:
             break
     sps = steps / el
-    return {"value": sps * NS_PER_DAY_PER_STEP_S, "unit": "ns/day", "cores": 1, "kind": "port",
+    nt = oracle.num_threads()
+    return {"value": sps * NS_PER_DAY_PER_STEP_S, "unit": "ns/day", "cores": nt, "kind": "port",
             "sample": "%d whole steps of the 93k workload (pair+exclusion FP32 fixed-point twin, "
-                      "FP64 PME), NL rebuild excluded, %.1f s, 1 thread" % (steps, el)}, pme
+                      "FP64 PME), NL rebuild excluded, %.1f s, %d OpenMP threads"
+                      % (steps, el, nt)}, pme
 
 
 def accuracy(sysd, nb, counts, rows, pme, gpu_frc):

@@ -44,6 +44,8 @@ _p = ctypes.c_void_p
 
 
 def _declare(L):
+    L.orc_num_threads.restype = _i
+    L.orc_num_threads.argtypes = []
     L.orc_get_beta.restype = _f
     L.orc_get_beta.argtypes = [_f, _f]
     L.orc_fft_size.restype = _i
@@ -88,6 +90,11 @@ def _c(a, dt):
 # --------------------------------------------------------------------------
 # scalar host precompute
 # --------------------------------------------------------------------------
+def num_threads():
+    """OpenMP threads the twin, FFT and PME loops use (OMP_NUM_THREADS)."""
+    return int(lib().orc_num_threads())
+
+
 def get_beta(cutoff, tolerance):
     return float(lib().orc_get_beta(cutoff, tolerance))
 

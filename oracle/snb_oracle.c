@@ -36,6 +36,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORC_EXPORT __attribute__((visibility("default")))
 
@@ -213,6 +216,39 @@ static inline int lj_pair_index(int a, int b)
 
 /* neighbor_list.cpp:618-637: the reference tests membership only when j lies
  * between the first and the last entry of i's exclusion list. */
+/* OpenMP helpers: thread count (ORC threads = OMP_NUM_THREADS when built with
+ * -fopenmp, else 1) and the fixed-order sum of per-thread integer arrays. */
+static inline int orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+static inline int orc_thread_id(void)
+{
+#ifdef _OPENMP
+    return omp_get_thread_num();
+#else
+    return 0;
+#endif
+}
+
+static void reduce_parts(const int64_t* part, int nt, size_t len, int64_t* out)
+{
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (size_t k = 0; k < len; k++)
+    {
+        int64_t s = 0;
+        for (int t = 0; t < nt; t++) s += part[(size_t)t * len + k];
+        out[k] += s;
+    }
+}
+
+ORC_EXPORT int orc_num_threads(void) { return orc_threads(); }
+
 static int is_excluded(int i, int j, const int* excl_start, const int* excl_num,
                        const int* excl_list)
 {
@@ -450,6 +486,15 @@ ORC_EXPORT void orc_pair_twin(int n, const float* crd, const float* charge,
                               int64_t* vir)
 {
     float rc2 = cutoff * cutoff;
+    /* rows in parallel; the i- and j-side force terms go to per-thread
+     * arrays (integer sums, so the result does not depend on the split);
+     * energies and virials belong to row i alone */
+    int nt = orc_threads();
+    int64_t* part = (int64_t*)calloc((size_t)nt * 3 * n + 1, sizeof(int64_t));
+#pragma omp parallel num_threads(nt)
+    {
+    int64_t* f = part + (size_t)orc_thread_id() * 3 * n;
+#pragma omp for schedule(dynamic, 64)
     for (int i = 0; i < n; i++)
     {
         const float* ri = crd + 3 * i;
@@ -478,12 +523,12 @@ ORC_EXPORT void orc_pair_twin(int n, const float* crd, const float* charge,
             float fabs_ = flj - fc;
             float fx = fabs_ * d[0], fy = fabs_ * d[1], fz = fabs_ * d[2];
             int64_t ix = to_fixed(fx), iy = to_fixed(fy), iz = to_fixed(fz);
-            frc[i] += ix;
-            frc[n + i] += iy;
-            frc[2 * n + i] += iz;
-            frc[j] -= ix;
-            frc[n + j] -= iy;
-            frc[2 * n + j] -= iz;
+            f[i] += ix;
+            f[n + i] += iy;
+            f[2 * n + i] += iz;
+            f[j] -= ix;
+            f[n + j] -= iy;
+            f[2 * n + j] -= iz;
             if (e_lj || e_c)
             {
                 /* energies in FP64 from the float displacement */
@@ -512,6 +557,9 @@ ORC_EXPORT void orc_pair_twin(int n, const float* crd, const float* charge,
             }
         }
     }
+    }
+    reduce_parts(part, nt, 3 * (size_t)n, frc);
+    free(part);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -567,6 +615,12 @@ ORC_EXPORT void orc_excl_twin(int n, const float* crd, const float* charge,
                               const int* excl_list, int64_t* frc, int64_t* e,
                               int64_t* vir)
 {
+    int nt = orc_threads();
+    int64_t* part = (int64_t*)calloc((size_t)nt * 3 * n + 1, sizeof(int64_t));
+#pragma omp parallel num_threads(nt)
+    {
+    int64_t* f = part + (size_t)orc_thread_id() * 3 * n;
+#pragma omp for schedule(dynamic, 256)
     for (int i = 0; i < n; i++)
     {
         float qi = charge[i];
@@ -586,12 +640,12 @@ ORC_EXPORT void orc_excl_twin(int n, const float* crd, const float* charge,
             double fa = (qq * hd) * ((rid * rid) * rid);
             double fx = fa * ddx, fy = fa * ddy, fz = fa * ddz;
             int64_t ix = to_fixed_d(fx), iy = to_fixed_d(fy), iz = to_fixed_d(fz);
-            frc[i] += ix;
-            frc[n + i] += iy;
-            frc[2 * n + i] += iz;
-            frc[j] -= ix;
-            frc[n + j] -= iy;
-            frc[2 * n + j] -= iz;
+            f[i] += ix;
+            f[n + i] += iy;
+            f[2 * n + i] += iz;
+            f[j] -= ix;
+            f[n + j] -= iy;
+            f[2 * n + j] -= iz;
             if (e) e[i] += to_fixed_d(-((qq * erfd) * rid));
             if (vir)
             {
@@ -604,6 +658,9 @@ ORC_EXPORT void orc_excl_twin(int n, const float* crd, const float* charge,
             }
         }
     }
+    }
+    reduce_parts(part, nt, 3 * (size_t)n, frc);
+    free(part);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -676,20 +733,28 @@ ORC_EXPORT void orc_fft3d_f64(double* data_ri, int n0, int n1, int n2, int sign)
     cplx* a = (cplx*)data_ri;
     int nmax = n0 > n1 ? n0 : n1;
     nmax = nmax > n2 ? nmax : n2;
-    cplx* work = (cplx*)malloc(sizeof(cplx) * 3 * nmax);
+    int nt = orc_threads();
+    cplx* works = (cplx*)malloc(sizeof(cplx) * 3 * nmax * (size_t)nt);
+    /* each line is transformed by one thread in the same operation order */
     cplx* r2 = make_roots(n2, sign);
-    for (int i = 0; i < n0; i++)
-        for (int j = 0; j < n1; j++) fft1d(a + ((size_t)i * n1 + j) * n2, n2, 1, r2, work);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int ij = 0; ij < n0 * n1; ij++)
+        fft1d(a + (size_t)ij * n2, n2, 1, r2, works + (size_t)orc_thread_id() * 3 * nmax);
     free(r2);
     cplx* r1 = make_roots(n1, sign);
-    for (int i = 0; i < n0; i++)
-        for (int k = 0; k < n2; k++) fft1d(a + (size_t)i * n1 * n2 + k, n1, n2, r1, work);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int ik = 0; ik < n0 * n2; ik++)
+    {
+        int i = ik / n2, k = ik - i * n2;
+        fft1d(a + (size_t)i * n1 * n2 + k, n1, n2, r1, works + (size_t)orc_thread_id() * 3 * nmax);
+    }
     free(r1);
     cplx* r0 = make_roots(n0, sign);
-    for (int j = 0; j < n1; j++)
-        for (int k = 0; k < n2; k++) fft1d(a + (size_t)j * n2 + k, n0, n1 * n2, r0, work);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int jk = 0; jk < n1 * n2; jk++)
+        fft1d(a + (size_t)jk, n0, n1 * n2, r0, works + (size_t)orc_thread_id() * 3 * nmax);
     free(r0);
-    free(work);
+    free(works);
 }
 
 /* Naive O(N^2) DFT of the same array, to pin orc_fft3d_f64 on small grids. */
@@ -775,8 +840,13 @@ ORC_EXPORT void orc_pme_f64(int n, const float* crd, const float* charge,
     cplx* grid = (cplx*)calloc(nall, sizeof(cplx));
     int* u = (int*)malloc(sizeof(int) * 3 * n);
     double* frd = (double*)malloc(sizeof(double) * 3 * n);
+    /* spread into per-thread grids, summed in thread order afterwards */
+    int nt = orc_threads();
+    double* tg = (double*)calloc((size_t)nt * nall, sizeof(double));
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int i = 0; i < n; i++)
     {
+        double* gq = tg + (size_t)orc_thread_id() * nall;
         float frf[3];
         atom_near_f(crd + 3 * i, rcell, fx, fy, fz, u + 3 * i, frf);
         for (int c = 0; c < 3; c++) frd[3 * i + c] = frf[c];
@@ -794,9 +864,17 @@ ORC_EXPORT void orc_pme_f64(int n, const float* crd, const float* charge,
                     size_t idx = ((size_t)wrapi(u[3 * i] - kx, fx) * fy +
                                   wrapi(u[3 * i + 1] - ky, fy)) * fz +
                                  wrapi(u[3 * i + 2] - kz, fz);
-                    grid[idx].re += charge[i] * w[0][kx] * w[1][ky] * w[2][kz];
+                    gq[idx] += charge[i] * w[0][kx] * w[1][ky] * w[2][kz];
                 }
     }
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (size_t p = 0; p < nall; p++)
+    {
+        double acc = 0.0;
+        for (int t = 0; t < nt; t++) acc += tg[(size_t)t * nall + p];
+        grid[p].re = acc;
+    }
+    free(tg);
     double* q = (double*)malloc(sizeof(double) * nall);
     for (size_t p = 0; p < nall; p++) q[p] = grid[p].re;
     orc_fft3d_f64((double*)grid, fx, fy, fz, -1);
@@ -805,7 +883,9 @@ ORC_EXPORT void orc_pme_f64(int n, const float* crd, const float* charge,
     double* bc = (double*)malloc(sizeof(double) * (size_t)fx * fy * fc);
     orc_bc_f64(fx, fy, fz, rcell, beta, volume, bc);
     double mpre = ORC_PI_D * ORC_PI_D / ((double)beta * beta);
-    double v[6] = {0, 0, 0, 0, 0, 0};
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+#pragma omp parallel for num_threads(nt) schedule(static) \
+    reduction(+ : v0, v1, v2, v3, v4, v5)
     for (int kx = 0; kx < fx; kx++)
         for (int ky = 0; ky < fy; ky++)
             for (int kz = 0; kz < fz; kz++)
@@ -832,16 +912,17 @@ ORC_EXPORT void orc_pme_f64(int n, const float* crd, const float* charge,
                     double msq = mx * mx + my * my + mz * mz;
                     double cf = 2.0 / msq * (1.0 + mpre * msq);
                     double hb = 0.5 * b * w * a2;
-                    v[0] -= hb * (1.0 - cf * mx * mx);
-                    v[1] -= hb * (0.0 - cf * my * mx);
-                    v[2] -= hb * (1.0 - cf * my * my);
-                    v[3] -= hb * (0.0 - cf * mz * mx);
-                    v[4] -= hb * (0.0 - cf * mz * my);
-                    v[5] -= hb * (1.0 - cf * mz * mz);
+                    v0 -= hb * (1.0 - cf * mx * mx);
+                    v1 -= hb * (0.0 - cf * my * mx);
+                    v2 -= hb * (1.0 - cf * my * my);
+                    v3 -= hb * (0.0 - cf * mz * mx);
+                    v4 -= hb * (0.0 - cf * mz * my);
+                    v5 -= hb * (1.0 - cf * mz * mz);
                 }
                 grid[idx].re *= b;
                 grid[idx].im *= b;
             }
+    double v[6] = {v0, v1, v2, v3, v4, v5};
     orc_fft3d_f64((double*)grid, fx, fy, fz, +1);
     double e = 0;
     for (size_t p = 0; p < nall; p++) e += q[p] * grid[p].re;
@@ -852,6 +933,7 @@ ORC_EXPORT void orc_pme_f64(int n, const float* crd, const float* charge,
     if (phi_out)
         for (size_t p = 0; p < nall; p++) phi_out[p] = grid[p].re;
     /* gather (PM_force.cpp:826-890) */
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int i = 0; i < n; i++)
     {
         double w[3][4], dw[3][4];

@@ -115,6 +115,7 @@ void free_nl(SnbNl& L)
     dfree(L.sorted_cell);
     dfree(L.nl16);
     dfree(L.row_count);
+    dfree(L.row_split);
     dfree(L.win_size);
     dfree(L.stage_base);
     dfree(L.stage);
@@ -518,6 +519,7 @@ int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
                    window_cover(W.wz, W.s, W.nz);
         if (!rc) rc = dalloc(&L.nl16, (size_t)n * L.max_nb);
         if (!rc) rc = dalloc(&L.row_count, n);
+        if (!rc) rc = dalloc(&L.row_split, n);
         if (!rc) rc = dalloc(&L.win_size, W.nblock + 1);
         if (!rc) rc = dalloc(&L.stage_base, W.nblock + 1);
         // every atom sits in at most invmax windows
@@ -534,6 +536,9 @@ int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
             return rc;
         }
         snb_launch_fill_int(c->stream, L.row_count, n, 0);
+        snb_launch_fill_int(c->stream, L.row_split, n, 0);
+        float near = cutoff + 0.25f * skin;
+        L.near2 = near * near;
     }
     // ATOM_GROUP rows point into the padded table (neighbor_list.cpp:764-771)
     std::vector<snb_atom_group> h(n > 0 ? n : 1);
@@ -863,10 +868,25 @@ int snb_pme_excluded_force(snb_context* c, int atom_numbers, const int* d_atom_l
     DeviceGuard g(c->device);
     SnbBox box = snb_box(cell, rcell);
     int rc;
+    if (c->overlap && !need_energy && !need_virial && !d_frc)
+    {
+        // force-only: the correction goes onto the reciprocal stream, into the
+        // reciprocal accumulator, ahead of the PME pipeline of the step
+        hipStream_t home = c->stream;
+        if (!c->inputs_marked) SNB_CHECK(hipEventRecord(c->ev_inputs, home));
+        SNB_CHECK(hipStreamWaitEvent(c->pme_stream, c->ev_inputs, 0));
+        c->stream = c->pme_stream;
+        rc = snb_launch_excl(c, d_crd, d_charge, box, c->pme.beta, es, el, en, 0, 0, c->fx_pme);
+        c->stream = home;
+        if (rc) return rc;
+        SNB_CHECK(hipEventRecord(c->ev_pme, c->pme_stream));
+        c->pme_pending = true;
+        return SNB_OK;
+    }
     if (need_energy && (rc = snb_launch_fill_i64(c->stream, c->e_atom + 2 * (size_t)c->n, c->n)))
         return rc;
     rc = snb_launch_excl(c, d_crd, d_charge, box, c->pme.beta, es, el, en, need_energy,
-                         need_virial);
+                         need_virial, c->fx);
     if (rc) return rc;
     if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
     if (need_energy)
@@ -908,7 +928,9 @@ int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatr
         hipStream_t home = c->stream;
         if (c->overlap)
         {
-            if ((rc = snb_join_pme(c))) return rc;  // the previous call's results are consumed
+            // ev_inputs is recorded after every earlier use of the reciprocal
+            // accumulator on the context stream (a flush comes before the next
+            // step's NL update), so the pipeline may start once it fires
             if (!c->inputs_marked) SNB_CHECK(hipEventRecord(c->ev_inputs, home));
             SNB_CHECK(hipStreamWaitEvent(c->pme_stream, c->ev_inputs, 0));
             c->inputs_marked = false;

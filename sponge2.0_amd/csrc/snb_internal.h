@@ -64,7 +64,9 @@ struct SnbNl
     bool export_public = true;   // also write the reference-layout list at rebuild
     SnbWin win;
     uint16_t* nl16 = nullptr;    // n * max_nb window slots, rows by sorted position
-    int* row_count = nullptr;    // n
+    int* row_count = nullptr;    // n: entries of each row
+    int* row_split = nullptr;    // n: near pairs of each row (front part, snb_row_entry)
+    float near2 = 0.0f;          // (cutoff + skin/4)^2: near/far split of the rows
     int* win_size = nullptr;     // nblock + 1
     int* stage_base = nullptr;   // nblock + 1 (exclusive scan of win_size)
     long long* stage = nullptr;  // nblock * wcap * 3
@@ -249,7 +251,7 @@ int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBo
 int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge,
                     const SnbBox& box, float beta,
                     const int* ex_start, const int* ex_list, const int* ex_num,
-                    int need_energy, int need_virial);
+                    int need_energy, int need_virial, int64_t* fx);
 
 // PME (snb_pme.hip)
 int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box);

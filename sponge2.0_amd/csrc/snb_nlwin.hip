@@ -104,8 +104,9 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
     const int* __restrict__ need, SnbWin W, const int* __restrict__ start,
     const float4* __restrict__ sorted_crd, SnbBox box, float cs2,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
-    const int* __restrict__ ex_num, const int2* __restrict__ ex_rng, int max_nb,
-    uint16_t* __restrict__ nl16, int* __restrict__ row_count, int* __restrict__ status)
+    const int* __restrict__ ex_num, const int2* __restrict__ ex_rng, int max_nb, float near2,
+    uint16_t* __restrict__ nl16, int* __restrict__ row_count, int* __restrict__ row_split,
+    int* __restrict__ status)
 {
     if (need[0] == 0) return;
     __shared__ int first[SNB_WIN_MAX_RANGES];
@@ -117,7 +118,8 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
     __shared__ int run_p0[SNB_NLC_MAXRUN];
     __shared__ int run_slot0[SNB_NLC_MAXRUN];
     __shared__ int nrun_s;
-    __shared__ int rowcnt[SNB_NLC_MAXA];
+    __shared__ int rowcnt[SNB_NLC_MAXA];   // near pairs (front of the row)
+    __shared__ int rowfar[SNB_NLC_MAXA];   // skin pairs (back of the row)
     const int c = blockIdx.x;
     const int cb = start[c], m = start[c + 1] - cb;
     if (m == 0) return;
@@ -125,12 +127,20 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
     const int cz = c % W.nz, cy = (c / W.nz) % W.ny, cx = c / (W.nz * W.ny);
     const int x0 = (cx / W.a) * W.a, y0 = (cy / W.b) * W.b, z0 = (cz / W.s) * W.s;
     if (wave == 0) win_table_wave(W, start, x0, y0, z0, lane, first, pref);
-    for (int a = tid; a < SNB_NLC_MAXA; a += blockDim.x) rowcnt[a] = 0;
+    for (int a = tid; a < SNB_NLC_MAXA; a += blockDim.x)
+    {
+        rowcnt[a] = 0;
+        rowfar[a] = 0;
+    }
     __syncthreads();
     if (pref[SNB_WIN_MAX_RANGES] > 65535)
     {
         // the window does not fit 16-bit slots: no rows, flagged
-        for (int a = tid; a < m; a += blockDim.x) row_count[cb + a] = 0;
+        for (int a = tid; a < m; a += blockDim.x)
+        {
+            row_count[cb + a] = 0;
+            row_split[cb + a] = 0;
+        }
         if (tid == 0) atomicOr(status, SNB_FLAG_NL_OVERFLOW);
         return;
     }
@@ -209,12 +219,12 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
             const int id_i = __float_as_int(qi.w);
             const int2 rng_i = ex_num ? ex_rng[id_i] : make_int2(1, 0);
             uint16_t* row = nl16 + (size_t)s * max_nb;
-            int count = rowcnt[a];
+            int count = rowcnt[a], cfar = rowfar[a];
             for (int t0 = 0; t0 < clen; t0 += 64)
             {
                 const int t = t0 + lane;
                 const int g = chunk0 + t;
-                bool ok = false;
+                bool ok = false, nearp = false;
                 // the cell's own atoms come first (g < m): larger sorted index only
                 if (t < clen && (g >= m || g > a))
                 {
@@ -241,7 +251,9 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
                         ddy = ddy - (ky * box.c[2] + kz * box.c[4]);
                         ddz = ddz - kz * box.c[5];
                     }
-                    ok = (ddx * ddx + ddy * ddy) + ddz * ddz < cs2;
+                    const float d2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                    ok = d2 < cs2;
+                    nearp = d2 < near2;
                     // the lower id's list decides; its id window rules out
                     // most pairs without touching the list
                     const int2 rl = id_i < id_j ? rng_i : cex[t];
@@ -249,15 +261,33 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
                     if (ok && hi >= rl.x && hi <= rl.y)
                         ok = !excluded_pair(id_i, id_j, ex_start, ex_list, ex_num);
                 }
-                const uint64_t mask = __ballot(ok);
+                // near pairs (within rc + skin/4, the ones that can come into
+                // range before the next rebuild) from the front, the rest from
+                // the back: the pair kernel meets them in that order and whole
+                // wavefronts of far pairs skip the force evaluation
+                const uint64_t mnear = __ballot(ok && nearp);
+                const uint64_t mfar = __ballot(ok && !nearp);
                 if (ok)
                 {
-                    int pos = count + __popcll(mask & lower);
-                    if (pos < max_nb) row[pos] = cslot[t];
+                    if (nearp)
+                    {
+                        int pos = count + __popcll(mnear & lower);
+                        if (pos < max_nb) row[pos] = cslot[t];
+                    }
+                    else
+                    {
+                        int pos = cfar + __popcll(mfar & lower);
+                        if (pos < max_nb) row[max_nb - 1 - pos] = cslot[t];
+                    }
                 }
-                count += __popcll(mask);
+                count += __popcll(mnear);
+                cfar += __popcll(mfar);
             }
-            if (lane == 0) rowcnt[a] = count;
+            if (lane == 0)
+            {
+                rowcnt[a] = count;
+                rowfar[a] = cfar;
+            }
         }
         __syncthreads();
     }
@@ -265,13 +295,16 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
     {
         if (a < SNB_NLC_MAXA)
         {
-            int cnt = rowcnt[a];
-            row_count[cb + a] = cnt < max_nb ? cnt : max_nb;
-            if (cnt > max_nb) atomicOr(status, SNB_FLAG_NL_OVERFLOW);
+            // an overflowing row (flagged) keeps max_nb valid slots
+            int cn = rowcnt[a], cf = rowfar[a];
+            row_count[cb + a] = cn + cf < max_nb ? cn + cf : max_nb;
+            row_split[cb + a] = cn < max_nb ? cn : max_nb;
+            if (cn + cf > max_nb) atomicOr(status, SNB_FLAG_NL_OVERFLOW);
         }
         else
         {
             row_count[cb + a] = 0;
+            row_split[cb + a] = 0;
             atomicOr(status, SNB_FLAG_CELL_OVERFLOW);
         }
     }
@@ -351,6 +384,7 @@ __global__ void __launch_bounds__(64) k_nl_export(const int* __restrict__ need, 
                                                   const int* __restrict__ sorted_cell,
                                                   const int* __restrict__ order,
                                                   const int* __restrict__ row_count,
+                                                  const int* __restrict__ row_split,
                                                   const uint16_t* __restrict__ nl16, int max_nb,
                                                   int* __restrict__ pub_count,
                                                   int* __restrict__ serial)
@@ -364,10 +398,11 @@ __global__ void __launch_bounds__(64) k_nl_export(const int* __restrict__ need, 
     win_table_wave(W, start, x0, y0, z0, lane, first, pref);
     __syncthreads();
     int id_i = order[s];
-    int cnt = row_count[s];
+    int cnt = row_count[s], split = row_split[s];
     for (int k = lane; k < cnt; k += 64)
     {
-        int sj = win_slot_to_sorted(W, first, pref, nl16[(size_t)s * max_nb + k]);
+        int sj = win_slot_to_sorted(W, first, pref,
+                                    nl16[(size_t)s * max_nb + snb_row_entry(k, split, max_nb)]);
         int id_j = order[sj];
         int lo = id_i < id_j ? id_i : id_j, hi = id_i < id_j ? id_j : id_i;
         int pos = atomicAdd(&pub_count[lo], 1);
@@ -432,6 +467,7 @@ __global__ void __launch_bounds__(64) k_nl_copy_internal(int n, SnbWin W,
                                                          const int* __restrict__ sorted_cell,
                                                          const int* __restrict__ order,
                                                          const int* __restrict__ row_count,
+                                                         const int* __restrict__ row_split,
                                                          const uint16_t* __restrict__ nl16,
                                                          int max_nb, int* __restrict__ counts,
                                                          int* __restrict__ rows)
@@ -444,11 +480,11 @@ __global__ void __launch_bounds__(64) k_nl_copy_internal(int n, SnbWin W,
     win_table_wave(W, start, x0, y0, z0, lane, first, pref);
     __syncthreads();
     int id_i = order[s];
-    int cnt = row_count[s];
+    int cnt = row_count[s], split = row_split[s];
     if (lane == 0) counts[id_i] = cnt;
     for (int k = lane; k < cnt; k += 64)
-        rows[(size_t)id_i * max_nb + k] =
-            order[win_slot_to_sorted(W, first, pref, nl16[(size_t)s * max_nb + k])];
+        rows[(size_t)id_i * max_nb + k] = order[win_slot_to_sorted(
+            W, first, pref, nl16[(size_t)s * max_nb + snb_row_entry(k, split, max_nb)])];
 }
 
 // Per atom id: first and last entry of its exclusion list (an empty window
@@ -574,11 +610,13 @@ int snb_launch_nl_window(snb_context* ctx, const SnbBox& box, const int* ex_star
     if (ortho)
         hipLaunchKernelGGL(k_nl_build_cell<true>, dim3(L.ncell), dim3(512), 0, s, L.need, W,
                            L.cell_start, L.sorted_crd, box, L.cs2, ex_start, ex_list, ex_num,
-                           L.ex_rng, L.max_nb, L.nl16, L.row_count, ctx->status);
+                           L.ex_rng, L.max_nb, L.near2, L.nl16, L.row_count, L.row_split,
+                           ctx->status);
     else
         hipLaunchKernelGGL(k_nl_build_cell<false>, dim3(L.ncell), dim3(512), 0, s, L.need, W,
                            L.cell_start, L.sorted_crd, box, L.cs2, ex_start, ex_list, ex_num,
-                           L.ex_rng, L.max_nb, L.nl16, L.row_count, ctx->status);
+                           L.ex_rng, L.max_nb, L.near2, L.nl16, L.row_count, L.row_split,
+                           ctx->status);
     hipLaunchKernelGGL(k_win_sizes, dim3(W.nblock), dim3(64), 0, s, L.need, W, L.cell_start,
                        L.win_size);
     hipLaunchKernelGGL(k_block_work, dim3(W.nblock), dim3(64), 0, s, L.need, W, L.cell_start,
@@ -599,7 +637,8 @@ int snb_launch_nl_window(snb_context* ctx, const SnbBox& box, const int* ex_star
         hipLaunchKernelGGL(k_fill_need, dim3((n + 255) / 256), dim3(256), 0, s, L.need,
                            L.pub_count, n);
         hipLaunchKernelGGL(k_nl_export, dim3(n), dim3(64), 0, s, L.need, n, W, L.cell_start,
-                           L.sorted_cell, L.order, L.row_count, L.nl16, L.max_nb, L.pub_count,
+                           L.sorted_cell, L.order, L.row_count, L.row_split, L.nl16, L.max_nb,
+                           L.pub_count,
                            L.serial);
         hipLaunchKernelGGL(k_nl_pub_finish, dim3(n), dim3(64), 0, s, L.need, n, L.pub_count,
                            L.max_nb, L.serial, L.dnl, ctx->status);
@@ -613,7 +652,8 @@ int snb_launch_nl_copy_internal(snb_context* ctx, int* counts, int* rows)
     SnbNl& L = ctx->nl;
     if (ctx->n <= 0) return SNB_OK;
     hipLaunchKernelGGL(k_nl_copy_internal, dim3(ctx->n), dim3(64), 0, ctx->stream, ctx->n, L.win,
-                       L.cell_start, L.sorted_cell, L.order, L.row_count, L.nl16, L.max_nb, counts,
+                       L.cell_start, L.sorted_cell, L.order, L.row_count, L.row_split, L.nl16,
+                       L.max_nb, counts,
                        rows);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;

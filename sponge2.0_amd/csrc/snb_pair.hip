@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(256) k_excl(int n, const snb_vector* __restric
 int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge,
                     const SnbBox& box, float beta,
                     const int* ex_start, const int* ex_list, const int* ex_num,
-                    int need_energy, int need_virial)
+                    int need_energy, int need_virial, int64_t* fx)
 {
     int n = ctx->n;
     if (n <= 0) return SNB_OK;
@@ -255,7 +255,7 @@ int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge
     snb_prof_begin(ctx, SNB_PROF_EXCL);
 #define SNB_EX_LAUNCH(E, V)                                                                  \
     hipLaunchKernelGGL((k_excl<E, V>), g, dim3(256), 0, ctx->stream, n, crd, charge, box, beta, \
-                       ex_start, ex_list, ex_num, ctx->fx, ctx->e_atom, ctx->vir_atom)
+                       ex_start, ex_list, ex_num, fx, ctx->e_atom, ctx->vir_atom)
     if (need_energy && need_virial)
         SNB_EX_LAUNCH(true, true);
     else if (need_energy)
@@ -450,7 +450,8 @@ __device__ __forceinline__ int win_slot_sorted(int slot, bool ov, const int* slo
 template <bool NEED_ENERGY, bool NEED_VIRIAL, bool ORTHO>
 __global__ void __launch_bounds__(1024) k_pair_win(
     int n, SnbWin W, const int* __restrict__ start, const int* __restrict__ sorted_cell,
-    const int* __restrict__ row_count, const uint16_t* __restrict__ nl16, int max_nb,
+    const int* __restrict__ row_count, const int* __restrict__ row_split,
+    const uint16_t* __restrict__ nl16, int max_nb,
     const float4* __restrict__ satom, const int* __restrict__ stype,
     const int* __restrict__ order, const float* __restrict__ lj_a,
     const float* __restrict__ lj_b, SnbBox box, float rc2, float beta,
@@ -525,7 +526,7 @@ __global__ void __launch_bounds__(1024) k_pair_win(
         int r0 = start[colbase + z0], r1 = start[colbase + z1];
         for (int si = r0 + wave; si < r1; si += nwave)
         {
-            const int cnt = row_count[si];
+            const int cnt = row_count[si], split = row_split[si];
             const uint16_t* e16 = nl16 + (size_t)si * max_nb;
             const float4 ai = satom[si];
             const int ti = stype[si];
@@ -535,8 +536,8 @@ __global__ void __launch_bounds__(1024) k_pair_win(
             {
                 // two entries per lane: issue both gathers before either pair
                 const int ka = k0 + lane, kb = ka + 64;
-                const int sla = ka < cnt ? (int)e16[ka] : -1;
-                const int slb = kb < cnt ? (int)e16[kb] : -1;
+                const int sla = ka < cnt ? (int)e16[snb_row_entry(ka, split, max_nb)] : -1;
+                const int slb = kb < cnt ? (int)e16[snb_row_entry(kb, split, max_nb)] : -1;
                 const int sja = sla >= 0 ? win_slot_sorted(sla, ov, slot_s, rfirst, rpref, W.nrange) : si;
                 const int sjb = slb >= 0 ? win_slot_sorted(slb, ov, slot_s, rfirst, rpref, W.nrange) : si;
                 const float4 aja = satom[sja], ajb = satom[sjb];
@@ -688,7 +689,8 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
     snb_prof_begin(ctx, SNB_PROF_PAIR);
 #define SNB_WIN_LAUNCH(E, V, O)                                                                 \
     hipLaunchKernelGGL((k_pair_win<E, V, O>), dim3(W.nblock), dim3(1024), lds, s, n, W,         \
-                       L.cell_start, L.sorted_cell, L.row_count, L.nl16, L.max_nb, ctx->satom, \
+                       L.cell_start, L.sorted_cell, L.row_count, L.row_split, L.nl16, L.max_nb, \
+                       ctx->satom,                                                             \
                        ctx->stype, L.order, ctx->lj.a, ctx->lj.b, box, rc2, beta,              \
                        L.stage_base, L.stage, (size_t)n * L.invmax, ctx->fx, ctx->e_sorted,   \
                        ctx->v_sorted, ctx->lj.owner_attribution ? 0 : 1, ctx->e_atom,          \

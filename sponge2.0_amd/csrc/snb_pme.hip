@@ -203,8 +203,9 @@ __device__ void load_twiddles(float2* dst, const float2* src, int n)
 
 static int lines_per_block(int n)
 {
-    // two ping-pong buffers of lines*n float2 within ~48 KB of LDS
-    int l = 49152 / (16 * n);
+    // two ping-pong buffers of lines*n float2 within ~32 KB of LDS, small
+    // enough to share a compute unit with the direct-space kernel
+    int l = 32768 / (16 * n);
     if (l > 64) l = 64;
     if (l < 1) l = 1;
     return l;
@@ -578,25 +579,54 @@ struct SnbRc
     float v[6];
 };
 
-__global__ void k_pme_prep(int n, const snb_vector* __restrict__ crd,
-                           const float* __restrict__ charge, SnbRc rc, int fx, int fy,
-                           int fz, int bx, int by, int bz, int nby, int nbz,
-                           int4* __restrict__ atom_u, float4* __restrict__ atom_fr,
-                           int* __restrict__ slot, int* __restrict__ cnt)
+// Per atom: base grid point, spline fractions and brick (PME_Atom_Near), and
+// its slot in the brick's list.  With LDS_HIST the workgroup first counts its
+// atoms per brick in LDS and takes one global counter add per brick it
+// touches (a single round trip for the whole workgroup); otherwise one add
+// per distinct brick of each wavefront.
+#define SNB_PREP_HIST_MAX 8192
+template <bool LDS_HIST>
+__global__ void __launch_bounds__(256) k_pme_prep(int n, const snb_vector* __restrict__ crd,
+                                                  const float* __restrict__ charge, SnbRc rc,
+                                                  int fx, int fy, int fz, int bx, int by, int bz,
+                                                  int nby, int nbz, int nbrick,
+                                                  int4* __restrict__ atom_u,
+                                                  float4* __restrict__ atom_fr,
+                                                  int* __restrict__ slot, int* __restrict__ cnt)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int hist[LDS_HIST ? SNB_PREP_HIST_MAX : 1];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (LDS_HIST)
+    {
+        for (int b = threadIdx.x; b < nbrick; b += blockDim.x) hist[b] = 0;
+        __syncthreads();
+    }
+    int br = -1;
+    if (i < n)
+    {
+        const snb_vector ri = crd[i];
+        float4 r = make_float4(ri.x, ri.y, ri.z, charge[i]);
+        int ux, uy, uz;
+        float a, b, c;
+        atom_near(r, rc.v, fx, fy, fz, ux, uy, uz, a, b, c);
+        br = ((ux / bx) * nby + uy / by) * nbz + uz / bz;
+        atom_u[i] = make_int4(ux, uy, uz, br);
+        atom_fr[i] = make_float4(a, b, c, r.w);
+    }
+    if (LDS_HIST)
+    {
+        int local = br >= 0 ? atomicAdd(&hist[br], 1) : 0;
+        __syncthreads();
+        for (int b = threadIdx.x; b < nbrick; b += blockDim.x)
+        {
+            int k = hist[b];
+            if (k > 0) hist[b] = atomicAdd(&cnt[b], k);
+        }
+        __syncthreads();
+        if (br >= 0) slot[i] = hist[br] + local;
+        return;
+    }
     if (i >= n) return;
-    const snb_vector ri = crd[i];
-    float4 r = make_float4(ri.x, ri.y, ri.z, charge[i]);
-    int ux, uy, uz;
-    float a, b, c;
-    atom_near(r, rc.v, fx, fy, fz, ux, uy, uz, a, b, c);
-    int br = ((ux / bx) * nby + uy / by) * nbz + uz / bz;
-    atom_u[i] = make_int4(ux, uy, uz, br);
-    atom_fr[i] = make_float4(a, b, c, r.w);
-    // one counter atomic per distinct brick of the wavefront (neighbouring
-    // ids are mostly in the same brick, so per-atom atomics would queue on
-    // one address)
     const int lane = threadIdx.x & 63;
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     unsigned long long todo = __ballot(1);
@@ -627,6 +657,18 @@ __device__ __forceinline__ int wrap_axis(int v, int n) { return v < 0 ? v + n : 
 // touches points u-3..u on each axis, so a brick receives from atoms whose
 // base lies in the brick itself or in the next brick on each axis (bricks are
 // >= 4 points wide): 8 source bricks.
+// Bits k (0..3) for which stencil point u - k of an atom at offset d = (u - o)
+// mod n from a brick origin o falls in the brick [o, o + b): max(0, d - b + 1)
+// <= k <= min(3, d); all four when the axis is one brick (it wraps inside).
+__device__ __forceinline__ int stencil_mask(int d, int b, bool single)
+{
+    if (single) return 15;
+    int k0 = d - b + 1;
+    k0 = k0 < 0 ? 0 : (k0 > 4 ? 4 : k0);
+    int k1 = d < 3 ? d : 3;
+    return ((2 << k1) - 1) & ~((1 << k0) - 1);
+}
+
 __global__ void __launch_bounds__(256) k_pme_spread(
     const int4* __restrict__ atom_u, const float4* __restrict__ atom_fr,
     const int* __restrict__ start, const int* __restrict__ brick_atoms, int fx, int fy, int fz,
@@ -639,6 +681,10 @@ __global__ void __launch_bounds__(256) k_pme_spread(
     const int B = blockIdx.x;
     const int Bz = B % nbz, By = (B / nbz) % nby, Bx = B / (nbz * nby);
     const int ox = Bx * bx, oy = By * by, oz = Bz * bz;
+    // Atoms of this brick and of its +1 neighbours (their stencils reach down
+    // by 3 points).  For an atom at base point u, stencil point u - k lies in
+    // the brick when 0 <= d - k < b with d = (u - o) mod n: only that k range
+    // is visited, so every stencil point is handled by exactly one brick.
     for (int s = 0; s < 8; s++)
     {
         int sx = s >> 2, sy = (s >> 1) & 1, sz = s & 1;
@@ -649,6 +695,16 @@ __global__ void __launch_bounds__(256) k_pme_spread(
         {
             int i = brick_atoms[a];
             int4 u = atom_u[i];
+            int dx = u.x - ox, dy = u.y - oy, dz = u.z - oz;
+            dx += dx < 0 ? fx : 0;
+            dy += dy < 0 ? fy : 0;
+            dz += dz < 0 ? fz : 0;
+            // valid stencil offsets per axis as bit masks (an axis with a
+            // single brick wraps inside it: all four)
+            const int mx = stencil_mask(dx, bx, nbx == 1);
+            const int my = stencil_mask(dy, by, nby == 1);
+            const int mz = stencil_mask(dz, bz, nbz == 1);
+            if (mx == 0 || my == 0 || mz == 0) continue;
             float4 fr = atom_fr[i];
             float wx[4], wy[4], wz[4];
 #pragma unroll
@@ -661,21 +717,25 @@ __global__ void __launch_bounds__(256) k_pme_spread(
 #pragma unroll
             for (int kx = 0; kx < 4; kx++)
             {
-                int lx = wrap_axis(u.x - kx, fx) - ox;
-                if ((unsigned)lx >= (unsigned)bx) continue;
+                if (!((mx >> kx) & 1)) continue;
+                int lx = dx - kx;
+                lx += lx < 0 ? fx : 0;
                 float qx = fr.w * wx[kx];
 #pragma unroll
                 for (int ky = 0; ky < 4; ky++)
                 {
-                    int ly = wrap_axis(u.y - ky, fy) - oy;
-                    if ((unsigned)ly >= (unsigned)by) continue;
+                    if (!((my >> ky) & 1)) continue;
+                    int ly = dy - ky;
+                    ly += ly < 0 ? fy : 0;
                     float qxy = qx * wy[ky];
+                    int row = (lx * by + ly) * bz;
 #pragma unroll
                     for (int kz = 0; kz < 4; kz++)
                     {
-                        int lz = wrap_axis(u.z - kz, fz) - oz;
-                        if ((unsigned)lz >= (unsigned)bz) continue;
-                        atomicAdd((unsigned long long*)&acc[(lx * by + ly) * bz + lz],
+                        if (!((mz >> kz) & 1)) continue;
+                        int lz = dz - kz;
+                        lz += lz < 0 ? fz : 0;
+                        atomicAdd((unsigned long long*)&acc[row + lz],
                                   (unsigned long long)snb_to_fixed(qxy * wz[kz]));
                     }
                 }
@@ -1006,9 +1066,14 @@ int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* c
     if (n > 0)
     {
         if ((rc = snb_launch_fill_int(s, P.brick_count, P.nbrick + 1, 0))) return rc;
-        hipLaunchKernelGGL(k_pme_prep, dim3((n + 255) / 256), dim3(256), 0, s, n, crd, charge, r6,
-                           P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nby, P.nbz, P.atom_u,
-                           P.atom_fr, P.atom_slot, P.brick_count);
+        if (P.nbrick <= SNB_PREP_HIST_MAX)
+            hipLaunchKernelGGL(k_pme_prep<true>, dim3((n + 255) / 256), dim3(256), 0, s, n, crd,
+                               charge, r6, P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nby, P.nbz,
+                               P.nbrick, P.atom_u, P.atom_fr, P.atom_slot, P.brick_count);
+        else
+            hipLaunchKernelGGL(k_pme_prep<false>, dim3((n + 255) / 256), dim3(256), 0, s, n, crd,
+                               charge, r6, P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nby, P.nbz,
+                               P.nbrick, P.atom_u, P.atom_fr, P.atom_slot, P.brick_count);
         if ((rc = snb_launch_exclusive_scan(s, P.brick_count, P.brick_start, P.nbrick))) return rc;
         hipLaunchKernelGGL(k_pme_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, P.atom_u,
                            P.atom_slot, P.brick_start, P.brick_atoms);

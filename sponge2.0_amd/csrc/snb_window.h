@@ -53,6 +53,14 @@ SNB_WIN_HD int snb_win_block_of(const SnbWin& W, int x, int y, int z)
 
 SNB_WIN_HD int snb_wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
+// A row of the internal list keeps its near pairs from the front of its
+// max_nb slots and its far (skin) pairs from the back: entry k of a row with
+// `split` near pairs.
+SNB_WIN_HD int snb_row_entry(int k, int split, int max_nb)
+{
+    return k < split ? k : max_nb - 1 - (k - split);
+}
+
 // Range k = (u * wv + v) * 2 + part of the window of block (x0, y0, z0):
 // first cell index and number of cells.  Part 0 starts at the window's first
 // z-cell; part 1 is the continuation after the periodic wrap (may be empty).

@@ -524,12 +524,18 @@ def test_pme_stream_overlap_same_results(water_9k, overlap):
         nb.set_pme_overlap(ov)
         nb.step(step=0, forced_nl=True, need_energy=True, need_virial=True)
         nb.zero_outputs()
-        nb.step(step=1, need_energy=True, need_virial=True)
+        nb.step(step=1)                        # force-only: correction on the PME stream
+        torch.cuda.synchronize()
+        f_plain = nb.frc.cpu().numpy().copy()
+        nb.zero_outputs()
+        nb.step(step=2, need_energy=True, need_virial=True)
         torch.cuda.synchronize()
         res.append((nb.frc.cpu().numpy().copy(), nb.atom_virial.cpu().numpy().copy(),
-                    nb.energies(), nb.reciprocal_virial()))
+                    nb.energies(), nb.reciprocal_virial(), f_plain))
         nb.close()
     np.testing.assert_array_equal(res[0][0], res[1][0])
     np.testing.assert_array_equal(res[0][1], res[1][1])
     assert res[0][2] == res[1][2]
     assert list(res[0][3]) == list(res[1][3])
+    np.testing.assert_array_equal(res[0][4], res[1][4])
+    np.testing.assert_array_equal(res[0][4], res[0][0])   # fixed coordinates: same forces
