@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU-baseline sample budget (0 disables)")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run the reciprocal pipeline on the context stream (diagnostics)")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager launches instead of HIP-graph replays of each kind of step")
     ap.add_argument("--profile-steps", type=int, default=200,
@@ -127,6 +129,8 @@ def main():
     # the pair kernel reads the library's internal cell-ordered list; nothing
     # here reads the reference-layout copy
     nb.set_export(False)
+    if args.no_overlap:
+        nb.set_pme_overlap(False)
     torch.cuda.synchronize()
 
     profiling = [False]

@@ -510,7 +510,9 @@ int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
         free_nl(L);
         return rc;
     }
-    L.windowed = choose_window(L.nx, L.ny, L.nz, (double)n / L.ncell, c->max_lds, 2 * c->num_cu,
+    // the windowed kernels address the list with 32-bit byte offsets
+    L.windowed = (double)n * L.max_nb * sizeof(uint16_t) < 4294967296.0 &&
+                 choose_window(L.nx, L.ny, L.nz, (double)n / L.ncell, c->max_lds, 2 * c->num_cu,
                                &L.win);
     if (L.windowed)
     {

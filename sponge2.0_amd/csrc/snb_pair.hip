@@ -23,6 +23,15 @@ __device__ __forceinline__ void pbc_disp(float& dx, float& dy, float& dz, const 
     dz = dz - kz * box.c[5];
 }
 
+// Element i of an array addressed with a 32-bit byte offset: the load can use
+// the scalar-base + unsigned vector-offset form (the offset is computed in 32
+// bits, which holds for every array of this library).
+template <class T>
+__device__ __forceinline__ T ld32(const T* __restrict__ base, unsigned i)
+{
+    return *(const T*)((const char*)base + i * (unsigned)sizeof(T));
+}
+
 __device__ __forceinline__ int64_t wave_sum(int64_t v)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -527,29 +536,52 @@ __global__ void __launch_bounds__(1024) k_pair_win(
         for (int si = r0 + wave; si < r1; si += nwave)
         {
             const int cnt = row_count[si], split = row_split[si];
-            const uint16_t* e16 = nl16 + (size_t)si * max_nb;
+            const unsigned row0 = (unsigned)si * (unsigned)max_nb;  // n * max_nb < 2^31
             const float4 ai = satom[si];
             const int ti = stype[si];
             const int id_i = (NEED_ENERGY || NEED_VIRIAL) && ref_attr ? order[si] : 0;
             WinRowSums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            // list entries (window slots) of the first 128 entries; inside the
+            // loop the next 128 are fetched while the current ones are used.
+            // 32-bit byte offsets: the loads take the scalar-base + vector-
+            // offset form, no 64-bit address arithmetic per lane
+            int nxa = lane < cnt ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(lane, split, max_nb))
+                                 : -1;
+            int nxb = lane + 64 < cnt
+                          ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(lane + 64, split, max_nb))
+                          : -1;
             for (int k0 = 0; k0 < cnt; k0 += 128)
             {
                 // two entries per lane: issue both gathers before either pair
-                const int ka = k0 + lane, kb = ka + 64;
-                const int sla = ka < cnt ? (int)e16[snb_row_entry(ka, split, max_nb)] : -1;
-                const int slb = kb < cnt ? (int)e16[snb_row_entry(kb, split, max_nb)] : -1;
-                const int sja = sla >= 0 ? win_slot_sorted(sla, ov, slot_s, rfirst, rpref, W.nrange) : si;
-                const int sjb = slb >= 0 ? win_slot_sorted(slb, ov, slot_s, rfirst, rpref, W.nrange) : si;
-                const float4 aja = satom[sja], ajb = satom[sjb];
-                const int tja = stype[sja], tjb = stype[sjb];
-                const int pa = snb_lj_pair_index(ti, tja), pb = snb_lj_pair_index(ti, tjb);
-                const float Aa = lj_a[pa], Ba = lj_b[pa], Ab = lj_a[pb], Bb = lj_b[pb];
+                const int sla = nxa, slb = nxb;
+                const int ka2 = k0 + 128 + lane, kb2 = ka2 + 64;
+                nxa = ka2 < cnt ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(ka2, split, max_nb))
+                                : -1;
+                nxb = kb2 < cnt ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(kb2, split, max_nb))
+                                : -1;
+                const unsigned sja =
+                    sla >= 0 ? (unsigned)win_slot_sorted(sla, ov, slot_s, rfirst, rpref, W.nrange)
+                             : (unsigned)si;
+                const unsigned sjb =
+                    slb >= 0 ? (unsigned)win_slot_sorted(slb, ov, slot_s, rfirst, rpref, W.nrange)
+                             : (unsigned)si;
+                const float4 aja = ld32(satom, sja), ajb = ld32(satom, sjb);
+                const int tja = ld32(stype, sja), tjb = ld32(stype, sjb);
+                const unsigned pa = (unsigned)snb_lj_pair_index(ti, tja);
+                const unsigned pb = (unsigned)snb_lj_pair_index(ti, tjb);
+                const float Aa = ld32(lj_a, pa), Ba = ld32(lj_b, pa);
+                const float Ab = ld32(lj_a, pb), Bb = ld32(lj_b, pb);
                 float dxa = aja.x - ai.x, dya = aja.y - ai.y, dza = aja.z - ai.z;
                 float dxb = ajb.x - ai.x, dyb = ajb.y - ai.y, dzb = ajb.z - ai.z;
                 win_disp<ORTHO>(dxa, dya, dza, box);
                 win_disp<ORTHO>(dxb, dyb, dzb, box);
                 const float r2a = (dxa * dxa + dya * dya) + dza * dza;
                 const float r2b = (dxb * dxb + dyb * dyb) + dzb * dzb;
+                // keep the parameter gathers ahead of the branches (the
+                // compiler would otherwise sink them into the branches, where
+                // their latency is not hidden behind the distance test); an
+                // empty statement that reads the registers, nothing is emitted
+                asm volatile("" ::"v"(Aa), "v"(Ba), "v"(Ab), "v"(Bb), "v"(aja.w), "v"(ajb.w));
                 if (sla >= 0 && r2a < rc2)
                     win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxa, dya, dza, r2a, Aa, Ba, ai.w, aja.w,
                                                        beta, sla, sja, ov, W.wcap, acc, n, order,
