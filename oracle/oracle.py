@@ -75,6 +75,8 @@ def _declare(L):
     for name in ("orc_twin_expf", "orc_twin_erfc", "orc_twin_rsqrtf"):
         getattr(L, name).restype = _f
         getattr(L, name).argtypes = [_f]
+    L.orc_twin_coul_h.restype = _f
+    L.orc_twin_coul_h.argtypes = [_f, _f, _f]
     L.orc_to_fixed.restype = ctypes.c_int64
     L.orc_to_fixed.argtypes = [_f]
 

@@ -168,6 +168,60 @@ static inline void twin_erf_parts_d(double x, double* erf_out, double* h_out)
     *erf_out = g * s + g;
 }
 
+/* Screened-Coulomb force shape h(x) = erfc(x) + 2/sqrt(pi) x e^{-x^2}
+ * (the factor of qq/r^3 in Lennard_Jones_force.cpp:87-95, x = beta r) as the
+ * kernels evaluate it: TWIN_COUL_N cubic Hermite pieces over
+ * [0, beta rc (1 + 2^-7)], node k at k / scale with scale the float of
+ * N / X; the piece coefficients come from h and h' = -4/sqrt(pi) x^2 e^{-x^2}
+ * at both nodes in double, each rounded once to float.  Pinned against
+ * scipy's erfc in tests/test_oracle_pins.py. */
+#define TWIN_COUL_N 256
+static float twin_coul_table(float beta, float cutoff, float* tab)
+{
+    double X = (double)beta * (double)cutoff * (1.0 + 1.0 / 128.0);
+    if (!(X > 0.0))
+    {
+        for (int k = 0; k < TWIN_COUL_N; k++)
+        {
+            tab[4 * k] = 1.0f;
+            tab[4 * k + 1] = tab[4 * k + 2] = tab[4 * k + 3] = 0.0f;
+        }
+        return 0.0f;
+    }
+    float scale = (float)((double)TWIN_COUL_N / X);
+    double step = 1.0 / (double)scale;
+    double hprev = 0.0, dprev = 0.0;
+    for (int k = 0; k <= TWIN_COUL_N; k++)
+    {
+        double x = (double)k * step;
+        double e, g;
+        twin_erf_parts_d(x, &e, &g);
+        double hk = 1.0 - g;  /* erfc(x) + 2/sqrt(pi) x e^{-x^2} = 1 - (erf - that) */
+        double dk = ((-2.2567583341910252 * x) * x) * twin_exp_d(-(x * x)) * step;
+        if (k > 0)
+        {
+            float* c = tab + 4 * (k - 1);
+            c[0] = (float)hprev;
+            c[1] = (float)dprev;
+            c[2] = (float)((3.0 * (hk - hprev) - 2.0 * dprev) - dk);
+            c[3] = (float)((2.0 * (hprev - hk) + dprev) + dk);
+        }
+        hprev = hk;
+        dprev = dk;
+    }
+    return scale;
+}
+
+static inline float twin_coul_h(float x, float scale, const float* tab)
+{
+    float u = x * scale;
+    int k = (int)u;
+    if (k > TWIN_COUL_N - 1) k = TWIN_COUL_N - 1;
+    float t = u - (float)k;
+    const float* c = tab + 4 * k;
+    return fmaf(fmaf(fmaf(c[3], t, c[2]), t, c[1]), t, c[0]);
+}
+
 static inline int64_t to_fixed_d(double d)
 {
     double v = fma(d, ORC_FIXED_SCALE, 6755399441055744.0);
@@ -486,6 +540,8 @@ ORC_EXPORT void orc_pair_twin(int n, const float* crd, const float* charge,
                               int64_t* vir)
 {
     float rc2 = cutoff * cutoff;
+    float ctab[4 * TWIN_COUL_N];
+    float cscale = twin_coul_table(beta, cutoff, ctab);
     /* rows in parallel; the i- and j-side force terms go to per-thread
      * arrays (integer sums, so the result does not depend on the split);
      * energies and virials belong to row i alone */
@@ -515,11 +571,8 @@ ORC_EXPORT void orc_pair_twin(int n, const float* crd, const float* charge,
             float rinv6 = (rinv2 * rinv2) * rinv2;
             float flj = ((B - A * rinv6) * rinv6) * rinv2;
             float x = beta * r;
-            float ex = twin_expf(-(x * x));
-            float erfcv = twin_erfc_core(x, ex);
             float qq = qi * charge[j];
-            float fc = (qq * (rinv2 * rinv)) *
-                       ((x * ORC_TWO_OVER_SQRT_PI_F) * ex + erfcv);
+            float fc = (qq * (rinv2 * rinv)) * twin_coul_h(x, cscale, ctab);
             float fabs_ = flj - fc;
             float fx = fabs_ * d[0], fy = fabs_ * d[1], fz = fabs_ * d[2];
             int64_t ix = to_fixed(fx), iy = to_fixed(fy), iz = to_fixed(fz);
@@ -1162,4 +1215,11 @@ ORC_EXPORT void orc_twin_erf_parts(double x, double* erf_out, double* h_out)
 }
 ORC_EXPORT double orc_twin_exp_d(double y) { return twin_exp_d(y); }
 ORC_EXPORT float orc_twin_rsqrtf(float x) { return twin_rsqrtf(x); }
+/* h(x) of the pair kernels' Coulomb table for (beta, cutoff) */
+ORC_EXPORT float orc_twin_coul_h(float x, float beta, float cutoff)
+{
+    float tab[4 * TWIN_COUL_N];
+    float scale = twin_coul_table(beta, cutoff, tab);
+    return twin_coul_h(x, scale, tab);
+}
 ORC_EXPORT int64_t orc_to_fixed(float f) { return to_fixed(f); }

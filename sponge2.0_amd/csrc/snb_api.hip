@@ -148,8 +148,8 @@ bool choose_window_pass(int nx, int ny, int nz, double atoms_per_cell, int max_l
                         int min_blocks, SnbWin* W)
 {
     if (nx < 5 || ny < 5 || nz < 5) return false;
-    // LDS per slot: 3 x int64 accumulator + int slot map; plus two range tables
-    int wcap = (max_lds - (int)sizeof(int) * (2 * SNB_WIN_MAX_RANGES + 1)) / 28;
+    // LDS per slot: 3 x int64 accumulator + int slot map; plus the fixed part
+    int wcap = (max_lds - SNB_WIN_FIXED_LDS) / 28;
     if (wcap > 65535) wcap = 65535;
     static const int cand[][3] = {{2, 2, 8}, {2, 2, 6}, {2, 2, 4}, {2, 2, 3}, {2, 2, 2},
                                   {1, 2, 4}, {1, 1, 4}, {1, 1, 2}, {1, 1, 1}};
@@ -231,8 +231,33 @@ int prof_collect(snb_context* c, int g)
 }
 }  // namespace
 
+// Puts a deferred force-only exclusion correction onto pme_stream (after
+// the step's inputs are final), accumulating into fx_pme.
+static int launch_excl_later(snb_context* c)
+{
+    if (!c->excl_later.pending) return SNB_OK;
+    c->excl_later.pending = false;
+    hipStream_t home = c->stream;
+    if (!c->pme_pending)
+    {
+        if (!c->inputs_marked) SNB_CHECK(hipEventRecord(c->ev_inputs, home));
+        SNB_CHECK(hipStreamWaitEvent(c->pme_stream, c->ev_inputs, 0));
+    }
+    c->stream = c->pme_stream;
+    const auto& X = c->excl_later;
+    int rc = snb_launch_excl(c, X.crd, X.charge, X.box, X.beta, X.ex_start, X.ex_list, X.ex_num,
+                             0, 0, c->fx_pme);
+    c->stream = home;
+    if (rc) return rc;
+    SNB_CHECK(hipEventRecord(c->ev_pme, c->pme_stream));
+    c->pme_pending = true;
+    return SNB_OK;
+}
+
 int snb_join_pme(snb_context* c)
 {
+    int rc = launch_excl_later(c);
+    if (rc) return rc;
     if (!c->pme_pending) return SNB_OK;
     SNB_CHECK(hipStreamWaitEvent(c->stream, c->ev_pme, 0));
     c->pme_pending = false;
@@ -382,6 +407,8 @@ int snb_context_destroy(snb_context* c)
     }
     dfree(c->lj.a);
     dfree(c->lj.b);
+    dfree(c->lj.ab2);
+    dfree(c->lj.coul);
     delete c;
     return SNB_OK;
 }
@@ -642,7 +669,7 @@ int snb_nl_set_window_capacity(snb_context* c, int slots)
     if (!c || slots < 64) return SNB_ERR_ARGUMENT;
     if (!c->nl.init) return SNB_ERR_NOT_INITIALIZED;
     if (!c->nl.windowed) return SNB_OK;
-    int most = (c->max_lds - (int)sizeof(int) * (2 * SNB_WIN_MAX_RANGES + 1)) / 28;
+    int most = (c->max_lds - SNB_WIN_FIXED_LDS) / 28;
     if (most > 65535) most = 65535;
     DeviceGuard g(c->device);
     int rc = snb_join_pme(c);
@@ -699,12 +726,27 @@ int snb_lj_initial(snb_context* c, int ntypes, const float* h_a, const float* h_
     SnbLj& J = c->lj;
     dfree(J.a);
     dfree(J.b);
+    dfree(J.ab2);
     size_t npair = (size_t)ntypes * (ntypes + 1) / 2;
     int rc = dalloc(&J.a, npair);
     if (!rc) rc = dalloc(&J.b, npair);
+    if (!rc) rc = dalloc(&J.ab2, (size_t)ntypes * ntypes);
+    if (!rc && !J.coul) rc = dalloc(&J.coul, SNB_COUL_N);
     if (rc) return rc;
     SNB_CHECK(hipMemcpy(J.a, h_a, sizeof(float) * npair, hipMemcpyHostToDevice));
     SNB_CHECK(hipMemcpy(J.b, h_b, sizeof(float) * npair, hipMemcpyHostToDevice));
+    // square table for the windowed kernel, both parameters times 2^32 (an
+    // exact power-of-two scaling)
+    std::vector<float2> ab2((size_t)ntypes * ntypes);
+    for (int a = 0; a < ntypes; a++)
+        for (int b = 0; b < ntypes; b++)
+        {
+            int p = snb_lj_pair_index(a, b);
+            ab2[(size_t)a * ntypes + b] =
+                make_float2(h_a[p] * 4294967296.0f, h_b[p] * 4294967296.0f);
+        }
+    SNB_CHECK(hipMemcpy(J.ab2, ab2.data(), sizeof(float2) * ab2.size(), hipMemcpyHostToDevice));
+    J.coul_beta = -1.0f;
     if (c->n > 0) SNB_CHECK(hipMemcpy(c->type, h_type, sizeof(int) * c->n, hipMemcpyHostToDevice));
     J.ntypes = ntypes;
     J.cutoff = cutoff;
@@ -732,6 +774,26 @@ int snb_lj_initial(snb_context* c, int ntypes, const float* h_a, const float* h_
     return SNB_OK;
 }
 
+// The Coulomb force-shape table for beta and the LJ cutoff, rebuilt when
+// either changed.  The copy is ordered on the context stream after every
+// kernel that may still read the old table.
+static int coulomb_table(snb_context* c, float beta)
+{
+    SnbLj& J = c->lj;
+    if (J.coul_beta == beta && J.coul_cutoff == J.cutoff) return SNB_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    SNB_CHECK(hipStreamIsCapturing(c->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return SNB_ERR_UNSUPPORTED;  // not inside a capture
+    SNB_CHECK(hipStreamSynchronize(c->stream));  // the previous host copy is consumed
+    J.coul_host.assign(4 * SNB_COUL_N, 0.0f);
+    J.coul_scale = snb_coul_table(beta, J.cutoff, J.coul_host.data());
+    SNB_CHECK(hipMemcpyAsync(J.coul, J.coul_host.data(), sizeof(float) * 4 * SNB_COUL_N,
+                             hipMemcpyHostToDevice, c->stream));
+    J.coul_beta = beta;
+    J.coul_cutoff = J.cutoff;
+    return SNB_OK;
+}
+
 int snb_lj_pme_direct_force(snb_context* c, int atom_numbers, int local_atom_numbers,
                             int solvent_numbers, int ghost_numbers, const snb_vector* d_crd,
                             const float* d_charge, snb_vector* d_frc, snb_ltmatrix3 cell,
@@ -745,9 +807,12 @@ int snb_lj_pme_direct_force(snb_context* c, int atom_numbers, int local_atom_num
     if (atom_numbers != c->n || local_atom_numbers != c->n) return SNB_ERR_ARGUMENT;
     if (c->n == 0) return SNB_OK;
     if (!d_crd || !d_charge || !d_nl) return SNB_ERR_ARGUMENT;
+    // beta rc above 4.9 means an Ewald tolerance below 1e-11 (erfc(4.9) = 4e-12)
+    if (!(pme_beta >= 0.0f) || pme_beta * c->lj.cutoff > 4.9f) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
     SnbBox box = snb_box(cell, rcell);
     int rc;
+    if ((rc = coulomb_table(c, pme_beta))) return rc;
     if (need_atom_energy)
     {
         // the per-atom LJ and direct arrays restart each energy call
@@ -872,17 +937,19 @@ int snb_pme_excluded_force(snb_context* c, int atom_numbers, const int* d_atom_l
     int rc;
     if (c->overlap && !need_energy && !need_virial && !d_frc)
     {
-        // force-only: the correction goes onto the reciprocal stream, into the
-        // reciprocal accumulator, ahead of the PME pipeline of the step
-        hipStream_t home = c->stream;
-        if (!c->inputs_marked) SNB_CHECK(hipEventRecord(c->ev_inputs, home));
-        SNB_CHECK(hipStreamWaitEvent(c->pme_stream, c->ev_inputs, 0));
-        c->stream = c->pme_stream;
-        rc = snb_launch_excl(c, d_crd, d_charge, box, c->pme.beta, es, el, en, 0, 0, c->fx_pme);
-        c->stream = home;
-        if (rc) return rc;
-        SNB_CHECK(hipEventRecord(c->ev_pme, c->pme_stream));
-        c->pme_pending = true;
+        // force-only: the correction runs on the reciprocal stream, into the
+        // reciprocal accumulator, behind the step's PME pipeline (it is only
+        // needed at the flush); a join launches it if no PME call comes
+        if ((rc = launch_excl_later(c))) return rc;  // an earlier one still waiting
+        auto& X = c->excl_later;
+        X.pending = true;
+        X.crd = d_crd;
+        X.charge = d_charge;
+        X.ex_start = es;
+        X.ex_list = el;
+        X.ex_num = en;
+        X.box = box;
+        X.beta = c->pme.beta;
         return SNB_OK;
     }
     if (need_energy && (rc = snb_launch_fill_i64(c->stream, c->e_atom + 2 * (size_t)c->n, c->n)))
@@ -947,6 +1014,7 @@ int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatr
             c->stream = home;
             if (!rc && hipEventRecord(c->ev_pme, c->pme_stream) != hipSuccess) rc = SNB_ERR_HIP;
             if (!rc) c->pme_pending = true;
+            if (!rc) rc = launch_excl_later(c);  // behind the pipeline, same stream
         }
         if (rc) return rc;
         P.have_energy = need_energy != 0;

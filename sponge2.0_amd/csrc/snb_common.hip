@@ -74,8 +74,10 @@ int snb_launch_fill_i64(hipStream_t s, int64_t* p, size_t n)
 
 // Exclusive scan of n ints by one 1024-thread workgroup: each thread scans a
 // contiguous chunk serially, the chunk totals are scanned in LDS.  Used for
-// cell and brick offsets (n <= a few 1e5).
-__global__ void __launch_bounds__(1024) k_exclusive_scan(const int* __restrict__ in,
+// cell and brick offsets (n <= a few 1e5).  CLEAR zeroes the input after use
+// (counters that must start the next call at zero).
+template <bool CLEAR>
+__global__ void __launch_bounds__(1024) k_exclusive_scan(int* __restrict__ in,
                                                          int* __restrict__ out, int n)
 {
     __shared__ int part[1024];
@@ -99,13 +101,21 @@ __global__ void __launch_bounds__(1024) k_exclusive_scan(const int* __restrict__
         int v = in[i];
         out[i] = run;
         run += v;
+        if (CLEAR) in[i] = 0;
     }
     if (t == 1023) out[n] = part[1023];
 }
 
 int snb_launch_exclusive_scan(hipStream_t s, const int* in, int* out, int n)
 {
-    hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, in, out, n);
+    hipLaunchKernelGGL(k_exclusive_scan<false>, dim3(1), dim3(1024), 0, s, (int*)in, out, n);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
+int snb_launch_exclusive_scan_clear(hipStream_t s, int* in, int* out, int n)
+{
+    hipLaunchKernelGGL(k_exclusive_scan<true>, dim3(1), dim3(1024), 0, s, in, out, n);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }

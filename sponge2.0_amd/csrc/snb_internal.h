@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/sponge_nb.h"
 #include "snb_math.h"
 #include "snb_window.h"
@@ -87,6 +89,12 @@ struct SnbLj
     float cutoff = 10.0f;
     float* a = nullptr;  // pair tables
     float* b = nullptr;
+    // (A, B) of type pair (ti, tj) at ti * ntypes + tj, scaled by 2^32
+    float2* ab2 = nullptr;
+    // Coulomb force shape table (SNB_COUL_N pieces) for coul_beta and the cutoff
+    float4* coul = nullptr;
+    float coul_scale = 0.0f, coul_beta = -1.0f, coul_cutoff = -1.0f;
+    std::vector<float> coul_host;
     double c6_total = 0.0;
     float long_range_factor = 0.0f;
     double last_long_energy = 0.0;
@@ -114,7 +122,7 @@ struct SnbPme
     float2* spec = nullptr;   // [fx][fy][fc]
     int4* atom_u = nullptr;   // ux, uy, uz, brick
     float4* atom_fr = nullptr;  // frx, fry, frz, q
-    int* brick_count = nullptr;  // nbrick + 1
+    int* brick_count = nullptr;  // nbrick + 1 (zero between calls)
     int* brick_start = nullptr;  // nbrick + 1
     int* atom_slot = nullptr;
     int* brick_atoms = nullptr;
@@ -136,6 +144,12 @@ struct SnbProf
     int count[SNB_PROF_GROUPS] = {0, 0, 0, 0};
     double done_ms[SNB_PROF_GROUPS] = {0, 0, 0, 0};
     int done_n[SNB_PROF_GROUPS] = {0, 0, 0, 0};
+};
+
+struct SnbBox
+{
+    float c[6];   // cell a11 a21 a22 a31 a32 a33
+    float rc[6];  // rcell
 };
 
 struct snb_context
@@ -178,6 +192,19 @@ struct snb_context
     hipEvent_t ev_pme = nullptr;
     bool inputs_marked = false;
     bool pme_pending = false;
+    // a force-only exclusion correction waiting to go onto pme_stream behind
+    // the step's PME pipeline (or at the next join, when no PME call comes)
+    struct
+    {
+        bool pending = false;
+        const snb_vector* crd = nullptr;
+        const float* charge = nullptr;
+        const int* ex_start = nullptr;
+        const int* ex_list = nullptr;
+        const int* ex_num = nullptr;
+        SnbBox box;
+        float beta = 0.0f;
+    } excl_later;
     SnbNl nl;
     SnbLj lj;
     SnbPme pme;
@@ -202,11 +229,6 @@ void snb_prof_end(snb_context* ctx, int group);
         if (e_ != hipSuccess) return SNB_ERR_HIP; \
     } while (0)
 
-struct SnbBox
-{
-    float c[6];   // cell a11 a21 a22 a31 a32 a33
-    float rc[6];  // rcell
-};
 
 static inline SnbBox snb_box(snb_ltmatrix3 cell, snb_ltmatrix3 rcell)
 {
@@ -224,6 +246,7 @@ int snb_launch_fill_int(hipStream_t s, int* p, int n, int v);
 int snb_launch_fill_i64(hipStream_t s, int64_t* p, size_t n);
 int snb_launch_add_i64(hipStream_t s, const int64_t* a, const int64_t* b, int64_t* out, size_t n);
 int snb_launch_exclusive_scan(hipStream_t s, const int* in, int* out, int n);
+int snb_launch_exclusive_scan_clear(hipStream_t s, int* in, int* out, int n);
 int snb_launch_flush_forces(snb_context* ctx, snb_vector* frc);  // joins the PME stream
 int snb_launch_flush_atom_energy(snb_context* ctx, float* atom_energy, float* direct_energy,
                                  int which_mask);

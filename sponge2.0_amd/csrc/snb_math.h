@@ -74,7 +74,9 @@ SNB_HD float snb_erfc_core(float x, float ex)
 }
 
 // 1/sqrt(x), x > 0 normal: seed 0x5f375a86 - (bits >> 1) (Lomont, "Fast
-// inverse square root", 2003), three Newton steps y <- y (1.5 - (x/2) y^2).
+// inverse square root", 2003; relative error below 3.5e-2), three Newton
+// steps y <- y (1.5 - (x/2) y^2): the error squares each step (x 1.5),
+// 1.8e-3, 5e-6, then 4e-11 before the float rounding of the steps.
 SNB_HD float snb_rsqrtf(float x)
 {
     float y = snb_f_from_bits(0x5f375a86u - (snb_bits_from_f(x) >> 1));
@@ -83,6 +85,44 @@ SNB_HD float snb_rsqrtf(float x)
     y = y * __builtin_fmaf(-(h * y), y, 1.5f);
     y = y * __builtin_fmaf(-(h * y), y, 1.5f);
     return y;
+}
+
+// round(v) to int64, ties to even, for a float v that already carries the
+// 2^32 fixed-point scale (|v| < 2^51): v + 1.5*2^52 in double rounds to an
+// integer, which is the same as snb_to_fixed(v / 2^32).
+SNB_HD int64_t snb_fixed_from_scaled(float v)
+{
+    double d = (double)v + 6755399441055744.0;
+    return (int64_t)__builtin_bit_cast(uint64_t, d) - (int64_t)0x4338000000000000LL;
+}
+
+// ---------------------------------------------------------------------------
+// Screened-Coulomb force shape h(x) = erfc(x) + (2/sqrt(pi)) x e^{-x^2}, the
+// factor of q_i q_j / r^3 in the direct-space force (Lennard_Jones_force.cpp
+// :87-95 with x = beta r).  Tabulated as SNB_COUL_N cubic Hermite pieces on
+// x in [0, X], X = beta * cutoff * (1 + 2^-7): node k at x_k = k / scale,
+// scale = (float)(SNB_COUL_N / X); piece k holds {c0, c1, c2, c3} of
+// h(x_k + t / scale) = c0 + c1 t + c2 t^2 + c3 t^3, t in [0, 1], from h and
+// h'(x) = -(4/sqrt(pi)) x^2 e^{-x^2} at both ends, evaluated in double by the
+// functions below and rounded once to float.  Interpolation error
+// max|h''''| dx^4 / 384 < 1e-9 at 256 pieces over x <= 4.
+// ---------------------------------------------------------------------------
+#define SNB_COUL_N 256
+
+// Piece index and local coordinate of x >= 0; x a little past X (the rounding
+// of r = r2 * rsqrt(r2) at the cutoff) stays on the last piece.
+SNB_HD int snb_coul_piece(float x, float scale, float* t)
+{
+    float u = x * scale;
+    int k = (int)u;
+    k = k < SNB_COUL_N - 1 ? k : SNB_COUL_N - 1;
+    *t = u - (float)k;
+    return k;
+}
+
+SNB_HD float snb_coul_eval(float t, float c0, float c1, float c2, float c3)
+{
+    return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c3, t, c2), t, c1), t, c0);
 }
 
 // ---------------------------------------------------------------------------
@@ -134,6 +174,47 @@ SNB_HD void snb_erf_parts_d(double x, double* erf_out, double* h_out)
     }
     *h_out = g * s;
     *erf_out = g * s + g;
+}
+
+// The SNB_COUL_N pieces of h (4 floats each, layout above) for beta and
+// cutoff; returns scale.  h(x) = 1 - (erf(x) - (2/sqrt(pi)) x e^{-x^2}), the
+// second term straight from the series, so there is no cancellation.
+SNB_HD float snb_coul_table(float beta, float cutoff, float* tab)
+{
+    const double X = (double)beta * (double)cutoff * (1.0 + 1.0 / 128.0);
+    if (!(X > 0.0))
+    {
+        // beta = 0: h = 1 everywhere, every x lands on piece 0 at t = 0
+        for (int k = 0; k < SNB_COUL_N; k++)
+        {
+            tab[4 * k] = 1.0f;
+            tab[4 * k + 1] = tab[4 * k + 2] = tab[4 * k + 3] = 0.0f;
+        }
+        return 0.0f;
+    }
+    const float scale = (float)((double)SNB_COUL_N / X);
+    const double dx = 1.0 / (double)scale;
+    const double four_over_sqrt_pi = 2.2567583341910252;
+    double h0 = 0.0, d0 = 0.0;
+    for (int k = 0; k <= SNB_COUL_N; k++)
+    {
+        const double x = (double)k * dx;
+        double e, g;
+        snb_erf_parts_d(x, &e, &g);
+        const double h1 = 1.0 - g;
+        const double d1 = ((-four_over_sqrt_pi * x) * x) * snb_exp_d(-(x * x)) * dx;
+        if (k > 0)
+        {
+            float* c = tab + 4 * (k - 1);
+            c[0] = (float)h0;
+            c[1] = (float)d0;
+            c[2] = (float)((3.0 * (h1 - h0) - 2.0 * d0) - d1);
+            c[3] = (float)((2.0 * (h0 - h1) + d0) + d1);
+        }
+        h0 = h1;
+        d0 = d1;
+    }
+    return scale;
 }
 
 // round(d * 2^32) to nearest-even for |d| < 2^19.

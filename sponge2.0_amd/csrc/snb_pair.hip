@@ -43,11 +43,33 @@ __device__ __forceinline__ void atomic_add_i64(int64_t* p, int64_t v)
     atomicAdd((unsigned long long*)p, (unsigned long long)v);
 }
 
+// Force factor of one pair (Lennard_Jones_force.cpp:73-97): LJ plus the
+// erfc-screened Coulomb term, float arithmetic of the library's own math,
+// the Coulomb shape from the Hermite table (ctab, SNB_COUL_N float4 pieces).
+// A, B and qq may carry a common power-of-two scale; the result then carries
+// it exactly (every operation is a product or difference of scaled terms).
+__device__ __forceinline__ float pair_force_factor(float r2, float A, float B, float qq,
+                                                   float beta, float cscale,
+                                                   const float4* ctab)
+{
+    float rinv = snb_rsqrtf(r2);
+    float r = r2 * rinv;
+    float rinv2 = rinv * rinv;
+    float rinv6 = (rinv2 * rinv2) * rinv2;
+    float flj = ((B - A * rinv6) * rinv6) * rinv2;
+    float t;
+    int k = snb_coul_piece(beta * r, cscale, &t);
+    float4 c = ctab[k];
+    float fc = (qq * (rinv2 * rinv)) * snb_coul_eval(t, c.x, c.y, c.z, c.w);
+    return flj - fc;
+}
+
 template <bool NEED_ENERGY, bool NEED_VIRIAL>
 __global__ void __launch_bounds__(256) k_lj_direct(
     int n, const snb_atom_group* __restrict__ nl, const float4* __restrict__ atoms,
     const int* __restrict__ type, const float* __restrict__ lj_a,
-    const float* __restrict__ lj_b, SnbBox box, float rc2, float beta,
+    const float* __restrict__ lj_b, const float4* __restrict__ coul_tab, float cscale,
+    SnbBox box, float rc2, float beta,
     int64_t* __restrict__ fx, int64_t* __restrict__ e_atom, int64_t* __restrict__ vir)
 {
     const int lane = threadIdx.x & 63;
@@ -70,17 +92,7 @@ __global__ void __launch_bounds__(256) k_lj_direct(
             if (!(r2 < rc2)) continue;
             const int p = snb_lj_pair_index(ti, type[j]);
             const float A = lj_a[p], B = lj_b[p];
-            float rinv = snb_rsqrtf(r2);
-            float r = r2 * rinv;
-            float rinv2 = rinv * rinv;
-            float rinv6 = (rinv2 * rinv2) * rinv2;
-            float flj = ((B - A * rinv6) * rinv6) * rinv2;
-            float x = beta * r;
-            float ex = snb_expf(-(x * x));
-            float erfcv = snb_erfc_core(x, ex);
-            float qq = ai.w * aj.w;
-            float fc = (qq * (rinv2 * rinv)) * ((x * SNB_TWO_OVER_SQRT_PI) * ex + erfcv);
-            float fa = flj - fc;
+            const float fa = pair_force_factor(r2, A, B, ai.w * aj.w, beta, cscale, coul_tab);
             float fxv = fa * dx, fyv = fa * dy, fzv = fa * dz;
             int64_t ix = snb_to_fixed(fxv), iy = snb_to_fixed(fyv), iz = snb_to_fixed(fzv);
             sx += ix;
@@ -161,7 +173,8 @@ int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBo
     snb_prof_begin(ctx, SNB_PROF_PAIR);
 #define SNB_LJ_LAUNCH(E, V)                                                                   \
     hipLaunchKernelGGL((k_lj_direct<E, V>), dim3(blocks), dim3(256), 0, ctx->stream, n, nl,    \
-                       ctx->atom4, ctx->type, ctx->lj.a, ctx->lj.b, box, rc2, beta, ctx->fx, \
+                       ctx->atom4, ctx->type, ctx->lj.a, ctx->lj.b, ctx->lj.coul,             \
+                       ctx->lj.coul_scale, box, rc2, beta, ctx->fx,                            \
                        ctx->e_atom, ctx->vir_atom)
     if (need_energy && need_virial)
         SNB_LJ_LAUNCH(true, true);
@@ -310,17 +323,20 @@ __global__ void k_pack_sorted(int n, const int* __restrict__ order,
 // on the id-ordered arrays otherwise (energy/virial steps only); with
 // ref_attr == 0 every pair's terms stay with the row owner.
 
-// Periodic displacement; ORTHO drops the off-diagonal terms, which is the
-// same arithmetic when cell and rcell are diagonal (x + 0 == x and 0 * c == 0
-// change no bits that reach the result).
+// Periodic displacement.  ORTHO (diagonal cell) drops the off-diagonal terms
+// and takes k = rint(s) for floor(s + 0.5).  For every pair that can reach
+// the cutoff this is the reference's arithmetic bit for bit: s lies within
+// rc / L < 1/2 of an integer there, so both give the same k.  The two differ
+// only for s within an ulp of a half-integer, i.e. for pairs half a box
+// length apart, which are beyond the cutoff either way.
 template <bool ORTHO>
 __device__ __forceinline__ void win_disp(float& dx, float& dy, float& dz, const SnbBox& box)
 {
     if (ORTHO)
     {
-        float kx = floorf(dx * box.rc[0] + 0.5f);
-        float ky = floorf(dy * box.rc[2] + 0.5f);
-        float kz = floorf(dz * box.rc[5] + 0.5f);
+        const float kx = rintf(dx * box.rc[0]);
+        const float ky = rintf(dy * box.rc[2]);
+        const float kz = rintf(dz * box.rc[5]);
         dx = dx - kx * box.c[0];
         dy = dy - ky * box.c[2];
         dz = dz - kz * box.c[5];
@@ -329,50 +345,40 @@ __device__ __forceinline__ void win_disp(float& dx, float& dy, float& dz, const 
         pbc_disp(dx, dy, dz, box);
 }
 
-// Force factor of one pair (Lennard_Jones_force.cpp:73-97): LJ plus the
-// erfc-screened Coulomb term, float arithmetic of the library's own math.
-__device__ __forceinline__ float pair_force_factor(float r2, float A, float B, float qq,
-                                                   float beta)
-{
-    float rinv = snb_rsqrtf(r2);
-    float r = r2 * rinv;
-    float rinv2 = rinv * rinv;
-    float rinv6 = (rinv2 * rinv2) * rinv2;
-    float flj = ((B - A * rinv6) * rinv6) * rinv2;
-    float x = beta * r;
-    float ex = snb_expf(-(x * x));
-    float erfcv = snb_erfc_core(x, ex);
-    float fc = (qq * (rinv2 * rinv)) * ((x * SNB_TWO_OVER_SQRT_PI) * ex + erfcv);
-    return flj - fc;
-}
-
 struct WinRowSums
 {
     int64_t sx, sy, sz, elj, ec, v0, v1, v2, v3, v4, v5;
 };
 
-// Everything one in-cutoff pair contributes: -f to the j accumulator (LDS
-// slot, or the global array in overflow mode), +f and the row owner's share
-// of energy/virial to the register sums.
+// Everything one in-cutoff pair contributes: f (force on the row atom i) to
+// the register sums, the j share to the j accumulator, the row owner's share
+// of energy/virial to the register sums.  A2, B2 and qi2 carry the 2^32
+// fixed-point scale (exact power-of-two scaling), so f comes out scaled and
+// its conversion is one double add.  The LDS accumulators hold the negated
+// force (j receives +f, the row atom -sum f at the row's end), which saves
+// the 64-bit negation per pair; the global arrays of the overflow mode hold
+// the force itself.
 template <bool NEED_ENERGY, bool NEED_VIRIAL>
 __device__ __forceinline__ void win_pair(WinRowSums& S, float dx, float dy, float dz, float r2,
-                                         float A, float B, float qi, float qj, float beta,
+                                         float A2, float B2, float qi2, float qj, float beta,
+                                         float cscale, const float4* ctab,
                                          int slot, int sj, bool ov, int wcap, long long* acc,
                                          int n, const int* __restrict__ order,
                                          int64_t* __restrict__ fx, int ref_attr, int id_i,
                                          int64_t* __restrict__ e_atom, int64_t* __restrict__ vir)
 {
-    float fa = pair_force_factor(r2, A, B, qi * qj, beta);
-    float fxv = fa * dx, fyv = fa * dy, fzv = fa * dz;
-    int64_t ix = snb_to_fixed(fxv), iy = snb_to_fixed(fyv), iz = snb_to_fixed(fzv);
+    const float fa2 = pair_force_factor(r2, A2, B2, qi2 * qj, beta, cscale, ctab);
+    const float fx2 = fa2 * dx, fy2 = fa2 * dy, fz2 = fa2 * dz;
+    int64_t ix = snb_fixed_from_scaled(fx2), iy = snb_fixed_from_scaled(fy2),
+            iz = snb_fixed_from_scaled(fz2);
     S.sx += ix;
     S.sy += iy;
     S.sz += iz;
     if (!ov)
     {
-        atomicAdd((unsigned long long*)&acc[slot], (unsigned long long)(-ix));
-        atomicAdd((unsigned long long*)&acc[wcap + slot], (unsigned long long)(-iy));
-        atomicAdd((unsigned long long*)&acc[2 * wcap + slot], (unsigned long long)(-iz));
+        atomicAdd((unsigned long long*)&acc[slot], (unsigned long long)ix);
+        atomicAdd((unsigned long long*)&acc[wcap + slot], (unsigned long long)iy);
+        atomicAdd((unsigned long long*)&acc[2 * wcap + slot], (unsigned long long)iz);
     }
     else
     {
@@ -391,6 +397,9 @@ __device__ __forceinline__ void win_pair(WinRowSums& S, float dx, float dy, floa
         }
         if (NEED_ENERGY)
         {
+            // unscaled parameters (2^-32 scaling is exact)
+            const float A = A2 * 2.3283064365386963e-10f, B = B2 * 2.3283064365386963e-10f;
+            const float qi = qi2 * 2.3283064365386963e-10f;
             double ddx = dx, ddy = dy, ddz = dz;
             double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
             double rd = __builtin_sqrt(r2d);
@@ -415,6 +424,8 @@ __device__ __forceinline__ void win_pair(WinRowSums& S, float dx, float dy, floa
         }
         if (NEED_VIRIAL)
         {
+            const float fxv = fx2 * 2.3283064365386963e-10f, fyv = fy2 * 2.3283064365386963e-10f,
+                        fzv = fz2 * 2.3283064365386963e-10f;
             int64_t p0 = snb_to_fixed(-(fxv * dx));
             int64_t p1 = snb_to_fixed(-(fxv * dy + fyv * dx));
             int64_t p2 = snb_to_fixed(-(fyv * dy));
@@ -462,21 +473,23 @@ __global__ void __launch_bounds__(1024) k_pair_win(
     const int* __restrict__ row_count, const int* __restrict__ row_split,
     const uint16_t* __restrict__ nl16, int max_nb,
     const float4* __restrict__ satom, const int* __restrict__ stype,
-    const int* __restrict__ order, const float* __restrict__ lj_a,
-    const float* __restrict__ lj_b, SnbBox box, float rc2, float beta,
+    const int* __restrict__ order, const float2* __restrict__ ab2, int ntypes,
+    const float4* __restrict__ coul_tab, float cscale, SnbBox box, float rc2, float beta,
     const int* __restrict__ stage_base, long long* __restrict__ stage, size_t stage_cap,
     int64_t* __restrict__ fx, int64_t* __restrict__ e_sorted, int64_t* __restrict__ v_sorted,
     int ref_attr, int64_t* __restrict__ e_atom, int64_t* __restrict__ vir,
     const int* __restrict__ perm)
 {
     extern __shared__ long long smem[];
-    long long* acc = smem;                          // [3][wcap]
-    int* slot_s = (int*)(smem + 3 * (size_t)W.wcap);  // [wcap]
+    float4* ctab = (float4*)smem;                   // [SNB_COUL_N]
+    long long* acc = smem + 2 * SNB_COUL_N;         // [3][wcap]
+    int* slot_s = (int*)(acc + 3 * (size_t)W.wcap);  // [wcap]
     int* rfirst = slot_s + W.wcap;                  // [128]
     int* rpref = rfirst + SNB_WIN_MAX_RANGES;       // [129]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nwave = blockDim.x >> 6;
     const int w = perm[blockIdx.x];  // heaviest blocks first
+    if (tid < SNB_COUL_N) ctab[tid] = coul_tab[tid];
     int x0, y0, z0;
     snb_win_block(W, w, x0, y0, z0);
     // range table: sizes, then a wave-0 scan
@@ -538,7 +551,8 @@ __global__ void __launch_bounds__(1024) k_pair_win(
             const int cnt = row_count[si], split = row_split[si];
             const unsigned row0 = (unsigned)si * (unsigned)max_nb;  // n * max_nb < 2^31
             const float4 ai = satom[si];
-            const int ti = stype[si];
+            const float qi2 = ai.w * 4294967296.0f;  // 2^32 fixed-point scale
+            const unsigned trow = (unsigned)(stype[si] * ntypes);
             const int id_i = (NEED_ENERGY || NEED_VIRIAL) && ref_attr ? order[si] : 0;
             WinRowSums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
             // list entries (window slots) of the first 128 entries; inside the
@@ -567,10 +581,8 @@ __global__ void __launch_bounds__(1024) k_pair_win(
                              : (unsigned)si;
                 const float4 aja = ld32(satom, sja), ajb = ld32(satom, sjb);
                 const int tja = ld32(stype, sja), tjb = ld32(stype, sjb);
-                const unsigned pa = (unsigned)snb_lj_pair_index(ti, tja);
-                const unsigned pb = (unsigned)snb_lj_pair_index(ti, tjb);
-                const float Aa = ld32(lj_a, pa), Ba = ld32(lj_b, pa);
-                const float Ab = ld32(lj_a, pb), Bb = ld32(lj_b, pb);
+                const float2 ABa = ld32(ab2, trow + (unsigned)tja);
+                const float2 ABb = ld32(ab2, trow + (unsigned)tjb);
                 float dxa = aja.x - ai.x, dya = aja.y - ai.y, dza = aja.z - ai.z;
                 float dxb = ajb.x - ai.x, dyb = ajb.y - ai.y, dzb = ajb.z - ai.z;
                 win_disp<ORTHO>(dxa, dya, dza, box);
@@ -581,15 +593,18 @@ __global__ void __launch_bounds__(1024) k_pair_win(
                 // compiler would otherwise sink them into the branches, where
                 // their latency is not hidden behind the distance test); an
                 // empty statement that reads the registers, nothing is emitted
-                asm volatile("" ::"v"(Aa), "v"(Ba), "v"(Ab), "v"(Bb), "v"(aja.w), "v"(ajb.w));
+                asm volatile("" ::"v"(ABa.x), "v"(ABa.y), "v"(ABb.x), "v"(ABb.y), "v"(aja.w),
+                             "v"(ajb.w));
                 if (sla >= 0 && r2a < rc2)
-                    win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxa, dya, dza, r2a, Aa, Ba, ai.w, aja.w,
-                                                       beta, sla, sja, ov, W.wcap, acc, n, order,
-                                                       fx, ref_attr, id_i, e_atom, vir);
+                    win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxa, dya, dza, r2a, ABa.x, ABa.y, qi2,
+                                                       aja.w, beta, cscale, ctab, sla, sja, ov,
+                                                       W.wcap, acc, n, order, fx, ref_attr, id_i,
+                                                       e_atom, vir);
                 if (slb >= 0 && r2b < rc2)
-                    win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxb, dyb, dzb, r2b, Ab, Bb, ai.w, ajb.w,
-                                                       beta, slb, sjb, ov, W.wcap, acc, n, order,
-                                                       fx, ref_attr, id_i, e_atom, vir);
+                    win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxb, dyb, dzb, r2b, ABb.x, ABb.y, qi2,
+                                                       ajb.w, beta, cscale, ctab, slb, sjb, ov,
+                                                       W.wcap, acc, n, order, fx, ref_attr, id_i,
+                                                       e_atom, vir);
             }
             int64_t sx = wave_sum(S.sx), sy = wave_sum(S.sy), sz = wave_sum(S.sz);
             int64_t elj = 0, ec = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
@@ -610,10 +625,12 @@ __global__ void __launch_bounds__(1024) k_pair_win(
                     int cz = sorted_cell[si] % W.nz;
                     int k = snb_win_range_of(W, z0, ua, vb + 2, cz - z0 + 2);
                     int slot_i = rpref[k] + (si - rfirst[k]);
-                    atomicAdd((unsigned long long*)&acc[slot_i], (unsigned long long)sx);
-                    atomicAdd((unsigned long long*)&acc[W.wcap + slot_i], (unsigned long long)sy);
+                    // negated-force convention of the LDS accumulators
+                    atomicAdd((unsigned long long*)&acc[slot_i], (unsigned long long)(-sx));
+                    atomicAdd((unsigned long long*)&acc[W.wcap + slot_i],
+                              (unsigned long long)(-sy));
                     atomicAdd((unsigned long long*)&acc[2 * W.wcap + slot_i],
-                              (unsigned long long)sz);
+                              (unsigned long long)(-sz));
                 }
                 else
                 {
@@ -645,9 +662,9 @@ __global__ void __launch_bounds__(1024) k_pair_win(
     {
         if ((size_t)(base + t) >= stage_cap) break;  // cannot happen: stage holds n x invmax
         long long* dst = stage + 3 * (size_t)(base + t);
-        dst[0] = ov ? 0 : acc[t];
-        dst[1] = ov ? 0 : acc[W.wcap + t];
-        dst[2] = ov ? 0 : acc[2 * W.wcap + t];
+        dst[0] = ov ? 0 : -acc[t];
+        dst[1] = ov ? 0 : -acc[W.wcap + t];
+        dst[2] = ov ? 0 : -acc[2 * W.wcap + t];
     }
 }
 
@@ -700,8 +717,7 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
     hipLaunchKernelGGL(k_pack_sorted, dim3((n + 255) / 256), dim3(256), 0, s, n, L.order, crd,
                        charge, ctx->satom, ctx->status);
     float rc2 = ctx->lj.cutoff * ctx->lj.cutoff;
-    size_t lds = sizeof(long long) * 3 * (size_t)W.wcap + sizeof(int) * (size_t)W.wcap +
-                 sizeof(int) * (2 * SNB_WIN_MAX_RANGES + 1);
+    size_t lds = 28 * (size_t)W.wcap + SNB_WIN_FIXED_LDS;
     if (L.win_lds_set != lds)
     {
         // dynamic LDS above 64 KiB is granted per kernel
@@ -723,7 +739,8 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
     hipLaunchKernelGGL((k_pair_win<E, V, O>), dim3(W.nblock), dim3(1024), lds, s, n, W,         \
                        L.cell_start, L.sorted_cell, L.row_count, L.row_split, L.nl16, L.max_nb, \
                        ctx->satom,                                                             \
-                       ctx->stype, L.order, ctx->lj.a, ctx->lj.b, box, rc2, beta,              \
+                       ctx->stype, L.order, ctx->lj.ab2, ctx->lj.ntypes, ctx->lj.coul,         \
+                       ctx->lj.coul_scale, box, rc2, beta,                                     \
                        L.stage_base, L.stage, (size_t)n * L.invmax, ctx->fx, ctx->e_sorted,   \
                        ctx->v_sorted, ctx->lj.owner_attribution ? 0 : 1, ctx->e_atom,          \
                        ctx->vir_atom, L.block_perm)

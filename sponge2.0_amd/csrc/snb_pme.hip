@@ -9,13 +9,13 @@
 //           +1 neighbours deposit their 4x4x4 B-spline stencils into an int64
 //           fixed-point LDS tile, which is written once (no global atomics,
 //           result independent of atom order; PME_Q_Spread :762-811);
-//   FFT     LDS-resident line transforms (radix 4/2/3/5/7 Stockham): one
-//           workgroup per x-plane does z R2C + y forward; then x forward +
-//           influence-function product + energy/virial accumulation + x
-//           inverse in one kernel; then per x-plane y inverse + z C2R
-//           (hipFFT R2C/C2R + PME_BCFQ + PME_Sum_Virial, :1100-1122,
-//           :814-823, :1021-1065).  Planes too large for LDS take separate z
-//           and y line passes;
+//   FFT     LDS-resident line transforms (radix 4/2/3/5/7 Stockham): z R2C
+//           and y forward line passes (small LDS, they run beside the
+//           direct-space kernel); x forward + influence-function product +
+//           energy/virial accumulation + x inverse in one kernel; then per
+//           x-plane y inverse + z C2R in one kernel (hipFFT R2C/C2R +
+//           PME_BCFQ + PME_Sum_Virial, :1100-1122, :814-823, :1021-1065).
+//           Planes too large for LDS take separate y and z line passes;
 //   gather  one thread per atom sums the 64 stencil points of the potential
 //           grid in a fixed order (PME_Final :826-890) and adds the force in
 //           fixed point.
@@ -389,10 +389,9 @@ __global__ void __launch_bounds__(256) k_fft_x_conv(
 }
 
 // ---------------------------------------------------------------------------
-// Fused plane passes: one workgroup per x-plane does the z and y transforms
-// in LDS.  Two real z-rows travel as one complex row (row 2p as the real
-// part, row 2p+1 as the imaginary part) and are separated by the conjugate
-// symmetry of real transforms, which halves the z work.
+// Fused inverse plane pass: one workgroup per x-plane does the y and z
+// transforms in LDS.  Two real z-rows travel as one complex row (row 2p as the
+// real part, row 2p+1 as the imaginary part), which halves the z work.
 // LDS: tw_z[fz] | tw_y[fy] | A[cap] | B[cap], cap = max(ceil(fy/2)*fz, fc*fy).
 // ---------------------------------------------------------------------------
 __host__ __device__ static size_t plane_cap(int fy, int fz, int fc)
@@ -404,57 +403,6 @@ __host__ __device__ static size_t plane_cap(int fy, int fz, int fc)
 static size_t plane_lds(int fy, int fz, int fc)
 {
     return sizeof(float2) * ((size_t)fz + fy + 2 * plane_cap(fy, fz, fc));
-}
-
-// real[x][fy][fz] -> spec[x][fy][fc] (z R2C then y forward), exponent -1
-__global__ void __launch_bounds__(1024) k_fft_plane_fwd(const float* __restrict__ real,
-                                                        float2* __restrict__ spec, int fy,
-                                                        int fz, int fc, SnbFftPlan plan_z,
-                                                        SnbFftPlan plan_y,
-                                                        const float2* __restrict__ twzg,
-                                                        const float2* __restrict__ twyg)
-{
-    extern __shared__ float2 smem[];
-    const int cap = (int)plane_cap(fy, fz, fc);
-    float2* twz = smem;
-    float2* twy = twz + fz;
-    float2* A = twy + fy;
-    float2* B = A + cap;
-    load_twiddles(twz, twzg, fz);
-    load_twiddles(twy, twyg, fy);
-    const int x = blockIdx.x;
-    const int np = (fy + 1) / 2;
-    const float* plane = real + (size_t)x * fy * fz;
-    for (int t = threadIdx.x; t < np * fz; t += blockDim.x)
-    {
-        int p = t / fz, z = t - p * fz;
-        float re = plane[(size_t)(2 * p) * fz + z];
-        float im = (2 * p + 1 < fy) ? plane[(size_t)(2 * p + 1) * fz + z] : 0.0f;
-        A[t] = make_float2(re, im);
-    }
-    __syncthreads();
-    float2* res = fft_lines(A, B, plan_z, np, twz, -1.0f);
-    float2* O = (res == A) ? B : A;
-    // separate: Xa[k] = (Z[k] + conj Z[-k]) / 2, Xb[k] = (Z[k] - conj Z[-k]) / (2i);
-    // write as y-lines O[kz][y]
-    for (int t = threadIdx.x; t < np * fc; t += blockDim.x)
-    {
-        int p = t / fc, k = t - p * fc;
-        float2 zk = res[p * fz + k];
-        float2 zm = res[p * fz + (k == 0 ? 0 : fz - k)];
-        float2 xa = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        float2 xb = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
-        O[k * fy + 2 * p] = xa;
-        if (2 * p + 1 < fy) O[k * fy + 2 * p + 1] = xb;
-    }
-    __syncthreads();
-    float2* res2 = fft_lines(O, res, plan_y, fc, twy, -1.0f);
-    float2* out = spec + (size_t)x * fy * fc;
-    for (int t = threadIdx.x; t < fy * fc; t += blockDim.x)
-    {
-        int y = t / fc, k = t - y * fc;
-        out[t] = res2[k * fy + y];
-    }
 }
 
 // spec[x][fy][fc] -> real[x][fy][fz] (y inverse then z C2R), exponent +1
@@ -580,29 +528,37 @@ struct SnbRc
 };
 
 // Per atom: base grid point, spline fractions and brick (PME_Atom_Near), and
-// its slot in the brick's list.  With LDS_HIST the workgroup first counts its
-// atoms per brick in LDS and takes one global counter add per brick it
-// touches (a single round trip for the whole workgroup); otherwise one add
-// per distinct brick of each wavefront.
+// its slot in the brick's list.  Atoms are visited in the brick order of the
+// previous call (any permutation of 0..n-1 is correct; that one keeps each
+// workgroup's atoms in few bricks, so the brick counters see few global
+// atomics).  With LDS_HIST the workgroup counts its atoms per brick in LDS,
+// lists the bricks it touched, and takes one global counter add per listed
+// brick (a single round trip); otherwise one add per distinct brick of each
+// wavefront.
 #define SNB_PREP_HIST_MAX 8192
 template <bool LDS_HIST>
 __global__ void __launch_bounds__(256) k_pme_prep(int n, const snb_vector* __restrict__ crd,
                                                   const float* __restrict__ charge, SnbRc rc,
                                                   int fx, int fy, int fz, int bx, int by, int bz,
                                                   int nby, int nbz, int nbrick,
+                                                  const int* __restrict__ visit,
                                                   int4* __restrict__ atom_u,
                                                   float4* __restrict__ atom_fr,
                                                   int* __restrict__ slot, int* __restrict__ cnt)
 {
     __shared__ int hist[LDS_HIST ? SNB_PREP_HIST_MAX : 1];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int touched[LDS_HIST ? 256 : 1];
+    __shared__ int ntouched;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = v < n ? visit[v] : -1;
     if (LDS_HIST)
     {
         for (int b = threadIdx.x; b < nbrick; b += blockDim.x) hist[b] = 0;
+        if (threadIdx.x == 0) ntouched = 0;
         __syncthreads();
     }
     int br = -1;
-    if (i < n)
+    if (i >= 0)
     {
         const snb_vector ri = crd[i];
         float4 r = make_float4(ri.x, ri.y, ri.z, charge[i]);
@@ -615,18 +571,23 @@ __global__ void __launch_bounds__(256) k_pme_prep(int n, const snb_vector* __res
     }
     if (LDS_HIST)
     {
-        int local = br >= 0 ? atomicAdd(&hist[br], 1) : 0;
-        __syncthreads();
-        for (int b = threadIdx.x; b < nbrick; b += blockDim.x)
+        int local = 0;
+        if (br >= 0)
         {
-            int k = hist[b];
-            if (k > 0) hist[b] = atomicAdd(&cnt[b], k);
+            local = atomicAdd(&hist[br], 1);
+            if (local == 0) touched[atomicAdd(&ntouched, 1)] = br;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < ntouched)
+        {
+            const int b = touched[threadIdx.x];
+            hist[b] = atomicAdd(&cnt[b], hist[b]);
         }
         __syncthreads();
         if (br >= 0) slot[i] = hist[br] + local;
         return;
     }
-    if (i >= n) return;
+    if (i < 0) return;
     const int lane = threadIdx.x & 63;
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     unsigned long long todo = __ballot(1);
@@ -885,7 +846,16 @@ int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box)
     SNB_CHECK(hipMalloc(&P.atom_fr, sizeof(float4) * (n > 0 ? n : 1)));
     SNB_CHECK(hipMalloc(&P.atom_slot, sizeof(int) * (n > 0 ? n : 1)));
     SNB_CHECK(hipMalloc(&P.brick_atoms, sizeof(int) * (n > 0 ? n : 1)));
+    {
+        // the prep kernel visits atoms in the previous call's brick order;
+        // the first call visits them in index order
+        std::vector<int> iota(n > 0 ? n : 1);
+        for (size_t k = 0; k < iota.size(); k++) iota[k] = (int)k;
+        SNB_CHECK(hipMemcpy(P.brick_atoms, iota.data(), sizeof(int) * iota.size(),
+                            hipMemcpyHostToDevice));
+    }
     SNB_CHECK(hipMalloc(&P.brick_count, sizeof(int) * (P.nbrick + 1)));
+    SNB_CHECK(hipMemset(P.brick_count, 0, sizeof(int) * (P.nbrick + 1)));
     SNB_CHECK(hipMalloc(&P.brick_start, sizeof(int) * (P.nbrick + 1)));
     int W = fft_w(P.fc, P.fx);
     P.conv_blocks = P.fy * ((P.fc + W - 1) / W);
@@ -933,8 +903,6 @@ static int plane_ok(snb_context* ctx, bool* ok)
     *ok = lds <= (size_t)ctx->max_lds;
     if (*ok && P.plane_lds_set != lds)
     {
-        SNB_CHECK(hipFuncSetAttribute((const void*)k_fft_plane_fwd,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         SNB_CHECK(hipFuncSetAttribute((const void*)k_fft_plane_inv,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         P.plane_lds_set = lds;
@@ -973,16 +941,8 @@ int snb_launch_fft_r2c(snb_context* ctx, const float* real, float2* cplx)
 {
     SnbPme& P = ctx->pme;
     hipStream_t s = ctx->stream;
-    bool ok;
-    int prc = plane_ok(ctx, &ok);
-    if (prc) return prc;
-    if (ok)
-    {
-        hipLaunchKernelGGL(k_fft_plane_fwd, dim3(P.fx), dim3(1024), plane_lds(P.fy, P.fz, P.fc),
-                           s, real, cplx, P.fy, P.fz, P.fc, P.plan_z, P.plan_y, P.tw_z, P.tw_y);
-        SNB_CHECK(hipGetLastError());
-        return SNB_OK;
-    }
+    // the forward transform runs while the direct-space kernel holds most of
+    // the LDS: line passes with small LDS share compute units with it
     int nrow = P.fx * P.fy;
     int Lz = lines_per_block(P.fz);
     hipLaunchKernelGGL(k_fft_z_r2c, dim3((nrow + Lz - 1) / Lz), dim3(256), lds_lines(P.fz, Lz),
@@ -1065,16 +1025,20 @@ int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* c
     snb_prof_begin(ctx, SNB_PROF_PME);
     if (n > 0)
     {
-        if ((rc = snb_launch_fill_int(s, P.brick_count, P.nbrick + 1, 0))) return rc;
+        // brick counts are zero on entry (cleared at set-up and by the scan
+        // of the previous call)
         if (P.nbrick <= SNB_PREP_HIST_MAX)
             hipLaunchKernelGGL(k_pme_prep<true>, dim3((n + 255) / 256), dim3(256), 0, s, n, crd,
                                charge, r6, P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nby, P.nbz,
-                               P.nbrick, P.atom_u, P.atom_fr, P.atom_slot, P.brick_count);
+                               P.nbrick, P.brick_atoms, P.atom_u, P.atom_fr, P.atom_slot,
+                               P.brick_count);
         else
             hipLaunchKernelGGL(k_pme_prep<false>, dim3((n + 255) / 256), dim3(256), 0, s, n, crd,
                                charge, r6, P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nby, P.nbz,
-                               P.nbrick, P.atom_u, P.atom_fr, P.atom_slot, P.brick_count);
-        if ((rc = snb_launch_exclusive_scan(s, P.brick_count, P.brick_start, P.nbrick))) return rc;
+                               P.nbrick, P.brick_atoms, P.atom_u, P.atom_fr, P.atom_slot,
+                               P.brick_count);
+        if ((rc = snb_launch_exclusive_scan_clear(s, P.brick_count, P.brick_start, P.nbrick)))
+            return rc;
         hipLaunchKernelGGL(k_pme_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, P.atom_u,
                            P.atom_slot, P.brick_start, P.brick_atoms);
     }

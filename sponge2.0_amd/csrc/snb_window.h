@@ -24,6 +24,10 @@
 #endif
 
 #define SNB_WIN_MAX_RANGES 128
+// LDS of the pair kernel besides the per-slot arrays (28 bytes per slot: three
+// int64 accumulators and the slot -> atom map): the two range tables and the
+// 256-piece Coulomb table (16 bytes a piece)
+#define SNB_WIN_FIXED_LDS (4 * (2 * SNB_WIN_MAX_RANGES + 1) + 16 * 256)
 
 struct SnbWin
 {

@@ -304,6 +304,30 @@ def test_twin_math_error_bounds():
         assert abs(L.orc_twin_exp_d(float(yv)) - np.exp(yv)) <= 3e-16 * np.exp(yv)
 
 
+@pytest.mark.parametrize("beta,rc", [(0.3123, 10.0), (0.3472, 8.0), (0.26, 12.0), (0.45, 10.0)])
+def test_twin_coulomb_table_against_erfc(beta, rc):
+    # h(x) = erfc(x) + 2/sqrt(pi) x e^{-x^2} of the pair kernels' Hermite
+    # table against scipy in double, over the whole tabulated range (x a bit
+    # past beta rc included): the float coefficients bound the error near
+    # h ~ 1 (absolute 1e-7); the relative error in the tail stays small too
+    L = oracle.lib()
+    x = np.linspace(0.0, beta * rc * 1.0078, 4001).astype(np.float32)
+    got = np.array([L.orc_twin_coul_h(float(v), beta, rc) for v in x])
+    xd = x.astype(np.float64)
+    ref = sp_erfc(xd) + 2.0 / np.sqrt(np.pi) * xd * np.exp(-xd * xd)
+    assert np.max(np.abs(got - ref)) < 1.5e-7
+    assert np.max(np.abs(got - ref) / ref) < 1e-5
+    # nodes reproduce h itself; a planted sign error in h' (the Hermite
+    # slope) would show as a bias of order dx * h' mid-piece
+    mid = (np.arange(256) + 0.5) * (beta * rc * (1 + 2.0 ** -7)) / 256
+    gm = np.array([L.orc_twin_coul_h(float(v), beta, rc) for v in mid.astype(np.float32)])
+    md = mid.astype(np.float32).astype(np.float64)
+    rm = sp_erfc(md) + 2.0 / np.sqrt(np.pi) * md * np.exp(-md * md)
+    assert np.max(np.abs(gm - rm)) < 1.5e-7
+    # beta = 0 (plain cutoff Coulomb): h = 1 everywhere
+    assert L.orc_twin_coul_h(1.7, 0.0, rc) == 1.0
+
+
 def lib_call(fn, v):
     return float(fn(float(v)))
 
