@@ -630,6 +630,16 @@ __device__ __forceinline__ int stencil_mask(int d, int b, bool single)
     return ((2 << k1) - 1) & ~((1 << k0) - 1);
 }
 
+// The workgroup takes the candidate atoms of the 8 source bricks in chunks of
+// 256 (one per thread).  A candidate whose stencil reaches the brick gets an
+// LDS record: its 12 spline weights (x weights times q 2^32, the fixed-point
+// scale, which is exact) and its offsets and masks.  Then each thread owns a
+// stencil column (atom, kx, ky) of 16 listed atoms at a time and adds its up
+// to 4 z points to the int64 LDS tile: ((q wx) wy) wz per point, the
+// reference's product order, rounded to fixed point.  For an atom at offset
+// d, stencil point u - k lies in the brick when 0 <= d - k < b: only that k
+// range is visited, so every stencil point is added by exactly one brick.
+#define SNB_SPREAD_LIST 256
 __global__ void __launch_bounds__(256) k_pme_spread(
     const int4* __restrict__ atom_u, const float4* __restrict__ atom_fr,
     const int* __restrict__ start, const int* __restrict__ brick_atoms, int fx, int fy, int fz,
@@ -637,77 +647,103 @@ __global__ void __launch_bounds__(256) k_pme_spread(
 {
     extern __shared__ long long acc[];
     const int npt = bx * by * bz;
-    for (int t = threadIdx.x; t < npt; t += blockDim.x) acc[t] = 0;
-    __syncthreads();
+    float* lw = (float*)(acc + npt);                // [SNB_SPREAD_LIST][12]
+    int* lmeta = (int*)(lw + SNB_SPREAD_LIST * 12);  // [SNB_SPREAD_LIST]
+    __shared__ int src_first[8], src_pref[9], nlist;
+    const int t = threadIdx.x;
+    for (int p = t; p < npt; p += blockDim.x) acc[p] = 0;
     const int B = blockIdx.x;
     const int Bz = B % nbz, By = (B / nbz) % nby, Bx = B / (nbz * nby);
     const int ox = Bx * bx, oy = By * by, oz = Bz * bz;
-    // Atoms of this brick and of its +1 neighbours (their stencils reach down
-    // by 3 points).  For an atom at base point u, stencil point u - k lies in
-    // the brick when 0 <= d - k < b with d = (u - o) mod n: only that k range
-    // is visited, so every stencil point is handled by exactly one brick.
-    for (int s = 0; s < 8; s++)
+    if (t < 8)
     {
-        int sx = s >> 2, sy = (s >> 1) & 1, sz = s & 1;
-        if ((sx && nbx == 1) || (sy && nby == 1) || (sz && nbz == 1)) continue;
-        int sb = (((Bx + sx) % nbx) * nby + (By + sy) % nby) * nbz + (Bz + sz) % nbz;
-        int a0 = start[sb], a1 = start[sb + 1];
-        for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x)
+        const int sx = t >> 2, sy = (t >> 1) & 1, sz = t & 1;
+        int f = 0, c = 0;
+        if (!((sx && nbx == 1) || (sy && nby == 1) || (sz && nbz == 1)))
         {
-            int i = brick_atoms[a];
-            int4 u = atom_u[i];
+            int sb = (((Bx + sx) % nbx) * nby + (By + sy) % nby) * nbz + (Bz + sz) % nbz;
+            f = start[sb];
+            c = start[sb + 1] - f;
+        }
+        src_first[t] = f;
+        src_pref[t + 1] = c;
+    }
+    __syncthreads();
+    if (t == 0)
+    {
+        src_pref[0] = 0;
+        for (int k = 1; k <= 8; k++) src_pref[k] += src_pref[k - 1];
+        nlist = 0;
+    }
+    __syncthreads();
+    const int ncand = src_pref[8];
+    const int kx = (t >> 2) & 3, ky = t & 3;
+    for (int c0 = 0; c0 < ncand; c0 += blockDim.x)
+    {
+        const int ci = c0 + t;
+        if (ci < ncand)
+        {
+            int sidx = 0;
+            while (sidx < 7 && src_pref[sidx + 1] <= ci) sidx++;
+            const int i = brick_atoms[src_first[sidx] + (ci - src_pref[sidx])];
+            const int4 u = atom_u[i];
             int dx = u.x - ox, dy = u.y - oy, dz = u.z - oz;
             dx += dx < 0 ? fx : 0;
             dy += dy < 0 ? fy : 0;
             dz += dz < 0 ? fz : 0;
-            // valid stencil offsets per axis as bit masks (an axis with a
-            // single brick wraps inside it: all four)
             const int mx = stencil_mask(dx, bx, nbx == 1);
             const int my = stencil_mask(dy, by, nby == 1);
             const int mz = stencil_mask(dz, bz, nbz == 1);
-            if (mx == 0 || my == 0 || mz == 0) continue;
-            float4 fr = atom_fr[i];
-            float wx[4], wy[4], wz[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++)
+            if (mx != 0 && my != 0 && mz != 0)
             {
-                wx[k] = snb_bspline_w(k, fr.x);
-                wy[k] = snb_bspline_w(k, fr.y);
-                wz[k] = snb_bspline_w(k, fr.z);
-            }
+                // dx < bx + 3 <= 19 (or < fx <= 16 on a one-brick axis): 5 bits
+                const float4 fr = atom_fr[i];
+                const int slot = atomicAdd(&nlist, 1);
+                const float q2 = fr.w * 4294967296.0f;
+                float* w = lw + slot * 12;
 #pragma unroll
-            for (int kx = 0; kx < 4; kx++)
-            {
-                if (!((mx >> kx) & 1)) continue;
-                int lx = dx - kx;
-                lx += lx < 0 ? fx : 0;
-                float qx = fr.w * wx[kx];
-#pragma unroll
-                for (int ky = 0; ky < 4; ky++)
+                for (int k = 0; k < 4; k++)
                 {
-                    if (!((my >> ky) & 1)) continue;
-                    int ly = dy - ky;
-                    ly += ly < 0 ? fy : 0;
-                    float qxy = qx * wy[ky];
-                    int row = (lx * by + ly) * bz;
-#pragma unroll
-                    for (int kz = 0; kz < 4; kz++)
-                    {
-                        if (!((mz >> kz) & 1)) continue;
-                        int lz = dz - kz;
-                        lz += lz < 0 ? fz : 0;
-                        atomicAdd((unsigned long long*)&acc[row + lz],
-                                  (unsigned long long)snb_to_fixed(qxy * wz[kz]));
-                    }
+                    w[k] = q2 * snb_bspline_w(k, fr.x);
+                    w[4 + k] = snb_bspline_w(k, fr.y);
+                    w[8 + k] = snb_bspline_w(k, fr.z);
                 }
+                lmeta[slot] = dx | (dy << 5) | (dz << 10) | (mx << 15) | (my << 19) | (mz << 23);
             }
         }
+        __syncthreads();
+        const int nl = nlist;
+        for (int a = t >> 4; a < nl; a += 16)
+        {
+            const int meta = lmeta[a];
+            const int mx = (meta >> 15) & 15, my = (meta >> 19) & 15;
+            if (!((mx >> kx) & 1) || !((my >> ky) & 1)) continue;
+            const int dx = meta & 31, dy = (meta >> 5) & 31, dz = (meta >> 10) & 31;
+            const int mz = (meta >> 23) & 15;
+            const float* w = lw + a * 12;
+            const float qxy = w[kx] * w[4 + ky];
+            int lx = dx - kx, ly = dy - ky;
+            lx += lx < 0 ? fx : 0;
+            ly += ly < 0 ? fy : 0;
+            const int row = (lx * by + ly) * bz;
+#pragma unroll
+            for (int kz = 0; kz < 4; kz++)
+            {
+                if (!((mz >> kz) & 1)) continue;
+                int lz = dz - kz;
+                lz += lz < 0 ? fz : 0;
+                atomicAdd((unsigned long long*)&acc[row + lz],
+                          (unsigned long long)snb_fixed_from_scaled(qxy * w[8 + kz]));
+            }
+        }
+        __syncthreads();
+        if (t == 0) nlist = 0;
+        __syncthreads();
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < npt; t += blockDim.x)
+    for (int p = t; p < npt; p += blockDim.x)
     {
-        int lz = t % bz, ly = (t / bz) % by, lx = t / (bz * by);
-        q[((size_t)(ox + lx) * fy + (oy + ly)) * fz + (oz + lz)] = snb_fixed_to_float(acc[t]);
+        int lz = p % bz, ly = (p / bz) % by, lx = p / (bz * by);
+        q[((size_t)(ox + lx) * fy + (oy + ly)) * fz + (oz + lz)] = snb_fixed_to_float(acc[p]);
     }
 }
 
@@ -1046,7 +1082,7 @@ int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* c
     {
         if ((rc = snb_launch_fill_int(s, P.brick_start, P.nbrick + 1, 0))) return rc;
     }
-    size_t lds = sizeof(long long) * (size_t)P.bx * P.by * P.bz;
+    size_t lds = sizeof(long long) * (size_t)P.bx * P.by * P.bz + 13 * sizeof(float) * SNB_SPREAD_LIST;
     hipLaunchKernelGGL(k_pme_spread, dim3(P.nbrick), dim3(256), lds, s, P.atom_u, P.atom_fr,
                        P.brick_start, P.brick_atoms, P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nbx,
                        P.nby, P.nbz, P.q);
