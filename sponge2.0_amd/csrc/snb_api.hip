@@ -233,31 +233,8 @@ int prof_collect(snb_context* c, int g)
 
 // Puts a deferred force-only exclusion correction onto pme_stream (after
 // the step's inputs are final), accumulating into fx_pme.
-static int launch_excl_later(snb_context* c)
-{
-    if (!c->excl_later.pending) return SNB_OK;
-    c->excl_later.pending = false;
-    hipStream_t home = c->stream;
-    if (!c->pme_pending)
-    {
-        if (!c->inputs_marked) SNB_CHECK(hipEventRecord(c->ev_inputs, home));
-        SNB_CHECK(hipStreamWaitEvent(c->pme_stream, c->ev_inputs, 0));
-    }
-    c->stream = c->pme_stream;
-    const auto& X = c->excl_later;
-    int rc = snb_launch_excl(c, X.crd, X.charge, X.box, X.beta, X.ex_start, X.ex_list, X.ex_num,
-                             0, 0, c->fx_pme);
-    c->stream = home;
-    if (rc) return rc;
-    SNB_CHECK(hipEventRecord(c->ev_pme, c->pme_stream));
-    c->pme_pending = true;
-    return SNB_OK;
-}
-
 int snb_join_pme(snb_context* c)
 {
-    int rc = launch_excl_later(c);
-    if (rc) return rc;
     if (!c->pme_pending) return SNB_OK;
     SNB_CHECK(hipStreamWaitEvent(c->stream, c->ev_pme, 0));
     c->pme_pending = false;
@@ -935,23 +912,6 @@ int snb_pme_excluded_force(snb_context* c, int atom_numbers, const int* d_atom_l
     DeviceGuard g(c->device);
     SnbBox box = snb_box(cell, rcell);
     int rc;
-    if (c->overlap && !need_energy && !need_virial && !d_frc)
-    {
-        // force-only: the correction runs on the reciprocal stream, into the
-        // reciprocal accumulator, behind the step's PME pipeline (it is only
-        // needed at the flush); a join launches it if no PME call comes
-        if ((rc = launch_excl_later(c))) return rc;  // an earlier one still waiting
-        auto& X = c->excl_later;
-        X.pending = true;
-        X.crd = d_crd;
-        X.charge = d_charge;
-        X.ex_start = es;
-        X.ex_list = el;
-        X.ex_num = en;
-        X.box = box;
-        X.beta = c->pme.beta;
-        return SNB_OK;
-    }
     if (need_energy && (rc = snb_launch_fill_i64(c->stream, c->e_atom + 2 * (size_t)c->n, c->n)))
         return rc;
     rc = snb_launch_excl(c, d_crd, d_charge, box, c->pme.beta, es, el, en, need_energy,
@@ -1014,7 +974,6 @@ int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatr
             c->stream = home;
             if (!rc && hipEventRecord(c->ev_pme, c->pme_stream) != hipSuccess) rc = SNB_ERR_HIP;
             if (!rc) c->pme_pending = true;
-            if (!rc) rc = launch_excl_later(c);  // behind the pipeline, same stream
         }
         if (rc) return rc;
         P.have_energy = need_energy != 0;

@@ -192,19 +192,6 @@ struct snb_context
     hipEvent_t ev_pme = nullptr;
     bool inputs_marked = false;
     bool pme_pending = false;
-    // a force-only exclusion correction waiting to go onto pme_stream behind
-    // the step's PME pipeline (or at the next join, when no PME call comes)
-    struct
-    {
-        bool pending = false;
-        const snb_vector* crd = nullptr;
-        const float* charge = nullptr;
-        const int* ex_start = nullptr;
-        const int* ex_list = nullptr;
-        const int* ex_num = nullptr;
-        SnbBox box;
-        float beta = 0.0f;
-    } excl_later;
     SnbNl nl;
     SnbLj lj;
     SnbPme pme;

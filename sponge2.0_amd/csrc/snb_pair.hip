@@ -555,34 +555,35 @@ __global__ void __launch_bounds__(1024) k_pair_win(
             const unsigned trow = (unsigned)(stype[si] * ntypes);
             const int id_i = (NEED_ENERGY || NEED_VIRIAL) && ref_attr ? order[si] : 0;
             WinRowSums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            // list entries (window slots) of the first 128 entries; inside the
-            // loop the next 128 are fetched while the current ones are used.
-            // 32-bit byte offsets: the loads take the scalar-base + vector-
-            // offset form, no 64-bit address arithmetic per lane
-            int nxa = lane < cnt ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(lane, split, max_nb))
-                                 : -1;
-            int nxb = lane + 64 < cnt
-                          ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(lane + 64, split, max_nb))
-                          : -1;
+            // Software pipeline over 128-entry steps (two entries per lane):
+            // step k uses atoms whose records were fetched during step k - 1,
+            // fetches the records of step k + 1 (slot map, then position and
+            // type) and the list entries of step k + 2.  32-bit byte offsets:
+            // the loads take the scalar-base + vector-offset form.
+            auto entry = [&](int k) -> int {
+                return k < cnt ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(k, split, max_nb))
+                               : -1;
+            };
+            auto sorted_of = [&](int sl) -> unsigned {
+                return sl >= 0 ? (unsigned)win_slot_sorted(sl, ov, slot_s, rfirst, rpref, W.nrange)
+                               : (unsigned)si;
+            };
+            int sla = entry(lane), slb = entry(lane + 64);
+            unsigned sja = sorted_of(sla), sjb = sorted_of(slb);
+            float4 aja = ld32(satom, sja), ajb = ld32(satom, sjb);
+            int tja = ld32(stype, sja), tjb = ld32(stype, sjb);
+            int nxa = entry(lane + 128), nxb = entry(lane + 192);
             for (int k0 = 0; k0 < cnt; k0 += 128)
             {
-                // two entries per lane: issue both gathers before either pair
-                const int sla = nxa, slb = nxb;
-                const int ka2 = k0 + 128 + lane, kb2 = ka2 + 64;
-                nxa = ka2 < cnt ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(ka2, split, max_nb))
-                                : -1;
-                nxb = kb2 < cnt ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(kb2, split, max_nb))
-                                : -1;
-                const unsigned sja =
-                    sla >= 0 ? (unsigned)win_slot_sorted(sla, ov, slot_s, rfirst, rpref, W.nrange)
-                             : (unsigned)si;
-                const unsigned sjb =
-                    slb >= 0 ? (unsigned)win_slot_sorted(slb, ov, slot_s, rfirst, rpref, W.nrange)
-                             : (unsigned)si;
-                const float4 aja = ld32(satom, sja), ajb = ld32(satom, sjb);
-                const int tja = ld32(stype, sja), tjb = ld32(stype, sjb);
                 const float2 ABa = ld32(ab2, trow + (unsigned)tja);
                 const float2 ABb = ld32(ab2, trow + (unsigned)tjb);
+                // records of the next step
+                const int sla_n = nxa, slb_n = nxb;
+                const unsigned sja_n = sorted_of(sla_n), sjb_n = sorted_of(slb_n);
+                const float4 aja_n = ld32(satom, sja_n), ajb_n = ld32(satom, sjb_n);
+                const int tja_n = ld32(stype, sja_n), tjb_n = ld32(stype, sjb_n);
+                nxa = entry(k0 + 256 + lane);
+                nxb = entry(k0 + 320 + lane);
                 float dxa = aja.x - ai.x, dya = aja.y - ai.y, dza = aja.z - ai.z;
                 float dxb = ajb.x - ai.x, dyb = ajb.y - ai.y, dzb = ajb.z - ai.z;
                 win_disp<ORTHO>(dxa, dya, dza, box);
@@ -593,8 +594,7 @@ __global__ void __launch_bounds__(1024) k_pair_win(
                 // compiler would otherwise sink them into the branches, where
                 // their latency is not hidden behind the distance test); an
                 // empty statement that reads the registers, nothing is emitted
-                asm volatile("" ::"v"(ABa.x), "v"(ABa.y), "v"(ABb.x), "v"(ABb.y), "v"(aja.w),
-                             "v"(ajb.w));
+                asm volatile("" ::"v"(ABa.x), "v"(ABa.y), "v"(ABb.x), "v"(ABb.y));
                 if (sla >= 0 && r2a < rc2)
                     win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxa, dya, dza, r2a, ABa.x, ABa.y, qi2,
                                                        aja.w, beta, cscale, ctab, sla, sja, ov,
@@ -605,6 +605,14 @@ __global__ void __launch_bounds__(1024) k_pair_win(
                                                        ajb.w, beta, cscale, ctab, slb, sjb, ov,
                                                        W.wcap, acc, n, order, fx, ref_attr, id_i,
                                                        e_atom, vir);
+                sla = sla_n;
+                slb = slb_n;
+                sja = sja_n;
+                sjb = sjb_n;
+                aja = aja_n;
+                ajb = ajb_n;
+                tja = tja_n;
+                tjb = tjb_n;
             }
             int64_t sx = wave_sum(S.sx), sy = wave_sum(S.sy), sz = wave_sum(S.sz);
             int64_t elj = 0, ec = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;

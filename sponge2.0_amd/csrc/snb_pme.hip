@@ -749,14 +749,18 @@ __global__ void __launch_bounds__(256) k_pme_spread(
 
 // PME_Final + device_add_force (PM_force.cpp:826-890, :1011-1019): one thread
 // per atom, stencil order kx, ky, kz (outer to inner), s = fma(phi, g, s).
-__global__ void __launch_bounds__(256) k_pme_gather(int n, const int4* __restrict__ atom_u,
+// Atoms are visited in brick order (the lists of this call), so the lanes of a
+// wavefront read neighbouring grid points.
+__global__ void __launch_bounds__(256) k_pme_gather(int n, const int* __restrict__ visit,
+                                                    const int4* __restrict__ atom_u,
                                                     const float4* __restrict__ atom_fr,
                                                     const float* __restrict__ phi, int fx,
                                                     int fy, int fz, SnbRc rc, float upd,
                                                     int64_t* __restrict__ frc)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int i = visit[v];
     int4 u = atom_u[i];
     float4 fr = atom_fr[i];
     float wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
@@ -1107,7 +1111,8 @@ int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* c
     SNB_CHECK(hipGetLastError());
     if ((rc = fft_yz_inverse(ctx, P.spec, P.phi))) return rc;
     if (n > 0 && with_force)
-        hipLaunchKernelGGL(k_pme_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, P.atom_u,
+        hipLaunchKernelGGL(k_pme_gather, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                           P.brick_atoms, P.atom_u,
                            P.atom_fr, P.phi, P.fx, P.fy, P.fz, r6, (float)P.update_interval,
                            ctx->fx_pme);
     snb_prof_end(ctx, SNB_PROF_PME);
