@@ -102,6 +102,25 @@ def accuracy(sysd, nb, counts, rows, pme, gpu_frc):
     return float(np.sqrt(((g - ref) ** 2).sum() / (ref ** 2).sum()))
 
 
+def pairs_in_cutoff(sysd, counts, rows, cutoff):
+    """Number of list pairs closer than the cutoff (numpy over the GPU-built
+    list, minimum image in the diagonal box of this workload)."""
+    import numpy as np
+    crd = sysd["crd"].astype(np.float64)
+    L = np.array([sysd["cell"][0], sysd["cell"][2], sysd["cell"][5]], dtype=np.float64)
+    total = 0
+    for i0 in range(0, len(counts), 8192):
+        c = counts[i0:i0 + 8192]
+        blk = rows[i0:i0 + 8192]
+        mask = np.arange(blk.shape[1])[None, :] < c[:, None]
+        ii = np.nonzero(mask)
+        j = blk[ii]
+        d = crd[j] - crd[i0 + ii[0]]
+        d -= np.rint(d / L) * L
+        total += int(np.count_nonzero((d * d).sum(axis=1) < cutoff * cutoff))
+    return total
+
+
 def pair_traffic(kernel):
     """HBM bytes per launch of the pair kernel, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes over this same workload (scaled by what the
@@ -219,6 +238,7 @@ def main():
     if args.cpu_seconds > 0:
         base, pme = cpu_baseline(sysd, nb, counts, rows, args.cpu_seconds)
     err = None if args.no_accuracy else accuracy(sysd, nb, counts, rows, pme, gpu_frc)
+    npairs = None if counts is None else pairs_in_cutoff(sysd, counts, rows, nb.cutoff)
     per = {k: (v[0] / max(v[1], 1)) for k, v in prof.items()}
     pair_kernel = "k_pair_win" if nb.window_info()["windowed"] else "k_lj_direct"
     out = {
@@ -244,7 +264,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": pair_traffic(pair_kernel),
                      "kernel": pair_kernel, "alg_bytes_per_launch": alg_bytes,
                      "avg_launch_us": pair_avg_s * 1e6, "launches": pair_n,
-                     "pair_interactions_per_s": None},
+                     "pairs_in_cutoff": npairs,
+                     "pair_interactions_per_s": None if npairs is None else npairs / pair_avg_s},
         "kernel_avg_us": {k: v * 1000.0 for k, v in per.items()},
         "pair_window": nb.window_info(),
         "launch": "hip-graph replay per step kind" if graphs else "eager",

@@ -173,7 +173,10 @@ int snb_lj_initial(snb_context* ctx, int atom_type_numbers, const float* h_LJ_A,
  * atom and d_atom_direct_pme_energy = Coulomb per atom; with need_virial:
  * d_atom_virial += per-atom virial (reference convention: -f (x) dr, with the
  * off-diagonals symmetrised, vector.hpp:336-347).  Any output may be NULL
- * (then kept in the fixed-point accumulators). */
+ * (then kept in the fixed-point accumulators).  SNB_ERR_ARGUMENT for
+ * pme_beta < 0 or pme_beta * cutoff > 4.9 (an Ewald tolerance below 1e-11);
+ * SNB_ERR_UNSUPPORTED when pme_beta changes while the stream is being
+ * captured into a graph (the Coulomb force table is rebuilt on a change). */
 int snb_lj_pme_direct_force(snb_context* ctx, int atom_numbers, int local_atom_numbers,
                             int solvent_numbers, int ghost_numbers,
                             const snb_vector* d_crd, const float* d_charge,

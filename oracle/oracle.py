@@ -72,7 +72,7 @@ def _declare(L):
     L.orc_twin_erf_parts.argtypes = [_d, ctypes.POINTER(_d), ctypes.POINTER(_d)]
     L.orc_twin_exp_d.restype = _d
     L.orc_twin_exp_d.argtypes = [_d]
-    for name in ("orc_twin_expf", "orc_twin_erfc", "orc_twin_rsqrtf"):
+    for name in ("orc_twin_rsqrtf",):
         getattr(L, name).restype = _f
         getattr(L, name).argtypes = [_f]
     L.orc_twin_coul_h.restype = _f

@@ -48,7 +48,6 @@
 #define ORC_PI_D 3.14159265358979323846
 /* PM_force.cpp:15-17 and Lennard_Jones_force.h:7 */
 #define ORC_PI_F 3.1415926f
-#define ORC_TWO_OVER_SQRT_PI_F 1.1283791670218446f
 #define ORC_FIXED_SCALE 4294967296.0 /* 2^32 */
 
 /* ------------------------------------------------------------------------ */
@@ -80,44 +79,6 @@ static inline int64_t to_fixed(float f)
 /* ------------------------------------------------------------------------ */
 /* FP32 math written from published methods (twin of the kernels' math)     */
 /* ------------------------------------------------------------------------ */
-/* exp(y) for y in [-87, 0]: Cody-Waite reduction y = n ln2 + r with ln2 split
- * into a 16-bit head and a tail, then a degree-6 polynomial for e^r fitted
- * (weighted least squares, Lawson iteration) on [-ln2/2, ln2/2], then 2^n
- * built directly in the exponent field. */
-static inline float twin_expf(float y)
-{
-    y = fmaxf(y, -87.0f);
-    float n = rintf(y * 1.44269504088896341f);
-    float r = fmaf(-n, 0.693145751953125f, y);
-    r = fmaf(-n, 1.428606765330187045e-06f, r);
-    float p = 0.0013836846f;
-    p = fmaf(p, r, 0.008374816f);
-    p = fmaf(p, r, 0.041668225f);
-    p = fmaf(p, r, 0.1666642f);
-    p = fmaf(p, r, 0.4999999f);
-    p = fmaf(p, r, 1.0f);
-    p = fmaf(p, r, 1.0f);
-    int32_t ni = (int32_t)n;
-    return p * f_from_bits((uint32_t)(ni + 127) << 23);
-}
-
-/* erfc(x) = t exp(-x^2) P(t), t = 1/(1 + 0.3275911 x) (the substitution of
- * Abramowitz & Stegun 7.1.26) with a degree-7 P fitted on x in [0, 5] for
- * minimum relative error (max 4.5e-8 in exact arithmetic). ex = exp(-x^2). */
-static inline float twin_erfc_core(float x, float ex)
-{
-    float t = 1.0f / fmaf(0.3275911f, x, 1.0f);
-    float q = -0.1413204f;
-    q = fmaf(q, t, 0.49561852f);
-    q = fmaf(q, t, -0.39975524f);
-    q = fmaf(q, t, 0.44215718f);
-    q = fmaf(q, t, 0.0353266f);
-    q = fmaf(q, t, 0.20112583f);
-    q = fmaf(q, t, 0.18191357f);
-    q = fmaf(q, t, 0.18493392f);
-    return (t * q) * ex;
-}
-
 /* 1/sqrt(x) for normal x > 0: integer seed 0x5f375a86 - (bits >> 1)
  * (Lomont, "Fast inverse square root", 2003) refined by three Newton steps
  * y <- y (1.5 - (x/2) y^2); only IEEE mul/fma, so it is reproducible. */
@@ -1207,8 +1168,6 @@ ORC_EXPORT double orc_lj_c6_total(int n, const int* type, int ntypes,
 }
 
 /* exposes the twin's scalar math for unit tests */
-ORC_EXPORT float orc_twin_expf(float y) { return twin_expf(y); }
-ORC_EXPORT float orc_twin_erfc(float x) { return twin_erfc_core(x, twin_expf(-(x * x))); }
 ORC_EXPORT void orc_twin_erf_parts(double x, double* erf_out, double* h_out)
 {
     twin_erf_parts_d(x, erf_out, h_out);

@@ -19,7 +19,6 @@
 
 #define SNB_FIXED_SCALE 4294967296.0          // 2^32
 #define SNB_FIXED_INV 2.3283064365386963e-10  // 2^-32
-#define SNB_TWO_OVER_SQRT_PI 1.1283791670218446f  // Lennard_Jones_force.h:7
 
 SNB_HD float snb_f_from_bits(uint32_t u) { return __builtin_bit_cast(float, u); }
 SNB_HD uint32_t snb_bits_from_f(float f) { return __builtin_bit_cast(uint32_t, f); }
@@ -34,44 +33,6 @@ SNB_HD int64_t snb_to_fixed(float f)
 
 SNB_HD float snb_fixed_to_float(int64_t v) { return (float)((double)v * SNB_FIXED_INV); }
 SNB_HD double snb_fixed_to_double(int64_t v) { return (double)v * SNB_FIXED_INV; }
-
-// exp(y), y in [-87, 0]: Cody-Waite reduction y = n ln2 + r (ln2 split into
-// a 16-bit head and a tail so n*head is exact), degree-6 polynomial for e^r
-// fitted by weighted least squares (Lawson iteration) on [-ln2/2, ln2/2],
-// then 2^n assembled in the exponent field.
-SNB_HD float snb_expf(float y)
-{
-    y = __builtin_fmaxf(y, -87.0f);
-    float n = __builtin_rintf(y * 1.44269504088896341f);
-    float r = __builtin_fmaf(-n, 0.693145751953125f, y);
-    r = __builtin_fmaf(-n, 1.428606765330187045e-06f, r);
-    float p = 0.0013836846f;
-    p = __builtin_fmaf(p, r, 0.008374816f);
-    p = __builtin_fmaf(p, r, 0.041668225f);
-    p = __builtin_fmaf(p, r, 0.1666642f);
-    p = __builtin_fmaf(p, r, 0.4999999f);
-    p = __builtin_fmaf(p, r, 1.0f);
-    p = __builtin_fmaf(p, r, 1.0f);
-    int32_t ni = (int32_t)n;
-    return p * snb_f_from_bits((uint32_t)(ni + 127) << 23);
-}
-
-// erfc(x), x >= 0, given ex = exp(-x^2): t = 1/(1 + 0.3275911 x) (the
-// substitution of Abramowitz & Stegun 7.1.26) and a degree-7 polynomial in t
-// fitted on [0, 5] for minimum relative error (4.5e-8 in exact arithmetic).
-SNB_HD float snb_erfc_core(float x, float ex)
-{
-    float t = 1.0f / __builtin_fmaf(0.3275911f, x, 1.0f);
-    float q = -0.1413204f;
-    q = __builtin_fmaf(q, t, 0.49561852f);
-    q = __builtin_fmaf(q, t, -0.39975524f);
-    q = __builtin_fmaf(q, t, 0.44215718f);
-    q = __builtin_fmaf(q, t, 0.0353266f);
-    q = __builtin_fmaf(q, t, 0.20112583f);
-    q = __builtin_fmaf(q, t, 0.18191357f);
-    q = __builtin_fmaf(q, t, 0.18493392f);
-    return (t * q) * ex;
-}
 
 // 1/sqrt(x), x > 0 normal: seed 0x5f375a86 - (bits >> 1) (Lomont, "Fast
 // inverse square root", 2003; relative error below 3.5e-2), three Newton

@@ -275,15 +275,6 @@ def test_nl_overflow_truncates(small_water):
 
 # ---------------------------------------------------- twin vs FP64 layer
 def test_twin_math_error_bounds():
-    # erfc on beta*r <= 3.5 (beta*rc = 2.75 at rc 10 A, tol 1e-5); the float
-    # rounding of the argument -x*x adds ~x^2 * 2^-24 relative error
-    x = np.linspace(0, 3.5, 7001).astype(np.float32)
-    got = np.array([lib_call(oracle.lib().orc_twin_erfc, v) for v in x[::10]])
-    ref = sp_erfc(x[::10].astype(np.float64))
-    assert np.max(np.abs(got - ref) / ref) < 1e-6
-    y = -np.linspace(0, 30, 301).astype(np.float32)
-    ge = np.array([lib_call(oracle.lib().orc_twin_expf, v) for v in y])
-    assert np.max(np.abs(ge - np.exp(y.astype(np.float64))) / np.exp(y.astype(np.float64))) < 3e-7
     r2 = np.geomspace(0.5, 200.0, 400).astype(np.float32)
     gr = np.array([lib_call(oracle.lib().orc_twin_rsqrtf, v) for v in r2])
     assert np.max(np.abs(gr * np.sqrt(r2.astype(np.float64)) - 1)) < 3e-7
