@@ -1,10 +1,11 @@
 /*
  * sponge_nb.h -- C ABI of the gfx950 nonbonded-force library (libsponge_nb.so).
  *
- * The library replaces three modules of SPONGE 2.0 on one GPU:
+ * The library replaces these modules of SPONGE 2.0 on one GPU:
  *   NEIGHBOR_LIST              SPONGE/neighbor_list/neighbor_list.{h,cpp}
  *   LENNARD_JONES_INFORMATION  SPONGE/Lennard_Jones_force/Lennard_Jones_force.{h,cpp}
  *   Particle_Mesh (PME)        SPONGE/PM_force/PM_force.{h,cpp}
+ *   NON_BOND_14                SPONGE/nb14/nb14.{h,cpp}
  * Each entry point below names the reference method it stands in for; the
  * argument lists keep the reference's order and meaning, with the module
  * object replaced by an snb_context.
@@ -196,6 +197,34 @@ int snb_lj_long_range_correction(snb_context* ctx, int need_pressure, snb_ltmatr
  * int64 sum), LJ_long = f/V.  Synchronises. */
 int snb_lj_energy(snb_context* ctx, double* lj_short, double* lj_long);
 
+/* -------------------------------------------------------------------- 1-4 */
+/* NON_BOND_14::Initial (nb14.cpp:103-185) with the pair list resolved by the
+ * caller the way Xponge::Native_Load_NB14 does (xponge/load/native/nb14.hpp:8-95):
+ * per pair the atoms a and b, A in the x12 form and B in the x6 form (the LJ
+ * scale already applied), and the Coulomb scale.  nb14_numbers = 0 leaves the
+ * module uninitialised and its force call a no-op, as in the reference. */
+int snb_nb14_initial(snb_context* ctx, int nb14_numbers, const int* h_atom_a,
+                     const int* h_atom_b, const float* h_A, const float* h_B,
+                     const float* h_cf_scale_factor);
+
+/* NON_BOND_14::Non_Bond_14_LJ_CF_Force_With_Atom_Energy_And_Virial
+ * (nb14.cpp:284-301, kernel :7-100): for each pair, with dr = pbc(r_b - r_a),
+ * f = -A r^-14 + B r^-8 - cf q_a q_b r^-3 (charges below 1e-10 in magnitude
+ * raised to 1e-10); d_frc[a] += f dr, d_frc[b] -= f dr; with
+ * need_atom_energy: d_atom_energy[a] += cf q_a q_b / r + A/12 r^-12 - B/6 r^-6
+ * (the module's per-atom energy restarts each energy call); with need_virial:
+ * d_atom_virial[a] += -f dr (x) dr.  NULL outputs stay in the fixed-point
+ * accumulators. */
+int snb_nb14_force(snb_context* ctx, const snb_vector* d_crd, const float* d_charge,
+                   snb_ltmatrix3 cell, snb_ltmatrix3 rcell, snb_vector* d_frc,
+                   int need_atom_energy, float* d_atom_energy, int need_virial,
+                   snb_ltmatrix3* d_atom_virial);
+
+/* NON_BOND_14::Step_Print (nb14.cpp:303-324): the sums of the 1-4 LJ
+ * (nb14_LJ) and Coulomb (nb14_EE) pair energies of the last energy call, as
+ * exact int64 sums.  Synchronises. */
+int snb_nb14_energy(snb_context* ctx, double* lj14, double* ee14);
+
 /* -------------------------------------------------------------------- PME */
 /* Particle_Mesh::Initial (PM_force.cpp:293-652) with the controller keys
  * resolved by the caller: PM_Direct_Tolerance (1e-5), PM_fftx/y/z (<= 0 =
@@ -239,7 +268,7 @@ int snb_pme_reciprocal_virial(snb_context* ctx, double* virial6);
 /* d_frc[i] += accumulated force, then clears the accumulator. */
 int snb_flush_forces(snb_context* ctx, snb_vector* d_frc);
 /* For outputs deferred with NULL: d_atom_energy[i] += the per-atom energies of
- * the last energy calls (LJ + direct Coulomb + exclusion correction) and
+ * the last energy calls (LJ + direct Coulomb + exclusion correction + 1-4) and
  * d_atom_direct_pme_energy[i] = the direct Coulomb part; either may be NULL. */
 int snb_flush_atom_energy(snb_context* ctx, float* d_atom_energy, float* d_atom_direct_pme_energy);
 /* d_atom_virial[i] += the accumulated per-atom virial, then clears it. */
@@ -248,7 +277,8 @@ int snb_flush_atom_virial(snb_context* ctx, snb_ltmatrix3* d_atom_virial);
 int snb_copy_fixed_forces(snb_context* ctx, int64_t* d_out);
 int snb_clear_fixed_forces(snb_context* ctx);
 /* Copies the int64 per-atom accumulators: which = 0 LJ energy, 1 direct
- * Coulomb energy, 2 exclusion-correction energy (n each), 3 virial (6 x n, SoA). */
+ * Coulomb energy, 2 exclusion-correction energy, 4 1-4 energy (n each),
+ * 3 virial (6 x n, SoA). */
 int snb_copy_fixed_atom_terms(snb_context* ctx, int which, int64_t* d_out);
 /* Copies the PME real-space grids of the last reciprocal call (float,
  * [fftx][ffty][fftz]): which = 0 charge grid Q, 1 potential grid (C2R output). */

@@ -61,6 +61,8 @@ def _declare(L):
     L.orc_pair_twin.argtypes = L.orc_pair_f64.argtypes
     L.orc_excl_f64.argtypes = [_i, _p, _p, _p, _p, _f, _p, _p, _p, _p, _p, _p]
     L.orc_excl_twin.argtypes = L.orc_excl_f64.argtypes
+    L.orc_nb14_f64.argtypes = [_i, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p]
+    L.orc_nb14_twin.argtypes = L.orc_nb14_f64.argtypes
     L.orc_fft3d_f64.argtypes = [_p, _i, _i, _i, _i]
     L.orc_dft3d_naive_f64.argtypes = [_p, _p, _i, _i, _i, _i]
     L.orc_pme_f64.argtypes = [_i, _p, _p, _p, _i, _i, _i, _f, _d, _p, _p, _p, _p, _p]
@@ -216,6 +218,47 @@ def excl_twin(crd, charge, cell, rcell, beta, excl, virial=True):
     return frc, e, vir
 
 
+def _nb14_arrays(nb14):
+    a, b, A, B, cf = nb14
+    return (_c(a, np.int32), _c(b, np.int32), _c(A, np.float32), _c(B, np.float32),
+            _c(cf, np.float32))
+
+
+def nb14_f64(crd, charge, cell, rcell, nb14, virial=True):
+    """1-4 pairs in FP64 (nb14.cpp:7-100).  nb14 = (atom_a, atom_b, A, B, cf)
+    with A, B in the x12 / x6 form.  Returns (frc[n,3], e_pair[2,m] LJ|Coulomb,
+    e_atom[n], vir[n,6] or None)."""
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    a, b, A, B, cf = _nb14_arrays(nb14)
+    m = a.shape[0]
+    frc = np.zeros((n, 3), np.float64)
+    ep = np.zeros(2 * max(m, 1), np.float64)
+    ea = np.zeros(n, np.float64)
+    vir = np.zeros((n, 6), np.float64) if virial else None
+    lib().orc_nb14_f64(n, _ptr(crd), _ptr(_c(charge, np.float32)), _ptr(_c(cell, np.float32)),
+                       _ptr(_c(rcell, np.float32)), m, _ptr(a), _ptr(b), _ptr(A), _ptr(B),
+                       _ptr(cf), _ptr(frc), _ptr(ep), _ptr(ea), _ptr(vir))
+    return frc, ep[:2 * m].reshape(2, m), ea, vir
+
+
+def nb14_twin(crd, charge, cell, rcell, nb14, virial=True):
+    """Float twin of the 1-4 kernel: (frc[3,n], e_pair[2,m], e_atom[n],
+    vir[6,n] or None), int64 at scale 2^32."""
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    a, b, A, B, cf = _nb14_arrays(nb14)
+    m = a.shape[0]
+    frc = np.zeros((3, n), np.int64)
+    ep = np.zeros(2 * max(m, 1), np.int64)
+    ea = np.zeros(n, np.int64)
+    vir = np.zeros((6, n), np.int64) if virial else None
+    lib().orc_nb14_twin(n, _ptr(crd), _ptr(_c(charge, np.float32)), _ptr(_c(cell, np.float32)),
+                        _ptr(_c(rcell, np.float32)), m, _ptr(a), _ptr(b), _ptr(A), _ptr(B),
+                        _ptr(cf), _ptr(frc), _ptr(ep), _ptr(ea), _ptr(vir))
+    return frc, ep[:2 * m].reshape(2, m), ea, vir
+
+
 # --------------------------------------------------------------------------
 # FFT and PME
 # --------------------------------------------------------------------------
@@ -308,7 +351,8 @@ def nonbonded_terms_f64(sysd, cutoff=None, skin=2.0, tolerance=1e-5, grid=None,
                         want_forces=True):
     """All mdout terms of the path for a system dict (see sponge_amd.systems):
     LJ_short, LJ_long, PM_direct, PM_reciprocal, PM_self, PM_correction, PM,
-    plus total nonbonded forces (double[n,3]) and virial (reference convention)."""
+    nb14_LJ and nb14_EE (0 without 1-4 pairs), plus total nonbonded forces
+    (double[n,3]) and per-atom virial (reference convention)."""
     crd, q = sysd["crd"], sysd["charge"]
     cell, rcell = sysd["cell"], sysd["rcell"]
     box = sysd["box"]
@@ -334,9 +378,16 @@ def nonbonded_terms_f64(sysd, cutoff=None, skin=2.0, tolerance=1e-5, grid=None,
                PM_correction=float(e_ex.sum()))
     out["LJ"] = out["LJ_short"] + out["LJ_long"]
     out["PM"] = out["PM_direct"] + e_rec + e_self + out["PM_correction"]
+    f14 = np.zeros_like(f_pair)
+    v14 = np.zeros_like(v_pair)
+    out["nb14_LJ"] = out["nb14_EE"] = 0.0
+    if "nb14" in sysd and len(sysd["nb14"][0]) > 0:
+        f14, ep14, _, v14 = nb14_f64(crd, q, cell, rcell, sysd["nb14"])
+        out["nb14_LJ"], out["nb14_EE"] = float(ep14[0].sum()), float(ep14[1].sum())
     if want_forces:
-        out["frc"] = f_pair + f_ex + f_rec
+        out["frc"] = f_pair + f_ex + f_rec + f14
         out["frc_pair"], out["frc_excl"], out["frc_recip"] = f_pair, f_ex, f_rec
-        out["virial_atoms"] = v_pair + v_ex
+        out["frc_nb14"] = f14
+        out["virial_atoms"] = v_pair + v_ex + v14
         out["virial_recip"] = v_rec
     return out

@@ -577,6 +577,128 @@ ORC_EXPORT void orc_pair_twin(int n, const float* crd, const float* charge,
 }
 
 /* ------------------------------------------------------------------------ */
+/* 1-4 pairs (nb14/nb14.cpp:7-100)                                          */
+/* ------------------------------------------------------------------------ */
+/* Per pair k: a = atom_a[k], b = atom_b[k], dr = pbc(r_b - r_a); charges
+ * below 1e-10 in magnitude become 1e-10;
+ *   f = -A r^-14 + B r^-8 - cf qa qb r^-3,  force f dr on a and -f dr on b;
+ *   E_cf = cf qa qb / r,  E_lj = 0.08333333 A r^-12 - 0.1666666 B r^-6
+ *   (the reference's constants; A, B carry the x12 / x6 of the LJ files);
+ *   atom a takes E_cf + E_lj and the virial -f dr (x) dr.
+ * The FP64 layer evaluates this in double from the float inputs.  e_pair
+ * (2m: LJ | Coulomb) and e_atom may be NULL. */
+static inline float nb14_charge(float q)
+{
+    return (double)fabsf(q) < 1e-10 ? (float)1e-10 : q;
+}
+
+ORC_EXPORT void orc_nb14_f64(int n, const float* crd, const float* charge, const float* cell,
+                             const float* rcell, int m, const int* atom_a, const int* atom_b,
+                             const float* A, const float* B, const float* cf, double* frc,
+                             double* e_pair, double* e_atom, double* vir)
+{
+    for (int k = 0; k < m; k++)
+    {
+        int a = atom_a[k], b = atom_b[k];
+        double d[3];
+        pbc_disp_d(crd + 3 * b, crd + 3 * a, cell, rcell, d);
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        double r = sqrt(r2);
+        double qa = nb14_charge(charge[a]), qb = nb14_charge(charge[b]);
+        double ir2 = 1.0 / r2;
+        double ir6 = ir2 * ir2 * ir2;
+        double f = -(double)A[k] * ir6 * ir6 * ir2 + (double)B[k] * ir6 * ir2 -
+                   (double)cf[k] * qa * qb * ir2 / r;
+        for (int c = 0; c < 3; c++)
+        {
+            frc[3 * a + c] += f * d[c];
+            frc[3 * b + c] -= f * d[c];
+        }
+        double ecf = (double)cf[k] * qa * qb / r;
+        double elj = 0.08333333 * (double)A[k] * ir6 * ir6 - 0.1666666 * (double)B[k] * ir6;
+        if (e_pair)
+        {
+            e_pair[k] = elj;
+            e_pair[m + k] = ecf;
+        }
+        if (e_atom) e_atom[a] += ecf + elj;
+        if (vir)
+        {
+            double* v = vir + 6 * a;
+            double fx = f * d[0], fy = f * d[1], fz = f * d[2];
+            v[0] -= fx * d[0];
+            v[1] -= fx * d[1] + fy * d[0];
+            v[2] -= fy * d[1];
+            v[3] -= fx * d[2] + fz * d[0];
+            v[4] -= fy * d[2] + fz * d[1];
+            v[5] -= fz * d[2];
+        }
+    }
+}
+
+/* Float twin of the 1-4 kernel: the float force in the reference's operation
+ * order (1/r^2 and 1/r as double quotients rounded to float, 1/r from the
+ * correctly rounded sqrtf), every term rounded to int64 at scale 2^32;
+ * energies in double from the float displacement.  Accumulators (+=): frc[3n]
+ * SoA, e_pair[2m] (LJ | Coulomb), e_atom[n], vir[6n] SoA; the last three may
+ * be NULL. */
+ORC_EXPORT void orc_nb14_twin(int n, const float* crd, const float* charge, const float* cell,
+                              const float* rcell, int m, const int* atom_a, const int* atom_b,
+                              const float* A, const float* B, const float* cf, int64_t* frc,
+                              int64_t* e_pair, int64_t* e_atom, int64_t* vir)
+{
+    for (int k = 0; k < m; k++)
+    {
+        int a = atom_a[k], b = atom_b[k];
+        float d[3];
+        pbc_disp_f(crd + 3 * b, crd + 3 * a, cell, rcell, d);
+        float dr2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+        float dr_2 = (float)(1.0 / (double)dr2);
+        float dr_4 = dr_2 * dr_2;
+        float dr_8 = dr_4 * dr_4;
+        float dr_14 = (dr_8 * dr_4) * dr_2;
+        float dr_1 = (float)(1.0 / (double)sqrtf(dr2));
+        float qa = nb14_charge(charge[a]), qb = nb14_charge(charge[b]);
+        float fcf = ((-((cf[k] * dr_2) * dr_1)) * qa) * qb;
+        float flj = (-A[k]) * dr_14 + B[k] * dr_8;
+        float f = flj + fcf;
+        float fv[3] = {f * d[0], f * d[1], f * d[2]};
+        for (int c = 0; c < 3; c++)
+        {
+            int64_t q = to_fixed(fv[c]);
+            frc[(size_t)c * n + a] += q;
+            frc[(size_t)c * n + b] -= q;
+        }
+        if (e_pair || e_atom)
+        {
+            double dx = d[0], dy = d[1], dz = d[2];
+            double r2d = (dx * dx + dy * dy) + dz * dz;
+            double d1 = 1.0 / sqrt(r2d);
+            double d2 = 1.0 / r2d;
+            double d4 = d2 * d2, d8 = d4 * d4;
+            int64_t pc = to_fixed_d((((double)cf[k] * d1) * (double)qa) * (double)qb);
+            int64_t pl = to_fixed_d(((0.08333333 * (double)A[k]) * d4) * d8 -
+                                    ((0.1666666 * (double)B[k]) * d4) * d2);
+            if (e_pair)
+            {
+                e_pair[k] += pl;
+                e_pair[m + k] += pc;
+            }
+            if (e_atom) e_atom[a] += pl + pc;
+        }
+        if (vir)
+        {
+            vir[0 * (size_t)n + a] += to_fixed(-(fv[0] * d[0]));
+            vir[1 * (size_t)n + a] += to_fixed(-(fv[0] * d[1] + fv[1] * d[0]));
+            vir[2 * (size_t)n + a] += to_fixed(-(fv[1] * d[1]));
+            vir[3 * (size_t)n + a] += to_fixed(-(fv[0] * d[2] + fv[2] * d[0]));
+            vir[4 * (size_t)n + a] += to_fixed(-(fv[1] * d[2] + fv[2] * d[1]));
+            vir[5 * (size_t)n + a] += to_fixed(-(fv[2] * d[2]));
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* PME exclusion correction (PM_force.cpp:1743-1814), factor 1 (one rank)   */
 /* ------------------------------------------------------------------------ */
 /* For j in excl(i): dr = pbc(r_j - r_i), x = beta r,

@@ -193,6 +193,18 @@ int window_cover(int extent, int size, int n)
     return (extent + size - 1) / size + (n % size ? 1 : 0);
 }
 
+void free_nb14(SnbNb14& M)
+{
+    dfree(M.a);
+    dfree(M.b);
+    dfree(M.A);
+    dfree(M.B);
+    dfree(M.cf);
+    dfree(M.pair_e);
+    M.count = 0;
+    M.init = false;
+}
+
 void free_pme(SnbPme& P)
 {
     dfree(P.tw_x);
@@ -318,7 +330,7 @@ int snb_context_create(snb_context** out, int device, void* hip_stream, int atom
         rc = SNB_ERR_HIP;
     if (!rc && hipEventCreateWithFlags(&c->ev_pme, hipEventDisableTiming) != hipSuccess)
         rc = SNB_ERR_HIP;
-    if (!rc) rc = dalloc(&c->e_atom, 3 * (size_t)n);
+    if (!rc) rc = dalloc(&c->e_atom, 4 * (size_t)n);
     if (!rc) rc = dalloc(&c->vir_atom, 6 * (size_t)n);
     if (!rc) rc = dalloc(&c->satom, n);
     if (!rc) rc = dalloc(&c->stype, n);
@@ -336,7 +348,7 @@ int snb_context_create(snb_context** out, int device, void* hip_stream, int atom
     snb_launch_fill_int(s, c->type, n, 0);
     snb_launch_fill_i64(s, c->fx, 3 * (size_t)n);
     snb_launch_fill_i64(s, c->fx_pme, 3 * (size_t)n);
-    snb_launch_fill_i64(s, c->e_atom, 3 * (size_t)n);
+    snb_launch_fill_i64(s, c->e_atom, 4 * (size_t)n);
     snb_launch_fill_i64(s, c->vir_atom, 6 * (size_t)n);
     snb_launch_fill_i64(s, c->scal, SNB_S_COUNT);
     snb_launch_fill_i64(s, (int64_t*)c->dscal, SNB_D_COUNT);
@@ -364,6 +376,7 @@ int snb_context_destroy(snb_context* c)
     dfree(c->status);
     dfree(c->fx);
     dfree(c->e_atom);
+    free_nb14(c->nb14);
     dfree(c->vir_atom);
     dfree(c->satom);
     dfree(c->stype);
@@ -848,6 +861,94 @@ int snb_lj_energy(snb_context* c, double* lj_short, double* lj_long)
     return SNB_OK;
 }
 
+// --------------------------------------------------------------------- 1-4
+int snb_nb14_initial(snb_context* c, int nb14_numbers, const int* h_atom_a, const int* h_atom_b,
+                     const float* h_A, const float* h_B, const float* h_cf_scale_factor)
+{
+    if (!c || nb14_numbers < 0) return SNB_ERR_ARGUMENT;
+    if (nb14_numbers > 0 && (!h_atom_a || !h_atom_b || !h_A || !h_B || !h_cf_scale_factor))
+        return SNB_ERR_ARGUMENT;
+    for (int k = 0; k < nb14_numbers; k++)
+        if (h_atom_a[k] < 0 || h_atom_a[k] >= c->n || h_atom_b[k] < 0 || h_atom_b[k] >= c->n)
+            return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SnbNb14& M = c->nb14;
+    free_nb14(M);
+    M.count = nb14_numbers;
+    if (nb14_numbers > 0)
+    {
+        size_t m = (size_t)nb14_numbers;
+        int rc = dalloc(&M.a, m);
+        if (!rc) rc = dalloc(&M.b, m);
+        if (!rc) rc = dalloc(&M.A, m);
+        if (!rc) rc = dalloc(&M.B, m);
+        if (!rc) rc = dalloc(&M.cf, m);
+        if (!rc) rc = dalloc(&M.pair_e, 2 * m);
+        if (rc)
+        {
+            free_nb14(M);
+            return rc;
+        }
+        SNB_CHECK(hipMemcpy(M.a, h_atom_a, sizeof(int) * m, hipMemcpyHostToDevice));
+        SNB_CHECK(hipMemcpy(M.b, h_atom_b, sizeof(int) * m, hipMemcpyHostToDevice));
+        SNB_CHECK(hipMemcpy(M.A, h_A, sizeof(float) * m, hipMemcpyHostToDevice));
+        SNB_CHECK(hipMemcpy(M.B, h_B, sizeof(float) * m, hipMemcpyHostToDevice));
+        SNB_CHECK(hipMemcpy(M.cf, h_cf_scale_factor, sizeof(float) * m, hipMemcpyHostToDevice));
+        SNB_CHECK(hipMemset(M.pair_e, 0, sizeof(int64_t) * 2 * m));
+    }
+    // no pairs: not initialised, and the force call does nothing, as in the
+    // reference (nb14.cpp:165-168, :288)
+    M.init = nb14_numbers > 0;
+    return SNB_OK;
+}
+
+int snb_nb14_force(snb_context* c, const snb_vector* d_crd, const float* d_charge,
+                   snb_ltmatrix3 cell, snb_ltmatrix3 rcell, snb_vector* d_frc,
+                   int need_atom_energy, float* d_atom_energy, int need_virial,
+                   snb_ltmatrix3* d_atom_virial)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->nb14.init) return SNB_OK;
+    if (!d_crd || !d_charge) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SnbBox box = snb_box(cell, rcell);
+    int rc;
+    // the module's per-atom energy restarts each energy call
+    if (need_atom_energy &&
+        (rc = snb_launch_fill_i64(c->stream, c->e_atom + 3 * (size_t)c->n, c->n)))
+        return rc;
+    if ((rc = snb_launch_nb14(c, d_crd, d_charge, box, need_atom_energy, need_virial))) return rc;
+    if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
+    if (need_atom_energy)
+    {
+        const int m = c->nb14.count;
+        if ((rc = snb_launch_sum_fixed(c, c->nb14.pair_e, m, SNB_S_E_NB14_LJ))) return rc;
+        if ((rc = snb_launch_sum_fixed(c, c->nb14.pair_e + m, m, SNB_S_E_NB14_EE))) return rc;
+        if ((rc = snb_launch_flush_atom_energy(c, d_atom_energy, nullptr, 8))) return rc;
+    }
+    if (need_virial && d_atom_virial && (rc = snb_launch_flush_virial(c, d_atom_virial)))
+        return rc;
+    return SNB_OK;
+}
+
+int snb_nb14_energy(snb_context* c, double* lj14, double* ee14)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->nb14.init)
+    {
+        if (lj14) *lj14 = 0.0;
+        if (ee14) *ee14 = 0.0;
+        return SNB_OK;
+    }
+    DeviceGuard g(c->device);
+    SNB_CHECK(hipStreamSynchronize(c->stream));
+    int64_t v[2] = {0, 0};
+    SNB_CHECK(hipMemcpy(v, c->scal + SNB_S_E_NB14_LJ, sizeof(v), hipMemcpyDeviceToHost));
+    if (lj14) *lj14 = snb_fixed_to_double(v[0]);
+    if (ee14) *ee14 = snb_fixed_to_double(v[1]);
+    return SNB_OK;
+}
+
 // ---------------------------------------------------------------------- PME
 int snb_pme_initial(snb_context* c, snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
                     snb_vector box_length, float cutoff, float tolerance, int fftx, int ffty,
@@ -1031,7 +1132,7 @@ int snb_flush_atom_energy(snb_context* c, float* d_atom_energy, float* d_direct)
 {
     if (!c) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
-    return snb_launch_flush_atom_energy(c, d_atom_energy, d_direct, 7);
+    return snb_launch_flush_atom_energy(c, d_atom_energy, d_direct, 15);
 }
 
 int snb_flush_atom_virial(snb_context* c, snb_ltmatrix3* d_atom_virial)
@@ -1062,10 +1163,11 @@ int snb_clear_fixed_forces(snb_context* c)
 
 int snb_copy_fixed_atom_terms(snb_context* c, int which, int64_t* d_out)
 {
-    if (!c || !d_out || which < 0 || which > 3) return SNB_ERR_ARGUMENT;
+    if (!c || !d_out || which < 0 || which > 4) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
-    const int64_t* src = which < 3 ? c->e_atom + (size_t)which * c->n : c->vir_atom;
-    size_t cnt = which < 3 ? (size_t)c->n : 6 * (size_t)c->n;
+    const int64_t* src = which == 3 ? c->vir_atom
+                                    : c->e_atom + (size_t)(which == 4 ? 3 : which) * c->n;
+    size_t cnt = which == 3 ? 6 * (size_t)c->n : (size_t)c->n;
     SNB_CHECK(hipMemcpyAsync(d_out, src, sizeof(int64_t) * cnt, hipMemcpyDeviceToDevice,
                              c->stream));
     return SNB_OK;

@@ -18,6 +18,8 @@ enum SnbScalarSlot
     SNB_S_E_CORR = 2,    // sum of per-atom exclusion-correction energy
     SNB_S_Q2 = 3,        // sum q^2
     SNB_S_Q = 4,         // sum q
+    SNB_S_E_NB14_LJ = 5,  // sum of 1-4 pair LJ energies
+    SNB_S_E_NB14_EE = 6,  // sum of 1-4 pair Coulomb energies
     SNB_S_COUNT = 8
 };
 
@@ -103,6 +105,20 @@ struct SnbLj
     bool owner_attribution = false;
 };
 
+// 1-4 pairs (NON_BOND_14): per pair the two atoms, A (x12 form), B (x6 form)
+// and the Coulomb scale, resolved by the caller as the reference's loader does
+struct SnbNb14
+{
+    bool init = false;
+    int count = 0;
+    int* a = nullptr;
+    int* b = nullptr;
+    float* A = nullptr;
+    float* B = nullptr;
+    float* cf = nullptr;
+    int64_t* pair_e = nullptr;  // 2 * count, fixed point: LJ | Coulomb of each pair
+};
+
 struct SnbPme
 {
     bool init = false;
@@ -169,7 +185,7 @@ struct snb_context
     int64_t* fx = nullptr;       // 3n, SoA: direct-space and exclusion forces
     int64_t* fx_pme = nullptr;   // 3n, SoA: reciprocal forces (own accumulator: the PME
                                  // pipeline may run concurrently on pme_stream)
-    int64_t* e_atom = nullptr;   // 3n: LJ | direct | correction
+    int64_t* e_atom = nullptr;   // 4n: LJ | direct | correction | 1-4
     int64_t* vir_atom = nullptr; // 6n, SoA
     int64_t* scal = nullptr;     // SNB_S_COUNT
     double* dscal = nullptr;     // SNB_D_COUNT
@@ -194,6 +210,7 @@ struct snb_context
     bool pme_pending = false;
     SnbNl nl;
     SnbLj lj;
+    SnbNb14 nb14;
     SnbPme pme;
     SnbProf prof;
 };
@@ -262,6 +279,8 @@ int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge
                     const SnbBox& box, float beta,
                     const int* ex_start, const int* ex_list, const int* ex_num,
                     int need_energy, int need_virial, int64_t* fx);
+int snb_launch_nb14(snb_context* ctx, const snb_vector* crd, const float* charge,
+                    const SnbBox& box, int need_energy, int need_virial);
 
 // PME (snb_pme.hip)
 int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box);

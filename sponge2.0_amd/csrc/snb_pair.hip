@@ -292,6 +292,105 @@ int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge
     return SNB_OK;
 }
 
+// Scaled 1-4 LJ and plain Coulomb of the 1-4 pairs (stands in for
+// Dihedral_14_LJ_CF_Force_With_Atom_Energy_And_Virial_Device, nb14.cpp:7-100).
+// One thread per pair.  The force is float, in the reference's operation
+// order: dr = pbc(r_b - r_a), 1/r^2 and 1/r as a double quotient rounded to
+// float, charges below 1e-10 in magnitude raised to 1e-10, f = -A r^-14 +
+// B r^-8 - cf qa qb r^-3; +f dr on a and -f dr on b, rounded to fixed point.
+// Energies in FP64 from the float displacement with the reference's
+// constants (cf qa qb / r and 0.08333333 A r^-12 - 0.1666666 B r^-6), kept
+// per pair and added to atom a; the virial -f dr (x) dr goes to atom a.
+// 1/r uses sqrtf (correctly rounded) where the reference takes norm3df.
+template <bool NEED_ENERGY, bool NEED_VIRIAL>
+__global__ void __launch_bounds__(256) k_nb14(int m, int n, const int* __restrict__ a14,
+                                              const int* __restrict__ b14,
+                                              const float* __restrict__ A14,
+                                              const float* __restrict__ B14,
+                                              const float* __restrict__ cf14,
+                                              const snb_vector* __restrict__ crd,
+                                              const float* __restrict__ charge, SnbBox box,
+                                              int64_t* __restrict__ fx,
+                                              int64_t* __restrict__ e_nb14,
+                                              int64_t* __restrict__ pair_e,
+                                              int64_t* __restrict__ vir)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int i = a14[k], j = b14[k];
+    const snb_vector ri = crd[i], rj = crd[j];
+    float dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+    pbc_disp(dx, dy, dz, box);
+    const float dr2 = (dx * dx + dy * dy) + dz * dz;
+    const float dr_2 = (float)(1.0 / (double)dr2);
+    const float dr_4 = dr_2 * dr_2;
+    const float dr_8 = dr_4 * dr_4;
+    const float dr_14 = (dr_8 * dr_4) * dr_2;
+    const float dr_1 = (float)(1.0 / (double)sqrtf(dr2));
+    float qi = charge[i], qj = charge[j];
+    if ((double)fabsf(qi) < 1e-10) qi = (float)1e-10;
+    if ((double)fabsf(qj) < 1e-10) qj = (float)1e-10;
+    const float A = A14[k], B = B14[k], cf = cf14[k];
+    const float fcf = ((-((cf * dr_2) * dr_1)) * qi) * qj;
+    const float flj = (-A) * dr_14 + B * dr_8;
+    const float f = flj + fcf;
+    const float fxv = f * dx, fyv = f * dy, fzv = f * dz;
+    const int64_t ix = snb_to_fixed(fxv), iy = snb_to_fixed(fyv), iz = snb_to_fixed(fzv);
+    atomic_add_i64(fx + i, ix);
+    atomic_add_i64(fx + n + i, iy);
+    atomic_add_i64(fx + 2 * n + i, iz);
+    atomic_add_i64(fx + j, -ix);
+    atomic_add_i64(fx + n + j, -iy);
+    atomic_add_i64(fx + 2 * n + j, -iz);
+    if (NEED_ENERGY)
+    {
+        const double ddx = dx, ddy = dy, ddz = dz;
+        const double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
+        const double d1 = 1.0 / __builtin_sqrt(r2d);
+        const double d2 = 1.0 / r2d;
+        const double d4 = d2 * d2, d8 = d4 * d4;
+        const int64_t pc = snb_to_fixed_d((((double)cf * d1) * (double)qi) * (double)qj);
+        const int64_t pl = snb_to_fixed_d(((0.08333333 * (double)A) * d4) * d8 -
+                                          ((0.1666666 * (double)B) * d4) * d2);
+        pair_e[k] = pl;
+        pair_e[m + k] = pc;
+        atomic_add_i64(e_nb14 + i, pl + pc);
+    }
+    if (NEED_VIRIAL)
+    {
+        atomic_add_i64(vir + i, snb_to_fixed(-(fxv * dx)));
+        atomic_add_i64(vir + (size_t)n + i, snb_to_fixed(-(fxv * dy + fyv * dx)));
+        atomic_add_i64(vir + (size_t)2 * n + i, snb_to_fixed(-(fyv * dy)));
+        atomic_add_i64(vir + (size_t)3 * n + i, snb_to_fixed(-(fxv * dz + fzv * dx)));
+        atomic_add_i64(vir + (size_t)4 * n + i, snb_to_fixed(-(fyv * dz + fzv * dy)));
+        atomic_add_i64(vir + (size_t)5 * n + i, snb_to_fixed(-(fzv * dz)));
+    }
+}
+
+int snb_launch_nb14(snb_context* ctx, const snb_vector* crd, const float* charge,
+                    const SnbBox& box, int need_energy, int need_virial)
+{
+    const SnbNb14& M = ctx->nb14;
+    const int m = M.count, n = ctx->n;
+    if (m <= 0) return SNB_OK;
+    dim3 g((m + 255) / 256);
+#define SNB_NB14_LAUNCH(E, V)                                                                \
+    hipLaunchKernelGGL((k_nb14<E, V>), g, dim3(256), 0, ctx->stream, m, n, M.a, M.b, M.A, M.B, \
+                       M.cf, crd, charge, box, ctx->fx, ctx->e_atom + 3 * (size_t)n, M.pair_e, \
+                       ctx->vir_atom)
+    if (need_energy && need_virial)
+        SNB_NB14_LAUNCH(true, true);
+    else if (need_energy)
+        SNB_NB14_LAUNCH(true, false);
+    else if (need_virial)
+        SNB_NB14_LAUNCH(false, true);
+    else
+        SNB_NB14_LAUNCH(false, false);
+#undef SNB_NB14_LAUNCH
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
 // ---------------------------------------------------------------------------
 // windowed path
 // ---------------------------------------------------------------------------

@@ -69,6 +69,10 @@ _SIGS = {
                                  _I, _P, _I, _P, _P], _I),
     "snb_lj_long_range_correction": ([_P, _I, _P, _I, _P, _F], _I),
     "snb_lj_energy": ([_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], _I),
+    "snb_nb14_initial": ([_P, _I, _P, _P, _P, _P, _P], _I),
+    "snb_nb14_force": ([_P, _P, _P, LTMatrix3, LTMatrix3, _P, _I, _P, _I, _P], _I),
+    "snb_nb14_energy": ([_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+                        _I),
     "snb_pme_initial": ([_P, LTMatrix3, LTMatrix3, Vector, _F, _F, _I, _I, _I, _F, _I], _I),
     "snb_pme_parameters": ([_P, ctypes.POINTER(_F), ctypes.POINTER(_I), ctypes.POINTER(_I),
                             ctypes.POINTER(_I)], _I),
@@ -176,6 +180,15 @@ class Nonbonded:
         t = np.ascontiguousarray(sysd["lj_type"], np.int32)
         check(L.snb_lj_initial(ctx, int(sysd["lj_ntypes"]), a.ctypes.data, b.ctypes.data,
                                t.ctypes.data, self.cutoff), "snb_lj_initial")
+        self.nb14 = None
+        if "nb14" in sysd and len(sysd["nb14"][0]) > 0:
+            a14, b14, A14, B14, cf14 = (np.ascontiguousarray(x, dt) for x, dt in
+                                        zip(sysd["nb14"], (np.int32, np.int32, np.float32,
+                                                           np.float32, np.float32)))
+            check(L.snb_nb14_initial(ctx, len(a14), a14.ctypes.data, b14.ctypes.data,
+                                     A14.ctypes.data, B14.ctypes.data, cf14.ctypes.data),
+                  "snb_nb14_initial")
+            self.nb14 = (a14, b14, A14, B14, cf14)
         g = grid if grid is not None else sysd.get("pme_grid", (0, 0, 0))
         check(L.snb_pme_initial(ctx, self.cell, self.rcell, Vector(*[float(x) for x in self.box]),
                                 self.cutoff, tolerance, int(g[0]), int(g[1]), int(g[2]), 1.0,
@@ -238,6 +251,19 @@ class Nonbonded:
             1 if need_virial else 0, _ptr(self.atom_virial) if frc else None),
             "snb_pme_excluded_force")
 
+    def nb14_force(self, need_energy=False, need_virial=False, frc=True):
+        check(self.L.snb_nb14_force(
+            self.ctx, _ptr(self.crd), _ptr(self.charge), self.cell, self.rcell,
+            _ptr(self.frc) if frc else None, 1 if need_energy else 0,
+            _ptr(self.atom_energy) if frc else None, 1 if need_virial else 0,
+            _ptr(self.atom_virial) if frc else None), "snb_nb14_force")
+
+    def nb14_energy(self):
+        lj, ee = ctypes.c_double(), ctypes.c_double()
+        check(self.L.snb_nb14_energy(self.ctx, ctypes.byref(lj), ctypes.byref(ee)),
+              "snb_nb14_energy")
+        return lj.value, ee.value
+
     def reciprocal(self, step=0, need_energy=False, need_virial=False, frc=True):
         check(self.L.snb_pme_reciprocal_force(
             self.ctx, _ptr(self.crd), self.cell, self.rcell, _ptr(self.charge),
@@ -295,7 +321,7 @@ class Nonbonded:
 
     def step(self, step=0, need_energy=False, need_virial=False, forced_nl=False, fused=True):
         """One force evaluation of the path in the reference's order
-        (main.cpp:307-334, :371-396, :474-478).  fused=True keeps every term
+        (main.cpp:307-334, :371-396, :455-458, :474-478).  fused=True keeps every term
         in the fixed-point accumulators and converts once at the end."""
         check(self.L.snb_nl_update(self.ctx, None, self.n, 0, _ptr(self.crd), self.cell,
                                    self.rcell, step, 1 if forced_nl else 0, None, None, None),
@@ -304,6 +330,8 @@ class Nonbonded:
         self.lj_direct(need_energy, need_virial, frc=not fused)
         if need_energy or need_virial:
             self.long_range(need_energy, need_virial)
+        if self.nb14 is not None:
+            self.nb14_force(need_energy, need_virial, frc=not fused)
         self.reciprocal(step, need_energy, need_virial, frc=not fused)
         if fused:
             self.flush()
@@ -335,6 +363,8 @@ class Nonbonded:
         check(self.L.snb_clear_fixed_forces(self.ctx), "snb_clear_fixed_forces")
 
     def fixed_atom_terms(self, which):
+        """which: 0 LJ, 1 direct, 2 exclusion correction, 4 1-4 energy (n each);
+        3 virial (6 x n)."""
         cnt = 6 * self.n if which == 3 else self.n
         t = self.torch.empty(cnt, dtype=self.torch.int64, device=self.dev)
         check(self.L.snb_copy_fixed_atom_terms(self.ctx, which, _ptr(t)),

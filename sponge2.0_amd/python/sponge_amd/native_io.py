@@ -10,6 +10,9 @@ The formats are the ones Xponge's native loaders read
   coordinate ``N [t0]`` / xyz per atom / ``Lx Ly Lz`` / ``alpha beta gamma``.
   exclude   ``N total`` / per atom ``count j1 j2 ...``.
   residue   ``N R`` / atoms per residue.
+  nb14      ``M`` / per pair ``a b lj_scale cf_scale`` (xponge/load/native/nb14.hpp:
+            A = lj_scale * the pair's x12 A, B likewise); nb14_extra ``M`` / per
+            pair ``a b A B cf_scale`` (A x12 and B x6 on load); extra pairs first.
 
 ``system_from_prefix`` assembles the arrays the nonbonded path needs, with the
 box converted to the reference's LTMatrix3 cell/rcell (MD_core/pbc.hpp:66-101).
@@ -72,6 +75,48 @@ def read_exclude(path):
     return start, numbers, np.array(lst, dtype=np.int32)
 
 
+def read_nb14(path, extra_path, lj_type, lj_a, lj_b):
+    """1-4 pair list as Xponge::Native_Load_NB14 builds it (nb14.hpp:8-95):
+    returns (atom_a, atom_b, A, B, cf) with the extra pairs first.  Either
+    path may be None."""
+    rows_x, rows_m = [], []
+    if extra_path is not None:
+        t = _tokens(extra_path)
+        rows_x = [t[1 + 5 * k:6 + 5 * k] for k in range(int(t[0]))]
+    if path is not None:
+        t = _tokens(path)
+        rows_m = [t[1 + 4 * k:5 + 4 * k] for k in range(int(t[0]))]
+    m = len(rows_x) + len(rows_m)
+    a = np.zeros(m, np.int32)
+    b = np.zeros(m, np.int32)
+    A = np.zeros(m, np.float32)
+    B = np.zeros(m, np.float32)
+    cf = np.zeros(m, np.float32)
+    for k, r in enumerate(rows_x):
+        a[k], b[k] = int(r[0]), int(r[1])
+        A[k] = np.float32(r[2]) * np.float32(12.0)
+        B[k] = np.float32(r[3]) * np.float32(6.0)
+        cf[k] = np.float32(r[4])
+    for q, r in enumerate(rows_m):
+        k = len(rows_x) + q
+        a[k], b[k] = int(r[0]), int(r[1])
+        ta, tb = int(lj_type[a[k]]), int(lj_type[b[k]])
+        hi, lo = max(ta, tb), min(ta, tb)
+        p = hi * (hi + 1) // 2 + lo
+        A[k] = np.float32(r[2]) * lj_a[p]
+        B[k] = np.float32(r[2]) * lj_b[p]
+        cf[k] = np.float32(r[3])
+    return a, b, A, B, cf
+
+
+def write_nb14(path, atom_a, atom_b, lj_scale, cf_scale):
+    """nb14 file of pairs with an LJ scale (the form read_nb14 resolves)."""
+    with open(path, "w") as f:
+        f.write("%d\n" % len(atom_a))
+        for a, b, s, c in zip(atom_a, atom_b, lj_scale, cf_scale):
+            f.write("%d %d %.6f %.6f\n" % (int(a), int(b), float(s), float(c)))
+
+
 def box_to_cell(box, angle=(90.0, 90.0, 90.0)):
     """MD_core/pbc.hpp:66-101 in float32: returns (cell, rcell) LTMatrix3."""
     f = np.float32
@@ -111,6 +156,10 @@ def system_from_prefix(directory, prefix):
              excl_numbers=en, excl_list=el)
     if os.path.exists(p("mass")):
         d["mass"] = read_scalar_list(p("mass"))
+    if os.path.exists(p("nb14")) or os.path.exists(p("nb14_extra")):
+        d["nb14"] = read_nb14(p("nb14") if os.path.exists(p("nb14")) else None,
+                              p("nb14_extra") if os.path.exists(p("nb14_extra")) else None,
+                              types, a, b)
     return d
 
 
@@ -139,6 +188,15 @@ def write_system(directory, prefix, sysd):
         np.savetxt(f, sysd["crd"], fmt="%.7f")
         f.write("%.7f %.7f %.7f\n" % tuple(sysd["box"]))
         f.write("%.7f %.7f %.7f\n" % tuple(sysd.get("angle", (90.0, 90.0, 90.0))))
+    if "nb14" in sysd:
+        a14, b14, A14, B14, cf14 = sysd["nb14"]
+        with open(p("nb14_extra"), "w") as f:
+            f.write("%d\n" % len(a14))
+            for k in range(len(a14)):
+                f.write("%d %d %.9e %.9e %.6f\n" % (int(a14[k]), int(b14[k]),
+                                                    float(A14[k] / np.float32(12.0)),
+                                                    float(B14[k] / np.float32(6.0)),
+                                                    float(cf14[k])))
     with open(p("exclude"), "w") as f:
         es, en, el = sysd["excl_start"], sysd["excl_numbers"], sysd["excl_list"]
         f.write("%d %d\n" % (n, int(en.sum())))

@@ -219,6 +219,21 @@ def mixed_system(n_waters=30000, n_chains=3, beads_per_chain=1000, box=96.5,
                     pairs.append((b0 + k, b0 + k + d))
     out = _finish(crd, charge, ltype, sig, eps, mass, pairs, box, cutoff,
                   n_waters=n_waters, n_beads=nb)
+    # 1-4 pairs along each chain: LJ scaled by 0.5, Coulomb by 0.833 (the
+    # form the nb14 loader resolves from the LJ tables)
+    a14, b14 = [], []
+    for c in range(n_chains):
+        b0 = nw + c * beads_per_chain
+        for k in range(beads_per_chain - 3):
+            a14.append(b0 + k)
+            b14.append(b0 + k + 3)
+    a14 = np.array(a14, np.int32)
+    b14 = np.array(b14, np.int32)
+    ta, tb = out["lj_type"][a14], out["lj_type"][b14]
+    hi, lo = np.maximum(ta, tb), np.minimum(ta, tb)
+    p14 = hi * (hi + 1) // 2 + lo
+    out["nb14"] = (a14, b14, np.float32(0.5) * out["lj_a"][p14],
+                   np.float32(0.5) * out["lj_b"][p14], np.full(len(a14), 0.833, np.float32))
     # No LJ site (water O, bead) within 2.2 A of another non-excluded LJ site,
     # and no two atoms of different molecules within 0.6 A (random water
     # orientations on the lattice can bring two H's to ~0.8 A; H carries no LJ).

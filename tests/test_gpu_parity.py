@@ -539,3 +539,66 @@ def test_pme_stream_overlap_same_results(water_9k, overlap):
     assert list(res[0][3]) == list(res[1][3])
     np.testing.assert_array_equal(res[0][4], res[1][4])
     np.testing.assert_array_equal(res[0][4], res[0][0])   # fixed coordinates: same forces
+
+
+def _with_zero_charges(sysd):
+    """Copy of a system with the charges of the first few 1-4 atoms set to 0
+    (the reference raises |q| < 1e-10 to 1e-10 in the 1-4 kernel)."""
+    s = dict(sysd)
+    q = np.array(s["charge"], np.float32).copy()
+    q[np.asarray(s["nb14"][0][:3])] = 0.0
+    s["charge"] = q
+    return s
+
+
+@pytest.mark.parametrize("name", ["gromacs_iter0", "mixed_small", "gromacs_q0"])
+@pytest.mark.parametrize("energy,virial", [(False, False), (True, True)])
+def test_nb14_bit_exact_with_twin(request, name, energy, virial):
+    import oracle
+    s = _with_zero_charges(request.getfixturevalue("gromacs_iter0")) if name == "gromacs_q0" \
+        else _get(request, name)
+    nb = _nb(s)
+    print("case", name, "pairs", len(s["nb14"][0]), flush=True)
+    nb.nb14_force(need_energy=energy, need_virial=virial, frc=False)
+    g_f = nb.fixed_forces()
+    t_f, t_ep, t_ea, t_v = oracle.nb14_twin(s["crd"], s["charge"], s["cell"], s["rcell"],
+                                            s["nb14"], virial=virial)
+    np.testing.assert_array_equal(g_f, t_f)
+    if energy:
+        np.testing.assert_array_equal(nb.fixed_atom_terms(4), t_ea)
+        lj, ee = nb.nb14_energy()
+        assert lj == float(t_ep[0].sum()) * 2.0 ** -32
+        assert ee == float(t_ep[1].sum()) * 2.0 ** -32
+        f64, ep64, _, _ = oracle.nb14_f64(s["crd"], s["charge"], s["cell"], s["rcell"],
+                                          s["nb14"], virial=False)
+        tot = np.abs(ep64).sum()
+        assert abs(lj - ep64[0].sum()) <= ENERGY_TOL * tot
+        assert abs(ee - ep64[1].sum()) <= ENERGY_TOL * tot
+        assert rel_rms(g_f.T * 2.0 ** -32, f64) < FORCE_TOL
+    if virial:
+        np.testing.assert_array_equal(nb.fixed_atom_terms(3), t_v)
+    nb.close()
+
+
+def test_nb14_outputs_flush_and_energy_restart(gromacs_iter0):
+    """Direct outputs: d_frc += forces, d_atom_energy += per-atom 1-4 energy
+    (restarted at each energy call, so two calls give twice the force and
+    the same energy sums)."""
+    import oracle
+    s = gromacs_iter0
+    nb = _nb(s)
+    nb.zero_outputs()
+    nb.nb14_force(need_energy=True, need_virial=False, frc=True)
+    nb.nb14_force(need_energy=True, need_virial=False, frc=True)
+    nb.sync()
+    t_f, t_ep, t_ea, _ = oracle.nb14_twin(s["crd"], s["charge"], s["cell"], s["rcell"],
+                                          s["nb14"], virial=False)
+    one = (t_f.astype(np.float64) * 2.0 ** -32).astype(np.float32).T
+    f = nb.frc.cpu().numpy()
+    np.testing.assert_array_equal(f, one + one)
+    e = nb.atom_energy.cpu().numpy()
+    ea = (t_ea.astype(np.float64) * 2.0 ** -32).astype(np.float32)
+    np.testing.assert_array_equal(e, ea + ea)
+    lj, ee = nb.nb14_energy()
+    assert lj == float(t_ep[0].sum()) * 2.0 ** -32
+    nb.close()

@@ -406,3 +406,91 @@ def test_lj_long_range_factor_histogram_equals_double_loop(tip3p_1011):
             hi, lo = max(i, j), min(i, j)
             tot += np.sum(t == i) * np.sum(t == j) * float(b[hi * (hi + 1) // 2 + lo])
     assert abs(oracle.lj_c6_total(t, s["lj_ntypes"], b) - tot) < 1e-9 * abs(tot)
+
+
+# --------------------------------------------------------------- 1-4 pairs
+def test_nb14_reader_resolves_fixture_files(gromacs_iter0):
+    """xponge/load/native/nb14.hpp:58-95: the extra pairs come first with A x12
+    and B x6; the scaled pairs take lj_scale times the pair's x12 / x6 entry."""
+    from sponge_amd import native_io
+    s = gromacs_iter0
+    a, b, A, B, cf = s["nb14"]
+    assert len(a) == 22 + 27
+    d = os.path.join(GOLDEN, "gromacs_charmm_tip3p", "iter0")
+    first = open(os.path.join(d, "sys_flexible_nb14_extra.txt")).read().split()[1:6]
+    assert (a[0], b[0]) == (int(first[0]), int(first[1]))
+    assert A[0] == np.float32(first[2]) * np.float32(12.0)
+    assert B[0] == np.float32(first[3]) * np.float32(6.0)
+    row = open(os.path.join(d, "sys_flexible_nb14.txt")).read().split()[1:5]
+    ta, tb = s["lj_type"][int(row[0])], s["lj_type"][int(row[1])]
+    hi, lo = max(ta, tb), min(ta, tb)
+    assert A[22] == np.float32(row[2]) * s["lj_a"][hi * (hi + 1) // 2 + lo]
+    assert cf[22] == np.float32(row[3])
+    # round trip through the writer (extra-pair form)
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        native_io.write_system(tmp, "t", s)
+        back = native_io.system_from_prefix(tmp, "t")["nb14"]
+    np.testing.assert_array_equal(back[0], a)
+    np.testing.assert_allclose(back[2], A, rtol=1e-6)
+
+
+def test_nb14_gromacs_fixture_terms(gromacs_iter0):
+    """GROMACS 2025.4 reference terms of the fixture: coulomb14 193.9056 within
+    the reference's nb14_ee tolerance of 0.2 kcal/mol
+    (benchmarks/comparison/tests/gromacs/tests/comp_alanine_dipeptide.py:77,
+    :110), and lj14 2.8505 (no tolerance there; 1e-3 relative here)."""
+    s = gromacs_iter0
+    f, ep, ea, v = oracle.nb14_f64(s["crd"], s["charge"], s["cell"], s["rcell"], s["nb14"])
+    assert abs(ep[1].sum() - 193.90558413001912) <= 0.2
+    assert abs(ep[0].sum() - 2.850482791586998) <= 1e-3 * 2.850482791586998
+    assert abs(ea.sum() - ep.sum()) < 1e-9 * abs(ep.sum())
+    # momentum: every pair adds +f and -f
+    assert np.abs(f.sum(0)).max() < 1e-9 * np.abs(f).max()
+
+
+def test_nb14_force_is_minus_energy_gradient():
+    """Central differences of the pair energy against the force on atom a,
+    for pairs with LJ only, Coulomb only, both, and a zero charge (raised to
+    1e-10 by the reference's TINY rule)."""
+    rng = np.random.default_rng(4)
+    L = 30.0
+    cell = np.array([L, 0, L, 0, 0, L], np.float32)
+    rcell = np.array([1 / L, 0, 1 / L, 0, 0, 1 / L], np.float32)
+    for A, B, qa, qb, cf in ((6.0e5, 700.0, 0.0, 0.0, 0.0), (0.0, 0.0, 7.0, -5.0, 0.833),
+                             (2.0e5, 365.0, -6.0, 3.0, 0.5), (2.0e5, 365.0, 0.0, 3.0, 1.0)):
+        d = rng.normal(size=3)
+        d *= 3.3 / np.linalg.norm(d)
+        base = np.array([[10.0, 11.0, 12.0], [10.0, 11.0, 12.0] + d], np.float64)
+        q = np.array([qa, qb], np.float32)
+        nb14 = (np.array([0]), np.array([1]), np.array([A], np.float32),
+                np.array([B], np.float32), np.array([cf], np.float32))
+
+        def energy(crd):
+            _, ep, _, _ = oracle.nb14_f64(crd.astype(np.float32), q, cell, rcell, nb14,
+                                          virial=False)
+            return ep.sum()
+
+        f, _, _, _ = oracle.nb14_f64(base.astype(np.float32), q, cell, rcell, nb14, virial=False)
+        h = 1e-2
+        g = np.zeros(3)
+        for c in range(3):
+            p, m = base.copy(), base.copy()
+            p[0, c] += h
+            m[0, c] -= h
+            g[c] = -(energy(p) - energy(m)) / (2 * h)
+        scale = np.abs(f[0]).max() + 1e-12
+        assert np.abs(g - f[0]).max() < 2e-3 * scale, (A, B, qa, qb, g, f[0])
+        np.testing.assert_allclose(f[1], -f[0], rtol=1e-12, atol=1e-300)
+
+
+def test_nb14_twin_matches_fp64(mixed_small):
+    s = mixed_small
+    f64, ep64, ea64, v64 = oracle.nb14_f64(s["crd"], s["charge"], s["cell"], s["rcell"],
+                                           s["nb14"])
+    ft, ept, eat, vt = oracle.nb14_twin(s["crd"], s["charge"], s["cell"], s["rcell"], s["nb14"])
+    assert rel_rms(ft.T * 2.0 ** -32, f64) < 1e-6
+    # energies: double arithmetic on the float displacement (its rounding,
+    # ~1e-7 relative, is amplified by the r^-12 term)
+    assert abs(ept.sum() * 2.0 ** -32 - ep64.sum()) < 1e-6 * np.abs(ep64).sum()
+    assert rel_rms(vt.T * 2.0 ** -32, v64) < 1e-6
