@@ -6,6 +6,9 @@
  *   LENNARD_JONES_INFORMATION  SPONGE/Lennard_Jones_force/Lennard_Jones_force.{h,cpp}
  *   Particle_Mesh (PME)        SPONGE/PM_force/PM_force.{h,cpp}
  *   NON_BOND_14                SPONGE/nb14/nb14.{h,cpp}
+ *   NVE leap-frog, MIDDLE_Langevin_INFORMATION, SETTLE
+ *                              SPONGE/MD_core/nve.hpp, SPONGE/thermostat/Middle_Langevin_MD.cpp,
+ *                              SPONGE/constrain/settle.cpp
  * Each entry point below names the reference method it stands in for; the
  * argument lists keep the reference's order and meaning, with the module
  * object replaced by an snb_context.
@@ -224,6 +227,59 @@ int snb_nb14_force(snb_context* ctx, const snb_vector* d_crd, const float* d_cha
  * (nb14_LJ) and Coulomb (nb14_EE) pair energies of the last energy call, as
  * exact int64 sums.  Synchronises. */
 int snb_nb14_energy(snb_context* ctx, double* lj14, double* ee14);
+
+/* ------------------------------------------------------------- integration */
+/* MD_INFORMATION::NVE_iteration::Leap_Frog (MD_core/nve.hpp:3-48):
+ * v += dt inv_m f, x += dt v, with |v| capped at max_velocity when it is > 0
+ * (velocity_max).  dt in the internal time unit (ps * 20.455). */
+int snb_leap_frog(snb_context* ctx, const float* d_inverse_mass, const snb_vector* d_frc,
+                  snb_vector* d_vel, snb_vector* d_crd, float dt, float max_velocity);
+
+/* MIDDLE_Langevin_INFORMATION::Initial (thermostat/Middle_Langevin_MD.cpp:137-245)
+ * with the controller keys resolved by the caller: target_temperature (K),
+ * thermostat_tau (ps; friction 1/tau), thermostat_seed, dt (ps),
+ * velocity_max (<= 0: none).  Masses in amu; a zero mass gets neither force
+ * nor noise.  The noise comes from the library's counter-based generator
+ * (Philox4x32-10 + Box-Muller) seeded with `seed`, not from curand: the
+ * sequence differs from the reference's, its distribution does not. */
+int snb_langevin_initial(snb_context* ctx, const float* h_mass, float target_temperature,
+                         float tau, int seed, float dt, float max_velocity);
+
+/* MIDDLE_Langevin_INFORMATION::MD_Iteration_Leap_Frog
+ * (Middle_Langevin_MD.cpp:305-331, kernel :5-66): per atom
+ * v1 = v + dt f/m; x1 = x + dt/2 v1; v = e^{-gamma dt} v1 + sqrt(kT(1 - e^{-2 gamma dt})/m) R;
+ * x = x1 + dt/2 v.  d_acc is accepted for the reference's signature and unused. */
+int snb_langevin_iterate(snb_context* ctx, const snb_vector* d_frc, snb_vector* d_vel,
+                         snb_vector* d_acc, snb_vector* d_crd);
+
+/* The thermostat's dt (internal unit) and e^{-gamma dt}, and the constraint
+ * factors main.cpp:650-652 sets after the iteration: v_factor = e^{-gamma dt},
+ * x_factor = (e^{-gamma dt} + 1)/2.  NULL outputs are skipped. */
+int snb_langevin_parameters(snb_context* ctx, float* dt_internal, float* exp_gamma,
+                            float* v_factor, float* x_factor);
+
+/* SETTLE::Initial (constrain/settle.cpp:342-571) for rigid triangles: the
+ * caller lists each triangle (A = the atom bonded to both others, B, C) with
+ * its three constraint lengths, as the reference finds them among the
+ * constraint pairs.  Two-atom rigid pairs are not supported here.  Zero
+ * triangles leaves SETTLE uninitialised (its calls are no-ops). */
+int snb_settle_initial(snb_context* ctx, int triangle_numbers, const int* h_atom_A,
+                       const int* h_atom_B, const int* h_atom_C, const float* h_rAB,
+                       const float* h_rAC, const float* h_rBC, const float* h_mass);
+
+/* SETTLE::Remember_Last_Coordinates (settle.cpp:573-640): B - A and C - A of
+ * every triangle before the iteration moves the atoms. */
+int snb_settle_remember(snb_context* ctx, const snb_vector* d_crd, snb_ltmatrix3 cell,
+                        snb_ltmatrix3 rcell);
+
+/* SETTLE::Do_SETTLE (settle.cpp:659-697, kernels :101-253): restores each
+ * triangle's geometry and corrects the velocities by
+ * v_factor / dt / x_factor times the position change; with need_pressure,
+ * d_stress[0] += (1/V) * the constraint virial.  dt (internal unit),
+ * v_factor and x_factor are the CONSTRAIN module's (dt, v_factor, x_factor). */
+int snb_settle_do(snb_context* ctx, const float* d_mass, snb_vector* d_crd, snb_ltmatrix3 cell,
+                  snb_ltmatrix3 rcell, snb_vector* d_vel, int need_pressure,
+                  snb_ltmatrix3* d_stress, float dt, float v_factor, float x_factor);
 
 /* -------------------------------------------------------------------- PME */
 /* Particle_Mesh::Initial (PM_force.cpp:293-652) with the controller keys

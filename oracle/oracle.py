@@ -391,3 +391,92 @@ def nonbonded_terms_f64(sysd, cutoff=None, skin=2.0, tolerance=1e-5, grid=None,
         out["virial_atoms"] = v_pair + v_ex + v14
         out["virial_recip"] = v_rec
     return out
+
+
+# --------------------------------------------------------------------------
+# Integration: leap-frog, middle-scheme Langevin, SETTLE (float twins)
+# --------------------------------------------------------------------------
+_u32p = ctypes.c_void_p
+
+
+def _int_sigs():
+    L = lib()
+    if getattr(L, "_int_sigs_done", False):
+        return L
+    L.orc_philox4x32_10.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
+    L.orc_langevin_normals.argtypes = [ctypes.c_int, ctypes.c_uint32, _u32p, _u32p]
+    L.orc_langevin_twin.argtypes = [ctypes.c_int, _u32p, _u32p, _u32p, ctypes.c_uint32, _u32p,
+                                    _u32p, _u32p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float]
+    L.orc_leap_frog_twin.argtypes = [ctypes.c_int, _u32p, _u32p, _u32p, _u32p, ctypes.c_float,
+                                     ctypes.c_float]
+    L.orc_settle_frame.argtypes = [ctypes.c_float] * 6 + [_u32p]
+    L.orc_settle_remember.argtypes = [ctypes.c_int, _u32p, _u32p, _u32p, _u32p, _u32p, _u32p]
+    L.orc_settle_twin.argtypes = [ctypes.c_int, _u32p, _u32p, _u32p, _u32p, _u32p, _u32p, _u32p,
+                                  _u32p, ctypes.c_float, ctypes.c_float, ctypes.c_float, _u32p,
+                                  _u32p]
+    L._int_sigs_done = True
+    return L
+
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    out = np.zeros(4, np.uint32)
+    _int_sigs().orc_philox4x32_10(_ptr(c), int(key[0]), int(key[1]), _ptr(out))
+    return out
+
+
+def langevin_normals(seed, counter):
+    """The 4 normal numbers atom i draws at stream position counter[i]."""
+    counter = np.ascontiguousarray(counter, np.uint32)
+    out = np.zeros((len(counter), 4), np.float32)
+    _int_sigs().orc_langevin_normals(len(counter), int(seed) & 0xFFFFFFFF, _ptr(counter),
+                                     _ptr(out))
+    return out
+
+
+def langevin_twin(inv_mass, sqrt_mass, counter, seed, frc, vel, crd, dt, half_dt, exp_gamma,
+                  vmax=0.0):
+    """One middle-scheme Langevin step in place (vel, crd float32 [n,3];
+    counter uint32 [n] advanced)."""
+    n = len(inv_mass)
+    _int_sigs().orc_langevin_twin(n, _ptr(_c(inv_mass, np.float32)),
+                                  _ptr(_c(sqrt_mass, np.float32)), _ptr(counter),
+                                  int(seed) & 0xFFFFFFFF, _ptr(_c(frc, np.float32)), _ptr(vel),
+                                  _ptr(crd), dt, half_dt, exp_gamma, vmax)
+
+
+def leap_frog_twin(inv_mass, frc, vel, crd, dt, vmax=0.0):
+    _int_sigs().orc_leap_frog_twin(len(inv_mass), _ptr(_c(inv_mass, np.float32)),
+                                   _ptr(_c(frc, np.float32)), _ptr(vel), _ptr(crd), dt, vmax)
+
+
+def settle_frame(rAB, rAC, rBC, mA, mB, mC):
+    out = np.zeros(5, np.float32)
+    _int_sigs().orc_settle_frame(rAB, rAC, rBC, mA, mB, mC, _ptr(out))
+    return out
+
+
+def settle_remember(tri, crd, cell, rcell):
+    tri = _c(tri, np.int32)
+    m = tri.shape[0]
+    ba = np.zeros((m, 3), np.float32)
+    ca = np.zeros((m, 3), np.float32)
+    _int_sigs().orc_settle_remember(m, _ptr(tri), _ptr(_c(crd, np.float32)),
+                                    _ptr(_c(cell, np.float32)), _ptr(_c(rcell, np.float32)),
+                                    _ptr(ba), _ptr(ca))
+    return ba, ca
+
+
+def settle_twin(tri, geo, mass, crd, cell, rcell, ba, ca, dt, v_factor, x_factor, vel,
+                virial=True):
+    """SETTLE in place on crd and vel (float32 [n,3]); returns the int64
+    constraint virial sums (6) or None."""
+    tri = _c(tri, np.int32)
+    vir = np.zeros(6, np.int64) if virial else None
+    _int_sigs().orc_settle_twin(tri.shape[0], _ptr(tri), _ptr(_c(geo, np.float32)),
+                                _ptr(_c(mass, np.float32)), _ptr(crd), _ptr(_c(cell, np.float32)),
+                                _ptr(_c(rcell, np.float32)), _ptr(_c(ba, np.float32)),
+                                _ptr(_c(ca, np.float32)), dt, v_factor, x_factor, _ptr(vel),
+                                _ptr(vir))
+    return vir

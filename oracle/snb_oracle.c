@@ -1304,3 +1304,319 @@ ORC_EXPORT float orc_twin_coul_h(float x, float beta, float cutoff)
     return twin_coul_h(x, scale, tab);
 }
 ORC_EXPORT int64_t orc_to_fixed(float f) { return to_fixed(f); }
+
+/* ------------------------------------------------------------------------ */
+/* Integration: leap-frog (MD_core/nve.hpp:3-48), middle-scheme Langevin     */
+/* (thermostat/Middle_Langevin_MD.cpp:5-66) and SETTLE (constrain/settle.cpp) */
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC 2011): ten rounds with the
+ * published multipliers 0xD2511F53 / 0xCD9E8D57 and Weyl increments
+ * 0x9E3779B9 / 0xBB67AE85.  Pinned by the Random123 known-answer vectors in
+ * tests/test_oracle_pins.py. */
+static void twin_philox(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+    for (int r = 0; r < 10; r++)
+    {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+static inline float twin_uniform01(uint32_t x)
+{
+    return (float)(x >> 8) * 5.9604644775390625e-08f + 2.98023223876953125e-08f;
+}
+
+/* ln u, u in (0, 1]: u = m 2^e, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
+ * s = (m - 1)/(m + 1), odd series to s^9 */
+static inline float twin_log_unit(float u)
+{
+    uint32_t b = bits_from_f(u);
+    int e = (int)(b >> 23) - 127;
+    float m = f_from_bits((b & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 1.41421356f)
+    {
+        m = m * 0.5f;
+        e += 1;
+    }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p = 1.0f / 9.0f;
+    p = fmaf(p, s2, 1.0f / 7.0f);
+    p = fmaf(p, s2, 0.2f);
+    p = fmaf(p, s2, 1.0f / 3.0f);
+    p = fmaf(p, s2, 1.0f);
+    return fmaf((float)e, 0.693147182f, (2.0f * s) * p);
+}
+
+/* sin, cos of 2 pi u: reduction to +-pi/4 by quarter turns, Taylor to a^9 */
+static inline void twin_sincos_2pi(float u, float* so, float* co)
+{
+    float x = u - rintf(u);
+    float jf = rintf(4.0f * x);
+    float a = (x - 0.25f * jf) * 6.28318548f;
+    float a2 = a * a;
+    float sp = 1.0f / 362880.0f;
+    sp = fmaf(sp, a2, -1.0f / 5040.0f);
+    sp = fmaf(sp, a2, 1.0f / 120.0f);
+    sp = fmaf(sp, a2, -1.0f / 6.0f);
+    float sa = fmaf(a * a2, sp, a);
+    float cp = 1.0f / 40320.0f;
+    cp = fmaf(cp, a2, -1.0f / 720.0f);
+    cp = fmaf(cp, a2, 1.0f / 24.0f);
+    cp = fmaf(cp, a2, -0.5f);
+    float ca = fmaf(a2, cp, 1.0f);
+    int j = (int)jf & 3;
+    float s = sa, c = ca;
+    if (j == 1) { s = ca; c = -sa; }
+    else if (j == 2) { s = -sa; c = -ca; }
+    else if (j == 3) { s = -ca; c = sa; }
+    *so = s;
+    *co = c;
+}
+
+static inline void twin_box_muller(uint32_t x0, uint32_t x1, float* n0, float* n1)
+{
+    float r = sqrtf(-2.0f * twin_log_unit(twin_uniform01(x0)));
+    float s, c;
+    twin_sincos_2pi(twin_uniform01(x1), &s, &c);
+    *n0 = r * c;
+    *n1 = r * s;
+}
+
+#define TWIN_LANGEVIN_KEY1 0x53504F4Eu
+
+ORC_EXPORT void orc_philox4x32_10(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out)
+{
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    twin_philox(c, k0, k1);
+    for (int k = 0; k < 4; k++) out[k] = c[k];
+}
+
+/* normal numbers the thermostat draws: for atom i at stream position c, the
+ * four Box-Muller outputs of Philox({i, c, 0, 0}, {seed, key1}) */
+ORC_EXPORT void orc_langevin_normals(int n, uint32_t seed, const uint32_t* counter, float* out4)
+{
+    for (int i = 0; i < n; i++)
+    {
+        uint32_t c[4] = {(uint32_t)i, counter[i], 0u, 0u};
+        twin_philox(c, seed, TWIN_LANGEVIN_KEY1);
+        twin_box_muller(c[0], c[1], out4 + 4 * i, out4 + 4 * i + 1);
+        twin_box_muller(c[2], c[3], out4 + 4 * i + 2, out4 + 4 * i + 3);
+    }
+}
+
+static inline void twin_cap_speed(float* v, float vmax)
+{
+    float v2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    float f = v2 > 0.0f ? fminf(1.0f, vmax * twin_rsqrtf(v2)) : 1.0f;
+    for (int k = 0; k < 3; k++) v[k] = f * v[k];
+}
+
+/* One middle-scheme Langevin step (counter[] advanced), or with seed < 0 a
+ * leap-frog step with inv_mass and dt only. */
+ORC_EXPORT void orc_langevin_twin(int n, const float* inv_mass, const float* sqrt_mass,
+                                  uint32_t* counter, uint32_t seed, const float* frc, float* vel,
+                                  float* crd, float dt, float half_dt, float exp_gamma,
+                                  float vmax)
+{
+    for (int i = 0; i < n; i++)
+    {
+        uint32_t c[4] = {(uint32_t)i, counter[i], 0u, 0u};
+        counter[i] = c[1] + 1u;
+        twin_philox(c, seed, TWIN_LANGEVIN_KEY1);
+        float r[4];
+        twin_box_muller(c[0], c[1], r, r + 1);
+        twin_box_muller(c[2], c[3], r + 2, r + 3);
+        float v[3], x1[3];
+        for (int k = 0; k < 3; k++)
+        {
+            float a = inv_mass[i] * frc[3 * i + k];
+            v[k] = vel[3 * i + k] + dt * a;
+            x1[k] = crd[3 * i + k] + half_dt * v[k];
+            v[k] = exp_gamma * v[k] + sqrt_mass[i] * r[k];
+        }
+        if (vmax > 0.0f) twin_cap_speed(v, vmax);
+        for (int k = 0; k < 3; k++)
+        {
+            vel[3 * i + k] = v[k];
+            crd[3 * i + k] = x1[k] + half_dt * v[k];
+        }
+    }
+}
+
+ORC_EXPORT void orc_leap_frog_twin(int n, const float* inv_mass, const float* frc, float* vel,
+                                   float* crd, float dt, float vmax)
+{
+    for (int i = 0; i < n; i++)
+    {
+        float v[3];
+        for (int k = 0; k < 3; k++) v[k] = vel[3 * i + k] + dt * (inv_mass[i] * frc[3 * i + k]);
+        if (vmax > 0.0f) twin_cap_speed(v, vmax);
+        for (int k = 0; k < 3; k++)
+        {
+            vel[3 * i + k] = v[k];
+            crd[3 * i + k] = crd[3 * i + k] + dt * v[k];
+        }
+    }
+}
+
+/* Get_Rabcde_From_SSS (settle.cpp:50-98), float */
+ORC_EXPORT void orc_settle_frame(float rAB, float rAC, float rBC, float mA, float mB, float mC,
+                                 float* out)
+{
+    float mt = (mA + mB) + mC;
+    float Ax = 0.0f, Ay = 0.0f, Bx = -rAB, By = 0.0f;
+    float ct = ((rBC * rBC - rAC * rAC) - rAB * rAB) / ((2.0f * rAC) * rAB);
+    float Cx = rAC * ct;
+    float st = sqrtf(1.0f - ct * ct);
+    float Cy = rAC * st;
+    float Ox = (Bx * mB + Cx * mC) / mt;
+    float Oy = (Cy * mC) / mt;
+    Ax -= Ox; Ay -= Oy; Bx -= Ox; By -= Oy; Cx -= Ox; Cy -= Oy;
+    ct = 1.0f / sqrtf(1.0f + ((Ax * Ax) / Ay) / Ay);
+    st = (ct * Ax) / Ay;
+    float ra = Ax * st + Ay * ct;
+    float rc = Bx * ct - By * st;
+    float rb = Bx * st + By * ct;
+    float rd = Cx * ct - Cy * st;
+    float re = Cx * st + Cy * ct;
+    if (ra < 0.0f) { ra = -ra; rb = -rb; re = -re; }
+    out[0] = ra; out[1] = rb; out[2] = rc; out[3] = rd; out[4] = re;
+}
+
+static inline void v3sub(const float* a, const float* b, float* o)
+{
+    o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2];
+}
+static inline float v3dot(const float* a, const float* b)
+{
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static inline void v3cross(const float* a, const float* b, float* o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline void v3unit(float* a)
+{
+    float s = twin_rsqrtf(v3dot(a, a));
+    a[0] = s * a[0]; a[1] = s * a[1]; a[2] = s * a[2];
+}
+
+/* remember: ba[3m], ca[3m] = B - A, C - A (minimum image) */
+ORC_EXPORT void orc_settle_remember(int m, const int* tri, const float* crd, const float* cell,
+                                    const float* rcell, float* ba, float* ca)
+{
+    for (int t = 0; t < m; t++)
+    {
+        const int A = tri[3 * t], B = tri[3 * t + 1], C = tri[3 * t + 2];
+        pbc_disp_f(crd + 3 * B, crd + 3 * A, cell, rcell, ba + 3 * t);
+        pbc_disp_f(crd + 3 * C, crd + 3 * A, cell, rcell, ca + 3 * t);
+    }
+}
+
+/* Float twin of the SETTLE kernel (settle.cpp:101-253): crd, vel updated;
+ * vir6 (+=, int64 at 2^32) gets the per-triangle constraint virial in FP64
+ * when vir6 != NULL. */
+ORC_EXPORT void orc_settle_twin(int m, const int* tri, const float* geo, const float* mass,
+                                float* crd, const float* cell, const float* rcell,
+                                const float* ba, const float* ca, float dt, float v_factor,
+                                float x_factor, float* vel, int64_t* vir6)
+{
+    for (int t = 0; t < m; t++)
+    {
+        const int A = tri[3 * t], B = tri[3 * t + 1], C = tri[3 * t + 2];
+        const float ra = geo[5 * t], rb = geo[5 * t + 1], rc = geo[5 * t + 2];
+        const float rd = geo[5 * t + 3], re = geo[5 * t + 4];
+        float rA[3] = {crd[3 * A], crd[3 * A + 1], crd[3 * A + 2]}, rB[3], rC[3];
+        pbc_disp_f(crd + 3 * B, rA, cell, rcell, rB);
+        pbc_disp_f(crd + 3 * C, rA, cell, rcell, rC);
+        const float mA = mass[A], mB = mass[B], mC = mass[C];
+        const float imt = 1.0f / ((mA + mB) + mC);
+        float rO[3], rA1[3], rB1[3], rC1[3];
+        for (int k = 0; k < 3; k++)
+        {
+            rO[k] = imt * (mB * rB[k] + mC * rC[k]) + rA[k];
+            rA1[k] = rA[k] - rO[k];
+            rB1[k] = rB[k] + rA1[k];
+            rC1[k] = rC[k] + rA1[k];
+        }
+        const float* rB0 = ba + 3 * t;
+        const float* rC0 = ca + 3 * t;
+        float ez[3], ex[3], ey[3];
+        v3cross(rB0, rC0, ez);
+        v3cross(rA1, ez, ex);
+        v3cross(ez, ex, ey);
+        v3unit(ex);
+        v3unit(ey);
+        v3unit(ez);
+        float b0x = v3dot(ex, rB0), b0y = v3dot(ey, rB0);
+        float c0x = v3dot(ex, rC0), c0y = v3dot(ey, rC0);
+        float a1z = v3dot(ez, rA1);
+        float b1x = v3dot(ex, rB1), b1y = v3dot(ey, rB1), b1z = v3dot(ez, rB1);
+        float c1x = v3dot(ex, rC1), c1y = v3dot(ey, rC1), c1z = v3dot(ez, rC1);
+        float sinphi = a1z / ra;
+        float cosphi = sqrtf(1.0f - sinphi * sinphi);
+        float sinpsi = ((b1z - c1z) - (rb - re) * sinphi) / ((rd - rc) * cosphi);
+        float cospsi = sqrtf(1.0f - sinpsi * sinpsi);
+        float a2y = ra * cosphi;
+        float b2x = rc * cospsi, b2y = rb * cosphi + (rc * sinpsi) * sinphi;
+        float c2x = rd * cospsi, c2y = re * cosphi + (rd * sinpsi) * sinphi;
+        float alpha = ((b2x * b0x + c2x * c0x) + b2y * b0y) + c2y * c0y;
+        float beta = (((-b2x) * b0y - c2x * c0y) + b2y * b0x) + c2y * c0x;
+        float gamma = ((b1y * b0x - b1x * b0y) + c1y * c0x) - c1x * c0y;
+        float ab = alpha * alpha + beta * beta;
+        float sintheta = (alpha * gamma - beta * sqrtf(ab - gamma * gamma)) / ab;
+        float costheta = sqrtf(1.0f - sintheta * sintheta);
+        float p3[3][3] = {{-a2y * sintheta, a2y * costheta, a1z},
+                          {b2x * costheta - b2y * sintheta, b2x * sintheta + b2y * costheta, b1z},
+                          {c2x * costheta - c2y * sintheta, c2x * sintheta + c2y * costheta, c1z}};
+        float r3[3][3];
+        for (int q = 0; q < 3; q++)
+            for (int k = 0; k < 3; k++)
+                r3[q][k] = (p3[q][0] * ex[k] + p3[q][1] * ey[k]) + p3[q][2] * ez[k];
+        const float kv = (v_factor / dt) / x_factor;
+        const float* r1[3] = {rA1, rB1, rC1};
+        const int at[3] = {A, B, C};
+        float d[3][3];
+        for (int q = 0; q < 3; q++)
+            for (int k = 0; k < 3; k++)
+            {
+                d[q][k] = r3[q][k] - r1[q][k];
+                vel[3 * at[q] + k] = vel[3 * at[q] + k] + kv * d[q][k];
+            }
+        if (vir6)
+        {
+            const float kf = ((1.0f / dt) / dt) / x_factor;
+            float fB[3], fC[3];
+            for (int k = 0; k < 3; k++)
+            {
+                fB[k] = (kf * mB) * d[1][k];
+                fC[k] = (kf * mC) * d[2][k];
+            }
+            double v[6] = {
+                (double)fB[0] * rB0[0] + (double)fC[0] * rC0[0],
+                ((double)fB[0] * rB0[1] + (double)fB[1] * rB0[0]) +
+                    ((double)fC[0] * rC0[1] + (double)fC[1] * rC0[0]),
+                (double)fB[1] * rB0[1] + (double)fC[1] * rC0[1],
+                ((double)fB[0] * rB0[2] + (double)fB[2] * rB0[0]) +
+                    ((double)fC[0] * rC0[2] + (double)fC[2] * rC0[0]),
+                ((double)fB[1] * rB0[2] + (double)fB[2] * rB0[1]) +
+                    ((double)fC[1] * rC0[2] + (double)fC[2] * rC0[1]),
+                (double)fB[2] * rB0[2] + (double)fC[2] * rC0[2]};
+            for (int k = 0; k < 6; k++) vir6[k] += to_fixed_d(v[k]);
+        }
+        for (int q = 0; q < 3; q++)
+            for (int k = 0; k < 3; k++) crd[3 * at[q] + k] = r3[q][k] + rO[k];
+    }
+}

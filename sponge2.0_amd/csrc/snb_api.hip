@@ -205,6 +205,64 @@ void free_nb14(SnbNb14& M)
     M.init = false;
 }
 
+void free_langevin(SnbLangevin& T)
+{
+    dfree(T.inv_mass);
+    dfree(T.sqrt_mass);
+    dfree(T.counter);
+    T.init = false;
+}
+
+void free_settle(SnbSettle& S)
+{
+    dfree(S.tri);
+    dfree(S.geo);
+    dfree(S.last_ba);
+    dfree(S.last_ca);
+    dfree(S.vir6);
+    S.count = 0;
+    S.init = false;
+}
+
+// Get_Rabcde_From_SSS (settle.cpp:50-98): the canonical frame of a triangle
+// with sides AB, AC, BC and masses mA, mB, mC (float, host).
+static void settle_frame(float rAB, float rAC, float rBC, float mA, float mB, float mC,
+                         float* out)
+{
+    const float mt = (mA + mB) + mC;
+    float Ax = 0.0f, Ay = 0.0f, Bx = -rAB, By = 0.0f;
+    float ct = ((rBC * rBC - rAC * rAC) - rAB * rAB) / ((2.0f * rAC) * rAB);
+    float Cx = rAC * ct;
+    float st = sqrtf(1.0f - ct * ct);
+    float Cy = rAC * st;
+    const float Ox = (Bx * mB + Cx * mC) / mt;
+    const float Oy = (Cy * mC) / mt;
+    Ax -= Ox;
+    Ay -= Oy;
+    Bx -= Ox;
+    By -= Oy;
+    Cx -= Ox;
+    Cy -= Oy;
+    ct = 1.0f / sqrtf(1.0f + ((Ax * Ax) / Ay) / Ay);
+    st = (ct * Ax) / Ay;
+    float ra = Ax * st + Ay * ct;
+    float rc = Bx * ct - By * st;
+    float rb = Bx * st + By * ct;
+    float rd = Cx * ct - Cy * st;
+    float re = Cx * st + Cy * ct;
+    if (ra < 0.0f)
+    {
+        ra = -ra;
+        rb = -rb;
+        re = -re;
+    }
+    out[0] = ra;
+    out[1] = rb;
+    out[2] = rc;
+    out[3] = rd;
+    out[4] = re;
+}
+
 void free_pme(SnbPme& P)
 {
     dfree(P.tw_x);
@@ -377,6 +435,8 @@ int snb_context_destroy(snb_context* c)
     dfree(c->fx);
     dfree(c->e_atom);
     free_nb14(c->nb14);
+    free_langevin(c->langevin);
+    free_settle(c->settle);
     dfree(c->vir_atom);
     dfree(c->satom);
     dfree(c->stype);
@@ -947,6 +1007,159 @@ int snb_nb14_energy(snb_context* c, double* lj14, double* ee14)
     if (lj14) *lj14 = snb_fixed_to_double(v[0]);
     if (ee14) *ee14 = snb_fixed_to_double(v[1]);
     return SNB_OK;
+}
+
+// -------------------------------------------------------------- integration
+int snb_leap_frog(snb_context* c, const float* d_inverse_mass, const snb_vector* d_frc,
+                  snb_vector* d_vel, snb_vector* d_crd, float dt, float max_velocity)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (c->n > 0 && (!d_inverse_mass || !d_frc || !d_vel || !d_crd)) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    return snb_launch_leap_frog(c, d_inverse_mass, d_frc, d_vel, d_crd, dt, max_velocity);
+}
+
+int snb_langevin_initial(snb_context* c, const float* h_mass, float target_temperature,
+                         float tau, int seed, float dt, float max_velocity)
+{
+    if (!c || (c->n > 0 && !h_mass) || !(target_temperature >= 0.0f) || !(tau > 0.0f) ||
+        !(dt > 0.0f))
+        return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SnbLangevin& T = c->langevin;
+    free_langevin(T);
+    const int n = c->n;
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    int rc = dalloc(&T.inv_mass, m);
+    if (!rc) rc = dalloc(&T.sqrt_mass, m);
+    if (!rc) rc = dalloc(&T.counter, m);
+    if (rc)
+    {
+        free_langevin(T);
+        return rc;
+    }
+    // Middle_Langevin_MD.cpp:172-245: friction 1/tau per ps, internal time
+    // unit 1/20.455 ps, kB 0.00198716 kcal/mol/K
+    const float time_unit = 20.455f, kb = 0.00198716f;
+    const float dti = dt * time_unit;
+    const float gamma = (1.0f / tau) / time_unit;
+    const float eg = expf(-gamma * dti);
+    const float sg = sqrtf((float)((1.0 - (double)(eg * eg)) * (double)target_temperature *
+                                   (double)kb));
+    std::vector<float> im(m, 0.0f), sm(m, 0.0f);
+    for (int i = 0; i < n; i++)
+    {
+        if (h_mass[i] != 0.0f)
+        {
+            im[i] = 1.0f / h_mass[i];
+            sm[i] = sg * sqrtf((float)(1.0 / (double)h_mass[i]));
+        }
+    }
+    SNB_CHECK(hipMemcpy(T.inv_mass, im.data(), sizeof(float) * m, hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemcpy(T.sqrt_mass, sm.data(), sizeof(float) * m, hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemset(T.counter, 0, sizeof(uint32_t) * m));
+    T.seed = (uint32_t)seed;
+    T.dt = dti;
+    T.half_dt = 0.5f * dti;
+    T.exp_gamma = eg;
+    T.max_velocity = max_velocity > 0.0f ? max_velocity : 0.0f;
+    T.init = true;
+    return SNB_OK;
+}
+
+int snb_langevin_iterate(snb_context* c, const snb_vector* d_frc, snb_vector* d_vel,
+                         snb_vector* d_acc, snb_vector* d_crd)
+{
+    (void)d_acc;
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->langevin.init) return SNB_ERR_NOT_INITIALIZED;
+    if (c->n > 0 && (!d_frc || !d_vel || !d_crd)) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    return snb_launch_langevin(c, d_frc, d_vel, d_crd);
+}
+
+int snb_langevin_parameters(snb_context* c, float* dt_internal, float* exp_gamma,
+                            float* v_factor, float* x_factor)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->langevin.init) return SNB_ERR_NOT_INITIALIZED;
+    const SnbLangevin& T = c->langevin;
+    if (dt_internal) *dt_internal = T.dt;
+    if (exp_gamma) *exp_gamma = T.exp_gamma;
+    // constrain.v_factor / x_factor as main.cpp:650-652 sets them
+    if (v_factor) *v_factor = T.exp_gamma;
+    if (x_factor) *x_factor = 0.5f * T.exp_gamma + 0.5f;
+    return SNB_OK;
+}
+
+int snb_settle_initial(snb_context* c, int triangle_numbers, const int* h_atom_A,
+                       const int* h_atom_B, const int* h_atom_C, const float* h_rAB,
+                       const float* h_rAC, const float* h_rBC, const float* h_mass)
+{
+    if (!c || triangle_numbers < 0) return SNB_ERR_ARGUMENT;
+    if (triangle_numbers > 0 &&
+        (!h_atom_A || !h_atom_B || !h_atom_C || !h_rAB || !h_rAC || !h_rBC || !h_mass))
+        return SNB_ERR_ARGUMENT;
+    for (int t = 0; t < triangle_numbers; t++)
+    {
+        const int a = h_atom_A[t], b = h_atom_B[t], d = h_atom_C[t];
+        if (a < 0 || a >= c->n || b < 0 || b >= c->n || d < 0 || d >= c->n || a == b ||
+            a == d || b == d)
+            return SNB_ERR_ARGUMENT;
+        if (!(h_rAB[t] > 0.0f) || !(h_rAC[t] > 0.0f) || !(h_rBC[t] > 0.0f)) return SNB_ERR_ARGUMENT;
+    }
+    DeviceGuard g(c->device);
+    SnbSettle& S = c->settle;
+    free_settle(S);
+    if (triangle_numbers == 0) return SNB_OK;  // not initialised: Do_SETTLE is a no-op
+    const size_t m = (size_t)triangle_numbers;
+    int rc = dalloc(&S.tri, m);
+    if (!rc) rc = dalloc(&S.geo, 5 * m);
+    if (!rc) rc = dalloc(&S.last_ba, m);
+    if (!rc) rc = dalloc(&S.last_ca, m);
+    if (!rc) rc = dalloc(&S.vir6, (size_t)6);
+    if (rc)
+    {
+        free_settle(S);
+        return rc;
+    }
+    std::vector<int4> tri(m);
+    std::vector<float> geo(5 * m);
+    for (size_t t = 0; t < m; t++)
+    {
+        tri[t] = make_int4(h_atom_A[t], h_atom_B[t], h_atom_C[t], 0);
+        settle_frame(h_rAB[t], h_rAC[t], h_rBC[t], h_mass[h_atom_A[t]], h_mass[h_atom_B[t]],
+                     h_mass[h_atom_C[t]], geo.data() + 5 * t);
+    }
+    SNB_CHECK(hipMemcpy(S.tri, tri.data(), sizeof(int4) * m, hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemcpy(S.geo, geo.data(), sizeof(float) * 5 * m, hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemset(S.vir6, 0, sizeof(int64_t) * 6));
+    S.count = triangle_numbers;
+    S.init = true;
+    return SNB_OK;
+}
+
+int snb_settle_remember(snb_context* c, const snb_vector* d_crd, snb_ltmatrix3 cell,
+                        snb_ltmatrix3 rcell)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->settle.init) return SNB_OK;
+    if (!d_crd) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    return snb_launch_settle_remember(c, d_crd, snb_box(cell, rcell));
+}
+
+int snb_settle_do(snb_context* c, const float* d_mass, snb_vector* d_crd, snb_ltmatrix3 cell,
+                  snb_ltmatrix3 rcell, snb_vector* d_vel, int need_pressure,
+                  snb_ltmatrix3* d_stress, float dt, float v_factor, float x_factor)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->settle.init) return SNB_OK;
+    if (!d_mass || !d_crd || !d_vel || !(dt > 0.0f) || !(x_factor > 0.0f))
+        return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    return snb_launch_settle(c, d_mass, d_crd, snb_box(cell, rcell), d_vel, need_pressure,
+                             d_stress, dt, v_factor, x_factor);
 }
 
 // ---------------------------------------------------------------------- PME

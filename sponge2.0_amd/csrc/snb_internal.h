@@ -119,6 +119,32 @@ struct SnbNb14
     int64_t* pair_e = nullptr;  // 2 * count, fixed point: LJ | Coulomb of each pair
 };
 
+// Middle-scheme Langevin thermostat (MIDDLE_Langevin_INFORMATION); times in
+// the internal unit (ps * 20.455)
+#define SNB_LANGEVIN_KEY1 0x53504F4Eu  // second Philox key word ("SPON")
+struct SnbLangevin
+{
+    bool init = false;
+    float* inv_mass = nullptr;   // 1/m (0 for m = 0)
+    float* sqrt_mass = nullptr;  // sqrt(kT (1 - e^{-2 gamma dt}) / m) (0 for m = 0)
+    uint32_t* counter = nullptr; // per-atom random-stream position
+    uint32_t seed = 0;
+    float dt = 0.0f, half_dt = 0.0f, exp_gamma = 1.0f, max_velocity = 0.0f;
+};
+
+// SETTLE rigid triangles: atoms (A, B, C) and the canonical frame
+// {ra, rb, rc, rd, re} per triangle
+struct SnbSettle
+{
+    bool init = false;
+    int count = 0;
+    int4* tri = nullptr;
+    float* geo = nullptr;           // 5 per triangle
+    snb_vector* last_ba = nullptr;  // previous step's B - A
+    snb_vector* last_ca = nullptr;  // previous step's C - A
+    int64_t* vir6 = nullptr;        // summed constraint virial, fixed point
+};
+
 struct SnbPme
 {
     bool init = false;
@@ -211,6 +237,8 @@ struct snb_context
     SnbNl nl;
     SnbLj lj;
     SnbNb14 nb14;
+    SnbLangevin langevin;
+    SnbSettle settle;
     SnbPme pme;
     SnbProf prof;
 };
@@ -281,6 +309,14 @@ int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge
                     int need_energy, int need_virial, int64_t* fx);
 int snb_launch_nb14(snb_context* ctx, const snb_vector* crd, const float* charge,
                     const SnbBox& box, int need_energy, int need_virial);
+int snb_launch_leap_frog(snb_context* ctx, const float* inv_mass, const snb_vector* frc,
+                         snb_vector* vel, snb_vector* crd, float dt, float vmax);
+int snb_launch_langevin(snb_context* ctx, const snb_vector* frc, snb_vector* vel,
+                        snb_vector* crd);
+int snb_launch_settle_remember(snb_context* ctx, const snb_vector* crd, const SnbBox& box);
+int snb_launch_settle(snb_context* ctx, const float* mass, snb_vector* crd, const SnbBox& box,
+                      snb_vector* vel, int need_pressure, snb_ltmatrix3* stress, float dt,
+                      float v_factor, float x_factor);
 
 // PME (snb_pme.hip)
 int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box);

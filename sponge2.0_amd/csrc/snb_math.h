@@ -185,6 +185,109 @@ SNB_HD int64_t snb_to_fixed_d(double d)
     return (int64_t)__builtin_bit_cast(uint64_t, v) - (int64_t)0x4338000000000000LL;
 }
 
+// ---------------------------------------------------------------------------
+// Random numbers for the Langevin thermostat
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC 2011): ten rounds of two 32x32->64 multiplications with the
+// published multipliers and Weyl key increments.
+SNB_HD void snb_philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+    for (int r = 0; r < 10; r++)
+    {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// uniform in (0, 1): the top 24 bits plus half a step, exact in float
+SNB_HD float snb_uniform01(uint32_t x)
+{
+    return (float)(x >> 8) * 5.9604644775390625e-08f + 2.98023223876953125e-08f;
+}
+
+// ln(u) for u in (0, 1]: u = m 2^e with m in [sqrt(1/2), sqrt(2)), then
+// ln m = 2 atanh(s) with s = (m - 1)/(m + 1), |s| < 0.1716, as the odd series
+// 2 s (1 + s^2/3 + s^4/5 + s^6/7 + s^8/9) (truncation below 1e-9).
+SNB_HD float snb_log_unit(float u)
+{
+    uint32_t b = snb_bits_from_f(u);
+    int e = (int)(b >> 23) - 127;
+    float m = snb_f_from_bits((b & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 1.41421356f)
+    {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p = 1.0f / 9.0f;
+    p = __builtin_fmaf(p, s2, 1.0f / 7.0f);
+    p = __builtin_fmaf(p, s2, 0.2f);
+    p = __builtin_fmaf(p, s2, 1.0f / 3.0f);
+    p = __builtin_fmaf(p, s2, 1.0f);
+    return __builtin_fmaf((float)e, 0.693147182f, (2.0f * s) * p);
+}
+
+// sin and cos of 2 pi u for u in (0, 1): x = u - rint(u) in [-1/2, 1/2], a
+// quarter-turn count j = rint(4x) and a remainder angle a = 2 pi (x - j/4)
+// in [-pi/4, pi/4], Taylor polynomials to a^9 / a^8 (error below 3e-9), then
+// the quarter turns.
+SNB_HD void snb_sincos_2pi(float u, float* s_out, float* c_out)
+{
+    const float x = u - __builtin_rintf(u);
+    const float jf = __builtin_rintf(4.0f * x);
+    const float a = (x - 0.25f * jf) * 6.28318548f;
+    const float a2 = a * a;
+    float sp = 1.0f / 362880.0f;
+    sp = __builtin_fmaf(sp, a2, -1.0f / 5040.0f);
+    sp = __builtin_fmaf(sp, a2, 1.0f / 120.0f);
+    sp = __builtin_fmaf(sp, a2, -1.0f / 6.0f);
+    const float sa = __builtin_fmaf(a * a2, sp, a);
+    float cp = 1.0f / 40320.0f;
+    cp = __builtin_fmaf(cp, a2, -1.0f / 720.0f);
+    cp = __builtin_fmaf(cp, a2, 1.0f / 24.0f);
+    cp = __builtin_fmaf(cp, a2, -0.5f);
+    const float ca = __builtin_fmaf(a2, cp, 1.0f);
+    const int j = (int)jf & 3;  // rotations by j quarter turns (j in -2..2)
+    float s = sa, c = ca;
+    if (j == 1)
+    {
+        s = ca;
+        c = -sa;
+    }
+    else if (j == 2)
+    {
+        s = -sa;
+        c = -ca;
+    }
+    else if (j == 3)
+    {
+        s = -ca;
+        c = sa;
+    }
+    *s_out = s;
+    *c_out = c;
+}
+
+// Two standard normal numbers from two 32-bit words (Box-Muller).
+SNB_HD void snb_box_muller(uint32_t x0, uint32_t x1, float* n0, float* n1)
+{
+    const float r = __builtin_sqrtf(-2.0f * snb_log_unit(snb_uniform01(x0)));
+    float s, c;
+    snb_sincos_2pi(snb_uniform01(x1), &s, &c);
+    *n0 = r * c;
+    *n1 = r * s;
+}
+
 // Triangular LJ type-pair index (Lennard_Jones_force.h:18-27).
 SNB_HD int snb_lj_pair_index(int a, int b)
 {

@@ -1,0 +1,333 @@
+// snb_md.hip -- integration on the device: leap-frog (stands in for
+// MD_INFORMATION::NVE_iteration::Leap_Frog, MD_core/nve.hpp:3-48), the
+// middle-scheme Langevin leap-frog (MIDDLE_Langevin_INFORMATION,
+// thermostat/Middle_Langevin_MD.cpp:5-66, :305-331) and SETTLE for rigid
+// triangles (constrain/settle.cpp:50-215, :573-697).
+//
+// The float updates keep the reference's operation order; there is no
+// contraction (-ffp-contract=off), so the CPU twin reproduces every bit.  The
+// Langevin noise comes from the library's own counter-based generator
+// (Philox4x32-10 with Box-Muller, snb_math.h): atom i at its c-th step draws
+// from counter {i, c, 0, 0}, with one counter per atom kept on the device, so
+// a replayed graph still advances the stream.
+#include "snb_internal.h"
+
+// |v| capped at vmax (Make_Vector_Not_Exceed_Value, utils/vector.hpp:207-212),
+// with 1/|v| from the library's rsqrt
+__device__ __forceinline__ void cap_speed(float& vx, float& vy, float& vz, float vmax)
+{
+    const float v2 = (vx * vx + vy * vy) + vz * vz;
+    const float f = v2 > 0.0f ? fminf(1.0f, vmax * snb_rsqrtf(v2)) : 1.0f;
+    vx = f * vx;
+    vy = f * vy;
+    vz = f * vz;
+}
+
+// NVE leap-frog: v += dt inv_m f, x += dt v.
+__global__ void __launch_bounds__(256) k_leap_frog(int n, const float* __restrict__ inv_mass,
+                                                   const snb_vector* __restrict__ frc,
+                                                   snb_vector* __restrict__ vel,
+                                                   snb_vector* __restrict__ crd, float dt,
+                                                   float vmax)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float im = inv_mass[i];
+    const snb_vector f = frc[i];
+    snb_vector v = vel[i], x = crd[i];
+    v.x = v.x + dt * (im * f.x);
+    v.y = v.y + dt * (im * f.y);
+    v.z = v.z + dt * (im * f.z);
+    if (vmax > 0.0f) cap_speed(v.x, v.y, v.z, vmax);
+    x.x = x.x + dt * v.x;
+    x.y = x.y + dt * v.y;
+    x.z = x.z + dt * v.z;
+    vel[i] = v;
+    crd[i] = x;
+}
+
+// Middle scheme (Liu Jian): v1 = v + dt inv_m f; x1 = x + dt/2 v1;
+// v2 = e^{-gamma dt} v1 + sqrt(kT (1 - e^{-2 gamma dt}) / m) R; x2 = x1 + dt/2 v2.
+__global__ void __launch_bounds__(256) k_langevin(int n, const float* __restrict__ inv_mass,
+                                                  const float* __restrict__ sqrt_mass,
+                                                  uint32_t* __restrict__ counter, uint32_t seed,
+                                                  const snb_vector* __restrict__ frc,
+                                                  snb_vector* __restrict__ vel,
+                                                  snb_vector* __restrict__ crd, float dt,
+                                                  float half_dt, float exp_gamma, float vmax)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c[4] = {(uint32_t)i, counter[i], 0u, 0u};
+    counter[i] = c[1] + 1u;
+    snb_philox4x32_10(c, seed, SNB_LANGEVIN_KEY1);
+    float r0, r1, r2, r3;
+    snb_box_muller(c[0], c[1], &r0, &r1);
+    snb_box_muller(c[2], c[3], &r2, &r3);
+    const float im = inv_mass[i], sm = sqrt_mass[i];
+    const snb_vector f = frc[i];
+    snb_vector v = vel[i], x = crd[i];
+    float ax = im * f.x, ay = im * f.y, az = im * f.z;
+    float vx = v.x + dt * ax, vy = v.y + dt * ay, vz = v.z + dt * az;
+    const float xx = x.x + half_dt * vx, xy = x.y + half_dt * vy, xz = x.z + half_dt * vz;
+    vx = exp_gamma * vx + sm * r0;
+    vy = exp_gamma * vy + sm * r1;
+    vz = exp_gamma * vz + sm * r2;
+    if (vmax > 0.0f) cap_speed(vx, vy, vz, vmax);
+    v.x = vx;
+    v.y = vy;
+    v.z = vz;
+    x.x = xx + half_dt * vx;
+    x.y = xy + half_dt * vy;
+    x.z = xz + half_dt * vz;
+    vel[i] = v;
+    crd[i] = x;
+}
+
+// ---------------------------------------------------------------------------
+// SETTLE
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ snb_vector vsub(snb_vector a, snb_vector b)
+{
+    return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+__device__ __forceinline__ snb_vector vadd(snb_vector a, snb_vector b)
+{
+    return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+__device__ __forceinline__ snb_vector vscale(float s, snb_vector a)
+{
+    return {s * a.x, s * a.y, s * a.z};
+}
+__device__ __forceinline__ float vdot(snb_vector a, snb_vector b)
+{
+    return (a.x * b.x + a.y * b.y) + a.z * b.z;
+}
+__device__ __forceinline__ snb_vector vcross(snb_vector a, snb_vector b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ snb_vector vunit(snb_vector a)
+{
+    return vscale(snb_rsqrtf(vdot(a, a)), a);
+}
+
+__device__ __forceinline__ snb_vector settle_disp(snb_vector a, snb_vector b, const SnbBox& box)
+{
+    // Get_Periodic_Displacement(a, b): a - b, minimum image
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    float sx = (dx * box.rc[0] + dy * box.rc[1]) + dz * box.rc[3];
+    float sy = dy * box.rc[2] + dz * box.rc[4];
+    float sz = dz * box.rc[5];
+    float kx = floorf(sx + 0.5f), ky = floorf(sy + 0.5f), kz = floorf(sz + 0.5f);
+    dx = dx - ((kx * box.c[0] + ky * box.c[1]) + kz * box.c[3]);
+    dy = dy - (ky * box.c[2] + kz * box.c[4]);
+    dz = dz - kz * box.c[5];
+    return {dx, dy, dz};
+}
+
+// remember_triangle_BA_CA (settle.cpp:3-23)
+__global__ void __launch_bounds__(256) k_settle_remember(int m, const int4* __restrict__ tri,
+                                                         const snb_vector* __restrict__ crd,
+                                                         SnbBox box,
+                                                         snb_vector* __restrict__ last_ba,
+                                                         snb_vector* __restrict__ last_ca)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int4 a = tri[t];
+    const snb_vector rA = crd[a.x];
+    last_ba[t] = settle_disp(crd[a.y], rA, box);
+    last_ca[t] = settle_disp(crd[a.z], rA, box);
+}
+
+// One rigid triangle per thread (settle_triangle + SETTLE_DO_TRIANGLE,
+// settle.cpp:101-253, after Miyamoto & Kollman, J. Comput. Chem. 13, 952
+// (1992)).  rB0, rC0: the previous step's B - A and C - A; rA1, rB1, rC1:
+// this step's positions about the centre of mass.  The triangle's canonical
+// frame {ra, rb, rc, rd, re} puts A at (0, ra), B at (rc, rb), C at (rd, re)
+// with the centre of mass at the origin.  The per-triangle virial (FP64, to
+// fixed point) is summed into vir6 when need_virial.
+__global__ void __launch_bounds__(256) k_settle(int m, const int4* __restrict__ tri,
+                                                const float* __restrict__ geo,
+                                                const float* __restrict__ mass,
+                                                snb_vector* __restrict__ crd, SnbBox box,
+                                                const snb_vector* __restrict__ last_ba,
+                                                const snb_vector* __restrict__ last_ca,
+                                                float dt, float v_factor, float x_factor,
+                                                snb_vector* __restrict__ vel, int need_virial,
+                                                int64_t* __restrict__ vir6)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t w[6] = {0, 0, 0, 0, 0, 0};
+    if (t < m)
+    {
+        const int4 a = tri[t];
+        const float ra = geo[5 * t], rb = geo[5 * t + 1], rc = geo[5 * t + 2];
+        const float rd = geo[5 * t + 3], re = geo[5 * t + 4];
+        snb_vector rA = crd[a.x];
+        snb_vector rB = settle_disp(crd[a.y], rA, box);
+        snb_vector rC = settle_disp(crd[a.z], rA, box);
+        const float mA = mass[a.x], mB = mass[a.y], mC = mass[a.z];
+        // centre of mass rO = (mB rB + mC rC)/(mA + mB + mC) + rA
+        const float inv_mt = 1.0f / ((mA + mB) + mC);
+        const snb_vector rO = vadd(vscale(inv_mt, vadd(vscale(mB, rB), vscale(mC, rC))), rA);
+        const snb_vector rA1 = vsub(rA, rO);
+        const snb_vector rB1 = vadd(rB, rA1);
+        const snb_vector rC1 = vadd(rC, rA1);
+        const snb_vector rB0 = last_ba[t], rC0 = last_ca[t];
+        // step 0: frame from the previous plane and this step's A
+        snb_vector ez = vcross(rB0, rC0);
+        snb_vector ex = vcross(rA1, ez);
+        snb_vector ey = vcross(ez, ex);
+        ex = vunit(ex);
+        ey = vunit(ey);
+        ez = vunit(ez);
+        // step 1: projections
+        const float b0x = vdot(ex, rB0), b0y = vdot(ey, rB0);
+        const float c0x = vdot(ex, rC0), c0y = vdot(ey, rC0);
+        const float a1z = vdot(ez, rA1);
+        const float b1x = vdot(ex, rB1), b1y = vdot(ey, rB1), b1z = vdot(ez, rB1);
+        const float c1x = vdot(ex, rC1), c1y = vdot(ey, rC1), c1z = vdot(ez, rC1);
+        // step 2: rotations psi (about y) and phi (about x)
+        const float sinphi = a1z / ra;
+        const float cosphi = sqrtf(1.0f - sinphi * sinphi);
+        const float sinpsi = ((b1z - c1z) - (rb - re) * sinphi) / ((rd - rc) * cosphi);
+        const float cospsi = sqrtf(1.0f - sinpsi * sinpsi);
+        const float a2y = ra * cosphi;
+        const float b2x = rc * cospsi, b2y = rb * cosphi + (rc * sinpsi) * sinphi;
+        const float c2x = rd * cospsi, c2y = re * cosphi + (rd * sinpsi) * sinphi;
+        // step 3: alpha, beta, gamma
+        const float alpha = ((b2x * b0x + c2x * c0x) + b2y * b0y) + c2y * c0y;
+        const float beta = (((-b2x) * b0y - c2x * c0y) + b2y * b0x) + c2y * c0x;
+        const float gamma = ((b1y * b0x - b1x * b0y) + c1y * c0x) - c1x * c0y;
+        // step 4: rotation theta about z
+        const float ab = alpha * alpha + beta * beta;
+        const float sintheta = (alpha * gamma - beta * sqrtf(ab - gamma * gamma)) / ab;
+        const float costheta = sqrtf(1.0f - sintheta * sintheta);
+        const float a3x = -a2y * sintheta, a3y = a2y * costheta, a3z = a1z;
+        const float b3x = b2x * costheta - b2y * sintheta;
+        const float b3y = b2x * sintheta + b2y * costheta, b3z = b1z;
+        const float c3x = c2x * costheta - c2y * sintheta;
+        const float c3y = c2x * sintheta + c2y * costheta, c3z = c1z;
+        // step 5: back to the lab frame
+        const snb_vector rA3 = {(a3x * ex.x + a3y * ey.x) + a3z * ez.x,
+                                (a3x * ex.y + a3y * ey.y) + a3z * ez.y,
+                                (a3x * ex.z + a3y * ey.z) + a3z * ez.z};
+        const snb_vector rB3 = {(b3x * ex.x + b3y * ey.x) + b3z * ez.x,
+                                (b3x * ex.y + b3y * ey.y) + b3z * ez.y,
+                                (b3x * ex.z + b3y * ey.z) + b3z * ez.z};
+        const snb_vector rC3 = {(c3x * ex.x + c3y * ey.x) + c3z * ez.x,
+                                (c3x * ex.y + c3y * ey.y) + c3z * ez.y,
+                                (c3x * ex.z + c3y * ey.z) + c3z * ez.z};
+        // step 6: velocity change and virial of the constraint forces
+        const float kv = (v_factor / dt) / x_factor;
+        const snb_vector dA = vsub(rA3, rA1), dB = vsub(rB3, rB1), dC = vsub(rC3, rC1);
+        vel[a.x] = vadd(vel[a.x], vscale(kv, dA));
+        vel[a.y] = vadd(vel[a.y], vscale(kv, dB));
+        vel[a.z] = vadd(vel[a.z], vscale(kv, dC));
+        if (need_virial)
+        {
+            const float kf = ((1.0f / dt) / dt) / x_factor;
+            const snb_vector fB = vscale(kf * mB, dB), fC = vscale(kf * mC, dC);
+            // Get_Virial_From_Force_Dis(fB, rB0) + (fC, rC0), in FP64
+            const double v[6] = {
+                (double)fB.x * rB0.x + (double)fC.x * rC0.x,
+                ((double)fB.x * rB0.y + (double)fB.y * rB0.x) +
+                    ((double)fC.x * rC0.y + (double)fC.y * rC0.x),
+                (double)fB.y * rB0.y + (double)fC.y * rC0.y,
+                ((double)fB.x * rB0.z + (double)fB.z * rB0.x) +
+                    ((double)fC.x * rC0.z + (double)fC.z * rC0.x),
+                ((double)fB.y * rB0.z + (double)fB.z * rB0.y) +
+                    ((double)fC.y * rC0.z + (double)fC.z * rC0.y),
+                (double)fB.z * rB0.z + (double)fC.z * rC0.z};
+            for (int k = 0; k < 6; k++) w[k] = snb_to_fixed_d(v[k]);
+        }
+        crd[a.x] = vadd(rA3, rO);
+        crd[a.y] = vadd(rB3, rO);
+        crd[a.z] = vadd(rC3, rO);
+    }
+    if (need_virial)
+    {
+        for (int k = 0; k < 6; k++)
+        {
+            int64_t s = w[k];
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            if ((threadIdx.x & 63) == 0 && s != 0)
+                atomicAdd((unsigned long long*)&vir6[k], (unsigned long long)s);
+        }
+    }
+}
+
+// stress += (1/V) * summed virial (Sum_Virial_Tensor_To_Stress, settle.cpp:307-340);
+// clears the sums
+__global__ void k_settle_stress(int64_t* __restrict__ vir6, float inv_volume,
+                                snb_ltmatrix3* __restrict__ stress)
+{
+    if (threadIdx.x != 0) return;
+    snb_ltmatrix3 s = stress[0];
+    s.a11 += inv_volume * snb_fixed_to_float(vir6[0]);
+    s.a21 += inv_volume * snb_fixed_to_float(vir6[1]);
+    s.a22 += inv_volume * snb_fixed_to_float(vir6[2]);
+    s.a31 += inv_volume * snb_fixed_to_float(vir6[3]);
+    s.a32 += inv_volume * snb_fixed_to_float(vir6[4]);
+    s.a33 += inv_volume * snb_fixed_to_float(vir6[5]);
+    stress[0] = s;
+    for (int k = 0; k < 6; k++) vir6[k] = 0;
+}
+
+int snb_launch_leap_frog(snb_context* ctx, const float* inv_mass, const snb_vector* frc,
+                         snb_vector* vel, snb_vector* crd, float dt, float vmax)
+{
+    const int n = ctx->n;
+    if (n <= 0) return SNB_OK;
+    hipLaunchKernelGGL(k_leap_frog, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
+                       inv_mass, frc, vel, crd, dt, vmax);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
+int snb_launch_langevin(snb_context* ctx, const snb_vector* frc, snb_vector* vel,
+                        snb_vector* crd)
+{
+    const SnbLangevin& T = ctx->langevin;
+    const int n = ctx->n;
+    if (n <= 0) return SNB_OK;
+    hipLaunchKernelGGL(k_langevin, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
+                       T.inv_mass, T.sqrt_mass, T.counter, T.seed, frc, vel, crd, T.dt,
+                       T.half_dt, T.exp_gamma, T.max_velocity);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
+int snb_launch_settle_remember(snb_context* ctx, const snb_vector* crd, const SnbBox& box)
+{
+    const SnbSettle& S = ctx->settle;
+    if (S.count <= 0) return SNB_OK;
+    hipLaunchKernelGGL(k_settle_remember, dim3((S.count + 255) / 256), dim3(256), 0,
+                       ctx->stream, S.count, S.tri, crd, box, S.last_ba, S.last_ca);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
+int snb_launch_settle(snb_context* ctx, const float* mass, snb_vector* crd, const SnbBox& box,
+                      snb_vector* vel, int need_pressure, snb_ltmatrix3* stress, float dt,
+                      float v_factor, float x_factor)
+{
+    const SnbSettle& S = ctx->settle;
+    if (S.count <= 0) return SNB_OK;
+    const int want = need_pressure && stress ? 1 : 0;
+    hipLaunchKernelGGL(k_settle, dim3((S.count + 255) / 256), dim3(256), 0, ctx->stream,
+                       S.count, S.tri, S.geo, mass, crd, box, S.last_ba, S.last_ca, dt,
+                       v_factor, x_factor, vel, want, S.vir6);
+    SNB_CHECK(hipGetLastError());
+    if (want)
+    {
+        // rcell a11 a22 a33 is 1/V for the lower-triangular cell
+        const float inv_volume = (box.rc[0] * box.rc[2]) * box.rc[5];
+        hipLaunchKernelGGL(k_settle_stress, dim3(1), dim3(64), 0, ctx->stream, S.vir6,
+                           inv_volume, stress);
+        SNB_CHECK(hipGetLastError());
+    }
+    return SNB_OK;
+}

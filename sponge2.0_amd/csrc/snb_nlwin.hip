@@ -233,9 +233,12 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
                     float ddx = qi.x - qj.x, ddy = qi.y - qj.y, ddz = qi.z - qj.z;
                     if (ORTHO)
                     {
-                        float kx = floorf(ddx * box.rc[0] + 0.5f);
-                        float ky = floorf(ddy * box.rc[2] + 0.5f);
-                        float kz = floorf(ddz * box.rc[5] + 0.5f);
+                        // rint(s) == floorf(s + 0.5) for every pair within
+                        // rc + skin (s lies within (rc + skin)/L < 1/2 of an
+                        // integer); they differ only half a box away
+                        float kx = rintf(ddx * box.rc[0]);
+                        float ky = rintf(ddy * box.rc[2]);
+                        float kz = rintf(ddz * box.rc[5]);
                         ddx = ddx - kx * box.c[0];
                         ddy = ddy - ky * box.c[2];
                         ddz = ddz - kz * box.c[5];
@@ -255,11 +258,13 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
                     ok = d2 < cs2;
                     nearp = d2 < near2;
                     // the lower id's list decides; its id window rules out
-                    // most pairs without touching the list
+                    // most pairs without touching the list, and decides alone
+                    // for a list of consecutive ids (first stored >= 0)
                     const int2 rl = id_i < id_j ? rng_i : cex[t];
                     const int hi = id_i < id_j ? id_j : id_i;
-                    if (ok && hi >= rl.x && hi <= rl.y)
-                        ok = !excluded_pair(id_i, id_j, ex_start, ex_list, ex_num);
+                    const int lo_id = rl.x >= 0 ? rl.x : ~rl.x;
+                    if (ok && hi >= lo_id && hi <= rl.y)
+                        ok = rl.x < 0 && !excluded_pair(id_i, id_j, ex_start, ex_list, ex_num);
                 }
                 // near pairs (within rc + skin/4, the ones that can come into
                 // range before the next rebuild) from the front, the rest from
@@ -503,6 +508,12 @@ __global__ void k_ex_range(const int* __restrict__ need, int n, const int* __res
     {
         int es = ex_start[i];
         r = make_int2(ex_list[es], ex_list[es + en - 1]);
+        // a list that is exactly first, first + 1, ..., last holds every id of
+        // its window, so the window test alone decides; other lists are
+        // marked (first stored as ~first) and searched
+        bool run = r.y - r.x == en - 1;
+        for (int t = 1; run && t < en; t++) run = ex_list[es + t] == r.x + t;
+        if (!run) r.x = ~r.x;
     }
     ex_rng[i] = r;
 }

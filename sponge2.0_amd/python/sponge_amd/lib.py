@@ -73,6 +73,13 @@ _SIGS = {
     "snb_nb14_force": ([_P, _P, _P, LTMatrix3, LTMatrix3, _P, _I, _P, _I, _P], _I),
     "snb_nb14_energy": ([_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
                         _I),
+    "snb_leap_frog": ([_P, _P, _P, _P, _P, _F, _F], _I),
+    "snb_langevin_initial": ([_P, _P, _F, _F, _I, _F, _F], _I),
+    "snb_langevin_iterate": ([_P, _P, _P, _P, _P], _I),
+    "snb_langevin_parameters": ([_P] + [ctypes.POINTER(_F)] * 4, _I),
+    "snb_settle_initial": ([_P, _I, _P, _P, _P, _P, _P, _P, _P], _I),
+    "snb_settle_remember": ([_P, _P, LTMatrix3, LTMatrix3], _I),
+    "snb_settle_do": ([_P, _P, _P, LTMatrix3, LTMatrix3, _P, _I, _P, _F, _F, _F], _I),
     "snb_pme_initial": ([_P, LTMatrix3, LTMatrix3, Vector, _F, _F, _I, _I, _I, _F, _I], _I),
     "snb_pme_parameters": ([_P, ctypes.POINTER(_F), ctypes.POINTER(_I), ctypes.POINTER(_I),
                             ctypes.POINTER(_I)], _I),
@@ -342,6 +349,59 @@ class Nonbonded:
             if need_virial:
                 check(self.L.snb_flush_atom_virial(self.ctx, _ptr(self.atom_virial)),
                       "snb_flush_atom_virial")
+
+    # -- integration (leap-frog / middle Langevin + SETTLE) --------------------
+    def md_initial(self, sysd, temperature=300.0, tau=1.0, seed=20260217, dt=0.002,
+                   velocities=None, settle=True):
+        """Sets up the device integrator of Main_Iteration (main.cpp:622-712):
+        masses, Maxwell velocities (or the given ones), the middle Langevin
+        thermostat (tau in ps, dt in ps) and SETTLE on the rigid waters."""
+        from . import systems
+        torch = self.torch
+        mass = np.ascontiguousarray(sysd["mass"], np.float32)
+        self.mass = torch.as_tensor(mass).to(self.dev)
+        inv = np.where(mass > 0, 1.0 / np.where(mass > 0, mass, 1.0), 0.0).astype(np.float32)
+        self.inv_mass = torch.as_tensor(inv).to(self.dev)
+        v = systems.maxwell_velocities(mass, temperature, seed) if velocities is None \
+            else np.asarray(velocities, np.float32)
+        self.vel = torch.as_tensor(v).to(self.dev).contiguous()
+        self.stress = torch.zeros(6, dtype=torch.float32, device=self.dev)
+        check(self.L.snb_langevin_initial(self.ctx, mass.ctypes.data, temperature, tau, seed, dt,
+                                          0.0), "snb_langevin_initial")
+        d, e, vf, xf = _F(), _F(), _F(), _F()
+        check(self.L.snb_langevin_parameters(self.ctx, ctypes.byref(d), ctypes.byref(e),
+                                             ctypes.byref(vf), ctypes.byref(xf)),
+              "snb_langevin_parameters")
+        self.dt_internal, self.exp_gamma = d.value, e.value
+        self.v_factor, self.x_factor = vf.value, xf.value
+        self.tri = None
+        if settle and "n_waters" in sysd:
+            a, b, c, rab, rac, rbc = systems.water_triangles(sysd)
+            check(self.L.snb_settle_initial(self.ctx, len(a), a.ctypes.data, b.ctypes.data,
+                                            c.ctypes.data, rab.ctypes.data, rac.ctypes.data,
+                                            rbc.ctypes.data, mass.ctypes.data),
+                  "snb_settle_initial")
+            self.tri = (a, b, c, rab, rac, rbc)
+
+    def md_iterate(self, need_pressure=False):
+        """Remember (SETTLE), Langevin leap-frog, SETTLE: the iteration half
+        of a step, after the forces (main.cpp:634-712)."""
+        check(self.L.snb_settle_remember(self.ctx, _ptr(self.crd), self.cell, self.rcell),
+              "snb_settle_remember")
+        check(self.L.snb_langevin_iterate(self.ctx, _ptr(self.frc), _ptr(self.vel), None,
+                                          _ptr(self.crd)), "snb_langevin_iterate")
+        check(self.L.snb_settle_do(self.ctx, _ptr(self.mass), _ptr(self.crd), self.cell,
+                                   self.rcell, _ptr(self.vel), 1 if need_pressure else 0,
+                                   _ptr(self.stress), self.dt_internal, self.v_factor,
+                                   self.x_factor), "snb_settle_do")
+
+    def md_step(self, step=0, need_energy=False, need_virial=False, forced_nl=False):
+        """One MD step on the device: zero the forces, the force path, then the
+        iteration (Main_Calculate_Force + Main_Iteration)."""
+        self.frc.zero_()
+        self.step(step=step, need_energy=need_energy, need_virial=need_virial,
+                  forced_nl=forced_nl, fused=True)
+        self.md_iterate(need_pressure=need_virial)
 
     # -- read-back helpers ----------------------------------------------------
     def sync(self):

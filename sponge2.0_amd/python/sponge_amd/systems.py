@@ -264,3 +264,34 @@ def config_gpu_93k(seed=20260217):
     d = mixed_system(seed=seed)
     d["pme_grid"] = (96, 96, 96)
     return d
+
+
+KB = 0.00198716  # kcal/mol/K (common.h:122)
+TIME_UNIT = 20.455  # internal time unit is 1/20.455 ps (common.h:136)
+
+
+def water_triangles(sysd):
+    """Rigid TIP3P triangles (O, H1, H2) of a generated system (waters come
+    first, three atoms each): atom arrays and the constraint lengths O-H,
+    O-H and H-H (the SETTLE triangle the reference finds among the constraint
+    pairs of a rigid water)."""
+    nw = int(sysd.get("n_waters", sysd["n"] // 3))
+    o = 3 * np.arange(nw, dtype=np.int32)
+    r_oh = np.float32(TIP3P_ROH)
+    r_hh = np.float32(2.0 * TIP3P_ROH * np.sin(TIP3P_THETA / 2))
+    return (o, o + 1, o + 2, np.full(nw, r_oh, np.float32), np.full(nw, r_oh, np.float32),
+            np.full(nw, r_hh, np.float32))
+
+
+def maxwell_velocities(mass, temperature=300.0, seed=20260217, remove_com=True):
+    """Maxwell-Boltzmann velocities in the internal unit (A per 1/20.455 ps):
+    sigma = sqrt(kB T / m) per component; zero-mass atoms get none; the
+    centre-of-mass velocity is removed."""
+    rng = np.random.default_rng(seed)
+    m = np.asarray(mass, np.float64)
+    sig = np.where(m > 0, np.sqrt(KB * temperature / np.where(m > 0, m, 1.0)), 0.0)
+    v = rng.normal(size=(len(m), 3)) * sig[:, None]
+    if remove_com and m.sum() > 0:
+        v -= (m[:, None] * v).sum(0) / m.sum()
+        v[m == 0] = 0.0
+    return v.astype(np.float32)
