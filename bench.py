@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=200,
                     help="eager steps, before the timed region, whose kernel-group times "
                          "give kernel_avg_us and the roofline")
+    ap.add_argument("--md-steps", type=int, default=500,
+                    help="also time this many whole MD steps (forces + middle Langevin + "
+                         "SETTLE) after the force-path measurement; 0 skips")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (scales the HBM byte counters "
                          "of a profiler run)")
@@ -106,6 +109,63 @@ def accuracy(sysd, nb, counts, rows, pme, gpu_frc):
                                     virial=False)[0]
     g = gpu_frc.astype(np.float64)
     return float(np.sqrt(((g - ref) ** 2).sum() / (ref ** 2).sum()))
+
+
+def md_phase(nb, sysd, args, kind):
+    """Whole MD steps of the same workload: the force path, then the middle
+    Langevin leap-frog (300 K, tau 0.1 ps) and SETTLE on the 30,000 waters,
+    from Maxwell velocities; replayed as one HIP graph per kind of step like
+    the force-path phase.  Runs last: it moves the atoms."""
+    import numpy as np
+    import torch
+    # tau 0.1 ps: the generated lattice releases its relaxation heat quickly
+    nb.md_initial(sysd, temperature=300.0, tau=0.1, seed=20260217, dt=0.002)
+    k0 = 0
+    for k in range(k0, k0 + 20):
+        nb.md_step(step=k, need_energy=False, need_virial=False, forced_nl=(k == 0))
+    k0 += 20
+    graphs = {}
+    if not args.no_graph:
+        for k in range(k0, k0 + max(args.refresh, args.energy_interval) + 1):
+            if kind(k) not in graphs:
+                rb, e = kind(k)
+                graph = torch.cuda.CUDAGraph()
+                side = torch.cuda.Stream(device=nb.dev)
+                home = nb.stream
+                nb.sync()
+                torch.cuda.synchronize()
+                nb.set_stream(side)
+                try:
+                    with torch.cuda.graph(graph, stream=side):
+                        nb.md_step(step=k, need_energy=e, need_virial=e)
+                finally:
+                    nb.set_stream(home)
+                torch.cuda.synchronize()
+                graphs[kind(k)] = graph
+        k0 += max(args.refresh, args.energy_interval) + 1
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(nb.stream)
+    for k in range(k0, k0 + args.md_steps):
+        if graphs:
+            graphs[kind(k)].replay()
+        else:
+            e = (k % args.energy_interval) == 0
+            nb.md_step(step=k, need_energy=e, need_virial=e)
+    ev1.record(nb.stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.md_steps
+    v = nb.vel.cpu().numpy().astype(np.float64)
+    m = np.asarray(sysd["mass"], np.float64)
+    nw = int(sysd["n_waters"])
+    dof = 6 * nw + 3 * (sysd["n"] - 3 * nw) - 3
+    t_kin = float((m[:, None] * v * v).sum() / (dof * 0.00198716))
+    return {"ns_per_day": 1000.0 / ms * NS_PER_DAY_PER_STEP_S, "ms_per_step": ms,
+            "steps": args.md_steps, "launch": "hip-graph replay per step kind" if graphs
+            else "eager", "integrator": "middle Langevin leap-frog, 300 K, tau 0.1 ps, 2 fs",
+            "constraints": "SETTLE on %d rigid waters" % nw, "t_kin_end": t_kin,
+            "status_flags": nb.status()}
 
 
 def pairs_in_cutoff(sysd, counts, rows, cutoff):
@@ -280,6 +340,8 @@ def main():
         "wall_s_timed": wall,
         "cpu_baseline": base,
     }
+    if args.md_steps > 0:
+        out["md"] = md_phase(nb, sysd, args, kind)
     print(json.dumps(out))
     nb.close()
 
