@@ -58,7 +58,7 @@ def parse():
 def cpu_baseline(sysd, nb, counts, rows, budget_s):
     """The CPU port (oracle/, OpenMP over the host's threads) on the same 93k
     workload: FP32 twin pair kernel over the GPU-built list + FP32 twin
-    exclusion correction + FP32 twin 1-4 pairs + FP64 PME reciprocal (spread,
+    exclusion correction + FP32 twin 1-4 pairs and bonds + FP64 PME reciprocal (spread,
     mixed-radix FFT, gather).  The list rebuild is left out (it favours the CPU).  Runs whole
     steps until budget_s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -76,6 +76,8 @@ def cpu_baseline(sysd, nb, counts, rows, budget_s):
         if "nb14" in s:
             oracle.nb14_twin(s["crd"], s["charge"], s["cell"], s["rcell"], s["nb14"],
                              virial=False)
+        if "bonds" in s:
+            oracle.bond_twin(s["crd"], s["cell"], s["rcell"], s["bonds"], virial=False)
         pme = oracle.pme_f64(s["crd"], s["charge"], s["rcell"], nb.grid, nb.beta, nb.volume)
         steps += 1
         el = time.perf_counter() - t0
@@ -84,8 +86,9 @@ def cpu_baseline(sysd, nb, counts, rows, budget_s):
     sps = steps / el
     nt = oracle.num_threads()
     return {"value": sps * NS_PER_DAY_PER_STEP_S, "unit": "ns/day", "cores": nt, "kind": "port",
-            "sample": "%d whole steps of the 93k workload (pair+exclusion+1-4 FP32 fixed-point twin, "
-                      "FP64 PME), NL rebuild excluded, %.1f s, %d OpenMP threads"
+            "sample": "%d whole steps of the 93k workload (pair+exclusion+1-4+bond FP32 "
+                      "fixed-point twin, FP64 PME), NL rebuild excluded, %.1f s, "
+                      "%d OpenMP threads"
                       % (steps, el, nt)}, pme
 
 
@@ -107,6 +110,8 @@ def accuracy(sysd, nb, counts, rows, pme, gpu_frc):
     if "nb14" in s:
         ref = ref + oracle.nb14_f64(s["crd"], s["charge"], s["cell"], s["rcell"], s["nb14"],
                                     virial=False)[0]
+    if "bonds" in s:
+        ref = ref + oracle.bond_f64(s["crd"], s["cell"], s["rcell"], s["bonds"], virial=False)[0]
     g = gpu_frc.astype(np.float64)
     return float(np.sqrt(((g - ref) ** 2).sum() / (ref ** 2).sum()))
 
@@ -321,7 +326,8 @@ def main():
         "dtype": "f32 pair/spline math, int64 fixed-point accumulation, f64 reductions",
         "data": "synthetic (seeded generator: 30,000 TIP3P + 3x1,000-bead chains)",
         "config": {"workload": "93,000-atom mixed system, plain nonbonded path "
-                               "(NL every %d steps + exclusion + LJ/direct + 1-4 pairs + PME 96^3), "
+                               "(NL every %d steps + exclusion + LJ/direct + 1-4 pairs + chain bonds "
+                               "+ PME 96^3), "
                                "fixed coordinates, no integrator" % args.refresh,
                    "atoms": n, "box_A": 96.5, "cutoff_A": 10.0, "skin_A": 2.0,
                    "pme_grid": [96, 96, 96], "dt_fs": 2.0, "nl_entries": e_list,

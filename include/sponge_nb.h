@@ -6,6 +6,7 @@
  *   LENNARD_JONES_INFORMATION  SPONGE/Lennard_Jones_force/Lennard_Jones_force.{h,cpp}
  *   Particle_Mesh (PME)        SPONGE/PM_force/PM_force.{h,cpp}
  *   NON_BOND_14                SPONGE/nb14/nb14.{h,cpp}
+ *   BOND                       SPONGE/bond/bond.{h,cpp}
  *   NVE leap-frog, MIDDLE_Langevin_INFORMATION, SETTLE
  *                              SPONGE/MD_core/nve.hpp, SPONGE/thermostat/Middle_Langevin_MD.cpp,
  *                              SPONGE/constrain/settle.cpp
@@ -228,6 +229,26 @@ int snb_nb14_force(snb_context* ctx, const snb_vector* d_crd, const float* d_cha
  * exact int64 sums.  Synchronises. */
 int snb_nb14_energy(snb_context* ctx, double* lj14, double* ee14);
 
+/* ------------------------------------------------------------------- bonds */
+/* BOND::Initial (bond/bond.cpp:66-157) with the list read by the caller
+ * (xponge/load/native/bond.hpp: "N" then "a b k r0" per bond; E = k (r - r0)^2).
+ * bond_numbers = 0 leaves the module uninitialised (calls are no-ops). */
+int snb_bond_initial(snb_context* ctx, int bond_numbers, const int* h_atom_a,
+                     const int* h_atom_b, const float* h_k, const float* h_r0);
+
+/* BOND::Bond_Force_With_Atom_Energy_And_Virial (bond.cpp:243-260, kernel
+ * :15-64): with dr = pbc(r_a - r_b), f = -2 k (r - r0) dr / r on a and -f on b;
+ * d_atom_energy[a] += k (r - r0)^2 (the module's per-atom energy restarts each
+ * energy call); d_atom_virial[a] += f (x) dr.  NULL outputs stay in the
+ * fixed-point accumulators. */
+int snb_bond_force(snb_context* ctx, const snb_vector* d_crd, snb_ltmatrix3 cell,
+                   snb_ltmatrix3 rcell, snb_vector* d_frc, int need_atom_energy,
+                   float* d_atom_energy, int need_virial, snb_ltmatrix3* d_atom_virial);
+
+/* BOND::Step_Print (bond.cpp:262-277): the sum of the bond energies of the
+ * last energy call (exact int64 sum).  Synchronises. */
+int snb_bond_energy(snb_context* ctx, double* energy);
+
 /* ------------------------------------------------------------- integration */
 /* MD_INFORMATION::NVE_iteration::Leap_Frog (MD_core/nve.hpp:3-48):
  * v += dt inv_m f, x += dt v, with |v| capped at max_velocity when it is > 0
@@ -324,7 +345,8 @@ int snb_pme_reciprocal_virial(snb_context* ctx, double* virial6);
 /* d_frc[i] += accumulated force, then clears the accumulator. */
 int snb_flush_forces(snb_context* ctx, snb_vector* d_frc);
 /* For outputs deferred with NULL: d_atom_energy[i] += the per-atom energies of
- * the last energy calls (LJ + direct Coulomb + exclusion correction + 1-4) and
+ * the last energy calls (LJ + direct Coulomb + exclusion correction + 1-4 +
+ * bond) and
  * d_atom_direct_pme_energy[i] = the direct Coulomb part; either may be NULL. */
 int snb_flush_atom_energy(snb_context* ctx, float* d_atom_energy, float* d_atom_direct_pme_energy);
 /* d_atom_virial[i] += the accumulated per-atom virial, then clears it. */
@@ -333,8 +355,8 @@ int snb_flush_atom_virial(snb_context* ctx, snb_ltmatrix3* d_atom_virial);
 int snb_copy_fixed_forces(snb_context* ctx, int64_t* d_out);
 int snb_clear_fixed_forces(snb_context* ctx);
 /* Copies the int64 per-atom accumulators: which = 0 LJ energy, 1 direct
- * Coulomb energy, 2 exclusion-correction energy, 4 1-4 energy (n each),
- * 3 virial (6 x n, SoA). */
+ * Coulomb energy, 2 exclusion-correction energy, 4 1-4 energy, 5 bond energy
+ * (n each), 3 virial (6 x n, SoA). */
 int snb_copy_fixed_atom_terms(snb_context* ctx, int which, int64_t* d_out);
 /* Copies the PME real-space grids of the last reciprocal call (float,
  * [fftx][ffty][fftz]): which = 0 charge grid Q, 1 potential grid (C2R output). */

@@ -63,6 +63,8 @@ def _declare(L):
     L.orc_excl_twin.argtypes = L.orc_excl_f64.argtypes
     L.orc_nb14_f64.argtypes = [_i, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p]
     L.orc_nb14_twin.argtypes = L.orc_nb14_f64.argtypes
+    L.orc_bond_f64.argtypes = [_i, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p]
+    L.orc_bond_twin.argtypes = L.orc_bond_f64.argtypes
     L.orc_fft3d_f64.argtypes = [_p, _i, _i, _i, _i]
     L.orc_dft3d_naive_f64.argtypes = [_p, _p, _i, _i, _i, _i]
     L.orc_pme_f64.argtypes = [_i, _p, _p, _p, _i, _i, _i, _f, _d, _p, _p, _p, _p, _p]
@@ -259,6 +261,42 @@ def nb14_twin(crd, charge, cell, rcell, nb14, virial=True):
     return frc, ep[:2 * m].reshape(2, m), ea, vir
 
 
+def bond_f64(crd, cell, rcell, bonds, virial=True):
+    """Harmonic bonds in FP64 (bond.cpp:15-64); bonds = (atom_a, atom_b, k, r0).
+    Returns (frc[n,3], e_pair[m], e_atom[n], vir[n,6] or None)."""
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    a, b, k, r0 = (_c(bonds[0], np.int32), _c(bonds[1], np.int32), _c(bonds[2], np.float32),
+                   _c(bonds[3], np.float32))
+    m = a.shape[0]
+    frc = np.zeros((n, 3), np.float64)
+    ep = np.zeros(max(m, 1), np.float64)
+    ea = np.zeros(n, np.float64)
+    vir = np.zeros((n, 6), np.float64) if virial else None
+    lib().orc_bond_f64(n, _ptr(crd), _ptr(_c(cell, np.float32)), _ptr(_c(rcell, np.float32)), m,
+                       _ptr(a), _ptr(b), _ptr(k), _ptr(r0), _ptr(frc), _ptr(ep), _ptr(ea),
+                       _ptr(vir))
+    return frc, ep[:m], ea, vir
+
+
+def bond_twin(crd, cell, rcell, bonds, virial=True):
+    """Float twin of the bond kernel: (frc[3,n], e_pair[m], e_atom[n],
+    vir[6,n] or None), int64 at 2^32."""
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    a, b, k, r0 = (_c(bonds[0], np.int32), _c(bonds[1], np.int32), _c(bonds[2], np.float32),
+                   _c(bonds[3], np.float32))
+    m = a.shape[0]
+    frc = np.zeros((3, n), np.int64)
+    ep = np.zeros(max(m, 1), np.int64)
+    ea = np.zeros(n, np.int64)
+    vir = np.zeros((6, n), np.int64) if virial else None
+    lib().orc_bond_twin(n, _ptr(crd), _ptr(_c(cell, np.float32)), _ptr(_c(rcell, np.float32)), m,
+                        _ptr(a), _ptr(b), _ptr(k), _ptr(r0), _ptr(frc), _ptr(ep), _ptr(ea),
+                        _ptr(vir))
+    return frc, ep[:m], ea, vir
+
+
 # --------------------------------------------------------------------------
 # FFT and PME
 # --------------------------------------------------------------------------
@@ -351,7 +389,7 @@ def nonbonded_terms_f64(sysd, cutoff=None, skin=2.0, tolerance=1e-5, grid=None,
                         want_forces=True):
     """All mdout terms of the path for a system dict (see sponge_amd.systems):
     LJ_short, LJ_long, PM_direct, PM_reciprocal, PM_self, PM_correction, PM,
-    nb14_LJ and nb14_EE (0 without 1-4 pairs), plus total nonbonded forces
+    nb14_LJ and nb14_EE (0 without 1-4 pairs), bond (0 without bonds), plus total forces
     (double[n,3]) and per-atom virial (reference convention)."""
     crd, q = sysd["crd"], sysd["charge"]
     cell, rcell = sysd["cell"], sysd["rcell"]
@@ -384,11 +422,18 @@ def nonbonded_terms_f64(sysd, cutoff=None, skin=2.0, tolerance=1e-5, grid=None,
     if "nb14" in sysd and len(sysd["nb14"][0]) > 0:
         f14, ep14, _, v14 = nb14_f64(crd, q, cell, rcell, sysd["nb14"])
         out["nb14_LJ"], out["nb14_EE"] = float(ep14[0].sum()), float(ep14[1].sum())
+    fb = np.zeros_like(f_pair)
+    vb = np.zeros_like(v_pair)
+    out["bond"] = 0.0
+    if "bonds" in sysd and len(sysd["bonds"][0]) > 0:
+        fb, epb, _, vb = bond_f64(crd, cell, rcell, sysd["bonds"])
+        out["bond"] = float(epb.sum())
     if want_forces:
-        out["frc"] = f_pair + f_ex + f_rec + f14
+        out["frc"] = f_pair + f_ex + f_rec + f14 + fb
+        out["frc_bond"] = fb
         out["frc_pair"], out["frc_excl"], out["frc_recip"] = f_pair, f_ex, f_rec
         out["frc_nb14"] = f14
-        out["virial_atoms"] = v_pair + v_ex + v14
+        out["virial_atoms"] = v_pair + v_ex + v14 + vb
         out["virial_recip"] = v_rec
     return out
 

@@ -31,6 +31,14 @@
  *   orc_excl_f64            closed form erf; E_excl + E_direct(no cutoff) = bare Coulomb
  *   orc_*_twin              the matching orc_*_f64 within float tolerance
  *   orc_ewald_f64           textbook Madelung constant of rock salt (1.747565)
+ *   orc_nb14_f64            GROMACS fixture coulomb14 / lj14 terms; central-difference gradient
+ *   orc_bond_f64            GROMACS fixture bond term (iterations 0-2) within the reference's
+ *                           0.5 kcal/mol; closed form E = k (r - r0)^2, |f| = 2 k |r - r0|;
+ *                           central-difference gradient; virial trace = sum f . dr
+ *   orc_philox4x32_10       Random123 known-answer vectors
+ *   orc_langevin_*          moments and a KS test of the normals; free-particle temperature
+ *   orc_settle_*            closed-form TIP3P frame; restored geometry, centre of mass and
+ *                           momentum after an unconstrained move
  */
 #include <math.h>
 #include <stdint.h>
@@ -694,6 +702,92 @@ ORC_EXPORT void orc_nb14_twin(int n, const float* crd, const float* charge, cons
             vir[3 * (size_t)n + a] += to_fixed(-(fv[0] * d[2] + fv[2] * d[0]));
             vir[4 * (size_t)n + a] += to_fixed(-(fv[1] * d[2] + fv[2] * d[1]));
             vir[5 * (size_t)n + a] += to_fixed(-(fv[2] * d[2]));
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Harmonic bonds (bond/bond.cpp:15-64)                                     */
+/* ------------------------------------------------------------------------ */
+/* dr = pbc(r_a - r_b), E = k (r - r0)^2, force f = -2 k (r - r0) dr / r on a,
+ * -f on b; atom a takes the energy and the virial f (x) dr.  FP64 layer:
+ * e_pair[m], e_atom[n], vir[n][6] may be NULL. */
+ORC_EXPORT void orc_bond_f64(int n, const float* crd, const float* cell, const float* rcell,
+                             int m, const int* atom_a, const int* atom_b, const float* k,
+                             const float* r0, double* frc, double* e_pair, double* e_atom,
+                             double* vir)
+{
+    for (int q = 0; q < m; q++)
+    {
+        int a = atom_a[q], b = atom_b[q];
+        double d[3];
+        pbc_disp_d(crd + 3 * a, crd + 3 * b, cell, rcell, d);
+        double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double x = r - (double)r0[q];
+        double s = -2.0 * (double)k[q] * x / r;
+        for (int c = 0; c < 3; c++)
+        {
+            frc[3 * a + c] += s * d[c];
+            frc[3 * b + c] -= s * d[c];
+        }
+        double e = (double)k[q] * x * x;
+        if (e_pair) e_pair[q] = e;
+        if (e_atom) e_atom[a] += e;
+        if (vir)
+        {
+            double* v = vir + 6 * a;
+            double f[3] = {s * d[0], s * d[1], s * d[2]};
+            v[0] += f[0] * d[0];
+            v[1] += f[0] * d[1] + f[1] * d[0];
+            v[2] += f[1] * d[1];
+            v[3] += f[0] * d[2] + f[2] * d[0];
+            v[4] += f[1] * d[2] + f[2] * d[1];
+            v[5] += f[2] * d[2];
+        }
+    }
+}
+
+/* Float twin of the bond kernel: the reference's float order
+ * (r = sqrtf(r^2), s = -(2 (r - r0) k) / r), terms rounded to int64 at 2^32;
+ * energies in double from the float displacement.  frc[3n], e_pair[m],
+ * e_atom[n], vir[6n] (+=; the last three may be NULL). */
+ORC_EXPORT void orc_bond_twin(int n, const float* crd, const float* cell, const float* rcell,
+                              int m, const int* atom_a, const int* atom_b, const float* k,
+                              const float* r0, int64_t* frc, int64_t* e_pair, int64_t* e_atom,
+                              int64_t* vir)
+{
+    for (int q = 0; q < m; q++)
+    {
+        int a = atom_a[q], b = atom_b[q];
+        float d[3];
+        pbc_disp_f(crd + 3 * a, crd + 3 * b, cell, rcell, d);
+        float r = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+        float x = r - r0[q];
+        float g = (2.0f * x) * k[q];
+        float s = (-g) / r;
+        float f[3] = {s * d[0], s * d[1], s * d[2]};
+        for (int c = 0; c < 3; c++)
+        {
+            int64_t v = to_fixed(f[c]);
+            frc[(size_t)c * n + a] += v;
+            frc[(size_t)c * n + b] -= v;
+        }
+        if (e_pair || e_atom)
+        {
+            double dx = d[0], dy = d[1], dz = d[2];
+            double dd = sqrt((dx * dx + dy * dy) + dz * dz) - (double)r0[q];
+            int64_t pe = to_fixed_d(((double)k[q] * dd) * dd);
+            if (e_pair) e_pair[q] += pe;
+            if (e_atom) e_atom[a] += pe;
+        }
+        if (vir)
+        {
+            vir[0 * (size_t)n + a] += to_fixed(f[0] * d[0]);
+            vir[1 * (size_t)n + a] += to_fixed(f[0] * d[1] + f[1] * d[0]);
+            vir[2 * (size_t)n + a] += to_fixed(f[1] * d[1]);
+            vir[3 * (size_t)n + a] += to_fixed(f[0] * d[2] + f[2] * d[0]);
+            vir[4 * (size_t)n + a] += to_fixed(f[1] * d[2] + f[2] * d[1]);
+            vir[5 * (size_t)n + a] += to_fixed(f[2] * d[2]);
         }
     }
 }

@@ -205,6 +205,17 @@ void free_nb14(SnbNb14& M)
     M.init = false;
 }
 
+void free_bond(SnbBond& M)
+{
+    dfree(M.a);
+    dfree(M.b);
+    dfree(M.k);
+    dfree(M.r0);
+    dfree(M.pair_e);
+    M.count = 0;
+    M.init = false;
+}
+
 void free_langevin(SnbLangevin& T)
 {
     dfree(T.inv_mass);
@@ -388,7 +399,7 @@ int snb_context_create(snb_context** out, int device, void* hip_stream, int atom
         rc = SNB_ERR_HIP;
     if (!rc && hipEventCreateWithFlags(&c->ev_pme, hipEventDisableTiming) != hipSuccess)
         rc = SNB_ERR_HIP;
-    if (!rc) rc = dalloc(&c->e_atom, 4 * (size_t)n);
+    if (!rc) rc = dalloc(&c->e_atom, 5 * (size_t)n);
     if (!rc) rc = dalloc(&c->vir_atom, 6 * (size_t)n);
     if (!rc) rc = dalloc(&c->satom, n);
     if (!rc) rc = dalloc(&c->stype, n);
@@ -406,7 +417,7 @@ int snb_context_create(snb_context** out, int device, void* hip_stream, int atom
     snb_launch_fill_int(s, c->type, n, 0);
     snb_launch_fill_i64(s, c->fx, 3 * (size_t)n);
     snb_launch_fill_i64(s, c->fx_pme, 3 * (size_t)n);
-    snb_launch_fill_i64(s, c->e_atom, 4 * (size_t)n);
+    snb_launch_fill_i64(s, c->e_atom, 5 * (size_t)n);
     snb_launch_fill_i64(s, c->vir_atom, 6 * (size_t)n);
     snb_launch_fill_i64(s, c->scal, SNB_S_COUNT);
     snb_launch_fill_i64(s, (int64_t*)c->dscal, SNB_D_COUNT);
@@ -435,6 +446,7 @@ int snb_context_destroy(snb_context* c)
     dfree(c->fx);
     dfree(c->e_atom);
     free_nb14(c->nb14);
+    free_bond(c->bond);
     free_langevin(c->langevin);
     free_settle(c->settle);
     dfree(c->vir_atom);
@@ -1009,6 +1021,81 @@ int snb_nb14_energy(snb_context* c, double* lj14, double* ee14)
     return SNB_OK;
 }
 
+// -------------------------------------------------------------------- bonds
+int snb_bond_initial(snb_context* c, int bond_numbers, const int* h_atom_a, const int* h_atom_b,
+                     const float* h_k, const float* h_r0)
+{
+    if (!c || bond_numbers < 0) return SNB_ERR_ARGUMENT;
+    if (bond_numbers > 0 && (!h_atom_a || !h_atom_b || !h_k || !h_r0)) return SNB_ERR_ARGUMENT;
+    for (int k = 0; k < bond_numbers; k++)
+        if (h_atom_a[k] < 0 || h_atom_a[k] >= c->n || h_atom_b[k] < 0 || h_atom_b[k] >= c->n)
+            return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    SnbBond& M = c->bond;
+    free_bond(M);
+    if (bond_numbers == 0) return SNB_OK;  // not initialised, as in the reference
+    const size_t m = (size_t)bond_numbers;
+    int rc = dalloc(&M.a, m);
+    if (!rc) rc = dalloc(&M.b, m);
+    if (!rc) rc = dalloc(&M.k, m);
+    if (!rc) rc = dalloc(&M.r0, m);
+    if (!rc) rc = dalloc(&M.pair_e, m);
+    if (rc)
+    {
+        free_bond(M);
+        return rc;
+    }
+    SNB_CHECK(hipMemcpy(M.a, h_atom_a, sizeof(int) * m, hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemcpy(M.b, h_atom_b, sizeof(int) * m, hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemcpy(M.k, h_k, sizeof(float) * m, hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemcpy(M.r0, h_r0, sizeof(float) * m, hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemset(M.pair_e, 0, sizeof(int64_t) * m));
+    M.count = bond_numbers;
+    M.init = true;
+    return SNB_OK;
+}
+
+int snb_bond_force(snb_context* c, const snb_vector* d_crd, snb_ltmatrix3 cell,
+                   snb_ltmatrix3 rcell, snb_vector* d_frc, int need_atom_energy,
+                   float* d_atom_energy, int need_virial, snb_ltmatrix3* d_atom_virial)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->bond.init) return SNB_OK;
+    if (!d_crd) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    int rc;
+    if (need_atom_energy &&
+        (rc = snb_launch_fill_i64(c->stream, c->e_atom + 4 * (size_t)c->n, c->n)))
+        return rc;
+    if ((rc = snb_launch_bond(c, d_crd, snb_box(cell, rcell), need_atom_energy, need_virial)))
+        return rc;
+    if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
+    if (need_atom_energy)
+    {
+        if ((rc = snb_launch_sum_fixed(c, c->bond.pair_e, c->bond.count, SNB_S_E_BOND))) return rc;
+        if ((rc = snb_launch_flush_atom_energy(c, d_atom_energy, nullptr, 16))) return rc;
+    }
+    if (need_virial && d_atom_virial && (rc = snb_launch_flush_virial(c, d_atom_virial)))
+        return rc;
+    return SNB_OK;
+}
+
+int snb_bond_energy(snb_context* c, double* energy)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->bond.init)
+    {
+        if (energy) *energy = 0.0;
+        return SNB_OK;
+    }
+    DeviceGuard g(c->device);
+    SNB_CHECK(hipStreamSynchronize(c->stream));
+    int64_t v = 0;
+    SNB_CHECK(hipMemcpy(&v, c->scal + SNB_S_E_BOND, sizeof(v), hipMemcpyDeviceToHost));
+    if (energy) *energy = snb_fixed_to_double(v);
+    return SNB_OK;
+}
+
 // -------------------------------------------------------------- integration
 int snb_leap_frog(snb_context* c, const float* d_inverse_mass, const snb_vector* d_frc,
                   snb_vector* d_vel, snb_vector* d_crd, float dt, float max_velocity)
@@ -1345,7 +1432,7 @@ int snb_flush_atom_energy(snb_context* c, float* d_atom_energy, float* d_direct)
 {
     if (!c) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
-    return snb_launch_flush_atom_energy(c, d_atom_energy, d_direct, 15);
+    return snb_launch_flush_atom_energy(c, d_atom_energy, d_direct, 31);
 }
 
 int snb_flush_atom_virial(snb_context* c, snb_ltmatrix3* d_atom_virial)
@@ -1376,10 +1463,10 @@ int snb_clear_fixed_forces(snb_context* c)
 
 int snb_copy_fixed_atom_terms(snb_context* c, int which, int64_t* d_out)
 {
-    if (!c || !d_out || which < 0 || which > 4) return SNB_ERR_ARGUMENT;
+    if (!c || !d_out || which < 0 || which > 5) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
     const int64_t* src = which == 3 ? c->vir_atom
-                                    : c->e_atom + (size_t)(which == 4 ? 3 : which) * c->n;
+                                    : c->e_atom + (size_t)(which >= 4 ? which - 1 : which) * c->n;
     size_t cnt = which == 3 ? 6 * (size_t)c->n : (size_t)c->n;
     SNB_CHECK(hipMemcpyAsync(d_out, src, sizeof(int64_t) * cnt, hipMemcpyDeviceToDevice,
                              c->stream));

@@ -160,7 +160,7 @@ __global__ void k_flush_atom_energy(int n, const int64_t* __restrict__ e, float*
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int64_t s = 0;
-    for (int k = 0; k < 4; k++)  // LJ | direct | exclusion correction | 1-4
+    for (int k = 0; k < 5; k++)  // LJ | direct | exclusion correction | 1-4 | bond
         if (which_mask & (1 << k)) s += e[(size_t)k * n + i];
     if (atom_energy) atom_energy[i] += snb_fixed_to_float(s);
     if (direct) direct[i] = snb_fixed_to_float(e[(size_t)n + i]);

@@ -20,6 +20,7 @@ enum SnbScalarSlot
     SNB_S_Q = 4,         // sum q
     SNB_S_E_NB14_LJ = 5,  // sum of 1-4 pair LJ energies
     SNB_S_E_NB14_EE = 6,  // sum of 1-4 pair Coulomb energies
+    SNB_S_E_BOND = 7,     // sum of bond energies
     SNB_S_COUNT = 8
 };
 
@@ -119,6 +120,18 @@ struct SnbNb14
     int64_t* pair_e = nullptr;  // 2 * count, fixed point: LJ | Coulomb of each pair
 };
 
+// Harmonic bonds (BOND): atoms, k (E = k (r - r0)^2) and r0 per bond
+struct SnbBond
+{
+    bool init = false;
+    int count = 0;
+    int* a = nullptr;
+    int* b = nullptr;
+    float* k = nullptr;
+    float* r0 = nullptr;
+    int64_t* pair_e = nullptr;  // count, fixed point
+};
+
 // Middle-scheme Langevin thermostat (MIDDLE_Langevin_INFORMATION); times in
 // the internal unit (ps * 20.455)
 #define SNB_LANGEVIN_KEY1 0x53504F4Eu  // second Philox key word ("SPON")
@@ -211,7 +224,7 @@ struct snb_context
     int64_t* fx = nullptr;       // 3n, SoA: direct-space and exclusion forces
     int64_t* fx_pme = nullptr;   // 3n, SoA: reciprocal forces (own accumulator: the PME
                                  // pipeline may run concurrently on pme_stream)
-    int64_t* e_atom = nullptr;   // 4n: LJ | direct | correction | 1-4
+    int64_t* e_atom = nullptr;   // 5n: LJ | direct | correction | 1-4 | bond
     int64_t* vir_atom = nullptr; // 6n, SoA
     int64_t* scal = nullptr;     // SNB_S_COUNT
     double* dscal = nullptr;     // SNB_D_COUNT
@@ -237,6 +250,7 @@ struct snb_context
     SnbNl nl;
     SnbLj lj;
     SnbNb14 nb14;
+    SnbBond bond;
     SnbLangevin langevin;
     SnbSettle settle;
     SnbPme pme;
@@ -309,6 +323,8 @@ int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge
                     int need_energy, int need_virial, int64_t* fx);
 int snb_launch_nb14(snb_context* ctx, const snb_vector* crd, const float* charge,
                     const SnbBox& box, int need_energy, int need_virial);
+int snb_launch_bond(snb_context* ctx, const snb_vector* crd, const SnbBox& box, int need_energy,
+                    int need_virial);
 int snb_launch_leap_frog(snb_context* ctx, const float* inv_mass, const snb_vector* frc,
                          snb_vector* vel, snb_vector* crd, float dt, float vmax);
 int snb_launch_langevin(snb_context* ctx, const snb_vector* frc, snb_vector* vel,

@@ -391,6 +391,85 @@ int snb_launch_nb14(snb_context* ctx, const snb_vector* crd, const float* charge
     return SNB_OK;
 }
 
+// Harmonic bonds (stands in for Bond_Force_With_Atom_Energy_And_Virial_Device,
+// bond/bond.cpp:15-64): dr = pbc(r_a - r_b), r = |dr| (sqrtf, where the
+// reference calls norm3df), f = -(2 k (r - r0)) / r * dr on a and -f on b in
+// the reference's float order, fixed point; the energy k (r - r0)^2 in FP64
+// from the float displacement, kept per bond and added to atom a; the virial
+// f (x) dr to atom a.
+template <bool NEED_ENERGY, bool NEED_VIRIAL>
+__global__ void __launch_bounds__(256) k_bond(int m, int n, const int* __restrict__ ba,
+                                              const int* __restrict__ bb,
+                                              const float* __restrict__ bk,
+                                              const float* __restrict__ br0,
+                                              const snb_vector* __restrict__ crd, SnbBox box,
+                                              int64_t* __restrict__ fx,
+                                              int64_t* __restrict__ e_bond,
+                                              int64_t* __restrict__ pair_e,
+                                              int64_t* __restrict__ vir)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int i = ba[k], j = bb[k];
+    const snb_vector ri = crd[i], rj = crd[j];
+    float dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
+    pbc_disp(dx, dy, dz, box);
+    const float r = sqrtf((dx * dx + dy * dy) + dz * dz);
+    const float kk = bk[k], r0 = br0[k];
+    const float d = r - r0;
+    const float g = (2.0f * d) * kk;
+    const float s = (-g) / r;
+    const float fxv = s * dx, fyv = s * dy, fzv = s * dz;
+    const int64_t ix = snb_to_fixed(fxv), iy = snb_to_fixed(fyv), iz = snb_to_fixed(fzv);
+    atomic_add_i64(fx + i, ix);
+    atomic_add_i64(fx + n + i, iy);
+    atomic_add_i64(fx + 2 * n + i, iz);
+    atomic_add_i64(fx + j, -ix);
+    atomic_add_i64(fx + n + j, -iy);
+    atomic_add_i64(fx + 2 * n + j, -iz);
+    if (NEED_ENERGY)
+    {
+        const double ddx = dx, ddy = dy, ddz = dz;
+        const double dd = __builtin_sqrt((ddx * ddx + ddy * ddy) + ddz * ddz) - (double)r0;
+        const int64_t pe = snb_to_fixed_d(((double)kk * dd) * dd);
+        pair_e[k] = pe;
+        atomic_add_i64(e_bond + i, pe);
+    }
+    if (NEED_VIRIAL)
+    {
+        atomic_add_i64(vir + i, snb_to_fixed(fxv * dx));
+        atomic_add_i64(vir + (size_t)n + i, snb_to_fixed(fxv * dy + fyv * dx));
+        atomic_add_i64(vir + (size_t)2 * n + i, snb_to_fixed(fyv * dy));
+        atomic_add_i64(vir + (size_t)3 * n + i, snb_to_fixed(fxv * dz + fzv * dx));
+        atomic_add_i64(vir + (size_t)4 * n + i, snb_to_fixed(fyv * dz + fzv * dy));
+        atomic_add_i64(vir + (size_t)5 * n + i, snb_to_fixed(fzv * dz));
+    }
+}
+
+int snb_launch_bond(snb_context* ctx, const snb_vector* crd, const SnbBox& box, int need_energy,
+                    int need_virial)
+{
+    const SnbBond& M = ctx->bond;
+    const int m = M.count, n = ctx->n;
+    if (m <= 0) return SNB_OK;
+    dim3 g((m + 255) / 256);
+#define SNB_BOND_LAUNCH(E, V)                                                               \
+    hipLaunchKernelGGL((k_bond<E, V>), g, dim3(256), 0, ctx->stream, m, n, M.a, M.b, M.k,    \
+                       M.r0, crd, box, ctx->fx, ctx->e_atom + 4 * (size_t)n, M.pair_e,       \
+                       ctx->vir_atom)
+    if (need_energy && need_virial)
+        SNB_BOND_LAUNCH(true, true);
+    else if (need_energy)
+        SNB_BOND_LAUNCH(true, false);
+    else if (need_virial)
+        SNB_BOND_LAUNCH(false, true);
+    else
+        SNB_BOND_LAUNCH(false, false);
+#undef SNB_BOND_LAUNCH
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
 // ---------------------------------------------------------------------------
 // windowed path
 // ---------------------------------------------------------------------------

@@ -73,6 +73,9 @@ _SIGS = {
     "snb_nb14_force": ([_P, _P, _P, LTMatrix3, LTMatrix3, _P, _I, _P, _I, _P], _I),
     "snb_nb14_energy": ([_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
                         _I),
+    "snb_bond_initial": ([_P, _I, _P, _P, _P, _P], _I),
+    "snb_bond_force": ([_P, _P, LTMatrix3, LTMatrix3, _P, _I, _P, _I, _P], _I),
+    "snb_bond_energy": ([_P, ctypes.POINTER(ctypes.c_double)], _I),
     "snb_leap_frog": ([_P, _P, _P, _P, _P, _F, _F], _I),
     "snb_langevin_initial": ([_P, _P, _F, _F, _I, _F, _F], _I),
     "snb_langevin_iterate": ([_P, _P, _P, _P, _P], _I),
@@ -196,6 +199,13 @@ class Nonbonded:
                                      A14.ctypes.data, B14.ctypes.data, cf14.ctypes.data),
                   "snb_nb14_initial")
             self.nb14 = (a14, b14, A14, B14, cf14)
+        self.bonds = None
+        if "bonds" in sysd and len(sysd["bonds"][0]) > 0:
+            ba, bb, bk, br = (np.ascontiguousarray(x, dt) for x, dt in
+                              zip(sysd["bonds"], (np.int32, np.int32, np.float32, np.float32)))
+            check(L.snb_bond_initial(ctx, len(ba), ba.ctypes.data, bb.ctypes.data,
+                                     bk.ctypes.data, br.ctypes.data), "snb_bond_initial")
+            self.bonds = (ba, bb, bk, br)
         g = grid if grid is not None else sysd.get("pme_grid", (0, 0, 0))
         check(L.snb_pme_initial(ctx, self.cell, self.rcell, Vector(*[float(x) for x in self.box]),
                                 self.cutoff, tolerance, int(g[0]), int(g[1]), int(g[2]), 1.0,
@@ -271,6 +281,18 @@ class Nonbonded:
               "snb_nb14_energy")
         return lj.value, ee.value
 
+    def bond_force(self, need_energy=False, need_virial=False, frc=True):
+        check(self.L.snb_bond_force(
+            self.ctx, _ptr(self.crd), self.cell, self.rcell, _ptr(self.frc) if frc else None,
+            1 if need_energy else 0, _ptr(self.atom_energy) if frc else None,
+            1 if need_virial else 0, _ptr(self.atom_virial) if frc else None),
+            "snb_bond_force")
+
+    def bond_energy(self):
+        e = ctypes.c_double()
+        check(self.L.snb_bond_energy(self.ctx, ctypes.byref(e)), "snb_bond_energy")
+        return e.value
+
     def reciprocal(self, step=0, need_energy=False, need_virial=False, frc=True):
         check(self.L.snb_pme_reciprocal_force(
             self.ctx, _ptr(self.crd), self.cell, self.rcell, _ptr(self.charge),
@@ -328,7 +350,7 @@ class Nonbonded:
 
     def step(self, step=0, need_energy=False, need_virial=False, forced_nl=False, fused=True):
         """One force evaluation of the path in the reference's order
-        (main.cpp:307-334, :371-396, :455-458, :474-478).  fused=True keeps every term
+        (main.cpp:307-334, :371-396, :439-441, :455-458, :474-478).  fused=True keeps every term
         in the fixed-point accumulators and converts once at the end."""
         check(self.L.snb_nl_update(self.ctx, None, self.n, 0, _ptr(self.crd), self.cell,
                                    self.rcell, step, 1 if forced_nl else 0, None, None, None),
@@ -339,6 +361,8 @@ class Nonbonded:
             self.long_range(need_energy, need_virial)
         if self.nb14 is not None:
             self.nb14_force(need_energy, need_virial, frc=not fused)
+        if self.bonds is not None:
+            self.bond_force(need_energy, need_virial, frc=not fused)
         self.reciprocal(step, need_energy, need_virial, frc=not fused)
         if fused:
             self.flush()
@@ -423,8 +447,8 @@ class Nonbonded:
         check(self.L.snb_clear_fixed_forces(self.ctx), "snb_clear_fixed_forces")
 
     def fixed_atom_terms(self, which):
-        """which: 0 LJ, 1 direct, 2 exclusion correction, 4 1-4 energy (n each);
-        3 virial (6 x n)."""
+        """which: 0 LJ, 1 direct, 2 exclusion correction, 4 1-4 energy, 5 bond
+        energy (n each); 3 virial (6 x n)."""
         cnt = 6 * self.n if which == 3 else self.n
         t = self.torch.empty(cnt, dtype=self.torch.int64, device=self.dev)
         check(self.L.snb_copy_fixed_atom_terms(self.ctx, which, _ptr(t)),

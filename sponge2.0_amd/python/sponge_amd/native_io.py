@@ -117,6 +117,26 @@ def write_nb14(path, atom_a, atom_b, lj_scale, cf_scale):
             f.write("%d %d %.6f %.6f\n" % (int(a), int(b), float(s), float(c)))
 
 
+def read_bond(path):
+    """Harmonic bond list as Xponge::Native_Load_Bonds reads it
+    (xponge/load/native/bond.hpp:8-40): a count, then "atom_a atom_b k r0"
+    rows (E = k (r - r0)^2).  Returns (atom_a, atom_b, k, r0)."""
+    t = _tokens(path)
+    m = int(t[0])
+    if len(t) < 1 + 4 * m:
+        raise ValueError("%s: the format of bond_in_file is not right" % path)
+    rows = np.array(t[1:1 + 4 * m], dtype=object).reshape(m, 4)
+    return (rows[:, 0].astype(np.int32), rows[:, 1].astype(np.int32),
+            rows[:, 2].astype(np.float32), rows[:, 3].astype(np.float32))
+
+
+def write_bond(path, atom_a, atom_b, k, r0):
+    with open(path, "w") as f:
+        f.write("%d\n" % len(atom_a))
+        for a, b, kk, r in zip(atom_a, atom_b, k, r0):
+            f.write("%d %d %.9g %.9g\n" % (int(a), int(b), float(kk), float(r)))
+
+
 def box_to_cell(box, angle=(90.0, 90.0, 90.0)):
     """MD_core/pbc.hpp:66-101 in float32: returns (cell, rcell) LTMatrix3."""
     f = np.float32
@@ -160,6 +180,8 @@ def system_from_prefix(directory, prefix):
         d["nb14"] = read_nb14(p("nb14") if os.path.exists(p("nb14")) else None,
                               p("nb14_extra") if os.path.exists(p("nb14_extra")) else None,
                               types, a, b)
+    if os.path.exists(p("bond")):
+        d["bonds"] = read_bond(p("bond"))
     return d
 
 
@@ -188,6 +210,8 @@ def write_system(directory, prefix, sysd):
         np.savetxt(f, sysd["crd"], fmt="%.7f")
         f.write("%.7f %.7f %.7f\n" % tuple(sysd["box"]))
         f.write("%.7f %.7f %.7f\n" % tuple(sysd.get("angle", (90.0, 90.0, 90.0))))
+    if "bonds" in sysd:
+        write_bond(p("bond"), *sysd["bonds"])
     if "nb14" in sysd:
         a14, b14, A14, B14, cf14 = sysd["nb14"]
         with open(p("nb14_extra"), "w") as f:

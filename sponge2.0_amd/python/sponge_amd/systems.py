@@ -234,6 +234,12 @@ def mixed_system(n_waters=30000, n_chains=3, beads_per_chain=1000, box=96.5,
     p14 = hi * (hi + 1) // 2 + lo
     out["nb14"] = (a14, b14, np.float32(0.5) * out["lj_a"][p14],
                    np.float32(0.5) * out["lj_b"][p14], np.full(len(a14), 0.833, np.float32))
+    # chain bonds: harmonic, r0 = the walk's bond length, k 310 kcal/mol/A^2
+    # (E = k (r - r0)^2, the C-C stretch of the united-atom chains)
+    ba = np.concatenate([nw + c * beads_per_chain + np.arange(beads_per_chain - 1)
+                         for c in range(n_chains)]).astype(np.int32)
+    out["bonds"] = (ba, (ba + 1).astype(np.int32), np.full(len(ba), 310.0, np.float32),
+                    np.full(len(ba), bond, np.float32))
     # No LJ site (water O, bead) within 2.2 A of another non-excluded LJ site,
     # and no two atoms of different molecules within 0.6 A (random water
     # orientations on the lattice can bring two H's to ~0.8 A; H carries no LJ).

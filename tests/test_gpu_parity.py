@@ -580,6 +580,65 @@ def test_nb14_bit_exact_with_twin(request, name, energy, virial):
     nb.close()
 
 
+def _stretched(sysd, seed=12, sd=0.03):
+    """Copy of a system with every coordinate moved by N(0, sd) A, so that the
+    bonds sit off their rest lengths."""
+    s = dict(sysd)
+    rng = np.random.default_rng(seed)
+    s["crd"] = (np.asarray(sysd["crd"], np.float32) +
+                rng.normal(0, sd, np.shape(sysd["crd"]))).astype(np.float32)
+    return s
+
+
+@pytest.mark.parametrize("name", ["gromacs_iter0", "mixed_small"])
+@pytest.mark.parametrize("energy,virial", [(False, False), (True, True)])
+def test_bond_bit_exact_with_twin(request, name, energy, virial):
+    import oracle
+    s = _get(request, name)
+    if name == "mixed_small":
+        s = _stretched(s)
+    nb = _nb(s)
+    print("case", name, "bonds", len(s["bonds"][0]), flush=True)
+    nb.bond_force(need_energy=energy, need_virial=virial, frc=False)
+    g_f = nb.fixed_forces()
+    t_f, t_ep, t_ea, t_v = oracle.bond_twin(s["crd"], s["cell"], s["rcell"], s["bonds"],
+                                            virial=virial)
+    np.testing.assert_array_equal(g_f, t_f)
+    if energy:
+        np.testing.assert_array_equal(nb.fixed_atom_terms(5), t_ea)
+        e = nb.bond_energy()
+        assert e == float(t_ep.sum()) * 2.0 ** -32
+        f64, ep64, _, _ = oracle.bond_f64(s["crd"], s["cell"], s["rcell"], s["bonds"],
+                                          virial=False)
+        assert abs(e - ep64.sum()) <= ENERGY_TOL * ep64.sum()
+        assert rel_rms(g_f.T * 2.0 ** -32, f64) < FORCE_TOL
+    if virial:
+        np.testing.assert_array_equal(nb.fixed_atom_terms(3), t_v)
+    nb.close()
+
+
+def test_bond_outputs_flush_and_energy_restart(gromacs_iter0):
+    """Direct outputs: d_frc += bond forces, d_atom_energy += per-atom bond
+    energy, d_atom_virial += f (x) dr; the per-bond energies restart at each
+    energy call, so two calls give twice the force and the same energy."""
+    import oracle
+    s = gromacs_iter0
+    nb = _nb(s)
+    nb.zero_outputs()
+    for _ in range(2):
+        nb.bond_force(need_energy=True, need_virial=True, frc=True)
+    nb.sync()
+    t_f, t_ep, t_ea, t_v = oracle.bond_twin(s["crd"], s["cell"], s["rcell"], s["bonds"])
+    one = (t_f.astype(np.float64) * 2.0 ** -32).astype(np.float32).T
+    np.testing.assert_array_equal(nb.frc.cpu().numpy(), one + one)
+    ea = (t_ea.astype(np.float64) * 2.0 ** -32).astype(np.float32)
+    np.testing.assert_array_equal(nb.atom_energy.cpu().numpy(), ea + ea)
+    v1 = (t_v.astype(np.float64) * 2.0 ** -32).astype(np.float32).T
+    np.testing.assert_array_equal(nb.atom_virial.cpu().numpy(), v1 + v1)
+    assert nb.bond_energy() == float(t_ep.sum()) * 2.0 ** -32
+    nb.close()
+
+
 def test_nb14_outputs_flush_and_energy_restart(gromacs_iter0):
     """Direct outputs: d_frc += forces, d_atom_energy += per-atom 1-4 energy
     (restarted at each energy call, so two calls give twice the force and

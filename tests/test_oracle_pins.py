@@ -494,3 +494,91 @@ def test_nb14_twin_matches_fp64(mixed_small):
     # ~1e-7 relative, is amplified by the r^-12 term)
     assert abs(ept.sum() * 2.0 ** -32 - ep64.sum()) < 1e-6 * np.abs(ep64).sum()
     assert rel_rms(vt.T * 2.0 ** -32, v64) < 1e-6
+
+
+# ------------------------------------------------------------ harmonic bonds
+def test_bond_reader_and_gromacs_fixture_terms():
+    """xponge/load/native/bond.hpp:8-40 reads the fixture's 1,426 bonds; their
+    energy on iterations 0-2 matches the GROMACS 2025.4 bond term within the
+    reference's tolerance of 0.5 kcal/mol
+    (benchmarks/comparison/tests/gromacs/tests/comp_alanine_dipeptide.py:69,
+    :107)."""
+    from sponge_amd import native_io
+    ref = json.load(open(os.path.join(GOLDEN, "gromacs_charmm_tip3p", "reference_terms.json")))
+    for it in (0, 1, 2):
+        s = gromacs_case(it)
+        a, b, k, r0 = s["bonds"]
+        assert len(a) == 1426
+        if it == 0:
+            row = open(os.path.join(GOLDEN, "gromacs_charmm_tip3p", "iter0",
+                                    "sys_flexible_bond.txt")).read().split()[1:5]
+            assert (a[0], b[0], k[0], r0[0]) == (int(row[0]), int(row[1]), np.float32(row[2]),
+                                                 np.float32(row[3]))
+        f, ep, ea, v = oracle.bond_f64(s["crd"], s["cell"], s["rcell"], s["bonds"])
+        want = [x for x in ref["entries"] if x["iteration"] == it][0]["bond"]
+        assert abs(ep.sum() - want) <= 0.5, (it, ep.sum(), want)
+        assert abs(ea.sum() - ep.sum()) < 1e-9 * ep.sum()
+        assert np.abs(f.sum(0)).max() < 1e-9 * np.abs(f).max()
+        # the virial trace of a pair force is f . dr = -2 k (r - r0) r
+        d = s["crd"][a].astype(np.float64) - s["crd"][b]
+        L = np.asarray(s["box"], np.float64)
+        d -= np.rint(d / L) * L
+        r = np.linalg.norm(d, axis=1)
+        tr = v[:, 0] + v[:, 2] + v[:, 5]
+        np.testing.assert_allclose(tr.sum(), np.sum(-2.0 * k * (r - r0) * r), rtol=1e-9)
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        native_io.write_system(tmp, "t", gromacs_case(0))
+        back = native_io.system_from_prefix(tmp, "t")["bonds"]
+    for x, y in zip(back, gromacs_case(0)["bonds"]):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_bond_closed_form_and_gradient():
+    """One bond of length r along a random direction (also across the periodic
+    boundary): E = k (r - r0)^2, |f| = 2 k |r - r0| pointing back to r0, and
+    central differences of E against the force on atom a."""
+    rng = np.random.default_rng(11)
+    L = 25.0
+    cell = np.array([L, 0, L, 0, 0, L], np.float32)
+    rcell = np.array([1 / L, 0, 1 / L, 0, 0, 1 / L], np.float32)
+    for r, r0, kk, at in ((1.6, 1.53, 310.0, 5.0), (1.4, 1.53, 310.0, 24.5),
+                          (0.95, 0.9572, 450.0, 0.2)):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        base = np.array([[at, at, at], np.array([at, at, at]) - r * u], np.float64) % L
+        bonds = (np.array([0]), np.array([1]), np.array([kk], np.float32),
+                 np.array([r0], np.float32))
+        f, ep, ea, _ = oracle.bond_f64(base.astype(np.float32), cell, rcell, bonds)
+        x = float(np.float32(r0))
+        assert abs(ep[0] - kk * (r - x) ** 2) < 1e-5 * max(1.0, ep[0])
+        np.testing.assert_allclose(f[0], -2 * kk * (r - x) * u, atol=2e-4 * kk)
+        np.testing.assert_allclose(f[1], -f[0], atol=1e-12)
+
+        def energy(c):
+            return oracle.bond_f64(c.astype(np.float32), cell, rcell, bonds, virial=False)[1][0]
+
+        h = 1e-3
+        g = np.zeros(3)
+        for c in range(3):
+            p, m = base.copy(), base.copy()
+            p[0, c] += h
+            m[0, c] -= h
+            g[c] = (energy(p) - energy(m)) / (2 * h)
+        np.testing.assert_allclose(f[0], -g, atol=2e-3 * max(1.0, np.abs(f[0]).max()))
+
+
+def test_bond_twin_matches_fp64(mixed_small):
+    """The float twin against the FP64 layer on the chain bonds, stretched
+    off r0 by N(0, 0.03) A: the force carries the float r - r0 cancellation
+    (~1e-7 A on a ~0.03 A stretch), the energy is double on the float
+    displacement."""
+    s = mixed_small
+    rng = np.random.default_rng(12)
+    crd = (s["crd"] + rng.normal(0, 0.03, s["crd"].shape)).astype(np.float32)
+    f64, ep64, ea64, v64 = oracle.bond_f64(crd, s["cell"], s["rcell"], s["bonds"])
+    ft, ept, eat, vt = oracle.bond_twin(crd, s["cell"], s["rcell"], s["bonds"])
+    assert rel_rms(ft.T * 2.0 ** -32, f64) < 5e-5
+    assert abs(ept.sum() * 2.0 ** -32 - ep64.sum()) < 1e-6 * ep64.sum()
+    np.testing.assert_array_equal(np.bincount(s["bonds"][0], ept, s["n"]).astype(np.int64), eat)
+    assert rel_rms(vt.T * 2.0 ** -32, v64) < 5e-5
