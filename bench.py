@@ -166,10 +166,15 @@ def md_phase(nb, sysd, args, kind):
     nw = int(sysd["n_waters"])
     dof = 6 * nw + 3 * (sysd["n"] - 3 * nw) - 3
     t_kin = float((m[:, None] * v * v).sum() / (dof * 0.00198716))
+    st = nb.stress.cpu().numpy().astype(np.float64)    # the last pressure step's stress
+    p_bar = (st[0] + st[2] + st[5]) / 3.0 * 6.946827162543585e4
     return {"ns_per_day": 1000.0 / ms * NS_PER_DAY_PER_STEP_S, "ms_per_step": ms,
             "steps": args.md_steps, "launch": "hip-graph replay per step kind" if graphs
             else "eager", "integrator": "middle Langevin leap-frog, 300 K, tau 0.1 ps, 2 fs",
             "constraints": "SETTLE on %d rigid waters" % nw, "t_kin_end": t_kin,
+            "pressure_bar_last": p_bar,
+            "pressure_steps": "every %d steps (virials, SETTLE virial, Get_pressure on the "
+                              "device)" % args.energy_interval,
             "status_flags": nb.status()}
 
 

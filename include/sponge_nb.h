@@ -10,6 +10,7 @@
  *   NVE leap-frog, MIDDLE_Langevin_INFORMATION, SETTLE
  *                              SPONGE/MD_core/nve.hpp, SPONGE/thermostat/Middle_Langevin_MD.cpp,
  *                              SPONGE/constrain/settle.cpp
+ *   stress / pressure assembly SPONGE/MD_core/sys.hpp, SPONGE/MD_core/MD_core.cpp
  * Each entry point below names the reference method it stands in for; the
  * argument lists keep the reference's order and meaning, with the module
  * object replaced by an snb_context.
@@ -301,6 +302,40 @@ int snb_settle_remember(snb_context* ctx, const snb_vector* d_crd, snb_ltmatrix3
 int snb_settle_do(snb_context* ctx, const float* d_mass, snb_vector* d_crd, snb_ltmatrix3 cell,
                   snb_ltmatrix3 rcell, snb_vector* d_vel, int need_pressure,
                   snb_ltmatrix3* d_stress, float dt, float v_factor, float x_factor);
+
+/* --------------------------------------------------------------- pressure */
+/* The stress of a pressure step is assembled as the reference does it: the
+ * caller zeroes d_virial_tensor and d_stress at the start of the step
+ * (MD_Reset_Atom_Energy_And_Virial_And_Force, MD_core.cpp:654-676), SETTLE adds its constraint
+ * virial, then these calls add the potential and kinetic parts.  Atom sums are
+ * exact (each float entry, or each FP64 product m v_a v_b, rounded to int64
+ * at 2^32), V = cell.a11 cell.a22 cell.a33 (Get_Volume, MD_core/sys.hpp:438-443).
+ * All three wait for the PME stream, whose reciprocal virial the caller
+ * passes, as the reference does, in d_atom_virial[0]. */
+
+/* system_information::Get_Potential_to_stress (sys.hpp:493-510, kernels :50-85):
+ * d_virial_tensor[0] += sum of d_atom_virial[0..atom_numbers); then
+ * d_stress[0] += d_virial_tensor[0] / V.  d_virial_tensor may be NULL
+ * (the sum alone is added). */
+int snb_get_potential_to_stress(snb_context* ctx, int atom_numbers,
+                                const snb_ltmatrix3* d_atom_virial, snb_ltmatrix3 cell,
+                                snb_ltmatrix3* d_virial_tensor, snb_ltmatrix3* d_stress);
+
+/* system_information::Get_Kinetic_to_stress (sys.hpp:512-524, kernel :14-48):
+ * d_stress[0] += sum m v (x) v / V, in the Get_Virial_From_Force_Dis layout
+ * (a21 = 2 m vx vy, a31 = 2 m vx vz, a32 = 2 m vy vz). */
+int snb_get_kinetic_to_stress(snb_context* ctx, int atom_numbers, const snb_vector* d_vel,
+                              const float* d_mass, snb_ltmatrix3 cell, snb_ltmatrix3* d_stress);
+
+/* MD_INFORMATION::Get_pressure (MD_core.cpp:948-975): the two calls above in
+ * one pass over the atoms, then, when h_stress or h_pressure is given, a
+ * synchronous copy of d_stress[0] and pressure = (a11 + a22 + a33) / 3
+ * (internal units; x 6.946827162543585e4 gives bar, common.h:142).  With both
+ * NULL nothing is copied and the call can be captured in a graph. */
+int snb_get_pressure(snb_context* ctx, int atom_numbers, const snb_vector* d_vel,
+                     const float* d_mass, const snb_ltmatrix3* d_atom_virial, snb_ltmatrix3 cell,
+                     snb_ltmatrix3* d_virial_tensor, snb_ltmatrix3* d_stress,
+                     snb_ltmatrix3* h_stress, float* h_pressure);
 
 /* -------------------------------------------------------------------- PME */
 /* Particle_Mesh::Initial (PM_force.cpp:293-652) with the controller keys

@@ -65,6 +65,7 @@ def _declare(L):
     L.orc_nb14_twin.argtypes = L.orc_nb14_f64.argtypes
     L.orc_bond_f64.argtypes = [_i, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p]
     L.orc_bond_twin.argtypes = L.orc_bond_f64.argtypes
+    L.orc_stress_twin.argtypes = [_i, _p, _p, _p, ctypes.c_float, _p, _p, _p]
     L.orc_fft3d_f64.argtypes = [_p, _i, _i, _i, _i]
     L.orc_dft3d_naive_f64.argtypes = [_p, _p, _i, _i, _i, _i]
     L.orc_pme_f64.argtypes = [_i, _p, _p, _p, _i, _i, _i, _f, _d, _p, _p, _p, _p, _p]
@@ -295,6 +296,22 @@ def bond_twin(crd, cell, rcell, bonds, virial=True):
                         _ptr(a), _ptr(b), _ptr(k), _ptr(r0), _ptr(frc), _ptr(ep), _ptr(ea),
                         _ptr(vir))
     return frc, ep[:m], ea, vir
+
+
+def stress_twin(inv_v, vt, stress, atom_virial=None, vel=None, mass=None):
+    """Float twin of the pressure assembly (sys.hpp:14-85, MD_core.cpp:948-975):
+    updates vt[6] and stress[6] (float32 arrays) in place; returns the int64
+    sums [potential 6, kinetic 6]."""
+    av = None if atom_virial is None else _c(atom_virial, np.float32)
+    v = None if vel is None else _c(vel, np.float32)
+    m = None if mass is None else _c(mass, np.float32)
+    n = av.shape[0] if av is not None else (v.shape[0] if v is not None else 0)
+    assert vt.dtype == np.float32 and stress.dtype == np.float32
+    assert vt.flags.c_contiguous and stress.flags.c_contiguous
+    sums = np.zeros(12, np.int64)
+    lib().orc_stress_twin(n, _ptr(av), _ptr(v), _ptr(m), ctypes.c_float(inv_v), _ptr(vt),
+                          _ptr(stress), _ptr(sums))
+    return sums
 
 
 # --------------------------------------------------------------------------

@@ -35,6 +35,10 @@
  *   orc_bond_f64            GROMACS fixture bond term (iterations 0-2) within the reference's
  *                           0.5 kcal/mol; closed form E = k (r - r0)^2, |f| = 2 k |r - r0|;
  *                           central-difference gradient; virial trace = sum f . dr
+ *   orc_stress_twin         trace of the kinetic part = 2 E_kin (E_kin summed independently);
+ *                           closed-form off-diagonal layout; potential part = numpy sum
+ *   orc_pme_f64 (virial)    central differences of E_rec under box scaling: each diagonal
+ *                           entry is +1/2 dE_rec/dlambda (the reference's sign and factor)
  *   orc_philox4x32_10       Random123 known-answer vectors
  *   orc_langevin_*          moments and a KS test of the normals; free-particle temperature
  *   orc_settle_*            closed-form TIP3P frame; restored geometry, centre of mass and
@@ -790,6 +794,50 @@ ORC_EXPORT void orc_bond_twin(int n, const float* crd, const float* cell, const 
             vir[5 * (size_t)n + a] += to_fixed(f[2] * d[2]);
         }
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Stress assembly (MD_core/sys.hpp:14-85, :493-524; MD_core.cpp:948-975)    */
+/* ------------------------------------------------------------------------ */
+static int64_t stress_fixed(double x) { return (int64_t)rint(x * ORC_FIXED_SCALE); }
+
+/* Float twin of the library's pressure assembly: sums of the atom virials
+ * (float entries) and of m v (x) v (FP64 products, off-diagonals 2 m v_a v_b)
+ * rounded per term to int64 at 2^32; then, in float,
+ *   vt += sum_pot;  stress += inv_v * vt;  stress += inv_v * sum_kin.
+ * atom_virial[6n] (a11 a21 a22 a31 a32 a33 per atom) or NULL (no potential
+ * part); vel[3n]/mass[n] or NULL (no kinetic part); vt[6], stress[6] in/out.
+ * sums[12] (may be NULL) receives the two int64 sums. */
+ORC_EXPORT void orc_stress_twin(int n, const float* atom_virial, const float* vel,
+                                const float* mass, float inv_v, float* vt, float* stress,
+                                int64_t* sums)
+{
+    int64_t w[12] = {0};
+    for (int i = 0; i < n; i++)
+    {
+        if (atom_virial)
+            for (int k = 0; k < 6; k++) w[k] += stress_fixed((double)atom_virial[6 * i + k]);
+        if (vel && mass)
+        {
+            double m = mass[i], x = vel[3 * i], y = vel[3 * i + 1], z = vel[3 * i + 2];
+            w[6] += stress_fixed(m * (x * x));
+            w[7] += stress_fixed(m * (2.0 * (x * y)));
+            w[8] += stress_fixed(m * (y * y));
+            w[9] += stress_fixed(m * (2.0 * (x * z)));
+            w[10] += stress_fixed(m * (2.0 * (y * z)));
+            w[11] += stress_fixed(m * (z * z));
+        }
+    }
+    if (atom_virial)
+        for (int k = 0; k < 6; k++)
+        {
+            vt[k] += (float)((double)w[k] * (1.0 / ORC_FIXED_SCALE));
+            stress[k] += inv_v * vt[k];
+        }
+    if (vel && mass)
+        for (int k = 0; k < 6; k++)
+            stress[k] += inv_v * (float)((double)w[6 + k] * (1.0 / ORC_FIXED_SCALE));
+    if (sums) memcpy(sums, w, sizeof(w));
 }
 
 /* ------------------------------------------------------------------------ */

@@ -1249,6 +1249,62 @@ int snb_settle_do(snb_context* c, const float* d_mass, snb_vector* d_crd, snb_lt
                              d_stress, dt, v_factor, x_factor);
 }
 
+// ----------------------------------------------------------------- pressure
+// 1 / V with V = cell.a11 cell.a22 cell.a33 in float (Get_Volume, sys.hpp:438-443)
+static float inverse_volume(const snb_ltmatrix3& cell)
+{
+    const float volume = (cell.a11 * cell.a22) * cell.a33;
+    return 1.0f / volume;
+}
+
+static int stress_call(snb_context* c, int atom_numbers, int pot, int kin,
+                       const snb_ltmatrix3* d_atom_virial, const snb_vector* d_vel,
+                       const float* d_mass, snb_ltmatrix3 cell, snb_ltmatrix3* d_virial_tensor,
+                       snb_ltmatrix3* d_stress)
+{
+    if (!c || atom_numbers < 0 || !d_stress) return SNB_ERR_ARGUMENT;
+    if (atom_numbers > 0 && ((pot && !d_atom_virial) || (kin && (!d_vel || !d_mass))))
+        return SNB_ERR_ARGUMENT;
+    if (!(cell.a11 > 0.0f && cell.a22 > 0.0f && cell.a33 > 0.0f)) return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    // the reciprocal virial may still be in flight on the PME stream
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
+    return snb_launch_stress(c, atom_numbers, pot, kin, d_atom_virial, d_vel, d_mass,
+                             inverse_volume(cell), d_virial_tensor, d_stress);
+}
+
+int snb_get_potential_to_stress(snb_context* c, int atom_numbers,
+                                const snb_ltmatrix3* d_atom_virial, snb_ltmatrix3 cell,
+                                snb_ltmatrix3* d_virial_tensor, snb_ltmatrix3* d_stress)
+{
+    return stress_call(c, atom_numbers, 1, 0, d_atom_virial, nullptr, nullptr, cell,
+                       d_virial_tensor, d_stress);
+}
+
+int snb_get_kinetic_to_stress(snb_context* c, int atom_numbers, const snb_vector* d_vel,
+                              const float* d_mass, snb_ltmatrix3 cell, snb_ltmatrix3* d_stress)
+{
+    return stress_call(c, atom_numbers, 0, 1, nullptr, d_vel, d_mass, cell, nullptr, d_stress);
+}
+
+int snb_get_pressure(snb_context* c, int atom_numbers, const snb_vector* d_vel,
+                     const float* d_mass, const snb_ltmatrix3* d_atom_virial, snb_ltmatrix3 cell,
+                     snb_ltmatrix3* d_virial_tensor, snb_ltmatrix3* d_stress,
+                     snb_ltmatrix3* h_stress, float* h_pressure)
+{
+    int rc = stress_call(c, atom_numbers, 1, 1, d_atom_virial, d_vel, d_mass, cell,
+                         d_virial_tensor, d_stress);
+    if (rc || (!h_stress && !h_pressure)) return rc;
+    DeviceGuard g(c->device);
+    snb_ltmatrix3 s;
+    SNB_CHECK(hipStreamSynchronize(c->stream));
+    SNB_CHECK(hipMemcpy(&s, d_stress, sizeof(s), hipMemcpyDeviceToHost));
+    if (h_stress) *h_stress = s;
+    if (h_pressure) *h_pressure = ((s.a11 + s.a22) + s.a33) / 3.0f;
+    return SNB_OK;
+}
+
 // ---------------------------------------------------------------------- PME
 int snb_pme_initial(snb_context* c, snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
                     snb_vector box_length, float cutoff, float tolerance, int fftx, int ffty,

@@ -21,7 +21,8 @@ enum SnbScalarSlot
     SNB_S_E_NB14_LJ = 5,  // sum of 1-4 pair LJ energies
     SNB_S_E_NB14_EE = 6,  // sum of 1-4 pair Coulomb energies
     SNB_S_E_BOND = 7,     // sum of bond energies
-    SNB_S_COUNT = 8
+    SNB_S_STRESS0 = 8,    // 12 words: atom-virial sums (6), then m v (x) v sums (6)
+    SNB_S_COUNT = 20
 };
 
 // Double scalar slots written by the PME reduction.
@@ -330,6 +331,11 @@ int snb_launch_leap_frog(snb_context* ctx, const float* inv_mass, const snb_vect
 int snb_launch_langevin(snb_context* ctx, const snb_vector* frc, snb_vector* vel,
                         snb_vector* crd);
 int snb_launch_settle_remember(snb_context* ctx, const snb_vector* crd, const SnbBox& box);
+// Pressure assembly: pot adds the atom-virial sum (into virial_tensor, when
+// given, and stress), kin the m v (x) v sum (into stress).
+int snb_launch_stress(snb_context* ctx, int n, int pot, int kin, const snb_ltmatrix3* atom_virial,
+                      const snb_vector* vel, const float* mass, float inv_volume,
+                      snb_ltmatrix3* virial_tensor, snb_ltmatrix3* stress);
 int snb_launch_settle(snb_context* ctx, const float* mass, snb_vector* crd, const SnbBox& box,
                       snb_vector* vel, int need_pressure, snb_ltmatrix3* stress, float dt,
                       float v_factor, float x_factor);

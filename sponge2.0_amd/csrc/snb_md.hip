@@ -331,3 +331,145 @@ int snb_launch_settle(snb_context* ctx, const float* mass, snb_vector* crd, cons
     }
     return SNB_OK;
 }
+
+// ---------------------------------------------------------------------------
+// pressure assembly
+// ---------------------------------------------------------------------------
+// Rounds a double to int64 at 2^32 (nearest, ties to even); unlike the magic-
+// constant conversion it covers |x| up to 2^31, which a system virial folded
+// into one atom's entry can approach.
+__device__ __forceinline__ int64_t stress_fixed(double x)
+{
+    return (int64_t)__builtin_rint(x * SNB_FIXED_SCALE);
+}
+
+// Sums over atoms of the atom virials (Get_Stress_Potential_Contribution,
+// MD_core/sys.hpp:50-78) and of m v (x) v in the Get_Virial_From_Force_Dis
+// layout, off-diagonals 2 m v_a v_b (Get_Stress_Kinetic_Contribution, :14-48).
+// Each entry or FP64 product is rounded to int64 at 2^32 before any addition,
+// so the sums are exact and independent of the order: acc[0..5] potential,
+// acc[6..11] kinetic.
+template <bool POT, bool KIN>
+__global__ void __launch_bounds__(256) k_stress_sums(int n,
+                                                     const snb_ltmatrix3* __restrict__ atom_virial,
+                                                     const snb_vector* __restrict__ vel,
+                                                     const float* __restrict__ mass,
+                                                     int64_t* __restrict__ acc)
+{
+    int64_t w[12];
+    for (int k = 0; k < 12; k++) w[k] = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        if (POT)
+        {
+            const snb_ltmatrix3 v = atom_virial[i];
+            w[0] += stress_fixed(v.a11);
+            w[1] += stress_fixed(v.a21);
+            w[2] += stress_fixed(v.a22);
+            w[3] += stress_fixed(v.a31);
+            w[4] += stress_fixed(v.a32);
+            w[5] += stress_fixed(v.a33);
+        }
+        if (KIN)
+        {
+            const snb_vector u = vel[i];
+            const double m = mass[i], x = u.x, y = u.y, z = u.z;
+            w[6] += stress_fixed(m * (x * x));
+            w[7] += stress_fixed(m * (2.0 * (x * y)));
+            w[8] += stress_fixed(m * (y * y));
+            w[9] += stress_fixed(m * (2.0 * (x * z)));
+            w[10] += stress_fixed(m * (2.0 * (y * z)));
+            w[11] += stress_fixed(m * (z * z));
+        }
+    }
+    __shared__ int64_t part[4][12];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = POT ? 0 : 6; k < (KIN ? 12 : 6); k++)
+    {
+        int64_t s = w[k];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) part[wave][k] = s;
+    }
+    __syncthreads();
+    const int k = threadIdx.x;
+    if (k < 12 && (POT || k >= 6) && (KIN || k < 6))
+    {
+        const int64_t s = part[0][k] + part[1][k] + part[2][k] + part[3][k];
+        if (s != 0) atomicAdd((unsigned long long*)&acc[k], (unsigned long long)s);
+    }
+}
+
+// virial_tensor += potential sums; stress += inv_volume * virial_tensor
+// (Get_Stress_From_virial, sys.hpp:80-85); stress += inv_volume * kinetic
+// sums; clears the sums.
+template <bool POT, bool KIN>
+__global__ void k_stress_finish(int64_t* __restrict__ acc, float inv_volume,
+                                snb_ltmatrix3* __restrict__ virial_tensor,
+                                snb_ltmatrix3* __restrict__ stress)
+{
+    if (threadIdx.x != 0) return;
+    snb_ltmatrix3 s = stress[0];
+    if (POT)
+    {
+        snb_ltmatrix3 t = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (virial_tensor) t = virial_tensor[0];
+        t.a11 += snb_fixed_to_float(acc[0]);
+        t.a21 += snb_fixed_to_float(acc[1]);
+        t.a22 += snb_fixed_to_float(acc[2]);
+        t.a31 += snb_fixed_to_float(acc[3]);
+        t.a32 += snb_fixed_to_float(acc[4]);
+        t.a33 += snb_fixed_to_float(acc[5]);
+        if (virial_tensor) virial_tensor[0] = t;
+        s.a11 += inv_volume * t.a11;
+        s.a21 += inv_volume * t.a21;
+        s.a22 += inv_volume * t.a22;
+        s.a31 += inv_volume * t.a31;
+        s.a32 += inv_volume * t.a32;
+        s.a33 += inv_volume * t.a33;
+    }
+    if (KIN)
+    {
+        s.a11 += inv_volume * snb_fixed_to_float(acc[6]);
+        s.a21 += inv_volume * snb_fixed_to_float(acc[7]);
+        s.a22 += inv_volume * snb_fixed_to_float(acc[8]);
+        s.a31 += inv_volume * snb_fixed_to_float(acc[9]);
+        s.a32 += inv_volume * snb_fixed_to_float(acc[10]);
+        s.a33 += inv_volume * snb_fixed_to_float(acc[11]);
+    }
+    stress[0] = s;
+    for (int k = 0; k < 12; k++) acc[k] = 0;
+}
+
+int snb_launch_stress(snb_context* ctx, int n, int pot, int kin, const snb_ltmatrix3* atom_virial,
+                      const snb_vector* vel, const float* mass, float inv_volume,
+                      snb_ltmatrix3* virial_tensor, snb_ltmatrix3* stress)
+{
+    if (!pot && !kin) return SNB_OK;
+    int64_t* acc = ctx->scal + SNB_S_STRESS0;
+    if (n > 0)
+    {
+        const int blocks = (n + 255) / 256 < 512 ? (n + 255) / 256 : 512;
+#define SNB_STRESS_SUMS(P, K)                                                                     \
+    hipLaunchKernelGGL((k_stress_sums<P, K>), dim3(blocks), dim3(256), 0, ctx->stream, n,      \
+                       atom_virial, vel, mass, acc)
+        if (pot && kin)
+            SNB_STRESS_SUMS(true, true);
+        else if (pot)
+            SNB_STRESS_SUMS(true, false);
+        else
+            SNB_STRESS_SUMS(false, true);
+#undef SNB_STRESS_SUMS
+        SNB_CHECK(hipGetLastError());
+    }
+    if (pot && kin)
+        hipLaunchKernelGGL((k_stress_finish<true, true>), dim3(1), dim3(64), 0, ctx->stream, acc,
+                           inv_volume, virial_tensor, stress);
+    else if (pot)
+        hipLaunchKernelGGL((k_stress_finish<true, false>), dim3(1), dim3(64), 0, ctx->stream,
+                           acc, inv_volume, virial_tensor, stress);
+    else
+        hipLaunchKernelGGL((k_stress_finish<false, true>), dim3(1), dim3(64), 0, ctx->stream,
+                           acc, inv_volume, virial_tensor, stress);
+    SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}

@@ -83,6 +83,10 @@ _SIGS = {
     "snb_settle_initial": ([_P, _I, _P, _P, _P, _P, _P, _P, _P], _I),
     "snb_settle_remember": ([_P, _P, LTMatrix3, LTMatrix3], _I),
     "snb_settle_do": ([_P, _P, _P, LTMatrix3, LTMatrix3, _P, _I, _P, _F, _F, _F], _I),
+    "snb_get_potential_to_stress": ([_P, _I, _P, LTMatrix3, _P, _P], _I),
+    "snb_get_kinetic_to_stress": ([_P, _I, _P, _P, LTMatrix3, _P], _I),
+    "snb_get_pressure": ([_P, _I, _P, _P, _P, LTMatrix3, _P, _P, ctypes.POINTER(LTMatrix3),
+                          ctypes.POINTER(_F)], _I),
     "snb_pme_initial": ([_P, LTMatrix3, LTMatrix3, Vector, _F, _F, _I, _I, _I, _F, _I], _I),
     "snb_pme_parameters": ([_P, ctypes.POINTER(_F), ctypes.POINTER(_I), ctypes.POINTER(_I),
                             ctypes.POINTER(_I)], _I),
@@ -390,6 +394,7 @@ class Nonbonded:
             else np.asarray(velocities, np.float32)
         self.vel = torch.as_tensor(v).to(self.dev).contiguous()
         self.stress = torch.zeros(6, dtype=torch.float32, device=self.dev)
+        self.virial_tensor = torch.zeros(6, dtype=torch.float32, device=self.dev)
         check(self.L.snb_langevin_initial(self.ctx, mass.ctypes.data, temperature, tau, seed, dt,
                                           0.0), "snb_langevin_initial")
         d, e, vf, xf = _F(), _F(), _F(), _F()
@@ -420,12 +425,40 @@ class Nonbonded:
                                    self.x_factor), "snb_settle_do")
 
     def md_step(self, step=0, need_energy=False, need_virial=False, forced_nl=False):
-        """One MD step on the device: zero the forces, the force path, then the
-        iteration (Main_Calculate_Force + Main_Iteration)."""
+        """One MD step on the device: zero the forces (and, on a pressure step,
+        the virials and the stress: MD_Reset_Atom_Energy_And_Virial_And_Force,
+        MD_core.cpp:654-676), the force path, then the iteration
+        (Main_Calculate_Force + Main_Iteration); a pressure step ends with the
+        stress assembly of Get_pressure (main.cpp:713-717), left on the device."""
         self.frc.zero_()
+        if need_virial:
+            self.atom_virial.zero_()
+            self.virial.zero_()
+            self.stress.zero_()
+            self.virial_tensor.zero_()
         self.step(step=step, need_energy=need_energy, need_virial=need_virial,
                   forced_nl=forced_nl, fused=True)
         self.md_iterate(need_pressure=need_virial)
+        if need_virial:
+            self.get_pressure(copy=False)
+
+    def get_pressure(self, copy=True):
+        """MD_INFORMATION::Get_pressure (MD_core.cpp:948-975) on the current
+        velocities: the reciprocal and long-range virial (this wrapper keeps it
+        in self.virial) goes into atom 0's entry, where the reference's
+        d_virial[0] holds it, then stress += (sum of atom virials + sum m v v) / V.
+        With copy, returns (stress[6], pressure) in internal units."""
+        self.atom_virial[0].add_(self.virial)
+        self.virial.zero_()
+        hs, hp = LTMatrix3(), _F()
+        check(self.L.snb_get_pressure(self.ctx, self.n, _ptr(self.vel), _ptr(self.mass),
+                                      _ptr(self.atom_virial), self.cell,
+                                      _ptr(self.virial_tensor), _ptr(self.stress),
+                                      ctypes.byref(hs) if copy else None,
+                                      ctypes.byref(hp) if copy else None), "snb_get_pressure")
+        if copy:
+            return np.array([hs.a11, hs.a21, hs.a22, hs.a31, hs.a32, hs.a33], np.float32), hp.value
+        return None
 
     # -- read-back helpers ----------------------------------------------------
     def sync(self):

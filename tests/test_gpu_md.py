@@ -161,3 +161,96 @@ def test_md_run_keeps_waters_rigid_and_temperature(water_9k):
     print("T_kin mean over the last 1 ps", t_mean, flush=True)
     assert abs(t_mean - 300.0) < 9.0
     nb.close()
+
+
+def test_pressure_assembly_bit_exact_with_twin(water_9k):
+    """snb_get_pressure / snb_get_potential_to_stress / snb_get_kinetic_to_stress
+    against the oracle's twin: random atom virials (one system-sized entry on
+    atom 0), velocities and masses; the stress and virial tensor start from
+    non-zero values, as after SETTLE; two calls in a row (the sums restart)."""
+    import oracle
+    from sponge_amd.lib import LTMatrix3, check
+    s = water_9k
+    nb = _nb(s)
+    torch = nb.torch
+    n = s["n"]
+    rng = np.random.default_rng(31)
+    av = rng.normal(0, 30.0, (n, 6)).astype(np.float32)
+    av[0] = [3.0e5, -1.0e4, 2.5e5, 7.0e3, -2.0e3, 2.8e5]
+    mass = np.asarray(s["mass"], np.float32)
+    vel = (rng.normal(0, 1, (n, 3)) / np.sqrt(mass)[:, None]).astype(np.float32)
+    d_av = torch.as_tensor(av).to(nb.dev)
+    d_vel = torch.as_tensor(vel).to(nb.dev)
+    d_mass = torch.as_tensor(mass).to(nb.dev)
+    st0 = np.array([0.5, 0.1, 0.5, -0.2, 0.05, 0.5], np.float32)
+    vt0 = np.array([1.0, 0.0, 2.0, 0.0, 0.0, 3.0], np.float32)
+    d_st = torch.as_tensor(st0).to(nb.dev)
+    d_vt = torch.as_tensor(vt0).to(nb.dev)
+    cell = s["cell"]
+    inv_v = np.float32(1.0) / ((np.float32(cell[0]) * np.float32(cell[2])) * np.float32(cell[5]))
+    st, vt = st0.copy(), vt0.copy()
+    P = ctypes.c_void_p
+    for rep in range(2):
+        hs, hp = LTMatrix3(), ctypes.c_float()
+        check(nb.L.snb_get_pressure(nb.ctx, n, P(d_vel.data_ptr()), P(d_mass.data_ptr()),
+                                    P(d_av.data_ptr()), nb.cell, P(d_vt.data_ptr()),
+                                    P(d_st.data_ptr()), ctypes.byref(hs), ctypes.byref(hp)),
+              "snb_get_pressure")
+        oracle.stress_twin(inv_v, vt, st, atom_virial=av, vel=vel, mass=mass)
+        got = np.array([hs.a11, hs.a21, hs.a22, hs.a31, hs.a32, hs.a33], np.float32)
+        print("rep", rep, "stress", got, "pressure", hp.value, flush=True)
+        np.testing.assert_array_equal(got, st)
+        np.testing.assert_array_equal(d_st.cpu().numpy(), st)
+        np.testing.assert_array_equal(d_vt.cpu().numpy(), vt)
+        assert hp.value == np.float32((st[0] + st[2]) + st[5]) / np.float32(3.0)
+    # the two halves on their own
+    check(nb.L.snb_get_potential_to_stress(nb.ctx, n, P(d_av.data_ptr()), nb.cell,
+                                           P(d_vt.data_ptr()), P(d_st.data_ptr())),
+          "snb_get_potential_to_stress")
+    oracle.stress_twin(inv_v, vt, st, atom_virial=av)
+    check(nb.L.snb_get_kinetic_to_stress(nb.ctx, n, P(d_vel.data_ptr()), P(d_mass.data_ptr()),
+                                         nb.cell, P(d_st.data_ptr())),
+          "snb_get_kinetic_to_stress")
+    oracle.stress_twin(inv_v, vt, st, vel=vel, mass=mass)
+    nb.sync()
+    np.testing.assert_array_equal(d_st.cpu().numpy(), st)
+    np.testing.assert_array_equal(d_vt.cpu().numpy(), vt)
+    nb.close()
+
+
+def test_md_pressure_steps_match_eager_assembly(water_9k):
+    """A pressure step of md_step (virials zeroed, force path with virials,
+    Langevin + SETTLE with the constraint virial, Get_pressure on the device)
+    gives the stress that the twin assembles from the step's own atom
+    virials, reciprocal virial, velocities and SETTLE part; the pressure is
+    finite."""
+    import oracle
+    s = water_9k
+    nb = _nb(s, refresh_interval=10)
+    nb.md_initial(s, temperature=300.0, tau=0.1, seed=5, dt=0.002)
+    for k in range(30):
+        nb.md_step(step=k, forced_nl=(k == 0))
+    # the same pressure step, taken apart: forces + virials, iteration, then assembly
+    nb.frc.zero_()
+    nb.atom_virial.zero_()
+    nb.virial.zero_()
+    nb.stress.zero_()
+    nb.virial_tensor.zero_()
+    nb.step(step=30, need_virial=True, fused=True)
+    nb.md_iterate(need_pressure=True)
+    nb.sync()
+    av = nb.atom_virial.cpu().numpy().copy()
+    rec = nb.virial.cpu().numpy().copy()
+    st = nb.stress.cpu().numpy().copy()          # SETTLE's part
+    vel = nb.vel.cpu().numpy().copy()
+    hs, p = nb.get_pressure(copy=True)
+    av[0] += rec
+    cell = s["cell"]
+    inv_v = np.float32(1.0) / ((np.float32(cell[0]) * np.float32(cell[2])) * np.float32(cell[5]))
+    vt = np.zeros(6, np.float32)
+    oracle.stress_twin(inv_v, vt, st, atom_virial=av, vel=vel,
+                       mass=np.asarray(s["mass"], np.float32))
+    np.testing.assert_array_equal(hs, st)
+    print("pressure (bar)", p * 6.946827162543585e4, flush=True)
+    assert np.isfinite(p)
+    nb.close()

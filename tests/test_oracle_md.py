@@ -136,3 +136,49 @@ def test_langevin_thermalises_free_particles():
     t_kin = float(np.mean(acc))
     assert abs(t_kin - T) < 0.01 * T, t_kin
     assert np.all(ctr == 800)
+
+
+# ------------------------------------------------------------ pressure assembly
+def test_stress_twin_kinetic_part_is_twice_kinetic_energy():
+    """Get_Kinetic_to_stress: V * trace(stress) = sum m |v|^2 = 2 E_kin, with
+    E_kin summed independently in FP64; the off-diagonal layout of
+    Get_Virial_From_Force_Dis(v, v) (a21 = 2 m vx vy ...) on one atom."""
+    rng = np.random.default_rng(21)
+    n = 50000
+    mass = np.where(np.arange(n) % 3 == 0, 15.999, 1.008).astype(np.float32)
+    vel = rng.normal(0, 1, (n, 3)).astype(np.float32) / np.sqrt(mass)[:, None]
+    inv_v = np.float32(1.0 / 27000.0)
+    vt = np.zeros(6, np.float32)
+    st = np.zeros(6, np.float32)
+    oracle.stress_twin(inv_v, vt, st, vel=vel, mass=mass)
+    ek2 = float((mass.astype(np.float64)[:, None] * vel.astype(np.float64) ** 2).sum())
+    tr = (float(st[0]) + float(st[2]) + float(st[5])) / float(inv_v)
+    assert abs(tr - ek2) < 2e-6 * ek2
+    assert np.all(vt == 0)
+    v1 = np.array([[0.3, -0.2, 0.7]], np.float32)
+    m1 = np.array([2.0], np.float32)
+    st = np.zeros(6, np.float32)
+    oracle.stress_twin(np.float32(1.0), vt, st, vel=v1, mass=m1)
+    x, y, z = (float(c) for c in v1[0])
+    want = 2.0 * np.array([x * x, 2 * x * y, y * y, 2 * x * z, 2 * y * z, z * z])
+    np.testing.assert_allclose(st, want, rtol=1e-6)
+
+
+def test_stress_twin_potential_part_is_the_atom_sum():
+    """Get_Potential_to_stress: virial_tensor += sum of the atom virials
+    (numpy FP64 sum of the same floats), stress += virial_tensor / V on top of
+    what the stress held (a SETTLE contribution, say); one large entry (a
+    system virial folded into atom 0) is summed exactly."""
+    rng = np.random.default_rng(22)
+    av = rng.normal(0, 30.0, (20000, 6)).astype(np.float32)
+    av[0] = [3.0e5, -1.0e4, 2.5e5, 7.0e3, -2.0e3, 2.8e5]
+    vt = np.array([1.0, 0, 2.0, 0, 0, 3.0], np.float32)
+    st = np.array([0.5, 0.1, 0.5, 0, 0, 0.5], np.float32)
+    vt0, st0 = vt.copy(), st.copy()
+    inv_v = np.float32(1.0 / 8.0e5)
+    sums = oracle.stress_twin(inv_v, vt, st, atom_virial=av)
+    ref = av.astype(np.float64).sum(0)
+    np.testing.assert_allclose(sums[:6] * 2.0 ** -32, ref, rtol=0, atol=1e-6 * np.abs(av).max())
+    assert np.all(sums[6:] == 0)
+    np.testing.assert_allclose(vt, vt0 + ref, rtol=2e-7, atol=1e-3)
+    np.testing.assert_allclose(st, st0 + float(inv_v) * (vt0 + ref), rtol=1e-6, atol=1e-9)

@@ -582,3 +582,37 @@ def test_bond_twin_matches_fp64(mixed_small):
     assert abs(ept.sum() * 2.0 ** -32 - ep64.sum()) < 1e-6 * ep64.sum()
     np.testing.assert_array_equal(np.bincount(s["bonds"][0], ept, s["n"]).astype(np.int64), eat)
     assert rel_rms(vt.T * 2.0 ** -32, v64) < 5e-5
+
+
+def test_pme_reciprocal_virial_is_half_the_strain_derivative(gromacs_iter0):
+    """The reciprocal virial as the reference forms it (PM_force.cpp:575-607,
+    PME_Sum_Virial :1021-1065) against central differences of E_rec when the box
+    and the coordinates are stretched along one axis (fractional coordinates
+    fixed, grid and beta fixed): each diagonal entry equals +1/2 dE_rec/dlambda.
+    (-dE/dlambda is the strain virial in the convention of the atom virials
+    f (x) dr, so the reference's reciprocal term carries the opposite sign and
+    half the size.)"""
+    s = gromacs_iter0
+    crd = np.asarray(s["crd"], np.float64)
+    L = np.asarray(s["box"], np.float64)
+    grid = (72, 72, 72)
+    beta = oracle.get_beta(12.0, 1e-5)
+
+    def erec(lam):
+        lam = np.asarray(lam, np.float64)
+        c = (crd * lam).astype(np.float32)
+        rc = np.array([1 / (L[0] * lam[0]), 0, 1 / (L[1] * lam[1]), 0, 0,
+                       1 / (L[2] * lam[2])], np.float32)
+        vol = float(np.float32(L[0] * lam[0]) * np.float32(L[1] * lam[1]) *
+                    np.float32(L[2] * lam[2]))
+        _, e, v, _, _ = oracle.pme_f64(c, s["charge"], rc, grid, beta, vol)
+        return e, v
+
+    _, vir = erec([1.0, 1.0, 1.0])
+    h = 1e-2
+    for axis, entry in ((0, 0), (1, 2), (2, 5)):
+        lp, lm = [1.0] * 3, [1.0] * 3
+        lp[axis] += h
+        lm[axis] -= h
+        d = (erec(lp)[0] - erec(lm)[0]) / (2 * h)
+        assert abs(vir[entry] - 0.5 * d) < 1e-3 * abs(vir[entry]), (axis, vir[entry], d)
