@@ -116,7 +116,11 @@ int snb_nl_initial(snb_context* ctx, float cutoff, float skin, snb_ltmatrix3 cel
  * runs on the device and gates the rebuild there (no host synchronisation).
  * atom_local may be
  * NULL (identity); ghost_numbers must be 0.  The exclusion arrays may be NULL
- * when snb_set_exclusions was called. */
+ * when snb_set_exclusions was called.  The cell may differ from the initial
+ * one (barostat steps): cells keep the initial width and the last cell of an
+ * axis takes the remainder, as in the reference; a box smaller than the
+ * initial cell count times that width returns SNB_ERR_BOX_TOO_SMALL
+ * (re-initialise, as Main_Box_Change_Largely does). */
 int snb_nl_update(snb_context* ctx, const int* d_atom_local, int local_atom_numbers,
                   int ghost_numbers, const snb_vector* d_crd, snb_ltmatrix3 cell,
                   snb_ltmatrix3 rcell, int step, int update,
@@ -348,6 +352,16 @@ int snb_pme_initial(snb_context* ctx, snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
                     int update_interval);
 int snb_pme_parameters(snb_context* ctx, float* beta, int* fftx, int* ffty, int* fftz);
 
+/* Particle_Mesh::Update_Box (PM_force.cpp:1251-1263, kernel up_box_bc :1168-1237):
+ * the small box change of a barostat step (Main_Box_Change, main.cpp:886-895).
+ * Recomputes the influence function (FP64, stored as float) and the
+ * neutralising factor for the new cell on the context stream; the grid size,
+ * beta and the brick layout stay.  g and dt only move the reference's domain
+ * corners and are not used.  A large change re-initialises instead
+ * (snb_pme_initial, as Main_Box_Change_Largely does). */
+int snb_pme_update_box(snb_context* ctx, snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
+                       snb_ltmatrix3 g, float dt);
+
 /* Particle_Mesh::MPI_PME_Excluded_Force_With_Atom_Energy (PM_force.cpp:1816-1839,
  * kernel :1743-1814) on one rank: removes the erf part of the Ewald
  * interaction of every excluded pair.  atom_local/atom_local_id may be NULL;
@@ -363,7 +377,9 @@ int snb_pme_excluded_force(snb_context* ctx, int atom_numbers, const int* d_atom
 /* Particle_Mesh::PME_Reciprocal_Force_With_Energy_And_Virial (PM_force.cpp:1067-1165):
  * B-spline spread, 3-D R2C FFT, influence-function convolution, C2R, force
  * gather; with need_energy adds reciprocal + self + neutralising energy to
- * d_potential[0]; with need_virial adds the reciprocal virial to d_virial[0]. */
+ * d_potential[0]; with need_virial adds the reciprocal virial to d_virial[0].
+ * rcell must be the one of the last snb_pme_initial / snb_pme_update_box
+ * (SNB_ERR_ARGUMENT otherwise: the influence function belongs to a box). */
 int snb_pme_reciprocal_force(snb_context* ctx, const snb_vector* d_crd, snb_ltmatrix3 cell,
                              snb_ltmatrix3 rcell, const float* d_charge, snb_vector* d_frc,
                              int need_virial, int need_energy, snb_ltmatrix3* d_virial,

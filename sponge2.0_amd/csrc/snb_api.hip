@@ -280,6 +280,7 @@ void free_pme(SnbPme& P)
     dfree(P.tw_y);
     dfree(P.tw_z);
     dfree(P.bc);
+    dfree(P.bmod);
     dfree(P.q);
     dfree(P.phi);
     dfree(P.spec);
@@ -656,6 +657,13 @@ int snb_nl_update(snb_context* c, const int* d_atom_local, int local_atom_number
     if (ghost_numbers != 0 || d_atom_local) return SNB_ERR_UNSUPPORTED;
     if (local_atom_numbers != c->n || (c->n > 0 && !d_crd)) return SNB_ERR_ARGUMENT;
     if (!orthorhombic(cell)) return SNB_ERR_UNSUPPORTED;
+    // cells keep the width of the initial box (grid_length) and the last one on
+    // each axis takes the remainder; a box that shrank below n cells of that
+    // width would leave a thinner cell and pairs beyond the +-2-cell stencil:
+    // re-initialise then (the reference does, main.cpp:880-884)
+    if (cell.a11 < c->nl.nx * c->nl.grid_length || cell.a22 < c->nl.ny * c->nl.grid_length ||
+        cell.a33 < c->nl.nz * c->nl.grid_length)
+        return SNB_ERR_BOX_TOO_SMALL;
     if (c->n == 0) return SNB_OK;
     DeviceGuard g(c->device);
     SnbBox box = snb_box(cell, rcell);
@@ -1338,6 +1346,29 @@ int snb_pme_initial(snb_context* c, snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
     return SNB_OK;
 }
 
+int snb_pme_update_box(snb_context* c, snb_ltmatrix3 cell, snb_ltmatrix3 rcell, snb_ltmatrix3 g,
+                       float dt)
+{
+    (void)g;  // g and dt move the domain-decomposition corners only (PM_force.cpp:1261-1262)
+    (void)dt;
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->pme.init) return SNB_ERR_NOT_INITIALIZED;
+    if (!(cell.a11 > 0.0f && cell.a22 > 0.0f && cell.a33 > 0.0f)) return SNB_ERR_ARGUMENT;
+    DeviceGuard guard(c->device);
+    SnbPme& P = c->pme;
+    // the previous step's pipeline may still read BC on the PME stream
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
+    // PM_force.cpp:1254-1256 (volume from the cell diagonal, float)
+    P.volume = (double)(cell.a11 * cell.a22 * cell.a33);
+    P.neutralizing_factor = (float)(-0.5 * M_PI / ((double)P.beta * P.beta * P.volume));
+    if ((rc = snb_pme_compute_bc(c, snb_box(cell, rcell)))) return rc;
+    // a pipeline of this step that was already cleared to start must also
+    // see the new table
+    if (c->overlap && c->inputs_marked) SNB_CHECK(hipEventRecord(c->ev_inputs, c->stream));
+    return SNB_OK;
+}
+
 int snb_pme_parameters(snb_context* c, float* beta, int* fx, int* fy, int* fz)
 {
     if (!c) return SNB_ERR_ARGUMENT;
@@ -1396,7 +1427,7 @@ int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatr
     if (c->n > 0 && (!d_crd || !d_charge)) return SNB_ERR_ARGUMENT;
     SnbPme& P = c->pme;
     for (int k = 0; k < 6; k++)
-        if ((&rcell.a11)[k] != P.rcell[k]) return SNB_ERR_UNSUPPORTED;  // box change: re-initialise
+        if ((&rcell.a11)[k] != P.rcell[k]) return SNB_ERR_ARGUMENT;  // box changed: snb_pme_update_box
     DeviceGuard g(c->device);
     SnbBox box = snb_box(cell, rcell);
     int rc;

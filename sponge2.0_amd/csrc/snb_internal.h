@@ -173,6 +173,7 @@ struct SnbPme
     float2* tw_y = nullptr;
     float2* tw_z = nullptr;
     float* bc = nullptr;      // [fx][fy][fc]
+    double* bmod = nullptr;   // B-spline moduli: fx, then fy, then fz entries
     float* q = nullptr;       // [fx][fy][fz]
     float* phi = nullptr;     // [fx][fy][fz]
     float2* spec = nullptr;   // [fx][fy][fc]
@@ -342,6 +343,9 @@ int snb_launch_settle(snb_context* ctx, const float* mass, snb_vector* crd, cons
 
 // PME (snb_pme.hip)
 int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box);
+// (Re)computes the influence function for the box on the context stream and
+// records its rcell (up_box_bc, PM_force.cpp:1168-1237).
+int snb_pme_compute_bc(snb_context* ctx, const SnbBox& box);
 int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* charge,
                          const SnbBox& box, int need_energy, int need_virial, int with_force);
 int snb_fft_forward_full(snb_context* ctx, const float* real, float2* cplx);

@@ -853,6 +853,44 @@ static int fft_w(int fc, int n)
     return fc < w ? fc : w;
 }
 
+// Influence function BC[kx][ky][kz] over the half spectrum (PM_force.cpp:544-612,
+// recomputed for a new box by up_box_bc, :1168-1237):
+//   BC = exp(-pi^2 m^2 / beta^2) / (pi m^2 V) * b1[kx] b2[ky] b3[kz],  BC[0] = 0,
+// with m = k' rcell^T (k' the signed index), in FP64, stored as float.
+// One block per (kx, ky) line, threads over kz.
+__global__ void k_pme_bc(int fx, int fy, int fc, const double* __restrict__ bmod, SnbBox box,
+                         double mpre, double volume, float* __restrict__ bc)
+{
+    const int kx = blockIdx.x / fy, ky = blockIdx.x % fy;
+    const double kxr = kx > fx / 2 ? kx - fx : kx;
+    const double kyr = ky > fy / 2 ? ky - fy : ky;
+    const double bxy = bmod[kx] * bmod[fx + ky];
+    for (int kz = threadIdx.x; kz < fc; kz += blockDim.x)
+    {
+        const double mx = kxr * box.rc[0];
+        const double my = kxr * box.rc[1] + kyr * box.rc[2];
+        const double mz = kxr * box.rc[3] + kyr * box.rc[4] + kz * box.rc[5];
+        const double msq = mx * mx + my * my + mz * mz;
+        const size_t idx = ((size_t)kx * fy + ky) * fc + kz;
+        bc[idx] = (kx + ky + kz == 0)
+                      ? 0.0f
+                      : (float)(exp(-mpre * msq) / (M_PI * msq * volume) * bxy *
+                                bmod[fx + fy + kz]);
+    }
+}
+
+int snb_pme_compute_bc(snb_context* ctx, const SnbBox& box)
+{
+    SnbPme& P = ctx->pme;
+    const double beta = P.beta;
+    const double mpre = M_PI * M_PI / (beta * beta);
+    hipLaunchKernelGGL(k_pme_bc, dim3(P.fx * P.fy), dim3(64), 0, ctx->stream, P.fx, P.fy, P.fc,
+                       P.bmod, box, mpre, P.volume, P.bc);
+    SNB_CHECK(hipGetLastError());
+    for (int c = 0; c < 6; c++) P.rcell[c] = box.rc[c];
+    return SNB_OK;
+}
+
 int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box)
 {
     SnbPme& P = ctx->pme;
@@ -871,7 +909,6 @@ int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box)
     P.nby = P.fy / P.by;
     P.nbz = P.fz / P.bz;
     P.nbrick = P.nbx * P.nby * P.nbz;
-    for (int c = 0; c < 6; c++) P.rcell[c] = box.rc[c];
     size_t nall = (size_t)P.fx * P.fy * P.fz;
     size_t nfft = (size_t)P.fx * P.fy * P.fc;
     int n = ctx->n;
@@ -900,36 +937,14 @@ int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box)
     int W = fft_w(P.fc, P.fx);
     P.conv_blocks = P.fy * ((P.fc + W - 1) / W);
     SNB_CHECK(hipMalloc(&P.partials, sizeof(double) * 8 * P.conv_blocks));
-    // influence function BC (PM_force.cpp:544-612) in double, stored as float
-    std::vector<double> b1(P.fx), b2(P.fy), b3(P.fz);
-    for (int k = 0; k < P.fx; k++) b1[k] = bspline_modulus(k, P.fx);
-    for (int k = 0; k < P.fy; k++) b2[k] = bspline_modulus(k, P.fy);
-    for (int k = 0; k < P.fz; k++) b3[k] = bspline_modulus(k, P.fz);
-    std::vector<float> h(nfft);
-    double beta = P.beta;
-    double mpre = M_PI * M_PI / (beta * beta);
-    for (int kx = 0; kx < P.fx; kx++)
-    {
-        double kxr = kx > P.fx / 2 ? kx - P.fx : kx;
-        for (int ky = 0; ky < P.fy; ky++)
-        {
-            double kyr = ky > P.fy / 2 ? ky - P.fy : ky;
-            for (int kz = 0; kz < P.fc; kz++)
-            {
-                double mx = kxr * box.rc[0];
-                double my = kxr * box.rc[1] + kyr * box.rc[2];
-                double mz = kxr * box.rc[3] + kyr * box.rc[4] + kz * box.rc[5];
-                double msq = mx * mx + my * my + mz * mz;
-                size_t idx = ((size_t)kx * P.fy + ky) * P.fc + kz;
-                h[idx] = (kx + ky + kz == 0)
-                             ? 0.0f
-                             : (float)(exp(-mpre * msq) / (M_PI * msq * P.volume) * b1[kx] *
-                                       b2[ky] * b3[kz]);
-            }
-        }
-    }
-    SNB_CHECK(hipMemcpy(P.bc, h.data(), sizeof(float) * nfft, hipMemcpyHostToDevice));
-    return SNB_OK;
+    // B-spline moduli |b(k)|^2 per axis (PM_force.cpp:40-62), then BC on the device
+    std::vector<double> bm(P.fx + P.fy + P.fz);
+    for (int k = 0; k < P.fx; k++) bm[k] = bspline_modulus(k, P.fx);
+    for (int k = 0; k < P.fy; k++) bm[P.fx + k] = bspline_modulus(k, P.fy);
+    for (int k = 0; k < P.fz; k++) bm[P.fx + P.fy + k] = bspline_modulus(k, P.fz);
+    SNB_CHECK(hipMalloc(&P.bmod, sizeof(double) * bm.size()));
+    SNB_CHECK(hipMemcpy(P.bmod, bm.data(), sizeof(double) * bm.size(), hipMemcpyHostToDevice));
+    return snb_pme_compute_bc(ctx, box);
 }
 
 static size_t lds_lines(int n, int L) { return sizeof(float2) * ((size_t)n + 2 * (size_t)L * n); }

@@ -88,6 +88,7 @@ _SIGS = {
     "snb_get_pressure": ([_P, _I, _P, _P, _P, LTMatrix3, _P, _P, ctypes.POINTER(LTMatrix3),
                           ctypes.POINTER(_F)], _I),
     "snb_pme_initial": ([_P, LTMatrix3, LTMatrix3, Vector, _F, _F, _I, _I, _I, _F, _I], _I),
+    "snb_pme_update_box": ([_P, LTMatrix3, LTMatrix3, LTMatrix3, _F], _I),
     "snb_pme_parameters": ([_P, ctypes.POINTER(_F), ctypes.POINTER(_I), ctypes.POINTER(_I),
                             ctypes.POINTER(_I)], _I),
     "snb_pme_excluded_force": ([_P, _I, _P, _P, _P, LTMatrix3, LTMatrix3, _P, _P, _P, _P, _P,
@@ -174,6 +175,7 @@ class Nonbonded:
         self.cell = LTMatrix3.of(sysd["cell"])
         self.rcell = LTMatrix3.of(sysd["rcell"])
         self.box = np.asarray(sysd["box"], np.float32)
+        self.angle = tuple(float(x) for x in sysd.get("angle", (90.0, 90.0, 90.0)))
         ctx = ctypes.c_void_p()
         check(L.snb_context_create(ctypes.byref(ctx), device,
                                    ctypes.c_void_p(self.stream.cuda_stream), self.n),
@@ -441,6 +443,19 @@ class Nonbonded:
         self.md_iterate(need_pressure=need_virial)
         if need_virial:
             self.get_pressure(copy=False)
+
+    def update_box(self, box):
+        """A barostat's small box change (Main_Box_Change, main.cpp:860-897):
+        new cell / rcell / volume for every later call and the PME influence
+        function for the new box (Particle_Mesh::Update_Box).  Coordinates are
+        the caller's to scale."""
+        from . import native_io
+        cell, rcell = native_io.box_to_cell(box, self.angle)
+        self.cell, self.rcell = LTMatrix3.of(cell), LTMatrix3.of(rcell)
+        self.box = np.asarray(box, np.float32)
+        self.volume = float(np.float32(cell[0]) * np.float32(cell[2]) * np.float32(cell[5]))
+        check(self.L.snb_pme_update_box(self.ctx, self.cell, self.rcell, LTMatrix3(), 0.0),
+              "snb_pme_update_box")
 
     def get_pressure(self, copy=True):
         """MD_INFORMATION::Get_pressure (MD_core.cpp:948-975) on the current

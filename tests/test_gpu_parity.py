@@ -661,3 +661,57 @@ def test_nb14_outputs_flush_and_energy_restart(gromacs_iter0):
     lj, ee = nb.nb14_energy()
     assert lj == float(t_ep[0].sum()) * 2.0 ** -32
     nb.close()
+
+
+# --------------------------------------------------------------------------
+# box change (barostat step)
+# --------------------------------------------------------------------------
+def test_box_update_matches_fresh_context(water_9k):
+    """Particle_Mesh::Update_Box + a neighbour-list update on the new cell:
+    after the box and the coordinates are scaled by 1.004, the step equals,
+    bit for bit, the step of a context built on the scaled system (same grid),
+    and stays within the force/energy bounds of the FP64 oracle; a reciprocal
+    call with a stale rcell and a box below the initial cells' width are
+    refused before anything is launched."""
+    import oracle
+    from sponge_amd import native_io
+    from sponge_amd.lib import SnbError
+    s = water_9k
+    nb = _nb(s)
+    e0, f0 = _full_terms(nb)
+    lam = 1.004
+    s2 = dict(s)
+    s2["box"] = (np.asarray(s["box"], np.float64) * lam).astype(np.float32)
+    s2["crd"] = (np.asarray(s["crd"], np.float64) * lam).astype(np.float32)
+    s2["cell"], s2["rcell"] = native_io.box_to_cell(s2["box"], s.get("angle", (90, 90, 90)))
+    old_rcell = nb.rcell
+    nb.update_box(s2["box"])
+    nb.crd.copy_(nb.torch.as_tensor(s2["crd"]))
+    e1, f1 = _full_terms(nb)
+    nb2 = _nb(s2, grid=nb.grid)
+    e2, f2 = _full_terms(nb2)
+    print("PM", e0["PM"], e1["PM"], e2["PM"], flush=True)
+    np.testing.assert_array_equal(f1, f2)
+    assert e1 == e2
+    assert list(nb.reciprocal_virial()) == list(nb2.reciprocal_virial())
+    ref = oracle.nonbonded_terms_f64(s2, cutoff=nb.cutoff, grid=nb.grid)
+    assert rel_rms(f1, ref["frc"]) < FORCE_TOL
+    assert abs(e1["PM"] - ref["PM"]) <= ENERGY_TOL * abs(ref["PM"])
+    assert abs(e1["LJ_long"] - ref["LJ_long"]) <= 1e-6 * abs(ref["LJ_long"])
+    # stale rcell: refused
+    with pytest.raises(SnbError):
+        from sponge_amd.lib import check, _ptr
+        check(nb.L.snb_pme_reciprocal_force(nb.ctx, _ptr(nb.crd), nb.cell, old_rcell,
+                                            _ptr(nb.charge), None, 0, 0, None, None, 0),
+              "snb_pme_reciprocal_force")
+    # a box thinner than the initial cell count allows: refused
+    small = np.asarray(s["box"], np.float32) * np.float32(0.9)
+    cell, rcell = native_io.box_to_cell(small)
+    from sponge_amd.lib import LTMatrix3
+    rc = nb.L.snb_nl_update(nb.ctx, None, nb.n, 0, _ptr(nb.crd), LTMatrix3.of(cell),
+                            LTMatrix3.of(rcell), 0, 1, None, None, None)
+    assert rc == 3                                  # SNB_ERR_BOX_TOO_SMALL
+    nb.sync()
+    assert nb.status() == 0
+    nb.close()
+    nb2.close()
