@@ -49,6 +49,14 @@ def parse():
     ap.add_argument("--md-steps", type=int, default=500,
                     help="also time this many whole MD steps (forces + middle Langevin + "
                          "SETTLE) after the force-path measurement; 0 skips")
+    ap.add_argument("--serve-requests", type=int, default=64,
+                    help="serving workload: this many water-box segment requests (N log-uniform "
+                         "1,500-50,000 atoms, Poisson arrivals at 40/s), served one at a time and "
+                         "with continuous batching; 0 skips (the full workload is 256)")
+    ap.add_argument("--serve-steps", type=int, default=1000,
+                    help="MD steps per serving request (2 fs)")
+    ap.add_argument("--serve-slots", type=int, default=4,
+                    help="segments in flight in the batched run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (scales the HBM byte counters "
                          "of a profiler run)")
@@ -114,6 +122,47 @@ def accuracy(sysd, nb, counts, rows, pme, gpu_frc):
         ref = ref + oracle.bond_f64(s["crd"], s["cell"], s["rcell"], s["bonds"], virial=False)[0]
     g = gpu_frc.astype(np.float64)
     return float(np.sqrt(((g - ref) ** 2).sum() / (ref ** 2).sum()))
+
+
+def serving_phase(args):
+    """The serving workload: the same seeded request stream served twice, one
+    segment at a time (sequential plain runs) and with continuous batching;
+    window shapes tuned once per grid class beforehand (untimed, both runs
+    use them).  Every request's final coordinates and velocities are compared
+    between the two runs (alone vs in the mix)."""
+    import numpy as np
+    from sponge_amd import serving
+    t = time.perf_counter()
+    tuned = serving.pretune(serving.make_requests(count=args.serve_requests,
+                                                  steps=args.serve_steps))
+    tune_s = time.perf_counter() - t
+    runs = {}
+    reqs = {}
+    for name, slots in (("sequential", 1), ("batched", args.serve_slots)):
+        rq = serving.make_requests(count=args.serve_requests, steps=args.serve_steps)
+        runs[name] = serving.Server(slots=slots).run(rq, realtime=True)
+        reqs[name] = rq
+    same = sum(int(np.array_equal(a.result["crd"], b.result["crd"]) and
+                   np.array_equal(a.result["vel"], b.result["vel"]))
+               for a, b in zip(reqs["sequential"], reqs["batched"]))
+    status = sum(r.result["status"] for v in reqs.values() for r in v)
+    seq, bat = runs["sequential"], runs["batched"]
+    return {"requests": args.serve_requests, "steps_per_request": args.serve_steps,
+            "atoms_total": bat["atoms_total"], "arrival": "Poisson, 40/s",
+            "batched_aggregate_ns_per_day": bat["aggregate_ns_per_day"],
+            "batched_p50_latency_s": bat["p50_latency_s"],
+            "batched_p99_latency_s": bat["p99_latency_s"],
+            "sequential_aggregate_ns_per_day": seq["aggregate_ns_per_day"],
+            "sequential_p50_latency_s": seq["p50_latency_s"],
+            "sequential_p99_latency_s": seq["p99_latency_s"],
+            "batched_over_sequential": bat["aggregate_ns_per_day"] / seq["aggregate_ns_per_day"],
+            "slots": args.serve_slots, "makespan_s": {"sequential": seq["makespan_s"],
+                                                      "batched": bat["makespan_s"]},
+            "same_alone_and_in_mix": "%d of %d requests bit-identical" % (same, len(reqs["batched"])),
+            "status_flags": status, "tune_s": tune_s,
+            "shapes": {str(k): list(v[0]) for k, v in tuned.items()},
+            "launch": "per-segment HIP stream, one captured graph per 10-step period, "
+                      "periods queued 2 ahead"}
 
 
 def md_phase(nb, sysd, args, kind):
@@ -353,8 +402,10 @@ def main():
     }
     if args.md_steps > 0:
         out["md"] = md_phase(nb, sysd, args, kind)
-    print(json.dumps(out))
     nb.close()
+    if args.serve_requests > 0:
+        out["serving"] = serving_phase(args)
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

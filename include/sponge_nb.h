@@ -160,6 +160,15 @@ int snb_lj_set_owner_attribution(snb_context* ctx, int enable);
  * A smaller reservation lets more work of the reciprocal stream share a
  * compute unit (no counterpart in the reference). */
 int snb_nl_set_window_capacity(snb_context* ctx, int slots);
+/* Block shape of the windowed pair path: each workgroup owns a x b x s cells
+ * (x, y, z); 0, 0, 0 selects the plain path (one row per thread group over
+ * the public list, global integer atomics).  The default comes from a fixed
+ * preference list; a tuner can time the shapes this call accepts and keep the
+ * fastest.  SNB_ERR_ARGUMENT when the grid cannot take the shape or its
+ * window does not fit the LDS.  Synchronises; the next snb_nl_update
+ * rebuilds.  Forces, energies and virials are the same for every shape (no
+ * counterpart in the reference). */
+int snb_nl_set_window_shape(snb_context* ctx, int a, int b, int s);
 /* Geometry of the windowed pair path (windowed = 0 when not used). */
 int snb_nl_window_info(snb_context* ctx, int* windowed, int* a, int* b, int* s, int* wcap,
                        int* nblock);

@@ -99,6 +99,8 @@ bool orthorhombic(snb_ltmatrix3 c)
     return c.a21 == 0.0f && c.a31 == 0.0f && c.a32 == 0.0f;
 }
 
+void free_window(SnbNl& L);
+
 void free_nl(SnbNl& L)
 {
     dfree(L.cell_count);
@@ -113,6 +115,12 @@ void free_nl(SnbNl& L)
     dfree(L.old_crd);
     dfree(L.order);
     dfree(L.sorted_cell);
+    free_window(L);
+    L.init = false;
+}
+
+void free_window(SnbNl& L)
+{
     dfree(L.nl16);
     dfree(L.row_count);
     dfree(L.row_split);
@@ -126,7 +134,6 @@ void free_nl(SnbNl& L)
     dfree(L.block_work);
     dfree(L.block_perm);
     L.windowed = false;
-    L.init = false;
 }
 
 // Chooses the block of the windowed pair path (snb_window.h): the largest
@@ -144,44 +151,58 @@ bool choose_window(int nx, int ny, int nz, double atoms_per_cell, int max_lds, i
            choose_window_pass(nx, ny, nz, atoms_per_cell, max_lds, 1, W);
 }
 
+// LDS slots of the largest window one workgroup can hold: 3 x int64
+// accumulator + an int slot map per slot, plus the fixed part
+static int window_slots_max(int max_lds)
+{
+    int wcap = (max_lds - SNB_WIN_FIXED_LDS) / 28;
+    return wcap > 65535 ? 65535 : wcap;
+}
+
+// Fills W for blocks of a x b x s cells when the grid allows that shape and
+// its expected window (atoms_per_cell per cell) fits `fill` of the LDS.
+static bool window_shape(int nx, int ny, int nz, int a, int b, int s, double atoms_per_cell,
+                         int max_lds, double fill, SnbWin* W)
+{
+    if (nx < 5 || ny < 5 || nz < 5 || a < 1 || b < 1 || s < 1) return false;
+    if (a > nx - 2 || b > ny - 4 || s > nz - 4) return false;
+    if (2 * (a + 2) * (b + 4) > SNB_WIN_MAX_RANGES) return false;
+    const int wcap = window_slots_max(max_lds);
+    double expect = (double)(a + 2) * (b + 4) * (s + 4) * atoms_per_cell;
+    if (expect > fill * wcap) return false;
+    W->nx = nx;
+    W->ny = ny;
+    W->nz = nz;
+    W->a = a;
+    W->b = b;
+    W->s = s;
+    W->nbx = (nx + a - 1) / a;
+    W->nby = (ny + b - 1) / b;
+    W->nbz = (nz + s - 1) / s;
+    W->nblock = W->nbx * W->nby * W->nbz;
+    W->wu = a + 2;
+    W->wv = b + 4;
+    W->wz = s + 4;
+    W->nrange = 2 * W->wu * W->wv;
+    // request LDS for 1.3x the mean window (the densest windows of the
+    // test systems reach 1.15x; a larger one takes the overflow path) so
+    // that small kernels of the reciprocal stream can share the CU
+    int want = (int)(1.3 * expect) + 64;
+    W->wcap = want < wcap ? want : wcap;
+    return true;
+}
+
 bool choose_window_pass(int nx, int ny, int nz, double atoms_per_cell, int max_lds,
                         int min_blocks, SnbWin* W)
 {
-    if (nx < 5 || ny < 5 || nz < 5) return false;
-    // LDS per slot: 3 x int64 accumulator + int slot map; plus the fixed part
-    int wcap = (max_lds - SNB_WIN_FIXED_LDS) / 28;
-    if (wcap > 65535) wcap = 65535;
     static const int cand[][3] = {{2, 2, 8}, {2, 2, 6}, {2, 2, 4}, {2, 2, 3}, {2, 2, 2},
                                   {1, 2, 4}, {1, 1, 4}, {1, 1, 2}, {1, 1, 1}};
     for (const auto& c : cand)
     {
         int a = c[0], b = c[1], s = c[2];
-        if (a > nx - 2 || b > ny - 4 || s > nz - 4) continue;
-        if (2 * (a + 2) * (b + 4) > SNB_WIN_MAX_RANGES) continue;
-        double expect = (double)(a + 2) * (b + 4) * (s + 4) * atoms_per_cell;
-        if (expect > 0.75 * wcap) continue;
         long nblk = (long)((nx + a - 1) / a) * ((ny + b - 1) / b) * ((nz + s - 1) / s);
         if (nblk < min_blocks) continue;
-        W->nx = nx;
-        W->ny = ny;
-        W->nz = nz;
-        W->a = a;
-        W->b = b;
-        W->s = s;
-        W->nbx = (nx + a - 1) / a;
-        W->nby = (ny + b - 1) / b;
-        W->nbz = (nz + s - 1) / s;
-        W->nblock = W->nbx * W->nby * W->nbz;
-        W->wu = a + 2;
-        W->wv = b + 4;
-        W->wz = s + 4;
-        W->nrange = 2 * W->wu * W->wv;
-        // request LDS for 1.3x the mean window (the densest windows of the
-        // test systems reach 1.15x; a larger one takes the overflow path) so
-        // that small kernels of the reciprocal stream can share the CU
-        int want = (int)(1.3 * expect) + 64;
-        W->wcap = want < wcap ? want : wcap;
-        return true;
+        if (window_shape(nx, ny, nz, a, b, s, atoms_per_cell, max_lds, 0.75, W)) return true;
     }
     return false;
 }
@@ -554,6 +575,46 @@ int snb_set_exclusions(snb_context* c, const int* h_start, const int* h_list, co
 }
 
 // ----------------------------------------------------------------- neighbor list
+static bool window_list_fits(const snb_context* c)
+{
+    return (double)c->n * c->nl.max_nb * sizeof(uint16_t) < 4294967296.0;
+}
+
+// Allocates the windowed path's buffers for the shape in c->nl.win.
+static int setup_window(snb_context* c)
+{
+    SnbNl& L = c->nl;
+    const SnbWin& W = L.win;
+    const int n = c->n;
+    int rc = SNB_OK;
+    L.invmax = window_cover(W.wu, W.a, W.nx) * window_cover(W.wv, W.b, W.ny) *
+               window_cover(W.wz, W.s, W.nz);
+    if (!rc) rc = dalloc(&L.nl16, (size_t)n * L.max_nb);
+    if (!rc) rc = dalloc(&L.row_count, n);
+    if (!rc) rc = dalloc(&L.row_split, n);
+    if (!rc) rc = dalloc(&L.win_size, W.nblock + 1);
+    if (!rc) rc = dalloc(&L.stage_base, W.nblock + 1);
+    // every atom sits in at most invmax windows
+    if (!rc) rc = dalloc(&L.stage, 3 * (size_t)n * L.invmax);
+    if (!rc) rc = dalloc(&L.inv_count, n);
+    if (!rc) rc = dalloc(&L.inv_list, (size_t)n * L.invmax);
+    if (!rc) rc = dalloc(&L.pub_count, n);
+    if (!rc) rc = dalloc(&L.ex_rng, n);
+    if (!rc) rc = dalloc(&L.block_work, W.nblock);
+    if (!rc) rc = dalloc(&L.block_perm, W.nblock);
+    if (rc)
+    {
+        free_window(L);
+        return rc;
+    }
+    L.windowed = true;
+    snb_launch_fill_int(c->stream, L.row_count, n, 0);
+    snb_launch_fill_int(c->stream, L.row_split, n, 0);
+    float near = L.cutoff + 0.25f * L.skin;
+    L.near2 = near * near;
+    return SNB_OK;
+}
+
 int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
                    snb_ltmatrix3 rcell, int max_neighbor_numbers, int max_atom_in_grid_numbers,
                    int refresh_interval, float skin_permit)
@@ -601,36 +662,13 @@ int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
         return rc;
     }
     // the windowed kernels address the list with 32-bit byte offsets
-    L.windowed = (double)n * L.max_nb * sizeof(uint16_t) < 4294967296.0 &&
+    L.windowed = window_list_fits(c) &&
                  choose_window(L.nx, L.ny, L.nz, (double)n / L.ncell, c->max_lds, 2 * c->num_cu,
                                &L.win);
-    if (L.windowed)
+    if (L.windowed && (rc = setup_window(c)))
     {
-        const SnbWin& W = L.win;
-        L.invmax = window_cover(W.wu, W.a, W.nx) * window_cover(W.wv, W.b, W.ny) *
-                   window_cover(W.wz, W.s, W.nz);
-        if (!rc) rc = dalloc(&L.nl16, (size_t)n * L.max_nb);
-        if (!rc) rc = dalloc(&L.row_count, n);
-        if (!rc) rc = dalloc(&L.row_split, n);
-        if (!rc) rc = dalloc(&L.win_size, W.nblock + 1);
-        if (!rc) rc = dalloc(&L.stage_base, W.nblock + 1);
-        // every atom sits in at most invmax windows
-        if (!rc) rc = dalloc(&L.stage, 3 * (size_t)n * L.invmax);
-        if (!rc) rc = dalloc(&L.inv_count, n);
-        if (!rc) rc = dalloc(&L.inv_list, (size_t)n * L.invmax);
-        if (!rc) rc = dalloc(&L.pub_count, n);
-        if (!rc) rc = dalloc(&L.ex_rng, n);
-        if (!rc) rc = dalloc(&L.block_work, W.nblock);
-        if (!rc) rc = dalloc(&L.block_perm, W.nblock);
-        if (rc)
-        {
-            free_nl(L);
-            return rc;
-        }
-        snb_launch_fill_int(c->stream, L.row_count, n, 0);
-        snb_launch_fill_int(c->stream, L.row_split, n, 0);
-        float near = cutoff + 0.25f * skin;
-        L.near2 = near * near;
+        free_nl(L);
+        return rc;
     }
     // ATOM_GROUP rows point into the padded table (neighbor_list.cpp:764-771)
     std::vector<snb_atom_group> h(n > 0 ? n : 1);
@@ -747,6 +785,28 @@ int snb_nl_set_window_capacity(snb_context* c, int slots)
     SNB_CHECK(hipStreamSynchronize(c->stream));  // no pair launch in flight uses the old size
     c->nl.win.wcap = slots < most ? slots : most;
     return SNB_OK;
+}
+
+int snb_nl_set_window_shape(snb_context* c, int a, int b, int s)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    SnbNl& L = c->nl;
+    if (!L.init) return SNB_ERR_NOT_INITIALIZED;
+    const bool plain = a == 0 && b == 0 && s == 0;
+    SnbWin W;
+    if (!plain && (!window_list_fits(c) ||
+                   !window_shape(L.nx, L.ny, L.nz, a, b, s, (double)c->n / L.ncell, c->max_lds,
+                                 1.0, &W)))
+        return SNB_ERR_ARGUMENT;
+    DeviceGuard g(c->device);
+    int rc = snb_join_pme(c);
+    if (rc) return rc;
+    SNB_CHECK(hipStreamSynchronize(c->stream));  // no launch in flight uses the old buffers
+    free_window(L);
+    L.built = false;  // the next snb_nl_update rebuilds for the new path
+    if (plain) return SNB_OK;
+    L.win = W;
+    return setup_window(c);
 }
 
 int snb_nl_window_info(snb_context* c, int* windowed, int* a, int* b, int* s, int* wcap,

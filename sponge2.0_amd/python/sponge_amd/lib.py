@@ -63,6 +63,7 @@ _SIGS = {
     "snb_nl_copy_internal": ([_P, _P, _P], _I),
     "snb_nl_window_info": ([_P] + [ctypes.POINTER(_I)] * 6, _I),
     "snb_nl_set_window_capacity": ([_P, _I], _I),
+    "snb_nl_set_window_shape": ([_P, _I, _I, _I], _I),
     "snb_nl_grid_shape": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "snb_lj_initial": ([_P, _I, _P, _P, _P, _F], _I),
     "snb_lj_pme_direct_force": ([_P, _I, _I, _I, _I, _P, _P, _P, LTMatrix3, LTMatrix3, _P, _F,
@@ -553,6 +554,15 @@ class Nonbonded:
     def set_window_capacity(self, slots):
         check(self.L.snb_nl_set_window_capacity(self.ctx, int(slots)),
               "snb_nl_set_window_capacity")
+
+    def set_window_shape(self, a, b, s):
+        """Block shape of the windowed pair path, (0, 0, 0) = plain path;
+        returns False when the grid or the LDS cannot take it."""
+        rc = self.L.snb_nl_set_window_shape(self.ctx, int(a), int(b), int(s))
+        if rc == 1:      # SNB_ERR_ARGUMENT: shape not possible here
+            return False
+        check(rc, "snb_nl_set_window_shape")
+        return True
 
     def window_info(self):
         v = [_I() for _ in range(6)]
