@@ -49,6 +49,12 @@ def parse():
     ap.add_argument("--md-steps", type=int, default=500,
                     help="also time this many whole MD steps (forces + middle Langevin + "
                          "SETTLE) after the force-path measurement; 0 skips")
+    ap.add_argument("--nve-steps", type=int, default=20000,
+                    help="precision study: NVE steps at 1 fs (SETTLE-rigid waters) after a "
+                         "Langevin equilibration, total-energy drift per ns per degree of "
+                         "freedom; 0 skips (the full study is 100,000)")
+    ap.add_argument("--nve-equil", type=int, default=2000,
+                    help="Langevin steps (300 K, tau 0.1 ps, 1 fs) before the NVE run")
     ap.add_argument("--serve-requests", type=int, default=64,
                     help="serving workload: this many water-box segment requests (N log-uniform "
                          "1,500-50,000 atoms, Poisson arrivals at 40/s), served one at a time and "
@@ -138,9 +144,13 @@ def serving_phase(args):
     tune_s = time.perf_counter() - t
     runs = {}
     reqs = {}
-    for name, slots in (("sequential", 1), ("batched", args.serve_slots)):
+    # the sequential runs use the plain path's default (reciprocal pipeline on
+    # its own stream beside the direct-space work); the batched run keeps
+    # each segment on one stream, its segments overlap each other instead
+    for name, slots, overlap in (("sequential", 1, True),
+                                 ("batched", args.serve_slots, False)):
         rq = serving.make_requests(count=args.serve_requests, steps=args.serve_steps)
-        runs[name] = serving.Server(slots=slots).run(rq, realtime=True)
+        runs[name] = serving.Server(slots=slots, overlap=overlap).run(rq, realtime=True)
         reqs[name] = rq
     same = sum(int(np.array_equal(a.result["crd"], b.result["crd"]) and
                    np.array_equal(a.result["vel"], b.result["vel"]))
@@ -163,6 +173,49 @@ def serving_phase(args):
             "shapes": {str(k): list(v[0]) for k, v in tuned.items()},
             "launch": "per-segment HIP stream, one captured graph per 10-step period, "
                       "periods queued 2 ahead"}
+
+
+def nve_phase(nb, sysd, args):
+    """Energy conservation of the FP32 / int64 fixed-point path: Maxwell
+    velocities at 300 K, a Langevin equilibration, then NVE leap-frog at 1 fs
+    with SETTLE; the total energy (potential of the path + bonds + 1-4 terms,
+    kinetic as the mean of the two half-step values around each sample) every
+    100 steps, and its linear drift per ns per degree of freedom."""
+    import numpy as np
+    nb.md_initial(sysd, temperature=300.0, tau=0.1, seed=20260218, dt=0.001)
+    for k in range(args.nve_equil):
+        nb.md_step(step=k, forced_nl=(k == 0))
+    nb.sync()
+    sample = 100
+    t_ps, etot, epot = [], [], []
+    t0 = time.perf_counter()
+    for k in range(args.nve_steps):
+        e = (k % sample) == 0
+        if e:
+            ke0 = nb.kinetic_energy()
+        nb.nve_step(step=k, need_energy=e, forced_nl=(k == 0))
+        if e:
+            u = nb.potential_energy()
+            ke1 = nb.kinetic_energy()
+            t_ps.append(k * 0.001)
+            epot.append(u)
+            etot.append(u + 0.5 * (ke0 + ke1))
+    nb.sync()
+    wall = time.perf_counter() - t0
+    nw = int(sysd["n_waters"])
+    dof = 6 * nw + 3 * (sysd["n"] - 3 * nw) - 3
+    t = np.asarray(t_ps)
+    E = np.asarray(etot)
+    slope = float(np.polyfit(t, E, 1)[0]) if len(t) > 1 else 0.0   # kcal/mol per ps
+    kt = 0.00198716 * 300.0
+    return {"steps": args.nve_steps, "dt_fs": 1.0, "equilibration_steps": args.nve_equil,
+            "dof": dof, "samples": len(E), "E_total_first": float(E[0]),
+            "E_total_last": float(E[-1]), "E_total_std": float(E.std()),
+            "drift_kcal_per_mol_per_ns_per_dof": slope * 1000.0 / dof,
+            "drift_kT_per_ns_per_dof": slope * 1000.0 / dof / kt,
+            "wall_s": wall, "status_flags": nb.status(),
+            "arithmetic": "FP32 pair/spline/integrator math, int64 fixed-point forces, "
+                          "FP64 energy reductions"}
 
 
 def md_phase(nb, sysd, args, kind):
@@ -402,6 +455,8 @@ def main():
     }
     if args.md_steps > 0:
         out["md"] = md_phase(nb, sysd, args, kind)
+    if args.nve_steps > 0:
+        out["nve"] = nve_phase(nb, sysd, args)
     nb.close()
     if args.serve_requests > 0:
         out["serving"] = serving_phase(args)

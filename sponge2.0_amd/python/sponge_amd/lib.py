@@ -458,6 +458,38 @@ class Nonbonded:
         check(self.L.snb_pme_update_box(self.ctx, self.cell, self.rcell, LTMatrix3(), 0.0),
               "snb_pme_update_box")
 
+    def nve_step(self, step=0, need_energy=False, forced_nl=False):
+        """One NVE step (mode NVE of Main_Iteration, main.cpp:634-712): the
+        force path, SETTLE's remember, the leap-frog update
+        (NVE_iteration::Leap_Frog, MD_core/nve.hpp:3-48) with the time step of
+        md_initial, then SETTLE with the constraint factors of a plain
+        leap-frog (v_factor = x_factor = 1, constrain/constrain.h:25-28)."""
+        self.frc.zero_()
+        self.step(step=step, need_energy=need_energy, forced_nl=forced_nl, fused=True)
+        check(self.L.snb_settle_remember(self.ctx, _ptr(self.crd), self.cell, self.rcell),
+              "snb_settle_remember")
+        check(self.L.snb_leap_frog(self.ctx, _ptr(self.inv_mass), _ptr(self.frc), _ptr(self.vel),
+                                   _ptr(self.crd), self.dt_internal, 0.0), "snb_leap_frog")
+        check(self.L.snb_settle_do(self.ctx, _ptr(self.mass), _ptr(self.crd), self.cell,
+                                   self.rcell, _ptr(self.vel), 0, _ptr(self.stress),
+                                   self.dt_internal, 1.0, 1.0), "snb_settle_do")
+
+    def potential_energy(self):
+        """LJ (short + long) + PM + 1-4 (LJ + Coulomb) + bond energies of the
+        last energy step (kcal/mol; synchronises)."""
+        e = self.energies()
+        u = e["LJ"] + e["PM"]
+        if self.nb14 is not None:
+            u += sum(self.nb14_energy())
+        if self.bonds is not None:
+            u += self.bond_energy()
+        return u
+
+    def kinetic_energy(self):
+        """0.5 sum m v^2 of the current velocities, in FP64 on the host."""
+        v = self.vel.cpu().numpy().astype(np.float64)
+        return 0.5 * float((self.mass.cpu().numpy().astype(np.float64)[:, None] * v * v).sum())
+
     def get_pressure(self, copy=True):
         """MD_INFORMATION::Get_pressure (MD_core.cpp:948-975) on the current
         velocities: the reciprocal and long-range virial (this wrapper keeps it

@@ -254,3 +254,56 @@ def test_md_pressure_steps_match_eager_assembly(water_9k):
     print("pressure (bar)", p * 6.946827162543585e4, flush=True)
     assert np.isfinite(p)
     nb.close()
+
+
+def test_nve_conserves_energy_and_final_frame_matches_fp64(water_9k):
+    """Precision: after 300 Langevin steps at 1 fs, 1,000 NVE leap-frog steps
+    with SETTLE keep the total energy (path potential + kinetic, sampled
+    every 50 steps) within 0.3% of the kinetic energy of its start, and the
+    fitted change over the 1 ps below 2e-3 kT per degree of freedom (the
+    long-run drift per ns is the bench's 20 ps measurement); on the final
+    (dynamically evolved) frame the forces are within 5e-5 relative RMS and
+    the LJ / PM energies within 1e-6 of the FP64 oracle."""
+    import oracle
+    from conftest import rel_rms
+    s = water_9k
+    nb = _nb(s, refresh_interval=10)
+    nb.md_initial(s, temperature=300.0, tau=0.1, seed=11, dt=0.001)
+    for k in range(300):
+        nb.md_step(step=k, forced_nl=(k == 0))
+    t, E = [], []
+    ke_start = None
+    for k in range(1000):
+        e = k % 50 == 0
+        if e:
+            ke0 = nb.kinetic_energy()
+        nb.nve_step(step=k, need_energy=e, forced_nl=(k == 0))
+        if e:
+            ke1 = nb.kinetic_energy()
+            ke = 0.5 * (ke0 + ke1)
+            ke_start = ke if ke_start is None else ke_start
+            t.append(k * 0.001)
+            E.append(nb.potential_energy() + ke)
+    E = np.asarray(E)
+    dof = 6 * (s["n"] // 3) - 3
+    slope = np.polyfit(np.asarray(t), E, 1)[0]
+    change_kt = slope * 1.0 / dof / (0.00198716 * 300.0)     # over the 1 ps
+    print("E_total", E[0], E[-1], "std", E.std(), "change kT/dof over 1 ps", change_kt,
+          flush=True)
+    assert np.abs(E - E[0]).max() < 3e-3 * ke_start
+    assert abs(change_kt) < 2e-3
+    assert nb.status() == 0
+    # the final frame against the FP64 oracle
+    s2 = dict(s)
+    s2["crd"] = nb.crd.cpu().numpy().copy()
+    nb.zero_outputs()
+    nb.step(step=0, need_energy=True, need_virial=True, forced_nl=True, fused=True)
+    nb.sync()
+    e = nb.energies()
+    ref = oracle.nonbonded_terms_f64(s2, cutoff=nb.cutoff, grid=nb.grid)
+    err = rel_rms(nb.frc.cpu().numpy().astype(np.float64), ref["frc"])
+    print("final frame force rel RMS", err, flush=True)
+    assert err < 5e-5
+    assert abs(e["LJ_short"] - ref["LJ_short"]) <= 1e-6 * abs(ref["LJ_short"])
+    assert abs(e["PM"] - ref["PM"]) <= 1e-6 * abs(ref["PM"])
+    nb.close()
