@@ -316,6 +316,20 @@ int snb_settle_do(snb_context* ctx, const float* d_mass, snb_vector* d_crd, snb_
                   snb_ltmatrix3 rcell, snb_vector* d_vel, int need_pressure,
                   snb_ltmatrix3* d_stress, float dt, float v_factor, float x_factor);
 
+/* Fused iteration (no counterpart in the reference, whose main.cpp:634-712
+ * makes these separate calls): SETTLE's Remember_Last_Coordinates, the middle
+ * Langevin update (MIDDLE_Langevin_INFORMATION::MD_Iteration_Leap_Frog) and
+ * Do_SETTLE with the thermostat's constraint factors, one thread per rigid
+ * triangle or free atom, with the forces taken from the context's fixed-point
+ * accumulators (the force calls of the step ran with d_frc = NULL) and
+ * cleared.  The result is bit-identical to snb_flush_forces into a zeroed
+ * force array followed by the three separate calls; d_frc is not written.
+ * Needs snb_langevin_initial; SNB_ERR_UNSUPPORTED when triangles share atoms.
+ * For callers that drive whole steps and do not read the float forces. */
+int snb_md_iterate_fused(snb_context* ctx, const float* d_mass, snb_vector* d_crd,
+                         snb_ltmatrix3 cell, snb_ltmatrix3 rcell, snb_vector* d_vel,
+                         int need_pressure, snb_ltmatrix3* d_stress);
+
 /* --------------------------------------------------------------- pressure */
 /* The stress of a pressure step is assembled as the reference does it: the
  * caller zeroes d_virial_tensor and d_stress at the start of the step

@@ -252,7 +252,10 @@ void free_settle(SnbSettle& S)
     dfree(S.last_ba);
     dfree(S.last_ca);
     dfree(S.vir6);
+    dfree(S.free_atoms);
     S.count = 0;
+    S.nfree = 0;
+    S.disjoint = false;
     S.init = false;
 }
 
@@ -1289,6 +1292,32 @@ int snb_settle_initial(snb_context* c, int triangle_numbers, const int* h_atom_A
     SNB_CHECK(hipMemcpy(S.tri, tri.data(), sizeof(int4) * m, hipMemcpyHostToDevice));
     SNB_CHECK(hipMemcpy(S.geo, geo.data(), sizeof(float) * 5 * m, hipMemcpyHostToDevice));
     SNB_CHECK(hipMemset(S.vir6, 0, sizeof(int64_t) * 6));
+    // atoms outside every triangle, and whether triangles share atoms
+    {
+        std::vector<int> uses(c->n > 0 ? c->n : 1, 0);
+        for (size_t t = 0; t < m; t++)
+        {
+            uses[h_atom_A[t]]++;
+            uses[h_atom_B[t]]++;
+            uses[h_atom_C[t]]++;
+        }
+        std::vector<int> fr;
+        S.disjoint = true;
+        for (int i = 0; i < c->n; i++)
+        {
+            if (uses[i] == 0) fr.push_back(i);
+            if (uses[i] > 1) S.disjoint = false;
+        }
+        S.nfree = (int)fr.size();
+        if ((rc = dalloc(&S.free_atoms, fr.size() > 0 ? fr.size() : (size_t)1)))
+        {
+            free_settle(S);
+            return rc;
+        }
+        if (!fr.empty())
+            SNB_CHECK(hipMemcpy(S.free_atoms, fr.data(), sizeof(int) * fr.size(),
+                                hipMemcpyHostToDevice));
+    }
     S.count = triangle_numbers;
     S.init = true;
     return SNB_OK;
@@ -1371,6 +1400,19 @@ int snb_get_pressure(snb_context* c, int atom_numbers, const snb_vector* d_vel,
     if (h_stress) *h_stress = s;
     if (h_pressure) *h_pressure = ((s.a11 + s.a22) + s.a33) / 3.0f;
     return SNB_OK;
+}
+
+int snb_md_iterate_fused(snb_context* c, const float* d_mass, snb_vector* d_crd,
+                         snb_ltmatrix3 cell, snb_ltmatrix3 rcell, snb_vector* d_vel,
+                         int need_pressure, snb_ltmatrix3* d_stress)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (!c->langevin.init) return SNB_ERR_NOT_INITIALIZED;
+    if (c->n > 0 && (!d_crd || !d_vel || (c->settle.init && !d_mass))) return SNB_ERR_ARGUMENT;
+    if (c->settle.init && !c->settle.disjoint) return SNB_ERR_UNSUPPORTED;
+    DeviceGuard g(c->device);
+    return snb_launch_md_fused(c, d_mass, d_crd, snb_box(cell, rcell), d_vel, need_pressure,
+                               d_stress);
 }
 
 // ---------------------------------------------------------------------- PME

@@ -157,6 +157,9 @@ struct SnbSettle
     snb_vector* last_ba = nullptr;  // previous step's B - A
     snb_vector* last_ca = nullptr;  // previous step's C - A
     int64_t* vir6 = nullptr;        // summed constraint virial, fixed point
+    int* free_atoms = nullptr;      // atoms in no triangle (the fused iteration's list)
+    int nfree = 0;
+    bool disjoint = false;          // no atom in two triangles (needed by the fused iteration)
 };
 
 struct SnbPme
@@ -332,6 +335,10 @@ int snb_launch_leap_frog(snb_context* ctx, const float* inv_mass, const snb_vect
 int snb_launch_langevin(snb_context* ctx, const snb_vector* frc, snb_vector* vel,
                         snb_vector* crd);
 int snb_launch_settle_remember(snb_context* ctx, const snb_vector* crd, const SnbBox& box);
+// Fused remember + middle Langevin + SETTLE with the forces taken from the
+// fixed-point accumulators (cleared).
+int snb_launch_md_fused(snb_context* ctx, const float* mass, snb_vector* crd, const SnbBox& box,
+                        snb_vector* vel, int need_pressure, snb_ltmatrix3* stress);
 // Pressure assembly: pot adds the atom-virial sum (into virial_tensor, when
 // given, and stress), kin the m v (x) v sum (into stress).
 int snb_launch_stress(snb_context* ctx, int n, int pot, int kin, const snb_ltmatrix3* atom_virial,

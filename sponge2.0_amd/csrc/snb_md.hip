@@ -48,16 +48,13 @@ __global__ void __launch_bounds__(256) k_leap_frog(int n, const float* __restric
 
 // Middle scheme (Liu Jian): v1 = v + dt inv_m f; x1 = x + dt/2 v1;
 // v2 = e^{-gamma dt} v1 + sqrt(kT (1 - e^{-2 gamma dt}) / m) R; x2 = x1 + dt/2 v2.
-__global__ void __launch_bounds__(256) k_langevin(int n, const float* __restrict__ inv_mass,
-                                                  const float* __restrict__ sqrt_mass,
-                                                  uint32_t* __restrict__ counter, uint32_t seed,
-                                                  const snb_vector* __restrict__ frc,
-                                                  snb_vector* __restrict__ vel,
-                                                  snb_vector* __restrict__ crd, float dt,
-                                                  float half_dt, float exp_gamma, float vmax)
+__device__ __forceinline__ void langevin_update(int i, const float* __restrict__ inv_mass,
+                                                const float* __restrict__ sqrt_mass,
+                                                uint32_t* __restrict__ counter, uint32_t seed,
+                                                snb_vector f, snb_vector& v, snb_vector& x,
+                                                float dt, float half_dt, float exp_gamma,
+                                                float vmax)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     uint32_t c[4] = {(uint32_t)i, counter[i], 0u, 0u};
     counter[i] = c[1] + 1u;
     snb_philox4x32_10(c, seed, SNB_LANGEVIN_KEY1);
@@ -65,8 +62,6 @@ __global__ void __launch_bounds__(256) k_langevin(int n, const float* __restrict
     snb_box_muller(c[0], c[1], &r0, &r1);
     snb_box_muller(c[2], c[3], &r2, &r3);
     const float im = inv_mass[i], sm = sqrt_mass[i];
-    const snb_vector f = frc[i];
-    snb_vector v = vel[i], x = crd[i];
     float ax = im * f.x, ay = im * f.y, az = im * f.z;
     float vx = v.x + dt * ax, vy = v.y + dt * ay, vz = v.z + dt * az;
     const float xx = x.x + half_dt * vx, xy = x.y + half_dt * vy, xz = x.z + half_dt * vz;
@@ -80,6 +75,21 @@ __global__ void __launch_bounds__(256) k_langevin(int n, const float* __restrict
     x.x = xx + half_dt * vx;
     x.y = xy + half_dt * vy;
     x.z = xz + half_dt * vz;
+}
+
+__global__ void __launch_bounds__(256) k_langevin(int n, const float* __restrict__ inv_mass,
+                                                  const float* __restrict__ sqrt_mass,
+                                                  uint32_t* __restrict__ counter, uint32_t seed,
+                                                  const snb_vector* __restrict__ frc,
+                                                  snb_vector* __restrict__ vel,
+                                                  snb_vector* __restrict__ crd, float dt,
+                                                  float half_dt, float exp_gamma, float vmax)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    snb_vector v = vel[i], x = crd[i];
+    langevin_update(i, inv_mass, sqrt_mass, counter, seed, frc[i], v, x, dt, half_dt, exp_gamma,
+                    vmax);
     vel[i] = v;
     crd[i] = x;
 }
@@ -148,34 +158,28 @@ __global__ void __launch_bounds__(256) k_settle_remember(int m, const int4* __re
 // frame {ra, rb, rc, rd, re} puts A at (0, ra), B at (rc, rb), C at (rd, re)
 // with the centre of mass at the origin.  The per-triangle virial (FP64, to
 // fixed point) is summed into vir6 when need_virial.
-__global__ void __launch_bounds__(256) k_settle(int m, const int4* __restrict__ tri,
-                                                const float* __restrict__ geo,
-                                                const float* __restrict__ mass,
-                                                snb_vector* __restrict__ crd, SnbBox box,
-                                                const snb_vector* __restrict__ last_ba,
-                                                const snb_vector* __restrict__ last_ca,
-                                                float dt, float v_factor, float x_factor,
-                                                snb_vector* __restrict__ vel, int need_virial,
-                                                int64_t* __restrict__ vir6)
+// The triangle update: positions pA, pB, pC (this step, unconstrained) and
+// velocities vA, vB, vC in, constrained positions and corrected velocities
+// out; w receives the fixed-point virial when need_virial.
+__device__ __forceinline__ void settle_core(const float* __restrict__ g, float mA, float mB,
+                                            float mC, snb_vector& pA, snb_vector& pB,
+                                            snb_vector& pC, snb_vector& vA, snb_vector& vB,
+                                            snb_vector& vC, snb_vector rB0, snb_vector rC0,
+                                            const SnbBox& box, float dt, float v_factor,
+                                            float x_factor, int need_virial, int64_t* w)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t w[6] = {0, 0, 0, 0, 0, 0};
-    if (t < m)
     {
-        const int4 a = tri[t];
-        const float ra = geo[5 * t], rb = geo[5 * t + 1], rc = geo[5 * t + 2];
-        const float rd = geo[5 * t + 3], re = geo[5 * t + 4];
-        snb_vector rA = crd[a.x];
-        snb_vector rB = settle_disp(crd[a.y], rA, box);
-        snb_vector rC = settle_disp(crd[a.z], rA, box);
-        const float mA = mass[a.x], mB = mass[a.y], mC = mass[a.z];
+        const float ra = g[0], rb = g[1], rc = g[2];
+        const float rd = g[3], re = g[4];
+        snb_vector rA = pA;
+        snb_vector rB = settle_disp(pB, rA, box);
+        snb_vector rC = settle_disp(pC, rA, box);
         // centre of mass rO = (mB rB + mC rC)/(mA + mB + mC) + rA
         const float inv_mt = 1.0f / ((mA + mB) + mC);
         const snb_vector rO = vadd(vscale(inv_mt, vadd(vscale(mB, rB), vscale(mC, rC))), rA);
         const snb_vector rA1 = vsub(rA, rO);
         const snb_vector rB1 = vadd(rB, rA1);
         const snb_vector rC1 = vadd(rC, rA1);
-        const snb_vector rB0 = last_ba[t], rC0 = last_ca[t];
         // step 0: frame from the previous plane and this step's A
         snb_vector ez = vcross(rB0, rC0);
         snb_vector ex = vcross(rA1, ez);
@@ -223,9 +227,9 @@ __global__ void __launch_bounds__(256) k_settle(int m, const int4* __restrict__ 
         // step 6: velocity change and virial of the constraint forces
         const float kv = (v_factor / dt) / x_factor;
         const snb_vector dA = vsub(rA3, rA1), dB = vsub(rB3, rB1), dC = vsub(rC3, rC1);
-        vel[a.x] = vadd(vel[a.x], vscale(kv, dA));
-        vel[a.y] = vadd(vel[a.y], vscale(kv, dB));
-        vel[a.z] = vadd(vel[a.z], vscale(kv, dC));
+        vA = vadd(vA, vscale(kv, dA));
+        vB = vadd(vB, vscale(kv, dB));
+        vC = vadd(vC, vscale(kv, dC));
         if (need_virial)
         {
             const float kf = ((1.0f / dt) / dt) / x_factor;
@@ -243,20 +247,114 @@ __global__ void __launch_bounds__(256) k_settle(int m, const int4* __restrict__ 
                 (double)fB.z * rB0.z + (double)fC.z * rC0.z};
             for (int k = 0; k < 6; k++) w[k] = snb_to_fixed_d(v[k]);
         }
-        crd[a.x] = vadd(rA3, rO);
-        crd[a.y] = vadd(rB3, rO);
-        crd[a.z] = vadd(rC3, rO);
+        pA = vadd(rA3, rO);
+        pB = vadd(rB3, rO);
+        pC = vadd(rC3, rO);
     }
-    if (need_virial)
+}
+
+// sums each wavefront's triangle virials into vir6 (int64, order-free)
+__device__ __forceinline__ void settle_virial_sum(const int64_t* w, int64_t* __restrict__ vir6)
+{
+    for (int k = 0; k < 6; k++)
     {
-        for (int k = 0; k < 6; k++)
-        {
-            int64_t s = w[k];
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-            if ((threadIdx.x & 63) == 0 && s != 0)
-                atomicAdd((unsigned long long*)&vir6[k], (unsigned long long)s);
-        }
+        int64_t s = w[k];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if ((threadIdx.x & 63) == 0 && s != 0)
+            atomicAdd((unsigned long long*)&vir6[k], (unsigned long long)s);
     }
+}
+
+__global__ void __launch_bounds__(256) k_settle(int m, const int4* __restrict__ tri,
+                                                const float* __restrict__ geo,
+                                                const float* __restrict__ mass,
+                                                snb_vector* __restrict__ crd, SnbBox box,
+                                                const snb_vector* __restrict__ last_ba,
+                                                const snb_vector* __restrict__ last_ca,
+                                                float dt, float v_factor, float x_factor,
+                                                snb_vector* __restrict__ vel, int need_virial,
+                                                int64_t* __restrict__ vir6)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t w[6] = {0, 0, 0, 0, 0, 0};
+    if (t < m)
+    {
+        const int4 a = tri[t];
+        snb_vector pA = crd[a.x], pB = crd[a.y], pC = crd[a.z];
+        snb_vector vA = vel[a.x], vB = vel[a.y], vC = vel[a.z];
+        settle_core(geo + 5 * t, mass[a.x], mass[a.y], mass[a.z], pA, pB, pC, vA, vB, vC,
+                    last_ba[t], last_ca[t], box, dt, v_factor, x_factor, need_virial, w);
+        vel[a.x] = vA;
+        vel[a.y] = vB;
+        vel[a.z] = vC;
+        crd[a.x] = pA;
+        crd[a.y] = pB;
+        crd[a.z] = pC;
+    }
+    if (need_virial) settle_virial_sum(w, vir6);
+}
+
+// Fused iteration: per rigid triangle, SETTLE's remember (B - A, C - A before
+// the update), the middle Langevin update of its three atoms and SETTLE; per
+// free atom (in no triangle), the Langevin update alone.  Forces come from
+// the fixed-point accumulators (direct + reciprocal, converted as a flush into
+// a zeroed force array converts them) and are cleared.  Thread t < ntri owns
+// triangle t, the next nfree threads the free atoms.
+__global__ void __launch_bounds__(256) k_md_fused(
+    int ntri, const int4* __restrict__ tri, const float* __restrict__ geo,
+    const float* __restrict__ mass, int nfree, const int* __restrict__ free_atoms, int n,
+    int64_t* __restrict__ fx, int64_t* __restrict__ fx_pme, const float* __restrict__ inv_mass,
+    const float* __restrict__ sqrt_mass, uint32_t* __restrict__ counter, uint32_t seed,
+    snb_vector* __restrict__ vel, snb_vector* __restrict__ crd, SnbBox box, float dt,
+    float half_dt, float exp_gamma, float vmax, float v_factor, float x_factor, int need_virial,
+    int64_t* __restrict__ vir6)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t w[6] = {0, 0, 0, 0, 0, 0};
+    auto force_of = [&](int i) {
+        snb_vector f;
+        f.x = 0.0f + snb_fixed_to_float(fx[i] + fx_pme[i]);
+        f.y = 0.0f + snb_fixed_to_float(fx[n + i] + fx_pme[n + i]);
+        f.z = 0.0f + snb_fixed_to_float(fx[2 * n + i] + fx_pme[2 * n + i]);
+        fx[i] = 0;
+        fx[n + i] = 0;
+        fx[2 * n + i] = 0;
+        fx_pme[i] = 0;
+        fx_pme[n + i] = 0;
+        fx_pme[2 * n + i] = 0;
+        return f;
+    };
+    if (t < ntri)
+    {
+        const int4 a = tri[t];
+        snb_vector pA = crd[a.x], pB = crd[a.y], pC = crd[a.z];
+        const snb_vector rB0 = settle_disp(pB, pA, box), rC0 = settle_disp(pC, pA, box);
+        snb_vector vA = vel[a.x], vB = vel[a.y], vC = vel[a.z];
+        langevin_update(a.x, inv_mass, sqrt_mass, counter, seed, force_of(a.x), vA, pA, dt,
+                        half_dt, exp_gamma, vmax);
+        langevin_update(a.y, inv_mass, sqrt_mass, counter, seed, force_of(a.y), vB, pB, dt,
+                        half_dt, exp_gamma, vmax);
+        langevin_update(a.z, inv_mass, sqrt_mass, counter, seed, force_of(a.z), vC, pC, dt,
+                        half_dt, exp_gamma, vmax);
+        settle_core(geo + 5 * t, mass[a.x], mass[a.y], mass[a.z], pA, pB, pC, vA, vB, vC, rB0,
+                    rC0, box, dt, v_factor, x_factor, need_virial, w);
+        vel[a.x] = vA;
+        vel[a.y] = vB;
+        vel[a.z] = vC;
+        crd[a.x] = pA;
+        crd[a.y] = pB;
+        crd[a.z] = pC;
+    }
+    else if (t - ntri < nfree)
+    {
+        const int i = free_atoms ? free_atoms[t - ntri] : t - ntri;
+        snb_vector v = vel[i], x = crd[i];
+        langevin_update(i, inv_mass, sqrt_mass, counter, seed, force_of(i), v, x, dt, half_dt,
+                        exp_gamma, vmax);
+        vel[i] = v;
+        crd[i] = x;
+    }
+    if (need_virial) settle_virial_sum(w, vir6);
 }
 
 // stress += (1/V) * summed virial (Sum_Virial_Tensor_To_Stress, settle.cpp:307-340);
@@ -471,5 +569,40 @@ int snb_launch_stress(snb_context* ctx, int n, int pot, int kin, const snb_ltmat
         hipLaunchKernelGGL((k_stress_finish<false, true>), dim3(1), dim3(64), 0, ctx->stream,
                            acc, inv_volume, virial_tensor, stress);
     SNB_CHECK(hipGetLastError());
+    return SNB_OK;
+}
+
+int snb_launch_md_fused(snb_context* ctx, const float* mass, snb_vector* crd, const SnbBox& box,
+                        snb_vector* vel, int need_pressure, snb_ltmatrix3* stress)
+{
+    const SnbLangevin& T = ctx->langevin;
+    const SnbSettle& S = ctx->settle;
+    const int n = ctx->n;
+    if (n <= 0) return SNB_OK;
+    int rc = snb_join_pme(ctx);
+    if (rc) return rc;
+    const int ntri = S.init ? S.count : 0;
+    const int nfree = S.init ? S.nfree : n;
+    const int* free_atoms = S.init ? S.free_atoms : nullptr;
+    const int want = need_pressure && stress && ntri > 0 ? 1 : 0;
+    const int total = ntri + nfree;
+    // constrain.v_factor / x_factor of the middle scheme (main.cpp:650-656)
+    const float v_factor = T.exp_gamma, x_factor = 0.5f * T.exp_gamma + 0.5f;
+    if (total > 0)
+    {
+        hipLaunchKernelGGL(k_md_fused, dim3((total + 255) / 256), dim3(256), 0, ctx->stream, ntri,
+                           S.tri, S.geo, mass, nfree, free_atoms, n, ctx->fx, ctx->fx_pme,
+                           T.inv_mass, T.sqrt_mass, T.counter, T.seed, vel, crd, box, T.dt,
+                           T.half_dt, T.exp_gamma, T.max_velocity, v_factor, x_factor, want,
+                           S.vir6);
+        SNB_CHECK(hipGetLastError());
+    }
+    if (want)
+    {
+        const float inv_volume = (box.rc[0] * box.rc[2]) * box.rc[5];
+        hipLaunchKernelGGL(k_settle_stress, dim3(1), dim3(64), 0, ctx->stream, S.vir6,
+                           inv_volume, stress);
+        SNB_CHECK(hipGetLastError());
+    }
     return SNB_OK;
 }

@@ -84,6 +84,7 @@ _SIGS = {
     "snb_settle_initial": ([_P, _I, _P, _P, _P, _P, _P, _P, _P], _I),
     "snb_settle_remember": ([_P, _P, LTMatrix3, LTMatrix3], _I),
     "snb_settle_do": ([_P, _P, _P, LTMatrix3, LTMatrix3, _P, _I, _P, _F, _F, _F], _I),
+    "snb_md_iterate_fused": ([_P, _P, _P, LTMatrix3, LTMatrix3, _P, _I, _P], _I),
     "snb_get_potential_to_stress": ([_P, _I, _P, LTMatrix3, _P, _P], _I),
     "snb_get_kinetic_to_stress": ([_P, _I, _P, _P, LTMatrix3, _P], _I),
     "snb_get_pressure": ([_P, _I, _P, _P, _P, LTMatrix3, _P, _P, ctypes.POINTER(LTMatrix3),
@@ -355,7 +356,8 @@ class Nonbonded:
         torch.cuda.synchronize()
         return graph
 
-    def step(self, step=0, need_energy=False, need_virial=False, forced_nl=False, fused=True):
+    def step(self, step=0, need_energy=False, need_virial=False, forced_nl=False, fused=True,
+             flush=True):
         """One force evaluation of the path in the reference's order
         (main.cpp:307-334, :371-396, :439-441, :455-458, :474-478).  fused=True keeps every term
         in the fixed-point accumulators and converts once at the end."""
@@ -372,7 +374,8 @@ class Nonbonded:
             self.bond_force(need_energy, need_virial, frc=not fused)
         self.reciprocal(step, need_energy, need_virial, frc=not fused)
         if fused:
-            self.flush()
+            if flush:
+                self.flush()
             if need_energy:
                 check(self.L.snb_flush_atom_energy(self.ctx, _ptr(self.atom_energy),
                                                    _ptr(self.direct_energy)),
@@ -427,21 +430,33 @@ class Nonbonded:
                                    _ptr(self.stress), self.dt_internal, self.v_factor,
                                    self.x_factor), "snb_settle_do")
 
-    def md_step(self, step=0, need_energy=False, need_virial=False, forced_nl=False):
+    def md_step(self, step=0, need_energy=False, need_virial=False, forced_nl=False,
+                fused_iterate=True):
         """One MD step on the device: zero the forces (and, on a pressure step,
         the virials and the stress: MD_Reset_Atom_Energy_And_Virial_And_Force,
         MD_core.cpp:654-676), the force path, then the iteration
         (Main_Calculate_Force + Main_Iteration); a pressure step ends with the
-        stress assembly of Get_pressure (main.cpp:713-717), left on the device."""
-        self.frc.zero_()
+        stress assembly of Get_pressure (main.cpp:713-717), left on the device.
+        fused_iterate takes the forces straight from the fixed-point
+        accumulators into one remember + Langevin + SETTLE pass
+        (snb_md_iterate_fused; bit-identical, self.frc is not written)."""
         if need_virial:
             self.atom_virial.zero_()
             self.virial.zero_()
             self.stress.zero_()
             self.virial_tensor.zero_()
-        self.step(step=step, need_energy=need_energy, need_virial=need_virial,
-                  forced_nl=forced_nl, fused=True)
-        self.md_iterate(need_pressure=need_virial)
+        if fused_iterate:
+            self.step(step=step, need_energy=need_energy, need_virial=need_virial,
+                      forced_nl=forced_nl, fused=True, flush=False)
+            check(self.L.snb_md_iterate_fused(self.ctx, _ptr(self.mass), _ptr(self.crd),
+                                              self.cell, self.rcell, _ptr(self.vel),
+                                              1 if need_virial else 0, _ptr(self.stress)),
+                  "snb_md_iterate_fused")
+        else:
+            self.frc.zero_()
+            self.step(step=step, need_energy=need_energy, need_virial=need_virial,
+                      forced_nl=forced_nl, fused=True)
+            self.md_iterate(need_pressure=need_virial)
         if need_virial:
             self.get_pressure(copy=False)
 

@@ -75,6 +75,7 @@ class Segment:
         nb = Nonbonded(s, cutoff=s["cutoff"], refresh_interval=chunk, stream=self.stream,
                        max_atom_in_grid_numbers=MAX_ATOMS_IN_CELL)
         nb.set_pme_overlap(overlap)
+        nb.set_export(False)       # nothing reads the reference-layout list here
         self.nb = nb
         self.shape = autotune.tune_window(nb, interval=chunk)[0] if tune else None
         with torch.cuda.stream(self.stream):
@@ -133,6 +134,7 @@ def pretune(requests, chunk=10):
         s = r.system()
         nb = Nonbonded(s, cutoff=s["cutoff"], refresh_interval=chunk,
                        max_atom_in_grid_numbers=MAX_ATOMS_IN_CELL)
+        nb.set_export(False)
         key = autotune.grid_key(nb)
         if key not in seen:
             seen[key] = autotune.tune_window(nb, interval=chunk)

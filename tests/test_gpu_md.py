@@ -307,3 +307,28 @@ def test_nve_conserves_energy_and_final_frame_matches_fp64(water_9k):
     assert abs(e["LJ_short"] - ref["LJ_short"]) <= 1e-6 * abs(ref["LJ_short"])
     assert abs(e["PM"] - ref["PM"]) <= 1e-6 * abs(ref["PM"])
     nb.close()
+
+
+def test_fused_iteration_bit_identical_to_separate_calls(mixed_small):
+    """snb_md_iterate_fused (forces straight from the fixed-point accumulators,
+    one remember + Langevin + SETTLE pass over triangles and free atoms)
+    against the flush + snb_settle_remember + snb_langevin_iterate +
+    snb_settle_do sequence: 25 steps of the mixed system (waters and bead
+    chains) with a pressure step, same seed; coordinates, velocities and the
+    stress agree bit for bit."""
+    s = mixed_small
+    res = []
+    for fused in (False, True):
+        nb = _nb(s, refresh_interval=10)
+        nb.md_initial(s, temperature=300.0, tau=0.1, seed=21, dt=0.002)
+        for k in range(25):
+            nb.md_step(step=k, forced_nl=(k == 0), need_virial=(k == 12), fused_iterate=fused)
+        nb.sync()
+        res.append((nb.crd.cpu().numpy().copy(), nb.vel.cpu().numpy().copy(),
+                    nb.stress.cpu().numpy().copy(), nb.status()))
+        nb.close()
+    print("stress", res[0][2], res[1][2], flush=True)
+    assert res[0][3] == 0 and res[1][3] == 0
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    np.testing.assert_array_equal(res[0][1], res[1][1])
+    np.testing.assert_array_equal(res[0][2], res[1][2])
