@@ -218,3 +218,30 @@ class Server:
                     p99_latency_s=float(np.percentile(lat, 99)),
                     atoms_total=int(sum(r.n_atoms for r in requests)),
                     slots=self.slots)
+
+
+def main(argv=None):
+    """Serves a seeded request stream and prints the summary as one JSON line:
+    python -m sponge_amd.serving --requests 16 --steps 200 --slots 4"""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--rate", type=float, default=40.0)
+    ap.add_argument("--overlap", action="store_true",
+                    help="reciprocal pipeline of each segment on its own stream")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--seed", type=int, default=20260217)
+    a = ap.parse_args(argv)
+    reqs = make_requests(count=a.requests, steps=a.steps, rate=a.rate, seed=a.seed)
+    if not a.no_tune:
+        pretune(reqs)
+    out = Server(slots=a.slots, tune=not a.no_tune, overlap=a.overlap).run(reqs)
+    out["status_flags"] = int(sum(r.result["status"] for r in reqs))
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
