@@ -382,37 +382,58 @@ __device__ __forceinline__ int win_slot_to_sorted(const SnbWin& W, const int* fi
     return first[lo] + (slot - pref[lo]);
 }
 
-// Reference-layout rows: each internal pair (s, s_j) goes to row min(id) as
-// max(id) (atomic append), rows are sorted afterwards.
-__global__ void __launch_bounds__(64) k_nl_export(const int* __restrict__ need, int n, SnbWin W,
-                                                  const int* __restrict__ start,
-                                                  const int* __restrict__ sorted_cell,
-                                                  const int* __restrict__ order,
-                                                  const int* __restrict__ row_count,
-                                                  const int* __restrict__ row_split,
-                                                  const uint16_t* __restrict__ nl16, int max_nb,
-                                                  int* __restrict__ pub_count,
-                                                  int* __restrict__ serial)
+// Reference-layout rows, one workgroup per window block: the block's window
+// table and, when the window fits, a slot -> sorted-index map are built once
+// in LDS; then every internal row of the block's cells sends each pair to row
+// min(id) as max(id) (atomic append; rows are sorted afterwards).
+#define SNB_EXPORT_MAP_CAP 8192
+__global__ void __launch_bounds__(256) k_nl_export_block(
+    const int* __restrict__ need, SnbWin W, const int* __restrict__ start,
+    const int* __restrict__ order, const int* __restrict__ row_count,
+    const int* __restrict__ row_split, const uint16_t* __restrict__ nl16, int max_nb,
+    int* __restrict__ pub_count, int* __restrict__ serial)
 {
     if (need[0] == 0) return;
     __shared__ int first[SNB_WIN_MAX_RANGES];
     __shared__ int pref[SNB_WIN_MAX_RANGES + 1];
-    int s = blockIdx.x, lane = threadIdx.x;
-    int cx, cy, cz, x0, y0, z0;
-    win_block_of_cell(W, sorted_cell[s], cx, cy, cz, x0, y0, z0);
-    win_table_wave(W, start, x0, y0, z0, lane, first, pref);
+    __shared__ int smap[SNB_EXPORT_MAP_CAP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int x0, y0, z0;
+    snb_win_block(W, blockIdx.x, x0, y0, z0);
+    if (wave == 0) win_table_wave(W, start, x0, y0, z0, lane, first, pref);
     __syncthreads();
-    int id_i = order[s];
-    int cnt = row_count[s], split = row_split[s];
-    for (int k = lane; k < cnt; k += 64)
-    {
-        int sj = win_slot_to_sorted(W, first, pref,
-                                    nl16[(size_t)s * max_nb + snb_row_entry(k, split, max_nb)]);
-        int id_j = order[sj];
-        int lo = id_i < id_j ? id_i : id_j, hi = id_i < id_j ? id_j : id_i;
-        int pos = atomicAdd(&pub_count[lo], 1);
-        if (pos < max_nb) serial[(size_t)lo * max_nb + pos] = hi;
-    }
+    const int total = pref[SNB_WIN_MAX_RANGES];
+    const bool use_map = total <= SNB_EXPORT_MAP_CAP;
+    if (use_map)
+        for (int slot = tid; slot < total; slot += blockDim.x)
+            smap[slot] = win_slot_to_sorted(W, first, pref, slot);
+    __syncthreads();
+    // the block's own cells (the last block of an axis may be partial)
+    const int xa = W.nx - x0 < W.a ? W.nx - x0 : W.a;
+    const int yb = W.ny - y0 < W.b ? W.ny - y0 : W.b;
+    const int zs = W.nz - z0 < W.s ? W.nz - z0 : W.s;
+    for (int u = 0; u < xa; u++)
+        for (int v = 0; v < yb; v++)
+        {
+            // the cells (x0 + u, y0 + v, z0 .. z0 + zs - 1) are consecutive
+            const int c0 = ((x0 + u) * W.ny + (y0 + v)) * W.nz + z0;
+            const int sb = start[c0], se = start[c0 + zs];
+            for (int srow = sb + wave; srow < se; srow += (int)(blockDim.x >> 6))
+            {
+                const int id_i = order[srow];
+                const int cnt = row_count[srow], split = row_split[srow];
+                const uint16_t* row = nl16 + (size_t)srow * max_nb;
+                for (int k = lane; k < cnt; k += 64)
+                {
+                    const int slot = row[snb_row_entry(k, split, max_nb)];
+                    const int sj = use_map ? smap[slot] : win_slot_to_sorted(W, first, pref, slot);
+                    const int id_j = order[sj];
+                    const int lo = id_i < id_j ? id_i : id_j, hi = id_i < id_j ? id_j : id_i;
+                    const int pos = atomicAdd(&pub_count[lo], 1);
+                    if (pos < max_nb) serial[(size_t)lo * max_nb + pos] = hi;
+                }
+            }
+        }
 }
 
 // Sorts each public row ascending (bitonic, one wavefront per row) and sets
@@ -647,10 +668,9 @@ int snb_launch_nl_window(snb_context* ctx, const SnbBox& box, const int* ex_star
     {
         hipLaunchKernelGGL(k_fill_need, dim3((n + 255) / 256), dim3(256), 0, s, L.need,
                            L.pub_count, n);
-        hipLaunchKernelGGL(k_nl_export, dim3(n), dim3(64), 0, s, L.need, n, W, L.cell_start,
-                           L.sorted_cell, L.order, L.row_count, L.row_split, L.nl16, L.max_nb,
-                           L.pub_count,
-                           L.serial);
+        hipLaunchKernelGGL(k_nl_export_block, dim3(W.nblock), dim3(256), 0, s, L.need, W,
+                           L.cell_start, L.order, L.row_count, L.row_split, L.nl16, L.max_nb,
+                           L.pub_count, L.serial);
         hipLaunchKernelGGL(k_nl_pub_finish, dim3(n), dim3(64), 0, s, L.need, n, L.pub_count,
                            L.max_nb, L.serial, L.dnl, ctx->status);
         SNB_CHECK(hipGetLastError());
