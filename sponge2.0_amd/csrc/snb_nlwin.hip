@@ -385,7 +385,9 @@ __device__ __forceinline__ int win_slot_to_sorted(const SnbWin& W, const int* fi
 // Reference-layout rows, one workgroup per window block: the block's window
 // table and, when the window fits, a slot -> sorted-index map are built once
 // in LDS; then every internal row of the block's cells sends each pair to row
-// min(id) as max(id) (atomic append; rows are sorted afterwards).
+// min(id) as max(id).  Pairs whose lower id is the row's own atom are
+// appended by the wavefront with one counter add per 64 entries; the others
+// take one atomic append each.  Rows are sorted afterwards.
 #define SNB_EXPORT_MAP_CAP 8192
 __global__ void __launch_bounds__(256) k_nl_export_block(
     const int* __restrict__ need, SnbWin W, const int* __restrict__ start,
@@ -418,19 +420,38 @@ __global__ void __launch_bounds__(256) k_nl_export_block(
             // the cells (x0 + u, y0 + v, z0 .. z0 + zs - 1) are consecutive
             const int c0 = ((x0 + u) * W.ny + (y0 + v)) * W.nz + z0;
             const int sb = start[c0], se = start[c0 + zs];
+            const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
             for (int srow = sb + wave; srow < se; srow += (int)(blockDim.x >> 6))
             {
                 const int id_i = order[srow];
                 const int cnt = row_count[srow], split = row_split[srow];
                 const uint16_t* row = nl16 + (size_t)srow * max_nb;
-                for (int k = lane; k < cnt; k += 64)
+                for (int k0 = 0; k0 < cnt; k0 += 64)
                 {
-                    const int slot = row[snb_row_entry(k, split, max_nb)];
-                    const int sj = use_map ? smap[slot] : win_slot_to_sorted(W, first, pref, slot);
-                    const int id_j = order[sj];
-                    const int lo = id_i < id_j ? id_i : id_j, hi = id_i < id_j ? id_j : id_i;
-                    const int pos = atomicAdd(&pub_count[lo], 1);
-                    if (pos < max_nb) serial[(size_t)lo * max_nb + pos] = hi;
+                    const int k = k0 + lane;
+                    int id_j = -1;
+                    if (k < cnt)
+                    {
+                        const int slot = row[snb_row_entry(k, split, max_nb)];
+                        const int sj =
+                            use_map ? smap[slot] : win_slot_to_sorted(W, first, pref, slot);
+                        id_j = order[sj];
+                    }
+                    const bool own = id_j > id_i;
+                    const uint64_t m_own = __ballot(own);
+                    int base = 0;
+                    if (lane == 0 && m_own) base = atomicAdd(&pub_count[id_i], (int)__popcll(m_own));
+                    base = __shfl(base, 0);
+                    if (own)
+                    {
+                        const int pos = base + (int)__popcll(m_own & lower);
+                        if (pos < max_nb) serial[(size_t)id_i * max_nb + pos] = id_j;
+                    }
+                    else if (id_j >= 0)
+                    {
+                        const int pos = atomicAdd(&pub_count[id_j], 1);
+                        if (pos < max_nb) serial[(size_t)id_j * max_nb + pos] = id_i;
+                    }
                 }
             }
         }
@@ -483,6 +504,64 @@ __global__ void __launch_bounds__(64) k_nl_pub_finish(const int* __restrict__ ne
                 }
             }
             __syncthreads();
+        }
+    for (int t = lane; t < c; t += 64) row[t] = v[t];
+}
+
+// The same sort with one wavefront per row and four rows per workgroup, each
+// wavefront in its own LDS slab (rows up to SNB_SORT_WAVE_CAP entries); the
+// compare-exchange stages need no workgroup barrier because a wavefront
+// executes its LDS operations in order.
+#define SNB_SORT_WAVE_CAP 2048
+__global__ void __launch_bounds__(256) k_nl_pub_finish_wave(const int* __restrict__ need, int n,
+                                                           const int* __restrict__ pub_count,
+                                                           int max_nb, int* __restrict__ serial,
+                                                           snb_atom_group* __restrict__ nl,
+                                                           int* __restrict__ status)
+{
+    if (need[0] == 0) return;
+    __shared__ int slab[4][SNB_SORT_WAVE_CAP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + wave;
+    if (i >= n) return;
+    int c = pub_count[i];
+    if (c > max_nb)
+    {
+        if (lane == 0) atomicOr(status, SNB_FLAG_NL_OVERFLOW);
+        c = max_nb;
+    }
+    if (lane == 0)
+    {
+        nl[i].atom_numbers = c;
+        nl[i].ghost_numbers = 0;
+    }
+    if (c <= 1) return;
+    int* v = slab[wave];
+    int P = 1;
+    while (P < c) P <<= 1;
+    int* row = serial + (size_t)i * max_nb;
+    for (int t = lane; t < P; t += 64) v[t] = t < c ? row[t] : 0x7fffffff;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1)
+        {
+            for (int t = lane; t < P; t += 64)
+            {
+                const int p = t ^ j;
+                if (p > t)
+                {
+                    const int a = v[t], b = v[p];
+                    const bool up = (t & k) == 0;
+                    if ((a > b) == up)
+                    {
+                        v[t] = b;
+                        v[p] = a;
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
     for (int t = lane; t < c; t += 64) row[t] = v[t];
 }
@@ -671,8 +750,12 @@ int snb_launch_nl_window(snb_context* ctx, const SnbBox& box, const int* ex_star
         hipLaunchKernelGGL(k_nl_export_block, dim3(W.nblock), dim3(256), 0, s, L.need, W,
                            L.cell_start, L.order, L.row_count, L.row_split, L.nl16, L.max_nb,
                            L.pub_count, L.serial);
-        hipLaunchKernelGGL(k_nl_pub_finish, dim3(n), dim3(64), 0, s, L.need, n, L.pub_count,
-                           L.max_nb, L.serial, L.dnl, ctx->status);
+        if (L.max_nb <= SNB_SORT_WAVE_CAP)
+            hipLaunchKernelGGL(k_nl_pub_finish_wave, dim3((n + 3) / 4), dim3(256), 0, s, L.need,
+                               n, L.pub_count, L.max_nb, L.serial, L.dnl, ctx->status);
+        else
+            hipLaunchKernelGGL(k_nl_pub_finish, dim3(n), dim3(64), 0, s, L.need, n, L.pub_count,
+                               L.max_nb, L.serial, L.dnl, ctx->status);
         SNB_CHECK(hipGetLastError());
     }
     return SNB_OK;
