@@ -35,3 +35,17 @@ def test_segment_system_matches_request():
     assert s["cutoff"] == 9.0
     assert r.system() is s
     assert serving.estimate_footprint(50000) > serving.estimate_footprint(1500) > 0
+
+
+def test_grid_key_matches_cell_rule():
+    """GRIDS::Initial (neighbor_list.cpp:58-60): floor(L / ((rc + skin) / 2)) cells per axis,
+    in float; a 27.35 A box with rc 9, skin 2 has 4 cells (27.35 / 5.5 = 4.97)."""
+    from sponge_amd import autotune
+
+    class Ctx:
+        cutoff = 9.0
+        box = np.array([27.35, 33.0, 55.0], np.float32)
+        n = 2000
+
+    assert autotune.grid_key(Ctx()) == (4, 6, 10)
+    assert autotune.grid_key(Ctx(), occupancy=True) == (4, 6, 10, 8)
