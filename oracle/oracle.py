@@ -66,6 +66,7 @@ def _declare(L):
     L.orc_bond_f64.argtypes = [_i, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p]
     L.orc_bond_twin.argtypes = L.orc_bond_f64.argtypes
     L.orc_stress_twin.argtypes = [_i, _p, _p, _p, ctypes.c_float, _p, _p, _p]
+    L.orc_angle_f64.argtypes = [_i, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p]
     L.orc_fft3d_f64.argtypes = [_p, _i, _i, _i, _i]
     L.orc_dft3d_naive_f64.argtypes = [_p, _p, _i, _i, _i, _i]
     L.orc_pme_f64.argtypes = [_i, _p, _p, _p, _i, _i, _i, _f, _d, _p, _p, _p, _p, _p]
@@ -296,6 +297,21 @@ def bond_twin(crd, cell, rcell, bonds, virial=True):
                         _ptr(a), _ptr(b), _ptr(k), _ptr(r0), _ptr(frc), _ptr(ep), _ptr(ea),
                         _ptr(vir))
     return frc, ep[:m], ea, vir
+
+
+def angle_f64(crd, cell, rcell, angles):
+    """Harmonic angles in FP64 (angle.cpp:6-83); angles = (a, b, c, k, theta0)
+    with b the vertex.  Returns (frc[n,3], e_pair[m])."""
+    crd = _c(crd, np.float32)
+    n = crd.shape[0]
+    a, b, c = (_c(angles[i], np.int32) for i in range(3))
+    k, t0 = _c(angles[3], np.float32), _c(angles[4], np.float32)
+    m = a.shape[0]
+    frc = np.zeros((n, 3), np.float64)
+    ep = np.zeros(max(m, 1), np.float64)
+    lib().orc_angle_f64(n, _ptr(crd), _ptr(_c(cell, np.float32)), _ptr(_c(rcell, np.float32)), m,
+                        _ptr(a), _ptr(b), _ptr(c), _ptr(k), _ptr(t0), _ptr(frc), _ptr(ep))
+    return frc, ep[:m]
 
 
 def stress_twin(inv_v, vt, stress, atom_virial=None, vel=None, mass=None):

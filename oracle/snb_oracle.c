@@ -39,6 +39,9 @@
  *                           closed-form off-diagonal layout; potential part = numpy sum
  *   orc_pme_f64 (virial)    central differences of E_rec under box scaling: each diagonal
  *                           entry is +1/2 dE_rec/dlambda (the reference's sign and factor)
+ *   orc_angle_f64           GROMACS fixture angle + Urey-Bradley term within the reference's
+ *                           0.5 kcal/mol; with the bonds and the path, the fixture's per-atom
+ *                           GROMACS forces on the water atoms; central-difference gradient
  *   orc_philox4x32_10       Random123 known-answer vectors
  *   orc_langevin_*          moments and a KS test of the normals; free-particle temperature
  *   orc_settle_*            closed-form TIP3P frame; restored geometry, centre of mass and
@@ -793,6 +796,48 @@ ORC_EXPORT void orc_bond_twin(int n, const float* crd, const float* cell, const 
             vir[4 * (size_t)n + a] += to_fixed(f[1] * d[2] + f[2] * d[1]);
             vir[5 * (size_t)n + a] += to_fixed(f[2] * d[2]);
         }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Harmonic angles (angle/angle.cpp:6-83), the angle half of UREY_BRADLEY     */
+/* (angle/Urey_Bradley_force.cpp:101-115; its bond half is orc_bond_f64)     */
+/* ------------------------------------------------------------------------ */
+/* FP64 only: a test aid that lets the checker assemble the total force of a
+ * fixture whose reference forces include angle terms.  With
+ * dij = pbc(r_a - r_b), dkj = pbc(r_c - r_b), cos t = dij.dkj / (|dij||dkj|)
+ * clamped to +-0.999999, E = k (t - t0)^2; g = -2 k (t - t0) / sin t;
+ * f_a = g cos t / |dij|^2 dij - g / (|dij||dkj|) dkj, f_c likewise,
+ * f_b = -f_a - f_c; energy to atom a. */
+ORC_EXPORT void orc_angle_f64(int n, const float* crd, const float* cell, const float* rcell,
+                              int m, const int* atom_a, const int* atom_b, const int* atom_c,
+                              const float* k, const float* theta0, double* frc, double* e_pair)
+{
+    for (int q = 0; q < m; q++)
+    {
+        int a = atom_a[q], b = atom_b[q], c = atom_c[q];
+        double dij[3], dkj[3];
+        pbc_disp_d(crd + 3 * a, crd + 3 * b, cell, rcell, dij);
+        pbc_disp_d(crd + 3 * c, crd + 3 * b, cell, rcell, dkj);
+        double rij2 = 1.0 / (dij[0] * dij[0] + dij[1] * dij[1] + dij[2] * dij[2]);
+        double rkj2 = 1.0 / (dkj[0] * dkj[0] + dkj[1] * dkj[1] + dkj[2] * dkj[2]);
+        double rr = sqrt(rij2 * rkj2);
+        double ct = (dij[0] * dkj[0] + dij[1] * dkj[1] + dij[2] * dkj[2]) * rr;
+        if (ct > 0.999999) ct = 0.999999;
+        if (ct < -0.999999) ct = -0.999999;
+        double th = acos(ct);
+        double dth = th - (double)theta0[q];
+        double g = -2.0 * (double)k[q] * dth / sin(th);
+        double cross = g * rr, self = g * ct;
+        for (int d = 0; d < 3; d++)
+        {
+            double fa = self * rij2 * dij[d] - cross * dkj[d];
+            double fc = self * rkj2 * dkj[d] - cross * dij[d];
+            frc[3 * a + d] += fa;
+            frc[3 * c + d] += fc;
+            frc[3 * b + d] -= fa + fc;
+        }
+        if (e_pair) e_pair[q] = (double)k[q] * dth * dth;
     }
 }
 

@@ -130,6 +130,22 @@ def read_bond(path):
             rows[:, 2].astype(np.float32), rows[:, 3].astype(np.float32))
 
 
+def read_urey_bradley(path):
+    """Urey-Bradley terms as Xponge::Native_Load_Urey_Bradley reads them
+    (xponge/load/native/urey_bradley.hpp:8-52): a count, then
+    "a b c angle_k angle_theta0 bond_k bond_r0" rows; the module is an angle
+    a-b-c plus a bond a-c (Urey_Bradley_force.cpp:56-72).  Returns
+    ((a, b, c, angle_k, theta0), (a, c, bond_k, bond_r0))."""
+    t = _tokens(path)
+    m = int(t[0])
+    if len(t) < 1 + 7 * m:
+        raise ValueError("%s: the format of urey_bradley_in_file is not right" % path)
+    rows = np.array(t[1:1 + 7 * m], dtype=object).reshape(m, 7)
+    a, b, c = (rows[:, i].astype(np.int32) for i in range(3))
+    ak, at0, bk, br0 = (rows[:, i].astype(np.float32) for i in range(3, 7))
+    return (a, b, c, ak, at0), (a, c, bk, br0)
+
+
 def write_bond(path, atom_a, atom_b, k, r0):
     with open(path, "w") as f:
         f.write("%d\n" % len(atom_a))
@@ -182,6 +198,8 @@ def system_from_prefix(directory, prefix):
                               types, a, b)
     if os.path.exists(p("bond")):
         d["bonds"] = read_bond(p("bond"))
+    if os.path.exists(p("urey_bradley")):
+        d["urey_bradley"] = read_urey_bradley(p("urey_bradley"))
     return d
 
 

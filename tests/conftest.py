@@ -71,3 +71,26 @@ def mixed_small():
     from sponge_amd import systems
     return _cached("mix13k", lambda: systems.mixed_system(
         n_waters=4000, n_chains=2, beads_per_chain=300, box=52.0, seed=11))
+
+
+def gromacs_water_atoms(sysd):
+    """Atoms of the fixture's TIP3P waters (O at -0.834 e followed by two H at
+    +0.417 e, charges in the files' units of 18.2223): their reference forces
+    hold only terms this path and the oracle's bond / angle aids compute."""
+    q = np.asarray(sysd["charge"], np.float64) / 18.2223
+    n = len(q)
+    mask = np.zeros(n, bool)
+    o = np.nonzero(np.abs(q + 0.834) < 1e-4)[0]
+    for i in o:
+        if i + 2 < n and abs(q[i + 1] - 0.417) < 1e-4 and abs(q[i + 2] - 0.417) < 1e-4:
+            mask[i:i + 3] = True
+    return mask
+
+
+def force_stats(ref, got):
+    """benchmarks/comparison/utils.py:122-152: max |dF|, RMS dF, mean cosine."""
+    ref = np.asarray(ref, np.float64)
+    got = np.asarray(got, np.float64)
+    d = got - ref
+    cos = np.mean(np.sum(ref * got, 1) / (np.linalg.norm(ref, axis=1) * np.linalg.norm(got, axis=1)))
+    return float(np.abs(d).max()), float(np.sqrt(np.mean(d * d))), float(cos)

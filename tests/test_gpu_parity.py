@@ -715,3 +715,28 @@ def test_box_update_matches_fresh_context(water_9k):
     assert nb.status() == 0
     nb.close()
     nb2.close()
+
+
+@pytest.mark.parametrize("it", [0, 1, 2])
+def test_gpu_forces_on_waters_vs_gromacs(it):
+    """The library's step forces (path + bonds + 1-4 pairs) plus the oracle's
+    angle / Urey-Bradley aid against GROMACS 2025.4's per-atom forces on the
+    fixture's water atoms, within the reference's tolerances (max |dF| <= 0.07,
+    RMS <= 0.01, cosine >= 0.999; comp_alanine_dipeptide.py:113-115)."""
+    import oracle
+    from conftest import force_stats, gromacs_water_atoms
+    s = gromacs_case(it)
+    nb = _nb(s)
+    nb.zero_outputs()
+    nb.step(forced_nl=True)
+    nb.sync()
+    f = nb.frc.cpu().numpy().astype(np.float64)
+    nb.close()
+    fa, _ = oracle.angle_f64(s["crd"], s["cell"], s["rcell"], s["urey_bradley"][0])
+    fu, _, _, _ = oracle.bond_f64(s["crd"], s["cell"], s["rcell"], s["urey_bradley"][1])
+    gmx = np.load(os.path.join(GOLDEN, "gromacs_charmm_tip3p", "iter%d" % it, "gromacs_forces.npy"),
+                  allow_pickle=False)
+    w = gromacs_water_atoms(s)
+    mx, rms, cos = force_stats(gmx[w], (f + fa + fu)[w])
+    print("iteration", it, "water atoms: max", mx, "rms", rms, "cos", cos, flush=True)
+    assert mx <= 0.07 and rms <= 0.01 and cos >= 0.999

@@ -616,3 +616,63 @@ def test_pme_reciprocal_virial_is_half_the_strain_derivative(gromacs_iter0):
         lm[axis] -= h
         d = (erec(lp)[0] - erec(lm)[0]) / (2 * h)
         assert abs(vir[entry] - 0.5 * d) < 1e-3 * abs(vir[entry]), (axis, vir[entry], d)
+
+
+# ------------------------------------------- per-atom forces against GROMACS
+@pytest.mark.parametrize("it", [0, 1, 2])
+def test_gromacs_fixture_forces_on_waters(it):
+    """GROMACS 2025.4 per-atom forces of the fixture (float64, kcal/mol/A,
+    benchmarks/comparison/reference/gromacs/forces/...) on the 2,106 water
+    atoms, whose forces hold the nonbonded path, the O-H bonds and the H-O-H
+    angles: the oracle's path + bonds + angle/Urey-Bradley terms agree within
+    the reference's own tolerances (max |dF| <= 0.07, RMS <= 0.01, cosine >=
+    0.999; comp_alanine_dipeptide.py:113-115).  The fixture's angle +
+    Urey-Bradley energy matches GROMACS's angle + urey_bradley within the
+    reference's 0.5 kcal/mol (:108)."""
+    from conftest import force_stats, gromacs_water_atoms
+    s = gromacs_case(it)
+    ref = json.load(open(os.path.join(GOLDEN, "gromacs_charmm_tip3p", "reference_terms.json")))
+    entry = [x for x in ref["entries"] if x["iteration"] == it][0]
+    r = oracle.nonbonded_terms_f64(s, cutoff=12.0)
+    fa, ea = oracle.angle_f64(s["crd"], s["cell"], s["rcell"], s["urey_bradley"][0])
+    fu, eu, _, _ = oracle.bond_f64(s["crd"], s["cell"], s["rcell"], s["urey_bradley"][1])
+    assert abs(ea.sum() + eu.sum() - (entry["angle"] + entry["urey_bradley"])) <= 0.5
+    gmx = np.load(os.path.join(GOLDEN, "gromacs_charmm_tip3p", "iter%d" % it, "gromacs_forces.npy"),
+                  allow_pickle=False)
+    w = gromacs_water_atoms(s)
+    assert w.sum() == 3 * 702
+    mx, rms, cos = force_stats(gmx[w], (r["frc"] + fa + fu)[w])
+    print("water atoms: max", mx, "rms", rms, "cos", cos)
+    assert mx <= 0.07 and rms <= 0.01 and cos >= 0.999
+
+
+def test_angle_force_is_minus_energy_gradient():
+    """Central differences of the angle energy against the forces on all three
+    atoms, for a bent and a nearly linear angle."""
+    rng = np.random.default_rng(5)
+    L = 30.0
+    cell = np.array([L, 0, L, 0, 0, L], np.float32)
+    rcell = np.array([1 / L, 0, 1 / L, 0, 0, 1 / L], np.float32)
+    for th in (1.2, 2.9):
+        b = np.array([15.0, 15.0, 15.0])
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        v = rng.normal(size=3)
+        v -= v.dot(u) * u
+        v /= np.linalg.norm(v)
+        a = b + 1.3 * u
+        c = b + 1.1 * (np.cos(th) * u + np.sin(th) * v)
+        base = np.array([a, b, c], np.float64)
+        ang = (np.array([0]), np.array([1]), np.array([2]), np.array([60.0], np.float32),
+               np.array([1.9], np.float32))
+        f, e = oracle.angle_f64(base.astype(np.float32), cell, rcell, ang)
+        h = 1e-3
+        for atom in range(3):
+            for d in range(3):
+                p, m = base.copy(), base.copy()
+                p[atom, d] += h
+                m[atom, d] -= h
+                g = (oracle.angle_f64(p.astype(np.float32), cell, rcell, ang)[1][0] -
+                     oracle.angle_f64(m.astype(np.float32), cell, rcell, ang)[1][0]) / (2 * h)
+                assert abs(f[atom, d] + g) < 2e-2 * max(1.0, np.abs(f).max()), (th, atom, d)
+        assert np.abs(f.sum(0)).max() < 1e-9 * max(1.0, np.abs(f).max())
