@@ -272,6 +272,12 @@ def config_gpu_93k(seed=20260217):
     return d
 
 
+def config_water_1m(seed=20260217):
+    """333,334 TIP3P waters (1,000,002 atoms) at 1.0 g/cm^3 (215.24 A box),
+    cutoff 10 A, PME grid 216^3 (automatic)."""
+    return tip3p_box(333334, 215.24, seed)
+
+
 KB = 0.00198716  # kcal/mol/K (common.h:122)
 TIME_UNIT = 20.455  # internal time unit is 1/20.455 ps (common.h:136)
 
