@@ -20,7 +20,7 @@
  * Reference citations are paths inside yuhaosimba/SPONGE2.0 (file:line).
  *
  * Parity pins (what checks each function; see tests/test_oracle_*.py):
- *   orc_get_beta            survey-measured reference values (rc 8, 12) + erfc(beta rc)/rc = tol
+ *   orc_get_beta            beta the reference computes at rc 8 and 12 + erfc(beta rc)/rc = tol
  *   orc_fft_size            reference grid sizes for the fixture boxes + 2/3/5/7-smoothness
  *   orc_bspline_moduli_f64  closed form |sum M4(k+1) e^{2 pi i m k/K}|^-2 at m=0, K/2
  *   orc_nl_build            brute-force set definition itself; invariants (no dup, j>i, d<rc+skin)
