@@ -337,8 +337,8 @@ int prof_collect(snb_context* c, int g)
 }
 }  // namespace
 
-// Puts a deferred force-only exclusion correction onto pme_stream (after
-// the step's inputs are final), accumulating into fx_pme.
+// Makes the context stream wait for the reciprocal pipeline queued on
+// pme_stream (ev_pme), if one is pending.
 int snb_join_pme(snb_context* c)
 {
     if (!c->pme_pending) return SNB_OK;
