@@ -169,6 +169,14 @@ int snb_nl_set_window_capacity(snb_context* ctx, int slots);
  * rebuilds.  Forces, energies and virials are the same for every shape (no
  * counterpart in the reference). */
 int snb_nl_set_window_shape(snb_context* ctx, int a, int b, int s);
+/* How the windowed pair kernel hands its window forces to the accumulators
+ * on force-only steps: 0 = through a per-block stage and a per-atom gather
+ * kernel, 1 = atomic integer adds at the end of each block, -1 (default) =
+ * by size (1 once the stage would exceed 128 MiB, about 310k atoms).  Energy
+ * and virial steps always use the stage.  Results are the same either way
+ * (integer sums); SNB_ERR_ARGUMENT for another mode (no counterpart in the
+ * reference). */
+int snb_lj_set_direct_j(snb_context* ctx, int mode);
 /* Geometry of the windowed pair path (windowed = 0 when not used). */
 int snb_nl_window_info(snb_context* ctx, int* windowed, int* a, int* b, int* s, int* wcap,
                        int* nblock);

@@ -812,6 +812,14 @@ int snb_nl_set_window_shape(snb_context* c, int a, int b, int s)
     return setup_window(c);
 }
 
+int snb_lj_set_direct_j(snb_context* c, int mode)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (mode < -1 || mode > 1) return SNB_ERR_ARGUMENT;
+    c->lj.j_direct = mode;
+    return SNB_OK;
+}
+
 int snb_nl_window_info(snb_context* c, int* windowed, int* a, int* b, int* s, int* wcap,
                        int* nblock)
 {

@@ -105,6 +105,9 @@ struct SnbLj
     // per-atom energy/virial of a pair: false = the lower atom id (the
     // reference's row atom), true = the windowed list's row owner (cheaper)
     bool owner_attribution = false;
+    // window forces of force-only steps: -1 by size, 0 through the stage and
+    // its per-atom gather, 1 by atomic adds at the end of each block
+    int j_direct = -1;
 };
 
 // 1-4 pairs (NON_BOND_14): per pair the two atoms, A (x12 form), B (x6 form)

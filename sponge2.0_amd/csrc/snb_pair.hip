@@ -656,7 +656,7 @@ __global__ void __launch_bounds__(1024) k_pair_win(
     const int* __restrict__ stage_base, long long* __restrict__ stage, size_t stage_cap,
     int64_t* __restrict__ fx, int64_t* __restrict__ e_sorted, int64_t* __restrict__ v_sorted,
     int ref_attr, int64_t* __restrict__ e_atom, int64_t* __restrict__ vir,
-    const int* __restrict__ perm)
+    const int* __restrict__ perm, int jdirect)
 {
     extern __shared__ long long smem[];
     float4* ctab = (float4*)smem;                   // [SNB_COUL_N]
@@ -843,6 +843,23 @@ __global__ void __launch_bounds__(1024) k_pair_win(
         }
     }
     __syncthreads();
+    if (!NEED_ENERGY && !NEED_VIRIAL && jdirect)
+    {
+        // force-only steps: the window's sums go straight to the id-ordered
+        // accumulators (integer sums, so the order of the adds does not
+        // matter); slots that received nothing are skipped
+        if (!ov)
+            for (int t = tid; t < total; t += blockDim.x)
+            {
+                const long long vx = acc[t], vy = acc[W.wcap + t], vz = acc[2 * W.wcap + t];
+                if ((vx | vy | vz) == 0) continue;
+                const int id = order[slot_s[t]];
+                if (vx) atomic_add_i64(fx + id, -vx);
+                if (vy) atomic_add_i64(fx + n + id, -vy);
+                if (vz) atomic_add_i64(fx + 2 * n + id, -vz);
+            }
+        return;
+    }
     const int base = stage_base[w];
     for (int t = tid; t < total; t += blockDim.x)
     {
@@ -920,6 +937,13 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
     // diagonal cell and rcell: the displacement skips the zero terms
     const bool ortho = box.c[1] == 0.0f && box.c[3] == 0.0f && box.c[4] == 0.0f &&
                        box.rc[1] == 0.0f && box.rc[3] == 0.0f && box.rc[4] == 0.0f;
+    // Force-only steps: the stage (about 430 bytes per atom at the preferred
+    // shapes) and its gather stay in the 256 MB last-level cache for small
+    // and mid-size systems; past 128 MiB of stage the blocks' atomic adds
+    // into the accumulators are cheaper than the stage's HBM round trip.
+    int jdirect = ctx->lj.j_direct;
+    if (jdirect < 0) jdirect = (size_t)n * 432 > ((size_t)128 << 20) ? 1 : 0;
+    if (need_energy || need_virial) jdirect = 0;
     snb_prof_begin(ctx, SNB_PROF_PAIR);
 #define SNB_WIN_LAUNCH(E, V, O)                                                                 \
     hipLaunchKernelGGL((k_pair_win<E, V, O>), dim3(W.nblock), dim3(1024), lds, s, n, W,         \
@@ -929,7 +953,7 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
                        ctx->lj.coul_scale, box, rc2, beta,                                     \
                        L.stage_base, L.stage, (size_t)n * L.invmax, ctx->fx, ctx->e_sorted,   \
                        ctx->v_sorted, ctx->lj.owner_attribution ? 0 : 1, ctx->e_atom,          \
-                       ctx->vir_atom, L.block_perm)
+                       ctx->vir_atom, L.block_perm, jdirect)
 #define SNB_WIN_LAUNCH_EV(O)                 \
     if (need_energy && need_virial)          \
         SNB_WIN_LAUNCH(true, true, O);       \
@@ -951,6 +975,7 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
 #undef SNB_WIN_LAUNCH
     snb_prof_end(ctx, SNB_PROF_PAIR);
     SNB_CHECK(hipGetLastError());
+    if (jdirect) return SNB_OK;
     hipLaunchKernelGGL(k_stage_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, L.inv_count,
                        L.inv_list, L.invmax, L.stage, (size_t)n * L.invmax, L.order, ctx->fx,
                        need_energy, need_virial,

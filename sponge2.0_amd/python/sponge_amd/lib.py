@@ -64,6 +64,7 @@ _SIGS = {
     "snb_nl_window_info": ([_P] + [ctypes.POINTER(_I)] * 6, _I),
     "snb_nl_set_window_capacity": ([_P, _I], _I),
     "snb_nl_set_window_shape": ([_P, _I, _I, _I], _I),
+    "snb_lj_set_direct_j": ([_P, _I], _I),
     "snb_nl_grid_shape": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "snb_lj_initial": ([_P, _I, _P, _P, _P, _F], _I),
     "snb_lj_pme_direct_force": ([_P, _I, _I, _I, _I, _P, _P, _P, LTMatrix3, LTMatrix3, _P, _F,
@@ -610,6 +611,11 @@ class Nonbonded:
             return False
         check(rc, "snb_nl_set_window_shape")
         return True
+
+    def set_direct_j(self, mode):
+        """Window forces of force-only steps: 0 through the stage and its
+        gather, 1 by atomic adds per block, -1 by system size (default)."""
+        check(self.L.snb_lj_set_direct_j(self.ctx, int(mode)), "snb_lj_set_direct_j")
 
     def window_info(self):
         v = [_I() for _ in range(6)]

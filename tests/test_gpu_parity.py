@@ -513,6 +513,35 @@ def test_window_overflow_path_bit_exact(water_9k, slots):
     nb.close()
 
 
+@pytest.mark.parametrize("slots", [None, 64])
+def test_window_direct_j_adds_bit_exact(water_9k, slots):
+    """Force-only steps with the window forces added atomically per block
+    (mode 1) match the twin bit for bit, as the staged gather (mode 0) does;
+    64 slots puts most blocks on the overflow path."""
+    import oracle
+    s = water_9k
+    t_f = None
+    for mode in (0, 1):
+        nb = _nb(s)
+        if slots:
+            nb.set_window_capacity(slots)
+        nb.set_direct_j(mode)
+        print("case mode", mode, "slots", slots, nb.window_info(), flush=True)
+        nb.nl_update(forced=True)
+        if t_f is None:
+            counts, rows = nb.nl_rows()
+            t_f = oracle.pair_twin(s["crd"], s["charge"], s["lj_type"], s["lj_a"], s["lj_b"],
+                                   s["cell"], s["rcell"], nb.cutoff, nb.beta, counts, rows)[0]
+        for rep in range(2):                   # the second call adds on top of the first
+            nb.lj_direct(need_energy=False, need_virial=False, frc=False)
+        np.testing.assert_array_equal(nb.fixed_forces(), 2 * t_f)
+        nb.close()
+    nb = _nb(s)
+    with pytest.raises(RuntimeError):
+        nb.set_direct_j(2)
+    nb.close()
+
+
 @pytest.mark.parametrize("overlap", [True, False])
 def test_pme_stream_overlap_same_results(water_9k, overlap):
     """The reciprocal pipeline on its own stream gives the same fused forces,
