@@ -188,7 +188,7 @@ def serving_phase(args):
                                                       "batched": bat["makespan_s"]},
             "same_alone_and_in_mix": "%d of %d requests bit-identical" % (same, len(reqs["batched"])),
             "status_flags": status, "tune_s": tune_s,
-            "shapes": {str(k): list(v[0]) for k, v in tuned.items()},
+            "shapes": {str(k): list(v[0]) + [v[2]] for k, v in tuned.items()},
             "launch": "per-segment HIP stream, one captured graph per 10-step period, "
                       "periods queued 2 ahead"}
 

@@ -7,8 +7,13 @@ blocks cut the window/stage overhead, small ones give more workgroups to fill
 the chip.  The library picks a shape from a fixed preference list; this tuner
 times every shape the library accepts for a context on that context's own
 workload (a full neighbour-list period of force steps at fixed coordinates)
-and keeps the fastest.  Every shape gives the same forces, energies and
-virials (integer fixed-point sums), so the choice affects time only.
+and keeps the fastest.  For each windowed shape it also times both ways the
+blocks hand their window forces to the accumulators on force-only steps
+(a staged per-atom gather, or atomic adds per block: the first is cheaper
+where the stage stays in the last-level cache and the gather's launch is
+hidden, the second removes a launch from small systems' dependent chain).
+Every choice gives the same forces, energies and virials (integer
+fixed-point sums), so it affects time only.
 """
 import numpy as np
 
@@ -50,22 +55,27 @@ def time_period(nb, interval, reps=2):
     return ev0.elapsed_time(ev1) / (reps * interval)
 
 
-def tune_window(nb, interval=10, reps=2, shapes=SHAPES, use_cache=True):
-    """Times each accepted shape and sets the fastest; returns (shape,
-    {shape: ms per step}).  Results are cached per grid key."""
+def tune_window(nb, interval=10, reps=2, shapes=SHAPES, modes=(0, 1), use_cache=True):
+    """Times each accepted shape (and, for windowed shapes, each hand-off mode
+    of snb_lj_set_direct_j) and sets the fastest; returns (shape, {(shape,
+    mode): ms per step}, mode).  Results are cached per grid key."""
     key = grid_key(nb)
     if use_cache and key in _CACHE:
-        best = _CACHE[key][0]
+        best, _, mode = _CACHE[key]
         if nb.set_window_shape(*best):
+            nb.set_direct_j(mode)
             return _CACHE[key]
     times = {}
     for shp in shapes:
         if not nb.set_window_shape(*shp):
             continue
-        times[shp] = time_period(nb, interval, reps)
-    best = min(times, key=times.get)
+        for mode in (modes if any(shp) else (-1,)):
+            nb.set_direct_j(mode)
+            times[(shp, mode)] = time_period(nb, interval, reps)
+    best, mode = min(times, key=times.get)
     nb.set_window_shape(*best)
+    nb.set_direct_j(mode)
     nb.zero_outputs()
     nb.clear_fixed_forces()
-    _CACHE[key] = (best, times)
-    return best, times
+    _CACHE[key] = (best, times, mode)
+    return _CACHE[key]

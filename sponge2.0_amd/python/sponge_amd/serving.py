@@ -9,8 +9,9 @@ Langevin leap-frog, SETTLE) on its stream, with a few periods queued ahead, so
 the kernels of all admitted segments overlap on the device.  Whenever a
 segment's last period completes it leaves, and arrived requests are admitted
 while slots and the HBM budget allow (admission by an upper estimate of the
-footprint; the measured one is reported).  Window shapes of the pair kernel
-are tuned per grid class on admission.  A segment's arithmetic does not
+footprint; the measured one is reported).  Window shapes of the pair kernel,
+and how its blocks hand the window forces on, are tuned per grid class on
+admission.  A segment's arithmetic does not
 depend on what runs beside it (integer fixed-point sums, counter-based
 noise), so its result is the same alone and in the mix.
 
@@ -77,7 +78,9 @@ class Segment:
         nb.set_pme_overlap(overlap)
         nb.set_export(False)       # nothing reads the reference-layout list here
         self.nb = nb
-        self.shape = autotune.tune_window(nb, interval=chunk)[0] if tune else None
+        # window shape and hand-off mode of the window forces, tuned per grid class
+        tuned = autotune.tune_window(nb, interval=chunk) if tune else (None, None, None)
+        self.shape, self.direct_j = tuned[0], tuned[2]
         with torch.cuda.stream(self.stream):
             nb.md_initial(s, temperature=temperature, tau=0.1, seed=req.seed, dt=0.002)
             # the first period runs eagerly (it builds the list from scratch)
@@ -120,7 +123,8 @@ class Segment:
         nb = self.nb
         nb.sync()
         self.req.result = dict(crd=nb.crd.cpu().numpy(), vel=nb.vel.cpu().numpy(),
-                               status=nb.status(), shape=self.shape, footprint=self.footprint)
+                               status=nb.status(), shape=self.shape, direct_j=self.direct_j,
+                               footprint=self.footprint)
         nb.close()
         self.nb = self.graph = None
 

@@ -9,6 +9,8 @@ pytestmark = pytest.mark.gpu
 
 
 def test_window_shapes_give_identical_results(water_9k):
+    """Every shape the library accepts, and both hand-off modes of the window
+    forces on force-only steps, give the same results bit for bit."""
     from sponge_amd import autotune
     from sponge_amd.lib import Nonbonded
     nb = Nonbonded(water_9k)
@@ -23,7 +25,16 @@ def test_window_shapes_give_identical_results(water_9k):
         nb.sync()
         got = (nb.frc.cpu().numpy().copy(), nb.atom_energy.cpu().numpy().copy(),
                nb.atom_virial.cpu().numpy().copy(), nb.energies())
+        # force-only steps in both hand-off modes of the window forces
+        for mode in (0, 1):
+            nb.set_direct_j(mode)
+            nb.zero_outputs()
+            nb.step(step=1)
+            nb.sync()
+            got += (nb.frc.cpu().numpy().copy(),)
+        nb.set_direct_j(-1)
         print("shape", shp, nb.window_info(), flush=True)
+        np.testing.assert_array_equal(got[4], got[5])
         if ref is None:
             ref = got
             continue
@@ -31,6 +42,7 @@ def test_window_shapes_give_identical_results(water_9k):
         np.testing.assert_array_equal(got[1], ref[1])
         np.testing.assert_array_equal(got[2], ref[2])
         assert got[3] == ref[3]
+        np.testing.assert_array_equal(got[4], ref[4])
     assert (0, 0, 0) in tried and len(tried) >= 5
     assert nb.status() == 0
     nb.close()
