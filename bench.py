@@ -69,7 +69,7 @@ def parse():
                          "full workload is 256)")
     ap.add_argument("--serve-steps", type=int, default=1000,
                     help="MD steps per serving request (2 fs)")
-    ap.add_argument("--serve-slots", type=int, default=4,
+    ap.add_argument("--serve-slots", type=int, default=8,
                     help="segments in flight in the batched run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (scales the HBM byte counters "

@@ -159,7 +159,7 @@ class Server:
     periods queued on its stream, so the device never waits for the host's
     bookkeeping."""
 
-    def __init__(self, slots=4, hbm_budget=64 * 2 ** 30, chunk=10, tune=True, overlap=False,
+    def __init__(self, slots=8, hbm_budget=64 * 2 ** 30, chunk=10, tune=True, overlap=False,
                  depth=2):
         self.slots, self.hbm_budget, self.chunk = slots, hbm_budget, chunk
         self.tune, self.overlap, self.depth = tune, overlap, depth
@@ -226,13 +226,13 @@ class Server:
 
 def main(argv=None):
     """Serves a seeded request stream and prints the summary as one JSON line:
-    python -m sponge_amd.serving --requests 16 --steps 200 --slots 4"""
+    python -m sponge_amd.serving --requests 16 --steps 200 --slots 8"""
     import argparse
     import json
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--requests", type=int, default=16)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--rate", type=float, default=40.0)
     ap.add_argument("--overlap", action="store_true",
                     help="reciprocal pipeline of each segment on its own stream")
