@@ -172,7 +172,8 @@ int snb_nl_set_window_shape(snb_context* ctx, int a, int b, int s);
 /* How the windowed pair kernel hands its window forces to the accumulators
  * on force-only steps: 0 = through a per-block stage and a per-atom gather
  * kernel, 1 = atomic integer adds at the end of each block, -1 (default) =
- * by size (1 once the stage would exceed 128 MiB, about 310k atoms).  Energy
+ * by size (1 below 65,536 atoms and once the stage would exceed 128 MiB,
+ * about 310k atoms; 0 between).  Energy
  * and virial steps always use the stage.  Results are the same either way
  * (integer sums); SNB_ERR_ARGUMENT for another mode (no counterpart in the
  * reference). */
