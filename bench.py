@@ -124,28 +124,76 @@ def cpu_baseline(sysd, nb, counts, rows, budget_s):
                       % (steps, el, nt)}, pme
 
 
-def accuracy(sysd, nb, counts, rows, pme, gpu_frc):
-    """Per-atom force relative RMS of the GPU step vs the FP64 oracle."""
+def frame_accuracy(nb, sysd, frame):
+    """One energy/virial step of the library on its current coordinates
+    against the FP64 oracle on the same coordinates and list."""
+    import numpy as np
+    s = dict(sysd)
+    s["crd"] = nb.crd.cpu().numpy().copy()
+    nb.zero_outputs()
+    nb.step(step=0, need_energy=True, need_virial=True, forced_nl=True, fused=True)
+    nb.sync()
+    gpu = {"frc": nb.frc.cpu().numpy(), "energy": nb.energies()}
+    if nb.nb14 is not None:
+        gpu["energy"]["nb14_LJ"], gpu["energy"]["nb14_EE"] = nb.nb14_energy()
+    if nb.bonds is not None:
+        gpu["energy"]["bond"] = nb.bond_energy()
+    # total virial: per-atom virials (pair, exclusion, 1-4, bond; float, as
+    # the reference's atom_virial) summed in FP64, plus the reciprocal tensor
+    gpu["virial"] = (nb.atom_virial.cpu().numpy().astype(np.float64).sum(axis=0)
+                     + nb.reciprocal_virial())
+    counts, rows = nb.internal_rows()
+    acc = accuracy(s, nb, counts, rows, None, gpu)
+    return {"force_rel_rms_vs_fp64": acc["force_rel_rms"],
+            "energy_rel_err_vs_fp64": acc["energy_rel_err"],
+            "virial_rel_err_vs_fp64": acc["virial_rel_err"],
+            "accuracy_frame": frame}
+
+
+def accuracy(sysd, nb, counts, rows, pme, gpu):
+    """Errors of the GPU energy/virial step against the FP64 oracle on the
+    same list: per-atom force relative RMS, each energy term's relative error
+    and the relative error (max norm) of the total virial tensor."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     s = sysd
     excl = (s["excl_start"], s["excl_numbers"], s["excl_list"])
-    f1, _, _, _ = oracle.pair_f64(s["crd"], s["charge"], s["lj_type"], s["lj_a"], s["lj_b"],
-                                  s["cell"], s["rcell"], nb.cutoff, nb.beta, counts, rows,
-                                  virial=False)
-    f2, _, _ = oracle.excl_f64(s["crd"], s["charge"], s["cell"], s["rcell"], nb.beta, excl,
-                               virial=False)
+    f1, e_lj, e_c, v1 = oracle.pair_f64(s["crd"], s["charge"], s["lj_type"], s["lj_a"],
+                                        s["lj_b"], s["cell"], s["rcell"], nb.cutoff, nb.beta,
+                                        counts, rows, virial=True)
+    f2, e_ex, v2 = oracle.excl_f64(s["crd"], s["charge"], s["cell"], s["rcell"], nb.beta, excl,
+                                   virial=True)
     if pme is None:
         pme = oracle.pme_f64(s["crd"], s["charge"], s["rcell"], nb.grid, nb.beta, nb.volume)
     ref = f1 + f2 + pme[0]
+    vir_atoms = v1 + v2
+    terms = {"LJ_short": float(np.sum(e_lj)), "PM_direct": float(np.sum(e_c)),
+             "PM_correction": float(np.sum(e_ex)), "PM_reciprocal": pme[1],
+             "PM_self": oracle.self_energy(s["charge"], nb.beta, nb.volume)}
+    terms["PM"] = (terms["PM_direct"] + terms["PM_correction"] + terms["PM_reciprocal"] +
+                   terms["PM_self"])
     if "nb14" in s:
-        ref = ref + oracle.nb14_f64(s["crd"], s["charge"], s["cell"], s["rcell"], s["nb14"],
-                                    virial=False)[0]
+        f14, ep14, _, v14 = oracle.nb14_f64(s["crd"], s["charge"], s["cell"], s["rcell"],
+                                            s["nb14"], virial=True)
+        ref, vir_atoms = ref + f14, vir_atoms + v14
+        terms["nb14_LJ"], terms["nb14_EE"] = float(ep14[0].sum()), float(ep14[1].sum())
     if "bonds" in s:
-        ref = ref + oracle.bond_f64(s["crd"], s["cell"], s["rcell"], s["bonds"], virial=False)[0]
-    g = gpu_frc.astype(np.float64)
-    return float(np.sqrt(((g - ref) ** 2).sum() / (ref ** 2).sum()))
+        fb, epb, _, vb = oracle.bond_f64(s["crd"], s["cell"], s["rcell"], s["bonds"],
+                                         virial=True)
+        ref, vir_atoms = ref + fb, vir_atoms + vb
+        terms["bond"] = float(epb.sum())
+    g = gpu["frc"].astype(np.float64)
+    out = {"force_rel_rms": float(np.sqrt(((g - ref) ** 2).sum() / (ref ** 2).sum()))}
+    # the four PM terms cancel heavily (self ~ -correction): each is taken
+    # relative to the PM total, the other terms relative to themselves
+    out["energy_rel_err"] = {k: abs(gpu["energy"][k] - v) / abs(terms["PM"] if k.startswith("PM_")
+                                                                else v)
+                             for k, v in terms.items() if k in gpu["energy"]}
+    vref = np.asarray(vir_atoms, np.float64).sum(axis=0) + np.asarray(pme[2])
+    vgot = gpu["virial"]
+    out["virial_rel_err"] = float(np.max(np.abs(vgot - vref)) / np.max(np.abs(vref)))
+    return out
 
 
 def serving_phase(args):
@@ -428,18 +476,10 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    # one more step with fresh outputs for the accuracy check
-    nb.zero_outputs()
-    nb.step(step=0, need_energy=True, need_virial=True, forced_nl=True, fused=True)
-    torch.cuda.synchronize()
-    gpu_frc = nb.frc.cpu().numpy()
-    counts, rows = (None, None)
-    if args.cpu_seconds > 0 or not args.no_accuracy:
-        counts, rows = nb.internal_rows()
-    base, pme = (None, None)
+    counts, rows = nb.internal_rows()
+    base = None
     if args.cpu_seconds > 0:
-        base, pme = cpu_baseline(sysd, nb, counts, rows, args.cpu_seconds)
-    err = None if args.no_accuracy else accuracy(sysd, nb, counts, rows, pme, gpu_frc)
+        base, _ = cpu_baseline(sysd, nb, counts, rows, args.cpu_seconds)
     npairs = None if counts is None else pairs_in_cutoff(sysd, counts, rows, nb.cutoff)
     per = {k: (v[0] / max(v[1], 1)) for k, v in prof.items()}
     pair_kernel = "k_pair_win" if nb.window_info()["windowed"] else "k_lj_direct"
@@ -481,13 +521,15 @@ def main():
         "kernel_avg_us": {k: v * 1000.0 for k, v in per.items()},
         "pair_window": nb.window_info(),
         "launch": "hip-graph replay per step kind" if graphs else "eager",
-        "force_rel_rms_vs_fp64": err,
         "status_flags": status,
         "wall_s_timed": wall,
         "cpu_baseline": base,
     }
     if args.md_steps > 0:
         out["md"] = md_phase(nb, sysd, args, kind)
+    if not args.no_accuracy:
+        out.update(frame_accuracy(nb, sysd, "after the md leg's %d steps" % (args.md_steps + 20)
+                                  if args.md_steps > 0 else "the initial coordinates"))
     if args.nve_steps > 0:
         out["nve"] = nve_phase(nb, sysd, args)
     nb.close()

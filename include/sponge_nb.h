@@ -103,7 +103,9 @@ int snb_set_exclusions(snb_context* ctx, const int* h_excluded_list_start,
  * neighbor_list_max_atom_in_grid_numbers (150), neighbor_list_refresh_interval
  * (0 = displacement criterion), neighbor_list_skin_permit (0.5).  Cells are
  * (cutoff+skin)/2 wide; boxes with < 3 cells on an axis return
- * SNB_ERR_BOX_TOO_SMALL. */
+ * SNB_ERR_BOX_TOO_SMALL.  The windowed pair path (below) is chosen only when
+ * max_atom_in_grid_numbers <= 256; a larger capacity keeps the plain path,
+ * whose cells hold up to max_atom_in_grid_numbers atoms as the reference's. */
 int snb_nl_initial(snb_context* ctx, float cutoff, float skin, snb_ltmatrix3 cell,
                    snb_ltmatrix3 rcell, int max_neighbor_numbers,
                    int max_atom_in_grid_numbers, int refresh_interval,
@@ -377,9 +379,13 @@ int snb_get_pressure(snb_context* ctx, int atom_numbers, const snb_vector* d_vel
 /* Particle_Mesh::Initial (PM_force.cpp:293-652) with the controller keys
  * resolved by the caller: PM_Direct_Tolerance (1e-5), PM_fftx/y/z (<= 0 =
  * automatic from box_length / PM_grid_spacing, common.cpp:639-659),
- * PME_update_interval (1).  beta comes from the tolerance (:244-272). */
+ * PME_update_interval (1).  beta comes from the tolerance (:244-272).
+ * no_direct_interaction_virtual_atom_numbers (:296, :467-479: extra
+ * reciprocal-only atoms past atom_numbers) must be 0: any other value returns
+ * SNB_ERR_UNSUPPORTED, since this library's atoms are the context's. */
 int snb_pme_initial(snb_context* ctx, snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
-                    snb_vector box_length, float cutoff, float tolerance,
+                    snb_vector box_length, float cutoff,
+                    int no_direct_interaction_virtual_atom_numbers, float tolerance,
                     int fftx, int ffty, int fftz, float grid_spacing,
                     int update_interval);
 int snb_pme_parameters(snb_context* ctx, float* beta, int* fftx, int* ffty, int* fftz);
@@ -411,7 +417,12 @@ int snb_pme_excluded_force(snb_context* ctx, int atom_numbers, const int* d_atom
  * gather; with need_energy adds reciprocal + self + neutralising energy to
  * d_potential[0]; with need_virial adds the reciprocal virial to d_virial[0].
  * rcell must be the one of the last snb_pme_initial / snb_pme_update_box
- * (SNB_ERR_ARGUMENT otherwise: the influence function belongs to a box). */
+ * (SNB_ERR_ARGUMENT otherwise: the influence function belongs to a box).
+ * With PME_update_interval k > 1 (:1079-1163) the force, scaled by k, and the
+ * virial come only on steps with step % k == 0; on the other steps nothing is
+ * added to d_frc or d_virial, and the reciprocal energy reported is the one of
+ * the last update step's grids (0 before the first), plus the self and
+ * neutralising terms of the current charges. */
 int snb_pme_reciprocal_force(snb_context* ctx, const snb_vector* d_crd, snb_ltmatrix3 cell,
                              snb_ltmatrix3 rcell, const float* d_charge, snb_vector* d_frc,
                              int need_virial, int need_energy, snb_ltmatrix3* d_virial,

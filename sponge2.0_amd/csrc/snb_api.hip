@@ -665,7 +665,7 @@ int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
         return rc;
     }
     // the windowed kernels address the list with 32-bit byte offsets
-    L.windowed = window_list_fits(c) &&
+    L.windowed = window_list_fits(c) && max_atom_in_grid_numbers <= SNB_NLC_MAXA &&
                  choose_window(L.nx, L.ny, L.nz, (double)n / L.ncell, c->max_lds, 2 * c->num_cu,
                                &L.win);
     if (L.windowed && (rc = setup_window(c)))
@@ -1425,11 +1425,14 @@ int snb_md_iterate_fused(snb_context* c, const float* d_mass, snb_vector* d_crd,
 
 // ---------------------------------------------------------------------- PME
 int snb_pme_initial(snb_context* c, snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
-                    snb_vector box_length, float cutoff, float tolerance, int fftx, int ffty,
-                    int fftz, float grid_spacing, int update_interval)
+                    snb_vector box_length, float cutoff,
+                    int no_direct_interaction_virtual_atom_numbers, float tolerance, int fftx,
+                    int ffty, int fftz, float grid_spacing, int update_interval)
 {
-    if (!c || !(cutoff > 0.0f) || !(tolerance > 0.0f) || update_interval <= 0)
+    if (!c || !(cutoff > 0.0f) || !(tolerance > 0.0f) || update_interval <= 0 ||
+        no_direct_interaction_virtual_atom_numbers < 0)
         return SNB_ERR_ARGUMENT;
+    if (no_direct_interaction_virtual_atom_numbers > 0) return SNB_ERR_UNSUPPORTED;
     if (grid_spacing <= 0.0f) grid_spacing = 1.0f;
     DeviceGuard g(c->device);
     SnbPme& P = c->pme;
@@ -1541,14 +1544,27 @@ int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatr
     DeviceGuard g(c->device);
     SnbBox box = snb_box(cell, rcell);
     int rc;
-    // PM_force.cpp:1080-1137: the reciprocal force (scaled by update_interval)
-    // and virial exist only on steps with step % update_interval == 0; on
-    // other steps only the energy is evaluated (here from the current
-    // coordinates, the reference reuses the grids of the last update step).
+    // PM_force.cpp:1079-1163: the reciprocal force (scaled by update_interval)
+    // and virial exist only on steps with step % update_interval == 0.  The
+    // energy of any other step is that of the grids of the last update step,
+    // plus the self term of the current charges.  So with an interval above 1
+    // every update step evaluates the reciprocal energy (it is a by-product of
+    // the convolution pass) and keeps it for the steps in between.
     bool do_force = (step % P.update_interval) == 0;
-    if (!do_force) need_virial = 0;
-    if (do_force || need_energy)
+    if (!do_force)
     {
+        need_virial = 0;
+        if (need_energy)
+        {
+            // after the last update step's pipeline, which wrote the kept energy
+            if ((rc = snb_join_pme(c))) return rc;
+            if ((rc = snb_launch_pme_self(c, d_charge))) return rc;
+            P.have_energy = true;
+        }
+    }
+    else
+    {
+        int keep_energy = need_energy || P.update_interval > 1;
         // the pipeline reads only crd/charge and module buffers and writes
         // fx_pme / dscal / the Q slots, so it may run beside the direct-space
         // work: on pme_stream, after the step's inputs are final
@@ -1563,10 +1579,9 @@ int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatr
             c->inputs_marked = false;
             c->stream = c->pme_stream;
         }
-        rc = need_energy ? snb_launch_charge_sums(c, d_charge) : SNB_OK;
+        rc = keep_energy ? snb_launch_charge_sums(c, d_charge) : SNB_OK;
         if (!rc)
-            rc = snb_launch_pme_recip(c, d_crd, d_charge, box, need_energy, need_virial,
-                                      do_force ? 1 : 0);
+            rc = snb_launch_pme_recip(c, d_crd, d_charge, box, keep_energy, need_virial, 1);
         if (c->overlap)
         {
             c->stream = home;
@@ -1574,7 +1589,7 @@ int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatr
             if (!rc) c->pme_pending = true;
         }
         if (rc) return rc;
-        P.have_energy = need_energy != 0;
+        P.have_energy = keep_energy != 0;
     }
     if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
     if ((d_potential && need_energy) || (d_virial && need_virial))

@@ -34,6 +34,11 @@ enum SnbDoubleSlot
     SNB_D_COUNT = 8
 };
 
+// Atoms per cell the windowed list builder keeps row counts for in LDS.  The
+// windowed path is taken only when max_atom_in_grid_numbers is at most this
+// (the reference's default is 150), so its overflow flag keeps the meaning
+// "a cell held more than max_atom_in_grid_numbers atoms".
+#define SNB_NLC_MAXA 256
 #define SNB_FFT_MAX_STAGES 16
 #define SNB_FFT_MAX_N 1024
 
@@ -325,6 +330,12 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
 // pair kernel (snb_pair.hip)
 int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBox& box,
                          float beta, int need_energy, int need_virial);
+// k_excl on any stream and accumulators: fx [3][n], e_atom with the
+// correction energy in its third slot (e_atom[2n + i]), vir [6][n]
+int snb_launch_excl_to(hipStream_t s, int n, const snb_vector* crd, const float* charge,
+                       const SnbBox& box, float beta, const int* ex_start, const int* ex_list,
+                       const int* ex_num, int need_energy, int need_virial, int64_t* fx,
+                       int64_t* e_atom, int64_t* vir);
 int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge,
                     const SnbBox& box, float beta,
                     const int* ex_start, const int* ex_list, const int* ex_num,
@@ -358,6 +369,8 @@ int snb_pme_setup_tables(snb_context* ctx, const SnbBox& box);
 int snb_pme_compute_bc(snb_context* ctx, const SnbBox& box);
 int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* charge,
                          const SnbBox& box, int need_energy, int need_virial, int with_force);
+// self + neutralising energy alone from the current charges (no grid work)
+int snb_launch_pme_self(snb_context* ctx, const float* charge);
 int snb_fft_forward_full(snb_context* ctx, const float* real, float2* cplx);
 int snb_launch_pme_energy_to_potential(snb_context* ctx, float* potential);
 int snb_launch_pme_virial_to(snb_context* ctx, snb_ltmatrix3* vir);

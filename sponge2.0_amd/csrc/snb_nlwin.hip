@@ -79,10 +79,9 @@ __device__ __forceinline__ void win_block_of_cell(const SnbWin& W, int c, int& c
     z0 = (cz / W.s) * W.s;
 }
 
-// Candidate capacity of one LDS chunk and the atoms per cell whose row counts
-// live in LDS (the reference caps a cell at max_atom_in_grid_numbers = 150).
+// Candidate capacity of one LDS chunk (the atoms per cell whose row counts
+// live in LDS, SNB_NLC_MAXA, are in snb_internal.h).
 #define SNB_NLC_CAP 1792
-#define SNB_NLC_MAXA 256
 #define SNB_NLC_MAXRUN 26
 
 // One 512-thread workgroup per cell.  The candidates of the cell's forward

@@ -266,18 +266,16 @@ __global__ void __launch_bounds__(256) k_excl(int n, const snb_vector* __restric
     }
 }
 
-int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge,
-                    const SnbBox& box, float beta,
-                    const int* ex_start, const int* ex_list, const int* ex_num,
-                    int need_energy, int need_virial, int64_t* fx)
+int snb_launch_excl_to(hipStream_t s, int n, const snb_vector* crd, const float* charge,
+                       const SnbBox& box, float beta, const int* ex_start, const int* ex_list,
+                       const int* ex_num, int need_energy, int need_virial, int64_t* fx,
+                       int64_t* e_atom, int64_t* vir)
 {
-    int n = ctx->n;
     if (n <= 0) return SNB_OK;
     dim3 g((n + 255) / 256);
-    snb_prof_begin(ctx, SNB_PROF_EXCL);
 #define SNB_EX_LAUNCH(E, V)                                                                  \
-    hipLaunchKernelGGL((k_excl<E, V>), g, dim3(256), 0, ctx->stream, n, crd, charge, box, beta, \
-                       ex_start, ex_list, ex_num, fx, ctx->e_atom, ctx->vir_atom)
+    hipLaunchKernelGGL((k_excl<E, V>), g, dim3(256), 0, s, n, crd, charge, box, beta,       \
+                       ex_start, ex_list, ex_num, fx, e_atom, vir)
     if (need_energy && need_virial)
         SNB_EX_LAUNCH(true, true);
     else if (need_energy)
@@ -287,9 +285,22 @@ int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge
     else
         SNB_EX_LAUNCH(false, false);
 #undef SNB_EX_LAUNCH
-    snb_prof_end(ctx, SNB_PROF_EXCL);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
+}
+
+int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge,
+                    const SnbBox& box, float beta,
+                    const int* ex_start, const int* ex_list, const int* ex_num,
+                    int need_energy, int need_virial, int64_t* fx)
+{
+    if (ctx->n <= 0) return SNB_OK;
+    snb_prof_begin(ctx, SNB_PROF_EXCL);
+    int rc = snb_launch_excl_to(ctx->stream, ctx->n, crd, charge, box, beta, ex_start, ex_list,
+                                ex_num, need_energy, need_virial, fx, ctx->e_atom,
+                                ctx->vir_atom);
+    snb_prof_end(ctx, SNB_PROF_EXCL);
+    return rc;
 }
 
 // Scaled 1-4 LJ and plain Coulomb of the 1-4 pairs (stands in for

@@ -473,7 +473,7 @@ __global__ void k_pme_finalize(const double* __restrict__ partials, int nblocks,
                                double beta, double nfactor, int want_self)
 {
     int c = threadIdx.x;
-    if (c < 7)
+    if (c < 7 && nblocks > 0)
     {
         double s = 0;
         for (int k = 0; k < nblocks; k++) s += partials[(size_t)k * 8 + c];
@@ -1139,6 +1139,18 @@ int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* c
                            ctx->dscal, ctx->scal, (double)P.beta, nfactor, need_energy ? 1 : 0);
         SNB_CHECK(hipGetLastError());
     }
+    return SNB_OK;
+}
+
+int snb_launch_pme_self(snb_context* ctx, const float* charge)
+{
+    SnbPme& P = ctx->pme;
+    int rc = snb_launch_charge_sums(ctx, charge);
+    if (rc) return rc;
+    double nfactor = -0.5 * M_PI / ((double)P.beta * P.beta * P.volume);
+    hipLaunchKernelGGL(k_pme_finalize, dim3(1), dim3(64), 0, ctx->stream, P.partials, 0,
+                       ctx->dscal, ctx->scal, (double)P.beta, nfactor, 1);
+    SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
 

@@ -90,7 +90,7 @@ _SIGS = {
     "snb_get_kinetic_to_stress": ([_P, _I, _P, _P, LTMatrix3, _P], _I),
     "snb_get_pressure": ([_P, _I, _P, _P, _P, LTMatrix3, _P, _P, ctypes.POINTER(LTMatrix3),
                           ctypes.POINTER(_F)], _I),
-    "snb_pme_initial": ([_P, LTMatrix3, LTMatrix3, Vector, _F, _F, _I, _I, _I, _F, _I], _I),
+    "snb_pme_initial": ([_P, LTMatrix3, LTMatrix3, Vector, _F, _I, _F, _I, _I, _I, _F, _I], _I),
     "snb_pme_update_box": ([_P, LTMatrix3, LTMatrix3, LTMatrix3, _F], _I),
     "snb_pme_parameters": ([_P, ctypes.POINTER(_F), ctypes.POINTER(_I), ctypes.POINTER(_I),
                             ctypes.POINTER(_I)], _I),
@@ -217,7 +217,7 @@ class Nonbonded:
             self.bonds = (ba, bb, bk, br)
         g = grid if grid is not None else sysd.get("pme_grid", (0, 0, 0))
         check(L.snb_pme_initial(ctx, self.cell, self.rcell, Vector(*[float(x) for x in self.box]),
-                                self.cutoff, tolerance, int(g[0]), int(g[1]), int(g[2]), 1.0,
+                                self.cutoff, 0, tolerance, int(g[0]), int(g[1]), int(g[2]), 1.0,
                                 update_interval), "snb_pme_initial")
         beta = ctypes.c_float()
         fx, fy, fz = _I(), _I(), _I()

@@ -27,7 +27,7 @@ from . import autotune, systems
 from .lib import Nonbonded
 
 WATER_VOLUME = 29.915      # A^3 per TIP3P water at 1.0 g/cm^3
-MAX_ATOMS_IN_CELL = 300
+MAX_ATOMS_IN_CELL = 256    # the largest per-cell capacity the windowed list builder keeps
 NS_PER_STEP = 2.0e-6       # 2 fs
 
 

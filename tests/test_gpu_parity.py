@@ -120,6 +120,66 @@ def test_pme_spread_and_gather_bit_exact_with_twin(request, name):
     nb.close()
 
 
+@pytest.mark.parametrize("overlap", [True, False])
+def test_pme_update_interval_two(small_water, overlap):
+    """PME_update_interval = 2 (PM_force.cpp:1079-1163): on step 0 the
+    reciprocal force is twice the interval-1 gather (bit-exact with the twin
+    at update_interval 2) and energy and virial match the FP64 oracle; on
+    step 1, with the atoms moved, no force and no virial are added and the
+    reported reciprocal energy is still that of step 0's grids; step 2 updates
+    again at the moved coordinates, equal to a fresh interval-1 context there."""
+    import torch
+    import oracle
+    s = small_water
+    nb = _nb(s, update_interval=2)
+    nb.set_pme_overlap(overlap)
+    nb.zero_outputs()
+    nb.reciprocal(step=0, need_energy=True, need_virial=True, frc=False)
+    phi = nb.pme_grid(1)
+    g0 = nb.fixed_forces()
+    np.testing.assert_array_equal(
+        g0, oracle.pme_gather_twin(s["crd"], s["charge"], s["rcell"], nb.grid, phi, 2.0))
+    ref = oracle.nonbonded_terms_f64(s, cutoff=nb.cutoff, grid=nb.grid)
+    e0 = nb.energies()
+    assert abs(e0["PM_reciprocal"] - ref["PM_reciprocal"]) <= ENERGY_TOL * abs(ref["PM"])
+    assert abs(e0["PM_self"] - ref["PM_self"]) <= ENERGY_TOL * abs(ref["PM"])
+    v0 = nb.reciprocal_virial()
+    assert np.max(np.abs(v0 - ref["virial_recip"])) <= 1e-5 * np.max(np.abs(ref["virial_recip"]))
+    pot0 = float(nb.potential.cpu()[0])
+    assert np.any(nb.virial.cpu().numpy() != 0)
+    # step 1: atoms moved, no update
+    rng = np.random.default_rng(5)
+    moved = (s["crd"] + rng.uniform(-0.05, 0.05, s["crd"].shape)).astype(np.float32)
+    nb.crd.copy_(torch.from_numpy(moved).to(nb.crd.device))
+    nb.clear_fixed_forces()
+    nb.zero_outputs()
+    nb.reciprocal(step=1, need_energy=True, need_virial=True, frc=False)
+    assert not np.any(nb.fixed_forces())
+    assert not np.any(nb.virial.cpu().numpy())
+    e1 = nb.energies()
+    assert e1["PM_reciprocal"] == e0["PM_reciprocal"]
+    assert e1["PM_self"] == e0["PM_self"]
+    assert float(nb.potential.cpu()[0]) == pot0
+    # step 2: update at the moved coordinates
+    nb.zero_outputs()
+    nb.reciprocal(step=2, need_energy=True, need_virial=True, frc=False)
+    g2 = nb.fixed_forces()
+    phi2 = nb.pme_grid(1)
+    np.testing.assert_array_equal(
+        g2, oracle.pme_gather_twin(moved, s["charge"], s["rcell"], nb.grid, phi2, 2.0))
+    s2 = dict(s)
+    s2["crd"] = moved
+    nb1 = _nb(s2)
+    nb1.set_pme_overlap(overlap)
+    nb1.reciprocal(step=0, need_energy=True, need_virial=True, frc=False)
+    np.testing.assert_array_equal(phi2, nb1.pme_grid(1))
+    e2, e21 = nb.energies(), nb1.energies()
+    assert e2["PM_reciprocal"] == e21["PM_reciprocal"]
+    assert list(nb.reciprocal_virial()) == list(nb1.reciprocal_virial())
+    nb.close()
+    nb1.close()
+
+
 @pytest.mark.parametrize("grid", [(48, 48, 48), (24, 24, 20), (28, 40, 36), (72, 72, 72),
                                   (96, 96, 96), (30, 42, 56)])
 def test_fft_matches_float64_dft_and_rocfft(water_9k, grid):
@@ -415,7 +475,9 @@ def test_window_rows_follow_moving_atoms(water_9k):
 
 def test_window_93k_workload_pair_exact(request):
     """The 93k-atom workload (30,000 waters + 3,000 beads, 96.5 A box): pair
-    forces bit-exact with the twin, fixed-point total exactly zero."""
+    forces bit-exact with the twin, fixed-point total exactly zero, on an
+    energy step and on force-only steps in the default hand-off of the window
+    forces (staged gather at this size) and with per-block atomic adds."""
     import oracle
     from sponge_amd import systems
     s = systems.config_gpu_93k()
@@ -432,6 +494,12 @@ def test_window_93k_workload_pair_exact(request):
                                            nb.beta, counts, rows, virial=False)
     np.testing.assert_array_equal(g_f, t_f)
     np.testing.assert_array_equal(nb.fixed_atom_terms(0), t_elj)
+    for mode in (-1, 1):
+        print("case 93k force-only, hand-off mode", mode, flush=True)
+        nb.set_direct_j(mode)
+        nb.clear_fixed_forces()
+        nb.lj_direct(False, False, frc=False)
+        np.testing.assert_array_equal(nb.fixed_forces(), t_f)
     nb.close()
 
 
@@ -553,7 +621,7 @@ def test_pme_stream_overlap_same_results(water_9k, overlap):
         nb.set_pme_overlap(ov)
         nb.step(step=0, forced_nl=True, need_energy=True, need_virial=True)
         nb.zero_outputs()
-        nb.step(step=1)                        # force-only: correction on the PME stream
+        nb.step(step=1)                        # force-only: reciprocal pipeline on the PME stream
         torch.cuda.synchronize()
         f_plain = nb.frc.cpu().numpy().copy()
         nb.zero_outputs()
