@@ -57,6 +57,7 @@ typedef struct snb_context snb_context;
 #define SNB_ERR_NOT_INITIALIZED 5
 #define SNB_ERR_OUT_OF_MEMORY 6
 #define SNB_ERR_DEVICE_STATUS 7 /* a sticky device flag is set, see snb_read_status */
+#define SNB_ERR_TOO_LARGE 8     /* atom_numbers above SNB_NOALLOC_MAX_ATOMS (allocation-free entry point) */
 
 /* sticky device flags (snb_read_status); the reference keeps the first three
  * as device ints but never reads them (neighbor_list.cpp:887-965) */
@@ -411,6 +412,28 @@ int snb_pme_excluded_force(snb_context* ctx, int atom_numbers, const int* d_atom
                            const int* d_excluded_numbers, snb_vector* d_frc,
                            int need_energy, float* d_atom_energy, int need_virial,
                            snb_ltmatrix3* d_atom_virial);
+
+/* Allocation-free variant of snb_pme_excluded_force.  It has no counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc), and it needs no context.  The caller passes inputs and outputs
+ * (device pointers, the exclusion CSR of snb_set_exclusions, beta of
+ * snb_pme_parameters) and the stream; frc, atom_energy and atom_virial are
+ * accumulated (+=) as by snb_pme_excluded_force, bit for bit.  The fixed-point
+ * accumulators live in a library-owned device array of SNB_NOALLOC_MAX_ATOMS
+ * x 80 bytes (10 MiB) per GPU, held until the process ends; larger systems
+ * return SNB_ERR_TOO_LARGE.  Every call on one GPU shares that array and
+ * clears what it uses, so calls that overlap on one GPU race; calls must be
+ * ordered (one stream, or events).  The first call on a device looks up the
+ * array's device address (hipGetSymbolAddress); no call allocates. */
+#define SNB_NOALLOC_MAX_ATOMS 131072
+int snb_pme_excluded_force_noalloc(int atom_numbers, const snb_vector* d_crd,
+                                   snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
+                                   const float* d_charge, float beta,
+                                   const int* d_excluded_list_start, const int* d_excluded_list,
+                                   const int* d_excluded_numbers, snb_vector* d_frc,
+                                   int need_energy, float* d_atom_energy, int need_virial,
+                                   snb_ltmatrix3* d_atom_virial, void* hip_stream);
 
 /* Particle_Mesh::PME_Reciprocal_Force_With_Energy_And_Virial (PM_force.cpp:1067-1165):
  * B-spline spread, 3-D R2C FFT, influence-function convolution, C2R, force

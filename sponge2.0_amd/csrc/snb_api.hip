@@ -383,6 +383,9 @@ const char* snb_status_string(int code)
         case SNB_ERR_NOT_INITIALIZED: return "module not initialised";
         case SNB_ERR_OUT_OF_MEMORY: return "device allocation failed";
         case SNB_ERR_DEVICE_STATUS: return "a device status flag is set";
+        case SNB_ERR_TOO_LARGE:
+            return "atom_numbers is above SNB_NOALLOC_MAX_ATOMS, the capacity of the "
+                   "allocation-free entry point's device array";
         default: return "unknown status";
     }
 }

@@ -96,6 +96,8 @@ _SIGS = {
                             ctypes.POINTER(_I)], _I),
     "snb_pme_excluded_force": ([_P, _I, _P, _P, _P, LTMatrix3, LTMatrix3, _P, _P, _P, _P, _P,
                                 _I, _P, _I, _P], _I),
+    "snb_pme_excluded_force_noalloc": ([_I, _P, LTMatrix3, LTMatrix3, _P, _F, _P, _P, _P, _P,
+                                        _I, _P, _I, _P, _P], _I),
     "snb_pme_reciprocal_force": ([_P, _P, LTMatrix3, LTMatrix3, _P, _P, _I, _I, _P, _P, _I], _I),
     "snb_pme_energy": ([_P] + [ctypes.POINTER(ctypes.c_double)] * 4, _I),
     "snb_pme_reciprocal_virial": ([_P, _P], _I),
