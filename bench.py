@@ -62,6 +62,12 @@ def parse():
                          "100,000)")
     ap.add_argument("--nve-equil", type=int, default=2000,
                     help="Langevin steps (300 K, tau 0.1 ps, 1 fs) before the NVE run")
+    ap.add_argument("--nve-f64-steps", type=int, default=None,
+                    help="NVE steps of the all-FP64 trajectory from the same state (default "
+                         "5,000, 0 for water1m; the full study is 100,000)")
+    ap.add_argument("--nve-compare-every", type=int, default=1000,
+                    help="FP64-trajectory steps between frames on which the FP32 path and the "
+                         "FP64 evaluation are compared")
     ap.add_argument("--serve-requests", type=int, default=None,
                     help="serving workload: this many water-box segment requests (N log-uniform "
                          "1,500-50,000 atoms, Poisson arrivals at 40/s), served one at a time and "
@@ -78,6 +84,7 @@ def parse():
     big = a.workload == "water1m"
     defaults = {"refresh": 20 if big else 10, "cpu_seconds": 0.0 if big else 15.0,
                 "md_steps": 0 if big else 500, "nve_steps": 0 if big else 20000,
+                "nve_f64_steps": 0 if big else 5000,
                 "serve_requests": 0 if big else 64}
     for k, v in defaults.items():
         if getattr(a, k) is None:
@@ -241,19 +248,40 @@ def serving_phase(args):
                       "periods queued 2 ahead"}
 
 
-def nve_phase(nb, sysd, args):
-    """Energy conservation of the FP32 / int64 fixed-point path: Maxwell
-    velocities at 300 K, a Langevin equilibration, then NVE leap-frog at 1 fs
-    with SETTLE; the total energy (potential of the path + bonds + 1-4 terms,
-    kinetic as the mean of the two half-step values around each sample) every
-    100 steps, and its linear drift per ns per degree of freedom."""
+def _drift(t_ps, etot, dof):
     import numpy as np
+    t, E = np.asarray(t_ps), np.asarray(etot)
+    slope = float(np.polyfit(t, E, 1)[0]) if len(t) > 1 else 0.0   # kcal/mol per ps
+    kt = 0.00198716 * 300.0
+    return {"samples": len(E), "E_total_first": float(E[0]), "E_total_last": float(E[-1]),
+            "E_total_std": float(E.std()),
+            "drift_kcal_per_mol_per_ns_per_dof": slope * 1000.0 / dof,
+            "drift_kT_per_ns_per_dof": slope * 1000.0 / dof / kt}
+
+
+def nve_phase(nb, sysd, args):
+    """Precision study: Maxwell velocities at 300 K and a Langevin
+    equilibration of the FP32 path, then from that one state two NVE
+    leap-frog trajectories at 1 fs with SETTLE: the FP32 / int64 path and the
+    all-FP64 evaluation of the same path (snb_f64_force / snb_f64_nve_iterate).
+    Total energy (path potential + kinetic as the mean of the two half-step
+    values) every 100 steps, linear drift per ns per degree of freedom for
+    both; on frames of the FP64 trajectory (every --nve-compare-every steps)
+    the FP32 path and the FP64 evaluation on the same float coordinates:
+    per-atom force relative RMS, each energy term's relative error (PM terms
+    relative to the PM total) and the total virial's."""
+    import numpy as np
+    import torch
+    from sponge_amd import lib
     nb.md_initial(sysd, temperature=300.0, tau=0.1, seed=20260218, dt=0.001)
     for k in range(args.nve_equil):
         nb.md_step(step=k, forced_nl=(k == 0))
     nb.sync()
+    crd0, vel0 = nb.crd.clone(), nb.vel.clone()
+    nw = int(sysd["n_waters"])
+    dof = 6 * nw + 3 * (sysd["n"] - 3 * nw) - 3
     sample = 100
-    t_ps, etot, epot = [], [], []
+    t_ps, etot = [], []
     t0 = time.perf_counter()
     for k in range(args.nve_steps):
         e = (k % sample) == 0
@@ -264,24 +292,104 @@ def nve_phase(nb, sysd, args):
             u = nb.potential_energy()
             ke1 = nb.kinetic_energy()
             t_ps.append(k * 0.001)
-            epot.append(u)
             etot.append(u + 0.5 * (ke0 + ke1))
+        if (k + 1) % 10000 == 0:
+            print("nve fp32: %d steps, E_total %.3f" % (k + 1, etot[-1]), file=sys.stderr,
+                  flush=True)
     nb.sync()
     wall = time.perf_counter() - t0
-    nw = int(sysd["n_waters"])
-    dof = 6 * nw + 3 * (sysd["n"] - 3 * nw) - 3
-    t = np.asarray(t_ps)
-    E = np.asarray(etot)
-    slope = float(np.polyfit(t, E, 1)[0]) if len(t) > 1 else 0.0   # kcal/mol per ps
-    kt = 0.00198716 * 300.0
-    return {"steps": args.nve_steps, "dt_fs": 1.0, "equilibration_steps": args.nve_equil,
-            "dof": dof, "samples": len(E), "E_total_first": float(E[0]),
-            "E_total_last": float(E[-1]), "E_total_std": float(E.std()),
-            "drift_kcal_per_mol_per_ns_per_dof": slope * 1000.0 / dof,
-            "drift_kT_per_ns_per_dof": slope * 1000.0 / dof / kt,
-            "wall_s": wall, "status_flags": nb.status(),
-            "arithmetic": "FP32 pair/spline/integrator math, int64 fixed-point forces, "
-                          "FP64 energy reductions"}
+    out = {"steps": args.nve_steps, "dt_fs": 1.0, "equilibration_steps": args.nve_equil,
+           "dof": dof}
+    out.update(_drift(t_ps, etot, dof))
+    out.update({"wall_s": wall, "status_flags": nb.status(),
+                "arithmetic": "FP32 pair/spline/integrator math, int64 fixed-point forces, "
+                              "FP64 energy reductions"})
+    if args.nve_f64_steps <= 0:
+        return out
+    # the FP64 trajectory from the same state
+    nb64 = lib.Nonbonded(sysd, grid=nb.grid, refresh_interval=10)
+    nb64.md_initial(sysd, temperature=300.0, tau=0.1, seed=20260218, dt=0.001)  # SETTLE
+    crd, vel = crd0.double().contiguous(), vel0.double().contiguous()
+    mass = torch.as_tensor(np.asarray(sysd["mass"], np.float64)).to(nb.dev)
+    inv = torch.where(mass > 0, 1.0 / torch.where(mass > 0, mass, torch.ones_like(mass)),
+                      torch.zeros_like(mass))
+    frc = torch.zeros_like(crd)
+    probe64 = torch.zeros_like(crd)
+
+    def ke64():
+        return 0.5 * float((mass[:, None] * vel * vel).sum())
+
+    def potential(t):
+        return t["LJ"] + t["PM"] + t["nb14_LJ"] + t["nb14_EE"] + t["bond"]
+
+    t64, e64, frames = [], [], []
+    dt = nb.dt_internal
+    t0 = time.perf_counter()
+    for k in range(args.nve_f64_steps):
+        e = (k % sample) == 0
+        if e:
+            ke0 = ke64()
+        frc.zero_()
+        terms, _ = nb64.f64_force(crd, frc, step=k, forced_nl=(k == 0), need_energy=e)
+        if args.nve_compare_every > 0 and k % args.nve_compare_every == 0:
+            frames.append(_compare_frame(nb, nb64, crd, probe64))
+        nb64.f64_nve_iterate(mass, inv, frc, vel, crd, dt)
+        if e:
+            t64.append(k * 0.001)
+            e64.append(potential(terms) + 0.5 * (ke0 + ke64()))
+        if (k + 1) % 5000 == 0:
+            print("nve fp64: %d steps, E_total %.3f" % (k + 1, e64[-1]), file=sys.stderr,
+                  flush=True)
+    torch.cuda.synchronize()
+    f64 = {"steps": args.nve_f64_steps, "wall_s": time.perf_counter() - t0,
+           "status_flags": nb64.status(),
+           "arithmetic": "FP64 pair/exclusion/1-4/bond/PME (direct FP64 DFTs) and leap-frog + "
+                         "SETTLE; forces summed in int64 at 2^40"}
+    f64.update(_drift(t64, e64, dof))
+    nb64.close()
+    out["fp64"] = f64
+    if frames:
+        keys = sorted(frames[0]["energy_rel_err"])
+        out["frames_compared"] = len(frames)
+        out["force_rel_rms_vs_fp64"] = {"max": max(f["force"] for f in frames),
+                                        "mean": float(np.mean([f["force"] for f in frames])),
+                                        "last": frames[-1]["force"]}
+        out["energy_rel_err_vs_fp64_max"] = {k: max(f["energy_rel_err"][k] for f in frames)
+                                             for k in keys}
+        out["potential_rel_err_vs_fp64_max"] = max(f["potential"] for f in frames)
+        out["virial_rel_err_vs_fp64_max"] = max(f["virial"] for f in frames)
+    return out
+
+
+def _compare_frame(nb, nb64, crd64, probe64):
+    """The FP32 path (context nb) and the FP64 evaluation (context nb64) on
+    the float rounding of the FP64 trajectory's current frame."""
+    import numpy as np
+    x32 = crd64.float()
+    nb.crd.copy_(x32)
+    nb.zero_outputs()
+    nb.step(step=0, need_energy=True, need_virial=True, forced_nl=True, fused=True)
+    nb.sync()
+    e32 = nb.energies()
+    e32["nb14_LJ"], e32["nb14_EE"] = nb.nb14_energy() if nb.nb14 is not None else (0.0, 0.0)
+    e32["bond"] = nb.bond_energy() if nb.bonds is not None else 0.0
+    f32 = nb.frc.cpu().numpy().astype(np.float64)
+    v32 = (nb.atom_virial.cpu().numpy().astype(np.float64).sum(axis=0) + nb.reciprocal_virial())
+    probe64.zero_()
+    t64, v64 = nb64.f64_force(x32.double().contiguous(), probe64, step=0, forced_nl=True,
+                              need_energy=True, need_virial=True)
+    f64 = probe64.cpu().numpy()
+    err = {}
+    for k, v in t64.items():
+        if k in ("LJ", "PM") or v == 0.0:
+            continue
+        scale = abs(t64["PM"]) if k.startswith("PM_") else abs(v)
+        err[k] = abs(e32[k] - v) / scale
+    u32 = e32["LJ"] + e32["PM"] + e32["nb14_LJ"] + e32["nb14_EE"] + e32["bond"]
+    u64 = t64["LJ"] + t64["PM"] + t64["nb14_LJ"] + t64["nb14_EE"] + t64["bond"]
+    return {"force": float(np.sqrt(((f32 - f64) ** 2).sum() / (f64 ** 2).sum())),
+            "energy_rel_err": err, "potential": abs(u32 - u64) / abs(u64),
+            "virial": float(np.max(np.abs(v32 - v64)) / np.max(np.abs(v64)))}
 
 
 def md_phase(nb, sysd, args, kind):

@@ -485,6 +485,36 @@ int snb_copy_pme_grid(snb_context* ctx, int which, float* d_out);
 int snb_fft_r2c(snb_context* ctx, const float* d_real, float* d_complex);
 int snb_fft_c2r(snb_context* ctx, const float* d_complex, float* d_real);
 
+/* ------------------------------------------------ FP64 reference evaluation */
+/* No counterpart in the reference (which builds in float): the whole force
+ * path of this context in double precision, for precision studies of the
+ * FP32 / int64 path (an FP64 trajectory from the same start, compared frame by
+ * frame and by energy drift).  Uses the context's modules (LJ tables, 1-4
+ * pairs, bonds, exclusions, PME grid and beta, SETTLE triangles) with double
+ * coordinates d_crd[3n] and forces d_frc[3n] (accumulated, +=):
+ * pairs over the reference-layout half list (the context's list, rebuilt from
+ * a float copy of d_crd by the NEIGHBOR_LIST::Update policy at this step; the
+ * list export is switched on), each pair, the exclusion correction, the 1-4
+ * pairs and the bonds in double; PME reciprocal with a double spread, direct
+ * FP64 DFTs along each axis, the influence function on the full grid and a
+ * double gather.  Forces are summed in int64 at 2^40, energies and the virial
+ * at 2^32 (order-independent).  With need_energy, h_terms[9] receives LJ_short,
+ * LJ_long, PM_direct, PM_reciprocal, PM_self, PM_correction, nb14_LJ,
+ * nb14_EE, bond; with need_virial, h_virial[6] the total virial tensor of the
+ * pair, exclusion, 1-4, bond and reciprocal terms (a11 a21 a22 a31 a32 a33,
+ * off-diagonal entries as symmetrised sums); both synchronise the stream. */
+int snb_f64_force(snb_context* ctx, const double* d_crd, const float* d_charge,
+                  snb_ltmatrix3 cell, snb_ltmatrix3 rcell, int step, int forced_nl,
+                  double* d_frc, int need_energy, int need_virial, double* h_terms,
+                  double* h_virial);
+/* One FP64 NVE step after snb_f64_force (no counterpart in the reference):
+ * SETTLE's remember, leap-frog v += f/m dt, x += v dt (NVE_iteration::Leap_Frog,
+ * MD_core/nve.hpp:3-48) and SETTLE with v_factor = x_factor = 1 on the
+ * context's triangles, all in double (dt in the internal time unit). */
+int snb_f64_nve_iterate(snb_context* ctx, const double* d_mass, const double* d_inverse_mass,
+                        const double* d_frc, double* d_vel, double* d_crd,
+                        snb_ltmatrix3 cell, snb_ltmatrix3 rcell, double dt);
+
 /* ------------------------------------------------------------ profiling */
 /* Kernel timing with HIP events recorded on the context stream around each
  * launch of a kernel group (no counterpart in the reference, whose

@@ -70,12 +70,13 @@ def _declare(L):
     L.orc_fft3d_f64.argtypes = [_p, _i, _i, _i, _i]
     L.orc_dft3d_naive_f64.argtypes = [_p, _p, _i, _i, _i, _i]
     L.orc_pme_f64.argtypes = [_i, _p, _p, _p, _i, _i, _i, _f, _d, _p, _p, _p, _p, _p]
+    L.orc_pme_f64_dfrac.argtypes = [_i, _p, _p, _p, _i, _i, _i, _f, _d, _p, _p, _p]
     L.orc_pme_spread_twin.argtypes = [_i, _p, _p, _p, _i, _i, _i, _p]
     L.orc_pme_gather_twin.argtypes = [_i, _p, _p, _p, _i, _i, _i, _p, _f, _p]
     L.orc_ewald_f64.argtypes = [_i, _p, _p, _p, _d, _d, _i, _p, _p, _p, _p]
     L.orc_lj_c6_total.restype = _d
     L.orc_lj_c6_total.argtypes = [_i, _p, _i, _p]
-    L.orc_twin_erf_parts.argtypes = [_d, ctypes.POINTER(_d), ctypes.POINTER(_d)]
+    L.orc_twin_erfc_parts.argtypes = [_d, ctypes.POINTER(_d), ctypes.POINTER(_d)]
     L.orc_twin_exp_d.restype = _d
     L.orc_twin_exp_d.argtypes = [_d]
     for name in ("orc_twin_rsqrtf",):
@@ -348,8 +349,11 @@ def dft3d_naive(a, sign=-1):
     return out
 
 
-def pme_f64(crd, charge, rcell, grid, beta, volume, want_grids=False):
-    """Returns (frc double[n,3], e_rec, vir[6], Q, phi) (grids only on request)."""
+def pme_f64(crd, charge, rcell, grid, beta, volume, want_grids=False, frac_double=False):
+    """Returns (frc double[n,3], e_rec, vir[6], Q, phi) (grids only on request).
+    frac_double: every step in double -- grid points, spline fractions and the
+    influence function's wave vectors -- where the default follows the kernels'
+    float steps (no grids then)."""
     crd = _c(crd, np.float32)
     n = crd.shape[0]
     fx, fy, fz = grid
@@ -358,6 +362,11 @@ def pme_f64(crd, charge, rcell, grid, beta, volume, want_grids=False):
     vir = np.zeros(6, np.float64)
     q = np.zeros(grid, np.float64) if want_grids else None
     phi = np.zeros(grid, np.float64) if want_grids else None
+    if frac_double:
+        lib().orc_pme_f64_dfrac(n, _ptr(crd), _ptr(_c(charge, np.float32)),
+                                _ptr(_c(rcell, np.float32)), fx, fy, fz, beta, volume,
+                                _ptr(frc), _ptr(e), _ptr(vir))
+        return frc, float(e[0]), vir, None, None
     lib().orc_pme_f64(n, _ptr(crd), _ptr(_c(charge, np.float32)),
                       _ptr(_c(rcell, np.float32)), fx, fy, fz, beta, volume,
                       _ptr(frc), _ptr(e), _ptr(vir), _ptr(q), _ptr(phi))
@@ -419,7 +428,7 @@ def lj_long_range_factor(ltype, ntypes, lj_b, cutoff):
 
 
 def nonbonded_terms_f64(sysd, cutoff=None, skin=2.0, tolerance=1e-5, grid=None,
-                        want_forces=True):
+                        want_forces=True, pme_frac_double=False):
     """All mdout terms of the path for a system dict (see sponge_amd.systems):
     LJ_short, LJ_long, PM_direct, PM_reciprocal, PM_self, PM_correction, PM,
     nb14_LJ and nb14_EE (0 without 1-4 pairs), bond (0 without bonds), plus total forces
@@ -439,7 +448,8 @@ def nonbonded_terms_f64(sysd, cutoff=None, skin=2.0, tolerance=1e-5, grid=None,
                                          sysd["lj_b"], cell, rcell, rc, beta, counts,
                                          rows)
     f_ex, e_ex, v_ex = excl_f64(crd, q, cell, rcell, beta, excl)
-    f_rec, e_rec, v_rec, _, _ = pme_f64(crd, q, rcell, grid, beta, vol)
+    f_rec, e_rec, v_rec, _, _ = pme_f64(crd, q, rcell, grid, beta, vol,
+                                        frac_double=pme_frac_double)
     e_self = self_energy(q, beta, vol)
     lrf = lj_long_range_factor(sysd["lj_type"], sysd["lj_ntypes"], sysd["lj_b"], rc)
     out = dict(beta=beta, grid=grid, volume=vol, nl_overflow=ov,

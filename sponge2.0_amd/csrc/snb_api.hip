@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "snb_internal.h"
@@ -253,6 +254,7 @@ void free_settle(SnbSettle& S)
     dfree(S.last_ca);
     dfree(S.vir6);
     dfree(S.free_atoms);
+    S.geo_d.clear();
     S.count = 0;
     S.nfree = 0;
     S.disjoint = false;
@@ -260,32 +262,33 @@ void free_settle(SnbSettle& S)
 }
 
 // Get_Rabcde_From_SSS (settle.cpp:50-98): the canonical frame of a triangle
-// with sides AB, AC, BC and masses mA, mB, mC (float, host).
-static void settle_frame(float rAB, float rAC, float rBC, float mA, float mB, float mC,
-                         float* out)
+// with sides AB, AC, BC and masses mA, mB, mC (host; float for the FP32 path,
+// double for the FP64 one).
+template <class T>
+static void settle_frame(T rAB, T rAC, T rBC, T mA, T mB, T mC, T* out)
 {
-    const float mt = (mA + mB) + mC;
-    float Ax = 0.0f, Ay = 0.0f, Bx = -rAB, By = 0.0f;
-    float ct = ((rBC * rBC - rAC * rAC) - rAB * rAB) / ((2.0f * rAC) * rAB);
-    float Cx = rAC * ct;
-    float st = sqrtf(1.0f - ct * ct);
-    float Cy = rAC * st;
-    const float Ox = (Bx * mB + Cx * mC) / mt;
-    const float Oy = (Cy * mC) / mt;
+    const T mt = (mA + mB) + mC;
+    T Ax = 0, Ay = 0, Bx = -rAB, By = 0;
+    T ct = ((rBC * rBC - rAC * rAC) - rAB * rAB) / ((T(2) * rAC) * rAB);
+    T Cx = rAC * ct;
+    T st = std::sqrt(T(1) - ct * ct);
+    T Cy = rAC * st;
+    const T Ox = (Bx * mB + Cx * mC) / mt;
+    const T Oy = (Cy * mC) / mt;
     Ax -= Ox;
     Ay -= Oy;
     Bx -= Ox;
     By -= Oy;
     Cx -= Ox;
     Cy -= Oy;
-    ct = 1.0f / sqrtf(1.0f + ((Ax * Ax) / Ay) / Ay);
+    ct = T(1) / std::sqrt(T(1) + ((Ax * Ax) / Ay) / Ay);
     st = (ct * Ax) / Ay;
-    float ra = Ax * st + Ay * ct;
-    float rc = Bx * ct - By * st;
-    float rb = Bx * st + By * ct;
-    float rd = Cx * ct - Cy * st;
-    float re = Cx * st + Cy * ct;
-    if (ra < 0.0f)
+    T ra = Ax * st + Ay * ct;
+    T rc = Bx * ct - By * st;
+    T rb = Bx * st + By * ct;
+    T rd = Cx * ct - Cy * st;
+    T re = Cx * st + Cy * ct;
+    if (ra < T(0))
     {
         ra = -ra;
         rb = -rb;
@@ -345,6 +348,40 @@ int snb_join_pme(snb_context* c)
     SNB_CHECK(hipStreamWaitEvent(c->stream, c->ev_pme, 0));
     c->pme_pending = false;
     return SNB_OK;
+}
+
+// Force-only work of the fused mode (no float output) that the reciprocal
+// stream takes over: the exclusion correction, the 1-4 pairs and the bonds
+// read only coordinates, charges and module tables, so they may run beside
+// the direct-space kernels, on pme_stream ahead of the reciprocal pipeline,
+// adding into the reciprocal accumulator (fx_pme).  That balances the two
+// streams of a step; the integer sums are the same either way.
+static bool side_ok(const snb_context* c, int need_energy, int need_virial, const void* d_frc)
+{
+    return c->overlap && c->pme_stream && !need_energy && !need_virial && !d_frc;
+}
+
+// Points the context at pme_stream after the step's inputs are final (the
+// event of the last NL update, or one recorded now); returns the home stream.
+static int side_begin(snb_context* c, hipStream_t* home)
+{
+    *home = c->stream;
+    if (!c->inputs_marked)
+    {
+        SNB_CHECK(hipEventRecord(c->ev_inputs, c->stream));
+        c->inputs_marked = true;  // the reciprocal call of this step waits on the same event
+    }
+    SNB_CHECK(hipStreamWaitEvent(c->pme_stream, c->ev_inputs, 0));
+    c->stream = c->pme_stream;
+    return SNB_OK;
+}
+
+static int side_end(snb_context* c, hipStream_t home, int rc)
+{
+    c->stream = home;
+    if (!rc && hipEventRecord(c->ev_pme, c->pme_stream) != hipSuccess) rc = SNB_ERR_HIP;
+    if (!rc) c->pme_pending = true;
+    return rc;
 }
 
 void snb_prof_begin(snb_context* c, int g)
@@ -477,6 +514,7 @@ int snb_context_destroy(snb_context* c)
     free_bond(c->bond);
     free_langevin(c->langevin);
     free_settle(c->settle);
+    snb_f64_free(c);
     dfree(c->vir_atom);
     dfree(c->satom);
     dfree(c->stype);
@@ -1071,7 +1109,14 @@ int snb_nb14_force(snb_context* c, const snb_vector* d_crd, const float* d_charg
     if (need_atom_energy &&
         (rc = snb_launch_fill_i64(c->stream, c->e_atom + 3 * (size_t)c->n, c->n)))
         return rc;
-    if ((rc = snb_launch_nb14(c, d_crd, d_charge, box, need_atom_energy, need_virial))) return rc;
+    if (side_ok(c, need_atom_energy, need_virial, d_frc))
+    {
+        hipStream_t home;
+        if ((rc = side_begin(c, &home))) return rc;
+        return side_end(c, home, snb_launch_nb14(c, d_crd, d_charge, box, 0, 0, c->fx_pme));
+    }
+    if ((rc = snb_launch_nb14(c, d_crd, d_charge, box, need_atom_energy, need_virial, c->fx)))
+        return rc;
     if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
     if (need_atom_energy)
     {
@@ -1149,7 +1194,14 @@ int snb_bond_force(snb_context* c, const snb_vector* d_crd, snb_ltmatrix3 cell,
     if (need_atom_energy &&
         (rc = snb_launch_fill_i64(c->stream, c->e_atom + 4 * (size_t)c->n, c->n)))
         return rc;
-    if ((rc = snb_launch_bond(c, d_crd, snb_box(cell, rcell), need_atom_energy, need_virial)))
+    if (side_ok(c, need_atom_energy, need_virial, d_frc))
+    {
+        hipStream_t home;
+        if ((rc = side_begin(c, &home))) return rc;
+        return side_end(c, home, snb_launch_bond(c, d_crd, snb_box(cell, rcell), 0, 0, c->fx_pme));
+    }
+    if ((rc = snb_launch_bond(c, d_crd, snb_box(cell, rcell), need_atom_energy, need_virial,
+                              c->fx)))
         return rc;
     if (d_frc && (rc = snb_launch_flush_forces(c, d_frc))) return rc;
     if (need_atom_energy)
@@ -1278,6 +1330,7 @@ int snb_settle_initial(snb_context* c, int triangle_numbers, const int* h_atom_A
         if (!(h_rAB[t] > 0.0f) || !(h_rAC[t] > 0.0f) || !(h_rBC[t] > 0.0f)) return SNB_ERR_ARGUMENT;
     }
     DeviceGuard g(c->device);
+    snb_f64_free(c);  // the FP64 step's copy of the frames is made again on next use
     SnbSettle& S = c->settle;
     free_settle(S);
     if (triangle_numbers == 0) return SNB_OK;  // not initialised: Do_SETTLE is a no-op
@@ -1294,11 +1347,14 @@ int snb_settle_initial(snb_context* c, int triangle_numbers, const int* h_atom_A
     }
     std::vector<int4> tri(m);
     std::vector<float> geo(5 * m);
+    S.geo_d.assign(5 * m, 0.0);
     for (size_t t = 0; t < m; t++)
     {
         tri[t] = make_int4(h_atom_A[t], h_atom_B[t], h_atom_C[t], 0);
-        settle_frame(h_rAB[t], h_rAC[t], h_rBC[t], h_mass[h_atom_A[t]], h_mass[h_atom_B[t]],
-                     h_mass[h_atom_C[t]], geo.data() + 5 * t);
+        settle_frame<float>(h_rAB[t], h_rAC[t], h_rBC[t], h_mass[h_atom_A[t]],
+                            h_mass[h_atom_B[t]], h_mass[h_atom_C[t]], geo.data() + 5 * t);
+        settle_frame<double>(h_rAB[t], h_rAC[t], h_rBC[t], h_mass[h_atom_A[t]],
+                             h_mass[h_atom_B[t]], h_mass[h_atom_C[t]], S.geo_d.data() + 5 * t);
     }
     SNB_CHECK(hipMemcpy(S.tri, tri.data(), sizeof(int4) * m, hipMemcpyHostToDevice));
     SNB_CHECK(hipMemcpy(S.geo, geo.data(), sizeof(float) * 5 * m, hipMemcpyHostToDevice));
@@ -1438,6 +1494,7 @@ int snb_pme_initial(snb_context* c, snb_ltmatrix3 cell, snb_ltmatrix3 rcell,
     if (no_direct_interaction_virtual_atom_numbers > 0) return SNB_ERR_UNSUPPORTED;
     if (grid_spacing <= 0.0f) grid_spacing = 1.0f;
     DeviceGuard g(c->device);
+    snb_f64_free(c);  // FP64 grids follow the PME grid
     SnbPme& P = c->pme;
     free_pme(P);
     P.cutoff = cutoff;
@@ -1516,6 +1573,14 @@ int snb_pme_excluded_force(snb_context* c, int atom_numbers, const int* d_atom_l
     DeviceGuard g(c->device);
     SnbBox box = snb_box(cell, rcell);
     int rc;
+    if (side_ok(c, need_energy, need_virial, d_frc))
+    {
+        hipStream_t home;
+        if ((rc = side_begin(c, &home))) return rc;
+        return side_end(c, home,
+                        snb_launch_excl(c, d_crd, d_charge, box, c->pme.beta, es, el, en, 0, 0,
+                                        c->fx_pme));
+    }
     if (need_energy && (rc = snb_launch_fill_i64(c->stream, c->e_atom + 2 * (size_t)c->n, c->n)))
         return rc;
     rc = snb_launch_excl(c, d_crd, d_charge, box, c->pme.beta, es, el, en, need_energy,

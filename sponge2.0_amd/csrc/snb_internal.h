@@ -166,6 +166,7 @@ struct SnbSettle
     snb_vector* last_ca = nullptr;  // previous step's C - A
     int64_t* vir6 = nullptr;        // summed constraint virial, fixed point
     int* free_atoms = nullptr;      // atoms in no triangle (the fused iteration's list)
+    std::vector<double> geo_d;      // the canonical frames in double (FP64 step)
     int nfree = 0;
     bool disjoint = false;          // no atom in two triangles (needed by the fused iteration)
 };
@@ -268,7 +269,11 @@ struct snb_context
     SnbSettle settle;
     SnbPme pme;
     SnbProf prof;
+    struct SnbF64* f64 = nullptr;  // FP64 evaluation state (snb_f64.hip), made on first use
 };
+
+// frees the FP64 evaluation state (snb_f64.hip)
+void snb_f64_free(snb_context* ctx);
 
 // Makes `stream` wait for the reciprocal pipeline launched last (no-op when
 // none is pending).
@@ -341,9 +346,9 @@ int snb_launch_excl(snb_context* ctx, const snb_vector* crd, const float* charge
                     const int* ex_start, const int* ex_list, const int* ex_num,
                     int need_energy, int need_virial, int64_t* fx);
 int snb_launch_nb14(snb_context* ctx, const snb_vector* crd, const float* charge,
-                    const SnbBox& box, int need_energy, int need_virial);
+                    const SnbBox& box, int need_energy, int need_virial, int64_t* fx);
 int snb_launch_bond(snb_context* ctx, const snb_vector* crd, const SnbBox& box, int need_energy,
-                    int need_virial);
+                    int need_virial, int64_t* fx);
 int snb_launch_leap_frog(snb_context* ctx, const float* inv_mass, const snb_vector* frc,
                          snb_vector* vel, snb_vector* crd, float dt, float vmax);
 int snb_launch_langevin(snb_context* ctx, const snb_vector* frc, snb_vector* vel,

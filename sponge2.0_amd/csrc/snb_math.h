@@ -10,6 +10,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "snb_erfcx_table.h"
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define SNB_HD __host__ __device__ __forceinline__
@@ -118,23 +120,25 @@ SNB_HD double snb_exp_d(double y)
     return p * snb_f64_from_bits((uint64_t)(ni + 1023) << 52);
 }
 
-// erf(x) and h(x) = erf(x) - (2/sqrt(pi)) x e^{-x^2} for 0 <= x <= 4.9 from
-// the all-positive series erf(x) = (2/sqrt(pi)) x e^{-x^2}
-// sum_{n>=0} (2x^2)^n / (1*3*...*(2n+1)) (Abramowitz & Stegun 7.1.6): the
-// n >= 1 part is h, so neither quantity suffers cancellation.
-SNB_HD void snb_erf_parts_d(double x, double* erf_out, double* h_out)
+// erfc(x) and g(x) = (2/sqrt(pi)) x e^{-x^2} for x >= 0, at a fixed cost:
+// erfc(x) = e^{-x^2} erfcx(x), erfcx (the scaled complementary error
+// function) from piecewise Chebyshev-interpolation polynomials of degree 12
+// on half-unit intervals of [0, 6) (snb_erfcx_table.h, relative error below
+// 1e-15), evaluated by Horner's rule in t = (x - centre) / 0.25; erfc(x) = 0
+// from x = 6 on (erfc(6) = 2e-17).  erf = 1 - erfc and the exclusion
+// correction's erf(x) - g(x) follow from these two without a series.
+SNB_HD double snb_erfc_parts_d(double x, double* g_out)
 {
-    double g = 1.1283791670955126 * x * snb_exp_d(-(x * x));
-    double a = 2.0 * x * x;
-    double t = 1.0, s = 0.0;
-    for (int n = 1; n < 80; n++)
-    {
-        t = t * (a / (double)(2 * n + 1));
-        s = s + t;
-        if (t <= 1e-17 * s) break;
-    }
-    *h_out = g * s;
-    *erf_out = g * s + g;
+    const double e = snb_exp_d(-(x * x));
+    *g_out = 1.1283791670955126 * x * e;
+    if (!(x < 6.0)) return 0.0;
+    int k = (int)(x * 2.0);
+    k = k < 0 ? 0 : k;
+    const double t = (x - ((double)k + 0.5) * 0.5) * 4.0;
+    const double* c = snb_erfcx_coef[k];
+    double p = c[SNB_ERFCX_DEG];
+    for (int j = SNB_ERFCX_DEG - 1; j >= 0; j--) p = __builtin_fma(p, t, c[j]);
+    return e * p;
 }
 
 // The SNB_COUL_N pieces of h (4 floats each, layout above) for beta and
@@ -160,9 +164,8 @@ SNB_HD float snb_coul_table(float beta, float cutoff, float* tab)
     for (int k = 0; k <= SNB_COUL_N; k++)
     {
         const double x = (double)k * dx;
-        double e, g;
-        snb_erf_parts_d(x, &e, &g);
-        const double h1 = 1.0 - g;
+        double g;
+        const double h1 = snb_erfc_parts_d(x, &g) + g;
         const double d1 = ((-four_over_sqrt_pi * x) * x) * snb_exp_d(-(x * x)) * dx;
         if (k > 0)
         {

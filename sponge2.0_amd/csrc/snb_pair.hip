@@ -112,9 +112,9 @@ __global__ void __launch_bounds__(256) k_lj_direct(
                 double ri6 = (ri2 * ri2) * ri2;
                 elj += snb_to_fixed_d((((double)A * (1.0 / 12.0)) * ri6 - (double)B * (1.0 / 6.0)) *
                                       ri6);
-                double erfd, hd;
-                snb_erf_parts_d((double)beta * rd, &erfd, &hd);
-                ec += snb_to_fixed_d((((double)ai.w * (double)aj.w) * (1.0 - erfd)) * rid);
+                double gd;
+                const double erfcd = snb_erfc_parts_d((double)beta * rd, &gd);
+                ec += snb_to_fixed_d((((double)ai.w * (double)aj.w) * erfcd) * rid);
             }
             if (NEED_VIRIAL)
             {
@@ -228,8 +228,9 @@ __global__ void __launch_bounds__(256) k_excl(int n, const snb_vector* __restric
         double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
         double rd = __builtin_sqrt(r2d);
         double rid = 1.0 / rd;
-        double erfd, hd;
-        snb_erf_parts_d((double)beta * rd, &erfd, &hd);
+        double gd;
+        const double erfd = 1.0 - snb_erfc_parts_d((double)beta * rd, &gd);
+        const double hd = erfd - gd;
         double qq = (double)ai.w * (double)aj.w;
         double fa = (qq * hd) * ((rid * rid) * rid);
         double fxv = fa * ddx, fyv = fa * ddy, fzv = fa * ddz;
@@ -379,7 +380,7 @@ __global__ void __launch_bounds__(256) k_nb14(int m, int n, const int* __restric
 }
 
 int snb_launch_nb14(snb_context* ctx, const snb_vector* crd, const float* charge,
-                    const SnbBox& box, int need_energy, int need_virial)
+                    const SnbBox& box, int need_energy, int need_virial, int64_t* fx)
 {
     const SnbNb14& M = ctx->nb14;
     const int m = M.count, n = ctx->n;
@@ -387,7 +388,7 @@ int snb_launch_nb14(snb_context* ctx, const snb_vector* crd, const float* charge
     dim3 g((m + 255) / 256);
 #define SNB_NB14_LAUNCH(E, V)                                                                \
     hipLaunchKernelGGL((k_nb14<E, V>), g, dim3(256), 0, ctx->stream, m, n, M.a, M.b, M.A, M.B, \
-                       M.cf, crd, charge, box, ctx->fx, ctx->e_atom + 3 * (size_t)n, M.pair_e, \
+                       M.cf, crd, charge, box, fx, ctx->e_atom + 3 * (size_t)n, M.pair_e,      \
                        ctx->vir_atom)
     if (need_energy && need_virial)
         SNB_NB14_LAUNCH(true, true);
@@ -458,7 +459,7 @@ __global__ void __launch_bounds__(256) k_bond(int m, int n, const int* __restric
 }
 
 int snb_launch_bond(snb_context* ctx, const snb_vector* crd, const SnbBox& box, int need_energy,
-                    int need_virial)
+                    int need_virial, int64_t* fx)
 {
     const SnbBond& M = ctx->bond;
     const int m = M.count, n = ctx->n;
@@ -466,7 +467,7 @@ int snb_launch_bond(snb_context* ctx, const snb_vector* crd, const SnbBox& box, 
     dim3 g((m + 255) / 256);
 #define SNB_BOND_LAUNCH(E, V)                                                               \
     hipLaunchKernelGGL((k_bond<E, V>), g, dim3(256), 0, ctx->stream, m, n, M.a, M.b, M.k,    \
-                       M.r0, crd, box, ctx->fx, ctx->e_atom + 4 * (size_t)n, M.pair_e,       \
+                       M.r0, crd, box, fx, ctx->e_atom + 4 * (size_t)n, M.pair_e,            \
                        ctx->vir_atom)
     if (need_energy && need_virial)
         SNB_BOND_LAUNCH(true, true);
@@ -597,9 +598,9 @@ __device__ __forceinline__ void win_pair(WinRowSums& S, float dx, float dy, floa
             double ri6 = (ri2 * ri2) * ri2;
             int64_t pl = snb_to_fixed_d((((double)A * (1.0 / 12.0)) * ri6 -
                                          (double)B * (1.0 / 6.0)) * ri6);
-            double erfd, hd;
-            snb_erf_parts_d((double)beta * rd, &erfd, &hd);
-            int64_t pc = snb_to_fixed_d((((double)qi * (double)qj) * (1.0 - erfd)) * rid);
+            double gd;
+            const double erfcd = snb_erfc_parts_d((double)beta * rd, &gd);
+            int64_t pc = snb_to_fixed_d((((double)qi * (double)qj) * erfcd) * rid);
             if (id_j >= 0)
             {
                 atomic_add_i64(e_atom + id_j, pl);

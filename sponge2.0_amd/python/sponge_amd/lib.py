@@ -19,7 +19,8 @@ FIXED_SCALE = 2.0 ** 32
 SNB_OK = 0
 STATUS_NAMES = {1: "invalid argument", 2: "HIP runtime error", 3: "box too small",
                 4: "unsupported configuration", 5: "module not initialised",
-                6: "device allocation failed", 7: "device status flag set"}
+                6: "device allocation failed", 7: "device status flag set",
+                8: "above the allocation-free entry point's capacity"}
 FLAG_NL_OVERFLOW, FLAG_CELL_OVERFLOW, FLAG_NONFINITE_CRD = 1, 2, 4
 
 
@@ -110,6 +111,9 @@ _SIGS = {
     "snb_copy_pme_grid": ([_P, _I, _P], _I),
     "snb_fft_r2c": ([_P, _P, _P], _I),
     "snb_fft_c2r": ([_P, _P, _P], _I),
+    "snb_f64_force": ([_P, _P, _P, LTMatrix3, LTMatrix3, _I, _I, _P, _I, _I, _P, _P], _I),
+    "snb_f64_nve_iterate": ([_P, _P, _P, _P, _P, _P, LTMatrix3, LTMatrix3, ctypes.c_double],
+                            _I),
     "snb_profile_enable": ([_P, _I], _I),
     "snb_profile_read": ([_P, _I, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)], _I),
 }
@@ -635,6 +639,37 @@ class Nonbonded:
         check(self.L.snb_profile_read(self.ctx, group, 1 if reset else 0, ctypes.byref(ms),
                                       ctypes.byref(cnt)), "snb_profile_read")
         return ms.value, cnt.value
+
+    # -- FP64 reference evaluation ---------------------------------------------
+    F64_TERMS = ("LJ_short", "LJ_long", "PM_direct", "PM_reciprocal", "PM_self",
+                 "PM_correction", "nb14_LJ", "nb14_EE", "bond")
+
+    def f64_force(self, crd64, frc64, step=0, forced_nl=False, need_energy=False,
+                  need_virial=False):
+        """The whole path in double on float64 device tensors crd64 / frc64
+        [n, 3] (frc64 accumulates).  Returns (terms dict or None, virial[6] or
+        None); both synchronise."""
+        terms = (ctypes.c_double * 9)() if need_energy else None
+        vir = (ctypes.c_double * 6)() if need_virial else None
+        check(self.L.snb_f64_force(self.ctx, _ptr(crd64), _ptr(self.charge), self.cell,
+                                   self.rcell, step, 1 if forced_nl else 0, _ptr(frc64),
+                                   1 if need_energy else 0, 1 if need_virial else 0,
+                                   ctypes.cast(terms, ctypes.c_void_p) if terms else None,
+                                   ctypes.cast(vir, ctypes.c_void_p) if vir else None),
+              "snb_f64_force")
+        out = None
+        if need_energy:
+            out = dict(zip(self.F64_TERMS, terms[:]))
+            out["LJ"] = out["LJ_short"] + out["LJ_long"]
+            out["PM"] = (out["PM_direct"] + out["PM_reciprocal"] + out["PM_self"]
+                         + out["PM_correction"])
+        return out, (np.array(vir[:]) if need_virial else None)
+
+    def f64_nve_iterate(self, mass64, inv_mass64, frc64, vel64, crd64, dt_internal):
+        """Remember, leap-frog and SETTLE in double (after f64_force)."""
+        check(self.L.snb_f64_nve_iterate(self.ctx, _ptr(mass64), _ptr(inv_mass64), _ptr(frc64),
+                                         _ptr(vel64), _ptr(crd64), self.cell, self.rcell,
+                                         float(dt_internal)), "snb_f64_nve_iterate")
 
     def energies(self):
         d = [ctypes.c_double() for _ in range(4)]

@@ -280,17 +280,26 @@ def test_twin_math_error_bounds():
     assert np.max(np.abs(gr * np.sqrt(r2.astype(np.float64)) - 1)) < 3e-7
     # FP64 pieces of the energy / exclusion path
     import ctypes
-    from scipy.special import erf
+    import mpmath
     L = oracle.lib()
-    for xv in np.linspace(0.0, 4.5, 181):
-        e_, h_ = ctypes.c_double(), ctypes.c_double()
-        L.orc_twin_erf_parts(float(xv), ctypes.byref(e_), ctypes.byref(h_))
-        ref_e = erf(xv)
-        ref_h = ref_e - 2 / np.sqrt(np.pi) * xv * np.exp(-xv * xv)
-        assert abs(e_.value - ref_e) <= 2e-15 * ref_e + 1e-300
-        # (the scipy reference itself cancels for small x: absolute bound there)
-        if xv > 0:
-            assert abs(h_.value - ref_h) <= 1e-14 * ref_h + 4e-17
+    mpmath.mp.dps = 30
+    for xv in np.concatenate([np.linspace(0.0, 6.5, 521), [0.5, 1.0, 5.999999]]):
+        c_, g_ = ctypes.c_double(), ctypes.c_double()
+        L.orc_twin_erfc_parts(float(xv), ctypes.byref(c_), ctypes.byref(g_))
+        ref_c = float(mpmath.erfc(mpmath.mpf(float(xv))))
+        ref_g = float(2 / mpmath.sqrt(mpmath.pi) * xv * mpmath.exp(-mpmath.mpf(float(xv)) ** 2))
+        if xv < 6.0:
+            # the rounding of x^2 alone moves e^{-x^2} by up to x^2 ulps
+            assert abs(c_.value - ref_c) <= (3e-15 + 2.5e-16 * xv * xv) * ref_c, (xv, c_.value)
+        else:
+            assert c_.value == 0.0 and ref_c < 2.2e-17
+        assert abs(g_.value - ref_g) <= (2e-15 + 2.5e-16 * xv * xv) * ref_g + 1e-300
+        # erf - g of the exclusion correction, formed from the two
+        if 0.3 <= xv < 6.0:
+            ref_h = float(mpmath.erf(mpmath.mpf(float(xv)))) - ref_g
+            h = (1.0 - c_.value) - g_.value
+            # 1 - erfc carries an absolute rounding of an ulp of 1
+            assert abs(h - ref_h) <= 2e-15 * ref_h + 4.5e-16, (xv, h, ref_h)
     for yv in np.linspace(-40.0, 0.0, 161):
         assert abs(L.orc_twin_exp_d(float(yv)) - np.exp(yv)) <= 3e-16 * np.exp(yv)
 
