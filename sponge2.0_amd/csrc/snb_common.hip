@@ -243,8 +243,22 @@ __global__ void k_charge_sums(int n, const float* __restrict__ q, int64_t* __res
         s2 += __shfl_xor(s2, off);
         s1 += __shfl_xor(s1, off);
     }
+    // one add per workgroup and word
+    __shared__ int64_t part[2][16];
+    const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     if ((threadIdx.x & 63) == 0)
     {
+        part[0][wave] = s2;
+        part[1][wave] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (int w = 1; w < nwave; w++)
+        {
+            s2 += part[0][w];
+            s1 += part[1][w];
+        }
         atomicAdd((unsigned long long*)(dst + SNB_S_Q2), (unsigned long long)s2);
         atomicAdd((unsigned long long*)(dst + SNB_S_Q), (unsigned long long)s1);
     }

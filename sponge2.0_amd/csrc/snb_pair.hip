@@ -509,9 +509,12 @@ __global__ void k_pack_sorted(int n, const int* __restrict__ order,
 // k_stage_gather sums each atom's stage entries.  Windows larger than the LDS
 // capacity fall back to integer atomics on the global accumulators.
 // Per-atom energies and virials go, as in the reference, to the lower atom id
-// of the pair: row-owner sums when the owner is the lower id, integer atomics
-// on the id-ordered arrays otherwise (energy/virial steps only); with
-// ref_attr == 0 every pair's terms stay with the row owner.
+// of the pair: row-owner sums when the owner is the lower id, otherwise
+// integer atomics on the j atom's entry of the sorted-order arrays (the j
+// atoms of one row lie in a few neighbouring cells, so a wavefront's adds
+// fall on few cache lines); k_stage_gather moves the sorted-order sums to the
+// id-ordered arrays.  With ref_attr == 0 every pair's terms stay with the row
+// owner.
 
 // Periodic displacement.  ORTHO (diagonal cell) drops the off-diagonal terms
 // and takes k = rint(s) for floor(s + 0.5).  For every pair that can reach
@@ -555,7 +558,8 @@ __device__ __forceinline__ void win_pair(WinRowSums& S, float dx, float dy, floa
                                          int slot, int sj, bool ov, int wcap, long long* acc,
                                          int n, const int* __restrict__ order,
                                          int64_t* __restrict__ fx, int ref_attr, int id_i,
-                                         int64_t* __restrict__ e_atom, int64_t* __restrict__ vir)
+                                         int64_t* __restrict__ e_sorted,
+                                         int64_t* __restrict__ v_sorted)
 {
     const float fa2 = pair_force_factor(r2, A2, B2, qi2 * qj, beta, cscale, ctab);
     const float fx2 = fa2 * dx, fy2 = fa2 * dy, fz2 = fa2 * dz;
@@ -579,12 +583,8 @@ __device__ __forceinline__ void win_pair(WinRowSums& S, float dx, float dy, floa
     }
     if (NEED_ENERGY || NEED_VIRIAL)
     {
-        int id_j = -1;
-        if (ref_attr)
-        {
-            id_j = order[sj];
-            if (id_j > id_i) id_j = -1;
-        }
+        // the lower id takes the pair's terms
+        const bool to_j = ref_attr && order[sj] < id_i;
         if (NEED_ENERGY)
         {
             // unscaled parameters (2^-32 scaling is exact)
@@ -601,10 +601,10 @@ __device__ __forceinline__ void win_pair(WinRowSums& S, float dx, float dy, floa
             double gd;
             const double erfcd = snb_erfc_parts_d((double)beta * rd, &gd);
             int64_t pc = snb_to_fixed_d((((double)qi * (double)qj) * erfcd) * rid);
-            if (id_j >= 0)
+            if (to_j)
             {
-                atomic_add_i64(e_atom + id_j, pl);
-                atomic_add_i64(e_atom + n + id_j, pc);
+                atomic_add_i64(e_sorted + sj, pl);
+                atomic_add_i64(e_sorted + n + sj, pc);
             }
             else
             {
@@ -622,14 +622,14 @@ __device__ __forceinline__ void win_pair(WinRowSums& S, float dx, float dy, floa
             int64_t p3 = snb_to_fixed(-(fxv * dz + fzv * dx));
             int64_t p4 = snb_to_fixed(-(fyv * dz + fzv * dy));
             int64_t p5 = snb_to_fixed(-(fzv * dz));
-            if (id_j >= 0)
+            if (to_j)
             {
-                atomic_add_i64(vir + id_j, p0);
-                atomic_add_i64(vir + (size_t)n + id_j, p1);
-                atomic_add_i64(vir + (size_t)2 * n + id_j, p2);
-                atomic_add_i64(vir + (size_t)3 * n + id_j, p3);
-                atomic_add_i64(vir + (size_t)4 * n + id_j, p4);
-                atomic_add_i64(vir + (size_t)5 * n + id_j, p5);
+                atomic_add_i64(v_sorted + sj, p0);
+                atomic_add_i64(v_sorted + (size_t)n + sj, p1);
+                atomic_add_i64(v_sorted + (size_t)2 * n + sj, p2);
+                atomic_add_i64(v_sorted + (size_t)3 * n + sj, p3);
+                atomic_add_i64(v_sorted + (size_t)4 * n + sj, p4);
+                atomic_add_i64(v_sorted + (size_t)5 * n + sj, p5);
             }
             else
             {
@@ -667,8 +667,7 @@ __global__ void __launch_bounds__(1024) k_pair_win(
     const float4* __restrict__ coul_tab, float cscale, SnbBox box, float rc2, float beta,
     const int* __restrict__ stage_base, long long* __restrict__ stage, size_t stage_cap,
     int64_t* __restrict__ fx, int64_t* __restrict__ e_sorted, int64_t* __restrict__ v_sorted,
-    int ref_attr, int64_t* __restrict__ e_atom, int64_t* __restrict__ vir,
-    const int* __restrict__ perm, int jdirect)
+    int ref_attr, const int* __restrict__ perm, int jdirect)
 {
     extern __shared__ long long smem[];
     float4* ctab = (float4*)smem;                   // [SNB_COUL_N]
@@ -749,7 +748,11 @@ __global__ void __launch_bounds__(1024) k_pair_win(
             // step k uses atoms whose records were fetched during step k - 1,
             // fetches the records of step k + 1 (slot map, then position and
             // type) and the list entries of step k + 2.  32-bit byte offsets:
-            // the loads take the scalar-base + vector-offset form.
+            // the loads take the scalar-base + vector-offset form.  Energy /
+            // virial variants fetch each step's records when it starts: their
+            // FP64 energies and row sums leave no registers for a second set
+            // of records, and a pipelined version spills.
+            constexpr bool PIPE = !(NEED_ENERGY || NEED_VIRIAL);
             auto entry = [&](int k) -> int {
                 return k < cnt ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(k, split, max_nb))
                                : -1;
@@ -758,22 +761,47 @@ __global__ void __launch_bounds__(1024) k_pair_win(
                 return sl >= 0 ? (unsigned)win_slot_sorted(sl, ov, slot_s, rfirst, rpref, W.nrange)
                                : (unsigned)si;
             };
-            int sla = entry(lane), slb = entry(lane + 64);
-            unsigned sja = sorted_of(sla), sjb = sorted_of(slb);
-            float4 aja = ld32(satom, sja), ajb = ld32(satom, sjb);
-            int tja = ld32(stype, sja), tjb = ld32(stype, sjb);
-            int nxa = entry(lane + 128), nxb = entry(lane + 192);
+            int sla = 0, slb = 0, tja = 0, tjb = 0, nxa = 0, nxb = 0;
+            unsigned sja = 0, sjb = 0;
+            float4 aja = {0, 0, 0, 0}, ajb = {0, 0, 0, 0};
+            auto records = [&](int k) {
+                sla = entry(k + lane);
+                slb = entry(k + 64 + lane);
+                sja = sorted_of(sla);
+                sjb = sorted_of(slb);
+                aja = ld32(satom, sja);
+                ajb = ld32(satom, sjb);
+                tja = ld32(stype, sja);
+                tjb = ld32(stype, sjb);
+            };
+            if (PIPE)
+            {
+                records(0);
+                nxa = entry(lane + 128);
+                nxb = entry(lane + 192);
+            }
             for (int k0 = 0; k0 < cnt; k0 += 128)
             {
+                if (!PIPE) records(k0);
                 const float2 ABa = ld32(ab2, trow + (unsigned)tja);
                 const float2 ABb = ld32(ab2, trow + (unsigned)tjb);
                 // records of the next step
-                const int sla_n = nxa, slb_n = nxb;
-                const unsigned sja_n = sorted_of(sla_n), sjb_n = sorted_of(slb_n);
-                const float4 aja_n = ld32(satom, sja_n), ajb_n = ld32(satom, sjb_n);
-                const int tja_n = ld32(stype, sja_n), tjb_n = ld32(stype, sjb_n);
-                nxa = entry(k0 + 256 + lane);
-                nxb = entry(k0 + 320 + lane);
+                int sla_n = 0, slb_n = 0, tja_n = 0, tjb_n = 0;
+                unsigned sja_n = 0, sjb_n = 0;
+                float4 aja_n = {0, 0, 0, 0}, ajb_n = {0, 0, 0, 0};
+                if (PIPE)
+                {
+                    sla_n = nxa;
+                    slb_n = nxb;
+                    sja_n = sorted_of(sla_n);
+                    sjb_n = sorted_of(slb_n);
+                    aja_n = ld32(satom, sja_n);
+                    ajb_n = ld32(satom, sjb_n);
+                    tja_n = ld32(stype, sja_n);
+                    tjb_n = ld32(stype, sjb_n);
+                    nxa = entry(k0 + 256 + lane);
+                    nxb = entry(k0 + 320 + lane);
+                }
                 float dxa = aja.x - ai.x, dya = aja.y - ai.y, dza = aja.z - ai.z;
                 float dxb = ajb.x - ai.x, dyb = ajb.y - ai.y, dzb = ajb.z - ai.z;
                 win_disp<ORTHO>(dxa, dya, dza, box);
@@ -789,20 +817,23 @@ __global__ void __launch_bounds__(1024) k_pair_win(
                     win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxa, dya, dza, r2a, ABa.x, ABa.y, qi2,
                                                        aja.w, beta, cscale, ctab, sla, sja, ov,
                                                        W.wcap, acc, n, order, fx, ref_attr, id_i,
-                                                       e_atom, vir);
+                                                       e_sorted, v_sorted);
                 if (slb >= 0 && r2b < rc2)
                     win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxb, dyb, dzb, r2b, ABb.x, ABb.y, qi2,
                                                        ajb.w, beta, cscale, ctab, slb, sjb, ov,
                                                        W.wcap, acc, n, order, fx, ref_attr, id_i,
-                                                       e_atom, vir);
-                sla = sla_n;
-                slb = slb_n;
-                sja = sja_n;
-                sjb = sjb_n;
-                aja = aja_n;
-                ajb = ajb_n;
-                tja = tja_n;
-                tjb = tjb_n;
+                                                       e_sorted, v_sorted);
+                if (PIPE)
+                {
+                    sla = sla_n;
+                    slb = slb_n;
+                    sja = sja_n;
+                    sjb = sjb_n;
+                    aja = aja_n;
+                    ajb = ajb_n;
+                    tja = tja_n;
+                    tjb = tjb_n;
+                }
             }
             int64_t sx = wave_sum(S.sx), sy = wave_sum(S.sy), sz = wave_sum(S.sz);
             int64_t elj = 0, ec = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
@@ -837,19 +868,31 @@ __global__ void __launch_bounds__(1024) k_pair_win(
                     atomic_add_i64(fx + n + idi, sy);
                     atomic_add_i64(fx + 2 * n + idi, sz);
                 }
+                // other rows' lower-id terms may land on si as well
+                // (ref_attr: the arrays were cleared before the launch)
                 if (NEED_ENERGY)
                 {
-                    e_sorted[si] = elj;
-                    e_sorted[(size_t)n + si] = ec;
+                    if (ref_attr)
+                    {
+                        atomic_add_i64(e_sorted + si, elj);
+                        atomic_add_i64(e_sorted + (size_t)n + si, ec);
+                    }
+                    else
+                    {
+                        e_sorted[si] = elj;
+                        e_sorted[(size_t)n + si] = ec;
+                    }
                 }
                 if (NEED_VIRIAL)
                 {
-                    v_sorted[si] = v0;
-                    v_sorted[(size_t)n + si] = v1;
-                    v_sorted[(size_t)2 * n + si] = v2;
-                    v_sorted[(size_t)3 * n + si] = v3;
-                    v_sorted[(size_t)4 * n + si] = v4;
-                    v_sorted[(size_t)5 * n + si] = v5;
+                    const int64_t vv[6] = {v0, v1, v2, v3, v4, v5};
+                    for (int c = 0; c < 6; c++)
+                    {
+                        if (ref_attr)
+                            atomic_add_i64(v_sorted + (size_t)c * n + si, vv[c]);
+                        else
+                            v_sorted[(size_t)c * n + si] = vv[c];
+                    }
                 }
             }
         }
@@ -960,6 +1003,12 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
     if (jdirect < 0)
         jdirect = (n < 65536 || (size_t)n * 432 > ((size_t)128 << 20)) ? 1 : 0;
     if (need_energy || need_virial) jdirect = 0;
+    const int ref_attr = ctx->lj.owner_attribution ? 0 : 1;
+    int rc;
+    if (ref_attr && need_energy && (rc = snb_launch_fill_i64(s, ctx->e_sorted, 2 * (size_t)n)))
+        return rc;
+    if (ref_attr && need_virial && (rc = snb_launch_fill_i64(s, ctx->v_sorted, 6 * (size_t)n)))
+        return rc;
     snb_prof_begin(ctx, SNB_PROF_PAIR);
 #define SNB_WIN_LAUNCH(E, V, O)                                                                 \
     hipLaunchKernelGGL((k_pair_win<E, V, O>), dim3(W.nblock), dim3(1024), lds, s, n, W,         \
@@ -968,8 +1017,7 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
                        ctx->stype, L.order, ctx->lj.ab2, ctx->lj.ntypes, ctx->lj.coul,         \
                        ctx->lj.coul_scale, box, rc2, beta,                                     \
                        L.stage_base, L.stage, (size_t)n * L.invmax, ctx->fx, ctx->e_sorted,   \
-                       ctx->v_sorted, ctx->lj.owner_attribution ? 0 : 1, ctx->e_atom,          \
-                       ctx->vir_atom, L.block_perm, jdirect)
+                       ctx->v_sorted, ref_attr, L.block_perm, jdirect)
 #define SNB_WIN_LAUNCH_EV(O)                 \
     if (need_energy && need_virial)          \
         SNB_WIN_LAUNCH(true, true, O);       \

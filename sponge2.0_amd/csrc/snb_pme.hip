@@ -472,14 +472,17 @@ __global__ void k_pme_finalize(const double* __restrict__ partials, int nblocks,
                                double* __restrict__ dscal, const int64_t* __restrict__ scal,
                                double beta, double nfactor, int want_self)
 {
-    int c = threadIdx.x;
+    // wavefront c sums component c of the per-workgroup partials in a fixed
+    // order (lane strides, then a butterfly); wavefront 7 the self term
+    const int lane = threadIdx.x & 63, c = threadIdx.x >> 6;
     if (c < 7 && nblocks > 0)
     {
         double s = 0;
-        for (int k = 0; k < nblocks; k++) s += partials[(size_t)k * 8 + c];
-        dscal[c] = s;
+        for (int k = lane; k < nblocks; k += 64) s += partials[(size_t)k * 8 + c];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) dscal[c] = s;
     }
-    if (c == 7 && want_self)
+    if (c == 7 && lane == 0 && want_self)
     {
         double q2 = snb_fixed_to_double(scal[SNB_S_Q2]);
         double q1 = snb_fixed_to_double(scal[SNB_S_Q]);
@@ -1135,7 +1138,7 @@ int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* c
     if (ev)
     {
         double nfactor = -0.5 * M_PI / ((double)P.beta * P.beta * P.volume);
-        hipLaunchKernelGGL(k_pme_finalize, dim3(1), dim3(64), 0, s, P.partials, P.conv_blocks,
+        hipLaunchKernelGGL(k_pme_finalize, dim3(1), dim3(512), 0, s, P.partials, P.conv_blocks,
                            ctx->dscal, ctx->scal, (double)P.beta, nfactor, need_energy ? 1 : 0);
         SNB_CHECK(hipGetLastError());
     }
@@ -1148,7 +1151,7 @@ int snb_launch_pme_self(snb_context* ctx, const float* charge)
     int rc = snb_launch_charge_sums(ctx, charge);
     if (rc) return rc;
     double nfactor = -0.5 * M_PI / ((double)P.beta * P.beta * P.volume);
-    hipLaunchKernelGGL(k_pme_finalize, dim3(1), dim3(64), 0, ctx->stream, P.partials, 0,
+    hipLaunchKernelGGL(k_pme_finalize, dim3(1), dim3(512), 0, ctx->stream, P.partials, 0,
                        ctx->dscal, ctx->scal, (double)P.beta, nfactor, 1);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
