@@ -139,11 +139,15 @@ int snb_nl_device_list(snb_context* ctx, snb_atom_group** d_nl, int** d_serial_t
  * (int[atom_numbers][max_neighbor_numbers]) of the public half list into
  * caller device buffers (the reference's h_nl readback, neighbor_list.h:42). */
 int snb_nl_copy_rows(snb_context* ctx, int* d_counts, int* d_rows);
-/* Whether snb_nl_update also writes the public list above (default 1, the
- * reference's behaviour).  The library's LJ kernel does not read it: with
- * every axis >= 5 cells it consumes an internal cell-ordered list of 16-bit
- * window slots with the same pair set.  Callers that never read d_nl may
- * switch the export off (no counterpart in the reference). */
+/* Whether snb_nl_update also writes the public list above (no counterpart
+ * in the reference): 1 (default) = yes, the reference's behaviour, with the
+ * entries of a row in an order that can differ between rebuilds (as the
+ * reference's, whose cell binning appends with atomics); 2 = yes, with every
+ * row sorted ascending (deterministic, about 1 ms more per rebuild at 93k
+ * atoms); 0 = no.  The library's LJ kernel does not read it: with every axis
+ * >= 5 cells it consumes an internal cell-ordered list of 16-bit window slots
+ * with the same pair set, and its integer force sums do not depend on the
+ * order of a row.  Callers that never read d_nl may switch the export off. */
 int snb_nl_set_export(snb_context* ctx, int enable);
 /* The internal list in atom ids: d_counts[i] and the row
  * d_rows[i * max_neighbor_numbers ..] of every pair whose internal row owner

@@ -812,7 +812,8 @@ int snb_lj_set_owner_attribution(snb_context* c, int enable)
 int snb_nl_set_export(snb_context* c, int enable)
 {
     if (!c) return SNB_ERR_ARGUMENT;
-    c->nl.export_public = enable != 0;
+    if (enable < 0 || enable > 2) return SNB_ERR_ARGUMENT;
+    c->nl.export_mode = enable;
     return SNB_OK;
 }
 

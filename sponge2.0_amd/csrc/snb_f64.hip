@@ -911,9 +911,9 @@ extern "C" int snb_f64_force(snb_context* c, const double* d_crd, const float* d
     if (!rc && n > 0)
     {
         // the reference-layout list, rebuilt from the float copy
-        if (!c->nl.export_public)
+        if (!c->nl.export_mode)
         {
-            c->nl.export_public = true;
+            c->nl.export_mode = 1;
             forced_nl = 1;
         }
         hipLaunchKernelGGL(k_f64_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, d_crd,

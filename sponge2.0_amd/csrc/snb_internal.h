@@ -72,7 +72,7 @@ struct SnbNl
     snb_vector* old_crd = nullptr;
     // windowed pair path (every axis >= 5 cells)
     bool windowed = false;
-    bool export_public = true;   // also write the reference-layout list at rebuild
+    int export_mode = 1;  // reference-layout list at rebuild: 0 no, 1 yes, 2 yes with sorted rows
     SnbWin win;
     uint16_t* nl16 = nullptr;    // n * max_nb window slots, rows by sorted position
     int* row_count = nullptr;    // n: entries of each row

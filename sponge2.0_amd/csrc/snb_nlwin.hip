@@ -382,13 +382,18 @@ __device__ __forceinline__ int win_slot_to_sorted(const SnbWin& W, const int* fi
 }
 
 // Reference-layout rows, one workgroup per window block: the block's window
-// table and, when the window fits, a slot -> sorted-index map are built once
-// in LDS; then every internal row of the block's cells sends each pair to row
-// min(id) as max(id).  Pairs whose lower id is the row's own atom are
-// appended by the wavefront with one counter add per 64 entries; the others
-// take one atomic append each.  Rows are sorted afterwards.
+// table and, when the window fits, the atom id of every window slot are built
+// once in LDS; then every internal row of the block's cells sends each pair
+// to row min(id) as max(id).  Pairs whose lower id is the row's own atom are
+// appended by the wavefront with one counter add per 64 entries.  The others
+// go to rows of other window atoms: a first pass counts them per window slot
+// in LDS, each slot reserves one range of its row with a single counter add,
+// and a second pass places the entries in that range with LDS counters.  A
+// window beyond the map takes one counter add per such entry instead.  The
+// order within a row follows the appends (rows are sorted afterwards when
+// asked for).
 #define SNB_EXPORT_MAP_CAP 8192
-__global__ void __launch_bounds__(256) k_nl_export_block(
+__global__ void __launch_bounds__(1024) k_nl_export_block(
     const int* __restrict__ need, SnbWin W, const int* __restrict__ start,
     const int* __restrict__ order, const int* __restrict__ row_count,
     const int* __restrict__ row_split, const uint16_t* __restrict__ nl16, int max_nb,
@@ -397,8 +402,10 @@ __global__ void __launch_bounds__(256) k_nl_export_block(
     if (need[0] == 0) return;
     __shared__ int first[SNB_WIN_MAX_RANGES];
     __shared__ int pref[SNB_WIN_MAX_RANGES + 1];
-    __shared__ int smap[SNB_EXPORT_MAP_CAP];
+    __shared__ int wid[SNB_EXPORT_MAP_CAP];  // atom id of each window slot
+    __shared__ int fwd[SNB_EXPORT_MAP_CAP];  // appends to each slot's row, then its next position
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nwave = (int)(blockDim.x >> 6);
     int x0, y0, z0;
     snb_win_block(W, blockIdx.x, x0, y0, z0);
     if (wave == 0) win_table_wave(W, start, x0, y0, z0, lane, first, pref);
@@ -407,53 +414,125 @@ __global__ void __launch_bounds__(256) k_nl_export_block(
     const bool use_map = total <= SNB_EXPORT_MAP_CAP;
     if (use_map)
         for (int slot = tid; slot < total; slot += blockDim.x)
-            smap[slot] = win_slot_to_sorted(W, first, pref, slot);
+        {
+            wid[slot] = order[win_slot_to_sorted(W, first, pref, slot)];
+            fwd[slot] = 0;
+        }
     __syncthreads();
-    // the block's own cells (the last block of an axis may be partial)
+    // the block's own cells (the last block of an axis may be partial); the
+    // cells (x0 + u, y0 + v, z0 .. z0 + zs - 1) are consecutive
     const int xa = W.nx - x0 < W.a ? W.nx - x0 : W.a;
     const int yb = W.ny - y0 < W.b ? W.ny - y0 : W.b;
     const int zs = W.nz - z0 < W.s ? W.nz - z0 : W.s;
-    for (int u = 0; u < xa; u++)
-        for (int v = 0; v < yb; v++)
-        {
-            // the cells (x0 + u, y0 + v, z0 .. z0 + zs - 1) are consecutive
-            const int c0 = ((x0 + u) * W.ny + (y0 + v)) * W.nz + z0;
-            const int sb = start[c0], se = start[c0 + zs];
-            const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-            for (int srow = sb + wave; srow < se; srow += (int)(blockDim.x >> 6))
+    auto each_row = [&](auto&& visit) {
+        for (int u = 0; u < xa; u++)
+            for (int v = 0; v < yb; v++)
             {
-                const int id_i = order[srow];
-                const int cnt = row_count[srow], split = row_split[srow];
-                const uint16_t* row = nl16 + (size_t)srow * max_nb;
-                for (int k0 = 0; k0 < cnt; k0 += 64)
+                const int c0 = ((x0 + u) * W.ny + (y0 + v)) * W.nz + z0;
+                const int sb = start[c0], se = start[c0 + zs];
+                for (int srow = sb + wave; srow < se; srow += nwave) visit(srow);
+            }
+    };
+    const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (use_map)
+    {
+        each_row([&](int srow) {
+            const int id_i = order[srow];
+            const int cnt = row_count[srow], split = row_split[srow];
+            const uint16_t* row = nl16 + (size_t)srow * max_nb;
+            for (int k = lane; k < cnt; k += 64)
+            {
+                const int slot = row[snb_row_entry(k, split, max_nb)];
+                if (wid[slot] < id_i) atomicAdd(&fwd[slot], 1);
+            }
+        });
+        __syncthreads();
+        for (int slot = tid; slot < total; slot += blockDim.x)
+        {
+            const int c = fwd[slot];
+            if (c > 0) fwd[slot] = atomicAdd(&pub_count[wid[slot]], c);
+        }
+        __syncthreads();
+        each_row([&](int srow) {
+            const int id_i = order[srow];
+            const int cnt = row_count[srow], split = row_split[srow];
+            const uint16_t* row = nl16 + (size_t)srow * max_nb;
+            int* own_row = serial + (size_t)id_i * max_nb;
+            for (int k0 = 0; k0 < cnt; k0 += 64)
+            {
+                const int k = k0 + lane;
+                int slot = 0, id_j = -1;
+                if (k < cnt)
                 {
-                    const int k = k0 + lane;
-                    int id_j = -1;
-                    if (k < cnt)
-                    {
-                        const int slot = row[snb_row_entry(k, split, max_nb)];
-                        const int sj =
-                            use_map ? smap[slot] : win_slot_to_sorted(W, first, pref, slot);
-                        id_j = order[sj];
-                    }
-                    const bool own = id_j > id_i;
-                    const uint64_t m_own = __ballot(own);
-                    int base = 0;
-                    if (lane == 0 && m_own) base = atomicAdd(&pub_count[id_i], (int)__popcll(m_own));
-                    base = __shfl(base, 0);
-                    if (own)
-                    {
-                        const int pos = base + (int)__popcll(m_own & lower);
-                        if (pos < max_nb) serial[(size_t)id_i * max_nb + pos] = id_j;
-                    }
-                    else if (id_j >= 0)
-                    {
-                        const int pos = atomicAdd(&pub_count[id_j], 1);
-                        if (pos < max_nb) serial[(size_t)id_j * max_nb + pos] = id_i;
-                    }
+                    slot = row[snb_row_entry(k, split, max_nb)];
+                    id_j = wid[slot];
+                }
+                const bool own = id_j > id_i;
+                const uint64_t m = __ballot(own);
+                int base = 0;
+                if (lane == 0 && m) base = atomicAdd(&pub_count[id_i], (int)__popcll(m));
+                base = __shfl(base, 0);
+                if (own)
+                {
+                    const int pos = base + (int)__popcll(m & lower);
+                    if (pos < max_nb) own_row[pos] = id_j;
+                }
+                else if (id_j >= 0)
+                {
+                    const int pos = atomicAdd(&fwd[slot], 1);
+                    if (pos < max_nb) serial[(size_t)id_j * max_nb + pos] = id_i;
                 }
             }
+        });
+        return;
+    }
+    each_row([&](int srow) {
+        const int id_i = order[srow];
+        const int cnt = row_count[srow], split = row_split[srow];
+        const uint16_t* row = nl16 + (size_t)srow * max_nb;
+        int* own_row = serial + (size_t)id_i * max_nb;
+        for (int k0 = 0; k0 < cnt; k0 += 64)
+        {
+            const int k = k0 + lane;
+            int id_j = -1;
+            if (k < cnt)
+                id_j = order[win_slot_to_sorted(W, first, pref,
+                                                row[snb_row_entry(k, split, max_nb)])];
+            const bool own = id_j > id_i;
+            const uint64_t m = __ballot(own);
+            int base = 0;
+            if (lane == 0 && m) base = atomicAdd(&pub_count[id_i], (int)__popcll(m));
+            base = __shfl(base, 0);
+            if (own)
+            {
+                const int pos = base + (int)__popcll(m & lower);
+                if (pos < max_nb) own_row[pos] = id_j;
+            }
+            else if (id_j >= 0)
+            {
+                const int pos = atomicAdd(&pub_count[id_j], 1);
+                if (pos < max_nb) serial[(size_t)id_j * max_nb + pos] = id_i;
+            }
         }
+    });
+}
+
+// ATOM_GROUP::atom_numbers of every public row (rows in append order).
+__global__ void k_nl_pub_counts(const int* __restrict__ need, int n,
+                                const int* __restrict__ pub_count, int max_nb,
+                                snb_atom_group* __restrict__ nl, int* __restrict__ status)
+{
+    if (need[0] == 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = pub_count[i];
+    if (c > max_nb)
+    {
+        atomicOr(status, SNB_FLAG_NL_OVERFLOW);
+        c = max_nb;
+    }
+    nl[i].atom_numbers = c;
+    nl[i].ghost_numbers = 0;
 }
 
 // Sorts each public row ascending (bitonic, one wavefront per row) and sets
@@ -742,14 +821,17 @@ int snb_launch_nl_window(snb_context* ctx, const SnbBox& box, const int* ex_star
     hipLaunchKernelGGL(k_stage_inverse, dim3(W.nblock), dim3(256), 0, s, L.need, W, L.cell_start,
                        L.stage_base, L.inv_count, L.inv_list, L.invmax, ctx->status);
     SNB_CHECK(hipGetLastError());
-    if (L.export_public)
+    if (L.export_mode)
     {
         hipLaunchKernelGGL(k_fill_need, dim3((n + 255) / 256), dim3(256), 0, s, L.need,
                            L.pub_count, n);
-        hipLaunchKernelGGL(k_nl_export_block, dim3(W.nblock), dim3(256), 0, s, L.need, W,
+        hipLaunchKernelGGL(k_nl_export_block, dim3(W.nblock), dim3(1024), 0, s, L.need, W,
                            L.cell_start, L.order, L.row_count, L.row_split, L.nl16, L.max_nb,
                            L.pub_count, L.serial);
-        if (L.max_nb <= SNB_SORT_WAVE_CAP)
+        if (L.export_mode == 1)
+            hipLaunchKernelGGL(k_nl_pub_counts, dim3((n + 255) / 256), dim3(256), 0, s, L.need,
+                               n, L.pub_count, L.max_nb, L.dnl, ctx->status);
+        else if (L.max_nb <= SNB_SORT_WAVE_CAP)
             hipLaunchKernelGGL(k_nl_pub_finish_wave, dim3((n + 3) / 4), dim3(256), 0, s, L.need,
                                n, L.pub_count, L.max_nb, L.serial, L.dnl, ctx->status);
         else

@@ -596,8 +596,11 @@ class Nonbonded:
         return counts.cpu().numpy()
 
     def set_export(self, enable):
-        """Whether rebuilds also write the reference-layout public list."""
-        check(self.L.snb_nl_set_export(self.ctx, 1 if enable else 0), "snb_nl_set_export")
+        """Whether rebuilds also write the reference-layout public list: 0 / False
+        no, 1 / True yes (entries of a row in append order), 2 yes with every
+        row sorted ascending."""
+        mode = int(enable) if not isinstance(enable, bool) else (1 if enable else 0)
+        check(self.L.snb_nl_set_export(self.ctx, mode), "snb_nl_set_export")
 
     def set_owner_attribution(self, enable):
         """Per-atom pair energy/virial to the internal row owner (True) or to

@@ -286,7 +286,10 @@ def test_reference_cpu_values_on_fixture(gromacs_iter0, tip3p_1011):
 
 
 def test_determinism_two_runs_identical(water_9k):
+    """Forces, energies and (with sorted rows) the public list are the same
+    bits on a second run."""
     nb = _nb(water_9k)
+    nb.set_export(2)
     outs = []
     for _ in range(2):
         nb.clear_fixed_forces()
@@ -428,6 +431,32 @@ def test_window_pair_bit_exact_mixed(mixed_small):
     assert _pairs(counts, rows) == _pairs(oc, orows)
     ic, ir = nb.internal_rows()
     assert _pairs(ic, ir) == _pairs(oc, orows)
+    nb.close()
+
+
+@pytest.mark.parametrize("name", ["water_9k", "mixed_small"])
+def test_export_modes_same_rows(request, name):
+    """The append-order export (default) and the sorted export hold the same
+    set in every row; sorted rows ascend; both equal the oracle's list."""
+    import oracle
+    s = _get(request, name)
+    nb = _nb(s)
+    nb.set_export(1)
+    nb.nl_update(forced=True)
+    c1, r1 = nb.nl_rows()
+    nb.set_export(2)
+    nb.nl_update(forced=True)
+    c2, r2 = nb.nl_rows()
+    assert nb.status() == 0
+    np.testing.assert_array_equal(c1, c2)
+    for i in range(len(c1)):
+        row2 = r2[i, :c2[i]]
+        assert np.all(np.diff(row2) > 0)
+        np.testing.assert_array_equal(np.sort(r1[i, :c1[i]]), row2)
+    oc, orows, _ = oracle.nl_build(s["crd"], s["cell"], s["rcell"], nb.cutoff, 2.0, _excl(s),
+                                   max_nb=nb.max_nb)
+    np.testing.assert_array_equal(c2, oc)
+    assert _pairs(c2, r2) == _pairs(oc, orows)
     nb.close()
 
 
