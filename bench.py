@@ -497,9 +497,11 @@ def main():
 
     if args.workload == "water1m":
         sysd = systems.config_water_1m()
-        # corner cells take the remainder of 215.24 / 6 A (see the capacity
-        # key of the reference's neighbour list): capacity raised from 150
-        nb = lib.Nonbonded(sysd, refresh_interval=args.refresh, max_atom_in_grid_numbers=300)
+        # corner cells take the remainder of 215.24 / 6 A, about 1.9 times a
+        # cell width per axis, and hold about 145 atoms: the capacity key of
+        # the reference's neighbour list is raised from 150 to 256, the most
+        # the windowed list path takes
+        nb = lib.Nonbonded(sysd, refresh_interval=args.refresh, max_atom_in_grid_numbers=256)
     else:
         sysd = systems.config_gpu_93k()
         nb = lib.Nonbonded(sysd, grid=(96, 96, 96), refresh_interval=args.refresh)

@@ -185,6 +185,14 @@ int snb_nl_set_window_shape(snb_context* ctx, int a, int b, int s);
  * (integer sums); SNB_ERR_ARGUMENT for another mode (no counterpart in the
  * reference). */
 int snb_lj_set_direct_j(snb_context* ctx, int mode);
+/* The windowed pair kernel's loop on force-only steps: 0 = a software
+ * pipeline within each list row, 1 = one pipeline per wavefront across the
+ * ends of its rows (faster alone; it holds more registers, which leaves
+ * fewer slots for the reciprocal pipeline's workgroups beside it), -1
+ * (default) = by size (1 from 262,144 atoms).  Results are the same bits
+ * either way; SNB_ERR_ARGUMENT for another mode (no counterpart in the
+ * reference). */
+int snb_lj_set_row_stream(snb_context* ctx, int mode);
 /* Geometry of the windowed pair path (windowed = 0 when not used). */
 int snb_nl_window_info(snb_context* ctx, int* windowed, int* a, int* b, int* s, int* wcap,
                        int* nblock);

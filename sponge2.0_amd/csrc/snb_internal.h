@@ -110,6 +110,9 @@ struct SnbLj
     // per-atom energy/virial of a pair: false = the lower atom id (the
     // reference's row atom), true = the windowed list's row owner (cheaper)
     bool owner_attribution = false;
+    // force-only pair loop: -1 by size, 0 pipelined within each row, 1
+    // pipelined across row ends
+    int row_stream = -1;
     // window forces of force-only steps: -1 by size, 0 through the stage and
     // its per-atom gather, 1 by atomic adds at the end of each block
     int j_direct = -1;

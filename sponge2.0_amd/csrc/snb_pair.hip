@@ -657,7 +657,7 @@ __device__ __forceinline__ int win_slot_sorted(int slot, bool ov, const int* slo
     return rfirst[lo] + (slot - rpref[lo]);
 }
 
-template <bool NEED_ENERGY, bool NEED_VIRIAL, bool ORTHO>
+template <bool NEED_ENERGY, bool NEED_VIRIAL, bool ORTHO, bool STREAM>
 __global__ void __launch_bounds__(1024) k_pair_win(
     int n, SnbWin W, const int* __restrict__ start, const int* __restrict__ sorted_cell,
     const int* __restrict__ row_count, const int* __restrict__ row_split,
@@ -728,170 +728,386 @@ __global__ void __launch_bounds__(1024) k_pair_win(
         for (int t = tid; t < 3 * W.wcap; t += blockDim.x) acc[t] = 0;
     }
     __syncthreads();
-    for (int col = 0; col < W.a * W.b; col++)
+    if constexpr (STREAM)
     {
-        int ua = col / W.b, vb = col - ua * W.b;
-        if (x0 + ua >= W.nx || y0 + vb >= W.ny) continue;
-        int colbase = ((x0 + ua) * W.ny + (y0 + vb)) * W.nz;
-        int z1 = z0 + W.s < W.nz ? z0 + W.s : W.nz;
-        int r0 = start[colbase + z0], r1 = start[colbase + z1];
-        for (int si = r0 + wave; si < r1; si += nwave)
+        // STREAM (force-only steps of large systems): one software pipeline
+        // per wavefront over the 128-entry steps (two entries per lane) of
+        // all its rows, across row ends, so that a row starts with its
+        // records already fetched.  Per
+        // iteration: the LJ parameters of step t (its types arrived during
+        // step t - 1), the records of step t + 1 (slot map, then position and
+        // type; its list entries arrived during step t - 1), the list entries
+        // of step t + 2, the metadata of the wavefront's next row, then the
+        // pairs of step t.  Records and entries alternate between two
+        // register sets (the loop body is written out twice), so no value
+        // is copied while its load is in flight; every load is issued on
+        // every iteration (clamped addresses) so that their count is fixed.
+        const int ncol = W.a * W.b;
+        struct RowM
         {
-            const int cnt = row_count[si], split = row_split[si];
-            const unsigned row0 = (unsigned)si * (unsigned)max_nb;  // n * max_nb < 2^31
-            const float4 ai = satom[si];
-            const float qi2 = ai.w * 4294967296.0f;  // 2^32 fixed-point scale
-            const unsigned trow = (unsigned)(stype[si] * ntypes);
-            const int id_i = (NEED_ENERGY || NEED_VIRIAL) && ref_attr ? order[si] : 0;
-            WinRowSums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            // Software pipeline over 128-entry steps (two entries per lane):
-            // step k uses atoms whose records were fetched during step k - 1,
-            // fetches the records of step k + 1 (slot map, then position and
-            // type) and the list entries of step k + 2.  32-bit byte offsets:
-            // the loads take the scalar-base + vector-offset form.  Energy /
-            // virial variants fetch each step's records when it starts: their
-            // FP64 energies and row sums leave no registers for a second set
-            // of records, and a pipelined version spills.
-            constexpr bool PIPE = !(NEED_ENERGY || NEED_VIRIAL);
-            auto entry = [&](int k) -> int {
-                return k < cnt ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(k, split, max_nb))
-                               : -1;
-            };
-            auto sorted_of = [&](int sl) -> unsigned {
-                return sl >= 0 ? (unsigned)win_slot_sorted(sl, ov, slot_s, rfirst, rpref, W.nrange)
-                               : (unsigned)si;
-            };
-            int sla = 0, slb = 0, tja = 0, tjb = 0, nxa = 0, nxb = 0;
-            unsigned sja = 0, sjb = 0;
-            float4 aja = {0, 0, 0, 0}, ajb = {0, 0, 0, 0};
-            auto records = [&](int k) {
-                sla = entry(k + lane);
-                slb = entry(k + 64 + lane);
-                sja = sorted_of(sla);
-                sjb = sorted_of(slb);
-                aja = ld32(satom, sja);
-                ajb = ld32(satom, sjb);
-                tja = ld32(stype, sja);
-                tjb = ld32(stype, sjb);
-            };
-            if (PIPE)
+            int si, col, cnt, split, type, cell;
+            float4 ai;
+        };
+        struct Rec
+        {
+            int sla, slb, tja, tjb;
+            unsigned sja, sjb;
+            float4 aja, ajb;
+        };
+        int cur_col = 0, cr0 = 0, cr1 = 0, cur_si = 0;
+        auto col_range = [&](int col) {
+            const int ua = col / W.b, vb = col - ua * W.b;
+            cr0 = cr1 = 0;
+            if (x0 + ua >= W.nx || y0 + vb >= W.ny) return;
+            const int colbase = ((x0 + ua) * W.ny + (y0 + vb)) * W.nz;
+            const int z1 = z0 + W.s < W.nz ? z0 + W.s : W.nz;
+            cr0 = start[colbase + z0];
+            cr1 = start[colbase + z1];
+        };
+        // cur_si: this wavefront's next row (-1 after its last)
+        auto seek = [&]() {
+            while (cur_si >= cr1)
             {
-                records(0);
-                nxa = entry(lane + 128);
-                nxb = entry(lane + 192);
-            }
-            for (int k0 = 0; k0 < cnt; k0 += 128)
-            {
-                if (!PIPE) records(k0);
-                const float2 ABa = ld32(ab2, trow + (unsigned)tja);
-                const float2 ABb = ld32(ab2, trow + (unsigned)tjb);
-                // records of the next step
-                int sla_n = 0, slb_n = 0, tja_n = 0, tjb_n = 0;
-                unsigned sja_n = 0, sjb_n = 0;
-                float4 aja_n = {0, 0, 0, 0}, ajb_n = {0, 0, 0, 0};
-                if (PIPE)
+                if (++cur_col >= ncol)
                 {
-                    sla_n = nxa;
-                    slb_n = nxb;
-                    sja_n = sorted_of(sla_n);
-                    sjb_n = sorted_of(slb_n);
-                    aja_n = ld32(satom, sja_n);
-                    ajb_n = ld32(satom, sjb_n);
-                    tja_n = ld32(stype, sja_n);
-                    tjb_n = ld32(stype, sjb_n);
-                    nxa = entry(k0 + 256 + lane);
-                    nxb = entry(k0 + 320 + lane);
+                    cur_si = -1;
+                    return;
                 }
-                float dxa = aja.x - ai.x, dya = aja.y - ai.y, dza = aja.z - ai.z;
-                float dxb = ajb.x - ai.x, dyb = ajb.y - ai.y, dzb = ajb.z - ai.z;
+                col_range(cur_col);
+                cur_si = cr0 + wave;
+            }
+        };
+        col_range(0);
+        cur_si = cr0 + wave;
+        seek();
+        // metadata of row cur_si (of row 0 when there is none: discarded)
+        auto load_row = [&]() -> RowM {
+            const int si = cur_si >= 0 ? cur_si : 0;
+            RowM r;
+            r.si = cur_si;
+            r.col = cur_col;
+            r.cnt = row_count[si];
+            r.split = row_split[si];
+            r.type = stype[si];
+            r.cell = sorted_cell[si];
+            r.ai = satom[si];
+            return r;
+        };
+        // the loaded metadata, in scalar registers (one value per wavefront)
+        auto uniform = [](const RowM& r) -> RowM {
+            RowM u;
+            u.si = __builtin_amdgcn_readfirstlane(r.si);
+            u.col = __builtin_amdgcn_readfirstlane(r.col);
+            u.cnt = __builtin_amdgcn_readfirstlane(r.cnt);
+            u.split = __builtin_amdgcn_readfirstlane(r.split);
+            u.type = __builtin_amdgcn_readfirstlane(r.type);
+            u.cell = __builtin_amdgcn_readfirstlane(r.cell);
+            u.ai.x = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r.ai.x)));
+            u.ai.y = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r.ai.y)));
+            u.ai.z = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r.ai.z)));
+            u.ai.w = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r.ai.w)));
+            return u;
+        };
+        auto entry_of = [&](const RowM& r, int k) -> int {
+            const int si = r.si >= 0 ? r.si : 0;
+            const int kk = k < r.cnt ? k : 0;
+            const int e = (int)ld32(nl16, (unsigned)si * (unsigned)max_nb +
+                                              (unsigned)snb_row_entry(kk, r.split, max_nb));
+            return r.si >= 0 && k < r.cnt ? e : -1;
+        };
+        auto records = [&](Rec& q, int sla, int slb, int si) {
+            si = si >= 0 ? si : 0;
+            q.sla = sla;
+            q.slb = slb;
+            q.sja = sla >= 0 ? (unsigned)win_slot_sorted(sla, ov, slot_s, rfirst, rpref, W.nrange)
+                             : (unsigned)si;
+            q.sjb = slb >= 0 ? (unsigned)win_slot_sorted(slb, ov, slot_s, rfirst, rpref, W.nrange)
+                             : (unsigned)si;
+            q.aja = ld32(satom, q.sja);
+            q.ajb = ld32(satom, q.sjb);
+            q.tja = ld32(stype, q.sja);
+            q.tjb = ld32(stype, q.sjb);
+        };
+        WinRowSums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        // the units of the compute, record and entry stages, the next row
+        RowM cRow = uniform(load_row());
+        if (cur_si >= 0)
+        {
+            cur_si += nwave;
+            seek();
+        }
+        RowM R = load_row();
+        int cK = 0;
+        Rec r0, r1;
+        int e0a = entry_of(cRow, lane), e0b = entry_of(cRow, lane + 64);
+        records(r0, e0a, e0b, cRow.si);
+        RowM nRow = cRow, eRow = cRow;
+        int nK = 0, eK = 0;
+        // the unit after E (R, the metadata of row cur_si, is loaded again
+        // after each call)
+        auto advance = [&]() {
+            if (eRow.si < 0) return;
+            if (eK + 128 < eRow.cnt)
+                eK += 128;
+            else
+            {
+                eRow = uniform(R);
+                eK = 0;
+                if (cur_si >= 0)
+                {
+                    cur_si += nwave;
+                    seek();
+                }
+            }
+        };
+        advance();
+        R = load_row();
+        nRow = eRow;
+        nK = eK;
+        int e1a = entry_of(eRow, eK + lane), e1b = entry_of(eRow, eK + 64 + lane);
+        int e2a = -1, e2b = -1;
+        // one iteration: compute with cur, fill nxt from (ea, eb), load the
+        // entries of the unit after E into (fa, fb)
+        auto step = [&](Rec& cur, Rec& nxt, int ea, int eb, int& fa, int& fb) {
+            const unsigned trow = (unsigned)(cRow.type * ntypes);
+            const float2 ABa = ld32(ab2, trow + (unsigned)cur.tja);
+            const float2 ABb = ld32(ab2, trow + (unsigned)cur.tjb);
+            records(nxt, ea, eb, nRow.si);
+            advance();
+            fa = entry_of(eRow, eK + lane);
+            fb = entry_of(eRow, eK + 64 + lane);
+            R = load_row();
+            if (cRow.si >= 0)
+            {
+                const float4 ai = cRow.ai;
+                const float qi2 = ai.w * 4294967296.0f;  // 2^32 fixed-point scale
+                float dxa = cur.aja.x - ai.x, dya = cur.aja.y - ai.y, dza = cur.aja.z - ai.z;
+                float dxb = cur.ajb.x - ai.x, dyb = cur.ajb.y - ai.y, dzb = cur.ajb.z - ai.z;
                 win_disp<ORTHO>(dxa, dya, dza, box);
                 win_disp<ORTHO>(dxb, dyb, dzb, box);
                 const float r2a = (dxa * dxa + dya * dya) + dza * dza;
                 const float r2b = (dxb * dxb + dyb * dyb) + dzb * dzb;
-                // keep the parameter gathers ahead of the branches (the
-                // compiler would otherwise sink them into the branches, where
-                // their latency is not hidden behind the distance test); an
-                // empty statement that reads the registers, nothing is emitted
+                // keep the parameter gathers ahead of the branches (an empty
+                // statement that reads the registers; nothing is emitted)
                 asm volatile("" ::"v"(ABa.x), "v"(ABa.y), "v"(ABb.x), "v"(ABb.y));
-                if (sla >= 0 && r2a < rc2)
-                    win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxa, dya, dza, r2a, ABa.x, ABa.y, qi2,
-                                                       aja.w, beta, cscale, ctab, sla, sja, ov,
-                                                       W.wcap, acc, n, order, fx, ref_attr, id_i,
-                                                       e_sorted, v_sorted);
-                if (slb >= 0 && r2b < rc2)
-                    win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxb, dyb, dzb, r2b, ABb.x, ABb.y, qi2,
-                                                       ajb.w, beta, cscale, ctab, slb, sjb, ov,
-                                                       W.wcap, acc, n, order, fx, ref_attr, id_i,
-                                                       e_sorted, v_sorted);
+                if (cur.sla >= 0 && r2a < rc2)
+                    win_pair<false, false>(S, dxa, dya, dza, r2a, ABa.x, ABa.y, qi2, cur.aja.w,
+                                           beta, cscale, ctab, cur.sla, cur.sja, ov, W.wcap, acc,
+                                           n, order, fx, 0, 0, e_sorted, v_sorted);
+                if (cur.slb >= 0 && r2b < rc2)
+                    win_pair<false, false>(S, dxb, dyb, dzb, r2b, ABb.x, ABb.y, qi2, cur.ajb.w,
+                                           beta, cscale, ctab, cur.slb, cur.sjb, ov, W.wcap, acc,
+                                           n, order, fx, 0, 0, e_sorted, v_sorted);
+                if (cK + 128 >= cRow.cnt)
+                {
+                    // the end of the row: its own force
+                    const int64_t sx = wave_sum(S.sx), sy = wave_sum(S.sy), sz = wave_sum(S.sz);
+                    if (lane == 0)
+                    {
+                        if (!ov)
+                        {
+                            const int ua = cRow.col / W.b, vb = cRow.col - ua * W.b;
+                            const int cz = cRow.cell % W.nz;
+                            const int k = snb_win_range_of(W, z0, ua, vb + 2, cz - z0 + 2);
+                            const int slot_i = rpref[k] + (cRow.si - rfirst[k]);
+                            // negated-force convention of the LDS accumulators
+                            atomicAdd((unsigned long long*)&acc[slot_i], (unsigned long long)(-sx));
+                            atomicAdd((unsigned long long*)&acc[W.wcap + slot_i],
+                                      (unsigned long long)(-sy));
+                            atomicAdd((unsigned long long*)&acc[2 * W.wcap + slot_i],
+                                      (unsigned long long)(-sz));
+                        }
+                        else
+                        {
+                            const int idi = order[cRow.si];
+                            atomic_add_i64(fx + idi, sx);
+                            atomic_add_i64(fx + n + idi, sy);
+                            atomic_add_i64(fx + 2 * n + idi, sz);
+                        }
+                    }
+                    S.sx = S.sy = S.sz = 0;
+                }
+            }
+            cRow = nRow;
+            cK = nK;
+            nRow = eRow;
+            nK = eK;
+        };
+        while (true)
+        {
+            step(r0, r1, e1a, e1b, e2a, e2b);
+            if (cRow.si < 0) break;
+            step(r1, r0, e2a, e2b, e1a, e1b);
+            if (cRow.si < 0) break;
+        }
+    }
+    else
+    {
+        for (int col = 0; col < W.a * W.b; col++)
+        {
+            int ua = col / W.b, vb = col - ua * W.b;
+            if (x0 + ua >= W.nx || y0 + vb >= W.ny) continue;
+            int colbase = ((x0 + ua) * W.ny + (y0 + vb)) * W.nz;
+            int z1 = z0 + W.s < W.nz ? z0 + W.s : W.nz;
+            int r0 = start[colbase + z0], r1 = start[colbase + z1];
+            for (int si = r0 + wave; si < r1; si += nwave)
+            {
+                const int cnt = row_count[si], split = row_split[si];
+                const unsigned row0 = (unsigned)si * (unsigned)max_nb;  // n * max_nb < 2^31
+                const float4 ai = satom[si];
+                const float qi2 = ai.w * 4294967296.0f;  // 2^32 fixed-point scale
+                const unsigned trow = (unsigned)(stype[si] * ntypes);
+                const int id_i = (NEED_ENERGY || NEED_VIRIAL) && ref_attr ? order[si] : 0;
+                WinRowSums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                // Software pipeline over 128-entry steps (two entries per lane):
+                // step k uses atoms whose records were fetched during step k - 1,
+                // fetches the records of step k + 1 (slot map, then position and
+                // type) and the list entries of step k + 2.  32-bit byte offsets:
+                // the loads take the scalar-base + vector-offset form.  Energy /
+                // virial variants fetch each step's records when it starts: their
+                // FP64 energies and row sums leave no registers for a second set
+                // of records, and a pipelined version spills.
+                constexpr bool PIPE = !(NEED_ENERGY || NEED_VIRIAL);
+                auto entry = [&](int k) -> int {
+                    return k < cnt ? (int)ld32(nl16, row0 + (unsigned)snb_row_entry(k, split, max_nb))
+                                   : -1;
+                };
+                auto sorted_of = [&](int sl) -> unsigned {
+                    return sl >= 0 ? (unsigned)win_slot_sorted(sl, ov, slot_s, rfirst, rpref, W.nrange)
+                                   : (unsigned)si;
+                };
+                int sla = 0, slb = 0, tja = 0, tjb = 0, nxa = 0, nxb = 0;
+                unsigned sja = 0, sjb = 0;
+                float4 aja = {0, 0, 0, 0}, ajb = {0, 0, 0, 0};
+                auto records = [&](int k) {
+                    sla = entry(k + lane);
+                    slb = entry(k + 64 + lane);
+                    sja = sorted_of(sla);
+                    sjb = sorted_of(slb);
+                    aja = ld32(satom, sja);
+                    ajb = ld32(satom, sjb);
+                    tja = ld32(stype, sja);
+                    tjb = ld32(stype, sjb);
+                };
                 if (PIPE)
                 {
-                    sla = sla_n;
-                    slb = slb_n;
-                    sja = sja_n;
-                    sjb = sjb_n;
-                    aja = aja_n;
-                    ajb = ajb_n;
-                    tja = tja_n;
-                    tjb = tjb_n;
+                    records(0);
+                    nxa = entry(lane + 128);
+                    nxb = entry(lane + 192);
                 }
-            }
-            int64_t sx = wave_sum(S.sx), sy = wave_sum(S.sy), sz = wave_sum(S.sz);
-            int64_t elj = 0, ec = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-            if (NEED_ENERGY)
-            {
-                elj = wave_sum(S.elj);
-                ec = wave_sum(S.ec);
-            }
-            if (NEED_VIRIAL)
-            {
-                v0 = wave_sum(S.v0); v1 = wave_sum(S.v1); v2 = wave_sum(S.v2);
-                v3 = wave_sum(S.v3); v4 = wave_sum(S.v4); v5 = wave_sum(S.v5);
-            }
-            if (lane == 0)
-            {
-                if (!ov)
+                for (int k0 = 0; k0 < cnt; k0 += 128)
                 {
-                    int cz = sorted_cell[si] % W.nz;
-                    int k = snb_win_range_of(W, z0, ua, vb + 2, cz - z0 + 2);
-                    int slot_i = rpref[k] + (si - rfirst[k]);
-                    // negated-force convention of the LDS accumulators
-                    atomicAdd((unsigned long long*)&acc[slot_i], (unsigned long long)(-sx));
-                    atomicAdd((unsigned long long*)&acc[W.wcap + slot_i],
-                              (unsigned long long)(-sy));
-                    atomicAdd((unsigned long long*)&acc[2 * W.wcap + slot_i],
-                              (unsigned long long)(-sz));
+                    if (!PIPE) records(k0);
+                    const float2 ABa = ld32(ab2, trow + (unsigned)tja);
+                    const float2 ABb = ld32(ab2, trow + (unsigned)tjb);
+                    // records of the next step
+                    int sla_n = 0, slb_n = 0, tja_n = 0, tjb_n = 0;
+                    unsigned sja_n = 0, sjb_n = 0;
+                    float4 aja_n = {0, 0, 0, 0}, ajb_n = {0, 0, 0, 0};
+                    if (PIPE)
+                    {
+                        sla_n = nxa;
+                        slb_n = nxb;
+                        sja_n = sorted_of(sla_n);
+                        sjb_n = sorted_of(slb_n);
+                        aja_n = ld32(satom, sja_n);
+                        ajb_n = ld32(satom, sjb_n);
+                        tja_n = ld32(stype, sja_n);
+                        tjb_n = ld32(stype, sjb_n);
+                        nxa = entry(k0 + 256 + lane);
+                        nxb = entry(k0 + 320 + lane);
+                    }
+                    float dxa = aja.x - ai.x, dya = aja.y - ai.y, dza = aja.z - ai.z;
+                    float dxb = ajb.x - ai.x, dyb = ajb.y - ai.y, dzb = ajb.z - ai.z;
+                    win_disp<ORTHO>(dxa, dya, dza, box);
+                    win_disp<ORTHO>(dxb, dyb, dzb, box);
+                    const float r2a = (dxa * dxa + dya * dya) + dza * dza;
+                    const float r2b = (dxb * dxb + dyb * dyb) + dzb * dzb;
+                    // keep the parameter gathers ahead of the branches (the
+                    // compiler would otherwise sink them into the branches, where
+                    // their latency is not hidden behind the distance test); an
+                    // empty statement that reads the registers, nothing is emitted
+                    asm volatile("" ::"v"(ABa.x), "v"(ABa.y), "v"(ABb.x), "v"(ABb.y));
+                    if (sla >= 0 && r2a < rc2)
+                        win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxa, dya, dza, r2a, ABa.x, ABa.y, qi2,
+                                                           aja.w, beta, cscale, ctab, sla, sja, ov,
+                                                           W.wcap, acc, n, order, fx, ref_attr, id_i,
+                                                           e_sorted, v_sorted);
+                    if (slb >= 0 && r2b < rc2)
+                        win_pair<NEED_ENERGY, NEED_VIRIAL>(S, dxb, dyb, dzb, r2b, ABb.x, ABb.y, qi2,
+                                                           ajb.w, beta, cscale, ctab, slb, sjb, ov,
+                                                           W.wcap, acc, n, order, fx, ref_attr, id_i,
+                                                           e_sorted, v_sorted);
+                    if (PIPE)
+                    {
+                        sla = sla_n;
+                        slb = slb_n;
+                        sja = sja_n;
+                        sjb = sjb_n;
+                        aja = aja_n;
+                        ajb = ajb_n;
+                        tja = tja_n;
+                        tjb = tjb_n;
+                    }
                 }
-                else
-                {
-                    int idi = order[si];
-                    atomic_add_i64(fx + idi, sx);
-                    atomic_add_i64(fx + n + idi, sy);
-                    atomic_add_i64(fx + 2 * n + idi, sz);
-                }
-                // other rows' lower-id terms may land on si as well
-                // (ref_attr: the arrays were cleared before the launch)
+                int64_t sx = wave_sum(S.sx), sy = wave_sum(S.sy), sz = wave_sum(S.sz);
+                int64_t elj = 0, ec = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
                 if (NEED_ENERGY)
                 {
-                    if (ref_attr)
-                    {
-                        atomic_add_i64(e_sorted + si, elj);
-                        atomic_add_i64(e_sorted + (size_t)n + si, ec);
-                    }
-                    else
-                    {
-                        e_sorted[si] = elj;
-                        e_sorted[(size_t)n + si] = ec;
-                    }
+                    elj = wave_sum(S.elj);
+                    ec = wave_sum(S.ec);
                 }
                 if (NEED_VIRIAL)
                 {
-                    const int64_t vv[6] = {v0, v1, v2, v3, v4, v5};
-                    for (int c = 0; c < 6; c++)
+                    v0 = wave_sum(S.v0); v1 = wave_sum(S.v1); v2 = wave_sum(S.v2);
+                    v3 = wave_sum(S.v3); v4 = wave_sum(S.v4); v5 = wave_sum(S.v5);
+                }
+                if (lane == 0)
+                {
+                    if (!ov)
+                    {
+                        int cz = sorted_cell[si] % W.nz;
+                        int k = snb_win_range_of(W, z0, ua, vb + 2, cz - z0 + 2);
+                        int slot_i = rpref[k] + (si - rfirst[k]);
+                        // negated-force convention of the LDS accumulators
+                        atomicAdd((unsigned long long*)&acc[slot_i], (unsigned long long)(-sx));
+                        atomicAdd((unsigned long long*)&acc[W.wcap + slot_i],
+                                  (unsigned long long)(-sy));
+                        atomicAdd((unsigned long long*)&acc[2 * W.wcap + slot_i],
+                                  (unsigned long long)(-sz));
+                    }
+                    else
+                    {
+                        int idi = order[si];
+                        atomic_add_i64(fx + idi, sx);
+                        atomic_add_i64(fx + n + idi, sy);
+                        atomic_add_i64(fx + 2 * n + idi, sz);
+                    }
+                    // other rows' lower-id terms may land on si as well
+                    // (ref_attr: the arrays were cleared before the launch)
+                    if (NEED_ENERGY)
                     {
                         if (ref_attr)
-                            atomic_add_i64(v_sorted + (size_t)c * n + si, vv[c]);
+                        {
+                            atomic_add_i64(e_sorted + si, elj);
+                            atomic_add_i64(e_sorted + (size_t)n + si, ec);
+                        }
                         else
-                            v_sorted[(size_t)c * n + si] = vv[c];
+                        {
+                            e_sorted[si] = elj;
+                            e_sorted[(size_t)n + si] = ec;
+                        }
+                    }
+                    if (NEED_VIRIAL)
+                    {
+                        const int64_t vv[6] = {v0, v1, v2, v3, v4, v5};
+                        for (int c = 0; c < 6; c++)
+                        {
+                            if (ref_attr)
+                                atomic_add_i64(v_sorted + (size_t)c * n + si, vv[c]);
+                            else
+                                v_sorted[(size_t)c * n + si] = vv[c];
+                        }
                     }
                 }
             }
@@ -979,12 +1195,16 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
     if (L.win_lds_set != lds)
     {
         // dynamic LDS above 64 KiB is granted per kernel
-        const void* fns[8] = {
-            (const void*)k_pair_win<true, true, true>, (const void*)k_pair_win<true, false, true>,
-            (const void*)k_pair_win<false, true, true>, (const void*)k_pair_win<false, false, true>,
-            (const void*)k_pair_win<true, true, false>, (const void*)k_pair_win<true, false, false>,
-            (const void*)k_pair_win<false, true, false>,
-            (const void*)k_pair_win<false, false, false>};
+        const void* fns[10] = {(const void*)k_pair_win<true, true, true, false>,
+                               (const void*)k_pair_win<true, false, true, false>,
+                               (const void*)k_pair_win<false, true, true, false>,
+                               (const void*)k_pair_win<false, false, true, false>,
+                               (const void*)k_pair_win<false, false, true, true>,
+                               (const void*)k_pair_win<true, true, false, false>,
+                               (const void*)k_pair_win<true, false, false, false>,
+                               (const void*)k_pair_win<false, true, false, false>,
+                               (const void*)k_pair_win<false, false, false, false>,
+                               (const void*)k_pair_win<false, false, false, true>};
         for (const void* f : fns)
             SNB_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         L.win_lds_set = lds;
@@ -1009,24 +1229,35 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
         return rc;
     if (ref_attr && need_virial && (rc = snb_launch_fill_i64(s, ctx->v_sorted, 6 * (size_t)n)))
         return rc;
+    // Force-only steps of large systems run the pipeline across row ends
+    // (STREAM): at 1M atoms the pair kernel takes 2.15 instead of 2.34 ms.
+    // At 93k atoms it is also faster alone (160 vs 170 us), but the step is
+    // not: it holds more registers (87 vs 76 per lane), and the PME
+    // workgroups that run beside it get fewer slots (0.2729 vs 0.2697 ms per
+    // step).
+    const int stream_mode = ctx->lj.row_stream;
+    const bool stream = !need_energy && !need_virial &&
+                        (stream_mode > 0 || (stream_mode < 0 && n >= (1 << 18)));
     snb_prof_begin(ctx, SNB_PROF_PAIR);
-#define SNB_WIN_LAUNCH(E, V, O)                                                                 \
-    hipLaunchKernelGGL((k_pair_win<E, V, O>), dim3(W.nblock), dim3(1024), lds, s, n, W,         \
+#define SNB_WIN_LAUNCH(E, V, O, ST)                                                             \
+    hipLaunchKernelGGL((k_pair_win<E, V, O, ST>), dim3(W.nblock), dim3(1024), lds, s, n, W,     \
                        L.cell_start, L.sorted_cell, L.row_count, L.row_split, L.nl16, L.max_nb, \
                        ctx->satom,                                                             \
                        ctx->stype, L.order, ctx->lj.ab2, ctx->lj.ntypes, ctx->lj.coul,         \
                        ctx->lj.coul_scale, box, rc2, beta,                                     \
                        L.stage_base, L.stage, (size_t)n * L.invmax, ctx->fx, ctx->e_sorted,   \
                        ctx->v_sorted, ref_attr, L.block_perm, jdirect)
-#define SNB_WIN_LAUNCH_EV(O)                 \
-    if (need_energy && need_virial)          \
-        SNB_WIN_LAUNCH(true, true, O);       \
-    else if (need_energy)                    \
-        SNB_WIN_LAUNCH(true, false, O);      \
-    else if (need_virial)                    \
-        SNB_WIN_LAUNCH(false, true, O);      \
-    else                                     \
-        SNB_WIN_LAUNCH(false, false, O);
+#define SNB_WIN_LAUNCH_EV(O)                     \
+    if (need_energy && need_virial)              \
+        SNB_WIN_LAUNCH(true, true, O, false);    \
+    else if (need_energy)                        \
+        SNB_WIN_LAUNCH(true, false, O, false);   \
+    else if (need_virial)                        \
+        SNB_WIN_LAUNCH(false, true, O, false);   \
+    else if (stream)                             \
+        SNB_WIN_LAUNCH(false, false, O, true);   \
+    else                                         \
+        SNB_WIN_LAUNCH(false, false, O, false);
     if (ortho)
     {
         SNB_WIN_LAUNCH_EV(true)

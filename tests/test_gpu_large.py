@@ -60,13 +60,15 @@ def test_water_1m_replicated_box_properties():
     # the last cell of each axis takes the remainder of box / cell width, so
     # a corner cell can hold several times the mean ~22 atoms: the per-cell
     # capacity key is raised from its default of 150, as the reference's
-    # Check_Overflow would raise it after flagging
-    nbs = Nonbonded(base, grid=(72, 72, 72), max_atom_in_grid_numbers=300)
+    # Check_Overflow would raise it after flagging (to 256, the most the
+    # windowed list path takes)
+    nbs = Nonbonded(base, grid=(72, 72, 72), max_atom_in_grid_numbers=256)
     e1, f1 = _terms(nbs)
     nbs.close()
-    nb = Nonbonded(sup, grid=(216, 216, 216), refresh_interval=20, max_atom_in_grid_numbers=300)
+    nb = Nonbonded(sup, grid=(216, 216, 216), refresh_interval=20, max_atom_in_grid_numbers=256)
     nb.set_export(False)
     print("window", nb.window_info(), flush=True)
+    assert nb.window_info()["windowed"] == 1
     nb.nl_update(forced=True)
     counts = nb.internal_counts()
     rho = n / float(sup["box"][0]) ** 3

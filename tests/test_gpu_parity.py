@@ -59,17 +59,21 @@ def test_neighbor_list_sets_equal_oracle(request, name):
 
 
 @pytest.mark.parametrize("name", SYSTEMS)
-@pytest.mark.parametrize("energy,virial,owner", [(False, False, False), (True, True, False),
-                                                 (True, True, True)])
-def test_pair_kernel_bit_exact_with_twin(request, name, energy, virial, owner):
+@pytest.mark.parametrize("energy,virial,owner,stream", [(False, False, False, 0),
+                                                        (False, False, False, 1),
+                                                        (True, True, False, 0),
+                                                        (True, True, True, 0)])
+def test_pair_kernel_bit_exact_with_twin(request, name, energy, virial, owner, stream):
     """Default: per-atom terms on the lower id, as the reference's rows hold
     them (twin over the public list).  owner=True: per-atom terms on the row
-    owner of the internal list (twin over the internal rows)."""
+    owner of the internal list (twin over the internal rows).  stream: the
+    force-only loop pipelined within rows (0) or across row ends (1)."""
     import oracle
     s = _get(request, name)
     nb = _nb(s)
-    print("case", name, energy, virial, owner, flush=True)
+    print("case", name, energy, virial, owner, stream, flush=True)
     nb.set_owner_attribution(owner)
+    nb.set_row_stream(stream)
     nb.nl_update(forced=True)
     counts, rows = nb.internal_rows() if owner else nb.nl_rows()
     nb.lj_direct(need_energy=energy, need_virial=virial, frc=False)
@@ -529,11 +533,13 @@ def test_window_93k_workload_pair_exact(request):
     np.testing.assert_array_equal(g_f, t_f)
     np.testing.assert_array_equal(nb.fixed_atom_terms(0), t_elj)
     for mode in (-1, 1):
-        print("case 93k force-only, hand-off mode", mode, flush=True)
-        nb.set_direct_j(mode)
-        nb.clear_fixed_forces()
-        nb.lj_direct(False, False, frc=False)
-        np.testing.assert_array_equal(nb.fixed_forces(), t_f)
+        for stream in (0, 1):
+            print("case 93k force-only, hand-off mode", mode, "stream", stream, flush=True)
+            nb.set_direct_j(mode)
+            nb.set_row_stream(stream)
+            nb.clear_fixed_forces()
+            nb.lj_direct(False, False, frc=False)
+            np.testing.assert_array_equal(nb.fixed_forces(), t_f)
     nb.close()
 
 
@@ -616,10 +622,11 @@ def test_window_overflow_path_bit_exact(water_9k, slots):
 
 
 @pytest.mark.parametrize("slots", [None, 64])
-def test_window_direct_j_adds_bit_exact(water_9k, slots):
+@pytest.mark.parametrize("stream", [0, 1])
+def test_window_direct_j_adds_bit_exact(water_9k, slots, stream):
     """Force-only steps with the window forces added atomically per block
-    (mode 1) match the twin bit for bit, as the staged gather (mode 0) does;
-    64 slots puts most blocks on the overflow path."""
+    (mode 1) match the twin bit for bit, as the staged gather (mode 0) does,
+    with either pair loop; 64 slots puts most blocks on the overflow path."""
     import oracle
     s = water_9k
     t_f = None
@@ -628,7 +635,8 @@ def test_window_direct_j_adds_bit_exact(water_9k, slots):
         if slots:
             nb.set_window_capacity(slots)
         nb.set_direct_j(mode)
-        print("case mode", mode, "slots", slots, nb.window_info(), flush=True)
+        nb.set_row_stream(stream)
+        print("case mode", mode, "slots", slots, "stream", stream, nb.window_info(), flush=True)
         nb.nl_update(forced=True)
         if t_f is None:
             counts, rows = nb.nl_rows()
