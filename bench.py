@@ -249,14 +249,25 @@ def serving_phase(args):
 
 
 def _drift(t_ps, etot, dof):
+    """Linear fit of the total energy over the whole run and over its part
+    after the first 10% (the switch from the thermostat leaves a transient),
+    plus the total energy at a few times."""
     import numpy as np
     t, E = np.asarray(t_ps), np.asarray(etot)
-    slope = float(np.polyfit(t, E, 1)[0]) if len(t) > 1 else 0.0   # kcal/mol per ps
     kt = 0.00198716 * 300.0
+
+    def slope(a, b):
+        return float(np.polyfit(a, b, 1)[0]) if len(a) > 1 else 0.0   # kcal/mol per ps
+    late = t >= 0.1 * t[-1]
+    s_all, s_late = slope(t, E), slope(t[late], E[late])
+    marks = sorted(set(int(i) for i in np.linspace(0, len(E) - 1, 6)) | {1, 2, 10})
     return {"samples": len(E), "E_total_first": float(E[0]), "E_total_last": float(E[-1]),
-            "E_total_std": float(E.std()),
-            "drift_kcal_per_mol_per_ns_per_dof": slope * 1000.0 / dof,
-            "drift_kT_per_ns_per_dof": slope * 1000.0 / dof / kt}
+            "E_total_std": float(E.std()), "E_total_std_after_10pct": float(E[late].std()),
+            "E_total_at_ps": {"%g" % t[i]: float(E[i]) for i in marks if i < len(E)},
+            "drift_kcal_per_mol_per_ns_per_dof": s_all * 1000.0 / dof,
+            "drift_kT_per_ns_per_dof": s_all * 1000.0 / dof / kt,
+            "drift_after_10pct_kcal_per_mol_per_ns_per_dof": s_late * 1000.0 / dof,
+            "drift_after_10pct_kT_per_ns_per_dof": s_late * 1000.0 / dof / kt}
 
 
 def nve_phase(nb, sysd, args):
