@@ -77,6 +77,10 @@ def parse():
                     help="MD steps per serving request (2 fs)")
     ap.add_argument("--serve-slots", type=int, default=8,
                     help="segments in flight in the batched run")
+    ap.add_argument("--export-steps", type=int, default=None,
+                    help="also time this many steps with the reference-layout list written at "
+                         "every rebuild (the drop-in default); 0 skips (default 200, 0 for "
+                         "water1m)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (scales the HBM byte counters "
                          "of a profiler run)")
@@ -85,7 +89,7 @@ def parse():
     defaults = {"refresh": 20 if big else 10, "cpu_seconds": 0.0 if big else 15.0,
                 "md_steps": 0 if big else 500, "nve_steps": 0 if big else 20000,
                 "nve_f64_steps": 0 if big else 5000,
-                "serve_requests": 0 if big else 64}
+                "serve_requests": 0 if big else 64, "export_steps": 0 if big else 200}
     for k, v in defaults.items():
         if getattr(a, k) is None:
             setattr(a, k, v)
@@ -484,15 +488,16 @@ def pairs_in_cutoff(sysd, counts, rows, cutoff):
     return total
 
 
-def pair_traffic(kernel):
-    """HBM bytes per launch of the pair kernel, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes over this same workload (scaled by what the
-    counters read for a 256 MiB copy); None when no such profile is present."""
+def pair_profile(kernel):
+    """The committed rocprofv3 figures of the pair kernel on this workload: HBM
+    bytes per launch from the FETCH_SIZE / WRITE_SIZE passes (scaled by what
+    the counters read for a 256 MiB copy) and its VALU issue from the SQ
+    counter passes; None when no such profile is present."""
     path = os.path.join(ROOT, "profiles", "k_pair_win_93k_mixed_traffic.json")
     if kernel != "k_pair_win" or not os.path.exists(path):
         return None
     with open(path) as f:
-        return float(json.load(f)["bytes_per_launch"])
+        return json.load(f)
 
 
 def main():
@@ -586,6 +591,32 @@ def main():
     n = sysd["n"]
     ms_per_step = ms / args.steps
     sps = 1000.0 / ms_per_step
+    export_leg = None
+    if graphs and args.export_steps > 0:
+        # the drop-in default: rebuilds also write the reference-layout list
+        # (rows in append order); same graphs per kind of step, captured again
+        nb.set_export(1)
+        k1 = k0 + args.steps
+        gx = {}
+        for k in range(k1, k1 + max(args.refresh, args.energy_interval) + 1):
+            if kind(k) not in gx:
+                rb, e = kind(k)
+                gx[kind(k)] = nb.capture_step(step=k, need_energy=e, need_virial=e)
+        torch.cuda.synchronize()
+        ex0 = torch.cuda.Event(enable_timing=True)
+        ex1 = torch.cuda.Event(enable_timing=True)
+        ex0.record(nb.stream)
+        for k in range(k1, k1 + args.export_steps):
+            gx[kind(k)].replay()
+        ex1.record(nb.stream)
+        torch.cuda.synchronize()
+        xms = ex0.elapsed_time(ex1) / args.export_steps
+        export_leg = {"steps": args.export_steps, "ms_per_step": xms,
+                      "ns_per_day": NS_PER_DAY_PER_STEP_S * 1000.0 / xms,
+                      "export": "reference-layout list written at every rebuild (rows in "
+                                "append order, snb_nl_set_export mode 1)"}
+        del gx
+        nb.set_export(False)
     pair_ms, pair_n = prof["pair"]
     pair_avg_s = pair_ms / max(pair_n, 1) / 1000.0
     alg_bytes = 4.0 * e_list + 44.0 * n
@@ -604,6 +635,7 @@ def main():
     npairs = None if counts is None else pairs_in_cutoff(sysd, counts, rows, nb.cutoff)
     per = {k: (v[0] / max(v[1], 1)) for k, v in prof.items()}
     pair_kernel = "k_pair_win" if nb.window_info()["windowed"] else "k_lj_direct"
+    prof_pair = pair_profile(pair_kernel) if args.workload == "mixed93k" else None
     out = {
         "metric": ("ns/day at 93k atoms; batched agg ns/day + p99 latency at force err<5e-5"
                    if args.workload == "mixed93k" else
@@ -634,7 +666,13 @@ def main():
                    "energy_interval": args.energy_interval},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": pair_traffic(pair_kernel) if args.workload == "mixed93k" else None,
+                     "traffic": (prof_pair["bytes_per_launch"] if prof_pair else None),
+                     "limiter": "VALU issue" if prof_pair else None,
+                     "valu_issue": None if not prof_pair else {
+                         "frac": prof_pair["valu_issue_frac"],
+                         "valu_wave_instructions_per_launch": prof_pair["sq_insts_valu"],
+                         "source": "profiles/k_pair_win_93k_mixed.md (SQ_INSTS_VALU over "
+                                   "256 CUs x GRBM_GUI_ACTIVE cycles per XCD)"},
                      "kernel": pair_kernel, "alg_bytes_per_launch": alg_bytes,
                      "avg_launch_us": pair_avg_s * 1e6, "launches": pair_n,
                      "pairs_in_cutoff": npairs,
@@ -642,6 +680,7 @@ def main():
         "kernel_avg_us": {k: v * 1000.0 for k, v in per.items()},
         "pair_window": nb.window_info(),
         "launch": "hip-graph replay per step kind" if graphs else "eager",
+        "export_on": export_leg,
         "status_flags": status,
         "wall_s_timed": wall,
         "cpu_baseline": base,
