@@ -617,6 +617,24 @@ __global__ void k_pme_scatter(int n, const int4* __restrict__ atom_u, const int*
 
 __device__ __forceinline__ int wrap_axis(int v, int n) { return v < 0 ? v + n : v; }
 
+// The device dispatches workgroups to its 8 XCDs round-robin by index, and
+// each XCD has its own L2.  A launch of xcd_grid(nblocks) workgroups maps
+// onto blocks 0 .. nblocks - 1 so that XCD x takes the contiguous range
+// [x * chunk, (x + 1) * chunk): neighbouring bricks (and atoms in brick
+// order), which read the same grid points and atom records, share one L2.
+// -1 for the padding workgroups.
+#define SNB_XCDS 8
+__device__ __forceinline__ int xcd_block(int nblocks)
+{
+    const int chunk = (nblocks + SNB_XCDS - 1) / SNB_XCDS;
+    const int b = (int)(blockIdx.x % SNB_XCDS) * chunk + (int)(blockIdx.x / SNB_XCDS);
+    return b < nblocks ? b : -1;
+}
+static inline unsigned xcd_grid(int nblocks)
+{
+    return (unsigned)(SNB_XCDS * ((nblocks + SNB_XCDS - 1) / SNB_XCDS));
+}
+
 // One workgroup per brick of bx*by*bz grid points.  An atom with base point u
 // touches points u-3..u on each axis, so a brick receives from atoms whose
 // base lies in the brick itself or in the next brick on each axis (bricks are
@@ -654,8 +672,9 @@ __global__ void __launch_bounds__(256) k_pme_spread(
     int* lmeta = (int*)(lw + SNB_SPREAD_LIST * 12);  // [SNB_SPREAD_LIST]
     __shared__ int src_first[8], src_pref[9], nlist;
     const int t = threadIdx.x;
+    const int B = xcd_block(nbx * nby * nbz);
+    if (B < 0) return;
     for (int p = t; p < npt; p += blockDim.x) acc[p] = 0;
-    const int B = blockIdx.x;
     const int Bz = B % nbz, By = (B / nbz) % nby, Bx = B / (nbz * nby);
     const int ox = Bx * bx, oy = By * by, oz = Bz * bz;
     if (t < 8)
@@ -761,7 +780,9 @@ __global__ void __launch_bounds__(256) k_pme_gather(int n, const int* __restrict
                                                     int fy, int fz, SnbRc rc, float upd,
                                                     int64_t* __restrict__ frc)
 {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int blk = xcd_block((n + (int)blockDim.x - 1) / (int)blockDim.x);
+    if (blk < 0) return;
+    const int v = blk * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int i = visit[v];
     int4 u = atom_u[i];
@@ -1105,7 +1126,7 @@ int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* c
         if ((rc = snb_launch_fill_int(s, P.brick_start, P.nbrick + 1, 0))) return rc;
     }
     size_t lds = sizeof(long long) * (size_t)P.bx * P.by * P.bz + 13 * sizeof(float) * SNB_SPREAD_LIST;
-    hipLaunchKernelGGL(k_pme_spread, dim3(P.nbrick), dim3(256), lds, s, P.atom_u, P.atom_fr,
+    hipLaunchKernelGGL(k_pme_spread, dim3(xcd_grid(P.nbrick)), dim3(256), lds, s, P.atom_u, P.atom_fr,
                        P.brick_start, P.brick_atoms, P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nbx,
                        P.nby, P.nbz, P.q);
     SNB_CHECK(hipGetLastError());
@@ -1129,7 +1150,7 @@ int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* c
     SNB_CHECK(hipGetLastError());
     if ((rc = fft_yz_inverse(ctx, P.spec, P.phi))) return rc;
     if (n > 0 && with_force)
-        hipLaunchKernelGGL(k_pme_gather, dim3((n + 255) / 256), dim3(256), 0, s, n,
+        hipLaunchKernelGGL(k_pme_gather, dim3(xcd_grid((n + 255) / 256)), dim3(256), 0, s, n,
                            P.brick_atoms, P.atom_u,
                            P.atom_fr, P.phi, P.fx, P.fy, P.fz, r6, (float)P.update_interval,
                            ctx->fx_pme);
