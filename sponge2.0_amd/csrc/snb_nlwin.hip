@@ -81,16 +81,22 @@ __device__ __forceinline__ void win_block_of_cell(const SnbWin& W, int c, int& c
 
 // Candidate capacity of one LDS chunk (the atoms per cell whose row counts
 // live in LDS, SNB_NLC_MAXA, are in snb_internal.h).
-#define SNB_NLC_CAP 1792
-#define SNB_NLC_MAXRUN 26
+// 1408 candidates (22 groups, 37 KB of LDS) hold a mean stencil of ~1,360
+// atoms at 6 A cells; four workgroups then fit a CU beside the reciprocal
+// pipeline's (1792 measured slower per step, though not alone)
+#define SNB_NLC_CAP 1408
+#define SNB_NLC_MAXRUN 63
 
 // One 512-thread workgroup per cell.  The candidates of the cell's forward
-// stencil -- 13 columns, z-runs split at the periodic wrap, at most 26 runs --
-// are staged in LDS once (in chunks of SNB_NLC_CAP) with their window slots,
-// and every atom of the cell tests them from LDS: wave w builds the rows of
-// atoms w, w + 8, ...  Candidate order is column, run, sorted index, so a row
-// lists its entries in that order; the first run is the cell itself, where
-// only larger sorted indices count.
+// stencil -- 13 columns of up to 5 cells, one run per cell, at most 63 runs
+// -- are staged in LDS once (in chunks of SNB_NLC_CAP) with their window
+// slots, and every atom of the cell tests them from LDS: wave w builds the
+// rows of atoms w, w + 8, ...  Candidate order is column, z (unwrapped), sorted
+// index, so a row lists its entries in that order; the first run is the cell
+// itself, where only larger sorted indices count.  For a diagonal box each
+// atom first marks the stencil cells whose box lies within the list radius
+// of its (wrapped) position, with a margin over float rounding, and skips the
+// groups of 64 candidates that lie in unmarked cells only.
 //
 // The distance is taken as q_i - q_j for the row atom i.  The reference takes
 // it from the higher id to the lower; the two agree bit for bit (negation is
@@ -105,7 +111,7 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
     const int* __restrict__ ex_num, const int2* __restrict__ ex_rng, int max_nb, float near2,
     uint16_t* __restrict__ nl16, int* __restrict__ row_count, int* __restrict__ row_split,
-    int* __restrict__ status)
+    int* __restrict__ status, float grid_length)
 {
     if (need[0] == 0) return;
     __shared__ int first[SNB_WIN_MAX_RANGES];
@@ -116,6 +122,8 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
     __shared__ int run_off[SNB_NLC_MAXRUN + 1];
     __shared__ int run_p0[SNB_NLC_MAXRUN];
     __shared__ int run_slot0[SNB_NLC_MAXRUN];
+    __shared__ int run_rel[SNB_NLC_MAXRUN];  // (dx + 2) << 6 | (dy + 2) << 3 | (dz + 2)
+    __shared__ int grp_r0[SNB_NLC_CAP / 64], grp_r1[SNB_NLC_CAP / 64];  // runs of each group
     __shared__ int nrun_s;
     __shared__ int rowcnt[SNB_NLC_MAXA];   // near pairs (front of the row)
     __shared__ int rowfar[SNB_NLC_MAXA];   // skin pairs (back of the row)
@@ -143,11 +151,24 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
         if (tid == 0) atomicOr(status, SNB_FLAG_NL_OVERFLOW);
         return;
     }
-    if (tid == 0)
+    if (wave == 0)
     {
-        int nr = 0, tot = 0;
-        for (int col = 0; col < 13; col++)
+        // lane r builds run r: the own column holds 3 cells (cz .. cz + 2),
+        // the 12 others 5 (cz - 2 .. cz + 2); sizes are then scanned
+        int size = 0;
+        if (lane < SNB_NLC_MAXRUN)
         {
+            int col, zu;
+            if (lane < 3)
+            {
+                col = 0;
+                zu = cz + lane;
+            }
+            else
+            {
+                col = 1 + (lane - 3) / 5;
+                zu = cz - 2 + (lane - 3) % 5;
+            }
             int dx, dy;
             if (col < 3)
             {
@@ -159,40 +180,29 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
                 dx = 1 + (col - 3) / 5;
                 dy = (col - 3) % 5 - 2;
             }
-            int u = (cx - x0) + dx, v = (cy - y0) + dy + 2;
-            int base = (snb_wrap(cx + dx, W.nx) * W.ny + snb_wrap(cy + dy, W.ny)) * W.nz;
-            int zlo = cz + (col == 0 ? 0 : -2), zhi = cz + 2;
-            int za[2], zb[2], zu[2], nr2;
-            if (zlo < 0)
-            {
-                za[0] = zlo + W.nz; zb[0] = W.nz - 1; zu[0] = zlo;
-                za[1] = 0; zb[1] = zhi; zu[1] = 0;
-                nr2 = 2;
-            }
-            else if (zhi >= W.nz)
-            {
-                za[0] = zlo; zb[0] = W.nz - 1; zu[0] = zlo;
-                za[1] = 0; zb[1] = zhi - W.nz; zu[1] = W.nz;
-                nr2 = 2;
-            }
-            else
-            {
-                za[0] = zlo; zb[0] = zhi; zu[0] = zlo;
-                nr2 = 1;
-            }
-            for (int r = 0; r < nr2; r++)
-            {
-                int k = snb_win_range_of(W, z0, u, v, zu[r] - z0 + 2);
-                int p0 = start[base + za[r]], p1 = start[base + zb[r] + 1];
-                run_p0[nr] = p0;
-                run_slot0[nr] = pref[k] + (p0 - first[k]);
-                run_off[nr] = tot;
-                tot += p1 - p0;
-                nr++;
-            }
+            const int u = (cx - x0) + dx, v = (cy - y0) + dy + 2;
+            const int base = (snb_wrap(cx + dx, W.nx) * W.ny + snb_wrap(cy + dy, W.ny)) * W.nz;
+            // zu unwrapped: the window range and the order of the cells
+            const int zw = snb_wrap(zu, W.nz);
+            const int k = snb_win_range_of(W, z0, u, v, zu - z0 + 2);
+            const int p0 = start[base + zw], p1 = start[base + zw + 1];
+            run_p0[lane] = p0;
+            run_slot0[lane] = pref[k] + (p0 - first[k]);
+            run_rel[lane] = ((dx + 2) << 6) | ((dy + 2) << 3) | (zu - cz + 2);
+            size = p1 - p0;
         }
-        run_off[nr] = tot;
-        nrun_s = nr;
+        int incl = size;
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            const int up = __shfl_up(incl, off);
+            if (lane >= off) incl += up;
+        }
+        if (lane < SNB_NLC_MAXRUN) run_off[lane] = incl - size;
+        if (lane == SNB_NLC_MAXRUN - 1)
+        {
+            run_off[SNB_NLC_MAXRUN] = incl;
+            nrun_s = SNB_NLC_MAXRUN;
+        }
     }
     __syncthreads();
     const int nrun = nrun_s, total = run_off[nrun];
@@ -202,13 +212,25 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
         const int clen = total - chunk0 < SNB_NLC_CAP ? total - chunk0 : SNB_NLC_CAP;
         for (int t = tid; t < clen; t += blockDim.x)
         {
-            int g = chunk0 + t, r = 0;
-            while (run_off[r + 1] <= g) r++;
+            // the last run starting at or before g (empty runs share offsets)
+            const int g = chunk0 + t;
+            int lo = 0, hi = nrun - 1;
+            while (lo < hi)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if (run_off[mid] <= g)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int r = lo;
             int off = g - run_off[r];
             float4 q = sorted_crd[run_p0[r] + off];
             cand[t] = q;
             cex[t] = ex_num ? ex_rng[__float_as_int(q.w)] : make_int2(1, 0);
             cslot[t] = (uint16_t)(run_slot0[r] + off);
+            if ((t & 63) == 0) grp_r0[t >> 6] = r;
+            if ((t & 63) == 63 || t == clen - 1) grp_r1[t >> 6] = r;
         }
         __syncthreads();
         for (int a = wave; a < m && a < SNB_NLC_MAXA; a += (int)(blockDim.x >> 6))
@@ -219,8 +241,38 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
             const int2 rng_i = ex_num ? ex_rng[id_i] : make_int2(1, 0);
             uint16_t* row = nl16 + (size_t)s * max_nb;
             int count = rowcnt[a], cfar = rowfar[a];
+            // stencil cells within reach of atom i: per axis the gap from its
+            // wrapped position (the binning's arithmetic) to the cell, a lower
+            // bound of the minimum-image distance (the last cell of an axis
+            // is wider, which only lengthens the true gap)
+            uint64_t reach = ~0ull;
+            if (ORTHO)
+            {
+                const float xw = qi.x - floorf(qi.x / box.c[0]) * box.c[0];
+                const float yw = qi.y - floorf(qi.y / box.c[2]) * box.c[2];
+                const float zw = qi.z - floorf(qi.z / box.c[5]) * box.c[5];
+                auto gap = [&](int d, int c, float w) -> float {
+                    float g = d > 0 ? (float)(c + d) * grid_length - w
+                                    : (d < 0 ? w - (float)(c + d + 1) * grid_length : 0.0f);
+                    return g > 0.0f ? g : 0.0f;
+                };
+                bool in = false;
+                if (lane < nrun)
+                {
+                    const int rel = run_rel[lane];
+                    const float gx = gap((rel >> 6) - 2, cx, xw);
+                    const float gy = gap(((rel >> 3) & 7) - 2, cy, yw);
+                    const float gz = gap((rel & 7) - 2, cz, zw);
+                    in = (gx * gx + gy * gy) + gz * gz < cs2 + 0.05f;
+                }
+                reach = __ballot(in);
+            }
             for (int t0 = 0; t0 < clen; t0 += 64)
             {
+                {
+                    const int r0 = grp_r0[t0 >> 6], r1 = grp_r1[t0 >> 6];
+                    if (!(reach & (((2ull << (r1 - r0)) - 1ull) << r0))) continue;
+                }
                 const int t = t0 + lane;
                 const int g = chunk0 + t;
                 bool ok = false, nearp = false;
@@ -800,12 +852,12 @@ int snb_launch_nl_window(snb_context* ctx, const SnbBox& box, const int* ex_star
         hipLaunchKernelGGL(k_nl_build_cell<true>, dim3(L.ncell), dim3(512), 0, s, L.need, W,
                            L.cell_start, L.sorted_crd, box, L.cs2, ex_start, ex_list, ex_num,
                            L.ex_rng, L.max_nb, L.near2, L.nl16, L.row_count, L.row_split,
-                           ctx->status);
+                           ctx->status, L.grid_length);
     else
         hipLaunchKernelGGL(k_nl_build_cell<false>, dim3(L.ncell), dim3(512), 0, s, L.need, W,
                            L.cell_start, L.sorted_crd, box, L.cs2, ex_start, ex_list, ex_num,
                            L.ex_rng, L.max_nb, L.near2, L.nl16, L.row_count, L.row_split,
-                           ctx->status);
+                           ctx->status, L.grid_length);
     hipLaunchKernelGGL(k_win_sizes, dim3(W.nblock), dim3(64), 0, s, L.need, W, L.cell_start,
                        L.win_size);
     hipLaunchKernelGGL(k_block_work, dim3(W.nblock), dim3(64), 0, s, L.need, W, L.cell_start,
