@@ -7,6 +7,7 @@
 - Served water-box segments (cutoff 9 A, PME spacing ~1 A) that span the
   plain and the windowed pair paths: each final frame against the FP64
   oracle, and each result the same alone and in the mix.
+- The 93k system's neighbour list against the oracle's build, row by row.
 
 Tolerances: per-atom force 5e-5 relative RMS; every energy term and the
 total virial tensor 1e-6 relative.
@@ -122,3 +123,33 @@ def test_served_segments_vs_fp64_and_alone():
         _frame_vs_fp64(nb, s)
         nb.close()
     assert paths == {False, True}
+
+
+def test_mixed_93k_neighbor_list_equals_oracle():
+    """The 93k system's public half list (row = lower id) equals the oracle's
+    cell-list build with the reference's float distance test and exclusion
+    rule: the same count in every row and the same entries (order within a
+    row aside)."""
+    import oracle
+    from sponge_amd import systems
+    from sponge_amd.lib import Nonbonded
+    s = systems.config_gpu_93k()
+    nb = Nonbonded(s, grid=(96, 96, 96))
+    assert nb.window_info()["windowed"] == 1
+    nb.nl_update(forced=True)
+    gc, gr = nb.nl_rows()
+    assert nb.status() == 0
+    oc, orows, _ = oracle.nl_build(s["crd"], s["cell"], s["rcell"], nb.cutoff, 2.0,
+                                   (s["excl_start"], s["excl_numbers"], s["excl_list"]),
+                                   max_nb=nb.max_nb)
+    print("entries", int(gc.sum()), int(oc.sum()), flush=True)
+    np.testing.assert_array_equal(gc, oc)
+    width = int(gc.max())
+    big = np.iinfo(np.int32).max
+    cols = np.arange(width)[None, :]
+    a = np.where(cols < gc[:, None], gr[:, :width], big)
+    b = np.where(cols < oc[:, None], orows[:, :width], big)
+    a.sort(axis=1)
+    b.sort(axis=1)
+    np.testing.assert_array_equal(a, b)
+    nb.close()
