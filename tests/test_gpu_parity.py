@@ -654,8 +654,10 @@ def test_window_direct_j_adds_bit_exact(water_9k, slots, stream):
 
 @pytest.mark.parametrize("overlap", [True, False])
 def test_pme_stream_overlap_same_results(water_9k, overlap):
-    """The reciprocal pipeline on its own stream gives the same fused forces,
-    energies and virials as on the context stream."""
+    """The reciprocal pipeline on its own stream (and, on force-only fused
+    steps, the exclusion correction, 1-4 pairs and bonds with it) gives the
+    same fused forces, energies and virials as everything on the context
+    stream."""
     import torch
     res = []
     for ov in (False, overlap):
@@ -663,7 +665,7 @@ def test_pme_stream_overlap_same_results(water_9k, overlap):
         nb.set_pme_overlap(ov)
         nb.step(step=0, forced_nl=True, need_energy=True, need_virial=True)
         nb.zero_outputs()
-        nb.step(step=1)                        # force-only: reciprocal pipeline on the PME stream
+        nb.step(step=1)                        # force-only: exclusion + reciprocal on the PME stream
         torch.cuda.synchronize()
         f_plain = nb.frc.cpu().numpy().copy()
         nb.zero_outputs()
