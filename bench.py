@@ -124,7 +124,7 @@ def cpu_baseline(sysd, nb, counts, rows, budget_s):
         pme = oracle.pme_f64(s["crd"], s["charge"], s["rcell"], nb.grid, nb.beta, nb.volume)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 50:
+        if el >= budget_s or steps >= 500:
             break
     sps = steps / el
     nt = oracle.num_threads()
