@@ -407,6 +407,51 @@ def _compare_frame(nb, nb64, crd64, probe64):
             "virial": float(np.max(np.abs(v32 - v64)) / np.max(np.abs(v64)))}
 
 
+class StepGraphs:
+    """Captured HIP graphs of the steps of a run, made before the timed
+    region: one graph per neighbour-list period (`period` steps from a
+    multiple of it, so one rebuild each) and one per single step for the
+    rest, keyed by the kinds of their steps (rebuild or not, energy or not).
+    A graph captured with one run of step numbers is replayed for every run
+    of the same kinds: the step number only decides those.  Replaying them in
+    order issues the same work as step-by-step replays, with one graph launch
+    per period instead of one per step."""
+
+    def __init__(self, nb, run_step, kind, period):
+        self.nb, self.run_step, self.kind, self.period = nb, run_step, kind, period
+        self.graphs = {}
+
+    def _capture(self, ks):
+        import torch
+        nb = self.nb
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(device=nb.dev)
+        home = nb.stream
+        nb.sync()
+        torch.cuda.synchronize()
+        nb.set_stream(side)
+        try:
+            with torch.cuda.graph(graph, stream=side):
+                for k in ks:
+                    self.run_step(k)
+        finally:
+            nb.set_stream(home)
+        torch.cuda.synchronize()
+        return graph
+
+    def plan(self, k0, steps):
+        """The graph sequence for steps k0 .. k0 + steps - 1 (captured now)."""
+        seq, k, P = [], k0, self.period
+        while k < k0 + steps:
+            ks = list(range(k, k + P)) if (P > 1 and k % P == 0 and k + P <= k0 + steps) else [k]
+            key = tuple(self.kind(j) for j in ks)
+            if key not in self.graphs:
+                self.graphs[key] = self._capture(ks)
+            seq.append(self.graphs[key])
+            k += len(ks)
+        return seq
+
+
 def md_phase(nb, sysd, args, kind):
     """Whole MD steps of the same workload: the force path, then the middle
     Langevin leap-frog (300 K, tau 0.1 ps) and SETTLE on the 30,000 waters,
@@ -420,35 +465,22 @@ def md_phase(nb, sysd, args, kind):
     for k in range(k0, k0 + 20):
         nb.md_step(step=k, need_energy=False, need_virial=False, forced_nl=(k == 0))
     k0 += 20
-    graphs = {}
+    def run_step(k):
+        e = (k % args.energy_interval) == 0
+        nb.md_step(step=k, need_energy=e, need_virial=e)
+    seq = None
     if not args.no_graph:
-        for k in range(k0, k0 + max(args.refresh, args.energy_interval) + 1):
-            if kind(k) not in graphs:
-                rb, e = kind(k)
-                graph = torch.cuda.CUDAGraph()
-                side = torch.cuda.Stream(device=nb.dev)
-                home = nb.stream
-                nb.sync()
-                torch.cuda.synchronize()
-                nb.set_stream(side)
-                try:
-                    with torch.cuda.graph(graph, stream=side):
-                        nb.md_step(step=k, need_energy=e, need_virial=e)
-                finally:
-                    nb.set_stream(home)
-                torch.cuda.synchronize()
-                graphs[kind(k)] = graph
-        k0 += max(args.refresh, args.energy_interval) + 1
+        seq = StepGraphs(nb, run_step, kind, args.refresh).plan(k0, args.md_steps)
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     ev0.record(nb.stream)
-    for k in range(k0, k0 + args.md_steps):
-        if graphs:
-            graphs[kind(k)].replay()
-        else:
-            e = (k % args.energy_interval) == 0
-            nb.md_step(step=k, need_energy=e, need_virial=e)
+    if seq is not None:
+        for g in seq:
+            g.replay()
+    else:
+        for k in range(k0, k0 + args.md_steps):
+            run_step(k)
     ev1.record(nb.stream)
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / args.md_steps
@@ -460,8 +492,9 @@ def md_phase(nb, sysd, args, kind):
     st = nb.stress.cpu().numpy().astype(np.float64)    # the last pressure step's stress
     p_bar = (st[0] + st[2] + st[5]) / 3.0 * 6.946827162543585e4
     return {"ns_per_day": 1000.0 / ms * NS_PER_DAY_PER_STEP_S, "ms_per_step": ms,
-            "steps": args.md_steps, "launch": "hip-graph replay per step kind" if graphs
-            else "eager", "integrator": "middle Langevin leap-frog, 300 K, tau 0.1 ps, 2 fs",
+            "steps": args.md_steps,
+            "launch": "hip-graph replay, one graph per neighbour-list period"
+            if seq is not None else "eager", "integrator": "middle Langevin leap-frog, 300 K, tau 0.1 ps, 2 fs",
             "constraints": "SETTLE on %d rigid waters" % nw, "t_kin_end": t_kin,
             "pressure_bar_last": p_bar,
             "pressure_steps": "every %d steps (virials, SETTLE virial, Get_pressure on the "
@@ -562,24 +595,25 @@ def main():
     def kind(k):
         return ((k + 1) % args.refresh == 0, (k % args.energy_interval) == 0)
 
-    graphs = {}
+    def run_step(k):
+        e = (k % args.energy_interval) == 0
+        nb.step(step=k, need_energy=e, need_virial=e)
+
+    # one graph per neighbour-list period (and per single step for the rest),
+    # keyed by the kinds of their steps, captured before the timed region
+    seq = None
     if not args.no_graph:
-        # one graph per kind of step: (rebuild, energy); captured with a step
-        # number of that kind, replayed for every step of that kind
-        for k in range(k0, k0 + max(args.refresh, args.energy_interval) + 1):
-            if kind(k) not in graphs:
-                rb, e = kind(k)
-                graphs[kind(k)] = nb.capture_step(step=k, need_energy=e, need_virial=e)
-        torch.cuda.synchronize()
+        seq = StepGraphs(nb, run_step, kind, args.refresh).plan(k0, args.steps)
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev0.record(nb.stream)
-    for k in range(k0, k0 + args.steps):
-        if graphs:
-            graphs[kind(k)].replay()
-        else:
+    if seq is not None:
+        for g in seq:
+            g.replay()
+    else:
+        for k in range(k0, k0 + args.steps):
             step(k)
     ev1.record(nb.stream)
     torch.cuda.synchronize()
@@ -592,22 +626,17 @@ def main():
     ms_per_step = ms / args.steps
     sps = 1000.0 / ms_per_step
     export_leg = None
-    if graphs and args.export_steps > 0:
+    if seq is not None and args.export_steps > 0:
         # the drop-in default: rebuilds also write the reference-layout list
-        # (rows in append order); same graphs per kind of step, captured again
+        # (rows in append order); the same graphs, captured again
         nb.set_export(1)
         k1 = k0 + args.steps
-        gx = {}
-        for k in range(k1, k1 + max(args.refresh, args.energy_interval) + 1):
-            if kind(k) not in gx:
-                rb, e = kind(k)
-                gx[kind(k)] = nb.capture_step(step=k, need_energy=e, need_virial=e)
-        torch.cuda.synchronize()
+        gx = StepGraphs(nb, run_step, kind, args.refresh).plan(k1, args.export_steps)
         ex0 = torch.cuda.Event(enable_timing=True)
         ex1 = torch.cuda.Event(enable_timing=True)
         ex0.record(nb.stream)
-        for k in range(k1, k1 + args.export_steps):
-            gx[kind(k)].replay()
+        for g in gx:
+            g.replay()
         ex1.record(nb.stream)
         torch.cuda.synchronize()
         xms = ex0.elapsed_time(ex1) / args.export_steps
@@ -679,7 +708,8 @@ def main():
                      "pair_interactions_per_s": None if npairs is None else npairs / pair_avg_s},
         "kernel_avg_us": {k: v * 1000.0 for k, v in per.items()},
         "pair_window": nb.window_info(),
-        "launch": "hip-graph replay per step kind" if graphs else "eager",
+        "launch": ("hip-graph replay, one graph per neighbour-list period of %d steps"
+                   % args.refresh) if seq is not None else "eager",
         "export_on": export_leg,
         "status_flags": status,
         "wall_s_timed": wall,

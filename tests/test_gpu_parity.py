@@ -652,6 +652,50 @@ def test_window_direct_j_adds_bit_exact(water_9k, slots, stream):
     nb.close()
 
 
+def test_period_graph_matches_eager_steps(water_9k):
+    """Ten whole force steps (a neighbour-list period, the rebuild in its last
+    step, energies and virials in its first) captured in one HIP graph give
+    the eager steps' accumulated float forces, energies and virials on every
+    replay."""
+    import torch
+    nb = _nb(water_9k, refresh_interval=10)
+    nb.set_export(False)
+    nb.step(step=0, forced_nl=True)
+    torch.cuda.synchronize()
+
+    def period(k0):
+        for k in range(k0, k0 + 10):
+            e = k % 100 == 0
+            nb.step(step=k, need_energy=e, need_virial=e)
+
+    nb.zero_outputs()
+    period(100)
+    torch.cuda.synchronize()
+    ref = (nb.frc.cpu().numpy().copy(), nb.atom_virial.cpu().numpy().copy(), nb.energies())
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream(device=nb.dev)
+    home = nb.stream
+    nb.sync()
+    torch.cuda.synchronize()
+    nb.set_stream(side)
+    try:
+        with torch.cuda.graph(g, stream=side):
+            period(200)
+    finally:
+        nb.set_stream(home)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        nb.zero_outputs()
+        g.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(nb.frc.cpu().numpy(), ref[0])
+        np.testing.assert_array_equal(nb.atom_virial.cpu().numpy(), ref[1])
+        assert nb.energies() == ref[2]
+        nb.step(step=5)                       # eager work between the replays
+        torch.cuda.synchronize()
+    nb.close()
+
+
 @pytest.mark.parametrize("overlap", [True, False])
 def test_pme_stream_overlap_same_results(water_9k, overlap):
     """The reciprocal pipeline on its own stream (and, on force-only fused
