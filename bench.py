@@ -646,6 +646,36 @@ def main():
                                 "append order, snb_nl_set_export mode 1)"}
         del gx
         nb.set_export(False)
+    serial = None
+    if not args.no_overlap and args.workload == "mixed93k":
+        # the same kernel groups with the reciprocal pipeline (and, on these
+        # force-only steps, the exclusion, 1-4 and bond terms) on the context
+        # stream: each group alone, without the other stream beside it
+        nb.set_pme_overlap(False)
+        k2 = k0 + args.steps + max(args.export_steps, 0)
+        for k in range(k2, k2 + 10):
+            if (k + 1) % args.refresh and k % args.energy_interval:
+                nb.step(step=k)
+        torch.cuda.synchronize()
+        nb.profile(True)
+        for g in (lib.PROF_PAIR, lib.PROF_PME):
+            nb.profile_read(g, reset=True)
+        done = 0
+        for k in range(k2 + 10, k2 + 1000):
+            if (k + 1) % args.refresh == 0 or k % args.energy_interval == 0:
+                continue
+            nb.step(step=k)
+            done += 1
+            if done == 50:
+                break
+        torch.cuda.synchronize()
+        sp, sq = nb.profile_read(lib.PROF_PAIR), nb.profile_read(lib.PROF_PME)
+        nb.profile(False)
+        nb.set_pme_overlap(True)
+        serial = {"pair": 1000.0 * sp[0] / max(sp[1], 1), "pme": 1000.0 * sq[0] / max(sq[1], 1),
+                  "steps": done,
+                  "note": "eager force-only steps with both chains on one stream; "
+                          "kernel_avg_us has them overlapped"}
     pair_ms, pair_n = prof["pair"]
     pair_avg_s = pair_ms / max(pair_n, 1) / 1000.0
     alg_bytes = 4.0 * e_list + 44.0 * n
@@ -707,6 +737,7 @@ def main():
                      "pairs_in_cutoff": npairs,
                      "pair_interactions_per_s": None if npairs is None else npairs / pair_avg_s},
         "kernel_avg_us": {k: v * 1000.0 for k, v in per.items()},
+        "kernel_avg_us_alone": serial,
         "pair_window": nb.window_info(),
         "launch": ("hip-graph replay, one graph per neighbour-list period of %d steps"
                    % args.refresh) if seq is not None else "eager",
