@@ -370,8 +370,14 @@ static int side_begin(snb_context* c, hipStream_t* home)
     {
         SNB_CHECK(hipEventRecord(c->ev_inputs, c->stream));
         c->inputs_marked = true;  // the reciprocal call of this step waits on the same event
+        c->pme_waits_inputs = false;
     }
-    SNB_CHECK(hipStreamWaitEvent(c->pme_stream, c->ev_inputs, 0));
+    // one wait per record: later work on pme_stream is ordered after it
+    if (!c->pme_waits_inputs)
+    {
+        SNB_CHECK(hipStreamWaitEvent(c->pme_stream, c->ev_inputs, 0));
+        c->pme_waits_inputs = true;
+    }
     c->stream = c->pme_stream;
     return SNB_OK;
 }
@@ -546,6 +552,7 @@ int snb_set_stream(snb_context* c, void* hip_stream)
     if (!c) return SNB_ERR_ARGUMENT;
     c->stream = (hipStream_t)hip_stream;
     c->inputs_marked = false;  // an inputs event recorded on the old stream is not reused
+    c->pme_waits_inputs = false;
     return SNB_OK;
 }
 
@@ -759,6 +766,7 @@ int snb_nl_update(snb_context* c, const int* d_atom_local, int local_atom_number
     {
         SNB_CHECK(hipEventRecord(c->ev_inputs, c->stream));
         c->inputs_marked = true;
+        c->pme_waits_inputs = false;
     }
     int forced = (update || !c->nl.built) ? 1 : 0;
     // With a fixed refresh interval the host knows the decision
@@ -1547,7 +1555,11 @@ int snb_pme_update_box(snb_context* c, snb_ltmatrix3 cell, snb_ltmatrix3 rcell, 
     if ((rc = snb_pme_compute_bc(c, snb_box(cell, rcell)))) return rc;
     // a pipeline of this step that was already cleared to start must also
     // see the new table
-    if (c->overlap && c->inputs_marked) SNB_CHECK(hipEventRecord(c->ev_inputs, c->stream));
+    if (c->overlap && c->inputs_marked)
+    {
+        SNB_CHECK(hipEventRecord(c->ev_inputs, c->stream));
+        c->pme_waits_inputs = false;
+    }
     return SNB_OK;
 }
 
@@ -1651,8 +1663,13 @@ int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatr
             // ev_inputs is recorded after every earlier use of the reciprocal
             // accumulator on the context stream (a flush comes before the next
             // step's NL update), so the pipeline may start once it fires
-            if (!c->inputs_marked) SNB_CHECK(hipEventRecord(c->ev_inputs, home));
-            SNB_CHECK(hipStreamWaitEvent(c->pme_stream, c->ev_inputs, 0));
+            if (!c->inputs_marked)
+            {
+                SNB_CHECK(hipEventRecord(c->ev_inputs, home));
+                c->pme_waits_inputs = false;
+            }
+            if (!c->pme_waits_inputs) SNB_CHECK(hipStreamWaitEvent(c->pme_stream, c->ev_inputs, 0));
+            c->pme_waits_inputs = false;  // the next step records the event again
             c->inputs_marked = false;
             c->stream = c->pme_stream;
         }

@@ -263,6 +263,7 @@ struct snb_context
     hipEvent_t ev_inputs = nullptr;
     hipEvent_t ev_pme = nullptr;
     bool inputs_marked = false;
+    bool pme_waits_inputs = false;  // pme_stream already waits for the latest ev_inputs record
     bool pme_pending = false;
     SnbNl nl;
     SnbLj lj;
