@@ -221,19 +221,24 @@ def serving_phase(args):
     tune_s = time.perf_counter() - t
     runs = {}
     reqs = {}
-    # the sequential runs use the plain path's default (reciprocal pipeline on
-    # its own stream beside the direct-space work); the batched run keeps
-    # each segment on one stream, its segments overlap each other instead
-    for name, slots, overlap in (("sequential", 1, True),
-                                 ("batched", args.serve_slots, False)):
+    # The sequential runs use the plain path's default (reciprocal pipeline
+    # on its own stream beside the direct-space work).  The batched run
+    # advances the admitted segments as one step batch (one dispatch per
+    # kernel over all segments, one captured graph per period); the streams
+    # run gives each segment its own stream and period graph instead.
+    for name, slots, overlap, launch in (("sequential", 1, True, "streams"),
+                                         ("batched", args.serve_slots, False, "batched"),
+                                         ("streams", args.serve_slots, False, "streams")):
         rq = serving.make_requests(count=args.serve_requests, steps=args.serve_steps)
-        runs[name] = serving.Server(slots=slots, overlap=overlap).run(rq, realtime=True)
+        runs[name] = serving.Server(slots=slots, overlap=overlap,
+                                    launch=launch).run(rq, realtime=True)
         reqs[name] = rq
-    same = sum(int(np.array_equal(a.result["crd"], b.result["crd"]) and
-                   np.array_equal(a.result["vel"], b.result["vel"]))
-               for a, b in zip(reqs["sequential"], reqs["batched"]))
+    same = sum(int(all(np.array_equal(a.result["crd"], b.result["crd"]) and
+                       np.array_equal(a.result["vel"], b.result["vel"])
+                       for b in (reqs["batched"][k], reqs["streams"][k])))
+               for k, a in enumerate(reqs["sequential"]))
     status = sum(r.result["status"] for v in reqs.values() for r in v)
-    seq, bat = runs["sequential"], runs["batched"]
+    seq, bat, strm = runs["sequential"], runs["batched"], runs["streams"]
     return {"requests": args.serve_requests, "steps_per_request": args.serve_steps,
             "atoms_total": bat["atoms_total"], "arrival": "Poisson, 40/s",
             "batched_aggregate_ns_per_day": bat["aggregate_ns_per_day"],
@@ -243,13 +248,22 @@ def serving_phase(args):
             "sequential_p50_latency_s": seq["p50_latency_s"],
             "sequential_p99_latency_s": seq["p99_latency_s"],
             "batched_over_sequential": bat["aggregate_ns_per_day"] / seq["aggregate_ns_per_day"],
+            "streams_aggregate_ns_per_day": strm["aggregate_ns_per_day"],
+            "streams_over_sequential": strm["aggregate_ns_per_day"] / seq["aggregate_ns_per_day"],
             "slots": args.serve_slots, "makespan_s": {"sequential": seq["makespan_s"],
-                                                      "batched": bat["makespan_s"]},
-            "same_alone_and_in_mix": "%d of %d requests bit-identical" % (same, len(reqs["batched"])),
+                                                      "batched": bat["makespan_s"],
+                                                      "streams": strm["makespan_s"]},
+            "batched_plans": bat["plans"], "batched_last_plan": bat["last_plan"],
+            "same_alone_and_in_mix": "%d of %d requests bit-identical in both batched runs"
+                                     % (same, len(reqs["batched"])),
             "status_flags": status, "tune_s": tune_s,
             "shapes": {str(k): list(v[0]) + [v[2]] for k, v in tuned.items()},
-            "launch": "per-segment HIP stream, one captured graph per 10-step period, "
-                      "periods queued 2 ahead"}
+            "launch": {"batched": "one step batch over the admitted segments: one dispatch per "
+                                  "kernel of a step, reciprocal pipeline on a second stream, one "
+                                  "captured graph per 10-step period, re-planned when a segment "
+                                  "joins or leaves, periods queued 2 ahead",
+                       "streams": "per-segment HIP stream, one captured graph per 10-step "
+                                  "period, periods queued 2 ahead"}}
 
 
 def _drift(t_ps, etot, dof):

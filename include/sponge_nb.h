@@ -543,6 +543,42 @@ int snb_f64_nve_iterate(snb_context* ctx, const double* d_mass, const double* d_
 int snb_profile_enable(snb_context* ctx, int enable);
 int snb_profile_read(snb_context* ctx, int group, int reset, double* total_ms, int* launches);
 
+/* ------------------------------------------------------- step batches */
+/* One dispatch per kernel for the same step of several contexts (no
+ * counterpart in the reference, which runs one system per process): for a
+ * server that advances many small systems in lockstep, where a step of each
+ * is a chain of short kernels and the device's dispatch rate, not its compute
+ * units, bounds the throughput.
+ *
+ * Between snb_batch_record_begin(b, ctx) and snb_batch_record_end(b) the
+ * entry points called for ctx (on this thread) launch nothing: they record
+ * their launches, with the arguments they would pass, after every check and
+ * host-side decision they make.  Record each member's calls (for example the
+ * steps of one neighbour-list period), then snb_batch_plan merges the
+ * records: launches of a kernel that has a batched form become one dispatch
+ * over a device table with one entry per member, the others stay as they
+ * were, and every member's launches keep their order.  snb_batch_launch
+ * issues the plan on one stream, any number of times (it can be captured in a
+ * HIP graph); each member's results are bit-identical to making its calls
+ * alone.  The recorded arguments are fixed: replaying a plan repeats the
+ * recorded steps' host-side decisions (for a fixed refresh interval, record
+ * one whole period).  Members need the reciprocal pipeline on the context
+ * stream (snb_set_pme_overlap(ctx, 0)) and profiling off
+ * (SNB_ERR_UNSUPPORTED otherwise).  A member must not be destroyed, and its
+ * caller buffers not freed, while a plan that holds it may still run. */
+typedef struct snb_batch snb_batch;
+int snb_batch_create(snb_batch** out, int device);
+int snb_batch_destroy(snb_batch* batch);
+int snb_batch_record_begin(snb_batch* batch, snb_context* ctx);
+int snb_batch_record_end(snb_batch* batch);
+/* launches = dispatches per snb_batch_launch, batched = how many of them are
+ * batched forms */
+int snb_batch_plan(snb_batch* batch, int* launches, int* batched);
+int snb_batch_launch(snb_batch* batch, void* hip_stream);
+/* members recorded, dispatches per launch, batched dispatches, and the member
+ * launches they replace */
+int snb_batch_info(snb_batch* batch, int* members, int* launches, int* batched, int* merged);
+
 #ifdef __cplusplus
 }
 #endif

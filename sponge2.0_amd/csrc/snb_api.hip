@@ -26,19 +26,6 @@ void dfree(T*& p)
     p = nullptr;
 }
 
-struct DeviceGuard
-{
-    int prev = -1;
-    explicit DeviceGuard(int dev)
-    {
-        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard()
-    {
-        int cur;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-    }
-};
 
 // PM_force.cpp:244-272: doubling, then 50 bisection steps, in float with the
 // host libm erfc.
@@ -1673,9 +1660,13 @@ int snb_pme_reciprocal_force(snb_context* c, const snb_vector* d_crd, snb_ltmatr
             c->inputs_marked = false;
             c->stream = c->pme_stream;
         }
-        rc = keep_energy ? snb_launch_charge_sums(c, d_charge) : SNB_OK;
-        if (!rc)
-            rc = snb_launch_pme_recip(c, d_crd, d_charge, box, keep_energy, need_virial, 1);
+        {
+            // recorded step batches: the reciprocal lane
+            SnbRecScope lane(1, false);
+            rc = keep_energy ? snb_launch_charge_sums(c, d_charge) : SNB_OK;
+            if (!rc)
+                rc = snb_launch_pme_recip(c, d_crd, d_charge, box, keep_energy, need_virial, 1);
+        }
         if (c->overlap)
         {
             c->stream = home;

@@ -6,11 +6,12 @@
 // caller's AoS VECTOR coordinates and charges into one 16-byte record per
 // atom, so a pair gather is a single float4 load.  Also raises the sticky
 // non-finite flag.
-__global__ void k_pack_atoms(int n, const snb_vector* __restrict__ crd,
-                             const float* __restrict__ charge, float4* __restrict__ out,
-                             int* __restrict__ status)
+__device__ __forceinline__ void pack_atoms_body(int n, const snb_vector* __restrict__ crd,
+                                                const float* __restrict__ charge,
+                                                float4* __restrict__ out,
+                                                int* __restrict__ status, unsigned bid)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     snb_vector r = crd[i];
     out[i] = make_float4(r.x, r.y, r.z, charge[i]);
@@ -18,12 +19,35 @@ __global__ void k_pack_atoms(int n, const snb_vector* __restrict__ crd,
         atomicOr(status, SNB_FLAG_NONFINITE_CRD);
 }
 
+__global__ void k_pack_atoms(int n, const snb_vector* __restrict__ crd,
+                             const float* __restrict__ charge, float4* __restrict__ out,
+                             int* __restrict__ status)
+{
+    pack_atoms_body(n, crd, charge, out, status, blockIdx.x);
+}
+
+struct PackAtomsArgs
+{
+    int n;
+    const snb_vector* crd;
+    const float* charge;
+    float4* out;
+    int* status;
+};
+
+__global__ void k_pack_atoms_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const PackAtomsArgs& a = snb_seg<PackAtomsArgs>(t, bid, nbid);
+    pack_atoms_body(a.n, a.crd, a.charge, a.out, a.status, bid);
+}
+
 int snb_launch_pack_atoms(snb_context* ctx, const snb_vector* crd, const float* charge)
 {
     int n = ctx->n;
     if (n <= 0) return SNB_OK;
-    hipLaunchKernelGGL(k_pack_atoms, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, crd,
-                       charge, ctx->atom4, ctx->status);
+    SNB_LAUNCH_B(k_pack_atoms, k_pack_atoms_batch, PackAtomsArgs, dim3((n + 255) / 256), dim3(256),
+                 0, ctx->stream, n, crd, charge, ctx->atom4, ctx->status);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -43,7 +67,7 @@ __global__ void k_fill_i64(int64_t* p, size_t n)
 int snb_launch_fill_int(hipStream_t s, int* p, int n, int v)
 {
     if (n <= 0) return SNB_OK;
-    hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    SNB_LAUNCH(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -58,8 +82,7 @@ __global__ void k_add_i64(const int64_t* __restrict__ a, const int64_t* __restri
 int snb_launch_add_i64(hipStream_t s, const int64_t* a, const int64_t* b, int64_t* out, size_t n)
 {
     if (n == 0) return SNB_OK;
-    hipLaunchKernelGGL(k_add_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, out,
-                       n);
+    SNB_LAUNCH(k_add_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, out, n);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -67,7 +90,7 @@ int snb_launch_add_i64(hipStream_t s, const int64_t* a, const int64_t* b, int64_
 int snb_launch_fill_i64(hipStream_t s, int64_t* p, size_t n)
 {
     if (n == 0) return SNB_OK;
-    hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    SNB_LAUNCH(k_fill_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -77,8 +100,8 @@ int snb_launch_fill_i64(hipStream_t s, int64_t* p, size_t n)
 // cell and brick offsets (n <= a few 1e5).  CLEAR zeroes the input after use
 // (counters that must start the next call at zero).
 template <bool CLEAR>
-__global__ void __launch_bounds__(1024) k_exclusive_scan(int* __restrict__ in,
-                                                         int* __restrict__ out, int n)
+__device__ __forceinline__ void exclusive_scan_body(int* __restrict__ in, int* __restrict__ out,
+                                                    int n)
 {
     __shared__ int part[1024];
     int t = threadIdx.x;
@@ -106,16 +129,41 @@ __global__ void __launch_bounds__(1024) k_exclusive_scan(int* __restrict__ in,
     if (t == 1023) out[n] = part[1023];
 }
 
+template <bool CLEAR>
+__global__ void __launch_bounds__(1024) k_exclusive_scan(int* __restrict__ in,
+                                                         int* __restrict__ out, int n)
+{
+    exclusive_scan_body<CLEAR>(in, out, n);
+}
+
+struct ScanArgs
+{
+    int* in;
+    int* out;
+    int n;
+};
+
+// one workgroup per context
+template <bool CLEAR>
+__global__ void __launch_bounds__(1024) k_exclusive_scan_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const ScanArgs& a = snb_seg<ScanArgs>(t, bid, nbid);
+    exclusive_scan_body<CLEAR>(a.in, a.out, a.n);
+}
+
 int snb_launch_exclusive_scan(hipStream_t s, const int* in, int* out, int n)
 {
-    hipLaunchKernelGGL(k_exclusive_scan<false>, dim3(1), dim3(1024), 0, s, (int*)in, out, n);
+    SNB_LAUNCH_B(k_exclusive_scan<false>, k_exclusive_scan_batch<false>, ScanArgs, dim3(1),
+                 dim3(1024), 0, s, (int*)in, out, n);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
 
 int snb_launch_exclusive_scan_clear(hipStream_t s, int* in, int* out, int n)
 {
-    hipLaunchKernelGGL(k_exclusive_scan<true>, dim3(1), dim3(1024), 0, s, in, out, n);
+    SNB_LAUNCH_B(k_exclusive_scan<true>, k_exclusive_scan_batch<true>, ScanArgs, dim3(1),
+                 dim3(1024), 0, s, in, out, n);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -145,8 +193,8 @@ int snb_launch_flush_forces(snb_context* ctx, snb_vector* frc)
     if (n <= 0) return SNB_OK;
     int rc = snb_join_pme(ctx);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_flush_forces, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
-                       ctx->fx, ctx->fx_pme, frc);
+    SNB_LAUNCH(k_flush_forces, dim3((n + 255) / 256), dim3(256), 0,
+               ctx->stream, n, ctx->fx, ctx->fx_pme, frc);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -171,8 +219,8 @@ int snb_launch_flush_atom_energy(snb_context* ctx, float* atom_energy, float* di
 {
     int n = ctx->n;
     if (n <= 0 || (!atom_energy && !direct)) return SNB_OK;
-    hipLaunchKernelGGL(k_flush_atom_energy, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
-                       n, ctx->e_atom, atom_energy, direct, which_mask);
+    SNB_LAUNCH(k_flush_atom_energy, dim3((n + 255) / 256), dim3(256), 0,
+               ctx->stream, n, ctx->e_atom, atom_energy, direct, which_mask);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -197,8 +245,8 @@ int snb_launch_flush_virial(snb_context* ctx, snb_ltmatrix3* atom_virial)
 {
     int n = ctx->n;
     if (n <= 0 || !atom_virial) return SNB_OK;
-    hipLaunchKernelGGL(k_flush_virial, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
-                       ctx->vir_atom, atom_virial);
+    SNB_LAUNCH(k_flush_virial, dim3((n + 255) / 256), dim3(256), 0,
+               ctx->stream, n, ctx->vir_atom, atom_virial);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -217,12 +265,11 @@ __global__ void k_sum_fixed(const int64_t* __restrict__ a, int n, int64_t* __res
 
 int snb_launch_sum_fixed(snb_context* ctx, const int64_t* arr, int n, int slot)
 {
-    hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctx->scal + slot, (size_t)1);
+    SNB_LAUNCH(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctx->scal + slot, (size_t)1);
     if (n <= 0) return SNB_OK;
     int blocks = (n + 1023) / 1024;
     if (blocks > 256) blocks = 256;
-    hipLaunchKernelGGL(k_sum_fixed, dim3(blocks), dim3(256), 0, ctx->stream, arr, n,
-                       ctx->scal + slot);
+    SNB_LAUNCH(k_sum_fixed, dim3(blocks), dim3(256), 0, ctx->stream, arr, n, ctx->scal + slot);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -266,14 +313,12 @@ __global__ void k_charge_sums(int n, const float* __restrict__ q, int64_t* __res
 
 int snb_launch_charge_sums(snb_context* ctx, const float* charge)
 {
-    hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctx->scal + SNB_S_Q2,
-                       (size_t)2);
+    SNB_LAUNCH(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctx->scal + SNB_S_Q2, (size_t)2);
     int n = ctx->n;
     if (n <= 0) return SNB_OK;
     int blocks = (n + 1023) / 1024;
     if (blocks > 256) blocks = 256;
-    hipLaunchKernelGGL(k_charge_sums, dim3(blocks), dim3(256), 0, ctx->stream, n, charge,
-                       ctx->scal);
+    SNB_LAUNCH(k_charge_sums, dim3(blocks), dim3(256), 0, ctx->stream, n, charge, ctx->scal);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -282,7 +327,7 @@ __global__ void k_add_scalar(float* dst, float v) { dst[0] += v; }
 
 int snb_launch_add_scalar(hipStream_t s, float* dst, float v)
 {
-    hipLaunchKernelGGL(k_add_scalar, dim3(1), dim3(1), 0, s, dst, v);
+    SNB_LAUNCH(k_add_scalar, dim3(1), dim3(1), 0, s, dst, v);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -302,7 +347,7 @@ __global__ void k_lj_long_range(snb_ltmatrix3* vir, float* pot, float vir_add, f
 int snb_launch_lj_long_range(hipStream_t s, snb_ltmatrix3* vir, float* pot, float vir_add,
                              float pot_add)
 {
-    hipLaunchKernelGGL(k_lj_long_range, dim3(1), dim3(1), 0, s, vir, pot, vir_add, pot_add);
+    SNB_LAUNCH(k_lj_long_range, dim3(1), dim3(1), 0, s, vir, pot, vir_add, pot_add);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
+#include <tuple>
 #include <vector>
 
 #include "../../include/sponge_nb.h"
@@ -295,6 +297,157 @@ void snb_prof_end(snb_context* ctx, int group);
     {                                             \
         hipError_t e_ = (x);                      \
         if (e_ != hipSuccess) return SNB_ERR_HIP; \
+    } while (0)
+
+// Makes `dev` current for the lifetime of the guard.
+struct DeviceGuard
+{
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int cur;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Launch recording (snb_batch.hip).  While a context's steps are recorded,
+// the launchers below store each kernel launch instead of making it: `solo`
+// makes it exactly as the single-context path would, on a given stream, with
+// the arguments evaluated at record time.  A kernel that also has a batched
+// form carries that form and its argument block, so the same launch of
+// several contexts can be issued as one dispatch over a SnbSegTab.
+// ---------------------------------------------------------------------------
+// Lanes: the reciprocal pipeline (lane 1) may run beside the direct-space
+// work (lane 0), as on a context's second stream.  It reads the coordinates
+// and charges and writes only its own buffers and accumulator, so it waits
+// for the last launch that writes coordinates (`writes_inputs`), and a lane-0
+// launch that follows reciprocal launches (`join`: the iteration reads the
+// reciprocal forces) waits for them.
+struct SnbRecOp
+{
+    std::function<void(hipStream_t)> solo;
+    const void* bfn = nullptr;  // batched kernel, or nullptr
+    std::vector<char> args;     // its argument block for this context
+    unsigned grid = 0, block = 0;
+    size_t lds = 0;
+    int lane = 0;
+    bool join = false, writes_inputs = false;
+};
+struct SnbRecorder
+{
+    std::vector<SnbRecOp> ops;
+    int lane = 0;                // lane of the launches being recorded
+    bool writes_inputs = false;  // they write coordinates
+    int prev_lane = 0;
+};
+// the recorder of the step being recorded on this thread, else nullptr
+extern thread_local SnbRecorder* snb_rec;
+
+static inline void snb_rec_push(SnbRecOp&& op)
+{
+    op.lane = snb_rec->lane;
+    op.join = op.lane == 0 && snb_rec->prev_lane == 1;
+    op.writes_inputs = snb_rec->writes_inputs;
+    snb_rec->prev_lane = op.lane;
+    snb_rec->ops.push_back(std::move(op));
+}
+
+// Sets the recorder's lane / coordinate-writer mark for the scope.
+struct SnbRecScope
+{
+    int lane0 = 0;
+    bool w0 = false;
+    SnbRecScope(int lane, bool writes_inputs)
+    {
+        if (!snb_rec) return;
+        lane0 = snb_rec->lane;
+        w0 = snb_rec->writes_inputs;
+        snb_rec->lane = lane;
+        snb_rec->writes_inputs = writes_inputs;
+    }
+    ~SnbRecScope()
+    {
+        if (!snb_rec) return;
+        snb_rec->lane = lane0;
+        snb_rec->writes_inputs = w0;
+    }
+};
+
+// Argument table of a batched launch over nseg contexts: workgroup b belongs
+// to the context s with blk0[s] <= b < blk0[s + 1] and is its workgroup
+// b - blk0[s] of blk0[s + 1] - blk0[s]; that context's argument block is
+// args + s * stride.
+struct SnbSegTab
+{
+    const char* args;
+    const int* blk0;
+    int nseg, stride;
+};
+
+template <class A>
+__device__ __forceinline__ const A& snb_seg(const SnbSegTab& t, unsigned& bid, unsigned& nbid)
+{
+    const int b = (int)blockIdx.x;
+    int s = 0;
+    while (s + 1 < t.nseg && t.blk0[s + 1] <= b) s++;
+    bid = (unsigned)(b - t.blk0[s]);
+    nbid = (unsigned)(t.blk0[s + 1] - t.blk0[s]);
+    return *(const A*)(t.args + (size_t)s * t.stride);
+}
+
+// A launch of a kernel without a batched form.
+#define SNB_LAUNCH(K, G, B, LDS, S, ...)                                                  \
+    do                                                                                    \
+    {                                                                                     \
+        if (snb_rec)                                                                      \
+        {                                                                                 \
+            const dim3 g_ = (G), b_ = (B);                                                \
+            const size_t l_ = (LDS);                                                      \
+            auto t_ = std::make_tuple(__VA_ARGS__);                                       \
+            SnbRecOp op_;                                                                 \
+            op_.solo = [=](hipStream_t s_) {                                              \
+                std::apply([&](auto... a_) { hipLaunchKernelGGL(K, g_, b_, l_, s_, a_...); }, \
+                           t_);                                                           \
+            };                                                                            \
+            snb_rec_push(std::move(op_));                                                 \
+        }                                                                                 \
+        else                                                                              \
+            hipLaunchKernelGGL(K, G, B, LDS, S, __VA_ARGS__);                             \
+    } while (0)
+
+// A launch of kernel K whose batched form KB takes a SnbSegTab of argument
+// blocks A, aggregate-initialised from the same arguments (1-D grids only).
+#define SNB_LAUNCH_B(K, KB, A, G, B, LDS, S, ...)                                         \
+    do                                                                                    \
+    {                                                                                     \
+        if (snb_rec)                                                                      \
+        {                                                                                 \
+            const dim3 g_ = (G), b_ = (B);                                                \
+            const size_t l_ = (LDS);                                                      \
+            auto t_ = std::make_tuple(__VA_ARGS__);                                       \
+            const A a_{__VA_ARGS__};                                                      \
+            SnbRecOp op_;                                                                 \
+            op_.solo = [=](hipStream_t s_) {                                              \
+                std::apply([&](auto... a_) { hipLaunchKernelGGL(K, g_, b_, l_, s_, a_...); }, \
+                           t_);                                                           \
+            };                                                                            \
+            if (g_.y == 1 && g_.z == 1 && b_.y == 1 && b_.z == 1)                         \
+            {                                                                             \
+                op_.bfn = (const void*)(KB);                                              \
+                op_.args.assign((const char*)&a_, (const char*)&a_ + sizeof(A));          \
+                op_.grid = g_.x;                                                          \
+                op_.block = b_.x;                                                         \
+                op_.lds = l_;                                                             \
+            }                                                                             \
+            snb_rec_push(std::move(op_));                                                 \
+        }                                                                                 \
+        else                                                                              \
+            hipLaunchKernelGGL(K, G, B, LDS, S, __VA_ARGS__);                             \
     } while (0)
 
 

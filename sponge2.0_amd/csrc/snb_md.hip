@@ -300,16 +300,17 @@ __global__ void __launch_bounds__(256) k_settle(int m, const int4* __restrict__ 
 // the fixed-point accumulators (direct + reciprocal, converted as a flush into
 // a zeroed force array converts them) and are cleared.  Thread t < ntri owns
 // triangle t, the next nfree threads the free atoms.
-__global__ void __launch_bounds__(256) k_md_fused(
+__device__ __forceinline__ void k_md_fused_body(
     int ntri, const int4* __restrict__ tri, const float* __restrict__ geo,
     const float* __restrict__ mass, int nfree, const int* __restrict__ free_atoms, int n,
     int64_t* __restrict__ fx, int64_t* __restrict__ fx_pme, const float* __restrict__ inv_mass,
     const float* __restrict__ sqrt_mass, uint32_t* __restrict__ counter, uint32_t seed,
     snb_vector* __restrict__ vel, snb_vector* __restrict__ crd, SnbBox box, float dt,
     float half_dt, float exp_gamma, float vmax, float v_factor, float x_factor, int need_virial,
-    int64_t* __restrict__ vir6)
+    int64_t* __restrict__ vir6,
+    unsigned bid, unsigned nbid)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = bid * blockDim.x + threadIdx.x;
     int64_t w[6] = {0, 0, 0, 0, 0, 0};
     auto force_of = [&](int i) {
         snb_vector f;
@@ -357,6 +358,59 @@ __global__ void __launch_bounds__(256) k_md_fused(
     if (need_virial) settle_virial_sum(w, vir6);
 }
 
+__global__ void __launch_bounds__(256) k_md_fused(
+    int ntri, const int4* __restrict__ tri, const float* __restrict__ geo,
+    const float* __restrict__ mass, int nfree, const int* __restrict__ free_atoms, int n,
+    int64_t* __restrict__ fx, int64_t* __restrict__ fx_pme, const float* __restrict__ inv_mass,
+    const float* __restrict__ sqrt_mass, uint32_t* __restrict__ counter, uint32_t seed,
+    snb_vector* __restrict__ vel, snb_vector* __restrict__ crd, SnbBox box, float dt,
+    float half_dt, float exp_gamma, float vmax, float v_factor, float x_factor, int need_virial,
+    int64_t* __restrict__ vir6)
+{
+    k_md_fused_body(
+        ntri, tri, geo, mass, nfree, free_atoms, n, fx, fx_pme, inv_mass, sqrt_mass, counter, seed,
+        vel, crd, box, dt, half_dt, exp_gamma, vmax, v_factor, x_factor, need_virial, vir6,
+        blockIdx.x, gridDim.x);
+}
+
+struct MdFusedArgs
+{
+    int ntri;
+    const int4* tri;
+    const float* geo;
+    const float* mass;
+    int nfree;
+    const int* free_atoms;
+    int n;
+    int64_t* fx;
+    int64_t* fx_pme;
+    const float* inv_mass;
+    const float* sqrt_mass;
+    uint32_t* counter;
+    uint32_t seed;
+    snb_vector* vel;
+    snb_vector* crd;
+    SnbBox box;
+    float dt;
+    float half_dt;
+    float exp_gamma;
+    float vmax;
+    float v_factor;
+    float x_factor;
+    int need_virial;
+    int64_t* vir6;
+};
+
+__global__ void __launch_bounds__(256) k_md_fused_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const MdFusedArgs& a = snb_seg<MdFusedArgs>(t, bid, nbid);
+    k_md_fused_body(
+        a.ntri, a.tri, a.geo, a.mass, a.nfree, a.free_atoms, a.n, a.fx, a.fx_pme, a.inv_mass,
+        a.sqrt_mass, a.counter, a.seed, a.vel, a.crd, a.box, a.dt, a.half_dt, a.exp_gamma, a.vmax,
+        a.v_factor, a.x_factor, a.need_virial, a.vir6, bid, nbid);
+}
+
 // stress += (1/V) * summed virial (Sum_Virial_Tensor_To_Stress, settle.cpp:307-340);
 // clears the sums
 __global__ void k_settle_stress(int64_t* __restrict__ vir6, float inv_volume,
@@ -377,10 +431,11 @@ __global__ void k_settle_stress(int64_t* __restrict__ vir6, float inv_volume,
 int snb_launch_leap_frog(snb_context* ctx, const float* inv_mass, const snb_vector* frc,
                          snb_vector* vel, snb_vector* crd, float dt, float vmax)
 {
+    SnbRecScope mark(0, true);  // writes coordinates (recorded step batches)
     const int n = ctx->n;
     if (n <= 0) return SNB_OK;
-    hipLaunchKernelGGL(k_leap_frog, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
-                       inv_mass, frc, vel, crd, dt, vmax);
+    SNB_LAUNCH(k_leap_frog, dim3((n + 255) / 256), dim3(256), 0,
+               ctx->stream, n, inv_mass, frc, vel, crd, dt, vmax);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -388,12 +443,12 @@ int snb_launch_leap_frog(snb_context* ctx, const float* inv_mass, const snb_vect
 int snb_launch_langevin(snb_context* ctx, const snb_vector* frc, snb_vector* vel,
                         snb_vector* crd)
 {
+    SnbRecScope mark(0, true);  // writes coordinates (recorded step batches)
     const SnbLangevin& T = ctx->langevin;
     const int n = ctx->n;
     if (n <= 0) return SNB_OK;
-    hipLaunchKernelGGL(k_langevin, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
-                       T.inv_mass, T.sqrt_mass, T.counter, T.seed, frc, vel, crd, T.dt,
-                       T.half_dt, T.exp_gamma, T.max_velocity);
+    SNB_LAUNCH(k_langevin, dim3((n + 255) / 256), dim3(256), 0,
+               ctx->stream, n, T.inv_mass, T.sqrt_mass, T.counter, T.seed, frc, vel, crd, T.dt, T.half_dt, T.exp_gamma, T.max_velocity);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -402,8 +457,8 @@ int snb_launch_settle_remember(snb_context* ctx, const snb_vector* crd, const Sn
 {
     const SnbSettle& S = ctx->settle;
     if (S.count <= 0) return SNB_OK;
-    hipLaunchKernelGGL(k_settle_remember, dim3((S.count + 255) / 256), dim3(256), 0,
-                       ctx->stream, S.count, S.tri, crd, box, S.last_ba, S.last_ca);
+    SNB_LAUNCH(k_settle_remember, dim3((S.count + 255) / 256), dim3(256), 0,
+               ctx->stream, S.count, S.tri, crd, box, S.last_ba, S.last_ca);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -412,19 +467,18 @@ int snb_launch_settle(snb_context* ctx, const float* mass, snb_vector* crd, cons
                       snb_vector* vel, int need_pressure, snb_ltmatrix3* stress, float dt,
                       float v_factor, float x_factor)
 {
+    SnbRecScope mark(0, true);  // writes coordinates (recorded step batches)
     const SnbSettle& S = ctx->settle;
     if (S.count <= 0) return SNB_OK;
     const int want = need_pressure && stress ? 1 : 0;
-    hipLaunchKernelGGL(k_settle, dim3((S.count + 255) / 256), dim3(256), 0, ctx->stream,
-                       S.count, S.tri, S.geo, mass, crd, box, S.last_ba, S.last_ca, dt,
-                       v_factor, x_factor, vel, want, S.vir6);
+    SNB_LAUNCH(k_settle, dim3((S.count + 255) / 256), dim3(256), 0,
+               ctx->stream, S.count, S.tri, S.geo, mass, crd, box, S.last_ba, S.last_ca, dt, v_factor, x_factor, vel, want, S.vir6);
     SNB_CHECK(hipGetLastError());
     if (want)
     {
         // rcell a11 a22 a33 is 1/V for the lower-triangular cell
         const float inv_volume = (box.rc[0] * box.rc[2]) * box.rc[5];
-        hipLaunchKernelGGL(k_settle_stress, dim3(1), dim3(64), 0, ctx->stream, S.vir6,
-                           inv_volume, stress);
+        SNB_LAUNCH(k_settle_stress, dim3(1), dim3(64), 0, ctx->stream, S.vir6, inv_volume, stress);
         SNB_CHECK(hipGetLastError());
     }
     return SNB_OK;
@@ -548,7 +602,7 @@ int snb_launch_stress(snb_context* ctx, int n, int pot, int kin, const snb_ltmat
     {
         const int blocks = (n + 255) / 256 < 512 ? (n + 255) / 256 : 512;
 #define SNB_STRESS_SUMS(P, K)                                                                     \
-    hipLaunchKernelGGL((k_stress_sums<P, K>), dim3(blocks), dim3(256), 0, ctx->stream, n,      \
+    SNB_LAUNCH((k_stress_sums<P, K>), dim3(blocks), dim3(256), 0, ctx->stream, n,      \
                        atom_virial, vel, mass, acc)
         if (pot && kin)
             SNB_STRESS_SUMS(true, true);
@@ -560,14 +614,14 @@ int snb_launch_stress(snb_context* ctx, int n, int pot, int kin, const snb_ltmat
         SNB_CHECK(hipGetLastError());
     }
     if (pot && kin)
-        hipLaunchKernelGGL((k_stress_finish<true, true>), dim3(1), dim3(64), 0, ctx->stream, acc,
-                           inv_volume, virial_tensor, stress);
+        SNB_LAUNCH((k_stress_finish<true, true>), dim3(1), dim3(64), 0,
+                   ctx->stream, acc, inv_volume, virial_tensor, stress);
     else if (pot)
-        hipLaunchKernelGGL((k_stress_finish<true, false>), dim3(1), dim3(64), 0, ctx->stream,
-                           acc, inv_volume, virial_tensor, stress);
+        SNB_LAUNCH((k_stress_finish<true, false>), dim3(1), dim3(64), 0,
+                   ctx->stream, acc, inv_volume, virial_tensor, stress);
     else
-        hipLaunchKernelGGL((k_stress_finish<false, true>), dim3(1), dim3(64), 0, ctx->stream,
-                           acc, inv_volume, virial_tensor, stress);
+        SNB_LAUNCH((k_stress_finish<false, true>), dim3(1), dim3(64), 0,
+                   ctx->stream, acc, inv_volume, virial_tensor, stress);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -575,6 +629,7 @@ int snb_launch_stress(snb_context* ctx, int n, int pot, int kin, const snb_ltmat
 int snb_launch_md_fused(snb_context* ctx, const float* mass, snb_vector* crd, const SnbBox& box,
                         snb_vector* vel, int need_pressure, snb_ltmatrix3* stress)
 {
+    SnbRecScope mark(0, true);  // writes coordinates (recorded step batches)
     const SnbLangevin& T = ctx->langevin;
     const SnbSettle& S = ctx->settle;
     const int n = ctx->n;
@@ -590,18 +645,15 @@ int snb_launch_md_fused(snb_context* ctx, const float* mass, snb_vector* crd, co
     const float v_factor = T.exp_gamma, x_factor = 0.5f * T.exp_gamma + 0.5f;
     if (total > 0)
     {
-        hipLaunchKernelGGL(k_md_fused, dim3((total + 255) / 256), dim3(256), 0, ctx->stream, ntri,
-                           S.tri, S.geo, mass, nfree, free_atoms, n, ctx->fx, ctx->fx_pme,
-                           T.inv_mass, T.sqrt_mass, T.counter, T.seed, vel, crd, box, T.dt,
-                           T.half_dt, T.exp_gamma, T.max_velocity, v_factor, x_factor, want,
-                           S.vir6);
+        SNB_LAUNCH_B(k_md_fused, k_md_fused_batch, MdFusedArgs, dim3((total + 255) / 256),
+                     dim3(256), 0,
+                     ctx->stream, ntri, S.tri, S.geo, mass, nfree, free_atoms, n, ctx->fx, ctx->fx_pme, T.inv_mass, T.sqrt_mass, T.counter, T.seed, vel, crd, box, T.dt, T.half_dt, T.exp_gamma, T.max_velocity, v_factor, x_factor, want, S.vir6);
         SNB_CHECK(hipGetLastError());
     }
     if (want)
     {
         const float inv_volume = (box.rc[0] * box.rc[2]) * box.rc[5];
-        hipLaunchKernelGGL(k_settle_stress, dim3(1), dim3(64), 0, ctx->stream, S.vir6,
-                           inv_volume, stress);
+        SNB_LAUNCH(k_settle_stress, dim3(1), dim3(64), 0, ctx->stream, S.vir6, inv_volume, stress);
         SNB_CHECK(hipGetLastError());
     }
     return SNB_OK;

@@ -36,25 +36,66 @@ __global__ void k_nl_check(int n, const snb_vector* __restrict__ crd,
     if ((dx * dx + dy * dy) + dz * dz > permit2) need[0] = 1;
 }
 
-__global__ void k_nl_set(int* need, int v) { need[0] = v; }
+__device__ __forceinline__ void k_nl_set_body(int* need, int v,
+    unsigned bid, unsigned nbid)
+{ need[0] = v; }
+
+__global__ void k_nl_set(int* need, int v)
+{
+    k_nl_set_body(need, v, blockIdx.x, gridDim.x);
+}
+
+struct NlSetArgs
+{
+    int* need;
+    int v;
+};
+
+__global__ void k_nl_set_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const NlSetArgs& a = snb_seg<NlSetArgs>(t, bid, nbid);
+    k_nl_set_body(a.need, a.v, bid, nbid);
+}
+
+__device__ __forceinline__ void k_nl_clear_counts_body(const int* __restrict__ need, int ncell, int* __restrict__ cnt,
+    unsigned bid, unsigned nbid)
+{
+    if (need[0] == 0) return;
+    int i = bid * blockDim.x + threadIdx.x;
+    if (i <= ncell) cnt[i] = 0;
+}
 
 __global__ void k_nl_clear_counts(const int* __restrict__ need, int ncell, int* __restrict__ cnt)
 {
-    if (need[0] == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i <= ncell) cnt[i] = 0;
+    k_nl_clear_counts_body(need, ncell, cnt, blockIdx.x, gridDim.x);
+}
+
+struct NlClearArgs
+{
+    const int* need;
+    int ncell;
+    int* cnt;
+};
+
+__global__ void k_nl_clear_counts_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const NlClearArgs& a = snb_seg<NlClearArgs>(t, bid, nbid);
+    k_nl_clear_counts_body(a.need, a.ncell, a.cnt, bid, nbid);
 }
 
 // Put_Atom_In_Grids (neighbor_list.cpp:206-275): the wrap into the cell and
 // the cell index use the reference's float operations; the slot inside the
 // cell comes from an atomic counter and is made deterministic later.
-__global__ void k_nl_bin(const int* __restrict__ need, int n, const snb_vector* __restrict__ crd,
+__device__ __forceinline__ void k_nl_bin_body(const int* __restrict__ need, int n, const snb_vector* __restrict__ crd,
                          snb_vector* __restrict__ old, int copy_old, SnbBox box,
                          float grid_length, int nx, int ny, int nz, int* __restrict__ atom_cell,
-                         int* __restrict__ atom_slot, int* __restrict__ cnt)
+                         int* __restrict__ atom_slot, int* __restrict__ cnt,
+    unsigned bid, unsigned nbid)
 {
     if (need[0] == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     snb_vector r = crd[i];
     if (copy_old) old[i] = r;
@@ -76,14 +117,75 @@ __global__ void k_nl_bin(const int* __restrict__ need, int n, const snb_vector* 
     atom_slot[i] = atomicAdd(&cnt[c], 1);
 }
 
+__global__ void k_nl_bin(const int* __restrict__ need, int n, const snb_vector* __restrict__ crd,
+                         snb_vector* __restrict__ old, int copy_old, SnbBox box,
+                         float grid_length, int nx, int ny, int nz, int* __restrict__ atom_cell,
+                         int* __restrict__ atom_slot, int* __restrict__ cnt)
+{
+    k_nl_bin_body(
+        need, n, crd, old, copy_old, box, grid_length, nx, ny, nz, atom_cell, atom_slot, cnt,
+        blockIdx.x, gridDim.x);
+}
+
+struct NlBinArgs
+{
+    const int* need;
+    int n;
+    const snb_vector* crd;
+    snb_vector* old;
+    int copy_old;
+    SnbBox box;
+    float grid_length;
+    int nx;
+    int ny;
+    int nz;
+    int* atom_cell;
+    int* atom_slot;
+    int* cnt;
+};
+
+__global__ void k_nl_bin_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const NlBinArgs& a = snb_seg<NlBinArgs>(t, bid, nbid);
+    k_nl_bin_body(
+        a.need, a.n, a.crd, a.old, a.copy_old, a.box, a.grid_length, a.nx, a.ny, a.nz, a.atom_cell,
+        a.atom_slot, a.cnt, bid, nbid);
+}
+
+__device__ __forceinline__ void k_nl_scatter_body(const int* __restrict__ need, int n, const int* __restrict__ atom_cell,
+                             const int* __restrict__ atom_slot, const int* __restrict__ start,
+                             int* __restrict__ sorted_id,
+    unsigned bid, unsigned nbid)
+{
+    if (need[0] == 0) return;
+    int i = bid * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sorted_id[start[atom_cell[i]] + atom_slot[i]] = i;
+}
+
 __global__ void k_nl_scatter(const int* __restrict__ need, int n, const int* __restrict__ atom_cell,
                              const int* __restrict__ atom_slot, const int* __restrict__ start,
                              int* __restrict__ sorted_id)
 {
-    if (need[0] == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    sorted_id[start[atom_cell[i]] + atom_slot[i]] = i;
+    k_nl_scatter_body(need, n, atom_cell, atom_slot, start, sorted_id, blockIdx.x, gridDim.x);
+}
+
+struct NlScatterArgs
+{
+    const int* need;
+    int n;
+    const int* atom_cell;
+    const int* atom_slot;
+    const int* start;
+    int* sorted_id;
+};
+
+__global__ void k_nl_scatter_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const NlScatterArgs& a = snb_seg<NlScatterArgs>(t, bid, nbid);
+    k_nl_scatter_body(a.need, a.n, a.atom_cell, a.atom_slot, a.start, a.sorted_id, bid, nbid);
 }
 
 // One wave per cell: rank the cell's atoms by id (O(m^2) with m ~ 20-60) and
@@ -91,18 +193,19 @@ __global__ void k_nl_scatter(const int* __restrict__ need, int n, const int* __r
 // the reference's per-cell capacity (max_atom_in_grid_numbers) for parity of
 // the overflow report; nothing is truncated here.
 #define SNB_NL_SORT_MAX 1024
-__global__ void __launch_bounds__(64) k_nl_cell_sort(const int* __restrict__ need, int ncell,
+__device__ __forceinline__ void k_nl_cell_sort_body(const int* __restrict__ need, int ncell,
                                                      const int* __restrict__ start,
                                                      const int* __restrict__ sorted_id,
                                                      const snb_vector* __restrict__ crd,
                                                      float4* __restrict__ sorted_crd,
                                                      int* __restrict__ order,
                                                      int* __restrict__ sorted_cell,
-                                                     int max_in_cell, int* __restrict__ status)
+                                                     int max_in_cell, int* __restrict__ status,
+    unsigned bid, unsigned nbid)
 {
     if (need[0] == 0) return;
     __shared__ int ids[SNB_NL_SORT_MAX];
-    int c = blockIdx.x;
+    int c = bid;
     if (c >= ncell) return;
     int b = start[c], m = start[c + 1] - b;
     int lane = threadIdx.x;
@@ -134,6 +237,43 @@ __global__ void __launch_bounds__(64) k_nl_cell_sort(const int* __restrict__ nee
     }
 }
 
+__global__ void __launch_bounds__(64) k_nl_cell_sort(const int* __restrict__ need, int ncell,
+                                                     const int* __restrict__ start,
+                                                     const int* __restrict__ sorted_id,
+                                                     const snb_vector* __restrict__ crd,
+                                                     float4* __restrict__ sorted_crd,
+                                                     int* __restrict__ order,
+                                                     int* __restrict__ sorted_cell,
+                                                     int max_in_cell, int* __restrict__ status)
+{
+    k_nl_cell_sort_body(
+        need, ncell, start, sorted_id, crd, sorted_crd, order, sorted_cell, max_in_cell, status,
+        blockIdx.x, gridDim.x);
+}
+
+struct NlCellSortArgs
+{
+    const int* need;
+    int ncell;
+    const int* start;
+    const int* sorted_id;
+    const snb_vector* crd;
+    float4* sorted_crd;
+    int* order;
+    int* sorted_cell;
+    int max_in_cell;
+    int* status;
+};
+
+__global__ void __launch_bounds__(64) k_nl_cell_sort_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const NlCellSortArgs& a = snb_seg<NlCellSortArgs>(t, bid, nbid);
+    k_nl_cell_sort_body(
+        a.need, a.ncell, a.start, a.sorted_id, a.crd, a.sorted_crd, a.order, a.sorted_cell,
+        a.max_in_cell, a.status, bid, nbid);
+}
+
 // The three axis offset sets of the +-2 stencil: five distinct cells when the
 // axis has >= 5 cells, otherwise every cell of the axis once (the reference
 // de-duplicates the same way, neighbor_list.cpp:5-49).
@@ -155,17 +295,18 @@ __device__ __forceinline__ int stencil_cells(int c, int ncell_axis, int* out)
 }
 
 // Row builder: one wavefront per atom i (rows in atom order).
-__global__ void __launch_bounds__(256) k_nl_search(
+__device__ __forceinline__ void k_nl_search_body(
     const int* __restrict__ need, int n, const snb_vector* __restrict__ crd,
     const int* __restrict__ atom_cell, const int* __restrict__ start,
     const float4* __restrict__ sorted_crd, SnbBox box, float cs2, int nx, int ny, int nz,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
     const int* __restrict__ ex_num, int max_nb, int* __restrict__ serial,
-    snb_atom_group* __restrict__ nl, int* __restrict__ status)
+    snb_atom_group* __restrict__ nl, int* __restrict__ status,
+    unsigned bid, unsigned nbid)
 {
     if (need[0] == 0) return;
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int i = bid * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
     snb_vector ri = crd[i];
     int c = atom_cell[i];
@@ -260,6 +401,50 @@ __global__ void __launch_bounds__(256) k_nl_search(
     }
 }
 
+__global__ void __launch_bounds__(256) k_nl_search(
+    const int* __restrict__ need, int n, const snb_vector* __restrict__ crd,
+    const int* __restrict__ atom_cell, const int* __restrict__ start,
+    const float4* __restrict__ sorted_crd, SnbBox box, float cs2, int nx, int ny, int nz,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+    const int* __restrict__ ex_num, int max_nb, int* __restrict__ serial,
+    snb_atom_group* __restrict__ nl, int* __restrict__ status)
+{
+    k_nl_search_body(
+        need, n, crd, atom_cell, start, sorted_crd, box, cs2, nx, ny, nz, ex_start, ex_list, ex_num,
+        max_nb, serial, nl, status, blockIdx.x, gridDim.x);
+}
+
+struct NlSearchArgs
+{
+    const int* need;
+    int n;
+    const snb_vector* crd;
+    const int* atom_cell;
+    const int* start;
+    const float4* sorted_crd;
+    SnbBox box;
+    float cs2;
+    int nx;
+    int ny;
+    int nz;
+    const int* ex_start;
+    const int* ex_list;
+    const int* ex_num;
+    int max_nb;
+    int* serial;
+    snb_atom_group* nl;
+    int* status;
+};
+
+__global__ void __launch_bounds__(256) k_nl_search_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const NlSearchArgs& a = snb_seg<NlSearchArgs>(t, bid, nbid);
+    k_nl_search_body(
+        a.need, a.n, a.crd, a.atom_cell, a.start, a.sorted_crd, a.box, a.cs2, a.nx, a.ny, a.nz,
+        a.ex_start, a.ex_list, a.ex_num, a.max_nb, a.serial, a.nl, a.status, bid, nbid);
+}
+
 int snb_launch_nl_update(snb_context* ctx, const snb_vector* crd, const SnbBox& box, int forced,
                          int step, const int* ex_start, const int* ex_list, const int* ex_num)
 {
@@ -269,41 +454,40 @@ int snb_launch_nl_update(snb_context* ctx, const snb_vector* crd, const SnbBox& 
     snb_prof_begin(ctx, SNB_PROF_NL);
     // NEIGHBOR_LIST::Update policy (neighbor_list.cpp:843-856)
     if (forced)
-        hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, L.need, 1);
+        SNB_LAUNCH_B(k_nl_set, k_nl_set_batch, NlSetArgs, dim3(1), dim3(1), 0, s, L.need, 1);
     else if (L.refresh_interval <= 0)
     {
-        hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, L.need, 0);
+        SNB_LAUNCH_B(k_nl_set, k_nl_set_batch, NlSetArgs, dim3(1), dim3(1), 0, s, L.need, 0);
         float permit2 = L.skin * L.skin * L.skin_permit * L.skin_permit;
-        hipLaunchKernelGGL(k_nl_check, dim3((n + 255) / 256), dim3(256), 0, s, n, crd,
-                           L.old_crd, box, permit2, L.need);
+        SNB_LAUNCH(k_nl_check, dim3((n + 255) / 256), dim3(256), 0, s, n, crd, L.old_crd, box,
+                   permit2, L.need);
     }
     else
-        hipLaunchKernelGGL(k_nl_set, dim3(1), dim3(1), 0, s, L.need,
-                           ((step + 1) % L.refresh_interval == 0) ? 1 : 0);
+        SNB_LAUNCH_B(k_nl_set, k_nl_set_batch, NlSetArgs, dim3(1), dim3(1), 0, s, L.need,
+                     ((step + 1) % L.refresh_interval == 0) ? 1 : 0);
     SNB_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_nl_clear_counts, dim3((L.ncell + 256) / 256), dim3(256), 0, s, L.need,
-                       L.ncell, L.cell_count);
-    hipLaunchKernelGGL(k_nl_bin, dim3((n + 255) / 256), dim3(256), 0, s, L.need, n, crd,
-                       L.old_crd, L.refresh_interval <= 0 ? 1 : 0, box, L.grid_length, L.nx,
-                       L.ny, L.nz, L.atom_cell, L.atom_slot, L.cell_count);
+    SNB_LAUNCH_B(k_nl_clear_counts, k_nl_clear_counts_batch, NlClearArgs,
+                 dim3((L.ncell + 256) / 256), dim3(256), 0, s, L.need, L.ncell, L.cell_count);
+    SNB_LAUNCH_B(k_nl_bin, k_nl_bin_batch, NlBinArgs, dim3((n + 255) / 256), dim3(256), 0, s,
+                 L.need, n, crd, L.old_crd,
+                 L.refresh_interval <= 0 ? 1 : 0, box, L.grid_length, L.nx, L.ny, L.nz, L.atom_cell, L.atom_slot, L.cell_count);
     SNB_CHECK(hipGetLastError());
     int rc = snb_launch_exclusive_scan(s, L.cell_count, L.cell_start, L.ncell);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_nl_scatter, dim3((n + 255) / 256), dim3(256), 0, s, L.need, n,
-                       L.atom_cell, L.atom_slot, L.cell_start, L.sorted_id);
-    hipLaunchKernelGGL(k_nl_cell_sort, dim3(L.ncell), dim3(64), 0, s, L.need, L.ncell,
-                       L.cell_start, L.sorted_id, crd, L.sorted_crd, L.order, L.sorted_cell,
-                       L.max_in_cell, ctx->status);
+    SNB_LAUNCH_B(k_nl_scatter, k_nl_scatter_batch, NlScatterArgs, dim3((n + 255) / 256), dim3(256),
+                 0, s, L.need, n, L.atom_cell, L.atom_slot, L.cell_start, L.sorted_id);
+    SNB_LAUNCH_B(k_nl_cell_sort, k_nl_cell_sort_batch, NlCellSortArgs, dim3(L.ncell), dim3(64), 0,
+                 s, L.need, L.ncell, L.cell_start, L.sorted_id, crd, L.sorted_crd, L.order,
+                 L.sorted_cell, L.max_in_cell, ctx->status);
     if (L.windowed)
     {
         rc = snb_launch_nl_window(ctx, box, ex_start, ex_list, ex_num);
         if (rc) return rc;
     }
     else
-        hipLaunchKernelGGL(k_nl_search, dim3((n + 3) / 4), dim3(256), 0, s, L.need, n, crd,
-                           L.atom_cell, L.cell_start, L.sorted_crd, box, L.cs2, L.nx, L.ny,
-                           L.nz, ex_start, ex_list, ex_num, L.max_nb, L.serial, L.dnl,
-                           ctx->status);
+        SNB_LAUNCH_B(k_nl_search, k_nl_search_batch, NlSearchArgs, dim3((n + 3) / 4), dim3(256), 0,
+                     s, L.need, n, crd, L.atom_cell, L.cell_start, L.sorted_crd, box, L.cs2, L.nx,
+                     L.ny, L.nz, ex_start, ex_list, ex_num, L.max_nb, L.serial, L.dnl, ctx->status);
     snb_prof_end(ctx, SNB_PROF_NL);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;

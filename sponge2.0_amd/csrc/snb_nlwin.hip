@@ -105,13 +105,14 @@ __device__ __forceinline__ void win_block_of_cell(const SnbWin& W, int c, int& c
 // >= 5 cells of >= (rc + skin)/2 per axis that is >= 1.25 (rc + skin), so
 // such a pair fails the test either way.
 template <bool ORTHO>
-__global__ void __launch_bounds__(512) k_nl_build_cell(
+__device__ __forceinline__ void k_nl_build_cell_body(
     const int* __restrict__ need, SnbWin W, const int* __restrict__ start,
     const float4* __restrict__ sorted_crd, SnbBox box, float cs2,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
     const int* __restrict__ ex_num, const int2* __restrict__ ex_rng, int max_nb, float near2,
     uint16_t* __restrict__ nl16, int* __restrict__ row_count, int* __restrict__ row_split,
-    int* __restrict__ status, float grid_length)
+    int* __restrict__ status, float grid_length,
+    unsigned bid, unsigned nbid)
 {
     if (need[0] == 0) return;
     __shared__ int first[SNB_WIN_MAX_RANGES];
@@ -127,7 +128,7 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
     __shared__ int nrun_s;
     __shared__ int rowcnt[SNB_NLC_MAXA];   // near pairs (front of the row)
     __shared__ int rowfar[SNB_NLC_MAXA];   // skin pairs (back of the row)
-    const int c = blockIdx.x;
+    const int c = bid;
     const int cb = start[c], m = start[c + 1] - cb;
     if (m == 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -366,15 +367,61 @@ __global__ void __launch_bounds__(512) k_nl_build_cell(
     }
 }
 
+template <bool ORTHO>
+__global__ void __launch_bounds__(512) k_nl_build_cell(
+    const int* __restrict__ need, SnbWin W, const int* __restrict__ start,
+    const float4* __restrict__ sorted_crd, SnbBox box, float cs2,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list,
+    const int* __restrict__ ex_num, const int2* __restrict__ ex_rng, int max_nb, float near2,
+    uint16_t* __restrict__ nl16, int* __restrict__ row_count, int* __restrict__ row_split,
+    int* __restrict__ status, float grid_length)
+{
+    k_nl_build_cell_body<ORTHO>(
+        need, W, start, sorted_crd, box, cs2, ex_start, ex_list, ex_num, ex_rng, max_nb, near2,
+        nl16, row_count, row_split, status, grid_length, blockIdx.x, gridDim.x);
+}
+
+struct NlBuildCellArgs
+{
+    const int* need;
+    SnbWin W;
+    const int* start;
+    const float4* sorted_crd;
+    SnbBox box;
+    float cs2;
+    const int* ex_start;
+    const int* ex_list;
+    const int* ex_num;
+    const int2* ex_rng;
+    int max_nb;
+    float near2;
+    uint16_t* nl16;
+    int* row_count;
+    int* row_split;
+    int* status;
+    float grid_length;
+};
+
+template <bool ORTHO>
+__global__ void __launch_bounds__(512) k_nl_build_cell_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const NlBuildCellArgs& a = snb_seg<NlBuildCellArgs>(t, bid, nbid);
+    k_nl_build_cell_body<ORTHO>(
+        a.need, a.W, a.start, a.sorted_crd, a.box, a.cs2, a.ex_start, a.ex_list, a.ex_num, a.ex_rng,
+        a.max_nb, a.near2, a.nl16, a.row_count, a.row_split, a.status, a.grid_length, bid, nbid);
+}
+
 // Number of window atoms of every block (one wavefront per block).
-__global__ void __launch_bounds__(64) k_win_sizes(const int* __restrict__ need, SnbWin W,
+__device__ __forceinline__ void k_win_sizes_body(const int* __restrict__ need, SnbWin W,
                                                   const int* __restrict__ start,
-                                                  int* __restrict__ win_size)
+                                                  int* __restrict__ win_size,
+    unsigned bid, unsigned nbid)
 {
     if (need[0] == 0) return;
     __shared__ int first[SNB_WIN_MAX_RANGES];
     __shared__ int pref[SNB_WIN_MAX_RANGES + 1];
-    const int w = blockIdx.x, lane = threadIdx.x;
+    const int w = bid, lane = threadIdx.x;
     int x0, y0, z0;
     snb_win_block(W, w, x0, y0, z0);
     win_table_wave(W, start, x0, y0, z0, lane, first, pref);
@@ -382,19 +429,42 @@ __global__ void __launch_bounds__(64) k_win_sizes(const int* __restrict__ need, 
     if (lane == 0) win_size[w] = pref[SNB_WIN_MAX_RANGES];
 }
 
+__global__ void __launch_bounds__(64) k_win_sizes(const int* __restrict__ need, SnbWin W,
+                                                  const int* __restrict__ start,
+                                                  int* __restrict__ win_size)
+{
+    k_win_sizes_body(need, W, start, win_size, blockIdx.x, gridDim.x);
+}
+
+struct WinSizesArgs
+{
+    const int* need;
+    SnbWin W;
+    const int* start;
+    int* win_size;
+};
+
+__global__ void __launch_bounds__(64) k_win_sizes_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const WinSizesArgs& a = snb_seg<WinSizesArgs>(t, bid, nbid);
+    k_win_sizes_body(a.need, a.W, a.start, a.win_size, bid, nbid);
+}
+
 // For every window slot of every block, record its stage position in the
 // list of the atom it holds (the per-step gather sums these).
-__global__ void __launch_bounds__(256) k_stage_inverse(const int* __restrict__ need, SnbWin W,
+__device__ __forceinline__ void k_stage_inverse_body(const int* __restrict__ need, SnbWin W,
                                                       const int* __restrict__ start,
                                                       const int* __restrict__ stage_base,
                                                       int* __restrict__ inv_count,
                                                       int* __restrict__ inv_list, int invmax,
-                                                      int* __restrict__ status)
+                                                      int* __restrict__ status,
+    unsigned bid, unsigned nbid)
 {
     if (need[0] == 0) return;
     __shared__ int first[SNB_WIN_MAX_RANGES];
     __shared__ int pref[SNB_WIN_MAX_RANGES + 1];
-    int w = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int w = bid, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int x0, y0, z0;
     snb_win_block(W, w, x0, y0, z0);
     if (wave == 0) win_table_wave(W, start, x0, y0, z0, lane, first, pref);
@@ -414,6 +484,37 @@ __global__ void __launch_bounds__(256) k_stage_inverse(const int* __restrict__ n
                 atomicOr(status, SNB_FLAG_NL_OVERFLOW);
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_stage_inverse(const int* __restrict__ need, SnbWin W,
+                                                      const int* __restrict__ start,
+                                                      const int* __restrict__ stage_base,
+                                                      int* __restrict__ inv_count,
+                                                      int* __restrict__ inv_list, int invmax,
+                                                      int* __restrict__ status)
+{
+    k_stage_inverse_body(
+        need, W, start, stage_base, inv_count, inv_list, invmax, status, blockIdx.x, gridDim.x);
+}
+
+struct StageInverseArgs
+{
+    const int* need;
+    SnbWin W;
+    const int* start;
+    const int* stage_base;
+    int* inv_count;
+    int* inv_list;
+    int invmax;
+    int* status;
+};
+
+__global__ void __launch_bounds__(256) k_stage_inverse_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const StageInverseArgs& a = snb_seg<StageInverseArgs>(t, bid, nbid);
+    k_stage_inverse_body(
+        a.need, a.W, a.start, a.stage_base, a.inv_count, a.inv_list, a.invmax, a.status, bid, nbid);
 }
 
 // slot -> sorted index through the LDS range table (binary search)
@@ -725,12 +826,13 @@ __global__ void __launch_bounds__(64) k_nl_copy_internal(int n, SnbWin W,
 // Per atom id: first and last entry of its exclusion list (an empty window
 // (1, 0) when it has none), the window the reference tests before searching
 // the list (neighbor_list.cpp:594-665).
-__global__ void k_ex_range(const int* __restrict__ need, int n, const int* __restrict__ ex_start,
+__device__ __forceinline__ void k_ex_range_body(const int* __restrict__ need, int n, const int* __restrict__ ex_start,
                            const int* __restrict__ ex_list, const int* __restrict__ ex_num,
-                           int2* __restrict__ ex_rng)
+                           int2* __restrict__ ex_rng,
+    unsigned bid, unsigned nbid)
 {
     if (need[0] == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int en = ex_num[i];
     int2 r = make_int2(1, 0);
@@ -748,14 +850,39 @@ __global__ void k_ex_range(const int* __restrict__ need, int n, const int* __res
     ex_rng[i] = r;
 }
 
+__global__ void k_ex_range(const int* __restrict__ need, int n, const int* __restrict__ ex_start,
+                           const int* __restrict__ ex_list, const int* __restrict__ ex_num,
+                           int2* __restrict__ ex_rng)
+{
+    k_ex_range_body(need, n, ex_start, ex_list, ex_num, ex_rng, blockIdx.x, gridDim.x);
+}
+
+struct ExRangeArgs
+{
+    const int* need;
+    int n;
+    const int* ex_start;
+    const int* ex_list;
+    const int* ex_num;
+    int2* ex_rng;
+};
+
+__global__ void k_ex_range_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const ExRangeArgs& a = snb_seg<ExRangeArgs>(t, bid, nbid);
+    k_ex_range_body(a.need, a.n, a.ex_start, a.ex_list, a.ex_num, a.ex_rng, bid, nbid);
+}
+
 // List entries owned by every block (one wavefront per block).
-__global__ void __launch_bounds__(64) k_block_work(const int* __restrict__ need, SnbWin W,
+__device__ __forceinline__ void k_block_work_body(const int* __restrict__ need, SnbWin W,
                                                    const int* __restrict__ start,
                                                    const int* __restrict__ row_count,
-                                                   int* __restrict__ work)
+                                                   int* __restrict__ work,
+    unsigned bid, unsigned nbid)
 {
     if (need[0] == 0) return;
-    const int w = blockIdx.x, lane = threadIdx.x;
+    const int w = bid, lane = threadIdx.x;
     int x0, y0, z0;
     snb_win_block(W, w, x0, y0, z0);
     const int z1 = z0 + W.s < W.nz ? z0 + W.s : W.nz;
@@ -772,13 +899,38 @@ __global__ void __launch_bounds__(64) k_block_work(const int* __restrict__ need,
     if (lane == 0) work[w] = sum;
 }
 
+__global__ void __launch_bounds__(64) k_block_work(const int* __restrict__ need, SnbWin W,
+                                                   const int* __restrict__ start,
+                                                   const int* __restrict__ row_count,
+                                                   int* __restrict__ work)
+{
+    k_block_work_body(need, W, start, row_count, work, blockIdx.x, gridDim.x);
+}
+
+struct BlockWorkArgs
+{
+    const int* need;
+    SnbWin W;
+    const int* start;
+    const int* row_count;
+    int* work;
+};
+
+__global__ void __launch_bounds__(64) k_block_work_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const BlockWorkArgs& a = snb_seg<BlockWorkArgs>(t, bid, nbid);
+    k_block_work_body(a.need, a.W, a.start, a.row_count, a.work, bid, nbid);
+}
+
 // Launch order of the pair kernel's blocks: descending work (ties by block
 // index), one workgroup bitonic sort; identity when there are more blocks
 // than SNB_ORDER_CAP.
 #define SNB_ORDER_CAP 4096
-__global__ void __launch_bounds__(1024) k_block_order(const int* __restrict__ need, int nblock,
+__device__ __forceinline__ void k_block_order_body(const int* __restrict__ need, int nblock,
                                                       const int* __restrict__ work,
-                                                      int* __restrict__ perm)
+                                                      int* __restrict__ perm,
+    unsigned bid, unsigned nbid)
 {
     if (need[0] == 0) return;
     __shared__ long long key[SNB_ORDER_CAP];
@@ -817,20 +969,85 @@ __global__ void __launch_bounds__(1024) k_block_order(const int* __restrict__ ne
         perm[t] = (int)(key[t] & 0xffffffffll);  // low word: the block index (>= 0)
 }
 
+__global__ void __launch_bounds__(1024) k_block_order(const int* __restrict__ need, int nblock,
+                                                      const int* __restrict__ work,
+                                                      int* __restrict__ perm)
+{
+    k_block_order_body(need, nblock, work, perm, blockIdx.x, gridDim.x);
+}
+
+struct BlockOrderArgs
+{
+    const int* need;
+    int nblock;
+    const int* work;
+    int* perm;
+};
+
+__global__ void __launch_bounds__(1024) k_block_order_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const BlockOrderArgs& a = snb_seg<BlockOrderArgs>(t, bid, nbid);
+    k_block_order_body(a.need, a.nblock, a.work, a.perm, bid, nbid);
+}
+
 // Clears p[0 .. n) on rebuild steps only.
-__global__ void k_fill_need(const int* __restrict__ need, int* __restrict__ p, int n)
+__device__ __forceinline__ void k_fill_need_body(const int* __restrict__ need, int* __restrict__ p, int n,
+    unsigned bid, unsigned nbid)
 {
     if (need[0] == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = bid * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+__global__ void k_fill_need(const int* __restrict__ need, int* __restrict__ p, int n)
+{
+    k_fill_need_body(need, p, n, blockIdx.x, gridDim.x);
+}
+
+struct FillNeedArgs
+{
+    const int* need;
+    int* p;
+    int n;
+};
+
+__global__ void k_fill_need_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const FillNeedArgs& a = snb_seg<FillNeedArgs>(t, bid, nbid);
+    k_fill_need_body(a.need, a.p, a.n, bid, nbid);
+}
+
+__device__ __forceinline__ void k_win_types_body(const int* __restrict__ need, int n, const int* __restrict__ order,
+                            const int* __restrict__ type, int* __restrict__ stype,
+    unsigned bid, unsigned nbid)
+{
+    if (need[0] == 0) return;
+    int s = bid * blockDim.x + threadIdx.x;
+    if (s < n) stype[s] = type[order[s]];
 }
 
 __global__ void k_win_types(const int* __restrict__ need, int n, const int* __restrict__ order,
                             const int* __restrict__ type, int* __restrict__ stype)
 {
-    if (need[0] == 0) return;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) stype[s] = type[order[s]];
+    k_win_types_body(need, n, order, type, stype, blockIdx.x, gridDim.x);
+}
+
+struct WinTypesArgs
+{
+    const int* need;
+    int n;
+    const int* order;
+    const int* type;
+    int* stype;
+};
+
+__global__ void k_win_types_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const WinTypesArgs& a = snb_seg<WinTypesArgs>(t, bid, nbid);
+    k_win_types_body(a.need, a.n, a.order, a.type, a.stype, bid, nbid);
 }
 
 // Called after the cell sort of a rebuild (same device flag).
@@ -841,54 +1058,54 @@ int snb_launch_nl_window(snb_context* ctx, const SnbBox& box, const int* ex_star
     hipStream_t s = ctx->stream;
     int n = ctx->n;
     const SnbWin& W = L.win;
-    hipLaunchKernelGGL(k_win_types, dim3((n + 255) / 256), dim3(256), 0, s, L.need, n, L.order,
-                       ctx->type, ctx->stype);
+    SNB_LAUNCH_B(k_win_types, k_win_types_batch, WinTypesArgs, dim3((n + 255) / 256), dim3(256), 0,
+                 s, L.need, n, L.order, ctx->type, ctx->stype);
     const bool ortho = box.c[1] == 0.0f && box.c[3] == 0.0f && box.c[4] == 0.0f &&
                        box.rc[1] == 0.0f && box.rc[3] == 0.0f && box.rc[4] == 0.0f;
     if (ex_num)
-        hipLaunchKernelGGL(k_ex_range, dim3((n + 255) / 256), dim3(256), 0, s, L.need, n,
-                           ex_start, ex_list, ex_num, L.ex_rng);
+        SNB_LAUNCH_B(k_ex_range, k_ex_range_batch, ExRangeArgs, dim3((n + 255) / 256), dim3(256), 0,
+                     s, L.need, n, ex_start, ex_list, ex_num, L.ex_rng);
     if (ortho)
-        hipLaunchKernelGGL(k_nl_build_cell<true>, dim3(L.ncell), dim3(512), 0, s, L.need, W,
-                           L.cell_start, L.sorted_crd, box, L.cs2, ex_start, ex_list, ex_num,
-                           L.ex_rng, L.max_nb, L.near2, L.nl16, L.row_count, L.row_split,
-                           ctx->status, L.grid_length);
+        SNB_LAUNCH_B(k_nl_build_cell<true>, k_nl_build_cell_batch<true>, NlBuildCellArgs,
+                     dim3(L.ncell), dim3(512), 0, s, L.need, W, L.cell_start, L.sorted_crd, box,
+                     L.cs2, ex_start, ex_list, ex_num, L.ex_rng, L.max_nb, L.near2, L.nl16,
+                     L.row_count, L.row_split, ctx->status, L.grid_length);
     else
-        hipLaunchKernelGGL(k_nl_build_cell<false>, dim3(L.ncell), dim3(512), 0, s, L.need, W,
-                           L.cell_start, L.sorted_crd, box, L.cs2, ex_start, ex_list, ex_num,
-                           L.ex_rng, L.max_nb, L.near2, L.nl16, L.row_count, L.row_split,
-                           ctx->status, L.grid_length);
-    hipLaunchKernelGGL(k_win_sizes, dim3(W.nblock), dim3(64), 0, s, L.need, W, L.cell_start,
-                       L.win_size);
-    hipLaunchKernelGGL(k_block_work, dim3(W.nblock), dim3(64), 0, s, L.need, W, L.cell_start,
-                       L.row_count, L.block_work);
-    hipLaunchKernelGGL(k_block_order, dim3(1), dim3(1024), 0, s, L.need, W.nblock, L.block_work,
-                       L.block_perm);
+        SNB_LAUNCH_B(k_nl_build_cell<false>, k_nl_build_cell_batch<false>, NlBuildCellArgs,
+                     dim3(L.ncell), dim3(512), 0, s, L.need, W, L.cell_start, L.sorted_crd, box,
+                     L.cs2, ex_start, ex_list, ex_num, L.ex_rng, L.max_nb, L.near2, L.nl16,
+                     L.row_count, L.row_split, ctx->status, L.grid_length);
+    SNB_LAUNCH_B(k_win_sizes, k_win_sizes_batch, WinSizesArgs, dim3(W.nblock), dim3(64), 0, s,
+                 L.need, W, L.cell_start, L.win_size);
+    SNB_LAUNCH_B(k_block_work, k_block_work_batch, BlockWorkArgs, dim3(W.nblock), dim3(64), 0, s,
+                 L.need, W, L.cell_start, L.row_count, L.block_work);
+    SNB_LAUNCH_B(k_block_order, k_block_order_batch, BlockOrderArgs, dim3(1), dim3(1024), 0, s,
+                 L.need, W.nblock, L.block_work, L.block_perm);
     SNB_CHECK(hipGetLastError());
     int rc = snb_launch_exclusive_scan(s, L.win_size, L.stage_base, W.nblock);
     if (rc) return rc;
     // the inverse lists persist between rebuilds: clear them on rebuilds only
-    hipLaunchKernelGGL(k_fill_need, dim3((n + 255) / 256), dim3(256), 0, s, L.need, L.inv_count,
-                       n);
-    hipLaunchKernelGGL(k_stage_inverse, dim3(W.nblock), dim3(256), 0, s, L.need, W, L.cell_start,
-                       L.stage_base, L.inv_count, L.inv_list, L.invmax, ctx->status);
+    SNB_LAUNCH_B(k_fill_need, k_fill_need_batch, FillNeedArgs, dim3((n + 255) / 256), dim3(256), 0,
+                 s, L.need, L.inv_count, n);
+    SNB_LAUNCH_B(k_stage_inverse, k_stage_inverse_batch, StageInverseArgs, dim3(W.nblock),
+                 dim3(256), 0, s, L.need, W, L.cell_start, L.stage_base, L.inv_count, L.inv_list,
+                 L.invmax, ctx->status);
     SNB_CHECK(hipGetLastError());
     if (L.export_mode)
     {
-        hipLaunchKernelGGL(k_fill_need, dim3((n + 255) / 256), dim3(256), 0, s, L.need,
-                           L.pub_count, n);
-        hipLaunchKernelGGL(k_nl_export_block, dim3(W.nblock), dim3(1024), 0, s, L.need, W,
-                           L.cell_start, L.order, L.row_count, L.row_split, L.nl16, L.max_nb,
-                           L.pub_count, L.serial);
+        SNB_LAUNCH_B(k_fill_need, k_fill_need_batch, FillNeedArgs, dim3((n + 255) / 256), dim3(256),
+                     0, s, L.need, L.pub_count, n);
+        SNB_LAUNCH(k_nl_export_block, dim3(W.nblock), dim3(1024), 0, s, L.need, W, L.cell_start,
+                   L.order, L.row_count, L.row_split, L.nl16, L.max_nb, L.pub_count, L.serial);
         if (L.export_mode == 1)
-            hipLaunchKernelGGL(k_nl_pub_counts, dim3((n + 255) / 256), dim3(256), 0, s, L.need,
-                               n, L.pub_count, L.max_nb, L.dnl, ctx->status);
+            SNB_LAUNCH(k_nl_pub_counts, dim3((n + 255) / 256), dim3(256), 0, s, L.need, n,
+                       L.pub_count, L.max_nb, L.dnl, ctx->status);
         else if (L.max_nb <= SNB_SORT_WAVE_CAP)
-            hipLaunchKernelGGL(k_nl_pub_finish_wave, dim3((n + 3) / 4), dim3(256), 0, s, L.need,
-                               n, L.pub_count, L.max_nb, L.serial, L.dnl, ctx->status);
+            SNB_LAUNCH(k_nl_pub_finish_wave, dim3((n + 3) / 4), dim3(256), 0, s, L.need, n,
+                       L.pub_count, L.max_nb, L.serial, L.dnl, ctx->status);
         else
-            hipLaunchKernelGGL(k_nl_pub_finish, dim3(n), dim3(64), 0, s, L.need, n, L.pub_count,
-                               L.max_nb, L.serial, L.dnl, ctx->status);
+            SNB_LAUNCH(k_nl_pub_finish, dim3(n), dim3(64), 0, s, L.need, n, L.pub_count, L.max_nb,
+                       L.serial, L.dnl, ctx->status);
         SNB_CHECK(hipGetLastError());
     }
     return SNB_OK;
@@ -898,10 +1115,8 @@ int snb_launch_nl_copy_internal(snb_context* ctx, int* counts, int* rows)
 {
     SnbNl& L = ctx->nl;
     if (ctx->n <= 0) return SNB_OK;
-    hipLaunchKernelGGL(k_nl_copy_internal, dim3(ctx->n), dim3(64), 0, ctx->stream, ctx->n, L.win,
-                       L.cell_start, L.sorted_cell, L.order, L.row_count, L.row_split, L.nl16,
-                       L.max_nb, counts,
-                       rows);
+    SNB_LAUNCH(k_nl_copy_internal,
+               dim3(ctx->n), dim3(64), 0, ctx->stream, ctx->n, L.win, L.cell_start, L.sorted_cell, L.order, L.row_count, L.row_split, L.nl16, L.max_nb, counts, rows);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }

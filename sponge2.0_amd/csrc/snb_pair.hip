@@ -65,16 +65,17 @@ __device__ __forceinline__ float pair_force_factor(float r2, float A, float B, f
 }
 
 template <bool NEED_ENERGY, bool NEED_VIRIAL>
-__global__ void __launch_bounds__(256) k_lj_direct(
+__device__ __forceinline__ void k_lj_direct_body(
     int n, const snb_atom_group* __restrict__ nl, const float4* __restrict__ atoms,
     const int* __restrict__ type, const float* __restrict__ lj_a,
     const float* __restrict__ lj_b, const float4* __restrict__ coul_tab, float cscale,
     SnbBox box, float rc2, float beta,
-    int64_t* __restrict__ fx, int64_t* __restrict__ e_atom, int64_t* __restrict__ vir)
+    int64_t* __restrict__ fx, int64_t* __restrict__ e_atom, int64_t* __restrict__ vir,
+    unsigned bid, unsigned nbid)
 {
     const int lane = threadIdx.x & 63;
-    const int waves = gridDim.x * 4;
-    for (int i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += waves)
+    const int waves = nbid * 4;
+    for (int i = bid * 4 + (threadIdx.x >> 6); i < n; i += waves)
     {
         const snb_atom_group g = nl[i];
         const float4 ai = atoms[i];
@@ -162,6 +163,47 @@ __global__ void __launch_bounds__(256) k_lj_direct(
     }
 }
 
+template <bool NEED_ENERGY, bool NEED_VIRIAL>
+__global__ void __launch_bounds__(256) k_lj_direct(
+    int n, const snb_atom_group* __restrict__ nl, const float4* __restrict__ atoms,
+    const int* __restrict__ type, const float* __restrict__ lj_a,
+    const float* __restrict__ lj_b, const float4* __restrict__ coul_tab, float cscale,
+    SnbBox box, float rc2, float beta,
+    int64_t* __restrict__ fx, int64_t* __restrict__ e_atom, int64_t* __restrict__ vir)
+{
+    k_lj_direct_body<NEED_ENERGY, NEED_VIRIAL>(
+        n, nl, atoms, type, lj_a, lj_b, coul_tab, cscale, box, rc2, beta, fx, e_atom, vir,
+        blockIdx.x, gridDim.x);
+}
+
+struct LjDirectArgs
+{
+    int n;
+    const snb_atom_group* nl;
+    const float4* atoms;
+    const int* type;
+    const float* lj_a;
+    const float* lj_b;
+    const float4* coul_tab;
+    float cscale;
+    SnbBox box;
+    float rc2;
+    float beta;
+    int64_t* fx;
+    int64_t* e_atom;
+    int64_t* vir;
+};
+
+template <bool NEED_ENERGY, bool NEED_VIRIAL>
+__global__ void __launch_bounds__(256) k_lj_direct_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const LjDirectArgs& a = snb_seg<LjDirectArgs>(t, bid, nbid);
+    k_lj_direct_body<NEED_ENERGY, NEED_VIRIAL>(
+        a.n, a.nl, a.atoms, a.type, a.lj_a, a.lj_b, a.coul_tab, a.cscale, a.box, a.rc2, a.beta,
+        a.fx, a.e_atom, a.vir, bid, nbid);
+}
+
 int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBox& box,
                          float beta, int need_energy, int need_virial)
 {
@@ -172,10 +214,10 @@ int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBo
     if (blocks > 65535 * 4) blocks = 65535 * 4;
     snb_prof_begin(ctx, SNB_PROF_PAIR);
 #define SNB_LJ_LAUNCH(E, V)                                                                   \
-    hipLaunchKernelGGL((k_lj_direct<E, V>), dim3(blocks), dim3(256), 0, ctx->stream, n, nl,    \
-                       ctx->atom4, ctx->type, ctx->lj.a, ctx->lj.b, ctx->lj.coul,             \
-                       ctx->lj.coul_scale, box, rc2, beta, ctx->fx,                            \
-                       ctx->e_atom, ctx->vir_atom)
+    SNB_LAUNCH_B((k_lj_direct<E, V>), (k_lj_direct_batch<E, V>), LjDirectArgs, dim3(blocks),   \
+                 dim3(256), 0, ctx->stream, n, nl, ctx->atom4, ctx->type, ctx->lj.a, ctx->lj.b, \
+                 ctx->lj.coul, ctx->lj.coul_scale, box, rc2, beta, ctx->fx, ctx->e_atom,        \
+                 ctx->vir_atom)
     if (need_energy && need_virial)
         SNB_LJ_LAUNCH(true, true);
     else if (need_energy)
@@ -196,7 +238,7 @@ int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBo
 // evaluated in FP64.  One thread per atom i; i-sums in registers, j-terms by
 // integer atomics (a few per atom).
 template <bool NEED_ENERGY, bool NEED_VIRIAL>
-__global__ void __launch_bounds__(256) k_excl(int n, const snb_vector* __restrict__ crd,
+__device__ __forceinline__ void k_excl_body(int n, const snb_vector* __restrict__ crd,
                                               const float* __restrict__ charge,
                                               SnbBox box, float beta,
                                               const int* __restrict__ ex_start,
@@ -204,9 +246,10 @@ __global__ void __launch_bounds__(256) k_excl(int n, const snb_vector* __restric
                                               const int* __restrict__ ex_num,
                                               int64_t* __restrict__ fx,
                                               int64_t* __restrict__ e_atom,
-                                              int64_t* __restrict__ vir)
+                                              int64_t* __restrict__ vir,
+    unsigned bid, unsigned nbid)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int cnt = ex_num[i];
     if (cnt <= 0) return;
@@ -267,6 +310,47 @@ __global__ void __launch_bounds__(256) k_excl(int n, const snb_vector* __restric
     }
 }
 
+template <bool NEED_ENERGY, bool NEED_VIRIAL>
+__global__ void __launch_bounds__(256) k_excl(int n, const snb_vector* __restrict__ crd,
+                                              const float* __restrict__ charge,
+                                              SnbBox box, float beta,
+                                              const int* __restrict__ ex_start,
+                                              const int* __restrict__ ex_list,
+                                              const int* __restrict__ ex_num,
+                                              int64_t* __restrict__ fx,
+                                              int64_t* __restrict__ e_atom,
+                                              int64_t* __restrict__ vir)
+{
+    k_excl_body<NEED_ENERGY, NEED_VIRIAL>(
+        n, crd, charge, box, beta, ex_start, ex_list, ex_num, fx, e_atom, vir, blockIdx.x,
+        gridDim.x);
+}
+
+struct ExclArgs
+{
+    int n;
+    const snb_vector* crd;
+    const float* charge;
+    SnbBox box;
+    float beta;
+    const int* ex_start;
+    const int* ex_list;
+    const int* ex_num;
+    int64_t* fx;
+    int64_t* e_atom;
+    int64_t* vir;
+};
+
+template <bool NEED_ENERGY, bool NEED_VIRIAL>
+__global__ void __launch_bounds__(256) k_excl_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const ExclArgs& a = snb_seg<ExclArgs>(t, bid, nbid);
+    k_excl_body<NEED_ENERGY, NEED_VIRIAL>(
+        a.n, a.crd, a.charge, a.box, a.beta, a.ex_start, a.ex_list, a.ex_num, a.fx, a.e_atom, a.vir,
+        bid, nbid);
+}
+
 int snb_launch_excl_to(hipStream_t s, int n, const snb_vector* crd, const float* charge,
                        const SnbBox& box, float beta, const int* ex_start, const int* ex_list,
                        const int* ex_num, int need_energy, int need_virial, int64_t* fx,
@@ -275,8 +359,8 @@ int snb_launch_excl_to(hipStream_t s, int n, const snb_vector* crd, const float*
     if (n <= 0) return SNB_OK;
     dim3 g((n + 255) / 256);
 #define SNB_EX_LAUNCH(E, V)                                                                  \
-    hipLaunchKernelGGL((k_excl<E, V>), g, dim3(256), 0, s, n, crd, charge, box, beta,       \
-                       ex_start, ex_list, ex_num, fx, e_atom, vir)
+    SNB_LAUNCH_B((k_excl<E, V>), (k_excl_batch<E, V>), ExclArgs, g, dim3(256), 0, s, n, crd, \
+                 charge, box, beta, ex_start, ex_list, ex_num, fx, e_atom, vir)
     if (need_energy && need_virial)
         SNB_EX_LAUNCH(true, true);
     else if (need_energy)
@@ -387,7 +471,7 @@ int snb_launch_nb14(snb_context* ctx, const snb_vector* crd, const float* charge
     if (m <= 0) return SNB_OK;
     dim3 g((m + 255) / 256);
 #define SNB_NB14_LAUNCH(E, V)                                                                \
-    hipLaunchKernelGGL((k_nb14<E, V>), g, dim3(256), 0, ctx->stream, m, n, M.a, M.b, M.A, M.B, \
+    SNB_LAUNCH((k_nb14<E, V>), g, dim3(256), 0, ctx->stream, m, n, M.a, M.b, M.A, M.B, \
                        M.cf, crd, charge, box, fx, ctx->e_atom + 3 * (size_t)n, M.pair_e,      \
                        ctx->vir_atom)
     if (need_energy && need_virial)
@@ -466,7 +550,7 @@ int snb_launch_bond(snb_context* ctx, const snb_vector* crd, const SnbBox& box, 
     if (m <= 0) return SNB_OK;
     dim3 g((m + 255) / 256);
 #define SNB_BOND_LAUNCH(E, V)                                                               \
-    hipLaunchKernelGGL((k_bond<E, V>), g, dim3(256), 0, ctx->stream, m, n, M.a, M.b, M.k,    \
+    SNB_LAUNCH((k_bond<E, V>), g, dim3(256), 0, ctx->stream, m, n, M.a, M.b, M.k,    \
                        M.r0, crd, box, fx, ctx->e_atom + 4 * (size_t)n, M.pair_e,            \
                        ctx->vir_atom)
     if (need_energy && need_virial)
@@ -487,17 +571,42 @@ int snb_launch_bond(snb_context* ctx, const snb_vector* crd, const SnbBox& box, 
 // ---------------------------------------------------------------------------
 // Packs coordinates and charges in sorted (cell) order: satom[s] = (crd, q) of
 // atom order[s]; raises the non-finite flag.
-__global__ void k_pack_sorted(int n, const int* __restrict__ order,
+__device__ __forceinline__ void k_pack_sorted_body(int n, const int* __restrict__ order,
                               const snb_vector* __restrict__ crd, const float* __restrict__ charge,
-                              float4* __restrict__ satom, int* __restrict__ status)
+                              float4* __restrict__ satom, int* __restrict__ status,
+    unsigned bid, unsigned nbid)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = bid * blockDim.x + threadIdx.x;
     if (s >= n) return;
     int id = order[s];
     snb_vector r = crd[id];
     satom[s] = make_float4(r.x, r.y, r.z, charge[id]);
     if (!(__builtin_isfinite(r.x) && __builtin_isfinite(r.y) && __builtin_isfinite(r.z)))
         atomicOr(status, SNB_FLAG_NONFINITE_CRD);
+}
+
+__global__ void k_pack_sorted(int n, const int* __restrict__ order,
+                              const snb_vector* __restrict__ crd, const float* __restrict__ charge,
+                              float4* __restrict__ satom, int* __restrict__ status)
+{
+    k_pack_sorted_body(n, order, crd, charge, satom, status, blockIdx.x, gridDim.x);
+}
+
+struct PackSortedArgs
+{
+    int n;
+    const int* order;
+    const snb_vector* crd;
+    const float* charge;
+    float4* satom;
+    int* status;
+};
+
+__global__ void k_pack_sorted_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const PackSortedArgs& a = snb_seg<PackSortedArgs>(t, bid, nbid);
+    k_pack_sorted_body(a.n, a.order, a.crd, a.charge, a.satom, a.status, bid, nbid);
 }
 
 // One workgroup per block of a x b columns x S z-cells (snb_window.h).  The
@@ -658,7 +767,7 @@ __device__ __forceinline__ int win_slot_sorted(int slot, bool ov, const int* slo
 }
 
 template <bool NEED_ENERGY, bool NEED_VIRIAL, bool ORTHO, bool STREAM>
-__global__ void __launch_bounds__(1024) k_pair_win(
+__device__ __forceinline__ void k_pair_win_body(
     int n, SnbWin W, const int* __restrict__ start, const int* __restrict__ sorted_cell,
     const int* __restrict__ row_count, const int* __restrict__ row_split,
     const uint16_t* __restrict__ nl16, int max_nb,
@@ -667,7 +776,8 @@ __global__ void __launch_bounds__(1024) k_pair_win(
     const float4* __restrict__ coul_tab, float cscale, SnbBox box, float rc2, float beta,
     const int* __restrict__ stage_base, long long* __restrict__ stage, size_t stage_cap,
     int64_t* __restrict__ fx, int64_t* __restrict__ e_sorted, int64_t* __restrict__ v_sorted,
-    int ref_attr, const int* __restrict__ perm, int jdirect)
+    int ref_attr, const int* __restrict__ perm, int jdirect,
+    unsigned bid, unsigned nbid)
 {
     extern __shared__ long long smem[];
     float4* ctab = (float4*)smem;                   // [SNB_COUL_N]
@@ -677,7 +787,7 @@ __global__ void __launch_bounds__(1024) k_pair_win(
     int* rpref = rfirst + SNB_WIN_MAX_RANGES;       // [129]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nwave = blockDim.x >> 6;
-    const int w = perm[blockIdx.x];  // heaviest blocks first
+    const int w = perm[bid];  // heaviest blocks first
     if (tid < SNB_COUL_N) ctab[tid] = coul_tab[tid];
     int x0, y0, z0;
     snb_win_block(W, w, x0, y0, z0);
@@ -1142,6 +1252,67 @@ __global__ void __launch_bounds__(1024) k_pair_win(
     }
 }
 
+template <bool NEED_ENERGY, bool NEED_VIRIAL, bool ORTHO, bool STREAM>
+__global__ void __launch_bounds__(1024) k_pair_win(
+    int n, SnbWin W, const int* __restrict__ start, const int* __restrict__ sorted_cell,
+    const int* __restrict__ row_count, const int* __restrict__ row_split,
+    const uint16_t* __restrict__ nl16, int max_nb,
+    const float4* __restrict__ satom, const int* __restrict__ stype,
+    const int* __restrict__ order, const float2* __restrict__ ab2, int ntypes,
+    const float4* __restrict__ coul_tab, float cscale, SnbBox box, float rc2, float beta,
+    const int* __restrict__ stage_base, long long* __restrict__ stage, size_t stage_cap,
+    int64_t* __restrict__ fx, int64_t* __restrict__ e_sorted, int64_t* __restrict__ v_sorted,
+    int ref_attr, const int* __restrict__ perm, int jdirect)
+{
+    k_pair_win_body<NEED_ENERGY, NEED_VIRIAL, ORTHO, STREAM>(
+        n, W, start, sorted_cell, row_count, row_split, nl16, max_nb, satom, stype, order, ab2,
+        ntypes, coul_tab, cscale, box, rc2, beta, stage_base, stage, stage_cap, fx, e_sorted,
+        v_sorted, ref_attr, perm, jdirect, blockIdx.x, gridDim.x);
+}
+
+struct PairWinArgs
+{
+    int n;
+    SnbWin W;
+    const int* start;
+    const int* sorted_cell;
+    const int* row_count;
+    const int* row_split;
+    const uint16_t* nl16;
+    int max_nb;
+    const float4* satom;
+    const int* stype;
+    const int* order;
+    const float2* ab2;
+    int ntypes;
+    const float4* coul_tab;
+    float cscale;
+    SnbBox box;
+    float rc2;
+    float beta;
+    const int* stage_base;
+    long long* stage;
+    size_t stage_cap;
+    int64_t* fx;
+    int64_t* e_sorted;
+    int64_t* v_sorted;
+    int ref_attr;
+    const int* perm;
+    int jdirect;
+};
+
+template <bool NEED_ENERGY, bool NEED_VIRIAL, bool ORTHO, bool STREAM>
+__global__ void __launch_bounds__(1024) k_pair_win_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const PairWinArgs& a = snb_seg<PairWinArgs>(t, bid, nbid);
+    k_pair_win_body<NEED_ENERGY, NEED_VIRIAL, ORTHO, STREAM>(
+        a.n, a.W, a.start, a.sorted_cell, a.row_count, a.row_split, a.nl16, a.max_nb, a.satom,
+        a.stype, a.order, a.ab2, a.ntypes, a.coul_tab, a.cscale, a.box, a.rc2, a.beta, a.stage_base,
+        a.stage, a.stage_cap, a.fx, a.e_sorted, a.v_sorted, a.ref_attr, a.perm, a.jdirect, bid,
+        nbid);
+}
+
 // Per sorted atom: sum of its stage entries into the id-ordered accumulators
 // (plus the row-owner energies / virials on those steps).
 __global__ void k_stage_gather(int n, const int* __restrict__ inv_count,
@@ -1188,8 +1359,8 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
     SnbNl& L = ctx->nl;
     const SnbWin& W = L.win;
     hipStream_t s = ctx->stream;
-    hipLaunchKernelGGL(k_pack_sorted, dim3((n + 255) / 256), dim3(256), 0, s, n, L.order, crd,
-                       charge, ctx->satom, ctx->status);
+    SNB_LAUNCH_B(k_pack_sorted, k_pack_sorted_batch, PackSortedArgs, dim3((n + 255) / 256),
+                 dim3(256), 0, s, n, L.order, crd, charge, ctx->satom, ctx->status);
     float rc2 = ctx->lj.cutoff * ctx->lj.cutoff;
     size_t lds = 28 * (size_t)W.wcap + SNB_WIN_FIXED_LDS;
     if (L.win_lds_set != lds)
@@ -1240,13 +1411,12 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
                         (stream_mode > 0 || (stream_mode < 0 && n >= (1 << 18)));
     snb_prof_begin(ctx, SNB_PROF_PAIR);
 #define SNB_WIN_LAUNCH(E, V, O, ST)                                                             \
-    hipLaunchKernelGGL((k_pair_win<E, V, O, ST>), dim3(W.nblock), dim3(1024), lds, s, n, W,     \
-                       L.cell_start, L.sorted_cell, L.row_count, L.row_split, L.nl16, L.max_nb, \
-                       ctx->satom,                                                             \
-                       ctx->stype, L.order, ctx->lj.ab2, ctx->lj.ntypes, ctx->lj.coul,         \
-                       ctx->lj.coul_scale, box, rc2, beta,                                     \
-                       L.stage_base, L.stage, (size_t)n * L.invmax, ctx->fx, ctx->e_sorted,   \
-                       ctx->v_sorted, ref_attr, L.block_perm, jdirect)
+    SNB_LAUNCH_B((k_pair_win<E, V, O, ST>), (k_pair_win_batch<E, V, O, ST>), PairWinArgs,       \
+                 dim3(W.nblock), dim3(1024), lds, s, n, W, L.cell_start, L.sorted_cell,        \
+                 L.row_count, L.row_split, L.nl16, L.max_nb, ctx->satom, ctx->stype, L.order,  \
+                 ctx->lj.ab2, ctx->lj.ntypes, ctx->lj.coul, ctx->lj.coul_scale, box, rc2, beta, \
+                 L.stage_base, L.stage, (size_t)n * L.invmax, ctx->fx, ctx->e_sorted,           \
+                 ctx->v_sorted, ref_attr, L.block_perm, jdirect)
 #define SNB_WIN_LAUNCH_EV(O)                     \
     if (need_energy && need_virial)              \
         SNB_WIN_LAUNCH(true, true, O, false);    \
@@ -1271,10 +1441,9 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
     snb_prof_end(ctx, SNB_PROF_PAIR);
     SNB_CHECK(hipGetLastError());
     if (jdirect) return SNB_OK;
-    hipLaunchKernelGGL(k_stage_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, L.inv_count,
-                       L.inv_list, L.invmax, L.stage, (size_t)n * L.invmax, L.order, ctx->fx,
-                       need_energy, need_virial,
-                       ctx->e_sorted, ctx->v_sorted, ctx->e_atom, ctx->vir_atom);
+    SNB_LAUNCH(k_stage_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, L.inv_count, L.inv_list,
+               L.invmax, L.stage, (size_t)n * L.invmax, L.order,
+               ctx->fx, need_energy, need_virial, ctx->e_sorted, ctx->v_sorted, ctx->e_atom, ctx->vir_atom);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }

@@ -212,10 +212,11 @@ static int lines_per_block(int n)
 }
 
 // z pass forward: real rows [nrow][fz] -> half spectra [nrow][fc]
-__global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ real,
+__device__ __forceinline__ void k_fft_z_r2c_body(const float* __restrict__ real,
                                                    float2* __restrict__ spec, int nrow,
                                                    SnbFftPlan plan, int fc,
-                                                   const float2* __restrict__ twg, int L)
+                                                   const float2* __restrict__ twg, int L,
+    unsigned bid, unsigned nbid)
 {
     extern __shared__ float2 smem[];
     int n = plan.n;
@@ -223,7 +224,7 @@ __global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ rea
     float2* a = smem + n;
     float2* b = a + L * n;
     load_twiddles(tw, twg, n);
-    int row0 = blockIdx.x * L;
+    int row0 = bid * L;
     int lines = min(L, nrow - row0);
     for (int t = threadIdx.x; t < lines * n; t += blockDim.x)
         a[t] = make_float2(real[(size_t)row0 * n + t], 0.0f);
@@ -236,11 +237,38 @@ __global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ rea
     }
 }
 
-// z pass inverse: half spectra [nrow][fc] -> real rows [nrow][fz]
-__global__ void __launch_bounds__(256) k_fft_z_c2r(const float2* __restrict__ spec,
-                                                   float* __restrict__ real, int nrow,
+__global__ void __launch_bounds__(256) k_fft_z_r2c(const float* __restrict__ real,
+                                                   float2* __restrict__ spec, int nrow,
                                                    SnbFftPlan plan, int fc,
                                                    const float2* __restrict__ twg, int L)
+{
+    k_fft_z_r2c_body(real, spec, nrow, plan, fc, twg, L, blockIdx.x, gridDim.x);
+}
+
+struct FftZr2cArgs
+{
+    const float* real;
+    float2* spec;
+    int nrow;
+    SnbFftPlan plan;
+    int fc;
+    const float2* twg;
+    int L;
+};
+
+__global__ void __launch_bounds__(256) k_fft_z_r2c_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const FftZr2cArgs& a = snb_seg<FftZr2cArgs>(t, bid, nbid);
+    k_fft_z_r2c_body(a.real, a.spec, a.nrow, a.plan, a.fc, a.twg, a.L, bid, nbid);
+}
+
+// z pass inverse: half spectra [nrow][fc] -> real rows [nrow][fz]
+__device__ __forceinline__ void k_fft_z_c2r_body(const float2* __restrict__ spec,
+                                                   float* __restrict__ real, int nrow,
+                                                   SnbFftPlan plan, int fc,
+                                                   const float2* __restrict__ twg, int L,
+    unsigned bid, unsigned nbid)
 {
     extern __shared__ float2 smem[];
     int n = plan.n;
@@ -248,7 +276,7 @@ __global__ void __launch_bounds__(256) k_fft_z_c2r(const float2* __restrict__ sp
     float2* a = smem + n;
     float2* b = a + L * n;
     load_twiddles(tw, twg, n);
-    int row0 = blockIdx.x * L;
+    int row0 = bid * L;
     int lines = min(L, nrow - row0);
     for (int t = threadIdx.x; t < lines * n; t += blockDim.x)
     {
@@ -269,10 +297,37 @@ __global__ void __launch_bounds__(256) k_fft_z_c2r(const float2* __restrict__ sp
         real[(size_t)row0 * n + t] = res[t].x;
 }
 
+__global__ void __launch_bounds__(256) k_fft_z_c2r(const float2* __restrict__ spec,
+                                                   float* __restrict__ real, int nrow,
+                                                   SnbFftPlan plan, int fc,
+                                                   const float2* __restrict__ twg, int L)
+{
+    k_fft_z_c2r_body(spec, real, nrow, plan, fc, twg, L, blockIdx.x, gridDim.x);
+}
+
+struct FftZc2rArgs
+{
+    const float2* spec;
+    float* real;
+    int nrow;
+    SnbFftPlan plan;
+    int fc;
+    const float2* twg;
+    int L;
+};
+
+__global__ void __launch_bounds__(256) k_fft_z_c2r_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const FftZc2rArgs& a = snb_seg<FftZc2rArgs>(t, bid, nbid);
+    k_fft_z_c2r_body(a.spec, a.real, a.nrow, a.plan, a.fc, a.twg, a.L, bid, nbid);
+}
+
 // y pass over spec[fx][fy][fc]: block = (x, chunk of W consecutive kz).
-__global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ spec, int fx, int fy,
+__device__ __forceinline__ void k_fft_y_body(float2* __restrict__ spec, int fx, int fy,
                                                int fc, SnbFftPlan plan,
-                                               const float2* __restrict__ twg, int W, float s)
+                                               const float2* __restrict__ twg, int W, float s,
+    unsigned bid, unsigned nbid)
 {
     extern __shared__ float2 smem[];
     int n = plan.n;  // == fy
@@ -281,8 +336,8 @@ __global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ spec, int fx
     float2* b = a + W * n;
     load_twiddles(tw, twg, n);
     int nchunk = (fc + W - 1) / W;
-    int x = blockIdx.x / nchunk;
-    int k0 = (blockIdx.x - x * nchunk) * W;
+    int x = bid / nchunk;
+    int k0 = (bid - x * nchunk) * W;
     int w = min(W, fc - k0);
     for (int t = threadIdx.x; t < n * w; t += blockDim.x)
     {
@@ -298,6 +353,32 @@ __global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ spec, int fx
     }
 }
 
+__global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ spec, int fx, int fy,
+                                               int fc, SnbFftPlan plan,
+                                               const float2* __restrict__ twg, int W, float s)
+{
+    k_fft_y_body(spec, fx, fy, fc, plan, twg, W, s, blockIdx.x, gridDim.x);
+}
+
+struct FftYArgs
+{
+    float2* spec;
+    int fx;
+    int fy;
+    int fc;
+    SnbFftPlan plan;
+    const float2* twg;
+    int W;
+    float s;
+};
+
+__global__ void __launch_bounds__(256) k_fft_y_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const FftYArgs& a = snb_seg<FftYArgs>(t, bid, nbid);
+    k_fft_y_body(a.spec, a.fx, a.fy, a.fc, a.plan, a.twg, a.W, a.s, bid, nbid);
+}
+
 // x pass with the convolution: block = (y, chunk of W consecutive kz).
 // Forward FFT along x, then per element (kx, y, kz):
 //   energy  += 1/2 * wfull * BC |F|^2        (wfull = 1 on kz = 0 and the
@@ -307,10 +388,11 @@ __global__ void __launch_bounds__(256) k_fft_y(float2* __restrict__ spec, int fx
 //   F *= BC
 // then inverse FFT along x.  Per-block partial sums go to partials[block][8].
 template <bool NEED_EV>
-__global__ void __launch_bounds__(256) k_fft_x_conv(
+__device__ __forceinline__ void k_fft_x_conv_body(
     float2* __restrict__ spec, const float* __restrict__ bc, int fx, int fy, int fz, int fc,
     SnbFftPlan plan, const float2* __restrict__ twg, int W, double r11, double r21,
-    double r22, double r31, double r32, double r33, double mpre, double* __restrict__ partials)
+    double r22, double r31, double r32, double r33, double mpre, double* __restrict__ partials,
+    unsigned bid, unsigned nbid)
 {
     extern __shared__ float2 smem[];
     int n = plan.n;  // == fx
@@ -319,8 +401,8 @@ __global__ void __launch_bounds__(256) k_fft_x_conv(
     float2* b = a + W * n;
     load_twiddles(tw, twg, n);
     int nchunk = (fc + W - 1) / W;
-    int y = blockIdx.x / nchunk;
-    int k0 = (blockIdx.x - y * nchunk) * W;
+    int y = bid / nchunk;
+    int k0 = (bid - y * nchunk) * W;
     int w = min(W, fc - k0);
     for (int t = threadIdx.x; t < n * w; t += blockDim.x)
     {
@@ -383,9 +465,51 @@ __global__ void __launch_bounds__(256) k_fft_x_conv(
         {
             double s = 0;
             for (int q = 0; q < (int)(blockDim.x >> 6); q++) s += red[threadIdx.x][q];
-            partials[(size_t)blockIdx.x * 8 + threadIdx.x] = s;
+            partials[(size_t)bid * 8 + threadIdx.x] = s;
         }
     }
+}
+
+template <bool NEED_EV>
+__global__ void __launch_bounds__(256) k_fft_x_conv(
+    float2* __restrict__ spec, const float* __restrict__ bc, int fx, int fy, int fz, int fc,
+    SnbFftPlan plan, const float2* __restrict__ twg, int W, double r11, double r21,
+    double r22, double r31, double r32, double r33, double mpre, double* __restrict__ partials)
+{
+    k_fft_x_conv_body<NEED_EV>(
+        spec, bc, fx, fy, fz, fc, plan, twg, W, r11, r21, r22, r31, r32, r33, mpre, partials,
+        blockIdx.x, gridDim.x);
+}
+
+struct FftXConvArgs
+{
+    float2* spec;
+    const float* bc;
+    int fx;
+    int fy;
+    int fz;
+    int fc;
+    SnbFftPlan plan;
+    const float2* twg;
+    int W;
+    double r11;
+    double r21;
+    double r22;
+    double r31;
+    double r32;
+    double r33;
+    double mpre;
+    double* partials;
+};
+
+template <bool NEED_EV>
+__global__ void __launch_bounds__(256) k_fft_x_conv_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const FftXConvArgs& a = snb_seg<FftXConvArgs>(t, bid, nbid);
+    k_fft_x_conv_body<NEED_EV>(
+        a.spec, a.bc, a.fx, a.fy, a.fz, a.fc, a.plan, a.twg, a.W, a.r11, a.r21, a.r22, a.r31, a.r32,
+        a.r33, a.mpre, a.partials, bid, nbid);
 }
 
 // ---------------------------------------------------------------------------
@@ -406,12 +530,13 @@ static size_t plane_lds(int fy, int fz, int fc)
 }
 
 // spec[x][fy][fc] -> real[x][fy][fz] (y inverse then z C2R), exponent +1
-__global__ void __launch_bounds__(1024) k_fft_plane_inv(const float2* __restrict__ spec,
+__device__ __forceinline__ void k_fft_plane_inv_body(const float2* __restrict__ spec,
                                                         float* __restrict__ real, int fy,
                                                         int fz, int fc, SnbFftPlan plan_z,
                                                         SnbFftPlan plan_y,
                                                         const float2* __restrict__ twzg,
-                                                        const float2* __restrict__ twyg)
+                                                        const float2* __restrict__ twyg,
+    unsigned bid, unsigned nbid)
 {
     extern __shared__ float2 smem[];
     const int cap = (int)plane_cap(fy, fz, fc);
@@ -421,7 +546,7 @@ __global__ void __launch_bounds__(1024) k_fft_plane_inv(const float2* __restrict
     float2* B = A + cap;
     load_twiddles(twz, twzg, fz);
     load_twiddles(twy, twyg, fy);
-    const int x = blockIdx.x;
+    const int x = bid;
     const int np = (fy + 1) / 2;
     const float2* in = spec + (size_t)x * fy * fc;
     for (int t = threadIdx.x; t < fy * fc; t += blockDim.x)
@@ -464,6 +589,37 @@ __global__ void __launch_bounds__(1024) k_fft_plane_inv(const float2* __restrict
         plane[(size_t)(2 * p) * fz + z] = v.x;
         if (2 * p + 1 < fy) plane[(size_t)(2 * p + 1) * fz + z] = v.y;
     }
+}
+
+__global__ void __launch_bounds__(1024) k_fft_plane_inv(const float2* __restrict__ spec,
+                                                        float* __restrict__ real, int fy,
+                                                        int fz, int fc, SnbFftPlan plan_z,
+                                                        SnbFftPlan plan_y,
+                                                        const float2* __restrict__ twzg,
+                                                        const float2* __restrict__ twyg)
+{
+    k_fft_plane_inv_body(spec, real, fy, fz, fc, plan_z, plan_y, twzg, twyg, blockIdx.x, gridDim.x);
+}
+
+struct PlaneInvArgs
+{
+    const float2* spec;
+    float* real;
+    int fy;
+    int fz;
+    int fc;
+    SnbFftPlan plan_z;
+    SnbFftPlan plan_y;
+    const float2* twzg;
+    const float2* twyg;
+};
+
+__global__ void __launch_bounds__(1024) k_fft_plane_inv_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const PlaneInvArgs& a = snb_seg<PlaneInvArgs>(t, bid, nbid);
+    k_fft_plane_inv_body(
+        a.spec, a.real, a.fy, a.fz, a.fc, a.plan_z, a.plan_y, a.twzg, a.twyg, bid, nbid);
 }
 
 // Fixed-order reduction of the per-block partials into dscal[0..6] and the
@@ -540,19 +696,20 @@ struct SnbRc
 // wavefront.
 #define SNB_PREP_HIST_MAX 8192
 template <bool LDS_HIST>
-__global__ void __launch_bounds__(256) k_pme_prep(int n, const snb_vector* __restrict__ crd,
+__device__ __forceinline__ void k_pme_prep_body(int n, const snb_vector* __restrict__ crd,
                                                   const float* __restrict__ charge, SnbRc rc,
                                                   int fx, int fy, int fz, int bx, int by, int bz,
                                                   int nby, int nbz, int nbrick,
                                                   const int* __restrict__ visit,
                                                   int4* __restrict__ atom_u,
                                                   float4* __restrict__ atom_fr,
-                                                  int* __restrict__ slot, int* __restrict__ cnt)
+                                                  int* __restrict__ slot, int* __restrict__ cnt,
+    unsigned bid, unsigned nbid)
 {
     __shared__ int hist[LDS_HIST ? SNB_PREP_HIST_MAX : 1];
     __shared__ int touched[LDS_HIST ? 256 : 1];
     __shared__ int ntouched;
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = bid * blockDim.x + threadIdx.x;
     const int i = v < n ? visit[v] : -1;
     if (LDS_HIST)
     {
@@ -607,12 +764,82 @@ __global__ void __launch_bounds__(256) k_pme_prep(int n, const snb_vector* __res
     }
 }
 
+template <bool LDS_HIST>
+__global__ void __launch_bounds__(256) k_pme_prep(int n, const snb_vector* __restrict__ crd,
+                                                  const float* __restrict__ charge, SnbRc rc,
+                                                  int fx, int fy, int fz, int bx, int by, int bz,
+                                                  int nby, int nbz, int nbrick,
+                                                  const int* __restrict__ visit,
+                                                  int4* __restrict__ atom_u,
+                                                  float4* __restrict__ atom_fr,
+                                                  int* __restrict__ slot, int* __restrict__ cnt)
+{
+    k_pme_prep_body<LDS_HIST>(
+        n, crd, charge, rc, fx, fy, fz, bx, by, bz, nby, nbz, nbrick, visit, atom_u, atom_fr, slot,
+        cnt, blockIdx.x, gridDim.x);
+}
+
+struct PmePrepArgs
+{
+    int n;
+    const snb_vector* crd;
+    const float* charge;
+    SnbRc rc;
+    int fx;
+    int fy;
+    int fz;
+    int bx;
+    int by;
+    int bz;
+    int nby;
+    int nbz;
+    int nbrick;
+    const int* visit;
+    int4* atom_u;
+    float4* atom_fr;
+    int* slot;
+    int* cnt;
+};
+
+template <bool LDS_HIST>
+__global__ void __launch_bounds__(256) k_pme_prep_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const PmePrepArgs& a = snb_seg<PmePrepArgs>(t, bid, nbid);
+    k_pme_prep_body<LDS_HIST>(
+        a.n, a.crd, a.charge, a.rc, a.fx, a.fy, a.fz, a.bx, a.by, a.bz, a.nby, a.nbz, a.nbrick,
+        a.visit, a.atom_u, a.atom_fr, a.slot, a.cnt, bid, nbid);
+}
+
+__device__ __forceinline__ void k_pme_scatter_body(int n, const int4* __restrict__ atom_u, const int* __restrict__ slot,
+                              const int* __restrict__ start, int* __restrict__ brick_atoms,
+    unsigned bid, unsigned nbid)
+{
+    int i = bid * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    brick_atoms[start[atom_u[i].w] + slot[i]] = i;
+}
+
 __global__ void k_pme_scatter(int n, const int4* __restrict__ atom_u, const int* __restrict__ slot,
                               const int* __restrict__ start, int* __restrict__ brick_atoms)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    brick_atoms[start[atom_u[i].w] + slot[i]] = i;
+    k_pme_scatter_body(n, atom_u, slot, start, brick_atoms, blockIdx.x, gridDim.x);
+}
+
+struct PmeScatterArgs
+{
+    int n;
+    const int4* atom_u;
+    const int* slot;
+    const int* start;
+    int* brick_atoms;
+};
+
+__global__ void k_pme_scatter_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const PmeScatterArgs& a = snb_seg<PmeScatterArgs>(t, bid, nbid);
+    k_pme_scatter_body(a.n, a.atom_u, a.slot, a.start, a.brick_atoms, bid, nbid);
 }
 
 __device__ __forceinline__ int wrap_axis(int v, int n) { return v < 0 ? v + n : v; }
@@ -624,10 +851,10 @@ __device__ __forceinline__ int wrap_axis(int v, int n) { return v < 0 ? v + n : 
 // order), which read the same grid points and atom records, share one L2.
 // -1 for the padding workgroups.
 #define SNB_XCDS 8
-__device__ __forceinline__ int xcd_block(int nblocks)
+__device__ __forceinline__ int xcd_block(int nblocks, unsigned bid)
 {
     const int chunk = (nblocks + SNB_XCDS - 1) / SNB_XCDS;
-    const int b = (int)(blockIdx.x % SNB_XCDS) * chunk + (int)(blockIdx.x / SNB_XCDS);
+    const int b = (int)(bid % SNB_XCDS) * chunk + (int)(bid / SNB_XCDS);
     return b < nblocks ? b : -1;
 }
 static inline unsigned xcd_grid(int nblocks)
@@ -661,10 +888,11 @@ __device__ __forceinline__ int stencil_mask(int d, int b, bool single)
 // d, stencil point u - k lies in the brick when 0 <= d - k < b: only that k
 // range is visited, so every stencil point is added by exactly one brick.
 #define SNB_SPREAD_LIST 256
-__global__ void __launch_bounds__(256) k_pme_spread(
+__device__ __forceinline__ void k_pme_spread_body(
     const int4* __restrict__ atom_u, const float4* __restrict__ atom_fr,
     const int* __restrict__ start, const int* __restrict__ brick_atoms, int fx, int fy, int fz,
-    int bx, int by, int bz, int nbx, int nby, int nbz, float* __restrict__ q)
+    int bx, int by, int bz, int nbx, int nby, int nbz, float* __restrict__ q,
+    unsigned bid, unsigned nbid)
 {
     extern __shared__ long long acc[];
     const int npt = bx * by * bz;
@@ -672,7 +900,7 @@ __global__ void __launch_bounds__(256) k_pme_spread(
     int* lmeta = (int*)(lw + SNB_SPREAD_LIST * 12);  // [SNB_SPREAD_LIST]
     __shared__ int src_first[8], src_pref[9], nlist;
     const int t = threadIdx.x;
-    const int B = xcd_block(nbx * nby * nbz);
+    const int B = xcd_block(nbx * nby * nbz, bid);
     if (B < 0) return;
     for (int p = t; p < npt; p += blockDim.x) acc[p] = 0;
     const int Bz = B % nbz, By = (B / nbz) % nby, Bx = B / (nbz * nby);
@@ -769,18 +997,56 @@ __global__ void __launch_bounds__(256) k_pme_spread(
     }
 }
 
+__global__ void __launch_bounds__(256) k_pme_spread(
+    const int4* __restrict__ atom_u, const float4* __restrict__ atom_fr,
+    const int* __restrict__ start, const int* __restrict__ brick_atoms, int fx, int fy, int fz,
+    int bx, int by, int bz, int nbx, int nby, int nbz, float* __restrict__ q)
+{
+    k_pme_spread_body(
+        atom_u, atom_fr, start, brick_atoms, fx, fy, fz, bx, by, bz, nbx, nby, nbz, q, blockIdx.x,
+        gridDim.x);
+}
+
+struct PmeSpreadArgs
+{
+    const int4* atom_u;
+    const float4* atom_fr;
+    const int* start;
+    const int* brick_atoms;
+    int fx;
+    int fy;
+    int fz;
+    int bx;
+    int by;
+    int bz;
+    int nbx;
+    int nby;
+    int nbz;
+    float* q;
+};
+
+__global__ void __launch_bounds__(256) k_pme_spread_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const PmeSpreadArgs& a = snb_seg<PmeSpreadArgs>(t, bid, nbid);
+    k_pme_spread_body(
+        a.atom_u, a.atom_fr, a.start, a.brick_atoms, a.fx, a.fy, a.fz, a.bx, a.by, a.bz, a.nbx,
+        a.nby, a.nbz, a.q, bid, nbid);
+}
+
 // PME_Final + device_add_force (PM_force.cpp:826-890, :1011-1019): one thread
 // per atom, stencil order kx, ky, kz (outer to inner), s = fma(phi, g, s).
 // Atoms are visited in brick order (the lists of this call), so the lanes of a
 // wavefront read neighbouring grid points.
-__global__ void __launch_bounds__(256) k_pme_gather(int n, const int* __restrict__ visit,
+__device__ __forceinline__ void k_pme_gather_body(int n, const int* __restrict__ visit,
                                                     const int4* __restrict__ atom_u,
                                                     const float4* __restrict__ atom_fr,
                                                     const float* __restrict__ phi, int fx,
                                                     int fy, int fz, SnbRc rc, float upd,
-                                                    int64_t* __restrict__ frc)
+                                                    int64_t* __restrict__ frc,
+    unsigned bid, unsigned nbid)
 {
-    const int blk = xcd_block((n + (int)blockDim.x - 1) / (int)blockDim.x);
+    const int blk = xcd_block((n + (int)blockDim.x - 1) / (int)blockDim.x, bid);
     if (blk < 0) return;
     const int v = blk * blockDim.x + threadIdx.x;
     if (v >= n) return;
@@ -827,6 +1093,40 @@ __global__ void __launch_bounds__(256) k_pme_gather(int n, const int* __restrict
     frc[i] += snb_to_fixed(mq * gx);
     frc[n + i] += snb_to_fixed(mq * gy);
     frc[2 * n + i] += snb_to_fixed(mq * gz);
+}
+
+__global__ void __launch_bounds__(256) k_pme_gather(int n, const int* __restrict__ visit,
+                                                    const int4* __restrict__ atom_u,
+                                                    const float4* __restrict__ atom_fr,
+                                                    const float* __restrict__ phi, int fx,
+                                                    int fy, int fz, SnbRc rc, float upd,
+                                                    int64_t* __restrict__ frc)
+{
+    k_pme_gather_body(
+        n, visit, atom_u, atom_fr, phi, fx, fy, fz, rc, upd, frc, blockIdx.x, gridDim.x);
+}
+
+struct PmeGatherArgs
+{
+    int n;
+    const int* visit;
+    const int4* atom_u;
+    const float4* atom_fr;
+    const float* phi;
+    int fx;
+    int fy;
+    int fz;
+    SnbRc rc;
+    float upd;
+    int64_t* frc;
+};
+
+__global__ void __launch_bounds__(256) k_pme_gather_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const PmeGatherArgs& a = snb_seg<PmeGatherArgs>(t, bid, nbid);
+    k_pme_gather_body(
+        a.n, a.visit, a.atom_u, a.atom_fr, a.phi, a.fx, a.fy, a.fz, a.rc, a.upd, a.frc, bid, nbid);
 }
 
 // ---------------------------------------------------------------------------
@@ -908,8 +1208,8 @@ int snb_pme_compute_bc(snb_context* ctx, const SnbBox& box)
     SnbPme& P = ctx->pme;
     const double beta = P.beta;
     const double mpre = M_PI * M_PI / (beta * beta);
-    hipLaunchKernelGGL(k_pme_bc, dim3(P.fx * P.fy), dim3(64), 0, ctx->stream, P.fx, P.fy, P.fc,
-                       P.bmod, box, mpre, P.volume, P.bc);
+    SNB_LAUNCH(k_pme_bc, dim3(P.fx * P.fy), dim3(64), 0,
+               ctx->stream, P.fx, P.fy, P.fc, P.bmod, box, mpre, P.volume, P.bc);
     SNB_CHECK(hipGetLastError());
     for (int c = 0; c < 6; c++) P.rcell[c] = box.rc[c];
     return SNB_OK;
@@ -999,19 +1299,20 @@ static int fft_yz_inverse(snb_context* ctx, float2* cplx, float* real)
     if (rc) return rc;
     if (ok)
     {
-        hipLaunchKernelGGL(k_fft_plane_inv, dim3(P.fx), dim3(1024), plane_lds(P.fy, P.fz, P.fc),
-                           s, cplx, real, P.fy, P.fz, P.fc, P.plan_z, P.plan_y, P.tw_z, P.tw_y);
+        SNB_LAUNCH_B(k_fft_plane_inv, k_fft_plane_inv_batch, PlaneInvArgs, dim3(P.fx), dim3(1024),
+                     plane_lds(P.fy, P.fz, P.fc), s, cplx, real, P.fy, P.fz, P.fc, P.plan_z,
+                     P.plan_y, P.tw_z, P.tw_y);
         SNB_CHECK(hipGetLastError());
         return SNB_OK;
     }
     int W = fft_w(P.fc, P.fy);
     int nchunk = (P.fc + W - 1) / W;
-    hipLaunchKernelGGL(k_fft_y, dim3(P.fx * nchunk), dim3(256), lds_lines(P.fy, W), s, cplx,
-                       P.fx, P.fy, P.fc, P.plan_y, P.tw_y, W, 1.0f);
+    SNB_LAUNCH_B(k_fft_y, k_fft_y_batch, FftYArgs, dim3(P.fx * nchunk), dim3(256),
+                 lds_lines(P.fy, W), s, cplx, P.fx, P.fy, P.fc, P.plan_y, P.tw_y, W, 1.0f);
     int nrow = P.fx * P.fy;
     int Lz = lines_per_block(P.fz);
-    hipLaunchKernelGGL(k_fft_z_c2r, dim3((nrow + Lz - 1) / Lz), dim3(256), lds_lines(P.fz, Lz),
-                       s, cplx, real, nrow, P.plan_z, P.fc, P.tw_z, Lz);
+    SNB_LAUNCH_B(k_fft_z_c2r, k_fft_z_c2r_batch, FftZc2rArgs, dim3((nrow + Lz - 1) / Lz),
+                 dim3(256), lds_lines(P.fz, Lz), s, cplx, real, nrow, P.plan_z, P.fc, P.tw_z, Lz);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -1024,12 +1325,12 @@ int snb_launch_fft_r2c(snb_context* ctx, const float* real, float2* cplx)
     // the LDS: line passes with small LDS share compute units with it
     int nrow = P.fx * P.fy;
     int Lz = lines_per_block(P.fz);
-    hipLaunchKernelGGL(k_fft_z_r2c, dim3((nrow + Lz - 1) / Lz), dim3(256), lds_lines(P.fz, Lz),
-                       s, real, cplx, nrow, P.plan_z, P.fc, P.tw_z, Lz);
+    SNB_LAUNCH_B(k_fft_z_r2c, k_fft_z_r2c_batch, FftZr2cArgs, dim3((nrow + Lz - 1) / Lz), dim3(256),
+                 lds_lines(P.fz, Lz), s, real, cplx, nrow, P.plan_z, P.fc, P.tw_z, Lz);
     int W = fft_w(P.fc, P.fy);
     int nchunk = (P.fc + W - 1) / W;
-    hipLaunchKernelGGL(k_fft_y, dim3(P.fx * nchunk), dim3(256), lds_lines(P.fy, W), s, cplx,
-                       P.fx, P.fy, P.fc, P.plan_y, P.tw_y, W, -1.0f);
+    SNB_LAUNCH_B(k_fft_y, k_fft_y_batch, FftYArgs, dim3(P.fx * nchunk), dim3(256),
+                 lds_lines(P.fy, W), s, cplx, P.fx, P.fy, P.fc, P.plan_y, P.tw_y, W, -1.0f);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -1068,8 +1369,8 @@ static int fft_x_plain(snb_context* ctx, float2* cplx, float sign)
     SnbPme& P = ctx->pme;
     int W = fft_w(P.fc, P.fx);
     int nchunk = (P.fc + W - 1) / W;
-    hipLaunchKernelGGL(k_fft_x, dim3(P.fy * nchunk), dim3(256), lds_lines(P.fx, W), ctx->stream,
-                       cplx, P.fx, P.fy, P.fc, P.plan_x, P.tw_x, W, sign);
+    SNB_LAUNCH(k_fft_x, dim3(P.fy * nchunk), dim3(256), lds_lines(P.fx, W),
+               ctx->stream, cplx, P.fx, P.fy, P.fc, P.plan_x, P.tw_x, W, sign);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -1107,28 +1408,28 @@ int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* c
         // brick counts are zero on entry (cleared at set-up and by the scan
         // of the previous call)
         if (P.nbrick <= SNB_PREP_HIST_MAX)
-            hipLaunchKernelGGL(k_pme_prep<true>, dim3((n + 255) / 256), dim3(256), 0, s, n, crd,
-                               charge, r6, P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nby, P.nbz,
-                               P.nbrick, P.brick_atoms, P.atom_u, P.atom_fr, P.atom_slot,
-                               P.brick_count);
+            SNB_LAUNCH_B(k_pme_prep<true>, k_pme_prep_batch<true>, PmePrepArgs,
+                         dim3((n + 255) / 256), dim3(256), 0, s, n, crd, charge, r6, P.fx, P.fy,
+                         P.fz, P.bx, P.by, P.bz, P.nby, P.nbz, P.nbrick, P.brick_atoms, P.atom_u,
+                         P.atom_fr, P.atom_slot, P.brick_count);
         else
-            hipLaunchKernelGGL(k_pme_prep<false>, dim3((n + 255) / 256), dim3(256), 0, s, n, crd,
-                               charge, r6, P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nby, P.nbz,
-                               P.nbrick, P.brick_atoms, P.atom_u, P.atom_fr, P.atom_slot,
-                               P.brick_count);
+            SNB_LAUNCH_B(k_pme_prep<false>, k_pme_prep_batch<false>, PmePrepArgs,
+                         dim3((n + 255) / 256), dim3(256), 0, s, n, crd, charge, r6, P.fx, P.fy,
+                         P.fz, P.bx, P.by, P.bz, P.nby, P.nbz, P.nbrick, P.brick_atoms, P.atom_u,
+                         P.atom_fr, P.atom_slot, P.brick_count);
         if ((rc = snb_launch_exclusive_scan_clear(s, P.brick_count, P.brick_start, P.nbrick)))
             return rc;
-        hipLaunchKernelGGL(k_pme_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, P.atom_u,
-                           P.atom_slot, P.brick_start, P.brick_atoms);
+        SNB_LAUNCH_B(k_pme_scatter, k_pme_scatter_batch, PmeScatterArgs, dim3((n + 255) / 256),
+                     dim3(256), 0, s, n, P.atom_u, P.atom_slot, P.brick_start, P.brick_atoms);
     }
     else
     {
         if ((rc = snb_launch_fill_int(s, P.brick_start, P.nbrick + 1, 0))) return rc;
     }
     size_t lds = sizeof(long long) * (size_t)P.bx * P.by * P.bz + 13 * sizeof(float) * SNB_SPREAD_LIST;
-    hipLaunchKernelGGL(k_pme_spread, dim3(xcd_grid(P.nbrick)), dim3(256), lds, s, P.atom_u, P.atom_fr,
-                       P.brick_start, P.brick_atoms, P.fx, P.fy, P.fz, P.bx, P.by, P.bz, P.nbx,
-                       P.nby, P.nbz, P.q);
+    SNB_LAUNCH_B(k_pme_spread, k_pme_spread_batch, PmeSpreadArgs, dim3(xcd_grid(P.nbrick)),
+                 dim3(256), lds, s, P.atom_u, P.atom_fr, P.brick_start, P.brick_atoms, P.fx, P.fy,
+                 P.fz, P.bx, P.by, P.bz, P.nbx, P.nby, P.nbz, P.q);
     SNB_CHECK(hipGetLastError());
     if ((rc = snb_launch_fft_r2c(ctx, P.q, P.spec))) return rc;
     int W = fft_w(P.fc, P.fx);
@@ -1136,31 +1437,30 @@ int snb_launch_pme_recip(snb_context* ctx, const snb_vector* crd, const float* c
     double mpre = M_PI * M_PI / ((double)P.beta * P.beta);
     bool ev = need_energy || need_virial;
     if (ev)
-        hipLaunchKernelGGL((k_fft_x_conv<true>), dim3(P.fy * nchunk), dim3(256),
-                           lds_lines(P.fx, W), s, P.spec, P.bc, P.fx, P.fy, P.fz, P.fc, P.plan_x,
-                           P.tw_x, W, (double)box.rc[0], (double)box.rc[1], (double)box.rc[2],
-                           (double)box.rc[3], (double)box.rc[4], (double)box.rc[5], mpre,
-                           P.partials);
+        SNB_LAUNCH_B((k_fft_x_conv<true>), (k_fft_x_conv_batch<true>), FftXConvArgs,
+                     dim3(P.fy * nchunk), dim3(256), lds_lines(P.fx, W), s, P.spec, P.bc, P.fx,
+                     P.fy, P.fz, P.fc, P.plan_x, P.tw_x, W, (double)box.rc[0], (double)box.rc[1],
+                     (double)box.rc[2], (double)box.rc[3], (double)box.rc[4], (double)box.rc[5],
+                     mpre, P.partials);
     else
-        hipLaunchKernelGGL((k_fft_x_conv<false>), dim3(P.fy * nchunk), dim3(256),
-                           lds_lines(P.fx, W), s, P.spec, P.bc, P.fx, P.fy, P.fz, P.fc, P.plan_x,
-                           P.tw_x, W, (double)box.rc[0], (double)box.rc[1], (double)box.rc[2],
-                           (double)box.rc[3], (double)box.rc[4], (double)box.rc[5], mpre,
-                           P.partials);
+        SNB_LAUNCH_B((k_fft_x_conv<false>), (k_fft_x_conv_batch<false>), FftXConvArgs,
+                     dim3(P.fy * nchunk), dim3(256), lds_lines(P.fx, W), s, P.spec, P.bc, P.fx,
+                     P.fy, P.fz, P.fc, P.plan_x, P.tw_x, W, (double)box.rc[0], (double)box.rc[1],
+                     (double)box.rc[2], (double)box.rc[3], (double)box.rc[4], (double)box.rc[5],
+                     mpre, P.partials);
     SNB_CHECK(hipGetLastError());
     if ((rc = fft_yz_inverse(ctx, P.spec, P.phi))) return rc;
     if (n > 0 && with_force)
-        hipLaunchKernelGGL(k_pme_gather, dim3(xcd_grid((n + 255) / 256)), dim3(256), 0, s, n,
-                           P.brick_atoms, P.atom_u,
-                           P.atom_fr, P.phi, P.fx, P.fy, P.fz, r6, (float)P.update_interval,
-                           ctx->fx_pme);
+        SNB_LAUNCH_B(k_pme_gather, k_pme_gather_batch, PmeGatherArgs,
+                     dim3(xcd_grid((n + 255) / 256)), dim3(256), 0, s, n, P.brick_atoms, P.atom_u,
+                     P.atom_fr, P.phi, P.fx, P.fy, P.fz, r6, (float)P.update_interval, ctx->fx_pme);
     snb_prof_end(ctx, SNB_PROF_PME);
     SNB_CHECK(hipGetLastError());
     if (ev)
     {
         double nfactor = -0.5 * M_PI / ((double)P.beta * P.beta * P.volume);
-        hipLaunchKernelGGL(k_pme_finalize, dim3(1), dim3(512), 0, s, P.partials, P.conv_blocks,
-                           ctx->dscal, ctx->scal, (double)P.beta, nfactor, need_energy ? 1 : 0);
+        SNB_LAUNCH(k_pme_finalize, dim3(1), dim3(512), 0, s, P.partials, P.conv_blocks,
+                   ctx->dscal, ctx->scal, (double)P.beta, nfactor, need_energy ? 1 : 0);
         SNB_CHECK(hipGetLastError());
     }
     return SNB_OK;
@@ -1172,8 +1472,8 @@ int snb_launch_pme_self(snb_context* ctx, const float* charge)
     int rc = snb_launch_charge_sums(ctx, charge);
     if (rc) return rc;
     double nfactor = -0.5 * M_PI / ((double)P.beta * P.beta * P.volume);
-    hipLaunchKernelGGL(k_pme_finalize, dim3(1), dim3(512), 0, ctx->stream, P.partials, 0,
-                       ctx->dscal, ctx->scal, (double)P.beta, nfactor, 1);
+    SNB_LAUNCH(k_pme_finalize, dim3(1), dim3(512), 0,
+               ctx->stream, P.partials, 0, ctx->dscal, ctx->scal, (double)P.beta, nfactor, 1);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -1186,8 +1486,7 @@ __global__ void k_pme_add_potential(const double* __restrict__ dscal, float* pot
 
 int snb_launch_pme_energy_to_potential(snb_context* ctx, float* potential)
 {
-    hipLaunchKernelGGL(k_pme_add_potential, dim3(1), dim3(1), 0, ctx->stream, ctx->dscal,
-                       potential);
+    SNB_LAUNCH(k_pme_add_potential, dim3(1), dim3(1), 0, ctx->stream, ctx->dscal, potential);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -1204,7 +1503,7 @@ __global__ void k_pme_add_virial(const double* __restrict__ dscal, snb_ltmatrix3
 
 int snb_launch_pme_virial_to(snb_context* ctx, snb_ltmatrix3* vir)
 {
-    hipLaunchKernelGGL(k_pme_add_virial, dim3(1), dim3(1), 0, ctx->stream, ctx->dscal, vir);
+    SNB_LAUNCH(k_pme_add_virial, dim3(1), dim3(1), 0, ctx->stream, ctx->dscal, vir);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }

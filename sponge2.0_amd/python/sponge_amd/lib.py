@@ -115,6 +115,13 @@ _SIGS = {
     "snb_f64_force": ([_P, _P, _P, LTMatrix3, LTMatrix3, _I, _I, _P, _I, _I, _P, _P], _I),
     "snb_f64_nve_iterate": ([_P, _P, _P, _P, _P, _P, LTMatrix3, LTMatrix3, ctypes.c_double],
                             _I),
+    "snb_batch_create": ([ctypes.POINTER(_P), _I], _I),
+    "snb_batch_destroy": ([_P], _I),
+    "snb_batch_record_begin": ([_P, _P], _I),
+    "snb_batch_record_end": ([_P], _I),
+    "snb_batch_plan": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
+    "snb_batch_launch": ([_P, _P], _I),
+    "snb_batch_info": ([_P] + [ctypes.POINTER(_I)] * 4, _I),
     "snb_profile_enable": ([_P, _I], _I),
     "snb_profile_read": ([_P, _I, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)], _I),
 }
@@ -696,3 +703,57 @@ class Nonbonded:
         check(self.L.snb_pme_reciprocal_virial(self.ctx, ctypes.cast(v, ctypes.c_void_p)),
               "snb_pme_reciprocal_virial")
         return np.array(v[:])
+
+
+class StepBatch:
+    """Several systems' MD steps in lockstep, one dispatch per kernel
+    (snb_batch_*).  record(nb, first_step, steps) records nb.md_step for
+    steps first_step .. first_step + steps - 1 (force-only, fused iteration)
+    without launching; plan() merges the records of all members; launch(stream)
+    issues them, as often as wanted (it can be captured in a HIP graph).
+    Every member's results are bit-identical to its own md_step calls.  The
+    members must stay alive while a launch of this batch may run."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        b = ctypes.c_void_p()
+        check(self.L.snb_batch_create(ctypes.byref(b), device), "snb_batch_create")
+        self.b = b
+        self.members = []
+        self.launches = self.batched = 0
+
+    def record(self, nb, first_step, steps):
+        check(self.L.snb_batch_record_begin(self.b, nb.ctx), "snb_batch_record_begin")
+        try:
+            for k in range(steps):
+                nb.md_step(step=first_step + k)
+        finally:
+            check(self.L.snb_batch_record_end(self.b), "snb_batch_record_end")
+        self.members.append(nb)
+
+    def plan(self):
+        a, b = _I(), _I()
+        check(self.L.snb_batch_plan(self.b, ctypes.byref(a), ctypes.byref(b)), "snb_batch_plan")
+        self.launches, self.batched = a.value, b.value
+        return self.launches, self.batched
+
+    def info(self):
+        v = [_I() for _ in range(4)]
+        check(self.L.snb_batch_info(self.b, *[ctypes.byref(x) for x in v]), "snb_batch_info")
+        return dict(zip(("members", "launches", "batched", "merged"), (x.value for x in v)))
+
+    def launch(self, stream):
+        check(self.L.snb_batch_launch(self.b, ctypes.c_void_p(stream.cuda_stream)),
+              "snb_batch_launch")
+
+    def close(self):
+        if self.b is not None:
+            self.L.snb_batch_destroy(self.b)
+            self.b = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -24,7 +24,7 @@ import time
 import numpy as np
 
 from . import autotune, systems
-from .lib import Nonbonded
+from .lib import Nonbonded, StepBatch
 
 WATER_VOLUME = 29.915      # A^3 per TIP3P water at 1.0 g/cm^3
 MAX_ATOMS_IN_CELL = 256    # the largest per-cell capacity the windowed list builder keeps
@@ -153,16 +153,62 @@ def estimate_footprint(n_atoms, max_nb=1200):
     return int(n_atoms * (max_nb * 6 + 4096) + 64 * 2 ** 20)
 
 
+class Member:
+    """An admitted request of the batched launch: its library context, built
+    and run for its first neighbour-list period (which builds the list) on
+    its own stream; `ready` fires when that period is done.  Its later
+    periods run inside the server's step batch."""
+
+    def __init__(self, req, chunk=10, tune=True, temperature=300.0):
+        import torch
+        self.req = req
+        s = req.system()
+        self.stream = torch.cuda.Stream()
+        nb = Nonbonded(s, cutoff=s["cutoff"], refresh_interval=chunk, stream=self.stream,
+                       max_atom_in_grid_numbers=MAX_ATOMS_IN_CELL)
+        nb.set_pme_overlap(False)  # one stream per member: the batch runs the lanes
+        nb.set_export(False)
+        self.nb = nb
+        tuned = autotune.tune_window(nb, interval=chunk) if tune else (None, None, None)
+        self.shape, self.direct_j = tuned[0], tuned[2]
+        with torch.cuda.stream(self.stream):
+            nb.md_initial(s, temperature=temperature, tau=0.1, seed=req.seed, dt=0.002)
+            for k in range(chunk):
+                nb.md_step(step=k, forced_nl=(k == 0))
+            self.ready = torch.cuda.Event()
+            self.ready.record(self.stream)
+        self.done = chunk       # steps queued so far
+        self.last = self.ready  # fires when the last queued step is done
+        self.joined = False
+
+    def collect(self):
+        nb = self.nb
+        self.req.result = dict(crd=nb.crd.cpu().numpy(), vel=nb.vel.cpu().numpy(),
+                               status=nb.status(), shape=self.shape, direct_j=self.direct_j,
+                               footprint=None)
+
+
 class Server:
     """Continuous batching: up to `slots` segments in flight, admitted in
-    arrival order while the HBM budget allows; each keeps up to `depth`
-    periods queued on its stream, so the device never waits for the host's
-    bookkeeping."""
+    arrival order while the HBM budget allows.
+
+    launch="batched" (default): the admitted segments advance together, one
+    neighbour-list period (chunk steps) at a time, as one step batch
+    (lib.StepBatch): each kernel of a step is one dispatch over all segments,
+    the reciprocal pipeline on a second stream beside the direct-space work,
+    and the period is one captured HIP graph, re-planned whenever a segment
+    joins or leaves.  launch="streams": each segment replays its own captured
+    period graph on its own stream.  Either way up to `depth` periods are
+    queued ahead, so the device does not wait for the host's bookkeeping,
+    and a segment's result is the same as when it runs alone."""
 
     def __init__(self, slots=8, hbm_budget=64 * 2 ** 30, chunk=10, tune=True, overlap=False,
-                 depth=2):
+                 depth=2, launch="batched", free_every=8 * 2 ** 30):
+        if launch not in ("batched", "streams"):
+            raise ValueError("launch is 'batched' or 'streams'")
         self.slots, self.hbm_budget, self.chunk = slots, hbm_budget, chunk
         self.tune, self.overlap, self.depth = tune, overlap, depth
+        self.launch, self.free_every = launch, free_every
 
     def run(self, requests, realtime=True, log=None):
         """Serves the requests (their .steps must be a multiple of chunk);
@@ -173,6 +219,8 @@ class Server:
             if r.steps % self.chunk:
                 raise ValueError("steps must be a multiple of the chunk")
             r.system()
+        if self.launch == "batched":
+            return self._run_batched(requests, realtime, log)
         pending = sorted(requests, key=lambda r: r.arrival)
         active = []
         used = 0
@@ -213,15 +261,133 @@ class Server:
             if busy:
                 time.sleep(0)
         torch.cuda.synchronize()
+        return self._summary(requests, t0, realtime, periods)
+
+    def _summary(self, requests, t0, realtime, periods, **extra):
         makespan = time.perf_counter() - t0
         lat = np.array([r.finished - r.arrival if realtime else r.finished for r in requests])
         sim_ns = sum(r.steps for r in requests) * NS_PER_STEP
-        return dict(requests=len(requests), makespan_s=makespan, periods=periods,
-                    aggregate_ns_per_day=sim_ns / makespan * 86400.0,
-                    p50_latency_s=float(np.percentile(lat, 50)),
-                    p99_latency_s=float(np.percentile(lat, 99)),
-                    atoms_total=int(sum(r.n_atoms for r in requests)),
-                    slots=self.slots)
+        out = dict(requests=len(requests), makespan_s=makespan, periods=periods,
+                   aggregate_ns_per_day=sim_ns / makespan * 86400.0,
+                   p50_latency_s=float(np.percentile(lat, 50)),
+                   p99_latency_s=float(np.percentile(lat, 99)),
+                   atoms_total=int(sum(r.n_atoms for r in requests)),
+                   slots=self.slots, launch=self.launch)
+        out.update(extra)
+        return out
+
+    def _run_batched(self, requests, realtime, log):
+        import torch
+        chunk = self.chunk
+        pending = sorted(requests, key=lambda r: r.arrival)
+        active = []      # members in the step batch
+        leaving = []     # members with every step queued, not yet complete
+        retired = []     # finished members and old batches / graphs, freed in groups
+        retired_bytes = 0
+        used = 0
+        stream = torch.cuda.Stream()
+        graph = None
+        dirty = False
+        inflight = []    # events of queued periods, oldest first
+        periods = plans = 0
+        launches = merged = 0
+
+        def free_retired():
+            # Freeing device memory waits for the device, so finished
+            # contexts, old plans and their graphs are freed in groups (after
+            # the queued periods, which may still use them), not one by one.
+            nonlocal retired_bytes
+            stream.synchronize()
+            for x in retired:
+                if isinstance(x, torch.cuda.CUDAGraph):
+                    x.reset()
+            for x in retired:
+                if not isinstance(x, torch.cuda.CUDAGraph):
+                    x.close()
+            retired.clear()
+            retired_bytes = 0
+
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
+        while pending or active or leaving:
+            now = clock()
+            while (pending and pending[0].arrival <= now and
+                   len(active) + len(leaving) < self.slots and
+                   used + retired_bytes + estimate_footprint(pending[0].n_atoms) <=
+                   self.hbm_budget):
+                r = pending.pop(0)
+                r.admitted = time.perf_counter() - t0
+                active.append(Member(r, chunk, self.tune))
+                used += estimate_footprint(r.n_atoms)
+                dirty = True
+            if (pending and pending[0].arrival <= now and retired and
+                    len(active) + len(leaving) < self.slots):
+                free_retired()   # memory, not slots, holds the next request back
+            for m in list(leaving):
+                if m.last.query():
+                    m.req.finished = time.perf_counter() - t0
+                    m.collect()
+                    leaving.remove(m)
+                    used -= estimate_footprint(m.req.n_atoms)
+                    retired.append(m.nb)
+                    retired_bytes += estimate_footprint(m.req.n_atoms)
+                    if log:
+                        log(m.req)
+            if retired_bytes > self.free_every:
+                free_retired()
+            while inflight and inflight[0].query():
+                inflight.pop(0)
+            if dirty and active:
+                # re-plan: each member records its next period
+                batch = StepBatch()
+                for m in active:
+                    batch.record(m.nb, m.done, chunk)
+                batch.plan()
+                info = batch.info()
+                launches, merged = info["launches"], info["merged"]
+                for m in active:
+                    if not m.joined:
+                        stream.wait_event(m.ready)
+                        m.joined = True
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.stream(stream):
+                    g.capture_begin()
+                    batch.launch(stream)
+                    g.capture_end()
+                if graph is not None:
+                    retired.extend(graph)
+                graph = (batch, g)
+                plans += 1
+                dirty = False
+            busy = False
+            while active and not dirty and len(inflight) < self.depth:
+                with torch.cuda.stream(stream):
+                    graph[1].replay()
+                    ev = torch.cuda.Event()
+                    ev.record(stream)
+                inflight.append(ev)
+                periods += 1
+                busy = True
+                for m in list(active):
+                    m.done += chunk
+                    m.last = ev
+                    if m.done >= m.req.steps:
+                        active.remove(m)
+                        leaving.append(m)
+                        dirty = True
+            if not busy:
+                if not active and not leaving and pending:
+                    time.sleep(max(0.0, min(0.002, pending[0].arrival - clock())))
+                else:
+                    time.sleep(0)
+        torch.cuda.synchronize()
+        out = self._summary(requests, t0, realtime, periods, plans=plans,
+                            last_plan={"launches": launches, "member_launches": merged})
+        if graph is not None:
+            retired.extend(graph)
+        free_retired()
+        return out
 
 
 def main(argv=None):
@@ -235,14 +401,15 @@ def main(argv=None):
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--rate", type=float, default=40.0)
     ap.add_argument("--overlap", action="store_true",
-                    help="reciprocal pipeline of each segment on its own stream")
+                    help="streams launch: reciprocal pipeline of each segment on its own stream")
+    ap.add_argument("--launch", choices=("batched", "streams"), default="batched")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--seed", type=int, default=20260217)
     a = ap.parse_args(argv)
     reqs = make_requests(count=a.requests, steps=a.steps, rate=a.rate, seed=a.seed)
     if not a.no_tune:
         pretune(reqs)
-    out = Server(slots=a.slots, tune=not a.no_tune, overlap=a.overlap).run(reqs)
+    out = Server(slots=a.slots, tune=not a.no_tune, overlap=a.overlap, launch=a.launch).run(reqs)
     out["status_flags"] = int(sum(r.result["status"] for r in reqs))
     print(json.dumps(out))
 

@@ -95,8 +95,9 @@ def test_mixed_93k_nve_evolved_frame_vs_fp64():
 def test_served_segments_vs_fp64_and_alone():
     """Eight requests of 1,500-12,000 atoms (cutoff 9 A; the small boxes run
     the plain pair path, the larger ones the windowed path), 100 steps each:
-    served together and one at a time, every final frame is the same bit for
-    bit, and each one meets the tolerances against the FP64 oracle."""
+    served together as one step batch and one at a time on the per-segment
+    path, every final frame is the same bit for bit, and each one meets the
+    tolerances against the FP64 oracle."""
     from sponge_amd import serving
     from sponge_amd.lib import Nonbonded
 
@@ -106,9 +107,9 @@ def test_served_segments_vs_fp64_and_alone():
 
     serving.pretune(requests())
     mixed = requests()
-    out = serving.Server(slots=8).run(mixed, realtime=False)
+    out = serving.Server(slots=8, launch="batched").run(mixed, realtime=False)
     alone = requests()
-    serving.Server(slots=1).run(alone, realtime=False)
+    serving.Server(slots=1, launch="streams").run(alone, realtime=False)
     print(out, flush=True)
     paths = set()
     for a, b in zip(alone, mixed):

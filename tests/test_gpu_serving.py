@@ -48,19 +48,21 @@ def test_window_shapes_give_identical_results(water_9k):
     nb.close()
 
 
-def test_segment_result_same_alone_and_in_mix():
+@pytest.mark.parametrize("launch", ["streams", "batched"])
+def test_segment_result_same_alone_and_in_mix(launch):
     """Three requests of 1,500 / 2,400 / 3,600 atoms, 30 steps each: served
-    one at a time and all three at once, the final coordinates and velocities
-    agree bit for bit."""
+    one at a time on the per-segment path and all three at once (per-segment
+    streams, or one step batch), the final coordinates and velocities agree
+    bit for bit."""
     from sponge_amd import serving
 
     def requests():
         return [serving.Request(k, n, 30, 0.0, 900 + k) for k, n in enumerate((1500, 2400, 3600))]
 
     alone = requests()
-    out1 = serving.Server(slots=1).run(alone, realtime=False)
+    out1 = serving.Server(slots=1, launch="streams").run(alone, realtime=False)
     mixed = requests()
-    out3 = serving.Server(slots=3).run(mixed, realtime=False)
+    out3 = serving.Server(slots=3, launch=launch).run(mixed, realtime=False)
     print(out1, out3, flush=True)
     for a, b in zip(alone, mixed):
         assert a.result["status"] == 0 and b.result["status"] == 0
@@ -68,3 +70,27 @@ def test_segment_result_same_alone_and_in_mix():
         np.testing.assert_array_equal(a.result["vel"], b.result["vel"])
         assert np.isfinite(a.result["crd"]).all()
     assert out3["requests"] == 3 and out3["p99_latency_s"] >= out3["p50_latency_s"] > 0
+
+
+def test_batched_server_staggered_admissions():
+    """Requests of different lengths (20-60 steps) arriving while others run:
+    the step batch is re-planned as segments join and leave, and each result
+    equals the segment run alone."""
+    from sponge_amd import serving
+
+    def requests():
+        return [serving.Request(k, n, st, 0.0, 700 + k)
+                for k, (n, st) in enumerate(((1600, 60), (5200, 20), (2600, 40), (7000, 30),
+                                             (3300, 50)))]
+
+    alone = requests()
+    serving.Server(slots=1, launch="streams").run(alone, realtime=False)
+    mixed = requests()
+    out = serving.Server(slots=3, launch="batched").run(mixed, realtime=False)
+    print(out, flush=True)
+    assert out["plans"] >= 3
+    for a, b in zip(alone, mixed):
+        print("case request", b.rid, b.n_atoms, b.steps, flush=True)
+        assert b.result["status"] == 0
+        np.testing.assert_array_equal(a.result["crd"], b.result["crd"])
+        np.testing.assert_array_equal(a.result["vel"], b.result["vel"])
