@@ -76,7 +76,9 @@ def parse():
     ap.add_argument("--serve-steps", type=int, default=1000,
                     help="MD steps per serving request (2 fs)")
     ap.add_argument("--serve-slots", type=int, default=8,
-                    help="segments in flight in the batched run")
+                    help="segments in flight in the per-segment streams run")
+    ap.add_argument("--serve-batch-slots", type=int, default=12,
+                    help="segments in flight in the step-batch run")
     ap.add_argument("--export-steps", type=int, default=None,
                     help="also time this many steps with the reference-layout list written at "
                          "every rebuild (the drop-in default); 0 skips (default 200, 0 for "
@@ -226,19 +228,25 @@ def serving_phase(args):
     # advances the admitted segments as one step batch (one dispatch per
     # kernel over all segments, one captured graph per period); the streams
     # run gives each segment its own stream and period graph instead.
-    for name, slots, overlap, launch in (("sequential", 1, True, "streams"),
-                                         ("batched", args.serve_slots, False, "batched"),
-                                         ("streams", args.serve_slots, False, "streams")):
+    # "plain": the sequential runs with the library's defaults, as a drop-in
+    # user gets them (default window shape, no tuner, reference-layout list
+    # written at every rebuild); "sequential": the same with the tuned shapes
+    # and the list export off.
+    for name, slots, overlap, launch, plain in (
+            ("plain", 1, True, "streams", True),
+            ("sequential", 1, True, "streams", False),
+            ("batched", args.serve_batch_slots, False, "batched", False),
+            ("streams", args.serve_slots, False, "streams", False)):
         rq = serving.make_requests(count=args.serve_requests, steps=args.serve_steps)
-        runs[name] = serving.Server(slots=slots, overlap=overlap,
-                                    launch=launch).run(rq, realtime=True)
+        runs[name] = serving.Server(slots=slots, overlap=overlap, launch=launch,
+                                    plain=plain).run(rq, realtime=True)
         reqs[name] = rq
-    same = sum(int(all(np.array_equal(a.result["crd"], b.result["crd"]) and
-                       np.array_equal(a.result["vel"], b.result["vel"])
-                       for b in (reqs["batched"][k], reqs["streams"][k])))
-               for k, a in enumerate(reqs["sequential"]))
+    same = sum(int(all(np.array_equal(a.result["crd"], reqs[o][k].result["crd"]) and
+                       np.array_equal(a.result["vel"], reqs[o][k].result["vel"])
+                       for o in ("sequential", "batched", "streams")))
+               for k, a in enumerate(reqs["plain"]))
     status = sum(r.result["status"] for v in reqs.values() for r in v)
-    seq, bat, strm = runs["sequential"], runs["batched"], runs["streams"]
+    pl, seq, bat, strm = runs["plain"], runs["sequential"], runs["batched"], runs["streams"]
     return {"requests": args.serve_requests, "steps_per_request": args.serve_steps,
             "atoms_total": bat["atoms_total"], "arrival": "Poisson, 40/s",
             "batched_aggregate_ns_per_day": bat["aggregate_ns_per_day"],
@@ -247,14 +255,24 @@ def serving_phase(args):
             "sequential_aggregate_ns_per_day": seq["aggregate_ns_per_day"],
             "sequential_p50_latency_s": seq["p50_latency_s"],
             "sequential_p99_latency_s": seq["p99_latency_s"],
-            "batched_over_sequential": bat["aggregate_ns_per_day"] / seq["aggregate_ns_per_day"],
+            "plain_aggregate_ns_per_day": pl["aggregate_ns_per_day"],
+            "batched_over_sequential": bat["aggregate_ns_per_day"] / pl["aggregate_ns_per_day"],
+            "batched_over_sequential_tuned": (bat["aggregate_ns_per_day"] /
+                                              seq["aggregate_ns_per_day"]),
+            "sequential_baselines": {"plain": "library defaults (default window shape, no "
+                                              "tuner, reference-layout list written at every "
+                                              "rebuild), reciprocal pipeline on its own stream",
+                                     "sequential": "tuned window shapes, list export off"},
             "streams_aggregate_ns_per_day": strm["aggregate_ns_per_day"],
-            "streams_over_sequential": strm["aggregate_ns_per_day"] / seq["aggregate_ns_per_day"],
-            "slots": args.serve_slots, "makespan_s": {"sequential": seq["makespan_s"],
+            "streams_over_sequential_tuned": (strm["aggregate_ns_per_day"] /
+                                              seq["aggregate_ns_per_day"]),
+            "slots": {"batched": args.serve_batch_slots, "streams": args.serve_slots},
+            "makespan_s": {"plain": pl["makespan_s"],
+                                                      "sequential": seq["makespan_s"],
                                                       "batched": bat["makespan_s"],
                                                       "streams": strm["makespan_s"]},
             "batched_plans": bat["plans"], "batched_last_plan": bat["last_plan"],
-            "same_alone_and_in_mix": "%d of %d requests bit-identical in both batched runs"
+            "same_alone_and_in_mix": "%d of %d requests bit-identical in all four runs"
                                      % (same, len(reqs["batched"])),
             "status_flags": status, "tune_s": tune_s,
             "shapes": {str(k): list(v[0]) + [v[2]] for k, v in tuned.items()},

@@ -96,6 +96,7 @@ int snb_batch_record_begin(snb_batch* b, snb_context* c)
         if (m == c) return SNB_ERR_ARGUMENT;
     b->member.push_back(c);
     b->rec.emplace_back();
+    b->rec.back().weight = (double)c->n;
     b->recording = (int)b->member.size() - 1;
     snb_rec = &b->rec.back();
     return SNB_OK;
@@ -185,11 +186,22 @@ int snb_batch_plan(snb_batch* b, int* launches, int* batched)
         p.lane = std::get<2>(key);
         std::vector<int> blk0(1, 0);
         std::vector<char> args;
+        // the members' workgroups, heaviest first: the dispatch's tail is
+        // then made of light workgroups
+        std::vector<int> take;
         for (int m = 0; m < M; m++)
         {
             if (cur[m] >= b->rec[m].ops.size()) continue;
             const SnbRecOp& o = b->rec[m].ops[cur[m]];
             if (std::make_tuple(o.bfn, o.block, o.lane) != key) continue;
+            take.push_back(m);
+        }
+        std::stable_sort(take.begin(), take.end(), [&](int x, int y) {
+            return b->rec[x].ops[cur[x]].weight > b->rec[y].ops[cur[y]].weight;
+        });
+        for (int m : take)
+        {
+            const SnbRecOp& o = b->rec[m].ops[cur[m]];
             cur[m]++;
             if (o.grid == 0) continue;
             if (p.nseg == 0)

@@ -337,6 +337,7 @@ struct SnbRecOp
     size_t lds = 0;
     int lane = 0;
     bool join = false, writes_inputs = false;
+    double weight = 0.0;  // work per workgroup (a batched dispatch starts the heaviest first)
 };
 struct SnbRecorder
 {
@@ -344,6 +345,8 @@ struct SnbRecorder
     int lane = 0;                // lane of the launches being recorded
     bool writes_inputs = false;  // they write coordinates
     int prev_lane = 0;
+    double weight = 0.0;         // the context's atom count
+    double weight_next = -1.0;   // set by a launcher for its next launch
 };
 // the recorder of the step being recorded on this thread, else nullptr
 extern thread_local SnbRecorder* snb_rec;
@@ -353,6 +356,8 @@ static inline void snb_rec_push(SnbRecOp&& op)
     op.lane = snb_rec->lane;
     op.join = op.lane == 0 && snb_rec->prev_lane == 1;
     op.writes_inputs = snb_rec->writes_inputs;
+    op.weight = snb_rec->weight_next >= 0.0 ? snb_rec->weight_next : snb_rec->weight;
+    snb_rec->weight_next = -1.0;
     snb_rec->prev_lane = op.lane;
     snb_rec->ops.push_back(std::move(op));
 }

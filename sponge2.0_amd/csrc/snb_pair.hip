@@ -1410,6 +1410,8 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
     const bool stream = !need_energy && !need_virial &&
                         (stream_mode > 0 || (stream_mode < 0 && n >= (1 << 18)));
     snb_prof_begin(ctx, SNB_PROF_PAIR);
+    // recorded step batches: the work of a workgroup is its rows
+    if (snb_rec) snb_rec->weight_next = (double)n / W.nblock;
 #define SNB_WIN_LAUNCH(E, V, O, ST)                                                             \
     SNB_LAUNCH_B((k_pair_win<E, V, O, ST>), (k_pair_win_batch<E, V, O, ST>), PairWinArgs,       \
                  dim3(W.nblock), dim3(1024), lds, s, n, W, L.cell_start, L.sorted_cell,        \
