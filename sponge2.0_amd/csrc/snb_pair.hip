@@ -1315,16 +1315,17 @@ __global__ void __launch_bounds__(1024) k_pair_win_batch(SnbSegTab t)
 
 // Per sorted atom: sum of its stage entries into the id-ordered accumulators
 // (plus the row-owner energies / virials on those steps).
-__global__ void k_stage_gather(int n, const int* __restrict__ inv_count,
+__device__ __forceinline__ void k_stage_gather_body(int n, const int* __restrict__ inv_count,
                                const int* __restrict__ inv_list, int invmax,
                                const long long* __restrict__ stage, size_t stage_cap,
                                const int* __restrict__ order,
                                int64_t* __restrict__ fx, int need_energy, int need_virial,
                                const int64_t* __restrict__ e_sorted,
                                const int64_t* __restrict__ v_sorted, int64_t* __restrict__ e_atom,
-                               int64_t* __restrict__ vir)
+                               int64_t* __restrict__ vir,
+    unsigned bid, unsigned nbid)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = bid * blockDim.x + threadIdx.x;
     if (s >= n) return;
     int id = order[s];
     int m = inv_count[s];
@@ -1349,6 +1350,47 @@ __global__ void k_stage_gather(int n, const int* __restrict__ inv_count,
     }
     if (need_virial)
         for (int c = 0; c < 6; c++) vir[(size_t)c * n + id] += v_sorted[(size_t)c * n + s];
+}
+
+__global__ void k_stage_gather(int n, const int* __restrict__ inv_count,
+                               const int* __restrict__ inv_list, int invmax,
+                               const long long* __restrict__ stage, size_t stage_cap,
+                               const int* __restrict__ order,
+                               int64_t* __restrict__ fx, int need_energy, int need_virial,
+                               const int64_t* __restrict__ e_sorted,
+                               const int64_t* __restrict__ v_sorted, int64_t* __restrict__ e_atom,
+                               int64_t* __restrict__ vir)
+{
+    k_stage_gather_body(
+        n, inv_count, inv_list, invmax, stage, stage_cap, order, fx, need_energy, need_virial,
+        e_sorted, v_sorted, e_atom, vir, blockIdx.x, gridDim.x);
+}
+
+struct StageGatherArgs
+{
+    int n;
+    const int* inv_count;
+    const int* inv_list;
+    int invmax;
+    const long long* stage;
+    size_t stage_cap;
+    const int* order;
+    int64_t* fx;
+    int need_energy;
+    int need_virial;
+    const int64_t* e_sorted;
+    const int64_t* v_sorted;
+    int64_t* e_atom;
+    int64_t* vir;
+};
+
+__global__ void k_stage_gather_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const StageGatherArgs& a = snb_seg<StageGatherArgs>(t, bid, nbid);
+    k_stage_gather_body(
+        a.n, a.inv_count, a.inv_list, a.invmax, a.stage, a.stage_cap, a.order, a.fx, a.need_energy,
+        a.need_virial, a.e_sorted, a.v_sorted, a.e_atom, a.vir, bid, nbid);
 }
 
 int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* charge,
@@ -1443,9 +1485,10 @@ int snb_launch_lj_window(snb_context* ctx, const snb_vector* crd, const float* c
     snb_prof_end(ctx, SNB_PROF_PAIR);
     SNB_CHECK(hipGetLastError());
     if (jdirect) return SNB_OK;
-    SNB_LAUNCH(k_stage_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, L.inv_count, L.inv_list,
-               L.invmax, L.stage, (size_t)n * L.invmax, L.order,
-               ctx->fx, need_energy, need_virial, ctx->e_sorted, ctx->v_sorted, ctx->e_atom, ctx->vir_atom);
+    SNB_LAUNCH_B(k_stage_gather, k_stage_gather_batch, StageGatherArgs, dim3((n + 255) / 256),
+                 dim3(256), 0, s, n, L.inv_count, L.inv_list, L.invmax, L.stage,
+                 (size_t)n * L.invmax, L.order, ctx->fx, need_energy, need_virial, ctx->e_sorted,
+                 ctx->v_sorted, ctx->e_atom, ctx->vir_atom);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
