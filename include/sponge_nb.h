@@ -193,6 +193,15 @@ int snb_lj_set_direct_j(snb_context* ctx, int mode);
  * either way; SNB_ERR_ARGUMENT for another mode (no counterpart in the
  * reference). */
 int snb_lj_set_row_stream(snb_context* ctx, int mode);
+/* Force-only steps of the row kernel (the plain path, used when the windowed
+ * path is off) for systems of at most 2,560 atoms: 1 = every atom's force
+ * accumulators in LDS, one workgroup per 64 rows and one global add per
+ * nonzero sum; 0 = global integer atomics per pair; -1 (default) = LDS only
+ * while recording a step batch, where several small systems share the
+ * device and the global atomics' throughput bounds the kernel (alone it is
+ * bound by their latency, and the two measure the same).  Results are the
+ * same either way (no counterpart in the reference). */
+int snb_lj_set_row_lds(snb_context* ctx, int mode);
 /* Geometry of the windowed pair path (windowed = 0 when not used). */
 int snb_nl_window_info(snb_context* ctx, int* windowed, int* a, int* b, int* s, int* wcap,
                        int* nblock);

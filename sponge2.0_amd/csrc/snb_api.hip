@@ -865,6 +865,14 @@ int snb_lj_set_row_stream(snb_context* c, int mode)
     return SNB_OK;
 }
 
+int snb_lj_set_row_lds(snb_context* c, int mode)
+{
+    if (!c) return SNB_ERR_ARGUMENT;
+    if (mode < -1 || mode > 1) return SNB_ERR_ARGUMENT;
+    c->lj.row_lds = mode;
+    return SNB_OK;
+}
+
 int snb_nl_window_info(snb_context* c, int* windowed, int* a, int* b, int* s, int* wcap,
                        int* nblock)
 {

@@ -118,6 +118,9 @@ struct SnbLj
     // window forces of force-only steps: -1 by size, 0 through the stage and
     // its per-atom gather, 1 by atomic adds at the end of each block
     int j_direct = -1;
+    // force-only row kernel of small systems with LDS accumulators: -1 inside
+    // recorded step batches only, 0 never, 1 always
+    int row_lds = -1;
 };
 
 // 1-4 pairs (NON_BOND_14): per pair the two atoms, A (x12 form), B (x6 form)

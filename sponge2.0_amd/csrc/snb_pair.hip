@@ -204,12 +204,128 @@ __global__ void __launch_bounds__(256) k_lj_direct_batch(SnbSegTab t)
         a.fx, a.e_atom, a.vir, bid, nbid);
 }
 
+// Force-only steps of the row kernel for small systems: every atom's force
+// accumulators in LDS (3 x n int64, n <= SNB_LJ_LDS_MAX), each workgroup a
+// run of SNB_LJ_LDS_ROWS consecutive rows, and one global add per nonzero
+// sum at the end.  The same integer sums as the global atomics of
+// k_lj_direct, with about a quarter of the global atomics: where several
+// small systems share the device (a step batch), the atomics' throughput,
+// not their latency, bounds the row kernel.
+#define SNB_LJ_LDS_MAX 2560
+#define SNB_LJ_LDS_ROWS 64
+__device__ __forceinline__ void k_lj_lds_body(
+    int n, const snb_atom_group* __restrict__ nl, const float4* __restrict__ atoms,
+    const int* __restrict__ type, const float* __restrict__ lj_a,
+    const float* __restrict__ lj_b, const float4* __restrict__ coul_tab, float cscale,
+    SnbBox box, float rc2, float beta, int64_t* __restrict__ fx, unsigned bid, unsigned nbid)
+{
+    extern __shared__ long long acc[];  // [3][n]
+    for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) acc[t] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int r0 = (int)bid * SNB_LJ_LDS_ROWS;
+    const int r1 = r0 + SNB_LJ_LDS_ROWS < n ? r0 + SNB_LJ_LDS_ROWS : n;
+    for (int i = r0 + wave; i < r1; i += nwave)
+    {
+        const snb_atom_group g = nl[i];
+        const float4 ai = atoms[i];
+        const int ti = type[i];
+        int64_t sx = 0, sy = 0, sz = 0;
+        for (int k = lane; k < g.atom_numbers; k += 64)
+        {
+            const int j = g.atom_serial[k];
+            const float4 aj = atoms[j];
+            float dx = aj.x - ai.x, dy = aj.y - ai.y, dz = aj.z - ai.z;
+            pbc_disp(dx, dy, dz, box);
+            float r2 = (dx * dx + dy * dy) + dz * dz;
+            if (!(r2 < rc2)) continue;
+            const int p = snb_lj_pair_index(ti, type[j]);
+            const float fa =
+                pair_force_factor(r2, lj_a[p], lj_b[p], ai.w * aj.w, beta, cscale, coul_tab);
+            const int64_t ix = snb_to_fixed(fa * dx), iy = snb_to_fixed(fa * dy),
+                          iz = snb_to_fixed(fa * dz);
+            sx += ix;
+            sy += iy;
+            sz += iz;
+            atomicAdd((unsigned long long*)&acc[j], (unsigned long long)(-ix));
+            atomicAdd((unsigned long long*)&acc[n + j], (unsigned long long)(-iy));
+            atomicAdd((unsigned long long*)&acc[2 * n + j], (unsigned long long)(-iz));
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        sz = wave_sum(sz);
+        if (lane == 0)
+        {
+            atomicAdd((unsigned long long*)&acc[i], (unsigned long long)sx);
+            atomicAdd((unsigned long long*)&acc[n + i], (unsigned long long)sy);
+            atomicAdd((unsigned long long*)&acc[2 * n + i], (unsigned long long)sz);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x)
+    {
+        const long long vx = acc[t], vy = acc[n + t], vz = acc[2 * n + t];
+        if (vx) atomic_add_i64(fx + t, vx);
+        if (vy) atomic_add_i64(fx + n + t, vy);
+        if (vz) atomic_add_i64(fx + 2 * n + t, vz);
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_lj_lds(
+    int n, const snb_atom_group* __restrict__ nl, const float4* __restrict__ atoms,
+    const int* __restrict__ type, const float* __restrict__ lj_a,
+    const float* __restrict__ lj_b, const float4* __restrict__ coul_tab, float cscale,
+    SnbBox box, float rc2, float beta, int64_t* __restrict__ fx)
+{
+    k_lj_lds_body(n, nl, atoms, type, lj_a, lj_b, coul_tab, cscale, box, rc2, beta, fx,
+                  blockIdx.x, gridDim.x);
+}
+
+struct LjLdsArgs
+{
+    int n;
+    const snb_atom_group* nl;
+    const float4* atoms;
+    const int* type;
+    const float* lj_a;
+    const float* lj_b;
+    const float4* coul_tab;
+    float cscale;
+    SnbBox box;
+    float rc2;
+    float beta;
+    int64_t* fx;
+};
+
+__global__ void __launch_bounds__(1024) k_lj_lds_batch(SnbSegTab t)
+{
+    unsigned bid, nbid;
+    const LjLdsArgs& a = snb_seg<LjLdsArgs>(t, bid, nbid);
+    k_lj_lds_body(a.n, a.nl, a.atoms, a.type, a.lj_a, a.lj_b, a.coul_tab, a.cscale, a.box,
+                  a.rc2, a.beta, a.fx, bid, nbid);
+}
+
 int snb_launch_lj_direct(snb_context* ctx, const snb_atom_group* nl, const SnbBox& box,
                          float beta, int need_energy, int need_virial)
 {
     int n = ctx->n;
     if (n <= 0) return SNB_OK;
     float rc2 = ctx->lj.cutoff * ctx->lj.cutoff;
+    // LDS accumulators: on request, or by default inside a recorded step batch
+    const int lds_mode = ctx->lj.row_lds;
+    if (!need_energy && !need_virial && n <= SNB_LJ_LDS_MAX &&
+        (lds_mode > 0 || (lds_mode < 0 && snb_rec)))
+    {
+        snb_prof_begin(ctx, SNB_PROF_PAIR);
+        SNB_LAUNCH_B(k_lj_lds, k_lj_lds_batch, LjLdsArgs,
+                     dim3((n + SNB_LJ_LDS_ROWS - 1) / SNB_LJ_LDS_ROWS), dim3(1024),
+                     sizeof(long long) * 3 * (size_t)n, ctx->stream, n, nl, ctx->atom4,
+                     ctx->type, ctx->lj.a, ctx->lj.b, ctx->lj.coul, ctx->lj.coul_scale, box, rc2,
+                     beta, ctx->fx);
+        snb_prof_end(ctx, SNB_PROF_PAIR);
+        SNB_CHECK(hipGetLastError());
+        return SNB_OK;
+    }
     int blocks = (n + 3) / 4;
     if (blocks > 65535 * 4) blocks = 65535 * 4;
     snb_prof_begin(ctx, SNB_PROF_PAIR);

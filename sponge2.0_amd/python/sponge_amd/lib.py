@@ -67,6 +67,7 @@ _SIGS = {
     "snb_nl_set_window_shape": ([_P, _I, _I, _I], _I),
     "snb_lj_set_direct_j": ([_P, _I], _I),
     "snb_lj_set_row_stream": ([_P, _I], _I),
+    "snb_lj_set_row_lds": ([_P, _I], _I),
     "snb_nl_grid_shape": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "snb_lj_initial": ([_P, _I, _P, _P, _P, _F], _I),
     "snb_lj_pme_direct_force": ([_P, _I, _I, _I, _I, _P, _P, _P, LTMatrix3, LTMatrix3, _P, _F,
@@ -638,6 +639,11 @@ class Nonbonded:
         """Force-only pair loop: 0 pipelined within each row, 1 across row
         ends, -1 by system size (default)."""
         check(self.L.snb_lj_set_row_stream(self.ctx, int(mode)), "snb_lj_set_row_stream")
+
+    def set_row_lds(self, mode):
+        """Row kernel of small systems on force-only steps: 1 LDS accumulators,
+        0 global atomics, -1 LDS only inside recorded step batches (snb_lj_set_row_lds)."""
+        check(self.L.snb_lj_set_row_lds(self.ctx, int(mode)), "snb_lj_set_row_lds")
 
     def window_info(self):
         v = [_I() for _ in range(6)]

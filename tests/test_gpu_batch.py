@@ -94,3 +94,23 @@ def test_step_batch_rejects_second_stream_and_double_record():
         b.plan()                     # planned once
     b.close()
     nb.close()
+
+
+def test_row_kernel_lds_accumulators_same_forces():
+    """The row kernel's LDS-accumulator form (small systems, force-only
+    steps) gives the same forces bit for bit as its global-atomic form."""
+    from sponge_amd import serving
+    from sponge_amd.lib import Nonbonded
+    s = serving.Request(0, 1600, 10, 0.0, 77).system()
+    nb = Nonbonded(s, cutoff=s["cutoff"], max_atom_in_grid_numbers=serving.MAX_ATOMS_IN_CELL)
+    assert nb.window_info()["windowed"] == 0      # a box of 4 cells per axis: the row kernel
+    got = []
+    for mode in (0, 1):
+        nb.set_row_lds(mode)
+        nb.zero_outputs()
+        nb.step(step=0, forced_nl=True)
+        nb.sync()
+        got.append(nb.frc.cpu().numpy().copy())
+    np.testing.assert_array_equal(got[0], got[1])
+    assert np.abs(got[0]).max() > 0 and nb.status() == 0
+    nb.close()
