@@ -574,7 +574,11 @@ int snb_profile_read(snb_context* ctx, int group, int reset, double* total_ms, i
  * one whole period).  Members need the reciprocal pipeline on the context
  * stream (snb_set_pme_overlap(ctx, 0)) and profiling off
  * (SNB_ERR_UNSUPPORTED otherwise).  A member must not be destroyed, and its
- * caller buffers not freed, while a plan that holds it may still run. */
+ * caller buffers not freed, while a plan that holds it may still run; the
+ * batch itself (its device table) likewise.  Recording is per thread (the
+ * calls between begin and end on the recording thread); make plans from one
+ * thread at a time.  If a recorded call fails, or snb_batch_plan does,
+ * destroy the batch: its records are incomplete. */
 typedef struct snb_batch snb_batch;
 int snb_batch_create(snb_batch** out, int device);
 int snb_batch_destroy(snb_batch* batch);

@@ -110,9 +110,29 @@ int snb_batch_record_end(snb_batch* b)
     return SNB_OK;
 }
 
+static int batch_plan(snb_batch* b);
+
 int snb_batch_plan(snb_batch* b, int* launches, int* batched)
 {
     if (!b || b->recording >= 0 || b->planned) return SNB_ERR_ARGUMENT;
+    b->plan.clear();
+    b->launches = b->batched = b->merged = 0;
+    const int rc = batch_plan(b);
+    if (rc)
+    {
+        // nothing of a failed plan is kept; the records stay for the caller
+        // to destroy the batch
+        b->plan.clear();
+        b->launches = b->batched = b->merged = 0;
+        return rc;
+    }
+    if (launches) *launches = b->launches;
+    if (batched) *batched = b->batched;
+    return SNB_OK;
+}
+
+static int batch_plan(snb_batch* b)
+{
     const int M = (int)b->member.size();
     std::vector<size_t> cur(M, 0);
     std::vector<char> host;
@@ -231,7 +251,7 @@ int snb_batch_plan(snb_batch* b, int* launches, int* batched)
     DeviceGuard g(b->device);
     int rc = SNB_OK;
     // dynamic LDS above 64 KiB is granted per kernel: the largest request
-    // of any plan so far
+    // of any plan so far (plans are made from one thread at a time)
     static std::map<std::pair<int, const void*>, size_t> granted;
     for (const auto& kv : lds_need)
     {
@@ -256,8 +276,6 @@ int snb_batch_plan(snb_batch* b, int* launches, int* batched)
     if (rc) return rc;
     b->planned = true;
     b->rec.clear();  // the plan holds what it needs
-    if (launches) *launches = b->launches;
-    if (batched) *batched = b->batched;
     return SNB_OK;
 }
 

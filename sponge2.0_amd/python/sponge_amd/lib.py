@@ -733,11 +733,17 @@ class StepBatch:
         try:
             for k in range(steps):
                 nb.md_step(step=first_step + k)
-        finally:
-            check(self.L.snb_batch_record_end(self.b), "snb_batch_record_end")
+        except BaseException:
+            # the member's record is incomplete: the batch cannot be planned
+            self.L.snb_batch_record_end(self.b)
+            self.close()
+            raise
+        check(self.L.snb_batch_record_end(self.b), "snb_batch_record_end")
         self.members.append(nb)
 
     def plan(self):
+        if self.b is None:
+            raise RuntimeError("the step batch was closed")
         a, b = _I(), _I()
         check(self.L.snb_batch_plan(self.b, ctypes.byref(a), ctypes.byref(b)), "snb_batch_plan")
         self.launches, self.batched = a.value, b.value
