@@ -578,7 +578,10 @@ int snb_profile_read(snb_context* ctx, int group, int reset, double* total_ms, i
  * batch itself (its device table) likewise.  Recording is per thread (the
  * calls between begin and end on the recording thread); make plans from one
  * thread at a time.  If a recorded call fails, or snb_batch_plan does,
- * destroy the batch: its records are incomplete. */
+ * destroy the batch: its records are incomplete.  Set-up calls take effect
+ * when made, not when a plan runs: make them outside a recording
+ * (snb_nl_set_window_shape and snb_pme_update_box return SNB_ERR_UNSUPPORTED
+ * inside one). */
 typedef struct snb_batch snb_batch;
 int snb_batch_create(snb_batch** out, int device);
 int snb_batch_destroy(snb_batch* batch);

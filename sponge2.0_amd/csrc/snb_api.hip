@@ -830,6 +830,7 @@ int snb_nl_set_window_capacity(snb_context* c, int slots)
 int snb_nl_set_window_shape(snb_context* c, int a, int b, int s)
 {
     if (!c) return SNB_ERR_ARGUMENT;
+    if (snb_rec) return SNB_ERR_UNSUPPORTED;  // changes buffers: not inside a recording
     SnbNl& L = c->nl;
     if (!L.init) return SNB_ERR_NOT_INITIALIZED;
     const bool plain = a == 0 && b == 0 && s == 0;
@@ -1537,6 +1538,7 @@ int snb_pme_update_box(snb_context* c, snb_ltmatrix3 cell, snb_ltmatrix3 rcell, 
     (void)g;  // g and dt move the domain-decomposition corners only (PM_force.cpp:1261-1262)
     (void)dt;
     if (!c) return SNB_ERR_ARGUMENT;
+    if (snb_rec) return SNB_ERR_UNSUPPORTED;  // changes module state: not inside a recording
     if (!c->pme.init) return SNB_ERR_NOT_INITIALIZED;
     if (!(cell.a11 > 0.0f && cell.a22 > 0.0f && cell.a33 > 0.0f)) return SNB_ERR_ARGUMENT;
     DeviceGuard guard(c->device);

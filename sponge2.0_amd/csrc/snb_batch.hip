@@ -6,8 +6,9 @@
 // calls), then merges them: launches of a kernel that has a batched form go
 // out as one dispatch over an argument table with one entry per member
 // (SnbSegTab), the rest as the single-context path makes them.  Each
-// member's launches keep their order, and the plan is issued on one stream,
-// so a member's arithmetic is that of its own launches: its results are
+// member's launches keep their order (the reciprocal pipeline's on a second
+// stream, ordered against the rest as a context's own second stream is), so
+// a member's arithmetic is that of its own launches: its results are
 // bit-identical to running it alone.
 #include <algorithm>
 #include <map>
