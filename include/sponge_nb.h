@@ -173,8 +173,8 @@ int snb_nl_set_window_capacity(snb_context* ctx, int slots);
  * preference list; a tuner can time the shapes this call accepts and keep the
  * fastest.  SNB_ERR_ARGUMENT when the grid cannot take the shape or its
  * window does not fit the LDS.  Synchronises; the next snb_nl_update
- * rebuilds.  Forces, energies and virials are the same for every shape (no
- * counterpart in the reference). */
+ * rebuilds (setting the shape in use changes nothing).  Forces, energies and
+ * virials are the same for every shape (no counterpart in the reference). */
 int snb_nl_set_window_shape(snb_context* ctx, int a, int b, int s);
 /* How the windowed pair kernel hands its window forces to the accumulators
  * on force-only steps: 0 = through a per-block stage and a per-atom gather

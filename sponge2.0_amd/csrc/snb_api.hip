@@ -839,6 +839,9 @@ int snb_nl_set_window_shape(snb_context* c, int a, int b, int s)
                    !window_shape(L.nx, L.ny, L.nz, a, b, s, (double)c->n / L.ncell, c->max_lds,
                                  1.0, &W)))
         return SNB_ERR_ARGUMENT;
+    // the shape in use: nothing to change
+    if (plain ? !L.windowed : (L.windowed && L.win.a == W.a && L.win.b == W.b && L.win.s == W.s))
+        return SNB_OK;
     DeviceGuard g(c->device);
     int rc = snb_join_pme(c);
     if (rc) return rc;

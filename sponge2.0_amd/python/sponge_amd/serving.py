@@ -172,15 +172,20 @@ class Member:
         nb.set_pme_overlap(False)  # one stream per member: the batch runs the lanes
         nb.set_export(False)
         self.nb = nb
-        tuned = autotune.tune_window(nb, interval=chunk) if tune else (None, None, None)
-        self.shape, self.direct_j = tuned[0], tuned[2]
         # In a step batch the pair dispatch holds the workgroups of every
         # member, so fewer, larger windows pay less per-workgroup set-up and
         # window hand-off: 2 x 2 x 2 where the grid takes it (measured on two
         # 8-member mixes: 4.47 -> 4.08 and 3.00 -> 2.95 ms per period against
-        # the shapes tuned alone; every other uniform shape was slower).
-        if tune and any(self.shape or ()) and nb.set_window_shape(*BATCH_SHAPE):
-            self.shape = BATCH_SHAPE
+        # the shapes tuned alone; every other uniform shape was slower), with
+        # the library's default hand-off (atomic adds below 65,536 atoms, the
+        # tuner's choice for every serving grid class); the tuner's shape
+        # otherwise.  One shape change per admission: each reallocates the
+        # window buffers.
+        if tune and nb.set_window_shape(*BATCH_SHAPE):
+            self.shape, self.direct_j = BATCH_SHAPE, -1
+        else:
+            tuned = autotune.tune_window(nb, interval=chunk) if tune else (None, None, None)
+            self.shape, self.direct_j = tuned[0], tuned[2]
         with torch.cuda.stream(self.stream):
             nb.md_initial(s, temperature=temperature, tau=0.1, seed=req.seed, dt=0.002)
             for k in range(chunk):
