@@ -329,11 +329,21 @@ int snb_langevin_parameters(snb_context* ctx, float* dt_internal, float* exp_gam
 /* SETTLE::Initial (constrain/settle.cpp:342-571) for rigid triangles: the
  * caller lists each triangle (A = the atom bonded to both others, B, C) with
  * its three constraint lengths, as the reference finds them among the
- * constraint pairs.  Two-atom rigid pairs are not supported here.  Zero
- * triangles leaves SETTLE uninitialised (its calls are no-ops). */
+ * constraint pairs.  Two-atom rigid pairs: snb_settle_pair_initial.  Zero
+ * triangles and zero pairs leave SETTLE uninitialised (its calls are no-ops). */
 int snb_settle_initial(snb_context* ctx, int triangle_numbers, const int* h_atom_A,
                        const int* h_atom_B, const int* h_atom_C, const float* h_rAB,
                        const float* h_rAC, const float* h_rBC, const float* h_mass);
+/* The rigid two-atom pairs of SETTLE::Initial (settle.cpp:342-571, CONSTRAIN_PAIR):
+ * atoms i and j at distance constant_r, constrain_k = 1/(m_i + m_j) from
+ * h_mass, as the reference finds them among the constraint pairs (no atom
+ * in two pairs, SNB_ERR_ARGUMENT otherwise).  snb_settle_remember keeps each
+ * pair's j - i and snb_settle_do applies settle_pair (:258-307) before the
+ * triangles, its virial included when pressure is asked for.  Zero pairs
+ * removes them.  The fused iteration (snb_md_iterate_fused) and the FP64 step
+ * take triangles only: SNB_ERR_UNSUPPORTED while pairs are set. */
+int snb_settle_pair_initial(snb_context* ctx, int pair_numbers, const int* h_atom_i,
+                            const int* h_atom_j, const float* h_constant_r, const float* h_mass);
 
 /* SETTLE::Remember_Last_Coordinates (settle.cpp:573-640): B - A and C - A of
  * every triangle before the iteration moves the atoms. */

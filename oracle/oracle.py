@@ -503,6 +503,10 @@ def _int_sigs():
     L.orc_settle_twin.argtypes = [ctypes.c_int, _u32p, _u32p, _u32p, _u32p, _u32p, _u32p, _u32p,
                                   _u32p, ctypes.c_float, ctypes.c_float, ctypes.c_float, _u32p,
                                   _u32p]
+    L.orc_settle_pair_remember.argtypes = [ctypes.c_int, _u32p, _u32p, _u32p, _u32p, _u32p]
+    L.orc_settle_pair_twin.argtypes = [ctypes.c_int, _u32p, _u32p, _u32p, _u32p, _u32p, _u32p,
+                                       _u32p, _u32p, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_float, _u32p, _u32p]
     L._int_sigs_done = True
     return L
 
@@ -554,6 +558,30 @@ def settle_remember(tri, crd, cell, rcell):
                                     _ptr(_c(cell, np.float32)), _ptr(_c(rcell, np.float32)),
                                     _ptr(ba), _ptr(ca))
     return ba, ca
+
+
+def settle_pair_remember(pairs, crd, cell, rcell):
+    """j - i (minimum image) of each rigid pair, float32 [m, 3]."""
+    pairs = _c(pairs, np.int32)
+    ab = np.zeros((pairs.shape[0], 3), np.float32)
+    _int_sigs().orc_settle_pair_remember(pairs.shape[0], _ptr(pairs), _ptr(_c(crd, np.float32)),
+                                         _ptr(_c(cell, np.float32)), _ptr(_c(rcell, np.float32)),
+                                         _ptr(ab))
+    return ab
+
+
+def settle_pair_twin(pairs, constant_r, constrain_k, mass, crd, cell, rcell, ab, dt, v_factor,
+                     x_factor, vel, virial=True):
+    """settle_pair in place on crd and vel (float32 [n,3]); returns the int64
+    constraint virial sums (6) or None."""
+    pairs = _c(pairs, np.int32)
+    vir = np.zeros(6, np.int64) if virial else None
+    _int_sigs().orc_settle_pair_twin(pairs.shape[0], _ptr(pairs), _ptr(_c(constant_r, np.float32)),
+                                     _ptr(_c(constrain_k, np.float32)), _ptr(_c(mass, np.float32)),
+                                     _ptr(crd), _ptr(_c(cell, np.float32)),
+                                     _ptr(_c(rcell, np.float32)), _ptr(_c(ab, np.float32)), dt,
+                                     v_factor, x_factor, _ptr(vel), _ptr(vir))
+    return vir
 
 
 def settle_twin(tri, geo, mass, crd, cell, rcell, ba, ca, dt, v_factor, x_factor, vel,

@@ -46,6 +46,8 @@
  *   orc_langevin_*          moments and a KS test of the normals; free-particle temperature
  *   orc_settle_*            closed-form TIP3P frame; restored geometry, centre of mass and
  *                           momentum after an unconstrained move
+ *   orc_settle_pair_*       restored pair distance, zero momentum change, correction along the
+ *                           remembered bond; a bond stretched along itself comes back exactly
  */
 #include <math.h>
 #include <stdint.h>
@@ -1878,5 +1880,63 @@ ORC_EXPORT void orc_settle_twin(int m, const int* tri, const float* geo, const f
         }
         for (int q = 0; q < 3; q++)
             for (int k = 0; k < 3; k++) crd[3 * at[q] + k] = r3[q][k] + rO[k];
+    }
+}
+
+/* remember_pair_AB (settle.cpp:25-44): ab[3m] = j - i (minimum image) */
+ORC_EXPORT void orc_settle_pair_remember(int m, const int* pairs, const float* crd,
+                                         const float* cell, const float* rcell, float* ab)
+{
+    for (int t = 0; t < m; t++)
+        pbc_disp_f(crd + 3 * pairs[2 * t + 1], crd + 3 * pairs[2 * t], cell, rcell, ab + 3 * t);
+}
+
+/* Float twin of settle_pair (settle.cpp:258-307): per pair the displacement
+ * k r2 (r2 the remembered j - i) with |r1 + k r2| = r0 for this step's
+ * j - i = r1, split by the inverse masses (constrain_k = 1/(m_i + m_j)); the
+ * velocities change by (v_factor / dt / x_factor) times the displacement;
+ * vir6 (+=, int64 at 2^32) gets the constraint force (kv m_j / dt / v_factor
+ * times j's displacement) times r2 in FP64 when vir6 != NULL. */
+ORC_EXPORT void orc_settle_pair_twin(int m, const int* pairs, const float* constant_r,
+                                     const float* constrain_k, const float* mass, float* crd,
+                                     const float* cell, const float* rcell, const float* ab,
+                                     float dt, float v_factor, float x_factor, float* vel,
+                                     int64_t* vir6)
+{
+    for (int t = 0; t < m; t++)
+    {
+        const int i = pairs[2 * t], j = pairs[2 * t + 1];
+        float r1[3];
+        pbc_disp_f(crd + 3 * j, crd + 3 * i, cell, rcell, r1);
+        const float* r2 = ab + 3 * t;
+        const float mA = mass[i], mB = mass[j];
+        const float r0r0 = constant_r[t] * constant_r[t];
+        const float r1r1 = v3dot(r1, r1), r1r2 = v3dot(r1, r2), r2r2 = v3dot(r2, r2);
+        const float k = (sqrtf((r1r2 * r1r2 - r1r1 * r2r2) + r2r2 * r0r0) - r1r2) / r2r2;
+        const float ci = (-mB) * constrain_k[t], cj = mA * constrain_k[t];
+        const float kv = (v_factor / dt) / x_factor;
+        float dj[3];
+        for (int q = 0; q < 3; q++)
+        {
+            const float kr2 = k * r2[q];
+            const float di = ci * kr2;
+            dj[q] = cj * kr2;
+            crd[3 * i + q] = crd[3 * i + q] + di;
+            crd[3 * j + q] = crd[3 * j + q] + dj[q];
+            vel[3 * i + q] = vel[3 * i + q] + kv * di;
+            vel[3 * j + q] = vel[3 * j + q] + kv * dj[q];
+        }
+        if (vir6)
+        {
+            const float kf = ((kv * mB) / dt) / v_factor;
+            const float f[3] = {kf * dj[0], kf * dj[1], kf * dj[2]};
+            const double v[6] = {(double)f[0] * r2[0],
+                                 (double)f[0] * r2[1] + (double)f[1] * r2[0],
+                                 (double)f[1] * r2[1],
+                                 (double)f[0] * r2[2] + (double)f[2] * r2[0],
+                                 (double)f[1] * r2[2] + (double)f[2] * r2[1],
+                                 (double)f[2] * r2[2]};
+            for (int q = 0; q < 6; q++) vir6[q] += to_fixed_d(v[q]);
+        }
     }
 }

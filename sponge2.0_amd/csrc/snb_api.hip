@@ -233,7 +233,8 @@ void free_langevin(SnbLangevin& T)
     T.init = false;
 }
 
-void free_settle(SnbSettle& S)
+// the triangles' state (snb_settle_initial)
+void free_settle_triangles(SnbSettle& S)
 {
     dfree(S.tri);
     dfree(S.geo);
@@ -246,6 +247,22 @@ void free_settle(SnbSettle& S)
     S.nfree = 0;
     S.disjoint = false;
     S.init = false;
+}
+
+// the pairs' state (snb_settle_pair_initial)
+void free_settle_pairs(SnbSettle& S)
+{
+    dfree(S.pairs);
+    dfree(S.pair_rk);
+    dfree(S.last_ab);
+    dfree(S.pair_vir6);
+    S.npair = 0;
+}
+
+void free_settle(SnbSettle& S)
+{
+    free_settle_triangles(S);
+    free_settle_pairs(S);
 }
 
 // Get_Rabcde_From_SSS (settle.cpp:50-98): the canonical frame of a triangle
@@ -1348,7 +1365,7 @@ int snb_settle_initial(snb_context* c, int triangle_numbers, const int* h_atom_A
     DeviceGuard g(c->device);
     snb_f64_free(c);  // the FP64 step's copy of the frames is made again on next use
     SnbSettle& S = c->settle;
-    free_settle(S);
+    free_settle_triangles(S);
     if (triangle_numbers == 0) return SNB_OK;  // not initialised: Do_SETTLE is a no-op
     const size_t m = (size_t)triangle_numbers;
     int rc = dalloc(&S.tri, m);
@@ -1358,7 +1375,7 @@ int snb_settle_initial(snb_context* c, int triangle_numbers, const int* h_atom_A
     if (!rc) rc = dalloc(&S.vir6, (size_t)6);
     if (rc)
     {
-        free_settle(S);
+        free_settle_triangles(S);
         return rc;
     }
     std::vector<int4> tri(m);
@@ -1394,7 +1411,7 @@ int snb_settle_initial(snb_context* c, int triangle_numbers, const int* h_atom_A
         S.nfree = (int)fr.size();
         if ((rc = dalloc(&S.free_atoms, fr.size() > 0 ? fr.size() : (size_t)1)))
         {
-            free_settle(S);
+            free_settle_triangles(S);
             return rc;
         }
         if (!fr.empty())
@@ -1406,11 +1423,55 @@ int snb_settle_initial(snb_context* c, int triangle_numbers, const int* h_atom_A
     return SNB_OK;
 }
 
+int snb_settle_pair_initial(snb_context* c, int pair_numbers, const int* h_atom_i,
+                            const int* h_atom_j, const float* h_constant_r, const float* h_mass)
+{
+    if (!c || pair_numbers < 0) return SNB_ERR_ARGUMENT;
+    if (pair_numbers > 0 && (!h_atom_i || !h_atom_j || !h_constant_r || !h_mass))
+        return SNB_ERR_ARGUMENT;
+    std::vector<char> used(c->n > 0 ? c->n : 1, 0);
+    for (int t = 0; t < pair_numbers; t++)
+    {
+        const int i = h_atom_i[t], j = h_atom_j[t];
+        if (i < 0 || i >= c->n || j < 0 || j >= c->n || i == j || used[i] || used[j])
+            return SNB_ERR_ARGUMENT;  // the reference's rigid pairs share no atom
+        if (!(h_constant_r[t] > 0.0f) || !(h_mass[i] + h_mass[j] > 0.0f)) return SNB_ERR_ARGUMENT;
+        used[i] = used[j] = 1;
+    }
+    DeviceGuard g(c->device);
+    SnbSettle& S = c->settle;
+    free_settle_pairs(S);
+    if (pair_numbers == 0) return SNB_OK;
+    const size_t m = (size_t)pair_numbers;
+    int rc = dalloc(&S.pairs, m);
+    if (!rc) rc = dalloc(&S.pair_rk, m);
+    if (!rc) rc = dalloc(&S.last_ab, m);
+    if (!rc) rc = dalloc(&S.pair_vir6, (size_t)6);
+    if (rc)
+    {
+        free_settle_pairs(S);
+        return rc;
+    }
+    std::vector<int2> pr(m);
+    std::vector<float2> rk(m);
+    for (size_t t = 0; t < m; t++)
+    {
+        pr[t] = make_int2(h_atom_i[t], h_atom_j[t]);
+        // constrain_k = 1 / (m_i + m_j) in float (settle.cpp:494-495)
+        rk[t] = make_float2(h_constant_r[t], 1.0f / (h_mass[h_atom_i[t]] + h_mass[h_atom_j[t]]));
+    }
+    SNB_CHECK(hipMemcpy(S.pairs, pr.data(), sizeof(int2) * m, hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemcpy(S.pair_rk, rk.data(), sizeof(float2) * m, hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemset(S.pair_vir6, 0, sizeof(int64_t) * 6));
+    S.npair = pair_numbers;
+    return SNB_OK;
+}
+
 int snb_settle_remember(snb_context* c, const snb_vector* d_crd, snb_ltmatrix3 cell,
                         snb_ltmatrix3 rcell)
 {
     if (!c) return SNB_ERR_ARGUMENT;
-    if (!c->settle.init) return SNB_OK;
+    if (!c->settle.init && c->settle.npair == 0) return SNB_OK;
     if (!d_crd) return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
     return snb_launch_settle_remember(c, d_crd, snb_box(cell, rcell));
@@ -1421,8 +1482,9 @@ int snb_settle_do(snb_context* c, const float* d_mass, snb_vector* d_crd, snb_lt
                   snb_ltmatrix3* d_stress, float dt, float v_factor, float x_factor)
 {
     if (!c) return SNB_ERR_ARGUMENT;
-    if (!c->settle.init) return SNB_OK;
-    if (!d_mass || !d_crd || !d_vel || !(dt > 0.0f) || !(x_factor > 0.0f))
+    if (!c->settle.init && c->settle.npair == 0) return SNB_OK;
+    if (!d_mass || !d_crd || !d_vel || !(dt > 0.0f) || !(x_factor > 0.0f) ||
+        (c->settle.npair > 0 && !(v_factor > 0.0f)))
         return SNB_ERR_ARGUMENT;
     DeviceGuard g(c->device);
     return snb_launch_settle(c, d_mass, d_crd, snb_box(cell, rcell), d_vel, need_pressure,
@@ -1493,6 +1555,7 @@ int snb_md_iterate_fused(snb_context* c, const float* d_mass, snb_vector* d_crd,
     if (!c->langevin.init) return SNB_ERR_NOT_INITIALIZED;
     if (c->n > 0 && (!d_crd || !d_vel || (c->settle.init && !d_mass))) return SNB_ERR_ARGUMENT;
     if (c->settle.init && !c->settle.disjoint) return SNB_ERR_UNSUPPORTED;
+    if (c->settle.npair > 0) return SNB_ERR_UNSUPPORTED;  // pairs: the separate calls
     DeviceGuard g(c->device);
     return snb_launch_md_fused(c, d_mass, d_crd, snb_box(cell, rcell), d_vel, need_pressure,
                                d_stress);

@@ -1006,6 +1006,7 @@ extern "C" int snb_f64_nve_iterate(snb_context* c, const double* d_mass,
     if (!c) return SNB_ERR_ARGUMENT;
     if (c->n > 0 && (!d_mass || !d_inverse_mass || !d_frc || !d_vel || !d_crd))
         return SNB_ERR_ARGUMENT;
+    if (c->settle.npair > 0) return SNB_ERR_UNSUPPORTED;  // rigid triangles only
     int dev_prev = -1;
     (void)hipGetDevice(&dev_prev);
     if (dev_prev != c->device) (void)hipSetDevice(c->device);

@@ -177,6 +177,12 @@ struct SnbSettle
     std::vector<double> geo_d;      // the canonical frames in double (FP64 step)
     int nfree = 0;
     bool disjoint = false;          // no atom in two triangles (needed by the fused iteration)
+    // rigid two-atom pairs (CONSTRAIN_PAIR)
+    int npair = 0;
+    int2* pairs = nullptr;          // atom i, atom j
+    float2* pair_rk = nullptr;      // constant_r, constrain_k = 1 / (m_i + m_j)
+    snb_vector* last_ab = nullptr;  // previous step's j - i
+    int64_t* pair_vir6 = nullptr;   // summed pair constraint virial, fixed point
 };
 
 struct SnbPme

@@ -151,6 +151,18 @@ __global__ void __launch_bounds__(256) k_settle_remember(int m, const int4* __re
     last_ca[t] = settle_disp(crd[a.z], rA, box);
 }
 
+// remember_pair_AB (settle.cpp:25-44): j - i (minimum image) before the update
+__global__ void __launch_bounds__(256) k_settle_pair_remember(int m, const int2* __restrict__ pairs,
+                                                              const snb_vector* __restrict__ crd,
+                                                              SnbBox box,
+                                                              snb_vector* __restrict__ last_ab)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int2 p = pairs[t];
+    last_ab[t] = settle_disp(crd[p.y], crd[p.x], box);
+}
+
 // One rigid triangle per thread (settle_triangle + SETTLE_DO_TRIANGLE,
 // settle.cpp:101-253, after Miyamoto & Kollman, J. Comput. Chem. 13, 952
 // (1992)).  rB0, rC0: the previous step's B - A and C - A; rA1, rB1, rC1:
@@ -263,6 +275,57 @@ __device__ __forceinline__ void settle_virial_sum(const int64_t* w, int64_t* __r
         if ((threadIdx.x & 63) == 0 && s != 0)
             atomicAdd((unsigned long long*)&vir6[k], (unsigned long long)s);
     }
+}
+
+// One rigid pair per thread (settle_pair, settle.cpp:258-307): with r1 this
+// step's unconstrained j - i and r2 the previous step's, the displacement
+// along r2 that restores |r1 + k r2| = r0, shared by the two atoms in inverse
+// proportion to their masses; the velocities take the same change over the
+// step, and the constraint force times r2 is the virial (in FP64, summed into
+// vir6 when need_virial).  Float operations in the reference's order.
+__global__ void __launch_bounds__(256) k_settle_pair(int m, const int2* __restrict__ pairs,
+                                                     const float2* __restrict__ rk,
+                                                     const float* __restrict__ mass,
+                                                     snb_vector* __restrict__ crd, SnbBox box,
+                                                     const snb_vector* __restrict__ last_ab,
+                                                     float dt, float v_factor, float x_factor,
+                                                     snb_vector* __restrict__ vel,
+                                                     int need_virial, int64_t* __restrict__ vir6)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t w[6] = {0, 0, 0, 0, 0, 0};
+    if (t < m)
+    {
+        const int2 p = pairs[t];
+        const float2 c = rk[t];
+        const snb_vector r1 = settle_disp(crd[p.y], crd[p.x], box);
+        const snb_vector r2 = last_ab[t];
+        const float mA = mass[p.x], mB = mass[p.y];
+        const float r0r0 = c.x * c.x, r1r1 = vdot(r1, r1), r1r2 = vdot(r1, r2),
+                    r2r2 = vdot(r2, r2);
+        const float k = (sqrtf((r1r2 * r1r2 - r1r1 * r2r2) + r2r2 * r0r0) - r1r2) / r2r2;
+        const snb_vector kr2 = vscale(k, r2);
+        const snb_vector di = vscale((-mB) * c.y, kr2);
+        const snb_vector dj = vscale(mA * c.y, kr2);
+        crd[p.x] = vadd(crd[p.x], di);
+        crd[p.y] = vadd(crd[p.y], dj);
+        const float kv = (v_factor / dt) / x_factor;
+        vel[p.x] = vadd(vel[p.x], vscale(kv, di));
+        vel[p.y] = vadd(vel[p.y], vscale(kv, dj));
+        if (need_virial)
+        {
+            // Get_Virial_From_Force_Dis(f, r2), f = (kv mB / dt / v_factor) dj
+            const snb_vector f = vscale(((kv * mB) / dt) / v_factor, dj);
+            const double v[6] = {(double)f.x * r2.x,
+                                 (double)f.x * r2.y + (double)f.y * r2.x,
+                                 (double)f.y * r2.y,
+                                 (double)f.x * r2.z + (double)f.z * r2.x,
+                                 (double)f.y * r2.z + (double)f.z * r2.y,
+                                 (double)f.z * r2.z};
+            for (int q = 0; q < 6; q++) w[q] = snb_to_fixed_d(v[q]);
+        }
+    }
+    if (need_virial) settle_virial_sum(w, vir6);
 }
 
 __global__ void __launch_bounds__(256) k_settle(int m, const int4* __restrict__ tri,
@@ -456,9 +519,12 @@ int snb_launch_langevin(snb_context* ctx, const snb_vector* frc, snb_vector* vel
 int snb_launch_settle_remember(snb_context* ctx, const snb_vector* crd, const SnbBox& box)
 {
     const SnbSettle& S = ctx->settle;
-    if (S.count <= 0) return SNB_OK;
-    SNB_LAUNCH(k_settle_remember, dim3((S.count + 255) / 256), dim3(256), 0,
-               ctx->stream, S.count, S.tri, crd, box, S.last_ba, S.last_ca);
+    if (S.npair > 0)
+        SNB_LAUNCH(k_settle_pair_remember, dim3((S.npair + 255) / 256), dim3(256), 0, ctx->stream,
+                   S.npair, S.pairs, crd, box, S.last_ab);
+    if (S.count > 0)
+        SNB_LAUNCH(k_settle_remember, dim3((S.count + 255) / 256), dim3(256), 0,
+                   ctx->stream, S.count, S.tri, crd, box, S.last_ba, S.last_ca);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
@@ -469,18 +535,29 @@ int snb_launch_settle(snb_context* ctx, const float* mass, snb_vector* crd, cons
 {
     SnbRecScope mark(0, true);  // writes coordinates (recorded step batches)
     const SnbSettle& S = ctx->settle;
-    if (S.count <= 0) return SNB_OK;
     const int want = need_pressure && stress ? 1 : 0;
-    SNB_LAUNCH(k_settle, dim3((S.count + 255) / 256), dim3(256), 0,
-               ctx->stream, S.count, S.tri, S.geo, mass, crd, box, S.last_ba, S.last_ca, dt, v_factor, x_factor, vel, want, S.vir6);
-    SNB_CHECK(hipGetLastError());
-    if (want)
+    // rcell a11 a22 a33 is 1/V for the lower-triangular cell
+    const float inv_volume = (box.rc[0] * box.rc[2]) * box.rc[5];
+    // pairs, then triangles (SETTLE::Do_SETTLE, settle.cpp:660-700); they share no atom
+    if (S.npair > 0)
     {
-        // rcell a11 a22 a33 is 1/V for the lower-triangular cell
-        const float inv_volume = (box.rc[0] * box.rc[2]) * box.rc[5];
-        SNB_LAUNCH(k_settle_stress, dim3(1), dim3(64), 0, ctx->stream, S.vir6, inv_volume, stress);
-        SNB_CHECK(hipGetLastError());
+        SNB_LAUNCH(k_settle_pair, dim3((S.npair + 255) / 256), dim3(256), 0, ctx->stream,
+                   S.npair, S.pairs, S.pair_rk, mass, crd, box, S.last_ab, dt, v_factor, x_factor,
+                   vel, want, S.pair_vir6);
+        if (want)
+            SNB_LAUNCH(k_settle_stress, dim3(1), dim3(64), 0, ctx->stream, S.pair_vir6,
+                       inv_volume, stress);
     }
+    if (S.count > 0)
+    {
+        SNB_LAUNCH(k_settle, dim3((S.count + 255) / 256), dim3(256), 0, ctx->stream, S.count,
+                   S.tri, S.geo, mass, crd, box, S.last_ba, S.last_ca, dt, v_factor, x_factor,
+                   vel, want, S.vir6);
+        if (want)
+            SNB_LAUNCH(k_settle_stress, dim3(1), dim3(64), 0, ctx->stream, S.vir6, inv_volume,
+                       stress);
+    }
+    SNB_CHECK(hipGetLastError());
     return SNB_OK;
 }
 

@@ -86,6 +86,7 @@ _SIGS = {
     "snb_langevin_iterate": ([_P, _P, _P, _P, _P], _I),
     "snb_langevin_parameters": ([_P] + [ctypes.POINTER(_F)] * 4, _I),
     "snb_settle_initial": ([_P, _I, _P, _P, _P, _P, _P, _P, _P], _I),
+    "snb_settle_pair_initial": ([_P, _I, _P, _P, _P, _P], _I),
     "snb_settle_remember": ([_P, _P, LTMatrix3, LTMatrix3], _I),
     "snb_settle_do": ([_P, _P, _P, LTMatrix3, LTMatrix3, _P, _I, _P, _F, _F, _F], _I),
     "snb_md_iterate_fused": ([_P, _P, _P, LTMatrix3, LTMatrix3, _P, _I, _P], _I),

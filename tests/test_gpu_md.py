@@ -116,6 +116,62 @@ def test_settle_bit_exact_with_twin(water_9k):
     nb.close()
 
 
+def test_settle_pair_bit_exact_with_twin(water_9k):
+    """Rigid two-atom pairs (settle_pair, settle.cpp:258-307): one O-H pair per
+    water of the 9k box (no triangles), remembered, moved at random, then
+    constrained on the device and by the float twin: coordinates, velocities
+    and the stress from the pair virial agree bit for bit; the fused
+    iteration refuses a context with pairs."""
+    import oracle
+    from sponge_amd import systems
+    from sponge_amd.lib import SnbError, check
+    s = water_9k
+    nb = _nb(s)
+    nb.md_initial(s, settle=False)
+    a, b, c, rab, rac, rbc = systems.water_triangles(s)
+    pairs = np.stack([a, b], 1).astype(np.int32)
+    r0 = np.asarray(rab, np.float32)
+    mass = np.asarray(s["mass"], np.float32)
+    pi = np.ascontiguousarray(pairs[:, 0])
+    pj = np.ascontiguousarray(pairs[:, 1])
+    check(nb.L.snb_settle_pair_initial(nb.ctx, len(pairs), pi.ctypes.data, pj.ctypes.data,
+                                       r0.ctypes.data, mass.ctypes.data),
+          "snb_settle_pair_initial")
+    ck = (np.float32(1.0) / (mass[pi] + mass[pj])).astype(np.float32)
+    crd0 = nb.crd.cpu().numpy().copy()
+    ab = oracle.settle_pair_remember(pairs, crd0, s["cell"], s["rcell"])
+    assert nb.L.snb_settle_remember(nb.ctx, ctypes.c_void_p(nb.crd.data_ptr()), nb.cell,
+                                    nb.rcell) == 0
+    rng = np.random.default_rng(9)
+    moved = (crd0 + rng.normal(0, 0.02, crd0.shape)).astype(np.float32)
+    vel = nb.vel.cpu().numpy().copy()
+    nb.crd.copy_(nb.torch.as_tensor(moved))
+    nb.stress.zero_()
+    assert nb.L.snb_settle_do(nb.ctx, ctypes.c_void_p(nb.mass.data_ptr()),
+                              ctypes.c_void_p(nb.crd.data_ptr()), nb.cell, nb.rcell,
+                              ctypes.c_void_p(nb.vel.data_ptr()), 1,
+                              ctypes.c_void_p(nb.stress.data_ptr()),
+                              ctypes.c_float(nb.dt_internal), ctypes.c_float(nb.v_factor),
+                              ctypes.c_float(nb.x_factor)) == 0
+    vir = oracle.settle_pair_twin(pairs, r0, ck, mass, moved, s["cell"], s["rcell"], ab,
+                                  np.float32(nb.dt_internal), np.float32(nb.v_factor),
+                                  np.float32(nb.x_factor), vel)
+    nb.sync()
+    np.testing.assert_array_equal(nb.crd.cpu().numpy(), moved)
+    np.testing.assert_array_equal(nb.vel.cpu().numpy(), vel)
+    inv_v = np.float32(np.float32(s["rcell"][0]) * np.float32(s["rcell"][2])) * \
+        np.float32(s["rcell"][5])
+    want = np.array([inv_v * np.float32(float(v) * 2.0 ** -32) for v in vir], np.float32)
+    np.testing.assert_array_equal(nb.stress.cpu().numpy(), want)
+    with pytest.raises(SnbError) as e:
+        check(nb.L.snb_md_iterate_fused(nb.ctx, ctypes.c_void_p(nb.mass.data_ptr()),
+                                        ctypes.c_void_p(nb.crd.data_ptr()), nb.cell, nb.rcell,
+                                        ctypes.c_void_p(nb.vel.data_ptr()), 0, None),
+              "snb_md_iterate_fused")
+    assert e.value.code == 4
+    nb.close()
+
+
 def _t_kin(nb, s):
     v = nb.vel.cpu().numpy().astype(np.float64)
     m = np.asarray(s["mass"], np.float64)

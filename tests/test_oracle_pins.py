@@ -685,3 +685,88 @@ def test_angle_force_is_minus_energy_gradient():
                      oracle.angle_f64(m.astype(np.float32), cell, rcell, ang)[1][0]) / (2 * h)
                 assert abs(f[atom, d] + g) < 2e-2 * max(1.0, np.abs(f).max()), (th, atom, d)
         assert np.abs(f.sum(0)).max() < 1e-9 * max(1.0, np.abs(f).max())
+
+
+def _pair_case(m=64, seed=5, r0=1.0, spread=0.03):
+    """Rigid pairs at distance r0 in random directions, remembered, then each
+    atom moved at random (a box large enough for no wrap)."""
+    import oracle
+    rng = np.random.default_rng(seed)
+    cell = np.array([60.0, 0, 60.0, 0, 0, 60.0], np.float32)
+    rcell = np.array([1 / 60.0, 0, 1 / 60.0, 0, 0, 1 / 60.0], np.float32)
+    u = rng.normal(size=(m, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    xi = rng.uniform(10, 50, (m, 3))
+    crd = np.empty((2 * m, 3), np.float32)
+    crd[0::2] = xi
+    crd[1::2] = xi + r0 * u
+    pairs = np.stack([np.arange(0, 2 * m, 2), np.arange(1, 2 * m, 2)], 1).astype(np.int32)
+    mass = rng.uniform(1.0, 16.0, 2 * m).astype(np.float32)
+    rk = np.full(m, r0, np.float32)
+    ck = (np.float32(1.0) / (mass[0::2] + mass[1::2])).astype(np.float32)
+    ab = oracle.settle_pair_remember(pairs, crd, cell, rcell)
+    moved = (crd + rng.normal(0, spread, crd.shape)).astype(np.float32)
+    return oracle, pairs, rk, ck, mass, moved, cell, rcell, ab
+
+
+def test_settle_pair_restores_distance_momentum_and_direction():
+    """settle_pair (settle.cpp:258-307): the pair distance is r0 again, the
+    mass-weighted displacement (and so the momentum change) is zero, and each
+    correction lies along the remembered bond j - i."""
+    oracle, pairs, rk, ck, mass, moved, cell, rcell, ab = _pair_case()
+    before = moved.copy()
+    vel = np.zeros_like(moved)
+    oracle.settle_pair_twin(pairs, rk, ck, mass, moved, cell, rcell, ab, np.float32(0.04),
+                            np.float32(1.0), np.float32(1.0), vel)
+    d = moved[pairs[:, 1]].astype(np.float64) - moved[pairs[:, 0]]
+    # coordinates up to 50 A: a float ulp there is 3.8e-6 A
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1.0, rtol=0, atol=4 * 3.8e-6)
+    # the velocity change is the displacement over the step, free of the
+    # coordinates' rounding: momentum and direction are checked on it
+    v = vel.astype(np.float64)
+    p = mass[pairs[:, 0], None] * v[pairs[:, 0]] + mass[pairs[:, 1], None] * v[pairs[:, 1]]
+    assert np.abs(p).max() <= 1e-6 * np.abs(mass[:, None] * v).max()
+    cr = np.cross(v[pairs[:, 1]], ab.astype(np.float64))
+    assert np.abs(cr).max() <= 1e-6 * np.abs(v).max() * np.abs(ab).max()
+    # and the coordinates moved by it times dt (v_factor = x_factor = 1), to
+    # their float resolution
+    np.testing.assert_allclose(moved - before, v * 0.04, rtol=0, atol=4 * 3.8e-6)
+
+
+def test_settle_pair_stretch_along_bond_and_virial():
+    """A bond stretched along itself comes back to r0 along the bond; the
+    virial sums equal (m_j dx_j / dt^2) (x) r2 summed over pairs (Get_Virial_From_Force_Dis
+    layout: xx, xy+yx, yy, xz+zx, yz+zy, zz) at v_factor = x_factor = 1."""
+    import oracle
+    cell = np.array([60.0, 0, 60.0, 0, 0, 60.0], np.float32)
+    rcell = np.array([1 / 60.0, 0, 1 / 60.0, 0, 0, 1 / 60.0], np.float32)
+    crd = np.array([[20, 20, 20], [21.2, 20.0, 20.0], [30, 30, 30], [30.6, 30.8, 30.0]],
+                   np.float32)
+    r0 = np.array([1.0, 1.0], np.float32)
+    crd[3] = crd[2] + np.array([0.6, 0.8, 0.0], np.float32)
+    pairs = np.array([[0, 1], [2, 3]], np.int32)
+    mass = np.array([16.0, 1.0, 12.0, 12.0], np.float32)
+    ck = (np.float32(1.0) / (mass[[0, 2]] + mass[[1, 3]])).astype(np.float32)
+    ref = crd.copy()
+    ref[1] = ref[0] + np.array([1.0, 0, 0], np.float32)      # pair 0 remembered at r0
+    ab = oracle.settle_pair_remember(pairs, ref, cell, rcell)
+    moved = crd.copy()
+    moved[3] = moved[2] + np.float32(1.3) * np.array([0.6, 0.8, 0.0], np.float32)  # stretched
+    vel = np.zeros_like(moved)
+    dt = np.float32(0.05)
+    vir = oracle.settle_pair_twin(pairs, r0, ck, mass, moved, cell, rcell, ab, dt,
+                                  np.float32(1.0), np.float32(1.0), vel)
+    np.testing.assert_allclose(np.linalg.norm(moved[1] - moved[0]), 1.0, rtol=1e-6)
+    np.testing.assert_allclose(np.linalg.norm(moved[3] - moved[2]), 1.0, rtol=1e-6)
+    assert abs(moved[1, 1] - 20.0) < 1e-6 and abs(moved[1, 2] - 20.0) < 1e-6
+    # m_j dx_j / dt^2 = m_j dv_j / dt (dv = dx / dt here), from the velocities,
+    # which carry no coordinate rounding
+    v = vel.astype(np.float64)
+    want = np.zeros((3, 3))
+    for t, (i, j) in enumerate(pairs):
+        f = mass[j] * v[j] / float(dt)
+        want += np.outer(f, ab[t].astype(np.float64))
+    got = np.asarray(vir, np.float64) * 2.0 ** -32
+    lay = np.array([want[0, 0], want[0, 1] + want[1, 0], want[1, 1], want[0, 2] + want[2, 0],
+                    want[1, 2] + want[2, 1], want[2, 2]])
+    np.testing.assert_allclose(got, lay, rtol=1e-5, atol=1e-6 * np.abs(lay).max())
