@@ -752,12 +752,30 @@ void SETTLE::Initial(CONTROLLER* c, snb_context* ctx, const NativeSystem& s)
         rbc.push_back(rlen[{std::min(b, d), std::max(b, d)}]);
         used[i] = used[j] = used[k] = 1;
     }
+    // rigid pairs: two atoms constrained to each other only (settle.cpp:432-445)
+    std::vector<int> PI, PJ;
+    std::vector<float> pr;
+    for (int i = 0; i < n; i++)
+    {
+        if (deg[i] != 1 || used[i]) continue;
+        const int j = link[2 * i];
+        if (j < 0 || deg[j] != 1 || used[j]) continue;
+        PI.push_back(i);
+        PJ.push_back(j);
+        pr.push_back(rlen[{std::min(i, j), std::max(i, j)}]);
+        used[i] = used[j] = 1;
+    }
     triangle_numbers = (int)A.size();
-    if (triangle_numbers == 0) return;
-    c->Check(snb_settle_initial(ctx, triangle_numbers, A.data(), B.data(), C.data(), rab.data(),
-                                rac.data(), rbc.data(), s.mass.data()),
-             "SETTLE::Initial");
-    is_initialized = true;
+    pair_numbers = (int)PI.size();
+    if (triangle_numbers > 0)
+        c->Check(snb_settle_initial(ctx, triangle_numbers, A.data(), B.data(), C.data(),
+                                    rab.data(), rac.data(), rbc.data(), s.mass.data()),
+                 "SETTLE::Initial");
+    if (pair_numbers > 0)
+        c->Check(snb_settle_pair_initial(ctx, pair_numbers, PI.data(), PJ.data(), pr.data(),
+                                         s.mass.data()),
+                 "SETTLE::Initial (pairs)");
+    is_initialized = triangle_numbers > 0 || pair_numbers > 0;
 }
 
 // ----------------------------------------------------- MIDDLE_Langevin_INFORMATION

@@ -166,12 +166,13 @@ struct BOND
     void Step_Print(CONTROLLER* controller, snb_context* ctx);
 };
 
-// SETTLE (constrain/settle.cpp) on the rigid triangles found among the
-// constraint pairs (bonds with an atom lighter than 3.3, constrain.cpp:78-117)
+// SETTLE (constrain/settle.cpp) on the rigid triangles and rigid pairs found
+// among the constraint pairs (bonds with an atom lighter than 3.3,
+// constrain.cpp:78-117)
 struct SETTLE
 {
     bool is_initialized = false;
-    int triangle_numbers = 0, constrain_pair_numbers = 0;
+    int triangle_numbers = 0, pair_numbers = 0, constrain_pair_numbers = 0;
     void Initial(CONTROLLER* controller, snb_context* ctx, const NativeSystem& sys);
 };
 
