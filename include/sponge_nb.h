@@ -53,7 +53,7 @@ typedef struct snb_context snb_context;
 #define SNB_ERR_ARGUMENT 1      /* bad argument value or NULL where required */
 #define SNB_ERR_HIP 2           /* a HIP runtime call failed */
 #define SNB_ERR_BOX_TOO_SMALL 3 /* fewer than 3 NL cells along an axis (neighbor_list.cpp:809-814) */
-#define SNB_ERR_UNSUPPORTED 4   /* ghosts, solvent split, triclinic NL, FFT size with a prime > 7 */
+#define SNB_ERR_UNSUPPORTED 4   /* ghosts, solvent split, FFT size with a prime > 7, ... */
 #define SNB_ERR_NOT_INITIALIZED 5
 #define SNB_ERR_OUT_OF_MEMORY 6
 #define SNB_ERR_DEVICE_STATUS 7 /* a sticky device flag is set, see snb_read_status */
@@ -106,7 +106,12 @@ int snb_set_exclusions(snb_context* ctx, const int* h_excluded_list_start,
  * (cutoff+skin)/2 wide; boxes with < 3 cells on an axis return
  * SNB_ERR_BOX_TOO_SMALL.  The windowed pair path (below) is chosen only when
  * max_atom_in_grid_numbers <= 256; a larger capacity keeps the plain path,
- * whose cells hold up to max_atom_in_grid_numbers atoms as the reference's. */
+ * whose cells hold up to max_atom_in_grid_numbers atoms as the reference's.
+ * A triclinic cell takes the plain path with the reference's neighbour-grid
+ * table (Find_Neighor_Grids_Device, :5-48, tilts truncated to whole grids):
+ * the list holds the pairs that table pairs up, which can leave out pairs
+ * within cutoff + skin for tilts below a grid width, as the reference's does;
+ * SNB_ERR_UNSUPPORTED when a grid would have more than 192 neighbour grids. */
 int snb_nl_initial(snb_context* ctx, float cutoff, float skin, snb_ltmatrix3 cell,
                    snb_ltmatrix3 rcell, int max_neighbor_numbers,
                    int max_atom_in_grid_numbers, int refresh_interval,

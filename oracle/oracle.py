@@ -132,6 +132,52 @@ def bc_table(fftx, ffty, fftz, rcell, beta, volume):
 # --------------------------------------------------------------------------
 # neighbor list
 # --------------------------------------------------------------------------
+def nl_cell_rule(crd, cell, cutoff, skin):
+    """The reference's grid neighbour rule for a lower-triangular cell
+    (neighbor_list.cpp:5-48 table, :206-231 binning): returns f(i, j) -> bool,
+    whether atom j's grid is in the table of atom i's grid.  Binning wraps
+    z, then y, then x by the cell rows and truncates by the grid length;
+    grid j is a neighbour of grid i when some image (sx, sy, sz in {-1, 0, 1})
+    puts i + shift - j in [lx, 2] x [ly, 2] x [-2, 2], the shift sx nx +
+    sy txy + sz txz, sy ny + sz tyz, sz nz with the tilts truncated to whole
+    grids, lx = -3 where sy a21 + sz a31 != 0, ly = -3 where sz a32 != 0."""
+    f = np.float32
+    c = np.asarray(cell, np.float32)
+    gl = (f(cutoff) + f(skin)) * f(0.5)
+    nx, ny, nz = (int(np.floor(c[k] / gl)) for k in (0, 2, 5))
+    r = np.array(crd, np.float32, copy=True)
+    k3 = np.floor(r[:, 2] / c[5])
+    r[:, 2] -= k3 * c[5]
+    r[:, 1] -= k3 * c[4]
+    k2 = np.floor(r[:, 1] / c[2])
+    r[:, 1] -= k2 * c[2]
+    r[:, 0] -= k3 * c[3] + k2 * c[1]
+    r[:, 0] -= np.floor(r[:, 0] / c[0]) * c[0]
+    g = np.stack([np.clip((r[:, 0] / gl).astype(np.int64), 0, nx - 1),
+                  np.clip((r[:, 1] / gl).astype(np.int64), 0, ny - 1),
+                  np.clip((r[:, 2] / gl).astype(np.int64), 0, nz - 1)], 1)
+    txy, txz, tyz = int(c[1] / gl), int(c[3] / gl), int(c[4] / gl)
+    # the rule depends on the two grids only: a table over grid pairs
+    gx, gy, gz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    cx, cy, cz = gx.ravel(), gy.ravel(), gz.ravel()
+    ok = np.zeros((cx.size, cx.size), bool)
+    for sx in (-1, 0, 1):
+        for sy in (-1, 0, 1):
+            for sz in (-1, 0, 1):
+                lx = -2 if f(sy) * c[1] + f(sz) * c[3] == 0 else -3
+                ly = -2 if f(sz) * c[4] == 0 else -3
+                dx = (cx + sx * nx + sy * txy + sz * txz)[:, None] - cx[None, :]
+                dy = (cy + sy * ny + sz * tyz)[:, None] - cy[None, :]
+                dz = (cz + sz * nz)[:, None] - cz[None, :]
+                ok |= (dx >= lx) & (dx <= 2) & (dy >= ly) & (dy <= 2) & (dz >= -2) & (dz <= 2)
+    gid = (g[:, 0] * ny + g[:, 1]) * nz + g[:, 2]
+
+    def rule(i, j):
+        """elementwise over arrays of atom indices too"""
+        return ok[gid[i], gid[j]]
+    return rule
+
+
 def nl_build(crd, cell, rcell, cutoff, skin, excl=None, max_nb=1200):
     """Returns (counts int32[n], rows int32[n, max_nb], overflow bool)."""
     crd = _c(crd, np.float32)

@@ -91,6 +91,9 @@ void free_window(SnbNl& L);
 
 void free_nl(SnbNl& L)
 {
+    dfree(L.nbr_cells);
+    dfree(L.nbr_count);
+    L.triclinic = false;
     dfree(L.cell_count);
     dfree(L.cell_start);
     dfree(L.atom_cell);
@@ -670,6 +673,60 @@ static int setup_window(snb_context* c)
     return SNB_OK;
 }
 
+// Find_Neighor_Grids_Device (neighbor_list.cpp:5-48) for a triclinic cell:
+// cell j is a neighbour of cell i when some periodic image (sx, sy, sz in
+// {-1, 0, 1}) puts j within [lx, 2] x [ly, 2] x [-2, 2] cells of i, an image
+// moving x by sx nx + sy txy + sz txz cells, y by sy ny + sz tyz and z by
+// sz nz, with the tilts in whole cells, (int)(a21 / grid_length) etc., and
+// lx = -3 (ly = -3) where the image moves x (y) by a tilt.  Each cell's list
+// in ascending cell order, as the reference's search over all cells finds it.
+static int triclinic_neighbour_cells(SnbNl& L, const snb_ltmatrix3& cell)
+{
+    const int nx = L.nx, ny = L.ny, nz = L.nz, ncell = L.ncell;
+    const int txy = (int)(cell.a21 / L.grid_length), txz = (int)(cell.a31 / L.grid_length),
+              tyz = (int)(cell.a32 / L.grid_length);
+    std::vector<int> cells((size_t)ncell * SNB_NL_MAX_NBR), count(ncell);
+    std::vector<int> mark(ncell, -1), found;
+    for (int ci = 0; ci < ncell; ci++)
+    {
+        const int ix = ci / (ny * nz), iy = (ci / nz) % ny, iz = ci % nz;
+        found.clear();
+        for (int sx = -1; sx <= 1; sx++)
+            for (int sy = -1; sy <= 1; sy++)
+                for (int sz = -1; sz <= 1; sz++)
+                {
+                    const int lx = sy * cell.a21 + sz * cell.a31 == 0.0f ? -2 : -3;
+                    const int ly = sz * cell.a32 == 0.0f ? -2 : -3;
+                    // delta = i + shift - j in [low, 2] per axis
+                    const int bx = ix + sx * nx + sy * txy + sz * txz;
+                    const int by = iy + sy * ny + sz * tyz;
+                    const int bz = iz + sz * nz;
+                    for (int jx = bx - 2; jx <= bx - lx; jx++)
+                        for (int jy = by - 2; jy <= by - ly; jy++)
+                            for (int jz = bz - 2; jz <= bz + 2; jz++)
+                            {
+                                if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz)
+                                    continue;
+                                const int cj = (jx * ny + jy) * nz + jz;
+                                if (mark[cj] == ci) continue;
+                                mark[cj] = ci;
+                                found.push_back(cj);
+                            }
+                }
+        if (found.size() > SNB_NL_MAX_NBR) return SNB_ERR_UNSUPPORTED;  // too skewed a cell
+        std::sort(found.begin(), found.end());
+        count[ci] = (int)found.size();
+        std::copy(found.begin(), found.end(), cells.begin() + (size_t)ci * SNB_NL_MAX_NBR);
+    }
+    int rc = dalloc(&L.nbr_cells, cells.size());
+    if (!rc) rc = dalloc(&L.nbr_count, (size_t)ncell);
+    if (rc) return rc;
+    SNB_CHECK(hipMemcpy(L.nbr_cells, cells.data(), sizeof(int) * cells.size(),
+                        hipMemcpyHostToDevice));
+    SNB_CHECK(hipMemcpy(L.nbr_count, count.data(), sizeof(int) * ncell, hipMemcpyHostToDevice));
+    return SNB_OK;
+}
+
 int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
                    snb_ltmatrix3 rcell, int max_neighbor_numbers, int max_atom_in_grid_numbers,
                    int refresh_interval, float skin_permit)
@@ -678,10 +735,10 @@ int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
     if (!c || !(cutoff > 0.0f) || !(skin >= 0.0f) || max_neighbor_numbers <= 0 ||
         max_atom_in_grid_numbers <= 0)
         return SNB_ERR_ARGUMENT;
-    if (!orthorhombic(cell)) return SNB_ERR_UNSUPPORTED;
     DeviceGuard g(c->device);
     SnbNl& L = c->nl;
     free_nl(L);
+    L.triclinic = !orthorhombic(cell);
     L.cutoff = cutoff;
     L.skin = skin;
     // neighbor_list.cpp:801-803, :705 -- the same float expressions
@@ -716,8 +773,14 @@ int snb_nl_initial(snb_context* c, float cutoff, float skin, snb_ltmatrix3 cell,
         free_nl(L);
         return rc;
     }
-    // the windowed kernels address the list with 32-bit byte offsets
-    L.windowed = window_list_fits(c) && max_atom_in_grid_numbers <= SNB_NLC_MAXA &&
+    if (L.triclinic && (rc = triclinic_neighbour_cells(L, cell)))
+    {
+        free_nl(L);
+        return rc;
+    }
+    // the windowed kernels address the list with 32-bit byte offsets; they
+    // take orthorhombic cells
+    L.windowed = !L.triclinic && window_list_fits(c) && max_atom_in_grid_numbers <= SNB_NLC_MAXA &&
                  choose_window(L.nx, L.ny, L.nz, (double)n / L.ncell, c->max_lds, 2 * c->num_cu,
                                &L.win);
     if (L.windowed && (rc = setup_window(c)))
@@ -749,7 +812,8 @@ int snb_nl_update(snb_context* c, const int* d_atom_local, int local_atom_number
     if (!c->nl.init) return SNB_ERR_NOT_INITIALIZED;
     if (ghost_numbers != 0 || d_atom_local) return SNB_ERR_UNSUPPORTED;
     if (local_atom_numbers != c->n || (c->n > 0 && !d_crd)) return SNB_ERR_ARGUMENT;
-    if (!orthorhombic(cell)) return SNB_ERR_UNSUPPORTED;
+    // a triclinic cell needs the neighbour-cell table made at snb_nl_initial
+    if (!orthorhombic(cell) && !c->nl.triclinic) return SNB_ERR_UNSUPPORTED;
     // cells keep the width of the initial box (grid_length) and the last one on
     // each axis takes the remainder; a box that shrank below n cells of that
     // width would leave a thinner cell and pairs beyond the +-2-cell stencil:

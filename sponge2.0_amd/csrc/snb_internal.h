@@ -41,6 +41,8 @@ enum SnbDoubleSlot
 // (the reference's default is 150), so its overflow flag keeps the meaning
 // "a cell held more than max_atom_in_grid_numbers atoms".
 #define SNB_NLC_MAXA 256
+// neighbour cells per cell of a triclinic grid (MAX_GRID_NEIGHBORS, neighbor_list.cpp:3)
+#define SNB_NL_MAX_NBR 192
 #define SNB_FFT_MAX_STAGES 16
 #define SNB_FFT_MAX_N 1024
 
@@ -72,6 +74,11 @@ struct SnbNl
     snb_atom_group* dnl = nullptr;  // n
     int* need = nullptr;         // rebuild flag (device)
     snb_vector* old_crd = nullptr;
+    // triclinic cell: each cell's neighbour cells from a table (the stencil
+    // is not a box of cells), SNB_NL_MAX_NBR per cell
+    bool triclinic = false;
+    int* nbr_cells = nullptr;  // ncell * SNB_NL_MAX_NBR
+    int* nbr_count = nullptr;  // ncell
     // windowed pair path (every axis >= 5 cells)
     bool windowed = false;
     int export_mode = 1;  // reference-layout list at rebuild: 0 no, 1 yes, 2 yes with sorted rows

@@ -302,6 +302,7 @@ __device__ __forceinline__ void k_nl_search_body(
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
     const int* __restrict__ ex_num, int max_nb, int* __restrict__ serial,
     snb_atom_group* __restrict__ nl, int* __restrict__ status,
+    const int* __restrict__ nbr_cells, const int* __restrict__ nbr_count,
     unsigned bid, unsigned nbid)
 {
     if (need[0] == 0) return;
@@ -344,55 +345,64 @@ __device__ __forceinline__ void k_nl_search_body(
     int* row = serial + (size_t)i * max_nb;
     int count = 0;
     const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int a = 0; a < nxs; a++)
-        for (int bq = 0; bq < nys; bq++)
+    // the candidates p0 .. p1 - 1 in cell order: the pairs of i among them
+    auto scan = [&](int p0, int p1) {
+        for (int base = p0; base < p1; base += 64)
         {
-            int col = (xs[a] * ny + ys[bq]) * nz;
-            for (int zr = 0; zr < nzr; zr++)
+            int p = base + lane;
+            bool ok = false;
+            int j = 0;
+            if (p < p1)
             {
-                int p0 = start[col + zr0[zr]], p1 = start[col + zr1[zr] + 1];
-                for (int base = p0; base < p1; base += 64)
+                float4 q = sorted_crd[p];
+                j = __float_as_int(q.w);
+                if (j > i)
                 {
-                    int p = base + lane;
-                    bool ok = false;
-                    int j = 0;
-                    if (p < p1)
+                    float dx = ri.x - q.x, dy = ri.y - q.y, dz = ri.z - q.z;
+                    float sx = (dx * box.rc[0] + dy * box.rc[1]) + dz * box.rc[3];
+                    float sy = dy * box.rc[2] + dz * box.rc[4];
+                    float sz = dz * box.rc[5];
+                    float kx = floorf(sx + 0.5f), ky = floorf(sy + 0.5f),
+                          kz = floorf(sz + 0.5f);
+                    dx = dx - ((kx * box.c[0] + ky * box.c[1]) + kz * box.c[3]);
+                    dy = dy - (ky * box.c[2] + kz * box.c[4]);
+                    dz = dz - kz * box.c[5];
+                    ok = (dx * dx + dy * dy) + dz * dz < cs2;
+                    if (ok && j >= elo && j <= ehi)
                     {
-                        float4 q = sorted_crd[p];
-                        j = __float_as_int(q.w);
-                        if (j > i)
-                        {
-                            float dx = ri.x - q.x, dy = ri.y - q.y, dz = ri.z - q.z;
-                            float sx = (dx * box.rc[0] + dy * box.rc[1]) + dz * box.rc[3];
-                            float sy = dy * box.rc[2] + dz * box.rc[4];
-                            float sz = dz * box.rc[5];
-                            float kx = floorf(sx + 0.5f), ky = floorf(sy + 0.5f),
-                                  kz = floorf(sz + 0.5f);
-                            dx = dx - ((kx * box.c[0] + ky * box.c[1]) + kz * box.c[3]);
-                            dy = dy - (ky * box.c[2] + kz * box.c[4]);
-                            dz = dz - kz * box.c[5];
-                            ok = (dx * dx + dy * dy) + dz * dz < cs2;
-                            if (ok && j >= elo && j <= ehi)
+                        for (int t = es; t < es + en; t++)
+                            if (ex_list[t] == j)
                             {
-                                for (int t = es; t < es + en; t++)
-                                    if (ex_list[t] == j)
-                                    {
-                                        ok = false;
-                                        break;
-                                    }
+                                ok = false;
+                                break;
                             }
-                        }
                     }
-                    uint64_t mask = __ballot(ok);
-                    if (ok)
-                    {
-                        int slot = count + __popcll(mask & lower);
-                        if (slot < max_nb) row[slot] = j;
-                    }
-                    count += __popcll(mask);
                 }
             }
+            uint64_t mask = __ballot(ok);
+            if (ok)
+            {
+                int slot = count + __popcll(mask & lower);
+                if (slot < max_nb) row[slot] = j;
+            }
+            count += __popcll(mask);
         }
+    };
+    if (nbr_cells)
+    {
+        // triclinic cell: the cell's neighbour cells from the table
+        const int* tab = nbr_cells + (size_t)c * SNB_NL_MAX_NBR;
+        const int m = nbr_count[c];
+        for (int k = 0; k < m; k++) scan(start[tab[k]], start[tab[k] + 1]);
+    }
+    else
+        for (int a = 0; a < nxs; a++)
+            for (int bq = 0; bq < nys; bq++)
+            {
+                int col = (xs[a] * ny + ys[bq]) * nz;
+                for (int zr = 0; zr < nzr; zr++)
+                    scan(start[col + zr0[zr]], start[col + zr1[zr] + 1]);
+            }
     if (lane == 0)
     {
         nl[i].atom_numbers = count < max_nb ? count : max_nb;
@@ -407,11 +417,12 @@ __global__ void __launch_bounds__(256) k_nl_search(
     const float4* __restrict__ sorted_crd, SnbBox box, float cs2, int nx, int ny, int nz,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list,
     const int* __restrict__ ex_num, int max_nb, int* __restrict__ serial,
-    snb_atom_group* __restrict__ nl, int* __restrict__ status)
+    snb_atom_group* __restrict__ nl, int* __restrict__ status,
+    const int* __restrict__ nbr_cells, const int* __restrict__ nbr_count)
 {
     k_nl_search_body(
         need, n, crd, atom_cell, start, sorted_crd, box, cs2, nx, ny, nz, ex_start, ex_list, ex_num,
-        max_nb, serial, nl, status, blockIdx.x, gridDim.x);
+        max_nb, serial, nl, status, nbr_cells, nbr_count, blockIdx.x, gridDim.x);
 }
 
 struct NlSearchArgs
@@ -434,6 +445,8 @@ struct NlSearchArgs
     int* serial;
     snb_atom_group* nl;
     int* status;
+    const int* nbr_cells;
+    const int* nbr_count;
 };
 
 __global__ void __launch_bounds__(256) k_nl_search_batch(SnbSegTab t)
@@ -442,7 +455,8 @@ __global__ void __launch_bounds__(256) k_nl_search_batch(SnbSegTab t)
     const NlSearchArgs& a = snb_seg<NlSearchArgs>(t, bid, nbid);
     k_nl_search_body(
         a.need, a.n, a.crd, a.atom_cell, a.start, a.sorted_crd, a.box, a.cs2, a.nx, a.ny, a.nz,
-        a.ex_start, a.ex_list, a.ex_num, a.max_nb, a.serial, a.nl, a.status, bid, nbid);
+        a.ex_start, a.ex_list, a.ex_num, a.max_nb, a.serial, a.nl, a.status, a.nbr_cells,
+        a.nbr_count, bid, nbid);
 }
 
 int snb_launch_nl_update(snb_context* ctx, const snb_vector* crd, const SnbBox& box, int forced,
@@ -487,7 +501,8 @@ int snb_launch_nl_update(snb_context* ctx, const snb_vector* crd, const SnbBox& 
     else
         SNB_LAUNCH_B(k_nl_search, k_nl_search_batch, NlSearchArgs, dim3((n + 3) / 4), dim3(256), 0,
                      s, L.need, n, crd, L.atom_cell, L.cell_start, L.sorted_crd, box, L.cs2, L.nx,
-                     L.ny, L.nz, ex_start, ex_list, ex_num, L.max_nb, L.serial, L.dnl, ctx->status);
+                     L.ny, L.nz, ex_start, ex_list, ex_num, L.max_nb, L.serial, L.dnl, ctx->status,
+                     L.nbr_cells, L.nbr_count);
     snb_prof_end(ctx, SNB_PROF_NL);
     SNB_CHECK(hipGetLastError());
     return SNB_OK;

@@ -925,3 +925,50 @@ def test_gpu_forces_on_waters_vs_gromacs(it):
     mx, rms, cos = force_stats(gmx[w], (f + fa + fu)[w])
     print("iteration", it, "water atoms: max", mx, "rms", rms, "cos", cos, flush=True)
     assert mx <= 0.07 and rms <= 0.01 and cos >= 0.999
+
+
+def _triclinic(s, angle=(85.0, 87.0, 84.0)):
+    """The system's box lengths with angles: a lower-triangular cell with
+    positive tilts (native_io.box_to_cell, MD_core/pbc.hpp:66-101)."""
+    from sponge_amd import native_io
+    t = dict(s)
+    t["cell"], t["rcell"] = native_io.box_to_cell(s["box"], angle)
+    t["angle"] = angle
+    return t
+
+
+@pytest.mark.parametrize("name", ["small_water", "water_9k"])
+def test_triclinic_neighbor_list_and_pair_kernel(request, name):
+    """A triclinic cell (neighbor_list.cpp:5-48, :206-231): the list holds the
+    pairs within rc + skin (the oracle's brute-force set under the triclinic
+    minimum image) whose grids the reference's neighbour-grid table pairs up
+    (oracle.nl_cell_rule), every row of it within the brute-force row; with
+    three grids per axis every grid is a neighbour and the two sets are equal.
+    The row kernel's forces over the list equal the float twin's bit for bit."""
+    import oracle
+    s = _triclinic(_get(request, name))
+    nb = _nb(s)
+    print("case", name, "cell", s["cell"], flush=True)
+    assert nb.window_info()["windowed"] == 0     # triclinic cells take the row kernel
+    nb.nl_update(forced=True)
+    gc, gr = nb.nl_rows()
+    assert nb.status() == 0
+    oc, orows, ov = oracle.nl_build(s["crd"], s["cell"], s["rcell"], nb.cutoff, 2.0, _excl(s),
+                                    max_nb=nb.max_nb)
+    assert not ov
+    rule = oracle.nl_cell_rule(s["crd"], s["cell"], nb.cutoff, 2.0)
+    dropped = 0
+    for i, (a, b) in enumerate(zip(_sorted_rows(gc, gr), _sorted_rows(oc, orows))):
+        want = b[rule(np.full(len(b), i), b)]
+        dropped += len(b) - len(want)
+        assert np.array_equal(a, want), "row %d differs" % i
+    print("pairs within rc + skin outside the grid table:", dropped, flush=True)
+    if name == "small_water":
+        assert dropped == 0
+    nb.lj_direct(frc=False)
+    g_f = nb.fixed_forces()
+    t_f, _, _, _ = oracle.pair_twin(s["crd"], s["charge"], s["lj_type"], s["lj_a"], s["lj_b"],
+                                    s["cell"], s["rcell"], nb.cutoff, nb.beta, gc, gr,
+                                    virial=True)
+    np.testing.assert_array_equal(g_f, t_f)
+    nb.close()

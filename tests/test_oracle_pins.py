@@ -770,3 +770,23 @@ def test_settle_pair_stretch_along_bond_and_virial():
     lay = np.array([want[0, 0], want[0, 1] + want[1, 0], want[1, 1], want[0, 2] + want[2, 0],
                     want[1, 2] + want[2, 1], want[2, 2]])
     np.testing.assert_allclose(got, lay, rtol=1e-5, atol=1e-6 * np.abs(lay).max())
+
+
+def test_grid_rule_keeps_every_pair_of_an_orthorhombic_cell():
+    """The reference's neighbour-grid rule (oracle.nl_cell_rule) reduces to the
+    +-2-grid stencil for an orthorhombic cell, which holds every pair within
+    rc + skin: no brute-force pair of the 9k water box falls outside it.
+    With tilts below a grid width (angles 85/87/84) the rule leaves some out."""
+    from sponge_amd import native_io, systems
+    s = systems.config_cpu_9k()
+    excl = (s["excl_start"], s["excl_numbers"], s["excl_list"])
+    out = {}
+    for angle in ((90.0, 90.0, 90.0), (85.0, 87.0, 84.0)):
+        cell, rcell = native_io.box_to_cell(s["box"], angle)
+        oc, orows, ov = oracle.nl_build(s["crd"], cell, rcell, 10.0, 2.0, excl)
+        assert not ov
+        rule = oracle.nl_cell_rule(s["crd"], cell, 10.0, 2.0)
+        out[angle] = sum(int(oc[i]) - int(rule(np.full(oc[i], i), orows[i, :oc[i]]).sum())
+                         for i in range(s["n"]))
+    assert out[(90.0, 90.0, 90.0)] == 0
+    assert out[(85.0, 87.0, 84.0)] > 0
