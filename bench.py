@@ -71,8 +71,8 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=None,
                     help="serving workload: this many water-box segment requests (N log-uniform "
                          "1,500-50,000 atoms, Poisson arrivals at 40/s), served one at a time and "
-                         "with continuous batching; 0 skips (default 64, 0 for water1m; the "
-                         "full workload is 256)")
+                         "with continuous batching; 0 skips (default 256, the full workload, 0 "
+                         "for water1m)")
     ap.add_argument("--serve-steps", type=int, default=1000,
                     help="MD steps per serving request (2 fs)")
     ap.add_argument("--serve-slots", type=int, default=8,
@@ -91,7 +91,7 @@ def parse():
     defaults = {"refresh": 20 if big else 10, "cpu_seconds": 0.0 if big else 15.0,
                 "md_steps": 0 if big else 500, "nve_steps": 0 if big else 20000,
                 "nve_f64_steps": 0 if big else 5000,
-                "serve_requests": 0 if big else 64, "export_steps": 0 if big else 200}
+                "serve_requests": 0 if big else 256, "export_steps": 0 if big else 200}
     for k, v in defaults.items():
         if getattr(a, k) is None:
             setattr(a, k, v)
@@ -241,6 +241,9 @@ def serving_phase(args):
         runs[name] = serving.Server(slots=slots, overlap=overlap, launch=launch,
                                     plain=plain).run(rq, realtime=True)
         reqs[name] = rq
+        print("serving %s: %.1f ns/day, makespan %.2f s" % (
+            name, runs[name]["aggregate_ns_per_day"], runs[name]["makespan_s"]),
+            file=sys.stderr, flush=True)
     same = sum(int(all(np.array_equal(a.result["crd"], reqs[o][k].result["crd"]) and
                        np.array_equal(a.result["vel"], reqs[o][k].result["vel"])
                        for o in ("sequential", "batched", "streams")))
