@@ -212,10 +212,11 @@ class Server:
     (lib.StepBatch): each kernel of a step is one dispatch over all segments,
     the reciprocal pipeline on a second stream beside the direct-space work,
     and the period is one captured HIP graph, re-planned whenever a segment
-    joins or leaves.  launch="streams": each segment replays its own captured
-    period graph on its own stream.  Either way up to `depth` periods are
-    queued ahead, so the device does not wait for the host's bookkeeping,
-    and a segment's result is the same as when it runs alone.  plain=True
+    joins or leaves (while a segment is admitted, the current graph keeps
+    the other members' periods queued).  launch="streams": each segment
+    replays its own captured period graph on its own stream.  Either way up
+    to `depth` periods are queued ahead, so the device does not wait for the
+    host's bookkeeping, and a segment's result is the same as when it runs alone.  plain=True
     (streams launch) runs each segment with the library's defaults, as a
     drop-in user would: default window shape, no tuner, the reference-layout
     list written at every rebuild."""
@@ -307,7 +308,9 @@ class Server:
         used = 0
         stream = torch.cuda.Stream()
         graph = None
-        dirty = False
+        members = []     # the members the current graph advances
+        dirty = False    # membership changed: re-plan
+        stale = False    # a member of the current graph left: no more replays
         inflight = []    # events of queued periods, oldest first
         periods = plans = 0
         launches = merged = 0
@@ -334,6 +337,29 @@ class Server:
             retired_bytes = 0
             tick("free", t)
 
+        def top_up():
+            # Queues periods of the current graph.  A member whose every step
+            # is queued leaves, and the graph is not replayed after that; a
+            # member that joined is not in it and waits for the re-plan.
+            nonlocal periods, dirty, stale
+            busy = False
+            while graph is not None and not stale and len(inflight) < self.depth:
+                with torch.cuda.stream(stream):
+                    graph[1].replay()
+                    ev = torch.cuda.Event()
+                    ev.record(stream)
+                inflight.append(ev)
+                periods += 1
+                busy = True
+                for m in members:
+                    m.done += chunk
+                    m.last = ev
+                    if m.done >= m.req.steps:
+                        active.remove(m)
+                        leaving.append(m)
+                        dirty = stale = True
+            return busy
+
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
@@ -350,6 +376,11 @@ class Server:
                 tick("admit", t)
                 used += estimate_footprint(r.n_atoms)
                 dirty = True
+                # An admission takes milliseconds of host time: keep the
+                # current members' periods queued meanwhile.
+                while inflight and inflight[0].query():
+                    inflight.pop(0)
+                top_up()
             if (pending and pending[0].arrival <= now and retired and
                     len(active) + len(leaving) < self.slots):
                 free_retired()   # memory, not slots, holds the next request back
@@ -393,24 +424,10 @@ class Server:
                 if graph is not None:
                     retired.extend(graph)
                 graph = (batch, g)
+                members = list(active)
                 plans += 1
-                dirty = False
-            busy = False
-            while active and not dirty and len(inflight) < self.depth:
-                with torch.cuda.stream(stream):
-                    graph[1].replay()
-                    ev = torch.cuda.Event()
-                    ev.record(stream)
-                inflight.append(ev)
-                periods += 1
-                busy = True
-                for m in list(active):
-                    m.done += chunk
-                    m.last = ev
-                    if m.done >= m.req.steps:
-                        active.remove(m)
-                        leaving.append(m)
-                        dirty = True
+                dirty = stale = False
+            busy = top_up()
             if not busy:
                 if not active and not leaving and pending:
                     time.sleep(max(0.0, min(0.002, pending[0].arrival - clock())))
