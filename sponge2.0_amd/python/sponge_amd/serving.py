@@ -237,8 +237,8 @@ class Server:
         earlier than its arrival time on the wall clock."""
         import torch
         for r in requests:
-            if r.steps % self.chunk:
-                raise ValueError("steps must be a multiple of the chunk")
+            if r.steps <= 0 or r.steps % self.chunk:
+                raise ValueError("steps must be a positive multiple of the chunk")
             r.system()
         if self.launch == "batched":
             return self._run_batched(requests, realtime, log)
@@ -372,10 +372,14 @@ class Server:
                 r = pending.pop(0)
                 r.admitted = time.perf_counter() - t0
                 t = time.perf_counter()
-                active.append(Member(r, chunk, self.tune))
+                m = Member(r, chunk, self.tune)
                 tick("admit", t)
                 used += estimate_footprint(r.n_atoms)
-                dirty = True
+                if m.done >= r.steps:
+                    leaving.append(m)    # its first period was all of it
+                else:
+                    active.append(m)
+                    dirty = True
                 # An admission takes milliseconds of host time: keep the
                 # current members' periods queued meanwhile.
                 while inflight and inflight[0].query():

@@ -1,6 +1,7 @@
 """Host-side logic of the serving layer (no GPU): the seeded request stream
 and the segment systems it describes."""
 import numpy as np
+import pytest
 from scipy import stats
 
 from sponge_amd import serving
@@ -49,3 +50,170 @@ def test_grid_key_matches_cell_rule():
 
     assert autotune.grid_key(Ctx()) == (4, 6, 10)
     assert autotune.grid_key(Ctx(), occupancy=True) == (4, 6, 10, 8)
+
+
+class _Fake:
+    """A stand-in device for the batched server's host loop: graph replays
+    apply their members' steps at once, and an event reports completion
+    after `lag` polls, so periods stay queued for a while as on a device."""
+
+    def __init__(self, lag):
+        self.lag = lag
+        self.capturing = None
+        self.replays = 0
+        self.replays_while_admitting = 0
+        self.unjoined = set()   # members built and not yet recorded into a plan
+        self.live = 0           # members built and not yet collected
+        self.slots = 0
+        self.errors = []
+
+    def install(self, monkeypatch):
+        import contextlib
+        import torch
+        fake = self
+
+        class Event:
+            def __init__(self):
+                self.left = 0
+
+            def record(self, stream=None):
+                self.left = fake.lag
+
+            def query(self):
+                self.left -= 1
+                return self.left < 0
+
+        class Stream:
+            def __init__(self):
+                self.waited = set()
+
+            def wait_event(self, ev):
+                self.waited.add(ev)
+
+            def synchronize(self):
+                pass
+
+        class Graph:
+            def __init__(self):
+                self.batch = None
+
+            def capture_begin(self):
+                fake.capturing = self
+
+            def capture_end(self):
+                fake.capturing = None
+
+            def replay(self):
+                fake.replays += 1
+                if fake.unjoined:
+                    fake.replays_while_admitting += 1
+                for nb in self.batch.rec:
+                    if nb.closed or nb.collected:
+                        fake.errors.append("replay of a retired member %d" % nb.req.rid)
+                    if nb.member.ready not in self.batch.stream.waited:
+                        fake.errors.append("member %d replayed before joining" % nb.req.rid)
+                    nb.steps_done += nb.chunk
+                    if nb.steps_done > nb.req.steps:
+                        fake.errors.append("member %d stepped past its end" % nb.req.rid)
+
+            def reset(self):
+                self.batch = None
+
+        class Nb:
+            def __init__(self, req, chunk, member):
+                self.req, self.chunk, self.member = req, chunk, member
+                self.steps_done = chunk          # the eager first period
+                self.closed = self.collected = False
+
+            def close(self):
+                self.closed = True
+
+        class Member:
+            def __init__(self, req, chunk=10, tune=True):
+                self.req = req
+                self.nb = Nb(req, chunk, self)
+                self.ready = Event()
+                self.ready.record()
+                self.done = chunk
+                self.last = self.ready
+                self.joined = False
+                fake.unjoined.add(self.nb)
+                fake.live += 1
+                if fake.live > fake.slots:
+                    fake.errors.append("%d members in flight" % fake.live)
+
+            def collect(self):
+                if self.nb.collected or self.nb.steps_done != self.req.steps:
+                    fake.errors.append("member %d collected twice or early" % self.req.rid)
+                self.nb.collected = True
+                fake.live -= 1
+                self.req.result = dict(steps=self.nb.steps_done, status=0)
+
+        class Batch:
+            def __init__(self):
+                self.rec = []
+                self.stream = None
+
+            def record(self, nb, first_step, steps):
+                if nb.steps_done != first_step:
+                    fake.errors.append("member %d recorded at step %d, has run %d" % (
+                        nb.req.rid, first_step, nb.steps_done))
+                fake.unjoined.discard(nb)
+                self.rec.append(nb)
+
+            def plan(self):
+                pass
+
+            def info(self):
+                return dict(launches=1, merged=len(self.rec))
+
+            def launch(self, stream):
+                self.stream = stream
+                fake.capturing.batch = self
+
+            def close(self):
+                pass
+
+        monkeypatch.setattr(torch.cuda, "Event", Event)
+        monkeypatch.setattr(torch.cuda, "Stream", Stream)
+        monkeypatch.setattr(torch.cuda, "CUDAGraph", Graph)
+        monkeypatch.setattr(torch.cuda, "synchronize", lambda *a, **k: None)
+        monkeypatch.setattr(torch.cuda, "stream", lambda s: contextlib.nullcontext())
+        monkeypatch.setattr(serving, "Member", Member)
+        monkeypatch.setattr(serving, "StepBatch", Batch)
+
+
+def test_batched_loop_steps_every_member_exactly(monkeypatch):
+    """The batched server's host loop (serving.py, Server._run_batched) on a
+    stand-in device: every request runs exactly its steps (one-period
+    requests included) and is collected once, after its last period; no
+    member is replayed before its first period is done or after it left;
+    each re-plan records a member at the step it has reached; the current
+    graph keeps being replayed while a new member is admitted; no more than
+    `slots` members are in flight."""
+    import torch  # noqa: F401  (the stand-ins replace attributes of torch.cuda)
+    fake = _Fake(lag=3)
+    fake.install(monkeypatch)
+    fake.slots = 4
+    rng = np.random.default_rng(5)
+    steps = [10, 20, 30, 50, 120, 10, 200, 40, 60, 90, 10, 150, 70, 30, 20, 250]
+    reqs = [serving.Request(k, 300, s, 0.001 * k, 100 + k) for k, s in enumerate(steps)]
+    rng.shuffle(reqs)
+    reqs.sort(key=lambda r: r.arrival)
+    for r in reqs:
+        r.sysd = {"n": 300}   # the stand-in members never read the system
+    out = serving.Server(slots=4, chunk=10, depth=2, launch="batched").run(reqs, realtime=False)
+    assert fake.errors == []
+    assert all(r.result is not None and r.result["steps"] == r.steps for r in reqs)
+    assert out["requests"] == len(reqs) and out["plans"] >= 2
+    assert fake.replays_while_admitting > 0
+    assert out["periods"] == fake.replays and fake.live == 0
+
+
+def test_server_rejects_step_counts_off_the_period():
+    """Step counts are whole neighbour-list periods (at least one)."""
+    for bad in (0, 15):
+        r = serving.Request(0, 300, bad, 0.0, 1)
+        r.sysd = {"n": 300}
+        with pytest.raises(ValueError):
+            serving.Server(chunk=10).run([r], realtime=False)
