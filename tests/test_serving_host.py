@@ -217,3 +217,61 @@ def test_server_rejects_step_counts_off_the_period():
         r.sysd = {"n": 300}
         with pytest.raises(ValueError):
             serving.Server(chunk=10).run([r], realtime=False)
+
+
+def test_streams_loop_steps_every_segment_exactly(monkeypatch):
+    """The per-segment streams loop (serving.py, Server.run with
+    launch="streams") with stand-in segments: each segment queues exactly the
+    periods after its eager first one (none for a one-period request), never
+    more than `depth` at a time, and is collected once, after the last one
+    has completed; no more than `slots` segments are in flight."""
+    import torch
+    errors, live = [], [0]
+    depth, slots, lag = 2, 3, 2
+
+    class Segment:
+        def __init__(self, req, chunk=10, tune=True, overlap=False, plain=False):
+            self.req, self.chunk = req, chunk
+            self.done = chunk
+            self.inflight = []      # polls left per queued period
+            self.collected = False
+            live[0] += 1
+            if live[0] > slots:
+                errors.append("%d segments in flight" % live[0])
+
+        def advance(self):
+            if self.done >= self.req.steps:
+                errors.append("segment %d advanced past its end" % self.req.rid)
+            self.inflight.append(lag)
+            self.done += self.chunk
+            if len(self.inflight) > depth:
+                errors.append("segment %d queued %d periods" % (self.req.rid, len(self.inflight)))
+
+        def retire(self):
+            self.inflight = [k - 1 for k in self.inflight]
+            while self.inflight and self.inflight[0] < 0:
+                self.inflight.pop(0)
+            return len(self.inflight)
+
+        @property
+        def queued_all(self):
+            return self.done >= self.req.steps
+
+        def collect(self):
+            if self.collected or self.inflight or self.done != self.req.steps:
+                errors.append("segment %d collected twice or early" % self.req.rid)
+            self.collected = True
+            live[0] -= 1
+            self.req.result = dict(steps=self.done, status=0)
+
+    monkeypatch.setattr(serving, "Segment", Segment)
+    monkeypatch.setattr(torch.cuda, "synchronize", lambda *a, **k: None)
+    steps = [10, 40, 20, 90, 10, 30, 60, 10, 120, 50]
+    reqs = [serving.Request(k, 300, s, 0.001 * k, 50 + k) for k, s in enumerate(steps)]
+    for r in reqs:
+        r.sysd = {"n": 300}
+    out = serving.Server(slots=slots, chunk=10, depth=depth, launch="streams").run(
+        reqs, realtime=False)
+    assert errors == [] and live[0] == 0
+    assert all(r.result["steps"] == r.steps for r in reqs)
+    assert out["periods"] == sum(s // 10 - 1 for s in steps)
