@@ -216,10 +216,10 @@ class Server:
     the other members' periods queued).  launch="streams": each segment
     replays its own captured period graph on its own stream.  Either way up
     to `depth` periods are queued ahead, so the device does not wait for the
-    host's bookkeeping, and a segment's result is the same as when it runs alone.  plain=True
-    (streams launch) runs each segment with the library's defaults, as a
-    drop-in user would: default window shape, no tuner, the reference-layout
-    list written at every rebuild."""
+    host's bookkeeping, and a segment's result is the same as when it runs
+    alone.  plain=True (streams launch) runs each segment with the library's
+    defaults, as a drop-in user would: default window shape, no tuner, the
+    reference-layout list written at every rebuild."""
 
     def __init__(self, slots=8, hbm_budget=64 * 2 ** 30, chunk=10, tune=True, overlap=False,
                  depth=2, launch="batched", free_every=8 * 2 ** 30, plain=False):
